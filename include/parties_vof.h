@@ -1,0 +1,212 @@
+/*
+ * parties_vof.h -- C ABI of the MI355X (gfx950) implementation of PARTIES'
+ * per-Runge-Kutta-substep Eulerian two-phase path.
+ *
+ * One handle (pv_bag) owns one MPI rank's whole 2-D subdomain on one GPU:
+ * every field lives in HBM, halos and boundary conditions are applied on the
+ * device, and the host only enqueues kernels.  Entry points keep the names of
+ * the reference routines they replace (prefixed pv_) and take the handle in
+ * place of the reference's Cart3d_bag*.  Reference anchors are relative to
+ * PARTIES/src/ of m-jalabert/PARTIES-VoF-IBM.
+ *
+ * Boundary set: x periodic, no-slip walls in y, z collapsed (TWOD_CARTESIAN,
+ * PERIODIC_NOSLIP_BOX, Include/Boundary.h:35,88-92), GRID_UNIFORM,
+ * VOF_DIFFUSE + SURFACE_TENSION + VOF_GRAVITY, CG_SOLVE (FULLY_IMPLICIT),
+ * no IBM (C_S == 0).
+ *
+ * Field layout seen through pv_get_field / pv_set_field ("canonical"):
+ *   double a[NYM+7][NXM+7]; cell/face (i,j) -> a[j+3][i+3], i,j in [-3, N+3].
+ * It is one k-slab of the reference's [k][j][i] block (Memory.c:36-96) with
+ * the same ghost depth (ghost_nodes = 3).  Index conventions follow
+ * Grid.c:62-64,187-202: NX = NXM+1, the "G" range is 0..NX-1 and includes
+ * the periodic image column i = NX-1 and the wall ghost row j = NY-1.
+ *
+ * Errors: every entry point returns 0 on success and a negative PV_E* code
+ * otherwise; pv_last_error() gives a thread-local message.  The reference
+ * has no status codes (it calls Display_throw_error, IO/Display.c:910-937);
+ * a C wrapper maps non-zero returns onto that fatal path.
+ *
+ * Threading: one host thread per handle; calls on one handle are stream
+ * ordered and must not overlap.  Handles on different GPUs are independent.
+ */
+#ifndef PARTIES_VOF_H
+#define PARTIES_VOF_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---- error codes ------------------------------------------------------ */
+#define PV_OK             0
+#define PV_EINVAL        -1   /* bad argument / size */
+#define PV_EHIP          -2   /* HIP runtime error */
+#define PV_ENOMEM        -3   /* device allocation failed */
+#define PV_ENOCONV       -4   /* pressure outer loop exceeded 10 passes */
+#define PV_ENODEV        -5   /* no usable gfx950 device */
+
+/* ---- pressure solver selection ---------------------------------------- */
+#define PV_PSOLVE_JACOBI_PCG 0   /* psolve_cg.c:130-293, the reference VOF path */
+#define PV_PSOLVE_FFT_PCG    1   /* PCG preconditioned by the FFT-x/Thomas-y solve
+                                    of psolve_fft.c:1536-1802, relative-residual stop */
+
+/* ---- arithmetic selection ---------------------------------------------- */
+#define PV_PREC_FP64  0          /* all FP64 (the plain path) */
+#define PV_PREC_MIXED 1          /* FP32 WENO / chemical potential / momentum
+                                    stencils, FP64 phi-flux accumulation, Poisson
+                                    and PCG inner products */
+
+/* Parameters: names and defaults follow IO/default.inp; derived values as in
+ * IO/Input.c:20-127 (Cn <= 0 -> 0.75 h, Pe_CH <= 0 -> 0.9/Cn, slab <= 0 -> h). */
+typedef struct pv_params {
+    int    NXM, NYM;                 /* [grid]      default 64, 64 */
+    double xmin, xmax, ymin, ymax;   /* [geometry]  default 0,1,0,1 */
+    double twod_slab_thickness;      /* [twod]      default -1 */
+    double Re;                       /* [flow]      default 100 */
+    double dp_dx;                    /* [flow]      default 0 */
+    double We, rho1, rho2, mu1, mu2; /* [vof]       default 1,1,1,1,1 */
+    double Cn, Pe_CH;                /* [vof]       default -1,-1 */
+    int    weno_order;               /* [vof]       default 5 */
+    int    ch_iter_max;              /* [vof]       default 300 */
+    double ch_tol;                   /* [vof]       default 1e-9 */
+    double cfl, default_dt, max_dt;  /* [simulation] 0.5, 1e-3, 0.05 */
+    int    constant_dt;              /* [simulation] 0 */
+    double grav[3];                  /* [particle]  {0,-1,0} */
+    double richardson;               /* [conc] richardson[0], default 1 */
+    int    P_CG_MAXIT;               /* [lsolve]    300 */
+    double P_CG_ETOL;                /* [lsolve]    1e-7 (absolute sqrt(r.z)) */
+    int    CG_MAXIT;                 /* [lsolve]    120 */
+    double CG_ETOL;                  /* [lsolve]    1e-9 (RMS residual) */
+    int    pressure_solver;          /* PV_PSOLVE_*  (not in the reference) */
+    double p_rel_tol;                /* FFT-PCG stop ||r||/||b|| (not in the reference) */
+    int    p_max_iter;               /* FFT-PCG iteration cap (not in the reference) */
+    int    precision;                /* PV_PREC_*   (not in the reference) */
+} pv_params;
+
+/* Fill *p with the reference defaults of IO/default.inp (2-D, no IBM). */
+void pv_params_default(pv_params *p);
+
+/* ---- fields (canonical layout above) ---------------------------------- */
+enum pv_field {
+    PV_F_C_L = 0, PV_F_C_G, PV_F_RHO, PV_F_MU,              /* VOF_InterfaceReconstruction.c:39-153 */
+    PV_F_PSI_LG, PV_F_LAP_C, PV_F_BULK_S, PV_F_PSI,
+    PV_F_CH_AUX1, PV_F_CH_AUX2, PV_F_CH_RHS_N, PV_F_CH_RHS_NM1,
+    PV_F_CH_RES, PV_F_CH_DIR, PV_F_FLUX_X, PV_F_FLUX_Y,
+    PV_F_FSX_NEW, PV_F_FSY_NEW, PV_F_FSX_OLD, PV_F_FSY_OLD,
+    PV_F_U, PV_F_V, PV_F_U_BC, PV_F_V_BC,                   /* Velocity.c */
+    PV_F_U_EXPL, PV_F_U_EXPL_OLD, PV_F_U_IMPL, PV_F_U_VISC, PV_F_U_VISC_OLD, PV_F_U_RHS,
+    PV_F_V_EXPL, PV_F_V_EXPL_OLD, PV_F_V_IMPL, PV_F_V_VISC, PV_F_V_VISC_OLD, PV_F_V_RHS,
+    PV_F_U_R, PV_F_U_D, PV_F_U_AD, PV_F_V_R, PV_F_V_D, PV_F_V_AD,
+    PV_F_P, PV_F_PHI, PV_F_P_RHS, PV_F_P_RES, PV_F_P_D, PV_F_P_AD, PV_F_M_INV, /* Pressure.c, psolve_cg.c */
+    PV_F_SCRATCH0, PV_F_SCRATCH1, PV_F_SCRATCH2,
+    PV_NFIELDS
+};
+
+/* Velocity_update_boundaries types (Include/definitions.h VEL_TYPE_*) */
+#define PV_VEL_TYPE_NORMAL 0
+#define PV_VEL_TYPE_CG     1
+
+/* Per-substep solver accounting (the Display_progress lines of the reference) */
+typedef struct pv_stats {
+    double time, dt, dt_old;
+    int    ntime, which_stage;
+    int    n_ch[3], n_u[3], n_v[3], n_p[3], n_pass[3]; /* iterations per stage */
+    double ch_res[3];         /* final relative residual of the biharmonic CG  */
+    double p_res[3];          /* final pressure residual (sqrt(r.z) or ||r||/||b||) */
+    double div_max[3];        /* Pressure_compute_velocity_divergence per stage */
+    double clamp_delta[3];    /* diffuse_bound_liquid_fraction return value     */
+    int    p_fallback[3];     /* learned preconditioner fell back to FFT-PCG    */
+} pv_stats;
+
+typedef struct pv_bag pv_bag;
+
+const char *pv_last_error(void);
+
+/* Grid_create + Memory/VoF/Pressure/Velocity *_create on the device
+ * (Cart3d.c:318-390; Grid.c:30-686, :797-925). */
+int pv_create(const pv_params *prm, int device, pv_bag **out);
+int pv_destroy(pv_bag *bag);
+int pv_get_params(const pv_bag *bag, pv_params *out);   /* derived values filled in */
+
+/* H2D / D2H of one field in canonical layout ((NYM+7)*(NXM+7) doubles).
+ * Stand-in for Resume_h5_data / Output_h5_data (IO/Resume.c:341, IO/Output.c:277). */
+int pv_set_field(pv_bag *bag, int field, const double *host);
+int pv_get_field(pv_bag *bag, int field, double *host);
+int pv_fill_field(pv_bag *bag, int field, double value);
+
+/* Initial state from a volume-fraction field F (values on the G range):
+ * VOF_DIFFUSE_set_boundary_values(F); VOF_DIFFUSE_init; density/viscosity;
+ * psi_LG + f_sigma bootstrap, new -> old (Cart3d.c:1094-1130,
+ * VOF_DIFFUSE.c:524-568); u = v = p = 0; t = 0, dt = default_dt and the
+ * cell-centre velocity (Temporal_int.c:163-199). */
+int pv_init_from_F(pv_bag *bag, const double *F_host);
+
+/* Time and stage state normally held in Parameters (time, dt, dt_old, which_stage). */
+int pv_set_time_state(pv_bag *bag, double time, double dt, double dt_old, int which_stage);
+int pv_get_stats(pv_bag *bag, pv_stats *out);
+
+/* ---- time integration (Include/Temporal.h:4-5) ------------------------- */
+/* nsteps iterations of the RK3 loop body (Temporal_int.c:399-448): CFL on
+ * stage 0 when time != 0, three substeps, time += dt * BETA2[rk]. */
+int pv_Temporal_int_rk3(pv_bag *bag, int nsteps);
+/* One substep at the current which_stage (Temporal_int.c:563-1112, no IBM). */
+int pv_Temporal_int_all_the_equations(pv_bag *bag);
+/* Dtime.c:24-249; writes the new dt (does not store it). */
+int pv_Dtime_cfl(pv_bag *bag, double *dt_out);
+
+/* ---- Cahn-Hilliard (Eulerian/Include/VOF_DIFFUSE.h:7-31) --------------- */
+int pv_VOF_DIFFUSE_set_boundary_values(pv_bag *bag, int field);              /* VOF_DIFFUSE.c:571-683 */
+int pv_VOF_DIFFUSE_compute_laplacian(pv_bag *bag, int in, int out);          /* VOF_DIFFUSE.c:838-863 */
+int pv_VOF_DIFFUSE_compute_bulk_S(pv_bag *bag);                              /* VOF_DIFFUSE.c:865-885 */
+int pv_VOF_DIFFUSE_compute_psi_LG(pv_bag *bag);                              /* VOF_DIFFUSE.c:907-929 */
+int pv_VOF_DIFFUSE_advect_WENO5(pv_bag *bag, int rhs_out);                   /* VOF_DIFFUSE.c:931-996 */
+int pv_VOF_DIFFUSE_explicit_diffusion(pv_bag *bag, int rhs_out);             /* VOF_DIFFUSE.c:998-1016 */
+int pv_VOF_DIFFUSE_solve_implicit_biharmonic(pv_bag *bag, int rhs_explicit,
+                                             int *iters, double *rel_res);   /* VOF_DIFFUSE.c:1018-1069 */
+int pv_VOF_DIFFUSE_bound_liquid_fraction(pv_bag *bag, double *mass_delta);   /* VOF_DIFFUSE.c:1093-1127 */
+int pv_VOF_DIFFUSE_update_phase_cache(pv_bag *bag);                          /* VOF_DIFFUSE.c:108-126 */
+int pv_VOF_DIFFUSE_step(pv_bag *bag);                                        /* VOF_DIFFUSE.c:1262-1359 */
+int pv_VOF_DIFFUSE_compute_f_sigma(pv_bag *bag);                             /* VOF_DIFFUSE.c:1363-1445 */
+int pv_VOF_update_density_viscosity(pv_bag *bag);                            /* VOF_Advection.c:743-748 -> VOF_DIFFUSE.c:1453-1488 */
+int pv_VOF_apply_f_sigma_old(pv_bag *bag);                                   /* VOF_SurfaceTension.c:1983-2041 */
+int pv_VOF_swap_f_sigma(pv_bag *bag);                                        /* VOF_SurfaceTension.c:2055-2071 */
+int pv_Velocity_add_gravity_2_RHS(pv_bag *bag);                              /* VOF_SurfaceTension.c:2075-2154 */
+
+/* ---- momentum (Eulerian/Include/Velocity.h:10-18,28) -------------------- */
+int pv_Velocity_cell_center(pv_bag *bag);                                    /* Velocity.c:278-357 */
+int pv_Velocity_u_set_implicit_explicit(pv_bag *bag);                        /* Velocity.c:366-762 */
+int pv_Velocity_v_set_implicit_explicit(pv_bag *bag);                        /* Velocity.c:771-1144 */
+int pv_Velocity_u_set_RHS(pv_bag *bag);                                      /* Velocity.c:1494-1728 */
+int pv_Velocity_v_set_RHS(pv_bag *bag);                                      /* Velocity.c:1739-1947 */
+int pv_Velocity_update_boundaries(pv_bag *bag, int field, int type);         /* Velocity.c:2427-2805 */
+int pv_Velocity_solve_cg(pv_bag *bag, int component, int *iters);            /* lsolver/msolve_cg.c:525-722; component 'u' or 'v' */
+int pv_matVec(pv_bag *bag, int out, int in, int component);                  /* lsolver/msolve_cg.c:255-493 */
+
+/* ---- pressure (Eulerian/Include/Pressure.h:8-13,32,43,51,65) ------------ */
+int pv_Pressure_set_RHS(pv_bag *bag);                                        /* Pressure.c:100-196 */
+int pv_Pressure_compute_preconditioner(pv_bag *bag);                         /* lsolver/psolve_cg.c:63-101 */
+int pv_Pressure_solve_cg(pv_bag *bag, int *iters);                           /* lsolver/psolve_cg.c:130-293 (or FFT-PCG) */
+int pv_Pressure_operator_variableCoeff(pv_bag *bag, int out, int in);        /* lsolver/psolve_cg.c:302-344 */
+int pv_Pressure_apply_BCs(pv_bag *bag, int field);                           /* lsolver/psolve_cg.c:358-529 + cell halo */
+int pv_Pressure_project_velocity(pv_bag *bag);                               /* Pressure.c:218-464 */
+int pv_Pressure_compute_velocity_divergence(pv_bag *bag, double *div_max);   /* Pressure.c:475-597 */
+/* Constant-coefficient Poisson solve, FFT in x (R2HC, 1/NXM scale, modified
+ * wavenumbers) and Thomas in y with the (0,0) mode pinned at j = 0:
+ * lsolver/psolve_fft.c:17-207, :467-591, :1536-1802 (z of length 1). */
+int pv_Pressure_solve_fft(pv_bag *bag, int rhs_field, int out_field);
+
+/* ---- initial volume fraction F on the G range (host only, no device) ----
+ * 6 x 6 sub-sampled tanh profile of width 2 sqrt(2) Cn, as in
+ * IO/VoF_Init.c:877-953 / :1032-1100 (F = 1 liquid, 0 gas).  The seeded
+ * generators use an in-library splitmix64 stream; they have no counterpart in
+ * the reference (its RT is single-mode and it has no random bubble field). */
+int pv_ic_rising_bubble(const pv_params *prm, double *F);      /* VoF_init_planar_rising_bubble_2d */
+int pv_ic_rayleigh_taylor(const pv_params *prm, double *F);    /* VoF_init_rayleigh_taylor_2d */
+int pv_ic_multimode_rt(const pv_params *prm, unsigned long long seed, int modes, double amp_max,
+                       double y0, double *F);
+int pv_ic_bubbles(const pv_params *prm, unsigned long long seed, int nmin, int nmax, double rmin,
+                  double rmax, int gas_inside, double *F, int *nplaced);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PARTIES_VOF_H */

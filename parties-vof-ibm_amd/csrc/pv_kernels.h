@@ -1,0 +1,188 @@
+/*
+ * pv_kernels.h -- interface between the C host layer (host directory) and the
+ * gfx950 kernels (csrc/).  Plain C: device pointers, sizes, a stream.
+ *
+ * Device field layout: one 2-D slab per field, row pitch P doubles (multiple
+ * of 16, i.e. 128 B), ghost depth 3.  Every field pointer handed to a
+ * launcher points at cell (0,0); cell (i,j), i,j in [-3, N+3], is at
+ * f[j*P + i].  Row i = 0 starts on a 128-byte boundary.
+ */
+#ifndef PV_KERNELS_H
+#define PV_KERNELS_H
+#include <stddef.h>
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Device-resident scalars of one handle (read and written by kernels, so a
+ * whole substep runs without host round trips). */
+typedef struct pvk_scal {
+    double dt, dt_old, time;
+    /* Cahn-Hilliard biharmonic CG (VOF_DIFFUSE.c:339-429) */
+    double ch_rz, ch_alpha, ch_beta, ch_bnorm, ch_rel;
+    int    ch_iter, ch_done, ch_maxit, pad0;
+    /* momentum Jacobi-PCG per component 0 = u, 1 = v (msolve_cg.c:525-722) */
+    double m_rz[2], m_alpha[2], m_beta[2], m_rms[2];
+    int    m_iter[2], m_done[2];
+    /* pressure PCG (psolve_cg.c:130-293 or FFT-PCG) */
+    double p_gamma, p_alpha, p_beta, p_res, p_bnorm, p_mean;
+    int    p_iter, p_done;
+    double clamp_delta, div_max, max_idt, dt_new;
+    double red[8];           /* raw reduction results, for tests */
+} pvk_scal;
+
+/* Geometry and metrics (device pointers offset so that m[i] is valid for
+ * i in [-3, N+2]); Grid.c:385-420 arrays. */
+typedef struct pvk_grid {
+    int NXM, NYM, P;
+    int nbx, nby;            /* interior launch grid (64 x 4 cell tiles) */
+    double slab;             /* dummy_z_slab_thickness (TwodOps.c:330-338) */
+    const double *idx_u, *idx_c, *dx_u, *dx_c;
+    const double *idy_v, *idy_c, *dy_v, *dy_c;
+} pvk_grid;
+
+/* Physical and solver constants (IO/default.inp names). */
+typedef struct pvk_phys {
+    double Re, iRe, Pe_CH, Cn, We, rho_g, mu_g, bih;
+    double gx, gy, Ri, dp_dx, rho1, rho2, cfl, max_dt, default_dt;
+    double ch_tol, CG_ETOL, P_CG_ETOL, p_rel_tol;
+    int    weno_order, constant_dt, CG_MAXIT, P_CG_MAXIT;
+    int    ch_iter_max, p_max_iter, precision, pad;
+} pvk_phys;
+
+/* Reduction scratch: per-block partial sums (deterministic two-stage sums). */
+typedef struct pvk_red {
+    double *part;            /* >= 4 * nblocks doubles */
+    int nblocks;
+} pvk_red;
+
+/* FFT-x / DCT-y constant-coefficient solve tables (csrc/pv_fft.hip). */
+typedef struct pvk_fft {
+    int ok;                   /* 0: sizes not supported (need powers of two) */
+    int logmx, logmy;         /* log2(NXM/2), log2(NYM/2) */
+    int rx, ry;               /* transforms per block: row and column kernels */
+    const double *twx, *twy;  /* (cos, sin)(2 pi t / M), t < M/2, interleaved */
+    const double *rtx, *rty;  /* (cos, sin)(2 pi k / N), k <= N/2 */
+    const double *dcty;       /* (cos, sin)(pi k / (2 NYM)), k < NYM */
+    const double *lamx, *lamy;/* 1-D eigenvalue magnitudes */
+    double *spec;             /* NXM * NYM spectral scratch, row-major */
+} pvk_fft;
+
+/* ---- ghost fills (kind codes in pv_kernels.hip k_ghost) ---- */
+void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s);          /* VOF_DIFFUSE_set_boundary_values */
+void pvk_halo_cell(const pvk_grid *g, double *f, hipStream_t s);
+void pvk_halo_u(const pvk_grid *g, double *f, hipStream_t s);
+void pvk_vel_bc(const pvk_grid *g, double *f, int comp, int type, hipStream_t s);
+void pvk_p_bc(const pvk_grid *g, double *f, hipStream_t s);          /* Pressure_apply_BCs + cell halo */
+void pvk_ghost_multi(const pvk_grid *g, double *f0, double *f1, double *f2, double *f3, int nf, int kind,
+                     hipStream_t s);
+void pvk_fill(double *f, size_t n, double val, hipStream_t s);
+void pvk_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s);
+
+/* ---- Cahn-Hilliard ---- */
+void pvk_laplacian(const pvk_grid *g, const double *in, double *out, hipStream_t s);
+void pvk_bulk_S(const pvk_grid *g, const pvk_phys *ph, const double *C, double *S, hipStream_t s);
+void pvk_weno_advect(const pvk_grid *g, const pvk_phys *ph, const double *C, const double *u,
+                     const double *v, double *rhs, hipStream_t s);
+void pvk_explicit_diffusion(const pvk_grid *g, const pvk_phys *ph, const double *C, double *rhs,
+                            hipStream_t s);
+void pvk_ch_rhs_init(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
+                     const double *C, const double *u, const double *v, const double *rn,
+                     double *rc, double *r, double *p, const pvk_red *rd, hipStream_t s);
+void pvk_ch_cg_init_from_b(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
+                           const double *C, double *b, double *r, double *p, const pvk_red *rd,
+                           hipStream_t s);
+void pvk_ch_finalize_init(const pvk_grid *g, pvk_scal *sc, int maxit, double tol, const pvk_red *rd,
+                          hipStream_t s);
+void pvk_ch_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const double *p,
+                  double *Ap, const pvk_red *rd, hipStream_t s);
+void pvk_ch_alpha(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s);
+void pvk_ch_update(const pvk_grid *g, pvk_scal *sc, double *x, const double *p, double *r,
+                   const double *Ap, const pvk_red *rd, hipStream_t s);
+void pvk_ch_beta(const pvk_grid *g, pvk_scal *sc, double tol, const pvk_red *rd, hipStream_t s);
+void pvk_ch_pupdate(const pvk_grid *g, pvk_scal *sc, const double *r, double *p, hipStream_t s);
+void pvk_clamp_props(const pvk_grid *g, const pvk_phys *ph, double *C, double *G, double *rho,
+                     double *mu, const pvk_red *rd, pvk_scal *sc, hipStream_t s);
+void pvk_phase_props(const pvk_grid *g, const pvk_phys *ph, const double *C, double *G, double *rho,
+                     double *mu, hipStream_t s);
+void pvk_psi_fsigma(const pvk_grid *g, const pvk_phys *ph, const double *C, const double *G,
+                    double *lap, double *psi, double *fx, double *fy, hipStream_t s);
+void pvk_psi_only(const pvk_grid *g, const pvk_phys *ph, const double *C, double *psi, hipStream_t s);
+void pvk_clamp_init(const pvk_grid *g, const double *F, double *C, hipStream_t s);
+
+/* ---- momentum ---- */
+void pvk_cell_center(const pvk_grid *g, const double *u, const double *v, double *ub, double *vb,
+                     hipStream_t s);
+void pvk_mom_terms(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc, int comp,
+                   const double *u, const double *v, const double *mu, const double *rho,
+                   const double *p, double *eold, double *vold, const double *fold,
+                   double *rhs, double *ex_out, double *im_out, double *ve_out, int write_terms,
+                   int add_force, hipStream_t s);
+void pvk_mom_force(const pvk_grid *g, const pvk_phys *ph, int comp, const double *rho, const double *fold,
+                   double *rhs, int which, hipStream_t s);
+void pvk_mom_matvec(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc, int comp,
+                    const double *x, const double *rho, const double *mu, double *Ax, hipStream_t s);
+void pvk_mom_cg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
+                     const double *x, const double *rhs, const double *rho, const double *mu,
+                     double *r, double *d, double *Ad, double *dg, const pvk_red *rd, hipStream_t s);
+void pvk_mom_cg_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
+                      const double *d, const double *rho, const double *mu, double *Ad,
+                      const pvk_red *rd, hipStream_t s);
+void pvk_mom_cg_update(const pvk_grid *g, pvk_scal *sc, int comp, double *x, const double *d,
+                       double *r, const double *Ad, const double *dg, const pvk_red *rd,
+                       hipStream_t s);
+void pvk_mom_cg_dupdate(const pvk_grid *g, pvk_scal *sc, int comp, const double *r,
+                        const double *dg, double *d, hipStream_t s);
+void pvk_mom_scalar(const pvk_grid *g, pvk_scal *sc, int comp, int which, int maxit, double tol,
+                    const pvk_red *rd, hipStream_t s);
+
+/* ---- pressure ---- */
+void pvk_p_rhs2(const pvk_grid *g, int stage, const pvk_scal *sc, const double *u, const double *v,
+                const double *rho, double *rhs, double *M, hipStream_t s);
+void pvk_p_operator(const pvk_grid *g, const double *rho, const double *x, double *Ax, hipStream_t s);
+void pvk_p_cg_init2(const pvk_grid *g, const double *rhs, const double *M, double *phi, double *r, double *d,
+                    int use_M, int shift_mean, const pvk_scal *sc, const pvk_red *rd, hipStream_t s);
+void pvk_p_cg_apply(const pvk_grid *g, pvk_scal *sc, const double *rho, const double *d, double *Ad,
+                    const pvk_red *rd, hipStream_t s);
+void pvk_p_cg_update(const pvk_grid *g, pvk_scal *sc, double *phi, const double *d, double *r,
+                     const double *Ad, const double *M, int use_M, const pvk_red *rd,
+                     hipStream_t s);
+void pvk_p_cg_dupdate(const pvk_grid *g, pvk_scal *sc, const double *z, const double *M, int use_M,
+                      double *d, hipStream_t s);
+void pvk_p_scalar2(const pvk_grid *g, pvk_scal *sc, int which, int mode, int maxit, double tol,
+                   const pvk_red *rd, int nb, hipStream_t s);
+void pvk_dot(const pvk_grid *g, const double *a, const double *b, const pvk_red *rd, hipStream_t s);
+void pvk_dot_sc(const pvk_grid *g, const pvk_scal *sc, const double *a, const double *b,
+                const pvk_red *rd, hipStream_t s);
+void pvk_p_mean(const pvk_grid *g, const double *x, const pvk_red *rd, hipStream_t s);
+void pvk_p_mean_finalize(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s);
+void pvk_p_sub_mean(const pvk_grid *g, const pvk_scal *sc, double *x, int ghosts, hipStream_t s);
+void pvk_project(const pvk_grid *g, int stage, const pvk_scal *sc, const double *rho,
+                 const double *phi, double *p, double *u, double *v, hipStream_t s);
+void pvk_divergence(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd,
+                    hipStream_t s);
+void pvk_div_finalize(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s);
+void pvk_pin_row0(const pvk_grid *g, double *x, hipStream_t s);
+
+/* ---- FFT-x / DCT-y solve (csrc/pv_fft.hip) ---- */
+void pvk_fft_solve(const pvk_grid *g, const pvk_fft *F, const double *in, double *out, int mean_free,
+                   hipStream_t s);
+void pvk_fft_solve_sc(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, const double *in,
+                      double *out, int mean_free, hipStream_t s);
+void pvk_fft_solve_dot(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, const double *in,
+                       double *out, int mean_free, double *part, hipStream_t s);
+int  pvk_fft_rows_blocks(const pvk_grid *g, const pvk_fft *F);
+
+/* ---- time step ---- */
+void pvk_cfl(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s);
+void pvk_cfl_finalize2(const pvk_grid *g, const pvk_phys *ph, pvk_scal *sc, double dx_min,
+                       const pvk_red *rd, hipStream_t s);
+void pvk_take_dt(pvk_scal *sc, hipStream_t s);
+void pvk_advance_time(pvk_scal *sc, int stage, hipStream_t s);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,1553 @@
+// pv_kernels.hip -- gfx950 kernels of the PARTIES 2-D two-phase RK3 substep.
+//
+// Every kernel is HBM-bound FP64 stencil or vector work.  Launch shape: 64 x 4
+// threads per block, one cell per thread, x (unit stride) across the 64-wide
+// wavefront so each wave row is one coalesced 512-byte segment.  Built with
+// -ffp-contract=off: every expression rounds as written, in the operand order
+// of the reference (PARTIES/src), so stencil outputs match the CPU oracle bit
+// for bit and only reduction order differs.
+//
+// Reductions are deterministic: each block writes one partial per sum, a
+// one-block kernel adds the partials in block order and then runs the solver's
+// scalar logic (alpha, beta, stop tests) on the device.  Solver loops never
+// need a host round trip per iteration; kernels early-exit on a device flag.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "pv_kernels.h"
+
+#define AT(a, i, j) (a)[(ptrdiff_t)(j) * P + (i)]
+#define BX 64
+#define BY 4
+#define NT (BX * BY)
+
+__constant__ double c_BET[3]  = {4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0};
+__constant__ double c_BET2[3] = {8.0 / 15.0, 2.0 / 15.0, 1.0 / 3.0};
+__constant__ double c_GAMB[3] = {2.0, 25.0 / 4.0, 9.0 / 2.0};
+__constant__ double c_ZETB[3] = {0.0, -17.0 / 4.0, -2.5};
+
+static inline dim3 grid2d(const pvk_grid *g) { return dim3(g->nbx, g->nby); }
+
+// ---------------------------------------------------------------------------
+// reductions
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ int flat_tid() { return threadIdx.y * BX + threadIdx.x; }
+__device__ __forceinline__ int flat_bid() { return blockIdx.y * gridDim.x + blockIdx.x; }
+
+// Sum NS values over the block in a fixed order and store them as this
+// block's partials part[k * nb + bid].
+template <int NS>
+__device__ __forceinline__ void block_partials(double (&v)[NS], double *part, int nb)
+{
+    __shared__ double sh[NS][NT / 64];
+    const int tid = flat_tid(), lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x += __shfl_down(x, off, 64);
+        if (lane == 0) sh[k][w] = x;
+    }
+    __syncthreads();
+    if (tid < NS) {
+        double s = sh[tid][0];
+#pragma unroll
+        for (int q = 1; q < NT / 64; q++) s += sh[tid][q];
+        part[(size_t)tid * nb + flat_bid()] = s;
+    }
+}
+
+template <int NS>
+__device__ __forceinline__ void block_max_partials(double (&v)[NS], double *part, int nb)
+{
+    __shared__ double sh[NS][NT / 64];
+    const int tid = flat_tid(), lane = tid & 63, w = tid >> 6;
+#pragma unroll
+    for (int k = 0; k < NS; k++) {
+        double x = v[k];
+#pragma unroll
+        for (int off = 32; off > 0; off >>= 1) x = fmax(x, __shfl_down(x, off, 64));
+        if (lane == 0) sh[k][w] = x;
+    }
+    __syncthreads();
+    if (tid < NS) {
+        double s = sh[tid][0];
+#pragma unroll
+        for (int q = 1; q < NT / 64; q++) s = fmax(s, sh[tid][q]);
+        part[(size_t)tid * nb + flat_bid()] = s;
+    }
+}
+
+// One-block final sum of partial set k (fixed order: strided serial, then tree).
+__device__ double final_sum(const double *part, int nb, int k, bool is_max)
+{
+    __shared__ double sh[256];
+    const int tid = threadIdx.x;
+    double s = is_max ? 0.0 : 0.0;
+    for (int b = tid; b < nb; b += 256) {
+        double x = part[(size_t)k * nb + b];
+        s = is_max ? fmax(s, x) : s + x;
+    }
+    sh[tid] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (tid < w) sh[tid] = is_max ? fmax(sh[tid], sh[tid + w]) : sh[tid] + sh[tid + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// ghost fills (final state of the reference's BC + Sendrecv sequences)
+// ---------------------------------------------------------------------------
+// Cell-type image of column i (Communication.c:146-295, period NXM).
+__device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >= n ? i - n : i); }
+// u-type image (Communication.c:162-169): unknowns 1..n, column 0 = column n.
+__device__ __forceinline__ int wrapu(int i, int n) { return i <= 0 ? i + n : (i > n ? i - n : i); }
+
+// kind: 0 = CH zero-gradient (VOF_DIFFUSE.c:571-683, 3 layers),
+//       1 = pressure Neumann (psolve_cg.c:358-529, 1 layer) + cell halo,
+//       2 = cell halo only, 3 = u-type halo only,
+//       4 = u velocity: -mirror row -1 / NYM + u-type halo (Velocity.c:2600-2682),
+//       5 = v velocity NORMAL: rows 0, NYM = 0 + cell halo (Velocity.c:2566-2644),
+//       6 = v velocity CG: cell halo only.
+__global__ void k_ghost(pvk_grid g, double *f0, double *f1, double *f2, double *f3, int nf, int kind)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    const int nrow = NXM + 7;                       // i in [-3, NXM+3]
+    const int nrowcells = 6 * nrow;                 // 3 ghost rows top and bottom
+    const int ncol = (kind == 3 || kind == 4) ? 7 : 6;
+    const int ncolcells = ncol * NYM;
+    int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrowcells + ncolcells) return;
+    int i, j;
+    if (t < nrowcells) {
+        int r = t / nrow;
+        i = t % nrow - 3;
+        j = (r < 3) ? -1 - r : NYM + (r - 3);
+    } else {
+        int c = (t - nrowcells) / NYM;
+        j = (t - nrowcells) % NYM;
+        if (ncol == 7) i = (c < 4) ? -c : NXM + 1 + (c - 4);
+        else i = (c < 3) ? -1 - c : NXM + (c - 3);
+    }
+    double *fs[4] = {f0, f1, f2, f3};
+    for (int q = 0; q < nf; q++) {
+        double *f = fs[q];
+        double val;
+        bool write = true;
+        switch (kind) {
+        case 0: {
+            if (i == NXM + 3 || j == NYM + 3) { write = false; val = 0.0; break; }
+            int ii = wrapc(i, NXM);
+            int jj = j < 0 ? 0 : (j >= NYM ? NYM - 1 : j);
+            val = AT(f, ii, jj);
+        } break;
+        case 1: {
+            if (i == NXM + 3) { write = false; val = 0.0; break; }
+            int ii = wrapc(i, NXM);
+            if (j == -1) val = AT(f, ii, 0);
+            else if (j == NYM) val = AT(f, ii, NYM - 1);
+            else if (j < -1 || j > NYM) { if (i >= 0 && i < NXM) { write = false; val = 0.0; break; } val = AT(f, ii, j); }
+            else val = AT(f, ii, j);
+        } break;
+        case 2: case 6: {
+            if (i >= 0 && i < NXM) { write = false; val = 0.0; break; }
+            if (i == NXM + 3) { write = false; val = 0.0; break; }
+            val = AT(f, wrapc(i, NXM), j);
+        } break;
+        case 3: {
+            if (i >= 1 && i <= NXM) { write = false; val = 0.0; break; }
+            val = AT(f, wrapu(i, NXM), j);
+        } break;
+        case 4: {
+            int ii = wrapu(i, NXM);
+            if (j == -1) val = -AT(f, ii, 0);
+            else if (j == NYM) val = -AT(f, ii, NYM - 1);
+            else if (i >= 1 && i <= NXM) { write = false; val = 0.0; }
+            else val = AT(f, ii, j);
+        } break;
+        case 5: default: {
+            if (i == NXM + 3) { write = false; val = 0.0; break; }
+            if (j == 0 || j == NYM) val = 0.0;
+            else if (i >= 0 && i < NXM) { write = false; val = 0.0; }
+            else val = AT(f, wrapc(i, NXM), j);
+        } break;
+        }
+        if (write) AT(f, i, j) = val;
+    }
+}
+
+// Rows 0 and NYM of a v field are interior-indexed rows that the NORMAL BC
+// zeroes; the frame kernel above only visits j < 0, j >= NYM and ghost
+// columns, so row 0 is cleared here.
+__global__ void k_zero_row(pvk_grid g, double *f, int j)
+{
+    const int P = g.P;
+    int i = blockIdx.x * blockDim.x + threadIdx.x - 3;
+    if (i <= g.NXM + 3) AT(f, i, j) = 0.0;
+}
+
+static void launch_ghost(const pvk_grid *g, double *f0, double *f1, double *f2, double *f3, int nf,
+                         int kind, hipStream_t s)
+{
+    int ncol = (kind == 3 || kind == 4) ? 7 : 6;
+    int n = 6 * (g->NXM + 7) + ncol * g->NYM;
+    hipLaunchKernelGGL(k_ghost, dim3((n + 255) / 256), dim3(256), 0, s, *g, f0, f1, f2, f3, nf, kind);
+}
+
+extern "C" void pvk_ghost_multi(const pvk_grid *g, double *f0, double *f1, double *f2, double *f3,
+                                int nf, int kind, hipStream_t s)
+{
+    launch_ghost(g, f0, f1, f2, f3, nf, kind, s);
+}
+extern "C" void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s) { launch_ghost(g, f, 0, 0, 0, 1, 0, s); }
+extern "C" void pvk_halo_cell(const pvk_grid *g, double *f, hipStream_t s) { launch_ghost(g, f, 0, 0, 0, 1, 2, s); }
+extern "C" void pvk_halo_u(const pvk_grid *g, double *f, hipStream_t s) { launch_ghost(g, f, 0, 0, 0, 1, 3, s); }
+extern "C" void pvk_p_bc(const pvk_grid *g, double *f, hipStream_t s) { launch_ghost(g, f, 0, 0, 0, 1, 1, s); }
+extern "C" void pvk_vel_bc(const pvk_grid *g, double *f, int comp, int type, hipStream_t s)
+{
+    if (comp == 'u') { launch_ghost(g, f, 0, 0, 0, 1, 4, s); return; }
+    if (type == 0) {
+        hipLaunchKernelGGL(k_zero_row, dim3((g->NXM + 7 + 255) / 256), dim3(256), 0, s, *g, f, 0);
+        launch_ghost(g, f, 0, 0, 0, 1, 5, s);
+    } else {
+        launch_ghost(g, f, 0, 0, 0, 1, 6, s);
+    }
+}
+
+__global__ void k_fill(double *f, size_t n, double val)
+{
+    size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) f[t] = val;
+}
+extern "C" void pvk_fill(double *f, size_t n, double val, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, f, n, val);
+}
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard stencils
+// ---------------------------------------------------------------------------
+// TwodOps_scalar_laplacian, planar (TwodOps.c:159-199)
+__device__ __forceinline__ double lap_at(const pvk_grid &g, const double *f, int i, int j)
+{
+    const int P = g.P;
+    double c = AT(f, i, j);
+    double dfdxE = (AT(f, i + 1, j) - c) * g.idx_c[i];
+    double dfdxW = (c - AT(f, i - 1, j)) * ((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]);
+    double dfdyN = (AT(f, i, j + 1) - c) * g.idy_c[j];
+    double dfdyS = (c - AT(f, i, j - 1)) * ((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]);
+    return (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
+}
+
+// Same formula applied to five already-evaluated centre/neighbour values.
+__device__ __forceinline__ double lap_vals(const pvk_grid &g, int i, int j, double c, double E,
+                                           double W, double N, double S)
+{
+    double dfdxE = (E - c) * g.idx_c[i];
+    double dfdxW = (c - W) * ((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]);
+    double dfdyN = (N - c) * g.idy_c[j];
+    double dfdyS = (c - S) * ((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]);
+    return (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
+}
+
+// L(L f) at interior (i,j) where the inner Laplacian obeys the CH boundary
+// rules of its own ghost fill: column NXM is column 0, row -1 is row 0 and
+// row NYM is row NYM-1 (compute_laplacian then set_boundary_values).
+__device__ __forceinline__ double bilap_at(const pvk_grid &g, const double *f, int i, int j)
+{
+    const int NXM = g.NXM, NYM = g.NYM;
+    double L0 = lap_at(g, f, i, j);
+    double LE = lap_at(g, f, (i + 1 == NXM) ? 0 : i + 1, j);
+    double LW = lap_at(g, f, (i == 0) ? NXM - 1 : i - 1, j);
+    double LN = (j + 1 == NYM) ? L0 : lap_at(g, f, i, j + 1);
+    double LS = (j == 0) ? L0 : lap_at(g, f, i, j - 1);
+    return lap_vals(g, i, j, L0, LE, LW, LN, LS);
+}
+
+// S(c) = 0.5 c - 1.5 c^2 + c^3 - c cs (1 - c - cs), cs = 0 (VOF_DIFFUSE.c:865-885)
+__device__ __forceinline__ double bulkS(double c)
+{
+    const double cs = 0.0;
+    return 0.5 * c - 1.5 * c * c + c * c * c - c * cs * (1.0 - c - cs);
+}
+
+// diffuse_weno5_left (VOF_DIFFUSE.c:431-453); pow(x, 2.0) evaluated as x*x
+__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4)
+{
+    const double eps = 1e-6;
+    double q0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
+    double q1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
+    double q2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
+    double t0 = v0 - 2.0 * v1 + v2, t1 = v0 - 4.0 * v1 + 3.0 * v2;
+    double t2 = v1 - 2.0 * v2 + v3, t3 = v1 - v3;
+    double t4 = v2 - 2.0 * v3 + v4, t5 = 3.0 * v2 - 4.0 * v3 + v4;
+    double b0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (t1 * t1);
+    double b1 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (t3 * t3);
+    double b2 = 13.0 / 12.0 * (t4 * t4) + 0.25 * (t5 * t5);
+    double a0 = 0.1 / ((eps + b0) * (eps + b0));
+    double a1 = 0.6 / ((eps + b1) * (eps + b1));
+    double a2 = 0.3 / ((eps + b2) * (eps + b2));
+    double asum = a0 + a1 + a2;
+    return (a0 * q0 + a1 * q1 + a2 * q2) / asum;
+}
+
+// diffuse_face_state_x / _y (VOF_DIFFUSE.c:455-483); face i sits between i-1 and i
+__device__ __forceinline__ double face_x(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
+{
+    const int P = g.P;
+    if (wo == 5) {
+        if (vel >= 0.0) return weno5(AT(C, i - 3, j), AT(C, i - 2, j), AT(C, i - 1, j), AT(C, i, j), AT(C, i + 1, j));
+        return weno5(AT(C, i + 2, j), AT(C, i + 1, j), AT(C, i, j), AT(C, i - 1, j), AT(C, i - 2, j));
+    }
+    return (vel >= 0.0) ? AT(C, i - 1, j) : AT(C, i, j);
+}
+__device__ __forceinline__ double face_y(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
+{
+    const int P = g.P;
+    if (wo == 5) {
+        if (vel >= 0.0) return weno5(AT(C, i, j - 3), AT(C, i, j - 2), AT(C, i, j - 1), AT(C, i, j), AT(C, i, j + 1));
+        return weno5(AT(C, i, j + 2), AT(C, i, j + 1), AT(C, i, j), AT(C, i, j - 1), AT(C, i, j - 2));
+    }
+    return (vel >= 0.0) ? AT(C, i, j - 1) : AT(C, i, j);
+}
+
+// -(div(u C)) at interior cell (i,j): VOF_DIFFUSE.c:931-996 + TwodOps.c:201-229
+__device__ __forceinline__ double weno_rhs_at(const pvk_grid &g, const pvk_phys &ph, const double *C,
+                                              const double *u, const double *v, int i, int j)
+{
+    const int P = g.P;
+    double uw = AT(u, i, j), ue = AT(u, i + 1, j);
+    double vs = AT(v, i, j), vn = AT(v, i, j + 1);
+    double fxW = uw * face_x(g, C, i, j, uw, ph.weno_order);
+    double fxE = ue * face_x(g, C, i + 1, j, ue, ph.weno_order);
+    double fyS = vs * face_y(g, C, i, j, vs, ph.weno_order);
+    double fyN = vn * face_y(g, C, i, j + 1, vn, ph.weno_order);
+    double div_x = (fxE - fxW) / g.dx_c[i];
+    double div_y = (fyN - fyS) / g.dy_c[j];
+    double div = div_x + div_y;
+    return -div;
+}
+
+// L(S(C)) / Pe at interior (i,j) (VOF_DIFFUSE.c:998-1016); C ghosts valid.
+__device__ __forceinline__ double diffusion_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
+{
+    const int P = g.P;
+    double S0 = bulkS(AT(C, i, j));
+    double lapS = lap_vals(g, i, j, S0, bulkS(AT(C, i + 1, j)), bulkS(AT(C, i - 1, j)),
+                           bulkS(AT(C, i, j + 1)), bulkS(AT(C, i, j - 1)));
+    return lapS / ph.Pe_CH;
+}
+
+// measure-weighted CH product contribution of interior cell (i,j) including
+// its duplicates in the G range (column NXM, row NYM); the second z slab is
+// accounted for by doubling the final sum.
+__device__ __forceinline__ double ch_w(const pvk_grid &g, int i, int j, double a, double b)
+{
+    double m = g.dx_u[i] * g.dy_v[j] * g.slab;
+    double s = (m * a) * b;
+    bool ix = (i == 0), jy = (j == g.NYM - 1);
+    if (ix) s += ((g.dx_u[g.NXM] * g.dy_v[j] * g.slab) * a) * b;
+    if (jy) s += ((g.dx_u[i] * g.dy_v[g.NYM] * g.slab) * a) * b;
+    if (ix && jy) s += ((g.dx_u[g.NXM] * g.dy_v[g.NYM] * g.slab) * a) * b;
+    return s;
+}
+
+#define CELL_IJ                                                   \
+    const int i = blockIdx.x * BX + threadIdx.x;                  \
+    const int j = blockIdx.y * BY + threadIdx.y;                  \
+    const bool in = (i < g.NXM) && (j < g.NYM);                   \
+    const int P = g.P;                                            \
+    (void)P;
+
+__global__ void k_laplacian(pvk_grid g, const double *in_f, double *out)
+{
+    CELL_IJ
+    if (in) AT(out, i, j) = lap_at(g, in_f, i, j);
+}
+extern "C" void pvk_laplacian(const pvk_grid *g, const double *in, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_laplacian, grid2d(g), dim3(BX, BY), 0, s, *g, in, out);
+}
+
+__global__ void k_bulk_S(pvk_grid g, const double *C, double *S)
+{
+    CELL_IJ
+    if (in) AT(S, i, j) = bulkS(AT(C, i, j));
+}
+extern "C" void pvk_bulk_S(const pvk_grid *g, const pvk_phys *ph, const double *C, double *S, hipStream_t s)
+{
+    (void)ph;
+    hipLaunchKernelGGL(k_bulk_S, grid2d(g), dim3(BX, BY), 0, s, *g, C, S);
+}
+
+__global__ void k_weno_advect(pvk_grid g, pvk_phys ph, const double *C, const double *u, const double *v, double *rhs)
+{
+    CELL_IJ
+    if (in) AT(rhs, i, j) = weno_rhs_at(g, ph, C, u, v, i, j);
+}
+extern "C" void pvk_weno_advect(const pvk_grid *g, const pvk_phys *ph, const double *C, const double *u,
+                                const double *v, double *rhs, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_weno_advect, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, u, v, rhs);
+}
+
+__global__ void k_explicit_diffusion(pvk_grid g, pvk_phys ph, const double *C, double *rhs)
+{
+    CELL_IJ
+    if (in) AT(rhs, i, j) = diffusion_at(g, ph, C, i, j);
+}
+extern "C" void pvk_explicit_diffusion(const pvk_grid *g, const pvk_phys *ph, const double *C, double *rhs,
+                                       hipStream_t s)
+{
+    hipLaunchKernelGGL(k_explicit_diffusion, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, rhs);
+}
+
+// Explicit CH right-hand side and start of the biharmonic CG in one sweep:
+//   rc = -div(uC) + L S(C) / Pe                         (VOF_DIFFUSE.c:1284-1296)
+//   b  = a_dt C + (g/b) rc [+ (z/b) rn] - (Cn^2/Pe) L^2 C (:1298-1317, :1047-1057)
+//   r  = p = b - (a_dt C + (Cn^2/Pe) L^2 C)               (:356-364)
+// rc overwrites rn in place (history shift ch_rhs_n <- rc, :1357-1358).
+__global__ void k_ch_rhs_init(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *C,
+                              const double *u, const double *v, double *rn, double *r, double *p,
+                              double *part, int nb)
+{
+    CELL_IJ
+    double vals[2] = {0.0, 0.0};
+    if (in) {
+        const double dt = sc->dt;
+        const double a_dt = 1.0 / (dt * c_BET[stage]);
+        double rc = weno_rhs_at(g, ph, C, u, v, i, j);
+        rc += diffusion_at(g, ph, C, i, j);
+        double c = AT(C, i, j);
+        double rv = (stage == 0) ? a_dt * c + c_GAMB[0] * rc
+                                 : a_dt * c + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
+        double L2 = bilap_at(g, C, i, j);
+        double b = rv - ph.bih * L2;
+        double Ap = a_dt * c + ph.bih * L2;
+        double rr = b - Ap;
+        AT(rn, i, j) = rc;
+        AT(r, i, j) = rr;
+        AT(p, i, j) = rr;
+        vals[0] = ch_w(g, i, j, b, b);
+        vals[1] = ch_w(g, i, j, rr, rr);
+    }
+    block_partials<2>(vals, part, nb);
+}
+extern "C" void pvk_ch_rhs_init(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
+                                const double *C, const double *u, const double *v, const double *rn,
+                                double *rc, double *r, double *p, const pvk_red *rd, hipStream_t s)
+{
+    (void)rc;
+    hipLaunchKernelGGL(k_ch_rhs_init, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, C, u, v,
+                       (double *)rn, r, p, rd->part, rd->nblocks);
+}
+
+// Biharmonic CG start from a given b (b already holds rhs - (Cn^2/Pe) L^2 C).
+__global__ void k_ch_cg_init_b(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *C,
+                               const double *b, double *r, double *p, double *part, int nb)
+{
+    CELL_IJ
+    double vals[2] = {0.0, 0.0};
+    if (in) {
+        const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+        double c = AT(C, i, j);
+        double Ap = a_dt * c + ph.bih * bilap_at(g, C, i, j);
+        double bb = AT(b, i, j);
+        double rr = bb - Ap;
+        AT(r, i, j) = rr;
+        AT(p, i, j) = rr;
+        vals[0] = ch_w(g, i, j, bb, bb);
+        vals[1] = ch_w(g, i, j, rr, rr);
+    }
+    block_partials<2>(vals, part, nb);
+}
+
+// b -= (Cn^2/Pe) L^2 C on the interior (VOF_DIFFUSE.c:1047-1057)
+__global__ void k_ch_sub_bih(pvk_grid g, pvk_phys ph, const double *C, double *b)
+{
+    CELL_IJ
+    if (in) AT(b, i, j) -= ph.bih * bilap_at(g, C, i, j);
+}
+
+extern "C" void pvk_ch_cg_init_from_b(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
+                                      const double *C, double *b, double *r, double *p, const pvk_red *rd,
+                                      hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ch_sub_bih, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, b);
+    hipLaunchKernelGGL(k_ch_cg_init_b, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, C, b, r, p,
+                       rd->part, rd->nblocks);
+}
+
+// CG bookkeeping after the start sweep (VOF_DIFFUSE.c:366-381)
+__global__ void k_ch_finalize_init(pvk_scal *sc, int maxit, double tol, const double *part, int nb)
+{
+    double bb = 2.0 * final_sum(part, nb, 0, false);
+    double rr = 2.0 * final_sum(part, nb, 1, false);
+    if (threadIdx.x == 0) {
+        double bnorm = sqrt(bb);
+        if (bnorm < 1e-30) bnorm = 1.0;
+        sc->ch_bnorm = bnorm;
+        sc->ch_rz = rr;
+        sc->ch_rel = sqrt(rr) / bnorm;
+        sc->ch_iter = 0;
+        sc->ch_maxit = maxit;
+        sc->ch_done = (sc->ch_rel < tol || maxit <= 0) ? 1 : 0;
+        sc->red[0] = bb;
+        sc->red[1] = rr;
+    }
+}
+extern "C" void pvk_ch_finalize_init(const pvk_grid *g, pvk_scal *sc, int maxit, double tol, const pvk_red *rd,
+                                     hipStream_t s)
+{
+    (void)g;
+    hipLaunchKernelGGL(k_ch_finalize_init, dim3(1), dim3(256), 0, s, sc, maxit, tol, rd->part, rd->nblocks);
+}
+
+// Ap = a_dt p + (Cn^2/Pe) L^2 p, partial <p, Ap> (VOF_DIFFUSE.c:318-337, :384-386)
+__global__ void k_ch_apply(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *p,
+                           double *Ap, double *part, int nb)
+{
+    if (sc->ch_done) return;
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) {
+        const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+        double pc = AT(p, i, j);
+        double ap = a_dt * pc + ph.bih * bilap_at(g, p, i, j);
+        AT(Ap, i, j) = ap;
+        vals[0] = ch_w(g, i, j, pc, ap);
+    }
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_ch_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const double *p,
+                             double *Ap, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ch_apply, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, p, Ap, rd->part,
+                       rd->nblocks);
+}
+
+__global__ void k_ch_alpha(pvk_scal *sc, const double *part, int nb)
+{
+    if (sc->ch_done) return;
+    double pAp = 2.0 * final_sum(part, nb, 0, false);
+    if (threadIdx.x == 0) {
+        if (fabs(pAp) < 1e-30) sc->ch_done = 2;
+        else sc->ch_alpha = sc->ch_rz / pAp;
+    }
+}
+extern "C" void pvk_ch_alpha(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s)
+{
+    (void)g;
+    hipLaunchKernelGGL(k_ch_alpha, dim3(1), dim3(256), 0, s, sc, rd->part, rd->nblocks);
+}
+
+// x += alpha p; r -= alpha Ap; partial <r, r> (VOF_DIFFUSE.c:390-405)
+__global__ void k_ch_update(pvk_grid g, const pvk_scal *sc, double *x, const double *p, double *r,
+                            const double *Ap, double *part, int nb)
+{
+    if (sc->ch_done) return;
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) {
+        const double alpha = sc->ch_alpha;
+        AT(x, i, j) += alpha * AT(p, i, j);
+        double rr = AT(r, i, j) - alpha * AT(Ap, i, j);
+        AT(r, i, j) = rr;
+        vals[0] = ch_w(g, i, j, rr, rr);
+    }
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_ch_update(const pvk_grid *g, pvk_scal *sc, double *x, const double *p, double *r,
+                              const double *Ap, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ch_update, grid2d(g), dim3(BX, BY), 0, s, *g, sc, x, p, r, Ap, rd->part, rd->nblocks);
+}
+
+// rel = ||r|| / ||b||, stop test, beta (VOF_DIFFUSE.c:403-414)
+__global__ void k_ch_beta(pvk_scal *sc, double tol, const double *part, int nb)
+{
+    if (sc->ch_done) return;
+    double rr = 2.0 * final_sum(part, nb, 0, false);
+    if (threadIdx.x == 0) {
+        sc->ch_iter += 1;
+        sc->ch_rel = sqrt(rr) / sc->ch_bnorm;
+        if (sc->ch_rel < tol) { sc->ch_done = 1; return; }
+        sc->ch_beta = rr / (sc->ch_rz + 1e-30);
+        sc->ch_rz = rr;
+        if (sc->ch_iter >= sc->ch_maxit) sc->ch_done = 3;
+    }
+}
+extern "C" void pvk_ch_beta(const pvk_grid *g, pvk_scal *sc, double tol, const pvk_red *rd, hipStream_t s)
+{
+    (void)g;
+    hipLaunchKernelGGL(k_ch_beta, dim3(1), dim3(256), 0, s, sc, tol, rd->part, rd->nblocks);
+}
+
+// p = z + beta p with z = r (VOF_DIFFUSE.c:416-422)
+__global__ void k_ch_pupdate(pvk_grid g, const pvk_scal *sc, const double *r, double *p)
+{
+    if (sc->ch_done) return;
+    CELL_IJ
+    if (in) AT(p, i, j) = AT(r, i, j) + sc->ch_beta * AT(p, i, j);
+}
+extern "C" void pvk_ch_pupdate(const pvk_grid *g, pvk_scal *sc, const double *r, double *p, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_ch_pupdate, grid2d(g), dim3(BX, BY), 0, s, *g, sc, r, p);
+}
+
+// diffuse_bound_liquid_fraction + update_phase_cache + density/viscosity
+// (VOF_DIFFUSE.c:1093-1127, :108-126, :1453-1488); partial of the mass delta.
+__global__ void k_clamp_props(pvk_grid g, pvk_phys ph, double *C, double *G, double *rho, double *mu,
+                              int do_clamp, double *part, int nb)
+{
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) {
+        const double cs = 0.0;
+        double old_cl = AT(C, i, j), new_cl = old_cl;
+        if (do_clamp) {
+            double max_cl = 1.0 - cs;
+            if (max_cl < 0.0) max_cl = 0.0;
+            if (max_cl > 1.0) max_cl = 1.0;
+            if (new_cl < 0.0) new_cl = 0.0;
+            if (new_cl > max_cl) new_cl = max_cl;
+            double d = new_cl - old_cl;
+            double m = (g.dx_u[i] * g.dy_v[j] * g.slab) / 2.0;
+            double s = d * m;
+            if (i == 0) s += d * ((g.dx_u[g.NXM] * g.dy_v[j] * g.slab) / 2.0);
+            if (j == g.NYM - 1) s += d * ((g.dx_u[i] * g.dy_v[g.NYM] * g.slab) / 2.0);
+            if (i == 0 && j == g.NYM - 1) s += d * ((g.dx_u[g.NXM] * g.dy_v[g.NYM] * g.slab) / 2.0);
+            vals[0] = s;
+            AT(C, i, j) = new_cl;
+        }
+        double gg = 1.0 - new_cl - cs;
+        gg = (gg < 0.0) ? 0.0 : ((gg > 1.0) ? 1.0 : gg);
+        AT(G, i, j) = gg;
+        AT(rho, i, j) = new_cl + 1.0 * cs + ph.rho_g * gg;
+        AT(mu, i, j) = new_cl + 1.0 * cs + ph.mu_g * gg;
+    }
+    if (do_clamp) block_partials<1>(vals, part, nb);
+}
+__global__ void k_clamp_finalize(pvk_scal *sc, const double *part, int nb)
+{
+    double s = 2.0 * final_sum(part, nb, 0, false);
+    if (threadIdx.x == 0) sc->clamp_delta = s;
+}
+extern "C" void pvk_clamp_props(const pvk_grid *g, const pvk_phys *ph, double *C, double *G, double *rho,
+                                double *mu, const pvk_red *rd, pvk_scal *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clamp_props, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, G, rho, mu, 1, rd->part,
+                       rd->nblocks);
+    hipLaunchKernelGGL(k_clamp_finalize, dim3(1), dim3(256), 0, s, sc, rd->part, rd->nblocks);
+    launch_ghost(g, C, G, rho, mu, 4, 0, s);
+}
+extern "C" void pvk_phase_props(const pvk_grid *g, const pvk_phys *ph, const double *C, double *G, double *rho,
+                                double *mu, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clamp_props, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, (double *)C, G, rho, mu, 0,
+                       (double *)0, 0);
+    launch_ghost(g, G, rho, mu, 0, 3, 0, s);
+}
+
+// psi_LG (VOF_DIFFUSE.c:907-929) and the potential-form capillary force on
+// u and v faces (VOF_DIFFUSE.c:1363-1445).  Thread i handles u-face
+// iu = (i == 0 ? NXM : i) (unknowns 1..NXM) and v-face (i, j), j >= 1.
+__device__ __forceinline__ double psi_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
+{
+    const int P = g.P;
+    double c = AT(C, i, j);
+    return c * c * c - 1.5 * c * c + 0.5 * c - ph.Cn * ph.Cn * lap_at(g, C, i, j);
+}
+
+__global__ void k_psi_fsigma(pvk_grid g, pvk_phys ph, const double *C, const double *G, double *psi_out,
+                             double *fx, double *fy)
+{
+    CELL_IJ
+    if (!in) return;
+    const double scale = 2.0 * 6.0 * sqrt(2.0) / (ph.Cn * ph.We + 1e-30);
+    const double cs = 0.0;
+    double psi_c = psi_at(g, ph, C, i, j);
+    AT(psi_out, i, j) = psi_c;
+    {   // u face iu, between cells iu-1 and iu
+        int iu = (i == 0) ? g.NXM : i;
+        double fval = 0.0;
+        if (j < g.NYM) {
+            double cl = 0.5 * (AT(C, iu, j) + AT(C, iu - 1, j));
+            double cg = 0.5 * (AT(G, iu, j) + AT(G, iu - 1, j));
+            if (!(cl < 0.005 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05)) {
+                double psi_e = (iu == g.NXM) ? psi_at(g, ph, C, 0, j) : psi_c;
+                double psi_w = psi_at(g, ph, C, iu - 1, j);
+                double psi_f = 0.5 * (psi_e + psi_w);
+                double gradC = (AT(C, iu, j) - AT(C, iu - 1, j)) * g.idx_c[iu - 1];
+                fval = scale * psi_f * gradC;
+            }
+        }
+        AT(fx, iu, j) = fval;
+    }
+    if (j >= 1) {   // v face (i, j), between cells j-1 and j
+        double fval = 0.0;
+        double cl = 0.5 * (AT(C, i, j) + AT(C, i, j - 1));
+        double cg = 0.5 * (AT(G, i, j) + AT(G, i, j - 1));
+        if (!(cl < 0.005 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05)) {
+            double psi_f = 0.5 * (psi_c + psi_at(g, ph, C, i, j - 1));
+            double gradC = (AT(C, i, j) - AT(C, i, j - 1)) * g.idy_c[j - 1];
+            fval = scale * psi_f * gradC;
+        }
+        AT(fy, i, j) = fval;
+    }
+}
+extern "C" void pvk_psi_fsigma(const pvk_grid *g, const pvk_phys *ph, const double *C, const double *G,
+                               double *lap, double *psi, double *fx, double *fy, hipStream_t s)
+{
+    (void)lap;
+    hipLaunchKernelGGL(k_psi_fsigma, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, G, psi, fx, fy);
+}
+
+// ---------------------------------------------------------------------------
+// momentum
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double harm(double a, double b) { return 2.0 * (a) * (b) / ((a) + (b) + 1e-12); }
+__device__ __forceinline__ double avg2(double a, double b) { return 0.5 * ((a) + (b)); }
+
+// Velocity_cell_center (Velocity.c:278-357) on the G range.
+__global__ void k_cell_center(pvk_grid g, const double *u, const double *v, double *ub, double *vb)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int P = g.P;
+    if (i <= g.NXM && j <= g.NYM) {
+        AT(ub, i, j) = 0.5 * (AT(u, i, j) + AT(u, i + 1, j));
+        AT(vb, i, j) = 0.5 * (AT(v, i, j) + AT(v, i, j + 1));
+    }
+}
+extern "C" void pvk_cell_center(const pvk_grid *g, const double *u, const double *v, double *ub, double *vb,
+                                hipStream_t s)
+{
+    dim3 gr((g->NXM + 1 + BX - 1) / BX, (g->NYM + 1 + BY - 1) / BY);
+    hipLaunchKernelGGL(k_cell_center, gr, dim3(BX, BY), 0, s, *g, u, v, ub, vb);
+}
+
+// Explicit, implicit-diagonal and explicit-viscous terms for u at face
+// (iu, j) (Velocity.c:366-762, VOF + VOF_DIFFUSE branches, planar).
+__device__ __forceinline__ void u_terms(const pvk_grid &g, const pvk_phys &ph, const double *u, const double *v,
+                                        const double *mu, int i, int j, double &ex, double &im, double &ve)
+{
+    const int P = g.P;
+    const double iRe = ph.iRe;
+    double muE = AT(mu, i, j), muW = AT(mu, i - 1, j);
+    double mu_yp = 2.0 * (0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) *
+                   (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) /
+                   ((0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) +
+                    (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) + 1e-12);
+    double mu_ym = 2.0 * (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) *
+                   (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i, j - 1))) /
+                   ((0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) +
+                    (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i, j - 1))) + 1e-12);
+    double uc = AT(u, i, j);
+    double dudxE = (AT(u, i + 1, j) - uc) * g.idx_u[i];
+    double dudxW = (uc - AT(u, i - 1, j)) * g.idx_u[i - 1];
+    double dudyN = (AT(u, i, j + 1) - uc) * g.idy_c[j];
+    double dudyS = (uc - AT(u, i, j - 1)) * g.idy_c[j - 1];
+    double d2udy2 = iRe * (mu_yp * dudyN - mu_ym * dudyS) * g.idy_v[j];
+    double dvdxN = (AT(v, i, j + 1) - AT(v, i - 1, j + 1)) * g.idx_c[i - 1];
+    double dvdxS = (AT(v, i, j) - AT(v, i - 1, j)) * g.idx_c[i - 1];
+    double d2udx2 = iRe * ((muE * dudxE - muW * dudxW) * g.idx_c[i - 1]);
+    double ddydvdx = iRe * (mu_yp * dvdxN - mu_ym * dvdxS) * g.idy_v[j];
+    double vCN = 0.5 * (AT(v, i, j + 1) + AT(v, i - 1, j + 1));
+    double vCS = 0.5 * (AT(v, i, j) + AT(v, i - 1, j));
+    double ubE = 0.5 * (uc + AT(u, i + 1, j));            // u_bc[i]
+    double ubW = 0.5 * (AT(u, i - 1, j) + uc);            // u_bc[i-1]
+    double uuE = ubE * ubE, uuW = ubW * ubW;
+    double uvN = 0.5 * (uc + AT(u, i, j + 1)) * vCN;
+    double uvS = 0.5 * (uc + AT(u, i, j - 1)) * vCS;
+    double duudx = (uuE - uuW) * g.idx_c[i - 1];
+    double duvdy = (uvN - uvS) * g.idy_v[j];
+    ex = -(duudx + duvdy + 0.0);
+    ve = d2udx2 + ddydvdx + 0.0;
+    im = d2udx2 + d2udy2 + 0.0 + 0.0;
+}
+
+// Same for v at face (i, j) (Velocity.c:771-1144).
+__device__ __forceinline__ void v_terms(const pvk_grid &g, const pvk_phys &ph, const double *u, const double *v,
+                                        const double *mu, int i, int j, double &ex, double &im, double &ve)
+{
+    const int P = g.P;
+    const double iRe = ph.iRe;
+    double mu_xp = 2.0 * (0.5 * (AT(mu, i + 1, j - 1) + AT(mu, i + 1, j))) *
+                   (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) /
+                   ((0.5 * (AT(mu, i + 1, j - 1) + AT(mu, i + 1, j))) +
+                    (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) + 1e-12);
+    double mu_xm = 2.0 * (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) *
+                   (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i - 1, j))) /
+                   ((0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) +
+                    (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i - 1, j))) + 1e-12);
+    double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
+    double vc = AT(v, i, j);
+    double dvdxE = (AT(v, i + 1, j) - vc) * g.idx_c[i];
+    double dvdxW = (vc - AT(v, i - 1, j)) * g.idx_c[i - 1];
+    double dvdyN = (AT(v, i, j + 1) - vc) * g.idy_v[j];
+    double dvdyS = (vc - AT(v, i, j - 1)) * g.idy_v[j - 1];
+    double d2vdx2 = iRe * ((mu_xp * dvdxE - mu_xm * dvdxW) * g.idx_u[i]);
+    double d2vdy2 = iRe * (mu_yp * dvdyN - mu_ym * dvdyS) * g.idy_c[j - 1];
+    double dudyE = (AT(u, i + 1, j) - AT(u, i + 1, j - 1)) * g.idy_c[j - 1];
+    double dudyW = (AT(u, i, j) - AT(u, i, j - 1)) * g.idy_c[j - 1];
+    double ddxdudy = iRe * ((mu_xp * dudyE - mu_xm * dudyW) * g.idx_u[i]);
+    double uCE = 0.5 * (AT(u, i + 1, j) + AT(u, i + 1, j - 1));
+    double uCW = 0.5 * (AT(u, i, j) + AT(u, i, j - 1));
+    double vuE = 0.5 * (vc + AT(v, i + 1, j)) * uCE;
+    double vuW = 0.5 * (vc + AT(v, i - 1, j)) * uCW;
+    double vbN = 0.5 * (vc + AT(v, i, j + 1));            // v_bc[j]
+    double vbS = 0.5 * (AT(v, i, j - 1) + vc);            // v_bc[j-1]
+    double vvN = vbN * vbN, vvS = vbS * vbS;
+    double dvudx = (vuE - vuW) * g.idx_u[i];
+    double dvvdy = (vvN - vvS) * g.idy_c[j - 1];
+    ex = -(dvudx + dvvdy + 0.0);
+    ve = ddxdudy + d2vdy2 + 0.0;
+    im = d2vdx2 + d2vdy2 + 0.0;
+}
+
+// Fused momentum right-hand side per face (Velocity.c:1494-1947) plus the
+// lagged capillary force (VOF_SurfaceTension.c:1983-2041) and gravity
+// (:2075-2154).  mode bit 1: store ex/im/ve; bit 2: assemble rhs and shift
+// the explicit history; bit 4: add f_sigma_old and gravity.
+__global__ void k_mom_terms(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp,
+                            const double *u, const double *v, const double *mu, const double *rho,
+                            const double *p, double *eold, double *vold, const double *fold, double *rhs,
+                            double *exo, double *imo, double *veo, int mode)
+{
+    CELL_IJ
+    if (!in) return;
+    int ii = i;
+    if (comp == 'u') ii = (i == 0) ? g.NXM : i;
+    else if (j == 0) return;
+    double ex, im, ve;
+    if (comp == 'u') u_terms(g, ph, u, v, mu, ii, j, ex, im, ve);
+    else v_terms(g, ph, u, v, mu, ii, j, ex, im, ve);
+    if (mode & 1) { AT(exo, ii, j) = ex; AT(imo, ii, j) = im; AT(veo, ii, j) = ve; }
+    if (mode & 2) {
+        const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+        const double G = c_GAMB[stage], Z = c_ZETB[stage];
+        double dp, rho_face, vel;
+        if (comp == 'u') {
+            dp = (AT(p, ii, j) - AT(p, ii - 1, j)) * g.idx_c[ii - 1];
+            rho_face = 0.5 * (AT(rho, ii, j) + AT(rho, ii - 1, j));
+            vel = AT(u, ii, j);
+        } else {
+            dp = (AT(p, ii, j) - AT(p, ii, j - 1)) * g.idy_c[j - 1];
+            rho_face = 0.5 * (AT(rho, ii, j) + AT(rho, ii, j - 1));
+            vel = AT(v, ii, j);
+        }
+        double conv = G * ex + Z * AT(eold, ii, j);
+        conv *= rho_face;
+        double r = rho_face * a_dt * vel + conv + im + (G * ve + Z * AT(vold, ii, j)) - 2.0 * dp;
+        if (comp == 'u') r += 2.0 * (-ph.dp_dx);
+        if (mode & 4) {
+            r += AT(fold, ii, j);
+            r += 2.0 * rho_face * ph.Ri * (comp == 'u' ? ph.gx : ph.gy);
+        }
+        AT(rhs, ii, j) = r;
+        AT(eold, ii, j) = ex;
+        AT(vold, ii, j) = ve;
+    }
+}
+extern "C" void pvk_mom_terms(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc, int comp,
+                              const double *u, const double *v, const double *mu, const double *rho,
+                              const double *p, double *eold, double *vold, const double *fold, double *rhs,
+                              double *ex_out, double *im_out, double *ve_out, int write_terms, int add_force,
+                              hipStream_t s)
+{
+    int mode = write_terms;
+    hipLaunchKernelGGL(k_mom_terms, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, u, v, mu, rho, p,
+                       eold, vold, fold, rhs, ex_out, im_out, ve_out, mode | (add_force ? 4 : 0));
+}
+
+// Force terms alone (VOF_apply_f_sigma_old / Velocity_add_gravity_2_RHS).
+__global__ void k_mom_force(pvk_grid g, pvk_phys ph, int comp, const double *rho, const double *fold,
+                            double *rhs, int which)
+{
+    CELL_IJ
+    if (!in) return;
+    int ii = i;
+    if (comp == 'u') ii = (i == 0) ? g.NXM : i;
+    else if (j == 0) return;
+    double rho_face = (comp == 'u') ? 0.5 * (AT(rho, ii, j) + AT(rho, ii - 1, j))
+                                    : 0.5 * (AT(rho, ii, j) + AT(rho, ii, j - 1));
+    if (which & 1) AT(rhs, ii, j) += AT(fold, ii, j);
+    if (which & 2) AT(rhs, ii, j) += 2.0 * rho_face * ph.Ri * (comp == 'u' ? ph.gx : ph.gy);
+}
+extern "C" void pvk_mom_force(const pvk_grid *g, const pvk_phys *ph, int comp, const double *rho,
+                              const double *fold, double *rhs, int which, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_force, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, comp, rho, fold, rhs, which);
+}
+
+// Jacobi diagonal (lsolver/msolve_cg.c:103-249) and operator (:255-493).
+__device__ __forceinline__ double mom_diag(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                           const double *rho, const double *mu, int comp, int i, int j)
+{
+    const int P = g.P;
+    const double iRe = ph.iRe;
+    double rho_face, d_time, dx, dy;
+    if (comp == 'u') {
+        rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
+        d_time = rho_face / (alpha_k * dt);
+        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
+        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
+        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
+        dx = iRe * ((mu_xp * g.idx_u[i] + mu_xm * g.idx_u[i - 1]) * g.idx_c[i - 1]);
+        dy = iRe * (mu_yp * g.idy_c[j] + mu_ym * g.idy_c[j - 1]) * g.idy_v[j];
+    } else {
+        rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
+        d_time = rho_face / (alpha_k * dt);
+        double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
+        double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
+        double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
+        dx = iRe * ((mu_xp * g.idx_c[i] + mu_xm * g.idx_c[i - 1]) * g.idx_u[i]);
+        dy = iRe * (mu_yp * g.idy_v[j] + mu_ym * g.idy_v[j - 1]) * g.idy_c[j - 1];
+    }
+    return d_time + dx + dy + 0.0 + 0.0;
+}
+
+__device__ __forceinline__ double mom_op(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                         const double *rho, const double *mu, const double *x, int comp, int i,
+                                         int j)
+{
+    const int P = g.P;
+    const double iRe = ph.iRe;
+    double xc = AT(x, i, j), op = 0.0;
+    if (comp == 'u') {
+        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
+        double factor_time = rho_face / (alpha_k * dt);
+        double Ax_val = factor_time * xc;
+        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
+        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
+        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
+        double flux_xp = mu_xp * (AT(x, i + 1, j) - xc) * g.idx_u[i];
+        double flux_xm = mu_xm * (xc - AT(x, i - 1, j)) * g.idx_u[i - 1];
+        double flux_yp = mu_yp * (AT(x, i, j + 1) - xc) * g.idy_c[j];
+        double flux_ym = mu_ym * (xc - AT(x, i, j - 1)) * g.idy_c[j - 1];
+        op += iRe * ((flux_xp - flux_xm) * g.idx_c[i - 1]);
+        op += iRe * (flux_yp - flux_ym) * g.idy_v[j];
+        op += 0.0;
+        return Ax_val - op;
+    } else {
+        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
+        double factor_time = rho_face / (alpha_k * dt);
+        double Ax_val = factor_time * xc;
+        double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
+        double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
+        double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
+        double flux_xp = mu_xp * (AT(x, i + 1, j) - xc) * g.idx_c[i];
+        double flux_xm = mu_xm * (xc - AT(x, i - 1, j)) * g.idx_c[i - 1];
+        double flux_yp = mu_yp * (AT(x, i, j + 1) - xc) * g.idy_v[j];
+        double flux_ym = mu_ym * (xc - AT(x, i, j - 1)) * g.idy_v[j - 1];
+        op += iRe * ((flux_xp - flux_xm) * g.idx_u[i]);
+        op += iRe * (flux_yp - flux_ym) * g.idy_c[j - 1];
+        return Ax_val - op;
+    }
+}
+
+#define MOM_IJ                                        \
+    CELL_IJ                                           \
+    int ii = i;                                       \
+    bool act = in;                                    \
+    if (comp == 'u') ii = (i == 0) ? g.NXM : i;       \
+    else if (j == 0) act = false;
+
+__global__ void k_mom_matvec(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp, const double *x,
+                             const double *rho, const double *mu, double *Ax)
+{
+    MOM_IJ
+    if (act) AT(Ax, ii, j) = mom_op(g, ph, c_BET[stage], sc->dt, rho, mu, x, comp, ii, j);
+}
+extern "C" void pvk_mom_matvec(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc, int comp,
+                               const double *x, const double *rho, const double *mu, double *Ax, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_matvec, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, x, rho, mu, Ax);
+}
+
+// r = rhs - A x; dg = diag + EPS; d = r / dg; partial <r, d> (msolve_cg.c:627-645)
+__global__ void k_mom_cg_init(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp, const double *x,
+                              const double *rhs, const double *rho, const double *mu, double *r, double *d,
+                              double *Ad, double *dg, double *part, int nb)
+{
+    MOM_IJ
+    double vals[1] = {0.0};
+    if (act) {
+        const double alpha_k = c_BET[stage], dt = sc->dt;
+        double ad = mom_op(g, ph, alpha_k, dt, rho, mu, x, comp, ii, j);
+        double rr = AT(rhs, ii, j) - ad;
+        double dgv = mom_diag(g, ph, alpha_k, dt, rho, mu, comp, ii, j) + 1e-14;
+        double dd = rr / dgv;
+        AT(Ad, ii, j) = ad;
+        AT(r, ii, j) = rr;
+        AT(dg, ii, j) = dgv;
+        AT(d, ii, j) = dd;
+        vals[0] = rr * dd;
+    }
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_mom_cg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
+                                const double *x, const double *rhs, const double *rho, const double *mu, double *r,
+                                double *d, double *Ad, double *dg, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_cg_init, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, x, rhs, rho, mu,
+                       r, d, Ad, dg, rd->part, rd->nblocks);
+}
+
+// Ad = A d; partial <d, Ad> (msolve_cg.c:665-672)
+__global__ void k_mom_cg_apply(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp, const double *d,
+                               const double *rho, const double *mu, double *Ad, double *part, int nb)
+{
+    const int c = (comp == 'u') ? 0 : 1;
+    if (sc->m_done[c]) return;
+    MOM_IJ
+    double vals[1] = {0.0};
+    if (act) {
+        double ad = mom_op(g, ph, c_BET[stage], sc->dt, rho, mu, d, comp, ii, j);
+        AT(Ad, ii, j) = ad;
+        vals[0] = AT(d, ii, j) * ad;
+    }
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_mom_cg_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
+                                 const double *d, const double *rho, const double *mu, double *Ad, const pvk_red *rd,
+                                 hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_cg_apply, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, d, rho, mu, Ad,
+                       rd->part, rd->nblocks);
+}
+
+// x += alpha d; r -= alpha Ad; partials of r^2/dg and r^2 (msolve_cg.c:676-690)
+__global__ void k_mom_cg_update(pvk_grid g, const pvk_scal *sc, int comp, double *x, const double *d, double *r,
+                                const double *Ad, const double *dg, double *part, int nb)
+{
+    const int c = (comp == 'u') ? 0 : 1;
+    if (sc->m_done[c]) return;
+    MOM_IJ
+    double vals[2] = {0.0, 0.0};
+    if (act) {
+        const double alpha = sc->m_alpha[c];
+        AT(x, ii, j) += alpha * AT(d, ii, j);
+        double ri = AT(r, ii, j) - alpha * AT(Ad, ii, j);
+        AT(r, ii, j) = ri;
+        vals[0] = ri * ri / AT(dg, ii, j);
+        vals[1] = ri * ri;
+    }
+    block_partials<2>(vals, part, nb);
+}
+extern "C" void pvk_mom_cg_update(const pvk_grid *g, pvk_scal *sc, int comp, double *x, const double *d, double *r,
+                                  const double *Ad, const double *dg, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_cg_update, grid2d(g), dim3(BX, BY), 0, s, *g, sc, comp, x, d, r, Ad, dg, rd->part,
+                       rd->nblocks);
+}
+
+// d = r / dg + beta d (msolve_cg.c:705-713)
+__global__ void k_mom_cg_dupdate(pvk_grid g, const pvk_scal *sc, int comp, const double *r, const double *dg,
+                                 double *d)
+{
+    const int c = (comp == 'u') ? 0 : 1;
+    if (sc->m_done[c]) return;
+    MOM_IJ
+    if (act) AT(d, ii, j) = AT(r, ii, j) / AT(dg, ii, j) + sc->m_beta[c] * AT(d, ii, j);
+}
+extern "C" void pvk_mom_cg_dupdate(const pvk_grid *g, pvk_scal *sc, int comp, const double *r, const double *dg,
+                                   double *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mom_cg_dupdate, grid2d(g), dim3(BX, BY), 0, s, *g, sc, comp, r, dg, d);
+}
+
+// which 0: rz after init; 1: alpha; 2: stop test + beta (msolve_cg.c:645-704)
+__global__ void k_mom_scalar(pvk_scal *sc, int comp, int which, int maxit, double tol, double in_tot,
+                             const double *part, int nb)
+{
+    const int c = (comp == 'u') ? 0 : 1;
+    const double EPS = 1e-14;
+    if (which == 0) {
+        double rz = final_sum(part, nb, 0, false);
+        if (threadIdx.x == 0) { sc->m_rz[c] = rz; sc->m_iter[c] = 0; sc->m_done[c] = (maxit <= 0); }
+        return;
+    }
+    if (sc->m_done[c]) return;
+    if (which == 1) {
+        double dAd = final_sum(part, nb, 0, false);
+        if (threadIdx.x == 0) { sc->m_alpha[c] = sc->m_rz[c] / (dAd + EPS); sc->m_iter[c] += 1; }
+        return;
+    }
+    double rz_new = final_sum(part, nb, 0, false);
+    double rr = final_sum(part, nb, 1, false);
+    if (threadIdx.x == 0) {
+        double rms = sqrt(in_tot * rr);
+        sc->m_rms[c] = rms;
+        if (rms < tol) { sc->m_done[c] = 1; return; }
+        sc->m_beta[c] = rz_new / (sc->m_rz[c] + EPS);
+        sc->m_rz[c] = rz_new;
+        if (sc->m_iter[c] >= maxit) sc->m_done[c] = 3;
+    }
+}
+extern "C" void pvk_mom_scalar(const pvk_grid *g, pvk_scal *sc, int comp, int which, int maxit, double tol,
+                               const pvk_red *rd, hipStream_t s)
+{
+    double in_tot = (comp == 'u') ? 1.0 / (double)((g->NXM) * (g->NYM) * 1)
+                                  : 1.0 / (double)((g->NXM) * (g->NYM - 1) * 1);
+    hipLaunchKernelGGL(k_mom_scalar, dim3(1), dim3(256), 0, s, sc, comp, which, maxit, tol, in_tot, rd->part,
+                       rd->nblocks);
+}
+
+// ---------------------------------------------------------------------------
+// pressure
+// ---------------------------------------------------------------------------
+// Pressure_set_RHS (Pressure.c:100-196, GRID_UNIFORM) and the harmonic-mean
+// Jacobi preconditioner (psolve_cg.c:63-101), cells i < NXM, j < NYM.
+__global__ void k_p_rhs(pvk_grid g, int stage, const pvk_scal *sc, const double *u, const double *v,
+                        const double *rho, double *rhs, double *M)
+{
+    CELL_IJ
+    if (!in) return;
+    const double a_dt = 1.0 / (sc->dt * 2.0 * c_BET[stage]);
+    const double ihdt = g.idx_u[1] * a_dt;
+    double dudx = (AT(u, i + 1, j) - AT(u, i, j)) * ihdt;
+    double dvdy = (AT(v, i, j + 1) - AT(v, i, j)) * ihdt;
+    AT(rhs, i, j) = dudx + dvdy;
+    if (M) {
+        const double idx2 = g.idx_c[1] * g.idx_c[1], idy2 = g.idy_c[1] * g.idy_c[1];
+        double r0 = AT(rho, i, j);
+        double bxp = 2.0 / (r0 + AT(rho, i + 1, j)), bxm = 2.0 / (r0 + AT(rho, i - 1, j));
+        double byp = 2.0 / (r0 + AT(rho, i, j + 1)), bym = 2.0 / (r0 + AT(rho, i, j - 1));
+        double aii = bxm * idx2 + bxp * idx2 + bym * idy2 + byp * idy2 + 0.0 + 0.0;
+        AT(M, i, j) = 1.0 / (1.0 * aii + 1e-12);
+    }
+}
+extern "C" void pvk_p_rhs2(const pvk_grid *g, int stage, const pvk_scal *sc, const double *u, const double *v,
+                           const double *rho, double *rhs, double *M, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_rhs, grid2d(g), dim3(BX, BY), 0, s, *g, stage, sc, u, v, rho, rhs, M);
+}
+
+// Pressure_operator_variableCoeff (psolve_cg.c:302-344, TwodOps.c:116-142)
+__device__ __forceinline__ double p_op(const pvk_grid &g, const double *rho, const double *phi, int i, int j)
+{
+    const int P = g.P;
+    const double idx2 = g.idx_c[1] * g.idx_c[1], idy2 = g.idy_c[1] * g.idy_c[1];
+    double r0 = AT(rho, i, j), c0 = AT(phi, i, j);
+    double cxp = (2.0 / (r0 + AT(rho, i + 1, j))) * idx2;
+    double cxm = (2.0 / (r0 + AT(rho, i - 1, j))) * idx2;
+    double cyp = (2.0 / (r0 + AT(rho, i, j + 1))) * idy2;
+    double cym = (2.0 / (r0 + AT(rho, i, j - 1))) * idy2;
+    double op = cxp * (AT(phi, i + 1, j) - c0) - cxm * (c0 - AT(phi, i - 1, j)) +
+                cyp * (AT(phi, i, j + 1) - c0) - cym * (c0 - AT(phi, i, j - 1)) + 0.0 * 0.0 - 0.0 * 0.0;
+    return 1.0 * op;
+}
+
+__global__ void k_p_operator(pvk_grid g, const double *rho, const double *x, double *Ax)
+{
+    CELL_IJ
+    if (in) AT(Ax, i, j) = p_op(g, rho, x, i, j);
+}
+extern "C" void pvk_p_operator(const pvk_grid *g, const double *rho, const double *x, double *Ax, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_operator, grid2d(g), dim3(BX, BY), 0, s, *g, rho, x, Ax);
+}
+
+// PCG start with x0 = 0 (psolve_cg.c:166-194): r = rhs - A 0 = rhs, z = M r,
+// d = z, partial <r, z>.  use_M = 0 leaves z = r (the FFT path overwrites d).
+__global__ void k_p_cg_init(pvk_grid g, const double *rhs, const double *M, double *phi, double *r, double *d,
+                            int use_M, double shift_mean_slot, const pvk_scal *sc, double *part, int nb)
+{
+    CELL_IJ
+    double vals[2] = {0.0, 0.0};
+    if (in) {
+        double rr = AT(rhs, i, j);
+        if (shift_mean_slot != 0.0) rr = rr - sc->p_mean;
+        double z = use_M ? rr * AT(M, i, j) : rr;
+        AT(phi, i, j) = 0.0;
+        AT(r, i, j) = rr;
+        AT(d, i, j) = z;
+        vals[0] = rr * z;
+        vals[1] = rr * rr;
+    }
+    block_partials<2>(vals, part, nb);
+}
+extern "C" void pvk_p_cg_init2(const pvk_grid *g, const double *rhs, const double *M, double *phi, double *r,
+                               double *d, int use_M, int shift_mean, const pvk_scal *sc, const pvk_red *rd,
+                               hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_cg_init, grid2d(g), dim3(BX, BY), 0, s, *g, rhs, M, phi, r, d, use_M,
+                       shift_mean ? 1.0 : 0.0, sc, rd->part, rd->nblocks);
+}
+
+// Ad = A d; partial <d, Ad> (psolve_cg.c:203-208)
+__global__ void k_p_cg_apply(pvk_grid g, const pvk_scal *sc, const double *rho, const double *d, double *Ad,
+                             double *part, int nb)
+{
+    if (sc->p_done) return;
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) {
+        double ad = p_op(g, rho, d, i, j);
+        AT(Ad, i, j) = ad;
+        vals[0] = AT(d, i, j) * ad;
+    }
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_p_cg_apply(const pvk_grid *g, pvk_scal *sc, const double *rho, const double *d, double *Ad,
+                               const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_cg_apply, grid2d(g), dim3(BX, BY), 0, s, *g, sc, rho, d, Ad, rd->part, rd->nblocks);
+}
+
+// phi += alpha d; r -= alpha Ad; z = M r; partials <r, z>, <r, r> (:211-230)
+__global__ void k_p_cg_update(pvk_grid g, const pvk_scal *sc, double *phi, const double *d, double *r,
+                              const double *Ad, const double *M, int use_M, double *part, int nb)
+{
+    if (sc->p_done) return;
+    CELL_IJ
+    double vals[2] = {0.0, 0.0};
+    if (in) {
+        const double alpha = sc->p_alpha;
+        AT(phi, i, j) += alpha * AT(d, i, j);
+        double rr = AT(r, i, j) - alpha * AT(Ad, i, j);
+        AT(r, i, j) = rr;
+        double z = use_M ? rr * AT(M, i, j) : rr;
+        vals[0] = rr * z;
+        vals[1] = rr * rr;
+    }
+    block_partials<2>(vals, part, nb);
+}
+extern "C" void pvk_p_cg_update(const pvk_grid *g, pvk_scal *sc, double *phi, const double *d, double *r,
+                                const double *Ad, const double *M, int use_M, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_cg_update, grid2d(g), dim3(BX, BY), 0, s, *g, sc, phi, d, r, Ad, M, use_M, rd->part,
+                       rd->nblocks);
+}
+
+// d = z + beta d, z = M r (Jacobi) or z given (FFT preconditioner) (:238-246)
+__global__ void k_p_cg_dupdate(pvk_grid g, const pvk_scal *sc, const double *zr, const double *M, int use_M,
+                               double *d)
+{
+    if (sc->p_done) return;
+    CELL_IJ
+    if (in) {
+        double z = use_M ? AT(zr, i, j) * AT(M, i, j) : AT(zr, i, j);
+        AT(d, i, j) = z + sc->p_beta * AT(d, i, j);
+    }
+}
+extern "C" void pvk_p_cg_dupdate(const pvk_grid *g, pvk_scal *sc, const double *z, const double *M, int use_M,
+                                 double *d, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_p_cg_dupdate, grid2d(g), dim3(BX, BY), 0, s, *g, sc, z, M, use_M, d);
+}
+
+// Scalar logic.  mode 0 = Jacobi-PCG (sqrt(r.z) absolute stop, psolve_cg.c),
+// mode 1 = FFT-PCG (||r|| / ||b|| stop).  which: 0 start, 1 alpha, 2 stop/beta,
+// 3 gamma after a new preconditioned residual (FFT path).
+__global__ void k_p_scalar(pvk_scal *sc, int which, int mode, int maxit, double tol, const double *part, int nb)
+{
+    if (which == 0) {
+        double rz = final_sum(part, nb, 0, false);
+        double rr = final_sum(part, nb, 1, false);
+        if (threadIdx.x == 0) {
+            sc->p_gamma = rz;
+            sc->p_iter = 0;
+            sc->p_done = 0;
+            if (mode == 0) {
+                sc->p_res = sqrt(rz);
+                if (sqrt(rz) < tol) sc->p_done = 4;   // early return: no gauge (:182-185)
+            } else {
+                double bn = sqrt(rr);
+                sc->p_bnorm = (bn == 0.0) ? 1.0 : bn;
+                sc->p_res = sqrt(rr) / sc->p_bnorm;
+                if (sc->p_res <= tol) sc->p_done = 1;
+            }
+            if (maxit <= 0 && !sc->p_done) sc->p_done = 3;
+        }
+        return;
+    }
+    if (sc->p_done) return;
+    if (which == 1) {
+        double dAd = final_sum(part, nb, 0, false);
+        if (threadIdx.x == 0) {
+            sc->p_alpha = (mode == 0) ? sc->p_gamma / (dAd + 1e-30) : sc->p_gamma / dAd;
+            sc->p_iter += 1;
+        }
+        return;
+    }
+    if (which == 2) {
+        double rz = final_sum(part, nb, 0, false);
+        double rr = final_sum(part, nb, 1, false);
+        if (threadIdx.x == 0) {
+            if (mode == 0) {
+                sc->p_res = sqrt(rz);
+                if (sqrt(rz) < tol) { sc->p_done = 1; return; }
+                sc->p_beta = rz / (sc->p_gamma + 1e-30);
+                sc->p_gamma = rz;
+            } else {
+                sc->p_res = sqrt(rr) / sc->p_bnorm;
+                if (sc->p_res <= tol) { sc->p_done = 1; return; }
+            }
+            if (sc->p_iter >= maxit) sc->p_done = 3;
+        }
+        return;
+    }
+    // which == 3: gamma_new = <r, z> with z from the FFT preconditioner;
+    // which == 4: first gamma = <r, z>, beta = 0 (so d = z)
+    double rz = final_sum(part, nb, 0, false);
+    if (threadIdx.x == 0) {
+        sc->p_beta = (which == 4) ? 0.0 : rz / sc->p_gamma;
+        sc->p_gamma = rz;
+    }
+}
+extern "C" void pvk_p_scalar2(const pvk_grid *g, pvk_scal *sc, int which, int mode, int maxit, double tol,
+                              const pvk_red *rd, int nb, hipStream_t s)
+{
+    (void)g;
+    hipLaunchKernelGGL(k_p_scalar, dim3(1), dim3(256), 0, s, sc, which, mode, maxit, tol, rd->part,
+                       nb > 0 ? nb : rd->nblocks);
+}
+
+// partial <a, b> over cells (i < NXM, j < NYM)
+__global__ void k_dot(pvk_grid g, const double *a, const double *b, double *part, int nb)
+{
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) vals[0] = AT(a, i, j) * AT(b, i, j);
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_dot(const pvk_grid *g, const double *a, const double *b, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot, grid2d(g), dim3(BX, BY), 0, s, *g, a, b, rd->part, rd->nblocks);
+}
+
+// partial sums of x for the mean (psolve_cg.c:255-279)
+__global__ void k_sum(pvk_grid g, const double *x, double *part, int nb)
+{
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) vals[0] = AT(x, i, j);
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_p_mean(const pvk_grid *g, const double *x, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sum, grid2d(g), dim3(BX, BY), 0, s, *g, x, rd->part, rd->nblocks);
+}
+__global__ void k_mean_finalize(pvk_scal *sc, double cnt, const double *part, int nb)
+{
+    double s = final_sum(part, nb, 0, false);
+    if (threadIdx.x == 0) sc->p_mean = s / cnt;
+}
+extern "C" void pvk_p_mean_finalize(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s)
+{
+    double cnt = (double)((long)g->NXM * (long)g->NYM);
+    hipLaunchKernelGGL(k_mean_finalize, dim3(1), dim3(256), 0, s, sc, cnt, rd->part, rd->nblocks);
+}
+__global__ void k_sub_mean(pvk_grid g, const pvk_scal *sc, double *x)
+{
+    CELL_IJ
+    if (in) AT(x, i, j) -= sc->p_mean;
+}
+extern "C" void pvk_p_sub_mean(const pvk_grid *g, const pvk_scal *sc, double *x, int ghosts, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_sub_mean, grid2d(g), dim3(BX, BY), 0, s, *g, sc, x);
+    if (ghosts) launch_ghost(g, x, 0, 0, 0, 1, 2, s);
+}
+
+// Pressure_project_velocity (Pressure.c:218-464): p += phi on cells, then
+// u -= dphi/dx * 2 beta dt / rho_face, v likewise; phi ghosts must be valid.
+__global__ void k_project(pvk_grid g, int stage, const pvk_scal *sc, const double *rho, const double *phi,
+                          double *p, double *u, double *v)
+{
+    CELL_IJ
+    if (!in) return;
+    const double mdt = 2.0 * c_BET[stage] * sc->dt;
+    AT(p, i, j) += AT(phi, i, j);
+    {
+        int iu = (i == 0) ? g.NXM : i;
+        double inv_rho_face = 2.0 / (AT(rho, iu, j) + AT(rho, iu - 1, j));
+        double idxdt = g.idx_c[iu - 1] * mdt;
+        AT(u, iu, j) -= (AT(phi, iu, j) - AT(phi, iu - 1, j)) * (idxdt * inv_rho_face);
+    }
+    if (j >= 1) {
+        double inv_rho_face = 2.0 / (AT(rho, i, j) + AT(rho, i, j - 1));
+        double idydt = g.idy_c[j - 1] * mdt;
+        AT(v, i, j) -= (AT(phi, i, j) - AT(phi, i, j - 1)) * (idydt * inv_rho_face);
+    }
+}
+extern "C" void pvk_project(const pvk_grid *g, int stage, const pvk_scal *sc, const double *rho, const double *phi,
+                            double *p, double *u, double *v, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_project, grid2d(g), dim3(BX, BY), 0, s, *g, stage, sc, rho, phi, p, u, v);
+}
+
+// max |div u| (Pressure.c:475-597)
+__global__ void k_divergence(pvk_grid g, const double *u, const double *v, double *part, int nb)
+{
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) {
+        double dudx = (AT(u, i + 1, j) - AT(u, i, j)) * g.idx_u[i];
+        double dvdy = (AT(v, i, j + 1) - AT(v, i, j)) * g.idy_v[j];
+        vals[0] = fabs(dudx + dvdy);
+    }
+    block_max_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_divergence(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_divergence, grid2d(g), dim3(BX, BY), 0, s, *g, u, v, rd->part, rd->nblocks);
+}
+__global__ void k_div_finalize(pvk_scal *sc, const double *part, int nb)
+{
+    double m = final_sum(part, nb, 0, true);
+    if (threadIdx.x == 0) sc->div_max = m;
+}
+extern "C" void pvk_div_finalize(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s)
+{
+    (void)g;
+    hipLaunchKernelGGL(k_div_finalize, dim3(1), dim3(256), 0, s, sc, rd->part, rd->nblocks);
+}
+
+// ---------------------------------------------------------------------------
+// time step (Dtime.c:24-249, FULLY_IMPLICIT, SURFACE_TENSION)
+// ---------------------------------------------------------------------------
+__global__ void k_cfl(pvk_grid g, const double *u, const double *v, double *part, int nb)
+{
+    const int i = blockIdx.x * BX + threadIdx.x;
+    const int j = blockIdx.y * BY + threadIdx.y;
+    const int P = g.P;
+    double vals[1] = {0.0};
+    if (i <= g.NXM && j <= g.NYM) {
+        double ub = 0.5 * (AT(u, i, j) + AT(u, i + 1, j));
+        double vb = 0.5 * (AT(v, i, j) + AT(v, i, j + 1));
+        double u_cfl = fabs(ub), v_cell = fabs(vb), v_part = fabs(vb - 0.0);
+        double v_cfl = (v_cell < v_part) ? v_part : v_cell;
+        vals[0] = u_cfl * g.idx_u[i] + v_cfl * g.idy_v[j] + 0.0;
+    }
+    block_max_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_cfl(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s)
+{
+    dim3 gr((g->NXM + 1 + BX - 1) / BX, (g->NYM + 1 + BY - 1) / BY);
+    hipLaunchKernelGGL(k_cfl, gr, dim3(BX, BY), 0, s, *g, u, v, rd->part, (int)(gr.x * gr.y));
+}
+__global__ void k_cfl_finalize(pvk_scal *sc, pvk_phys ph, double dx_min, const double *part, int nb)
+{
+    double max_idt = final_sum(part, nb, 0, true);
+    if (threadIdx.x == 0) {
+        sc->max_idt = max_idt;
+        double dt_cfl = ph.cfl / max_idt;
+        double dt_sigma = ph.cfl * sqrt(((ph.rho1 + ph.rho2) * pow(dx_min, 3) * ph.We) / (4.0 * 3.14159265358979323846));
+        dt_cfl = (dt_cfl > dt_sigma) ? dt_sigma : dt_cfl;
+        double dt;
+        if (ph.constant_dt == 1) dt = ph.default_dt;
+        else {
+            double dt_old = sc->dt;          // caller's dt becomes dt_old
+            dt = dt_cfl;
+            if (dt > 1.05 * dt_old) dt = 1.05 * dt_old;
+            if (dt < 0.95 * dt_old) dt = 0.95 * dt_old;
+            else dt = (dt > ph.max_dt) ? ph.max_dt : dt;
+        }
+        sc->dt_new = dt;
+    }
+}
+extern "C" void pvk_cfl_finalize2(const pvk_grid *g, const pvk_phys *ph, pvk_scal *sc, double dx_min,
+                                  const pvk_red *rd, hipStream_t s)
+{
+    int nb = ((g->NXM + 1 + BX - 1) / BX) * ((g->NYM + 1 + BY - 1) / BY);
+    hipLaunchKernelGGL(k_cfl_finalize, dim3(1), dim3(256), 0, s, sc, *ph, dx_min, rd->part, nb);
+}
+// dt_old = dt; dt = dt_new (Temporal_int.c:427-429)
+__global__ void k_take_dt(pvk_scal *sc)
+{
+    sc->dt_old = sc->dt;
+    sc->dt = sc->dt_new;
+}
+extern "C" void pvk_take_dt(pvk_scal *sc, hipStream_t s) { hipLaunchKernelGGL(k_take_dt, dim3(1), dim3(1), 0, s, sc); }
+// time += dt * BETA2[rk] (Temporal_int.c:445)
+__global__ void k_advance_time(pvk_scal *sc, int stage) { sc->time += sc->dt * c_BET2[stage]; }
+extern "C" void pvk_advance_time(pvk_scal *sc, int stage, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_advance_time, dim3(1), dim3(1), 0, s, sc, stage);
+}
+
+// ---------------------------------------------------------------------------
+// small helpers
+// ---------------------------------------------------------------------------
+__global__ void k_psi_only(pvk_grid g, pvk_phys ph, const double *C, double *psi)
+{
+    CELL_IJ
+    if (in) AT(psi, i, j) = psi_at(g, ph, C, i, j);
+}
+extern "C" void pvk_psi_only(const pvk_grid *g, const pvk_phys *ph, const double *C, double *psi, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_psi_only, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, psi);
+}
+
+// C_L = clamp01(F) (VOF_DIFFUSE.c:531-537); ghosts follow from a CH ghost fill.
+__global__ void k_clamp_init(pvk_grid g, const double *F, double *C)
+{
+    CELL_IJ
+    if (in) {
+        double f = AT(F, i, j);
+        AT(C, i, j) = (f < 0.0) ? 0.0 : ((f > 1.0) ? 1.0 : f);
+    }
+}
+extern "C" void pvk_clamp_init(const pvk_grid *g, const double *F, double *C, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_clamp_init, grid2d(g), dim3(BX, BY), 0, s, *g, F, C);
+}
+
+// whole-slab copy including ghosts (Array_copy_withghost)
+__global__ void k_copy_slab(pvk_grid g, const double *src, double *dst)
+{
+    const int P = g.P;
+    int i = blockIdx.x * BX + threadIdx.x - 3, j = blockIdx.y * BY + threadIdx.y - 3;
+    if (i <= g.NXM + 3 && j <= g.NYM + 3) AT(dst, i, j) = AT(src, i, j);
+}
+extern "C" void pvk_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s)
+{
+    dim3 gr((g->NXM + 7 + BX - 1) / BX, (g->NYM + 7 + BY - 1) / BY);
+    hipLaunchKernelGGL(k_copy_slab, gr, dim3(BX, BY), 0, s, *g, src, dst);
+}
+
+// partial <a, b>, skipped once the pressure solve has stopped
+__global__ void k_dot_sc(pvk_grid g, const pvk_scal *sc, const double *a, const double *b, double *part, int nb)
+{
+    if (sc->p_done) return;
+    CELL_IJ
+    double vals[1] = {0.0};
+    if (in) vals[0] = AT(a, i, j) * AT(b, i, j);
+    block_partials<1>(vals, part, nb);
+}
+extern "C" void pvk_dot_sc(const pvk_grid *g, const pvk_scal *sc, const double *a, const double *b,
+                           const pvk_red *rd, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot_sc, grid2d(g), dim3(BX, BY), 0, s, *g, sc, a, b, rd->part, rd->nblocks);
+}
+
+// Reference pin of the (0,0) mode at j = 0 (psolve_fft.c:1716-1722): for a
+// zero-mean right-hand side the pinned solution is the mean-free one shifted
+// so that the x-mean of row 0 vanishes.
+__global__ void k_pin_row0(pvk_grid g, double *x)
+{
+    __shared__ double sh[256];
+    const int P = g.P;
+    double s = 0.0;
+    for (int i = threadIdx.x; i < g.NXM; i += 256) s += AT(x, i, 0);
+    sh[threadIdx.x] = s;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
+        __syncthreads();
+    }
+    double mean = sh[0] / (double)g.NXM;
+    __syncthreads();
+    for (int j = 0; j < g.NYM; j++)
+        for (int i = threadIdx.x; i < g.NXM; i += 256) AT(x, i, j) -= mean;
+}
+extern "C" void pvk_pin_row0(const pvk_grid *g, double *x, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pin_row0, dim3(1), dim3(256), 0, s, *g, x);
+}

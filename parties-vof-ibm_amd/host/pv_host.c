@@ -1,0 +1,438 @@
+/*
+ * pv_host.c -- C host layer of the MI355X PARTIES substep.
+ *
+ * Owns one pv_bag (one 2-D subdomain on one GPU): device allocation, field
+ * I/O, and the sequencing of Temporal_int_all_the_equations
+ * (PARTIES/src/Temporal_int.c:563-1112) on a single HIP stream.  All
+ * arithmetic happens in csrc/pv_kernels.hip; solver scalars (alpha, beta,
+ * stop flags, dt) live in device memory so the host only enqueues work and
+ * reads a flag every few iterations.
+ */
+#include "pv_host.h"
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+static __thread char g_err[512];
+
+const char *pv_last_error(void) { return g_err; }
+
+int pv_set_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof g_err, fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return pv_set_error(PV_EHIP, "%s failed: %s (%s:%d)", #call,             \
+                                hipGetErrorString(e_), __FILE__, __LINE__);          \
+    } while (0)
+
+/* IO/default.inp defaults for the 2-D no-IBM path */
+void pv_params_default(pv_params *p)
+{
+    memset(p, 0, sizeof *p);
+    p->NXM = 64; p->NYM = 64;
+    p->xmin = 0.0; p->xmax = 1.0; p->ymin = 0.0; p->ymax = 1.0;
+    p->twod_slab_thickness = -1.0;
+    p->Re = 100.0; p->dp_dx = 0.0;
+    p->We = 1.0; p->rho1 = 1.0; p->rho2 = 1.0; p->mu1 = 1.0; p->mu2 = 1.0;
+    p->Cn = -1.0; p->Pe_CH = -1.0;
+    p->weno_order = 5; p->ch_iter_max = 300; p->ch_tol = 1e-9;
+    p->cfl = 0.5; p->default_dt = 1e-3; p->max_dt = 0.05; p->constant_dt = 0;
+    p->grav[0] = 0.0; p->grav[1] = -1.0; p->grav[2] = 0.0;
+    p->richardson = 1.0;
+    p->P_CG_MAXIT = 300; p->P_CG_ETOL = 1e-7;
+    p->CG_MAXIT = 120; p->CG_ETOL = 1e-9;
+    p->pressure_solver = PV_PSOLVE_JACOBI_PCG;
+    p->p_rel_tol = 1e-10; p->p_max_iter = 2000;
+    p->precision = PV_PREC_FP64;
+}
+
+/* IO/Input.c:20-127: slab thickness, Cn and Pe_CH defaults */
+void pv_derive_params(pv_params *p, double *slab)
+{
+    double Lx = p->xmax - p->xmin, Ly = p->ymax - p->ymin;
+    double hx = Lx / (double)p->NXM, hy = Ly / (double)p->NYM;
+    double s = p->twod_slab_thickness, href = hx, hz, h_ref;
+    if (hy < href) href = hy;
+    if (s <= 0.0) s = href;
+    if (s <= 0.0) s = 1.0;
+    p->twod_slab_thickness = s;
+    *slab = s;
+    hz = s / 1.0;
+    h_ref = hx;
+    if (hy < h_ref) h_ref = hy;
+    if (hz < h_ref) h_ref = hz;
+    if (p->Cn <= 0.0) p->Cn = 0.75 * h_ref;
+    if (p->Pe_CH <= 0.0 && p->Cn > 0.0) p->Pe_CH = 0.9 / p->Cn;
+}
+
+/* Grid.c:845-861 coordinates and Grid.c:385-420 metrics; arrays hold
+ * entries for indices -3 .. N+3 at offset 3. */
+void pv_host_metrics(const pv_params *p, pv_metrics *m)
+{
+    int NX = p->NXM + 1, NY = p->NYM + 1, i, j;
+    double dx = (p->xmax - p->xmin) / (double)(NX - 1);
+    double dy = (p->ymax - p->ymin) / (double)(NY - 1);
+    memset(m, 0, sizeof *m);
+    m->nx = p->NXM + 7; m->ny = p->NYM + 7;
+    m->xu = calloc(m->nx, sizeof(double)); m->xc = calloc(m->nx, sizeof(double));
+    m->yv = calloc(m->ny, sizeof(double)); m->yc = calloc(m->ny, sizeof(double));
+    m->dx_u = calloc(m->nx, sizeof(double)); m->dx_c = calloc(m->nx, sizeof(double));
+    m->idx_u = calloc(m->nx, sizeof(double)); m->idx_c = calloc(m->nx, sizeof(double));
+    m->dy_v = calloc(m->ny, sizeof(double)); m->dy_c = calloc(m->ny, sizeof(double));
+    m->idy_v = calloc(m->ny, sizeof(double)); m->idy_c = calloc(m->ny, sizeof(double));
+    for (i = -3; i < NX + 3; i++) {
+        m->xu[i + 3] = p->xmin + i * dx;
+        m->xc[i + 3] = p->xmin + (0.5 + i) * dx;
+    }
+    for (j = -3; j < NY + 3; j++) {
+        m->yv[j + 3] = p->ymin + j * dy;
+        m->yc[j + 3] = p->ymin + (0.5 + j) * dy;
+    }
+    for (i = -3; i < NX + 2; i++) {
+        m->dx_u[i + 3] = m->xu[i + 4] - m->xu[i + 3];
+        m->dx_c[i + 3] = m->xc[i + 4] - m->xc[i + 3];
+        m->idx_u[i + 3] = 1.0 / m->dx_u[i + 3];
+        m->idx_c[i + 3] = 1.0 / m->dx_c[i + 3];
+    }
+    for (j = -3; j < NY + 2; j++) {
+        m->dy_v[j + 3] = m->yv[j + 4] - m->yv[j + 3];
+        m->dy_c[j + 3] = m->yc[j + 4] - m->yc[j + 3];
+        m->idy_v[j + 3] = 1.0 / m->dy_v[j + 3];
+        m->idy_c[j + 3] = 1.0 / m->dy_c[j + 3];
+    }
+}
+
+void pv_host_metrics_free(pv_metrics *m)
+{
+    free(m->xu); free(m->xc); free(m->yv); free(m->yc);
+    free(m->dx_u); free(m->dx_c); free(m->idx_u); free(m->idx_c);
+    free(m->dy_v); free(m->dy_c); free(m->idy_v); free(m->idy_c);
+    memset(m, 0, sizeof *m);
+}
+
+/* ------------------------------------------------------------------ */
+/* handle                                                              */
+/* ------------------------------------------------------------------ */
+
+#define OFF0 13                     /* (OFF0 + 3) % 16 == 0: column 0 on 128 B */
+
+static double *fld(pv_bag *b, int f) { return b->f[f]; }
+
+static void fill_phys(pv_bag *b)
+{
+    pv_params *p = &b->prm;
+    pvk_phys *ph = &b->ph;
+    double gm;
+    memset(ph, 0, sizeof *ph);
+    ph->Re = p->Re; ph->iRe = 1.0 / p->Re; ph->Pe_CH = p->Pe_CH; ph->Cn = p->Cn; ph->We = p->We;
+    ph->rho_g = p->rho2 / p->rho1; ph->mu_g = p->mu2 / p->mu1;
+    ph->bih = p->Cn * p->Cn / p->Pe_CH;
+    gm = sqrt(p->grav[0] * p->grav[0] + p->grav[1] * p->grav[1]);
+    ph->gx = 0.0; ph->gy = 0.0;
+    if (gm > 0.0) { ph->gx = p->grav[0] / gm; ph->gy = p->grav[1] / gm; }
+    ph->Ri = p->richardson; ph->dp_dx = p->dp_dx;
+    ph->rho1 = p->rho1; ph->rho2 = p->rho2; ph->cfl = p->cfl; ph->max_dt = p->max_dt;
+    ph->default_dt = p->default_dt;
+    ph->ch_tol = (p->ch_tol > 0.0) ? p->ch_tol : p->CG_ETOL;
+    ph->CG_ETOL = p->CG_ETOL; ph->P_CG_ETOL = p->P_CG_ETOL; ph->p_rel_tol = p->p_rel_tol;
+    ph->weno_order = p->weno_order; ph->constant_dt = p->constant_dt;
+    ph->CG_MAXIT = p->CG_MAXIT; ph->P_CG_MAXIT = p->P_CG_MAXIT;
+    ph->ch_iter_max = (p->ch_iter_max > 0) ? p->ch_iter_max : p->CG_MAXIT;
+    ph->p_max_iter = (p->p_max_iter > 0) ? p->p_max_iter : 1000;
+    ph->precision = p->precision;
+}
+
+int pv_create(const pv_params *prm, int device, pv_bag **out)
+{
+    pv_bag *b;
+    int ndev = 0, k;
+    size_t bytes;
+    pv_metrics m;
+    hipDeviceProp_t prop;
+    *out = NULL;
+    if (!prm || prm->NXM < 8 || prm->NYM < 8)
+        return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
+        return pv_set_error(PV_ENODEV, "no HIP device %d (count %d)", device, ndev);
+    HIPCHK(hipSetDevice(device));
+    HIPCHK(hipGetDeviceProperties(&prop, device));
+    if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
+        return pv_set_error(PV_ENODEV, "device %d is %s; this library is built for gfx950", device,
+                            prop.gcnArchName);
+    b = (pv_bag *)calloc(1, sizeof *b);
+    if (!b) return pv_set_error(PV_ENOMEM, "host allocation failed");
+    b->prm = *prm;
+    pv_derive_params(&b->prm, &b->slab);
+    b->device = device;
+    b->NXM = prm->NXM; b->NYM = prm->NYM;
+    b->NXT = prm->NXM + 7; b->NYT = prm->NYM + 7;
+    b->P = ((b->NXT + 15) / 16) * 16;
+    b->alloc_elems = (size_t)OFF0 + (size_t)b->P * (size_t)b->NYT + 16;
+    bytes = b->alloc_elems * sizeof(double) * PV_FI_END;
+    if (hipMalloc((void **)&b->slabmem, bytes) != hipSuccess) {
+        free(b);
+        return pv_set_error(PV_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
+    }
+    HIPCHK(hipMemset(b->slabmem, 0, bytes));
+    for (k = 0; k < PV_FI_END; k++)
+        b->f[k] = b->slabmem + (size_t)k * b->alloc_elems + OFF0 + 3 * (size_t)b->P + 3;
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+
+    /* metrics */
+    pv_host_metrics(&b->prm, &m);
+    {
+        size_t nmx = m.nx, nmy = m.ny;
+        double *dm;
+        HIPCHK(hipMalloc((void **)&dm, sizeof(double) * (4 * nmx + 4 * nmy)));
+        b->metricmem = dm;
+        HIPCHK(hipMemcpy(dm + 0 * nmx, m.idx_u, nmx * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 1 * nmx, m.idx_c, nmx * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 2 * nmx, m.dx_u, nmx * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 3 * nmx, m.dx_c, nmx * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 4 * nmx + 0 * nmy, m.idy_v, nmy * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 4 * nmx + 1 * nmy, m.idy_c, nmy * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 4 * nmx + 2 * nmy, m.dy_v, nmy * 8, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(dm + 4 * nmx + 3 * nmy, m.dy_c, nmy * 8, hipMemcpyHostToDevice));
+        b->g.idx_u = dm + 0 * nmx + 3;
+        b->g.idx_c = dm + 1 * nmx + 3;
+        b->g.dx_u = dm + 2 * nmx + 3;
+        b->g.dx_c = dm + 3 * nmx + 3;
+        b->g.idy_v = dm + 4 * nmx + 0 * nmy + 3;
+        b->g.idy_c = dm + 4 * nmx + 1 * nmy + 3;
+        b->g.dy_v = dm + 4 * nmx + 2 * nmy + 3;
+        b->g.dy_c = dm + 4 * nmx + 3 * nmy + 3;
+        b->dx_min = (1.0 / m.idx_u[3] > 1.0 / m.idy_v[3]) ? 1.0 / m.idy_v[3] : 1.0 / m.idx_u[3];
+        b->h_idx_u0 = m.idx_u[3];
+        b->h_idy_v0 = m.idy_v[3];
+        memcpy(&b->hm, &m, sizeof m);
+    }
+    b->g.NXM = b->NXM; b->g.NYM = b->NYM; b->g.P = b->P;
+    b->g.nbx = (b->NXM + 63) / 64; b->g.nby = (b->NYM + 3) / 4;
+    b->g.slab = b->slab;
+    fill_phys(b);
+
+    /* reductions: up to 4 partial sets per launch, plus the CFL grid */
+    b->rd.nblocks = b->g.nbx * b->g.nby;
+    {
+        int nb_cfl = ((b->NXM + 1 + 63) / 64) * ((b->NYM + 1 + 3) / 4);
+        int nmax = (nb_cfl > b->rd.nblocks) ? nb_cfl : b->rd.nblocks;
+        HIPCHK(hipMalloc((void **)&b->rd.part, sizeof(double) * 4 * (size_t)nmax));
+    }
+    HIPCHK(hipMalloc((void **)&b->d_sc, sizeof(pvk_scal)));
+    HIPCHK(hipMemset(b->d_sc, 0, sizeof(pvk_scal)));
+    HIPCHK(hipHostMalloc((void **)&b->h_sc, sizeof(pvk_scal), hipHostMallocDefault));
+    memset(b->h_sc, 0, sizeof(pvk_scal));
+    b->time = 0.0; b->dt = b->dt_old = b->prm.default_dt; b->stage = 0; b->ntime = 1; b->time_nonzero = 0;
+    b->ch_chunk = 8; b->mom_chunk[0] = b->mom_chunk[1] = 4; b->p_chunk = 16;
+    {
+        int rc = pv_fft_create(b);
+        if (rc) return rc;
+    }
+    {
+        int rc = pv_sync_scalars_to_device(b);
+        if (rc) return rc;
+    }
+    *out = b;
+    return PV_OK;
+}
+
+int pv_destroy(pv_bag *b)
+{
+    if (!b) return PV_OK;
+    hipSetDevice(b->device);
+    hipStreamSynchronize(b->stream);
+    pv_fft_destroy(b);
+    hipFree(b->slabmem);
+    hipFree(b->metricmem);
+    hipFree(b->rd.part);
+    hipFree(b->d_sc);
+    hipHostFree(b->h_sc);
+    hipStreamDestroy(b->stream);
+    pv_host_metrics_free(&b->hm);
+    free(b);
+    return PV_OK;
+}
+
+int pv_get_params(const pv_bag *b, pv_params *out)
+{
+    if (!b || !out) return pv_set_error(PV_EINVAL, "null argument");
+    *out = b->prm;
+    return PV_OK;
+}
+
+static int check_field(pv_bag *b, int f)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (f < 0 || f >= PV_NFIELDS) return pv_set_error(PV_EINVAL, "bad field id %d", f);
+    return PV_OK;
+}
+
+int pv_set_field(pv_bag *b, int f, const double *host)
+{
+    int rc = check_field(b, f);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipMemcpy2DAsync(b->f[f] - 3 * (size_t)b->P - 3, (size_t)b->P * 8, host, (size_t)b->NXT * 8,
+                            (size_t)b->NXT * 8, b->NYT, hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return PV_OK;
+}
+
+int pv_get_field(pv_bag *b, int f, double *host)
+{
+    int rc = check_field(b, f);
+    if (rc) return rc;
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipMemcpy2DAsync(host, (size_t)b->NXT * 8, b->f[f] - 3 * (size_t)b->P - 3, (size_t)b->P * 8,
+                            (size_t)b->NXT * 8, b->NYT, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return PV_OK;
+}
+
+int pv_fill_field(pv_bag *b, int f, double value)
+{
+    int rc = check_field(b, f);
+    if (rc) return rc;
+    pvk_fill(b->f[f] - 3 * (size_t)b->P - 3, (size_t)b->P * b->NYT, value, b->stream);
+    HIPCHK(hipGetLastError());
+    return PV_OK;
+}
+
+/* host mirror of time state <-> device scalars */
+int pv_sync_scalars_to_device(pv_bag *b)
+{
+    b->h_sc->dt = b->dt; b->h_sc->dt_old = b->dt_old; b->h_sc->time = b->time;
+    HIPCHK(hipMemcpyAsync(b->d_sc, b->h_sc, sizeof(pvk_scal), hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return PV_OK;
+}
+
+int pv_read_scalars(pv_bag *b)
+{
+    HIPCHK(hipMemcpyAsync(b->h_sc, b->d_sc, sizeof(pvk_scal), hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    b->dt = b->h_sc->dt; b->dt_old = b->h_sc->dt_old; b->time = b->h_sc->time;
+    return PV_OK;
+}
+
+int pv_set_time_state(pv_bag *b, double time, double dt, double dt_old, int which_stage)
+{
+    if (!b || which_stage < 0 || which_stage > 2) return pv_set_error(PV_EINVAL, "bad time state");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    HIPCHK(hipMemcpy(b->h_sc, b->d_sc, sizeof(pvk_scal), hipMemcpyDeviceToHost));
+    b->time = time; b->dt = dt; b->dt_old = dt_old; b->stage = which_stage;
+    b->time_nonzero = (time != 0.0);
+    return pv_sync_scalars_to_device(b);
+}
+
+int pv_get_stats(pv_bag *b, pv_stats *out)
+{
+    int rc;
+    if (!b || !out) return pv_set_error(PV_EINVAL, "null argument");
+    HIPCHK(hipSetDevice(b->device));
+    rc = pv_read_scalars(b);
+    if (rc) return rc;
+    *out = b->st;
+    out->time = b->time; out->dt = b->dt; out->dt_old = b->dt_old;
+    out->ntime = b->ntime; out->which_stage = b->stage;
+    return PV_OK;
+}
+
+/* device handles for other host files */
+double *pv_dev_field(pv_bag *b, int f) { return fld(b, f); }
+
+/* ------------------------------------------------------------------ */
+/* FFT-x / DCT-y tables (csrc/pv_fft.hip)                              */
+/* ------------------------------------------------------------------ */
+static int ilog2_exact(int n)
+{
+    int l = 0;
+    if (n <= 0 || (n & (n - 1))) return -1;
+    while ((1 << l) < n) l++;
+    return l;
+}
+
+int pv_fft_create(pv_bag *b)
+{
+    const double PI_ = 3.14159265358979323846;
+    int nx = b->NXM, ny = b->NYM, lx = ilog2_exact(nx), ly = ilog2_exact(ny), k;
+    size_t n_twx, n_twy, n_rtx, n_rty, n_dct, total;
+    double *h, *d, *q;
+    double idx2, idy2;
+    memset(&b->fft, 0, sizeof b->fft);
+    b->fftmem = NULL;
+    if (lx < 3 || ly < 3) return PV_OK;          /* FFT-PCG unavailable for this size */
+    b->fft.logmx = lx - 1;
+    b->fft.logmy = ly - 1;
+    b->fft.rx = 8192 / nx; if (b->fft.rx < 1) b->fft.rx = 1; if (b->fft.rx > 16) b->fft.rx = 16;
+    b->fft.ry = 8192 / ny; if (b->fft.ry < 1) b->fft.ry = 1; if (b->fft.ry > 16) b->fft.ry = 16;
+    while (b->fft.ry > 1 && b->fft.ry * (ny / 4 + 1) > 8 * 512) b->fft.ry /= 2;
+    if (b->fft.ry * (ny / 4 + 1) > 8 * 512) return PV_OK;
+    n_twx = 2 * (size_t)(nx / 4 > 0 ? nx / 4 : 1); n_twy = 2 * (size_t)(ny / 4 > 0 ? ny / 4 : 1);
+    n_rtx = 2 * (size_t)(nx / 2 + 1); n_rty = 2 * (size_t)(ny / 2 + 1);
+    n_dct = 2 * (size_t)ny;
+    total = n_twx + n_twy + n_rtx + n_rty + n_dct + nx + ny;
+    h = (double *)calloc(total, sizeof(double));
+    if (!h) return pv_set_error(PV_ENOMEM, "FFT table allocation failed");
+    q = h;
+    for (k = 0; k < (int)(n_twx / 2); k++) { double a = 2.0 * PI_ * k / (nx / 2); q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
+    q += n_twx;
+    for (k = 0; k < (int)(n_twy / 2); k++) { double a = 2.0 * PI_ * k / (ny / 2); q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
+    q += n_twy;
+    for (k = 0; k <= nx / 2; k++) { double a = 2.0 * PI_ * k / nx; q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
+    q += n_rtx;
+    for (k = 0; k <= ny / 2; k++) { double a = 2.0 * PI_ * k / ny; q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
+    q += n_rty;
+    for (k = 0; k < ny; k++) { double a = PI_ * k / (2.0 * ny); q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
+    q += n_dct;
+    /* psolve_fft.c:577-583 modified wavenumbers with idx = idx_u[0]; the y
+     * coefficient idy_v * idy_c of the uniform Neumann tridiagonal. */
+    idx2 = b->hm.idx_u[3] * b->hm.idx_u[3];
+    idy2 = b->hm.idy_v[3] * b->hm.idy_c[3];
+    for (k = 0; k < nx; k++) {
+        int wave = (k < nx / 2) ? k : (nx - k);
+        q[k] = 2.0 * (1 - cos(2. * PI_ * wave / nx)) * idx2;
+    }
+    q += nx;
+    for (k = 0; k < ny; k++) q[k] = 2.0 * (1 - cos(PI_ * k / ny)) * idy2;
+    if (hipMalloc((void **)&d, sizeof(double) * (total + (size_t)nx * ny)) != hipSuccess) {
+        free(h);
+        return pv_set_error(PV_ENOMEM, "FFT table device allocation failed");
+    }
+    if (hipMemcpy(d, h, sizeof(double) * total, hipMemcpyHostToDevice) != hipSuccess) {
+        free(h);
+        return pv_set_error(PV_EHIP, "FFT table upload failed");
+    }
+    free(h);
+    b->fftmem = d;
+    q = d;
+    b->fft.twx = q; q += n_twx;
+    b->fft.twy = q; q += n_twy;
+    b->fft.rtx = q; q += n_rtx;
+    b->fft.rty = q; q += n_rty;
+    b->fft.dcty = q; q += n_dct;
+    b->fft.lamx = q; q += nx;
+    b->fft.lamy = q; q += ny;
+    b->fft.spec = q;
+    b->fft.ok = 1;
+    return PV_OK;
+}
+
+void pv_fft_destroy(pv_bag *b)
+{
+    if (b->fftmem) hipFree(b->fftmem);
+    b->fftmem = NULL;
+    b->fft.ok = 0;
+}

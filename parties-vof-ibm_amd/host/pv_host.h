@@ -1,0 +1,63 @@
+/*
+ * pv_host.h -- private state of one pv_bag (host layer only).
+ */
+#ifndef PV_HOST_H
+#define PV_HOST_H
+#include "../../include/parties_vof.h"
+#include "../csrc/pv_kernels.h"
+
+/* device-only work arrays after the public fields */
+enum pv_internal_field {
+    PV_FI_DG_U = PV_NFIELDS,   /* Jacobi diagonal + EPS, u (msolve_cg.c:103-249) */
+    PV_FI_DG_V,                /* same, v */
+    PV_FI_TMP,                 /* scratch */
+    PV_FI_Z,                   /* preconditioned residual of the pressure PCG */
+    PV_FI_END
+};
+#define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
+
+typedef struct pv_metrics {
+    int nx, ny;
+    double *xu, *xc, *yv, *yc;
+    double *dx_u, *dx_c, *idx_u, *idx_c;
+    double *dy_v, *dy_c, *idy_v, *idy_c;
+} pv_metrics;
+
+struct pv_bag {
+    pv_params prm;
+    double slab;
+    int device;
+    int NXM, NYM, NXT, NYT, P;
+    size_t alloc_elems;
+    double *slabmem;               /* all fields, one allocation */
+    double *f[PV_FI_END];          /* cell (0,0) of each field */
+    double *metricmem;
+    pv_metrics hm;                 /* host copy of the metrics */
+    double dx_min, h_idx_u0, h_idy_v0;
+    pvk_grid g;
+    pvk_phys ph;
+    pvk_red rd;
+    pvk_scal *d_sc;                /* device scalars */
+    pvk_scal *h_sc;                /* pinned host mirror */
+    hipStream_t stream;
+    /* host view of the time state */
+    double time, dt, dt_old;
+    int stage, ntime, time_nonzero;
+    pv_stats st;
+    int ch_chunk, mom_chunk[2], p_chunk;
+    /* FFT-x / DCT-y preconditioner */
+    pvk_fft fft;
+    double *fftmem;
+};
+
+int  pv_set_error(int code, const char *fmt, ...);
+void pv_derive_params(pv_params *p, double *slab);
+void pv_host_metrics(const pv_params *p, pv_metrics *m);
+void pv_host_metrics_free(pv_metrics *m);
+int  pv_sync_scalars_to_device(pv_bag *b);
+int  pv_read_scalars(pv_bag *b);
+double *pv_dev_field(pv_bag *b, int f);
+int  pv_fft_create(pv_bag *b);
+void pv_fft_destroy(pv_bag *b);
+
+#endif

@@ -1,0 +1,604 @@
+/*
+ * pv_substep.c -- sequencing of one RK3 substep on the device and the
+ * per-routine entry points of include/parties_vof.h.
+ *
+ * Reference: PARTIES/src/Temporal_int.c:399-448 (RK loop) and :563-1112
+ * (substep, no IBM); the routine each entry replaces is cited at its head.
+ */
+#include "pv_host.h"
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+
+#define HIPCHK(call)                                                                 \
+    do {                                                                             \
+        hipError_t e_ = (call);                                                      \
+        if (e_ != hipSuccess)                                                        \
+            return pv_set_error(PV_EHIP, "%s failed: %s (%s:%d)", #call,             \
+                                hipGetErrorString(e_), __FILE__, __LINE__);          \
+    } while (0)
+#define LAUNCHCHK() HIPCHK(hipGetLastError())
+#define RC(x) do { int rc_ = (x); if (rc_) return rc_; } while (0)
+
+#define G(b) (&(b)->g)
+#define S(b) ((b)->stream)
+#define FD(b, fid) ((b)->f[fid])
+
+static int comp_of_field(int f) { return (f == PV_F_U || f == PV_F_U_D) ? 'u' : 'v'; }
+
+static int begin(pv_bag *b)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->device));
+    return PV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* Cahn-Hilliard                                                       */
+/* ------------------------------------------------------------------ */
+
+/* diffuse_solve_biharmonic_pcg loop (VOF_DIFFUSE.c:383-423) after its start
+ * has been enqueued; x = C_L, r = CH_RES, p = CH_DIR, Ap = LAP_C. */
+static int ch_cg_loop(pv_bag *b, int stage)
+{
+    double *x = FD(b, PV_F_C_L), *r = FD(b, PV_F_CH_RES), *p = FD(b, PV_F_CH_DIR), *Ap = FD(b, PV_F_LAP_C);
+    double tol = b->ph.ch_tol;
+    int chunk = b->ch_chunk, total = 0;
+    pvk_bc_ch(G(b), p, S(b));
+    for (;;) {
+        int k;
+        for (k = 0; k < chunk; k++) {
+            pvk_ch_apply(G(b), &b->ph, stage, b->d_sc, p, Ap, &b->rd, S(b));
+            pvk_ch_alpha(G(b), b->d_sc, &b->rd, S(b));
+            pvk_ch_update(G(b), b->d_sc, x, p, r, Ap, &b->rd, S(b));
+            pvk_ch_beta(G(b), b->d_sc, tol, &b->rd, S(b));
+            pvk_ch_pupdate(G(b), b->d_sc, r, p, S(b));
+            pvk_bc_ch(G(b), p, S(b));
+        }
+        LAUNCHCHK();
+        total += chunk;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->ch_done) break;
+        if (total > b->ph.ch_iter_max + 8)
+            return pv_set_error(PV_EHIP, "biharmonic CG did not stop (device flag)");
+        chunk = 2;
+    }
+    b->ch_chunk = (b->h_sc->ch_iter > 2) ? b->h_sc->ch_iter - 1 : 2;
+    b->st.n_ch[stage] = b->h_sc->ch_iter;
+    b->st.ch_res[stage] = b->h_sc->ch_rel;
+    pvk_bc_ch(G(b), x, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+/* VOF_DIFFUSE_step (VOF_DIFFUSE.c:1262-1359) + VOF_update_density_viscosity */
+static int ch_step(pv_bag *b)
+{
+    int st = b->stage;
+    pvk_ch_rhs_init(G(b), &b->ph, st, b->d_sc, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
+                    FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), &b->rd, S(b));
+    pvk_ch_finalize_init(G(b), b->d_sc, b->ph.ch_iter_max, b->ph.ch_tol, &b->rd, S(b));
+    LAUNCHCHK();
+    RC(ch_cg_loop(b, st));
+    pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
+                    b->d_sc, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_set_boundary_values(pv_bag *b, int field)
+{
+    RC(begin(b));
+    if (field < 0 || field >= PV_NFIELDS) return pv_set_error(PV_EINVAL, "bad field");
+    pvk_bc_ch(G(b), FD(b, field), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_compute_laplacian(pv_bag *b, int in, int out)
+{
+    RC(begin(b));
+    pvk_bc_ch(G(b), FD(b, in), S(b));
+    pvk_laplacian(G(b), FD(b, in), FD(b, out), S(b));
+    pvk_bc_ch(G(b), FD(b, out), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_compute_bulk_S(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_bulk_S(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_BULK_S), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_BULK_S), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_compute_psi_LG(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_laplacian(G(b), FD(b, PV_F_C_L), FD(b, PV_F_LAP_C), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_LAP_C), S(b));
+    pvk_psi_only(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_PSI_LG), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_PSI_LG), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_advect_WENO5(pv_bag *b, int rhs_out)
+{
+    RC(begin(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_weno_advect(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V), FD(b, rhs_out), S(b));
+    pvk_bc_ch(G(b), FD(b, rhs_out), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_explicit_diffusion(pv_bag *b, int rhs_out)
+{
+    RC(begin(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_explicit_diffusion(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, rhs_out), S(b));
+    pvk_bc_ch(G(b), FD(b, rhs_out), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_solve_implicit_biharmonic(pv_bag *b, int rhs, int *iters, double *rel_res)
+{
+    RC(begin(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_ch_cg_init_from_b(G(b), &b->ph, b->stage, b->d_sc, FD(b, PV_F_C_L), FD(b, rhs), FD(b, PV_F_CH_RES),
+                          FD(b, PV_F_CH_DIR), &b->rd, S(b));
+    pvk_ch_finalize_init(G(b), b->d_sc, b->ph.ch_iter_max, b->ph.ch_tol, &b->rd, S(b));
+    LAUNCHCHK();
+    RC(ch_cg_loop(b, b->stage));
+    if (iters) *iters = b->h_sc->ch_iter;
+    if (rel_res) *rel_res = b->h_sc->ch_rel;
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_bound_liquid_fraction(pv_bag *b, double *mass_delta)
+{
+    RC(begin(b));
+    pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
+                    b->d_sc, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (mass_delta) *mass_delta = b->h_sc->clamp_delta;
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_update_phase_cache(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_phase_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_update_density_viscosity(pv_bag *b) { return pv_VOF_DIFFUSE_update_phase_cache(b); }
+
+int pv_VOF_DIFFUSE_step(pv_bag *b)
+{
+    RC(begin(b));
+    RC(ch_step(b));
+    RC(pv_read_scalars(b));
+    b->st.clamp_delta[b->stage] = b->h_sc->clamp_delta;
+    return PV_OK;
+}
+
+int pv_VOF_DIFFUSE_compute_f_sigma(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
+                   FD(b, PV_F_FSY_NEW), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_apply_f_sigma_old(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_mom_force(G(b), &b->ph, 'u', FD(b, PV_F_RHO), FD(b, PV_F_FSX_OLD), FD(b, PV_F_U_RHS), 1, S(b));
+    pvk_mom_force(G(b), &b->ph, 'v', FD(b, PV_F_RHO), FD(b, PV_F_FSY_OLD), FD(b, PV_F_V_RHS), 1, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_VOF_swap_f_sigma(pv_bag *b)
+{
+    double *t;
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    t = b->f[PV_F_FSX_OLD]; b->f[PV_F_FSX_OLD] = b->f[PV_F_FSX_NEW]; b->f[PV_F_FSX_NEW] = t;
+    t = b->f[PV_F_FSY_OLD]; b->f[PV_F_FSY_OLD] = b->f[PV_F_FSY_NEW]; b->f[PV_F_FSY_NEW] = t;
+    return PV_OK;
+}
+
+int pv_Velocity_add_gravity_2_RHS(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_mom_force(G(b), &b->ph, 'u', FD(b, PV_F_RHO), NULL, FD(b, PV_F_U_RHS), 2, S(b));
+    pvk_mom_force(G(b), &b->ph, 'v', FD(b, PV_F_RHO), NULL, FD(b, PV_F_V_RHS), 2, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* momentum                                                            */
+/* ------------------------------------------------------------------ */
+
+int pv_Velocity_cell_center(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_cell_center(G(b), FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_U_BC), FD(b, PV_F_V_BC), S(b));
+    pvk_halo_cell(G(b), FD(b, PV_F_U_BC), S(b));
+    pvk_halo_cell(G(b), FD(b, PV_F_V_BC), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+static int mom_terms(pv_bag *b, int comp, int mode, int add_force)
+{
+    int u = (comp == 'u');
+    pvk_mom_terms(G(b), &b->ph, b->stage, b->d_sc, comp, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_MU),
+                  FD(b, PV_F_RHO), FD(b, PV_F_P), FD(b, u ? PV_F_U_EXPL_OLD : PV_F_V_EXPL_OLD),
+                  FD(b, u ? PV_F_U_VISC_OLD : PV_F_V_VISC_OLD), FD(b, u ? PV_F_FSX_OLD : PV_F_FSY_OLD),
+                  FD(b, u ? PV_F_U_RHS : PV_F_V_RHS), FD(b, u ? PV_F_U_EXPL : PV_F_V_EXPL),
+                  FD(b, u ? PV_F_U_IMPL : PV_F_V_IMPL), FD(b, u ? PV_F_U_VISC : PV_F_V_VISC), mode, add_force,
+                  S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_Velocity_u_set_implicit_explicit(pv_bag *b) { RC(begin(b)); return mom_terms(b, 'u', 1, 0); }
+int pv_Velocity_v_set_implicit_explicit(pv_bag *b) { RC(begin(b)); return mom_terms(b, 'v', 1, 0); }
+int pv_Velocity_u_set_RHS(pv_bag *b) { RC(begin(b)); return mom_terms(b, 'u', 2, 0); }
+int pv_Velocity_v_set_RHS(pv_bag *b) { RC(begin(b)); return mom_terms(b, 'v', 2, 0); }
+
+int pv_Velocity_update_boundaries(pv_bag *b, int field, int type)
+{
+    RC(begin(b));
+    if (field < 0 || field >= PV_NFIELDS) return pv_set_error(PV_EINVAL, "bad field");
+    pvk_vel_bc(G(b), FD(b, field), comp_of_field(field), type, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_matVec(pv_bag *b, int out, int in, int component)
+{
+    RC(begin(b));
+    if (component != 'u' && component != 'v') return pv_set_error(PV_EINVAL, "component must be 'u' or 'v'");
+    pvk_mom_matvec(G(b), &b->ph, b->stage, b->d_sc, component, FD(b, in), FD(b, PV_F_RHO), FD(b, PV_F_MU),
+                   FD(b, out), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+/* Velocity_solve_cg (lsolver/msolve_cg.c:525-722) */
+static int mom_solve(pv_bag *b, int comp)
+{
+    int u = (comp == 'u'), c = u ? 0 : 1, st = b->stage, chunk = b->mom_chunk[c], total = 0;
+    double *x = FD(b, u ? PV_F_U : PV_F_V), *rhs = FD(b, u ? PV_F_U_RHS : PV_F_V_RHS);
+    double *r = FD(b, u ? PV_F_U_R : PV_F_V_R), *d = FD(b, u ? PV_F_U_D : PV_F_V_D);
+    double *Ad = FD(b, u ? PV_F_U_AD : PV_F_V_AD), *dg = FD(b, u ? PV_FI_DG_U : PV_FI_DG_V);
+    int kind = u ? 4 : 6;
+    pvk_mom_cg_init(G(b), &b->ph, st, b->d_sc, comp, x, rhs, FD(b, PV_F_RHO), FD(b, PV_F_MU), r, d, Ad, dg,
+                    &b->rd, S(b));
+    pvk_mom_scalar(G(b), b->d_sc, comp, 0, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
+    for (;;) {
+        int k;
+        for (k = 0; k < chunk; k++) {
+            pvk_ghost_multi(G(b), d, NULL, NULL, NULL, 1, kind, S(b));
+            pvk_mom_cg_apply(G(b), &b->ph, st, b->d_sc, comp, d, FD(b, PV_F_RHO), FD(b, PV_F_MU), Ad, &b->rd,
+                             S(b));
+            pvk_mom_scalar(G(b), b->d_sc, comp, 1, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
+            pvk_mom_cg_update(G(b), b->d_sc, comp, x, d, r, Ad, dg, &b->rd, S(b));
+            pvk_mom_scalar(G(b), b->d_sc, comp, 2, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
+            pvk_mom_cg_dupdate(G(b), b->d_sc, comp, r, dg, d, S(b));
+        }
+        LAUNCHCHK();
+        total += chunk;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->m_done[c]) break;
+        if (total > b->ph.CG_MAXIT + 8) return pv_set_error(PV_EHIP, "momentum CG did not stop (device flag)");
+        chunk = 2;
+    }
+    b->mom_chunk[c] = (b->h_sc->m_iter[c] > 2) ? b->h_sc->m_iter[c] - 1 : 2;
+    if (u) b->st.n_u[st] = b->h_sc->m_iter[c]; else b->st.n_v[st] = b->h_sc->m_iter[c];
+    return PV_OK;
+}
+
+int pv_Velocity_solve_cg(pv_bag *b, int component, int *iters)
+{
+    RC(begin(b));
+    if (component != 'u' && component != 'v') return pv_set_error(PV_EINVAL, "component must be 'u' or 'v'");
+    RC(mom_solve(b, component));
+    if (iters) *iters = b->h_sc->m_iter[component == 'u' ? 0 : 1];
+    return PV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* pressure                                                            */
+/* ------------------------------------------------------------------ */
+
+int pv_Pressure_set_RHS(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_p_rhs2(G(b), b->stage, b->d_sc, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_RHO), FD(b, PV_F_P_RHS), NULL,
+               S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_Pressure_compute_preconditioner(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_p_rhs2(G(b), b->stage, b->d_sc, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_RHO), FD(b, PV_FI_TMP),
+               FD(b, PV_F_M_INV), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_Pressure_operator_variableCoeff(pv_bag *b, int out, int in)
+{
+    RC(begin(b));
+    pvk_p_operator(G(b), FD(b, PV_F_RHO), FD(b, in), FD(b, out), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_Pressure_apply_BCs(pv_bag *b, int field)
+{
+    RC(begin(b));
+    pvk_p_bc(G(b), FD(b, field), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+/* Jacobi-PCG (psolve_cg.c:130-293); phi starts at zero. */
+static int p_solve_jacobi(pv_bag *b)
+{
+    double *phi = FD(b, PV_F_PHI), *rhs = FD(b, PV_F_P_RHS), *r = FD(b, PV_F_P_RES), *d = FD(b, PV_F_P_D);
+    double *Ad = FD(b, PV_F_P_AD), *M = FD(b, PV_F_M_INV);
+    int maxit = b->ph.P_CG_MAXIT, chunk = b->p_chunk, total = 0;
+    double tol = b->ph.P_CG_ETOL;
+    pvk_p_cg_init2(G(b), rhs, M, phi, r, d, 1, 0, b->d_sc, &b->rd, S(b));
+    pvk_p_scalar2(G(b), b->d_sc, 0, 0, maxit, tol, &b->rd, 0, S(b));
+    for (;;) {
+        int k;
+        for (k = 0; k < chunk; k++) {
+            pvk_p_bc(G(b), d, S(b));
+            pvk_p_cg_apply(G(b), b->d_sc, FD(b, PV_F_RHO), d, Ad, &b->rd, S(b));
+            pvk_p_scalar2(G(b), b->d_sc, 1, 0, maxit, tol, &b->rd, 0, S(b));
+            pvk_p_cg_update(G(b), b->d_sc, phi, d, r, Ad, M, 1, &b->rd, S(b));
+            pvk_p_scalar2(G(b), b->d_sc, 2, 0, maxit, tol, &b->rd, 0, S(b));
+            pvk_p_cg_dupdate(G(b), b->d_sc, r, M, 1, d, S(b));
+        }
+        LAUNCHCHK();
+        total += chunk;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (total > maxit + 16) return pv_set_error(PV_EHIP, "pressure CG did not stop (device flag)");
+        chunk = 4;
+    }
+    b->p_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 4;
+    return PV_OK;
+}
+
+/* PCG with the FFT-x / DCT-y preconditioner; relative-residual stop. */
+static int p_solve_fft_pcg(pv_bag *b)
+{
+    double *phi = FD(b, PV_F_PHI), *rhs = FD(b, PV_F_P_RHS), *r = FD(b, PV_F_P_RES), *d = FD(b, PV_F_P_D);
+    double *Ad = FD(b, PV_F_P_AD), *z = FD(b, PV_FI_Z);
+    int maxit = b->ph.p_max_iter, chunk = b->p_chunk, total = 0;
+    int nbr = pvk_fft_rows_blocks(G(b), &b->fft);
+    double tol = b->ph.p_rel_tol;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    pvk_p_mean(G(b), rhs, &b->rd, S(b));
+    pvk_p_mean_finalize(G(b), b->d_sc, &b->rd, S(b));
+    pvk_p_cg_init2(G(b), rhs, NULL, phi, r, d, 0, 1, b->d_sc, &b->rd, S(b));
+    pvk_p_scalar2(G(b), b->d_sc, 0, 1, maxit, tol, &b->rd, 0, S(b));
+    pvk_fft_solve_dot(G(b), &b->fft, b->d_sc, r, z, 1, b->rd.part, S(b));
+    pvk_p_scalar2(G(b), b->d_sc, 4, 1, maxit, tol, &b->rd, nbr, S(b));
+    pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
+    for (;;) {
+        int k;
+        for (k = 0; k < chunk; k++) {
+            pvk_p_bc(G(b), d, S(b));
+            pvk_p_cg_apply(G(b), b->d_sc, FD(b, PV_F_RHO), d, Ad, &b->rd, S(b));
+            pvk_p_scalar2(G(b), b->d_sc, 1, 1, maxit, tol, &b->rd, 0, S(b));
+            pvk_p_cg_update(G(b), b->d_sc, phi, d, r, Ad, NULL, 0, &b->rd, S(b));
+            pvk_p_scalar2(G(b), b->d_sc, 2, 1, maxit, tol, &b->rd, 0, S(b));
+            pvk_fft_solve_dot(G(b), &b->fft, b->d_sc, r, z, 1, b->rd.part, S(b));
+            pvk_p_scalar2(G(b), b->d_sc, 3, 1, maxit, tol, &b->rd, nbr, S(b));
+            pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
+        }
+        LAUNCHCHK();
+        total += chunk;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (total > maxit + 16) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
+        chunk = 4;
+    }
+    b->p_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 4;
+    return PV_OK;
+}
+
+/* Pressure_solve_cg + the mean-removal gauge (psolve_cg.c:255-279) */
+static int p_solve(pv_bag *b)
+{
+    if (b->prm.pressure_solver == PV_PSOLVE_FFT_PCG) RC(p_solve_fft_pcg(b));
+    else RC(p_solve_jacobi(b));
+    if (b->h_sc->p_done != 4) {
+        pvk_p_mean(G(b), FD(b, PV_F_PHI), &b->rd, S(b));
+        pvk_p_mean_finalize(G(b), b->d_sc, &b->rd, S(b));
+        pvk_p_sub_mean(G(b), b->d_sc, FD(b, PV_F_PHI), 1, S(b));
+        LAUNCHCHK();
+    }
+    b->st.n_p[b->stage] += b->h_sc->p_iter;
+    b->st.p_res[b->stage] = b->h_sc->p_res;
+    return PV_OK;
+}
+
+int pv_Pressure_solve_cg(pv_bag *b, int *iters)
+{
+    RC(begin(b));
+    RC(p_solve(b));
+    if (iters) *iters = b->h_sc->p_iter;
+    return PV_OK;
+}
+
+int pv_Pressure_project_velocity(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_p_bc(G(b), FD(b, PV_F_PHI), S(b));
+    pvk_project(G(b), b->stage, b->d_sc, FD(b, PV_F_RHO), FD(b, PV_F_PHI), FD(b, PV_F_P), FD(b, PV_F_U),
+                FD(b, PV_F_V), S(b));
+    pvk_p_bc(G(b), FD(b, PV_F_P), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+int pv_Pressure_compute_velocity_divergence(pv_bag *b, double *div_max)
+{
+    RC(begin(b));
+    pvk_divergence(G(b), FD(b, PV_F_U), FD(b, PV_F_V), &b->rd, S(b));
+    pvk_div_finalize(G(b), b->d_sc, &b->rd, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (div_max) *div_max = b->h_sc->div_max;
+    return PV_OK;
+}
+
+int pv_Pressure_solve_fft(pv_bag *b, int rhs_field, int out_field)
+{
+    RC(begin(b));
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT solve needs power-of-two NXM, NYM >= 8");
+    pvk_fft_solve(G(b), &b->fft, FD(b, rhs_field), FD(b, out_field), 1, S(b));
+    pvk_pin_row0(G(b), FD(b, out_field), S(b));
+    pvk_halo_cell(G(b), FD(b, out_field), S(b));
+    LAUNCHCHK();
+    HIPCHK(hipStreamSynchronize(S(b)));
+    return PV_OK;
+}
+
+/* ------------------------------------------------------------------ */
+/* substep, time step, RK3                                             */
+/* ------------------------------------------------------------------ */
+
+/* Temporal_int.c:1051-1107 */
+static int pressure_passes(pv_bag *b)
+{
+    int passes = 0, st = b->stage;
+    double div;
+    b->st.n_p[st] = 0;
+    do {
+        pvk_fill(FD(b, PV_F_PHI) - 3 * (size_t)b->P - 3, (size_t)b->P * b->NYT, 0.0, S(b));
+        pvk_p_rhs2(G(b), st, b->d_sc, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_RHO), FD(b, PV_F_P_RHS),
+                   FD(b, PV_F_M_INV), S(b));
+        LAUNCHCHK();
+        RC(p_solve(b));
+        RC(pv_Pressure_project_velocity(b));
+        pvk_vel_bc(G(b), FD(b, PV_F_U), 'u', PV_VEL_TYPE_NORMAL, S(b));
+        pvk_vel_bc(G(b), FD(b, PV_F_V), 'v', PV_VEL_TYPE_NORMAL, S(b));
+        RC(pv_Pressure_compute_velocity_divergence(b, &div));
+        passes++;
+        if (passes > 10) {
+            b->st.n_pass[st] = passes;
+            return pv_set_error(PV_ENOCONV, "Poisson solver did not converge (div %g after %d passes)", div,
+                                passes);
+        }
+    } while (div > 1e-6);
+    b->st.n_pass[st] = passes;
+    b->st.div_max[st] = div;
+    return PV_OK;
+}
+
+/* Temporal_int.c:563-1112 (no IBM / CONC / LES / RANS) */
+int pv_Temporal_int_all_the_equations(pv_bag *b)
+{
+    int st;
+    RC(begin(b));
+    st = b->stage;
+    RC(ch_step(b));
+    RC(mom_terms(b, 'u', 2, 1));
+    RC(mom_terms(b, 'v', 2, 1));
+    pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
+                   FD(b, PV_F_FSY_NEW), S(b));
+    LAUNCHCHK();
+    RC(mom_solve(b, 'u'));
+    RC(mom_solve(b, 'v'));
+    pvk_vel_bc(G(b), FD(b, PV_F_U), 'u', PV_VEL_TYPE_NORMAL, S(b));
+    pvk_vel_bc(G(b), FD(b, PV_F_V), 'v', PV_VEL_TYPE_NORMAL, S(b));
+    LAUNCHCHK();
+    RC(pressure_passes(b));
+    b->st.clamp_delta[st] = b->h_sc->clamp_delta;
+    RC(pv_VOF_swap_f_sigma(b));
+    return PV_OK;
+}
+
+/* Dtime.c:24-249 */
+int pv_Dtime_cfl(pv_bag *b, double *dt_out)
+{
+    RC(begin(b));
+    pvk_cfl(G(b), FD(b, PV_F_U), FD(b, PV_F_V), &b->rd, S(b));
+    pvk_cfl_finalize2(G(b), &b->ph, b->d_sc, b->dx_min, &b->rd, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (dt_out) *dt_out = b->h_sc->dt_new;
+    return PV_OK;
+}
+
+/* Temporal_int.c:399-448 */
+int pv_Temporal_int_rk3(pv_bag *b, int nsteps)
+{
+    int n, rk;
+    RC(begin(b));
+    for (n = 0; n < nsteps; n++) {
+        for (rk = 0; rk < 3; rk++) {
+            if (b->time_nonzero && rk == 0) {
+                pvk_cfl(G(b), FD(b, PV_F_U), FD(b, PV_F_V), &b->rd, S(b));
+                pvk_cfl_finalize2(G(b), &b->ph, b->d_sc, b->dx_min, &b->rd, S(b));
+                pvk_take_dt(b->d_sc, S(b));
+                LAUNCHCHK();
+            }
+            b->stage = rk;
+            RC(pv_Temporal_int_all_the_equations(b));
+            pvk_advance_time(b->d_sc, rk, S(b));
+            LAUNCHCHK();
+            b->time_nonzero = 1;
+        }
+        b->ntime++;
+    }
+    RC(pv_read_scalars(b));
+    return PV_OK;
+}
+
+/* Cart3d.c:1094-1130, VOF_DIFFUSE.c:524-568, Temporal_int.c:163-199 */
+int pv_init_from_F(pv_bag *b, const double *F_host)
+{
+    RC(begin(b));
+    if (!F_host) return pv_set_error(PV_EINVAL, "null F");
+    pvk_fill(b->slabmem, b->alloc_elems * PV_FI_END, 0.0, S(b));
+    LAUNCHCHK();
+    RC(pv_set_field(b, PV_F_SCRATCH2, F_host));
+    pvk_bc_ch(G(b), FD(b, PV_F_SCRATCH2), S(b));
+    pvk_clamp_init(G(b), FD(b, PV_F_SCRATCH2), FD(b, PV_F_C_L), S(b));
+    pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
+                    b->d_sc, S(b));
+    pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
+                   FD(b, PV_F_FSY_NEW), S(b));
+    pvk_copy(G(b), FD(b, PV_F_FSX_NEW), FD(b, PV_F_FSX_OLD), S(b));
+    pvk_copy(G(b), FD(b, PV_F_FSY_NEW), FD(b, PV_F_FSY_OLD), S(b));
+    LAUNCHCHK();
+    memset(&b->st, 0, sizeof b->st);
+    b->time = 0.0; b->dt = b->dt_old = b->prm.default_dt; b->stage = 0; b->ntime = 1; b->time_nonzero = 0;
+    memset(b->h_sc, 0, sizeof(pvk_scal));
+    RC(pv_sync_scalars_to_device(b));
+    RC(pv_Velocity_cell_center(b));
+    HIPCHK(hipStreamSynchronize(S(b)));
+    return PV_OK;
+}

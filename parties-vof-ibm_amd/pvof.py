@@ -1,0 +1,225 @@
+"""ctypes view of the C ABI in include/parties_vof.h.
+
+Host-side mirror for Python callers (tests, bench.py).  The product is the C/HIP
+library libparties_vof.so built in this directory; this module only loads it and
+declares argument types.  Nothing here computes physics.
+
+Reference parameter names follow PARTIES/src/IO/default.inp.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "_build", "libparties_vof.so")
+
+PV_PSOLVE_JACOBI_PCG = 0
+PV_PSOLVE_FFT_PCG = 1
+PV_PREC_FP64 = 0
+PV_PREC_MIXED = 1
+PV_VEL_TYPE_NORMAL = 0
+PV_VEL_TYPE_CG = 1
+
+FIELDS = [
+    "C_L", "C_G", "RHO", "MU",
+    "PSI_LG", "LAP_C", "BULK_S", "PSI",
+    "CH_AUX1", "CH_AUX2", "CH_RHS_N", "CH_RHS_NM1",
+    "CH_RES", "CH_DIR", "FLUX_X", "FLUX_Y",
+    "FSX_NEW", "FSY_NEW", "FSX_OLD", "FSY_OLD",
+    "U", "V", "U_BC", "V_BC",
+    "U_EXPL", "U_EXPL_OLD", "U_IMPL", "U_VISC", "U_VISC_OLD", "U_RHS",
+    "V_EXPL", "V_EXPL_OLD", "V_IMPL", "V_VISC", "V_VISC_OLD", "V_RHS",
+    "U_R", "U_D", "U_AD", "V_R", "V_D", "V_AD",
+    "P", "PHI", "P_RHS", "P_RES", "P_D", "P_AD", "M_INV",
+    "SCRATCH0", "SCRATCH1", "SCRATCH2",
+]
+F = {name: k for k, name in enumerate(FIELDS)}
+PV_NFIELDS = len(FIELDS)
+
+
+class Params(C.Structure):
+    _fields_ = [
+        ("NXM", C.c_int), ("NYM", C.c_int),
+        ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+        ("twod_slab_thickness", C.c_double),
+        ("Re", C.c_double), ("dp_dx", C.c_double),
+        ("We", C.c_double), ("rho1", C.c_double), ("rho2", C.c_double),
+        ("mu1", C.c_double), ("mu2", C.c_double),
+        ("Cn", C.c_double), ("Pe_CH", C.c_double),
+        ("weno_order", C.c_int), ("ch_iter_max", C.c_int), ("ch_tol", C.c_double),
+        ("cfl", C.c_double), ("default_dt", C.c_double), ("max_dt", C.c_double),
+        ("constant_dt", C.c_int),
+        ("grav", C.c_double * 3), ("richardson", C.c_double),
+        ("P_CG_MAXIT", C.c_int), ("P_CG_ETOL", C.c_double),
+        ("CG_MAXIT", C.c_int), ("CG_ETOL", C.c_double),
+        ("pressure_solver", C.c_int), ("p_rel_tol", C.c_double), ("p_max_iter", C.c_int),
+        ("precision", C.c_int),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [
+        ("time", C.c_double), ("dt", C.c_double), ("dt_old", C.c_double),
+        ("ntime", C.c_int), ("which_stage", C.c_int),
+        ("n_ch", C.c_int * 3), ("n_u", C.c_int * 3), ("n_v", C.c_int * 3),
+        ("n_p", C.c_int * 3), ("n_pass", C.c_int * 3),
+        ("ch_res", C.c_double * 3), ("p_res", C.c_double * 3),
+        ("div_max", C.c_double * 3), ("clamp_delta", C.c_double * 3),
+        ("p_fallback", C.c_int * 3),
+    ]
+
+    def as_dict(self):
+        out = {}
+        for name, _ in self._fields_:
+            val = getattr(self, name)
+            out[name] = list(val) if hasattr(val, "__len__") else val
+        return out
+
+
+def default_params(**kw):
+    """IO/default.inp defaults (2-D planar, no IBM); keyword overrides."""
+    p = Params()
+    p.NXM, p.NYM = 64, 64
+    p.xmin, p.xmax, p.ymin, p.ymax = 0.0, 1.0, 0.0, 1.0
+    p.twod_slab_thickness = -1.0
+    p.Re, p.dp_dx = 100.0, 0.0
+    p.We, p.rho1, p.rho2, p.mu1, p.mu2 = 1.0, 1.0, 1.0, 1.0, 1.0
+    p.Cn, p.Pe_CH = -1.0, -1.0
+    p.weno_order, p.ch_iter_max, p.ch_tol = 5, 300, 1e-9
+    p.cfl, p.default_dt, p.max_dt, p.constant_dt = 0.5, 1e-3, 0.05, 0
+    p.grav[0], p.grav[1], p.grav[2] = 0.0, -1.0, 0.0
+    p.richardson = 1.0
+    p.P_CG_MAXIT, p.P_CG_ETOL = 300, 1e-7
+    p.CG_MAXIT, p.CG_ETOL = 120, 1e-9
+    p.pressure_solver, p.p_rel_tol, p.p_max_iter = PV_PSOLVE_JACOBI_PCG, 1e-10, 2000
+    p.precision = PV_PREC_FP64
+    for k, v in kw.items():
+        if k == "grav":
+            for i in range(3):
+                p.grav[i] = v[i]
+        else:
+            setattr(p, k, v)
+    return p
+
+
+def config1_params(NXM=128, NYM=256, **kw):
+    """Planar rising bubble: 1x2 domain, x periodic, y walls, rho2/rho1 = 1e-3,
+    mu2/mu1 = 1e-2, Re = 100, We = Bo = 200, Cn = 0.01, Pe = 1/Cn, CFL 0.2."""
+    base = dict(NXM=NXM, NYM=NYM, xmax=1.0, ymax=2.0, Re=100.0, We=200.0,
+                rho1=1.0, rho2=1e-3, mu1=1.0, mu2=1e-2, Cn=0.01, Pe_CH=100.0,
+                cfl=0.2, default_dt=1e-4, max_dt=1e-3)
+    base.update(kw)
+    return default_params(**base)
+
+
+def shape_of(p):
+    return (p.NYM + 7, p.NXM + 7)
+
+
+def load_lib(path=LIB_PATH):
+    """Load the product library; raises if it has not been built."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run __graft_entry__.build() first")
+    lib = C.CDLL(path)
+    dp = C.POINTER(C.c_double)
+    vp = C.c_void_p
+    lib.pv_last_error.restype = C.c_char_p
+    lib.pv_params_default.argtypes = [C.POINTER(Params)]
+    lib.pv_create.argtypes = [C.POINTER(Params), C.c_int, C.POINTER(vp)]
+    lib.pv_destroy.argtypes = [vp]
+    lib.pv_get_params.argtypes = [vp, C.POINTER(Params)]
+    lib.pv_set_field.argtypes = [vp, C.c_int, dp]
+    lib.pv_get_field.argtypes = [vp, C.c_int, dp]
+    lib.pv_fill_field.argtypes = [vp, C.c_int, C.c_double]
+    lib.pv_init_from_F.argtypes = [vp, dp]
+    lib.pv_set_time_state.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
+    lib.pv_get_stats.argtypes = [vp, C.POINTER(Stats)]
+    lib.pv_Temporal_int_rk3.argtypes = [vp, C.c_int]
+    lib.pv_Dtime_cfl.argtypes = [vp, dp]
+    lib.pv_VOF_DIFFUSE_solve_implicit_biharmonic.argtypes = [vp, C.c_int, C.POINTER(C.c_int), dp]
+    lib.pv_VOF_DIFFUSE_bound_liquid_fraction.argtypes = [vp, dp]
+    lib.pv_Velocity_solve_cg.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
+    lib.pv_Pressure_solve_cg.argtypes = [vp, C.POINTER(C.c_int)]
+    lib.pv_Pressure_compute_velocity_divergence.argtypes = [vp, dp]
+    for name in ("pv_ic_rising_bubble", "pv_ic_rayleigh_taylor"):
+        getattr(lib, name).argtypes = [C.POINTER(Params), dp]
+    lib.pv_ic_multimode_rt.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_double, C.c_double, dp]
+    lib.pv_ic_bubbles.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_int, C.c_double,
+                                  C.c_double, C.c_int, dp, C.POINTER(C.c_int)]
+    return lib
+
+
+def _dptr(a):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+class Solver:
+    """Thin owner of one pv_bag (one 2-D subdomain on one GPU)."""
+
+    def __init__(self, params, lib=None, device=0):
+        self.lib = lib or load_lib()
+        self.h = C.c_void_p()
+        self.check(self.lib.pv_create(C.byref(params), device, C.byref(self.h)))
+        self.params = Params()
+        self.check(self.lib.pv_get_params(self.h, C.byref(self.params)))
+        self.shape = shape_of(self.params)
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"parties_vof error {rc}: {self.lib.pv_last_error().decode()}")
+        return rc
+
+    def close(self):
+        if self.h:
+            self.lib.pv_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def get(self, name):
+        a = np.zeros(self.shape)
+        self.check(self.lib.pv_get_field(self.h, F[name], _dptr(a)))
+        return a
+
+    def set(self, name, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        assert a.shape == self.shape
+        self.check(self.lib.pv_set_field(self.h, F[name], _dptr(a)))
+
+    def init_from_F(self, Fld):
+        Fld = np.ascontiguousarray(Fld, dtype=np.float64)
+        self.check(self.lib.pv_init_from_F(self.h, _dptr(Fld)))
+
+    def stats(self):
+        st = Stats()
+        self.check(self.lib.pv_get_stats(self.h, C.byref(st)))
+        return st.as_dict()
+
+    def call(self, name, *args):
+        return self.check(getattr(self.lib, "pv_" + name)(self.h, *args))
+
+
+def ic(lib, kind, params, **kw):
+    """Initial volume-fraction field F on the canonical layout."""
+    Fld = np.zeros(shape_of(params))
+    if kind == "rising_bubble":
+        lib.pv_ic_rising_bubble(C.byref(params), _dptr(Fld))
+    elif kind == "rayleigh_taylor":
+        lib.pv_ic_rayleigh_taylor(C.byref(params), _dptr(Fld))
+    elif kind == "multimode_rt":
+        lib.pv_ic_multimode_rt(C.byref(params), kw.get("seed", 20261202), kw.get("modes", 8),
+                               kw.get("amp_max", 0.05), kw.get("y0", 2.0), _dptr(Fld))
+    elif kind == "bubbles":
+        n = C.c_int(0)
+        lib.pv_ic_bubbles(C.byref(params), kw.get("seed", 20261202), kw.get("nmin", 16),
+                          kw.get("nmax", 64), kw.get("rmin", 0.01), kw.get("rmax", 0.08),
+                          kw.get("gas_inside", 1), _dptr(Fld), C.byref(n))
+    else:
+        raise ValueError(kind)
+    return Fld

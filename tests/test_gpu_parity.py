@@ -1,0 +1,236 @@
+"""GPU parity: each HIP entry point against the CPU oracle on the same state.
+
+Stencil kernels are compared bit for bit (both sides are built without FMA
+contraction and follow the reference's operand order), except WENO5, whose
+smoothness indicators use pow(x, 2.0) in the oracle (glibc, not always
+correctly rounded) and x*x on the GPU: 1e-13 relative there.  Iterative
+solves are compared on their converged solutions.
+"""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle_lib
+import pvof
+from helpers import copy_state, oracle_state, regions, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def params_small(**kw):
+    base = dict(ch_tol=1e-13, CG_ETOL=1e-7, CG_MAXIT=400, P_CG_ETOL=1e-12, P_CG_MAXIT=6000)
+    base.update(kw)
+    return pvof.config1_params(NXM=32, NYM=64, **base)
+
+
+@pytest.fixture(scope="module")
+def lib():
+    return pvof.load_lib()
+
+
+@pytest.fixture(scope="module")
+def state(gpu, lib):
+    p = params_small()
+    F = pvof.ic(lib, "rising_bubble", p)
+    o = oracle_state(p, F, 3)
+    return p, F, o
+
+
+def fresh(state, lib, stage=1):
+    p, F, o_base = state
+    o = oracle_lib.Oracle(p)
+    for name in pvof.FIELDS:
+        o.set(name, o_base.get(name))
+    st = o_base.stats()
+    o.call("set_time_state", st["time"], st["dt"], st["dt_old"], stage)
+    s = pvof.Solver(p, lib)
+    copy_state(o, s, stage)
+    return p, o, s
+
+
+def test_init_matches(gpu, lib):
+    p = params_small()
+    F = pvof.ic(lib, "rising_bubble", p)
+    o = oracle_lib.Oracle(p)
+    o.init_from_F(F)
+    s = pvof.Solver(p, lib)
+    s.init_from_F(F)
+    R = regions(p)
+    for name in ("C_L", "C_G", "RHO", "MU"):
+        assert np.array_equal(s.get(name)[R["frame"]], o.get(name)[R["frame"]]), name
+    for name in ("FSX_OLD", "FSY_OLD"):
+        assert np.array_equal(s.get(name)[R["G"]], o.get(name)[R["G"]]), name
+
+
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_weno_advect(state, lib, stage):
+    p, o, s = fresh(state, lib, stage)
+    R = regions(p)
+    o.call("VOF_DIFFUSE_advect_WENO5", pvof.F["CH_AUX2"])
+    s.call("VOF_DIFFUSE_advect_WENO5", pvof.F["CH_AUX2"])
+    assert relerr(s.get("CH_AUX2")[R["frame"]], o.get("CH_AUX2")[R["frame"]]) < 1e-13
+
+
+def test_explicit_diffusion(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    o.call("VOF_DIFFUSE_explicit_diffusion", pvof.F["CH_AUX1"])
+    s.call("VOF_DIFFUSE_explicit_diffusion", pvof.F["CH_AUX1"])
+    assert np.array_equal(s.get("CH_AUX1")[R["frame"]], o.get("CH_AUX1")[R["frame"]])
+
+
+def test_laplacian_and_psi(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    o.call("VOF_DIFFUSE_compute_laplacian", pvof.F["C_L"], pvof.F["SCRATCH0"])
+    s.call("VOF_DIFFUSE_compute_laplacian", pvof.F["C_L"], pvof.F["SCRATCH0"])
+    assert np.array_equal(s.get("SCRATCH0")[R["frame"]], o.get("SCRATCH0")[R["frame"]])
+    o.call("VOF_DIFFUSE_compute_psi_LG")
+    s.call("VOF_DIFFUSE_compute_psi_LG")
+    assert np.array_equal(s.get("PSI_LG")[R["frame"]], o.get("PSI_LG")[R["frame"]])
+
+
+def test_f_sigma(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    o.call("VOF_DIFFUSE_compute_psi_LG")
+    o.call("VOF_DIFFUSE_compute_f_sigma")
+    s.call("VOF_DIFFUSE_compute_f_sigma")
+    assert np.array_equal(s.get("FSX_NEW")[R["u"]], o.get("FSX_NEW")[R["u"]])
+    assert np.array_equal(s.get("FSY_NEW")[R["v"]], o.get("FSY_NEW")[R["v"]])
+    assert np.abs(o.get("FSX_NEW")[R["u"]]).max() > 0
+
+
+@pytest.mark.parametrize("stage", [0, 1, 2])
+def test_momentum_terms_and_rhs(state, lib, stage):
+    p, o, s = fresh(state, lib, stage)
+    R = regions(p)
+    # the substep refreshes the cell-centre velocity first (Temporal_int.c:625-628)
+    o.call("Velocity_cell_center")
+    s.call("Velocity_cell_center")
+    for c in ("u", "v"):
+        o.call(f"Velocity_{c}_set_implicit_explicit")
+        s.call(f"Velocity_{c}_set_implicit_explicit")
+        C_ = c.upper()
+        for fld in ("EXPL", "IMPL", "VISC"):
+            assert np.array_equal(s.get(f"{C_}_{fld}")[R[c]], o.get(f"{C_}_{fld}")[R[c]]), (c, fld)
+        o.call(f"Velocity_{c}_set_RHS")
+        s.call(f"Velocity_{c}_set_RHS")
+        for fld in ("RHS", "EXPL_OLD", "VISC_OLD"):
+            assert np.array_equal(s.get(f"{C_}_{fld}")[R[c]], o.get(f"{C_}_{fld}")[R[c]]), (c, fld)
+    o.call("VOF_apply_f_sigma_old")
+    s.call("VOF_apply_f_sigma_old")
+    o.call("Velocity_add_gravity_2_RHS")
+    s.call("Velocity_add_gravity_2_RHS")
+    assert np.array_equal(s.get("U_RHS")[R["u"]], o.get("U_RHS")[R["u"]])
+    assert np.array_equal(s.get("V_RHS")[R["v"]], o.get("V_RHS")[R["v"]])
+
+
+def test_matvec(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    for c, fld in (("u", "U"), ("v", "V")):
+        o.call("matVec", pvof.F["SCRATCH1"], pvof.F[fld], ord(c))
+        s.call("matVec", pvof.F["SCRATCH1"], pvof.F[fld], ord(c))
+        assert np.array_equal(s.get("SCRATCH1")[R[c]], o.get("SCRATCH1")[R[c]]), c
+
+
+def test_pressure_rhs_precond_operator(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    for name in ("Pressure_set_RHS", "Pressure_compute_preconditioner"):
+        o.call(name)
+        s.call(name)
+    assert np.array_equal(s.get("P_RHS")[R["cell"]], o.get("P_RHS")[R["cell"]])
+    assert np.array_equal(s.get("M_INV")[R["cell"]], o.get("M_INV")[R["cell"]])
+    o.call("Pressure_apply_BCs", pvof.F["P"])
+    s.call("Pressure_apply_BCs", pvof.F["P"])
+    o.call("Pressure_operator_variableCoeff", pvof.F["SCRATCH0"], pvof.F["P"])
+    s.call("Pressure_operator_variableCoeff", pvof.F["SCRATCH0"], pvof.F["P"])
+    assert np.array_equal(s.get("SCRATCH0")[R["cell"]], o.get("SCRATCH0")[R["cell"]])
+
+
+def test_cfl(state, lib):
+    p, o, s = fresh(state, lib, 0)
+    st = o.stats()
+    o.call("set_time_state", st["time"], st["dt"], st["dt"], 0)
+    dt_o = o.lib.pvo_Dtime_cfl(o.h)
+    out = C.c_double()
+    s.call("Dtime_cfl", C.byref(out))
+    assert out.value == dt_o
+
+
+def test_biharmonic_solve(state, lib):
+    p, o, s = fresh(state, lib, 1)
+    R = regions(p)
+    o.call("VOF_DIFFUSE_step")
+    s.call("VOF_DIFFUSE_step")
+    assert relerr(s.get("C_L")[R["frame"]], o.get("C_L")[R["frame"]]) < 1e-12
+    assert relerr(s.get("CH_RHS_N")[R["cell"]], o.get("CH_RHS_N")[R["cell"]]) < 1e-12
+    so, ss = o.stats(), s.stats()
+    assert abs(so["n_ch"][1] - ss["n_ch"][1]) <= 1
+
+
+def test_velocity_solve(state, lib):
+    p, o, s = fresh(state, lib, 2)
+    R = regions(p)
+    o.call("Velocity_cell_center")
+    s.call("Velocity_cell_center")
+    for c in ("u", "v"):
+        o.call(f"Velocity_{c}_set_implicit_explicit")
+        o.call(f"Velocity_{c}_set_RHS")
+        s.call(f"Velocity_{c}_set_RHS")
+        o.call("Velocity_solve_cg", ord(c))
+        it = C.c_int()
+        s.call("Velocity_solve_cg", ord(c), C.byref(it))
+        fld = c.upper()
+        assert relerr(s.get(fld)[R[c]], o.get(fld)[R[c]]) < 1e-10, c
+
+
+def test_pressure_solve_project(state, lib):
+    p, o, s = fresh(state, lib, 1)
+    R = regions(p)
+    for x in (o, s):
+        x.set("PHI", np.zeros(x.shape))
+        x.call("Pressure_set_RHS")
+        x.call("Pressure_compute_preconditioner")
+    o.call("Pressure_solve_cg")
+    it = C.c_int()
+    s.call("Pressure_solve_cg", C.byref(it))
+    assert relerr(s.get("PHI")[R["cell"]], o.get("PHI")[R["cell"]]) < 1e-9
+    o.call("Pressure_project_velocity")
+    s.call("Pressure_project_velocity")
+    for fld, r in (("U", "u"), ("V", "v"), ("P", "cell")):
+        assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-9, fld
+
+
+def test_fft_solve(state, lib):
+    p, o, s = fresh(state, lib)
+    R = regions(p)
+    rng = np.random.default_rng(5)
+    rhs = np.zeros(o.shape)
+    rhs[R["cell"]] = rng.standard_normal((p.NYM, p.NXM))
+    rhs[R["cell"]] -= rhs[R["cell"]].mean()
+    o.set("SCRATCH0", rhs)
+    s.set("SCRATCH0", rhs)
+    o.call("Pressure_solve_fft", pvof.F["SCRATCH0"], pvof.F["SCRATCH1"])
+    s.call("Pressure_solve_fft", pvof.F["SCRATCH0"], pvof.F["SCRATCH1"])
+    assert relerr(s.get("SCRATCH1")[R["cell"]], o.get("SCRATCH1")[R["cell"]]) < 1e-11
+
+
+@pytest.mark.parametrize("solver", [pvof.PV_PSOLVE_JACOBI_PCG, pvof.PV_PSOLVE_FFT_PCG])
+def test_trajectory_10_steps(gpu, lib, solver):
+    p = params_small(pressure_solver=solver, p_rel_tol=1e-13)
+    F = pvof.ic(lib, "rising_bubble", p)
+    o = oracle_lib.Oracle(p)
+    o.init_from_F(F)
+    s = pvof.Solver(p, lib)
+    s.init_from_F(F)
+    assert o.call("Temporal_int_rk3", 10) == 0
+    s.call("Temporal_int_rk3", 10)
+    R = regions(p)
+    so, ss = o.stats(), s.stats()
+    assert abs(so["time"] - ss["time"]) <= 1e-15 * so["time"]
+    for fld, r in (("C_L", "cell"), ("U", "u"), ("V", "v"), ("P", "cell")):
+        assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-11, fld
