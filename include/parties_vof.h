@@ -194,6 +194,33 @@ int pv_Pressure_compute_velocity_divergence(pv_bag *bag, double *div_max);   /* 
  * lsolver/psolve_fft.c:17-207, :467-591, :1536-1802 (z of length 1). */
 int pv_Pressure_solve_fft(pv_bag *bag, int rhs_field, int out_field);
 
+/* ---- kernel timers (per-phase tracing; replaces the MPI_Wtime buckets of
+ * Timer.c:9-63 / Display_time_results, IO/Display.c:573-655) ----
+ * When enabled, launches of the listed kernel classes are bracketed by HIP
+ * events on the handle's stream; totals are resolved at the next stream
+ * synchronisation.  Off by default (no events recorded). */
+enum pv_timer {
+    PV_T_CH_APPLY = 0,     /* biharmonic operator + <p,Ap>       */
+    PV_T_CH_UPDATE,        /* x, r update + <r,r>                */
+    PV_T_MOM_APPLY,        /* momentum operator + <d,Ad>         */
+    PV_T_MOM_UPDATE,       /* x, r update + two sums             */
+    PV_T_P_APPLY,          /* pressure operator + <d,Ad>         */
+    PV_T_P_UPDATE,         /* phi, r update + sums               */
+    PV_T_P_DUPDATE,        /* d = z + beta d                     */
+    PV_T_FFT_ROWS_FWD,     /* forward real FFT of rows           */
+    PV_T_FFT_COLS,         /* DCT-II / scale / DCT-III of columns */
+    PV_T_FFT_ROWS_INV,     /* inverse real FFT + <r,z>           */
+    PV_T_CH_RHS,           /* fused explicit CH right-hand side  */
+    PV_T_MOM_TERMS,        /* fused momentum right-hand side     */
+    PV_NTIMERS
+};
+int pv_timers_enable(pv_bag *bag, int on);
+int pv_timers_reset(pv_bag *bag);
+/* total milliseconds and launch counts per timer (arrays of PV_NTIMERS) */
+int pv_timers_read(pv_bag *bag, double *ms, long long *count);
+/* Block until all work queued on the handle's stream has finished. */
+int pv_synchronize(pv_bag *bag);
+
 /* ---- initial volume fraction F on the G range (host only, no device) ----
  * 6 x 6 sub-sampled tanh profile of width 2 sqrt(2) Cn, as in
  * IO/VoF_Init.c:877-953 / :1032-1100 (F = 1 liquid, 0 gas).  The seeded

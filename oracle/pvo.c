@@ -34,6 +34,15 @@ static const double GAMB[3] = { 2.0, 25.0 / 4.0, 9.0 / 2.0 };
 static const double ZETB[3] = { 0.0, -17.0 / 4.0, -2.5 };
 
 #define MAXD(a, b) ((a) < (b) ? (b) : (a))     /* definitions.h:44 */
+
+/* RK3 tables as used here: out[0..2] BETA, [3..5] BETA2, [6..8] GAMBETA, [9..11] ZETBETA */
+void pvo_rk3_tables(double *out)
+{
+    int k;
+    for (k = 0; k < 3; k++) {
+        out[k] = BET[k]; out[3 + k] = BET2[k]; out[6 + k] = GAMB[k]; out[9 + k] = ZETB[k];
+    }
+}
 #define MIND(a, b) ((a) > (b) ? (b) : (a))     /* definitions.h:45 */
 
 struct pvo_state {
@@ -51,6 +60,7 @@ struct pvo_state {
     double *zbuf;              /* psolve_cg.c:164 per-call z */
     double *wre, *wim, *wrow;  /* FFT workspace (length NXM) */
     double *twc, *tws;         /* cos/sin(2 pi m / NXM), m < NXM */
+    double *fre, *fim;         /* radix-2 FFT work (length NXM) */
     double *tsa, *tsb, *tsc, *tsr, *tsu, *tsg; /* Thomas workspace (NYM) */
 };
 
@@ -127,7 +137,7 @@ pvo_state *pvo_create(const pv_params *prm)
     MA(m_yv, s->NYT); MA(m_yc, s->NYT); MA(m_dy_v, s->NYT); MA(m_dy_c, s->NYT);
     MA(m_idy_v, s->NYT); MA(m_idy_c, s->NYT);
     MA(zbuf, s->nel); MA(wre, s->NXM); MA(wim, s->NXM); MA(wrow, s->NXM);
-    MA(twc, s->NXM); MA(tws, s->NXM);
+    MA(twc, s->NXM); MA(tws, s->NXM); MA(fre, s->NXM); MA(fim, s->NXM);
     MA(tsa, s->NYM); MA(tsb, s->NYM); MA(tsc, s->NYM); MA(tsr, s->NYM);
     MA(tsu, s->NYM); MA(tsg, s->NYM);
 #undef MA
@@ -148,7 +158,7 @@ void pvo_destroy(pvo_state *s)
     free(s->m_xu); free(s->m_xc); free(s->m_dx_u); free(s->m_dx_c);
     free(s->m_idx_u); free(s->m_idx_c); free(s->m_yv); free(s->m_yc);
     free(s->m_dy_v); free(s->m_dy_c); free(s->m_idy_v); free(s->m_idy_c);
-    free(s->zbuf); free(s->wre); free(s->wim); free(s->wrow); free(s->twc); free(s->tws);
+    free(s->zbuf); free(s->wre); free(s->wim); free(s->wrow); free(s->twc); free(s->tws); free(s->fre); free(s->fim);
     free(s->tsa); free(s->tsb); free(s->tsc); free(s->tsr); free(s->tsu); free(s->tsg);
     free(s);
 }
@@ -1084,29 +1094,73 @@ static int p_solve_jacobi(pvo_state *s, double *final_res)
 
 /* ---- FFT-x / tridiagonal-y constant-coefficient solve (A23) -------- */
 
+/* In-place iterative radix-2 complex FFT of length n (power of two) on
+ * s->fre / s->fim; sign = -1 forward, +1 inverse (unnormalised). */
+static void fft_pow2(pvo_state *s, int n, int sign)
+{
+    double *re = s->fre, *im = s->fim;
+    int i, j, k, len;
+    for (i = 1, j = 0; i < n; i++) {
+        int bit = n >> 1;
+        for (; j & bit; bit >>= 1) j ^= bit;
+        j ^= bit;
+        if (i < j) { double t = re[i]; re[i] = re[j]; re[j] = t; t = im[i]; im[i] = im[j]; im[j] = t; }
+    }
+    for (len = 2; len <= n; len <<= 1) {
+        int step = n / len;
+        for (i = 0; i < n; i += len)
+            for (k = 0; k < len / 2; k++) {
+                double wr = s->twc[k * step], wi = sign * s->tws[k * step];
+                double xr = re[i + k + len / 2], xi = im[i + k + len / 2];
+                double tr = wr * xr - wi * xi, ti = wr * xi + wi * xr;
+                re[i + k + len / 2] = re[i + k] - tr; im[i + k + len / 2] = im[i + k] - ti;
+                re[i + k] += tr; im[i + k] += ti;
+            }
+    }
+}
+
+static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
+
 /* Real DFT of length n in FFTW R2HC layout: hc[k] = Re X_k (k <= n/2),
  * hc[n-k] = Im X_k (0 < k < n/2), X_k = sum_m x_m exp(-2 pi i k m / n).
- * Plain O(n^2) evaluation with exact-angle reduction; test sizes only. */
+ * Radix-2 FFT for powers of two, direct summation otherwise. */
 static void r2hc(pvo_state *s, const double *x, double *hc, int n)
 {
     int k, m;
-    for (k = 0; k <= n / 2; k++) {
-        double re = 0.0, im = 0.0;
-        for (m = 0; m < n; m++) {
-            long km = ((long)k * m) % n;
-            re += x[m] * s->twc[km];
-            im -= x[m] * s->tws[km];
+    if (is_pow2(n)) {
+        for (m = 0; m < n; m++) { s->fre[m] = x[m]; s->fim[m] = 0.0; }
+        fft_pow2(s, n, -1);
+        for (k = 0; k <= n / 2; k++) { s->wre[k] = s->fre[k]; s->wim[k] = s->fim[k]; }
+    } else {
+        for (k = 0; k <= n / 2; k++) {
+            double re = 0.0, im = 0.0;
+            for (m = 0; m < n; m++) {
+                long km = ((long)k * m) % n;
+                re += x[m] * s->twc[km];
+                im -= x[m] * s->tws[km];
+            }
+            s->wre[k] = re; s->wim[k] = im;
         }
-        s->wre[k] = re; s->wim[k] = im;
     }
     for (k = 0; k <= n / 2; k++) hc[k] = s->wre[k];
     for (k = 1; k < (n + 1) / 2; k++) hc[n - k] = s->wim[k];
 }
 
 /* Inverse of r2hc without normalisation (FFTW HC2R). */
-static void hc2r(const pvo_state *s, const double *hc, double *x, int n)
+static void hc2r(pvo_state *s, const double *hc, double *x, int n)
 {
     int k, m;
+    if (is_pow2(n)) {
+        s->fre[0] = hc[0]; s->fim[0] = 0.0;
+        s->fre[n / 2] = hc[n / 2]; s->fim[n / 2] = 0.0;
+        for (k = 1; k < n / 2; k++) {
+            s->fre[k] = hc[k]; s->fim[k] = hc[n - k];
+            s->fre[n - k] = hc[k]; s->fim[n - k] = -hc[n - k];
+        }
+        fft_pow2(s, n, +1);
+        for (m = 0; m < n; m++) x[m] = s->fre[m];
+        return;
+    }
     for (m = 0; m < n; m++) {
         double acc = hc[0];
         for (k = 1; k < (n + 1) / 2; k++) {

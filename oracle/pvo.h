@@ -66,6 +66,7 @@ void   pvo_Pressure_apply_BCs(pvo_state *s, int field);
 void   pvo_Pressure_project_velocity(pvo_state *s);
 double pvo_Pressure_compute_velocity_divergence(pvo_state *s);
 void   pvo_Pressure_solve_fft(pvo_state *s, int rhs_field, int out_field);
+void   pvo_rk3_tables(double *out);
 
 #ifdef __cplusplus
 }

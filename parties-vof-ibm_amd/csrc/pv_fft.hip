@@ -292,3 +292,19 @@ extern "C" void pvk_fft_solve_dot(const pvk_grid *g, const pvk_fft *F, const pvk
 }
 
 extern "C" int pvk_fft_rows_blocks(const pvk_grid *g, const pvk_fft *F) { return (g->NYM + F->rx - 1) / F->rx; }
+
+// The three stages as separate launchers (the host brackets each with timers).
+extern "C" void pvk_fft_stage(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, int stage, const double *in,
+                              double *out, int mean_free, double *part, hipStream_t s)
+{
+    int nbr = (g->NYM + F->rx - 1) / F->rx, nbc = (g->NXM + F->ry - 1) / F->ry;
+    if (stage == 0)
+        hipLaunchKernelGGL(k_fft_rows_fwd, dim3(nbr), dim3(FT), lds_bytes(F->rx, g->NXM), s, *g, *F, sc, in, F->spec,
+                           1.0 / (double)g->NXM);
+    else if (stage == 1)
+        hipLaunchKernelGGL(k_dct_cols, dim3(nbc), dim3(FTC), lds_bytes(F->ry, g->NYM), s, *g, *F, sc, F->spec,
+                           mean_free);
+    else
+        hipLaunchKernelGGL(k_fft_rows_inv, dim3(nbr), dim3(FT), lds_bytes(F->rx, g->NXM), s, *g, *F, sc,
+                           (const double *)F->spec, out, in, part, nbr);
+}

@@ -174,6 +174,9 @@ void pvk_fft_solve_sc(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, c
 void pvk_fft_solve_dot(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, const double *in,
                        double *out, int mean_free, double *part, hipStream_t s);
 int  pvk_fft_rows_blocks(const pvk_grid *g, const pvk_fft *F);
+/* stage 0: rows forward, 1: columns, 2: rows inverse (+ <in, out> partials) */
+void pvk_fft_stage(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, int stage, const double *in,
+                   double *out, int mean_free, double *part, hipStream_t s);
 
 /* ---- time step ---- */
 void pvk_cfl(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s);

@@ -252,6 +252,7 @@ int pv_destroy(pv_bag *b)
     if (!b) return PV_OK;
     hipSetDevice(b->device);
     hipStreamSynchronize(b->stream);
+    if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
     hipFree(b->slabmem);
     hipFree(b->metricmem);
@@ -322,6 +323,10 @@ int pv_read_scalars(pv_bag *b)
 {
     HIPCHK(hipMemcpyAsync(b->h_sc, b->d_sc, sizeof(pvk_scal), hipMemcpyDeviceToHost, b->stream));
     HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->timers_on) {
+        int rc = pv_timers_flush(b);
+        if (rc) return rc;
+    }
     b->dt = b->h_sc->dt; b->dt_old = b->h_sc->dt_old; b->time = b->h_sc->time;
     return PV_OK;
 }
@@ -435,4 +440,96 @@ void pv_fft_destroy(pv_bag *b)
     if (b->fftmem) hipFree(b->fftmem);
     b->fftmem = NULL;
     b->fft.ok = 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* kernel timers                                                       */
+/* ------------------------------------------------------------------ */
+void pv_tbegin(pv_bag *b, int t)
+{
+    if (!b->timers_on) return;
+    if (b->tpending[t] >= PV_TPOOL) {
+        hipStreamSynchronize(b->stream);
+        pv_timers_flush(b);
+    }
+    hipEventRecord(b->tev[t][0][b->tpending[t]], b->stream);
+}
+
+void pv_tend(pv_bag *b, int t)
+{
+    if (!b->timers_on) return;
+    hipEventRecord(b->tev[t][1][b->tpending[t]], b->stream);
+    b->tpending[t]++;
+}
+
+/* caller has synchronised the stream */
+int pv_timers_flush(pv_bag *b)
+{
+    int t, k;
+    for (t = 0; t < PV_NTIMERS; t++) {
+        for (k = 0; k < b->tpending[t]; k++) {
+            float ms = 0.0f;
+            HIPCHK(hipEventElapsedTime(&ms, b->tev[t][0][k], b->tev[t][1][k]));
+            b->tms[t] += ms;
+            b->tcount[t]++;
+        }
+        b->tpending[t] = 0;
+    }
+    return PV_OK;
+}
+
+int pv_timers_enable(pv_bag *b, int on)
+{
+    int t, k;
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (on && !b->timers_on) {
+        for (t = 0; t < PV_NTIMERS; t++)
+            for (k = 0; k < PV_TPOOL; k++) {
+                HIPCHK(hipEventCreate(&b->tev[t][0][k]));
+                HIPCHK(hipEventCreate(&b->tev[t][1][k]));
+            }
+    } else if (!on && b->timers_on) {
+        pv_timers_flush(b);
+        for (t = 0; t < PV_NTIMERS; t++)
+            for (k = 0; k < PV_TPOOL; k++) {
+                hipEventDestroy(b->tev[t][0][k]);
+                hipEventDestroy(b->tev[t][1][k]);
+            }
+    }
+    b->timers_on = on ? 1 : 0;
+    return PV_OK;
+}
+
+int pv_timers_reset(pv_bag *b)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->timers_on) pv_timers_flush(b);
+    memset(b->tms, 0, sizeof b->tms);
+    memset(b->tcount, 0, sizeof b->tcount);
+    return PV_OK;
+}
+
+int pv_timers_read(pv_bag *b, double *ms, long long *count)
+{
+    int t;
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->timers_on) pv_timers_flush(b);
+    for (t = 0; t < PV_NTIMERS; t++) {
+        if (ms) ms[t] = b->tms[t];
+        if (count) count[t] = b->tcount[t];
+    }
+    return PV_OK;
+}
+
+int pv_synchronize(pv_bag *b)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return PV_OK;
 }

@@ -15,6 +15,7 @@ enum pv_internal_field {
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
+#define PV_TPOOL 64                /* event pairs per timer between syncs */
 
 typedef struct pv_metrics {
     int nx, ny;
@@ -45,6 +46,12 @@ struct pv_bag {
     int stage, ntime, time_nonzero;
     pv_stats st;
     int ch_chunk, mom_chunk[2], p_chunk;
+    /* kernel timers */
+    int timers_on;
+    hipEvent_t tev[PV_NTIMERS][2][PV_TPOOL];
+    int tpending[PV_NTIMERS];
+    double tms[PV_NTIMERS];
+    long long tcount[PV_NTIMERS];
     /* FFT-x / DCT-y preconditioner */
     pvk_fft fft;
     double *fftmem;
@@ -58,6 +65,9 @@ int  pv_sync_scalars_to_device(pv_bag *b);
 int  pv_read_scalars(pv_bag *b);
 double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
+void pv_tbegin(pv_bag *b, int t);
+void pv_tend(pv_bag *b, int t);
+int  pv_timers_flush(pv_bag *b);
 void pv_fft_destroy(pv_bag *b);
 
 #endif

@@ -48,9 +48,13 @@ static int ch_cg_loop(pv_bag *b, int stage)
     for (;;) {
         int k;
         for (k = 0; k < chunk; k++) {
+            pv_tbegin(b, PV_T_CH_APPLY);
             pvk_ch_apply(G(b), &b->ph, stage, b->d_sc, p, Ap, &b->rd, S(b));
+            pv_tend(b, PV_T_CH_APPLY);
             pvk_ch_alpha(G(b), b->d_sc, &b->rd, S(b));
+            pv_tbegin(b, PV_T_CH_UPDATE);
             pvk_ch_update(G(b), b->d_sc, x, p, r, Ap, &b->rd, S(b));
+            pv_tend(b, PV_T_CH_UPDATE);
             pvk_ch_beta(G(b), b->d_sc, tol, &b->rd, S(b));
             pvk_ch_pupdate(G(b), b->d_sc, r, p, S(b));
             pvk_bc_ch(G(b), p, S(b));
@@ -75,8 +79,10 @@ static int ch_cg_loop(pv_bag *b, int stage)
 static int ch_step(pv_bag *b)
 {
     int st = b->stage;
+    pv_tbegin(b, PV_T_CH_RHS);
     pvk_ch_rhs_init(G(b), &b->ph, st, b->d_sc, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
                     FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), &b->rd, S(b));
+    pv_tend(b, PV_T_CH_RHS);
     pvk_ch_finalize_init(G(b), b->d_sc, b->ph.ch_iter_max, b->ph.ch_tol, &b->rd, S(b));
     LAUNCHCHK();
     RC(ch_cg_loop(b, st));
@@ -243,12 +249,14 @@ int pv_Velocity_cell_center(pv_bag *b)
 static int mom_terms(pv_bag *b, int comp, int mode, int add_force)
 {
     int u = (comp == 'u');
+    pv_tbegin(b, PV_T_MOM_TERMS);
     pvk_mom_terms(G(b), &b->ph, b->stage, b->d_sc, comp, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_MU),
                   FD(b, PV_F_RHO), FD(b, PV_F_P), FD(b, u ? PV_F_U_EXPL_OLD : PV_F_V_EXPL_OLD),
                   FD(b, u ? PV_F_U_VISC_OLD : PV_F_V_VISC_OLD), FD(b, u ? PV_F_FSX_OLD : PV_F_FSY_OLD),
                   FD(b, u ? PV_F_U_RHS : PV_F_V_RHS), FD(b, u ? PV_F_U_EXPL : PV_F_V_EXPL),
                   FD(b, u ? PV_F_U_IMPL : PV_F_V_IMPL), FD(b, u ? PV_F_U_VISC : PV_F_V_VISC), mode, add_force,
                   S(b));
+    pv_tend(b, PV_T_MOM_TERMS);
     LAUNCHCHK();
     return PV_OK;
 }
@@ -292,10 +300,14 @@ static int mom_solve(pv_bag *b, int comp)
         int k;
         for (k = 0; k < chunk; k++) {
             pvk_ghost_multi(G(b), d, NULL, NULL, NULL, 1, kind, S(b));
+            pv_tbegin(b, PV_T_MOM_APPLY);
             pvk_mom_cg_apply(G(b), &b->ph, st, b->d_sc, comp, d, FD(b, PV_F_RHO), FD(b, PV_F_MU), Ad, &b->rd,
                              S(b));
+            pv_tend(b, PV_T_MOM_APPLY);
             pvk_mom_scalar(G(b), b->d_sc, comp, 1, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
+            pv_tbegin(b, PV_T_MOM_UPDATE);
             pvk_mom_cg_update(G(b), b->d_sc, comp, x, d, r, Ad, dg, &b->rd, S(b));
+            pv_tend(b, PV_T_MOM_UPDATE);
             pvk_mom_scalar(G(b), b->d_sc, comp, 2, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
             pvk_mom_cg_dupdate(G(b), b->d_sc, comp, r, dg, d, S(b));
         }
@@ -371,11 +383,17 @@ static int p_solve_jacobi(pv_bag *b)
         int k;
         for (k = 0; k < chunk; k++) {
             pvk_p_bc(G(b), d, S(b));
+            pv_tbegin(b, PV_T_P_APPLY);
             pvk_p_cg_apply(G(b), b->d_sc, FD(b, PV_F_RHO), d, Ad, &b->rd, S(b));
+            pv_tend(b, PV_T_P_APPLY);
             pvk_p_scalar2(G(b), b->d_sc, 1, 0, maxit, tol, &b->rd, 0, S(b));
+            pv_tbegin(b, PV_T_P_UPDATE);
             pvk_p_cg_update(G(b), b->d_sc, phi, d, r, Ad, M, 1, &b->rd, S(b));
+            pv_tend(b, PV_T_P_UPDATE);
             pvk_p_scalar2(G(b), b->d_sc, 2, 0, maxit, tol, &b->rd, 0, S(b));
+            pv_tbegin(b, PV_T_P_DUPDATE);
             pvk_p_cg_dupdate(G(b), b->d_sc, r, M, 1, d, S(b));
+            pv_tend(b, PV_T_P_DUPDATE);
         }
         LAUNCHCHK();
         total += chunk;
@@ -386,6 +404,20 @@ static int p_solve_jacobi(pv_bag *b)
     }
     b->p_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 4;
     return PV_OK;
+}
+
+/* z = L0^+ r with the <r, z> partials, each transform stage timed. */
+static void fft_precond(pv_bag *b, const double *r, double *z)
+{
+    pv_tbegin(b, PV_T_FFT_ROWS_FWD);
+    pvk_fft_stage(G(b), &b->fft, b->d_sc, 0, r, z, 1, b->rd.part, S(b));
+    pv_tend(b, PV_T_FFT_ROWS_FWD);
+    pv_tbegin(b, PV_T_FFT_COLS);
+    pvk_fft_stage(G(b), &b->fft, b->d_sc, 1, r, z, 1, b->rd.part, S(b));
+    pv_tend(b, PV_T_FFT_COLS);
+    pv_tbegin(b, PV_T_FFT_ROWS_INV);
+    pvk_fft_stage(G(b), &b->fft, b->d_sc, 2, r, z, 1, b->rd.part, S(b));
+    pv_tend(b, PV_T_FFT_ROWS_INV);
 }
 
 /* PCG with the FFT-x / DCT-y preconditioner; relative-residual stop. */
@@ -401,20 +433,28 @@ static int p_solve_fft_pcg(pv_bag *b)
     pvk_p_mean_finalize(G(b), b->d_sc, &b->rd, S(b));
     pvk_p_cg_init2(G(b), rhs, NULL, phi, r, d, 0, 1, b->d_sc, &b->rd, S(b));
     pvk_p_scalar2(G(b), b->d_sc, 0, 1, maxit, tol, &b->rd, 0, S(b));
-    pvk_fft_solve_dot(G(b), &b->fft, b->d_sc, r, z, 1, b->rd.part, S(b));
+    fft_precond(b, r, z);
     pvk_p_scalar2(G(b), b->d_sc, 4, 1, maxit, tol, &b->rd, nbr, S(b));
+    pv_tbegin(b, PV_T_P_DUPDATE);
     pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
+    pv_tend(b, PV_T_P_DUPDATE);
     for (;;) {
         int k;
         for (k = 0; k < chunk; k++) {
             pvk_p_bc(G(b), d, S(b));
+            pv_tbegin(b, PV_T_P_APPLY);
             pvk_p_cg_apply(G(b), b->d_sc, FD(b, PV_F_RHO), d, Ad, &b->rd, S(b));
+            pv_tend(b, PV_T_P_APPLY);
             pvk_p_scalar2(G(b), b->d_sc, 1, 1, maxit, tol, &b->rd, 0, S(b));
+            pv_tbegin(b, PV_T_P_UPDATE);
             pvk_p_cg_update(G(b), b->d_sc, phi, d, r, Ad, NULL, 0, &b->rd, S(b));
+            pv_tend(b, PV_T_P_UPDATE);
             pvk_p_scalar2(G(b), b->d_sc, 2, 1, maxit, tol, &b->rd, 0, S(b));
-            pvk_fft_solve_dot(G(b), &b->fft, b->d_sc, r, z, 1, b->rd.part, S(b));
+            fft_precond(b, r, z);
             pvk_p_scalar2(G(b), b->d_sc, 3, 1, maxit, tol, &b->rd, nbr, S(b));
+            pv_tbegin(b, PV_T_P_DUPDATE);
             pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
+            pv_tend(b, PV_T_P_DUPDATE);
         }
         LAUNCHCHK();
         total += chunk;

@@ -142,6 +142,10 @@ def load_lib(path=LIB_PATH):
     lib.pv_Velocity_solve_cg.argtypes = [vp, C.c_int, C.POINTER(C.c_int)]
     lib.pv_Pressure_solve_cg.argtypes = [vp, C.POINTER(C.c_int)]
     lib.pv_Pressure_compute_velocity_divergence.argtypes = [vp, dp]
+    lib.pv_synchronize.argtypes = [vp]
+    lib.pv_timers_enable.argtypes = [vp, C.c_int]
+    lib.pv_timers_reset.argtypes = [vp]
+    lib.pv_timers_read.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
     for name in ("pv_ic_rising_bubble", "pv_ic_rayleigh_taylor"):
         getattr(lib, name).argtypes = [C.POINTER(Params), dp]
     lib.pv_ic_multimode_rt.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_double, C.c_double, dp]
