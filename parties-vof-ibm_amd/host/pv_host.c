@@ -381,8 +381,9 @@ int pv_fft_create(pv_bag *b)
     if (lx < 3 || ly < 3) return PV_OK;          /* FFT-PCG unavailable for this size */
     b->fft.logmx = lx - 1;
     b->fft.logmy = ly - 1;
-    b->fft.rx = 8192 / nx; if (b->fft.rx < 1) b->fft.rx = 1; if (b->fft.rx > 16) b->fft.rx = 16;
-    b->fft.ry = 8192 / ny; if (b->fft.ry < 1) b->fft.ry = 1; if (b->fft.ry > 16) b->fft.ry = 16;
+    /* transforms per block: R * N/2 complex doubles of LDS, kept <= 32 KiB */
+    b->fft.rx = 4096 / nx; if (b->fft.rx < 1) b->fft.rx = 1; if (b->fft.rx > 16) b->fft.rx = 16;
+    b->fft.ry = 4096 / ny; if (b->fft.ry < 1) b->fft.ry = 1; if (b->fft.ry > 16) b->fft.ry = 16;
     while (b->fft.ry > 1 && b->fft.ry * (ny / 4 + 1) > 8 * 512) b->fft.ry /= 2;
     if (b->fft.ry * (ny / 4 + 1) > 8 * 512) return PV_OK;
     n_twx = 2 * (size_t)(nx / 4 > 0 ? nx / 4 : 1); n_twy = 2 * (size_t)(ny / 4 > 0 ? ny / 4 : 1);

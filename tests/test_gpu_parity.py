@@ -234,3 +234,22 @@ def test_trajectory_10_steps(gpu, lib, solver):
     assert abs(so["time"] - ss["time"]) <= 1e-15 * so["time"]
     for fld, r in (("C_L", "cell"), ("U", "u"), ("V", "v"), ("P", "cell")):
         assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-11, fld
+
+
+@pytest.mark.parametrize("n", [64, 512, 1024])
+def test_fft_solve_sizes(gpu, lib, n):
+    """Standalone FFT-x / DCT-y solve against the oracle's FFT + Thomas solve."""
+    p = pvof.default_params(NXM=n, NYM=n // 2, ymax=0.5)
+    o = oracle_lib.Oracle(p)
+    s = pvof.Solver(p, lib)
+    R = regions(p)
+    rng = np.random.default_rng(n)
+    rhs = np.zeros(o.shape)
+    rhs[R["cell"]] = rng.standard_normal((p.NYM, p.NXM))
+    rhs[R["cell"]] -= rhs[R["cell"]].mean()
+    for x in (o, s):
+        x.set("SCRATCH0", rhs)
+        x.call("Pressure_solve_fft", pvof.F["SCRATCH0"], pvof.F["SCRATCH1"])
+    got, ref = s.get("SCRATCH1")[R["cell"]], o.get("SCRATCH1")[R["cell"]]
+    assert np.isfinite(got).all()
+    assert relerr(got, ref) < 1e-10
