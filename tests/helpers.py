@@ -22,8 +22,8 @@ def relerr(a, b):
     return float(np.max(np.abs(a - b)) / scale)
 
 
-def oracle_state(p, F, nsteps):
-    o = oracle_lib.Oracle(p)
+def oracle_state(p, F, nsteps, lib=None):
+    o = oracle_lib.Oracle(p, lib)
     o.init_from_F(F)
     if nsteps:
         rc = o.call("Temporal_int_rk3", nsteps)

@@ -21,10 +21,13 @@ def build_oracle():
     subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle")], check=True)
 
 
-def load_oracle():
-    if not os.path.exists(ORACLE_SO):
-        build_oracle()
-    lib = C.CDLL(ORACLE_SO)
+def load_oracle(path=None):
+    """Load the oracle library (default: the one built from oracle/pvo.c)."""
+    if path is None:
+        path = ORACLE_SO
+        if not os.path.exists(path):
+            build_oracle()
+    lib = C.CDLL(path)
     vp, dp = C.c_void_p, C.POINTER(C.c_double)
     lib.pvo_create.restype = vp
     lib.pvo_create.argtypes = [C.POINTER(pvof.Params)]

@@ -39,6 +39,18 @@ def test_no_gpu_create_fails_cleanly():
     assert lib.pv_last_error()
 
 
+@pytest.mark.parametrize("kw", [dict(precision=pvof.PV_PREC_MIXED), dict(pressure_solver=7),
+                                dict(NXM=4)])
+def test_create_rejects_unsupported_params(kw):
+    """Unsupported settings fail with PV_EINVAL before any device is touched."""
+    lib = pvof.load_lib()
+    h = C.c_void_p()
+    p = pvof.default_params(**{"NXM": 16, "NYM": 16, **kw})
+    assert lib.pv_create(C.byref(p), 0, C.byref(h)) == -1
+    assert not h.value
+    assert lib.pv_last_error()
+
+
 def test_params_default_matches_reference_defaults():
     lib = pvof.load_lib()
     p = pvof.Params()

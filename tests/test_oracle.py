@@ -181,6 +181,35 @@ def test_weno_mirror_symmetry(olib):
     np.testing.assert_allclose(r2, r1[:, ::-1], atol=1e-12 * np.abs(r1).max())
 
 
+def weno_smooth_error(olib, n, amp=0.25, phase=0.3):
+    """Max error of the WENO5 flux divergence for u = 1 on cell averages of
+    0.5 + amp sin(2 pi x + phase); exact: -(c(x_{i+1/2}) - c(x_{i-1/2})) / dx."""
+    o = make(olib, NXM=n, NYM=8, ymax=8.0 / n)
+    p = o.params
+    R = regions(p)
+    dx = 1.0 / n
+    xe = np.arange(n + 1) * dx
+    prim = 0.5 * xe - amp / (2 * np.pi) * np.cos(2 * np.pi * xe + phase)
+    cface = 0.5 + amp * np.sin(2 * np.pi * xe + phase)
+    C_ = np.zeros(o.shape)
+    C_[R["cell"]] = ((prim[1:] - prim[:-1]) / dx)[None, :]
+    o.set("C_L", C_)
+    o.set("U", np.ones(o.shape))
+    o.set("V", np.zeros(o.shape))
+    o.call("VOF_DIFFUSE_advect_WENO5", pvof.F["CH_AUX2"])
+    rhs = o.get("CH_AUX2")[R["cell"]]
+    return float(np.abs(rhs + (cface[1:] - cface[:-1])[None, :] / dx).max())
+
+
+def test_weno_fifth_order_on_smooth_data(olib):
+    """Only the linear weights 0.1/0.6/0.3 make the three 3-point candidates a
+    5th-order scheme on smooth data; any other convex weighting (or a wrong
+    smoothness indicator) leaves 3rd order, which the quadratic test cannot see."""
+    e = [weno_smooth_error(olib, n) for n in (64, 128, 256)]
+    orders = [np.log2(e[k] / e[k + 1]) for k in range(2)]
+    assert min(orders) > 4.6, (e, orders)
+
+
 # ---------------------------------------------------------------- biharmonic
 def spectral_biharmonic_solve(b, a_dt, coef, dx, dy):
     """Exact inverse of a_dt I + coef L^2 (periodic x, Neumann y) by FFT x DCT."""
@@ -318,7 +347,7 @@ def test_fft_solve_residual_and_pin(olib):
 def test_momentum_operator_spd(olib):
     p = pvof.config1_params(NXM=32, NYM=64)
     F = pvof.ic(pvof.load_lib(), "rising_bubble", p)
-    o = oracle_state(p, F, 2)
+    o = oracle_state(p, F, 2, olib)
     R = regions(o.params)
     rng = np.random.default_rng(29)
     for comp, fld, reg in (("u", "U_D", "u"), ("v", "V_D", "v")):
@@ -340,7 +369,7 @@ def test_momentum_operator_spd(olib):
 def test_dt_limiter_truth_table(olib):
     p = pvof.config1_params(NXM=32, NYM=64, max_dt=1e-3)
     F = pvof.ic(pvof.load_lib(), "rising_bubble", p)
-    o = oracle_state(p, F, 2)
+    o = oracle_state(p, F, 2, olib)
     o.call("Velocity_cell_center")
     cfl_dt = o.lib.pvo_Dtime_cfl
     o.call("set_time_state", 0.1, 1.0, 1.0, 0)            # dt_cfl << 0.95 dt_old: floor, max_dt skipped
