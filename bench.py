@@ -33,14 +33,14 @@ import pvof  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDATE", "P_DUPDATE",
-          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS"]
+          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE"]
 # algorithmic HBM bytes per cell and launch (each distinct array read or
 # written once; stencil neighbours come from cache)
 BYTES_PER_CELL = {
     "CH_APPLY": 16, "CH_UPDATE": 48, "MOM_APPLY": 32, "MOM_UPDATE": 56,
     "P_APPLY": 24, "P_UPDATE": 48, "P_DUPDATE": 24,
     "FFT_ROWS_FWD": 16, "FFT_COLS": 16, "FFT_ROWS_INV": 24,
-    "CH_RHS": 56, "MOM_TERMS": 88,
+    "CH_RHS": 56, "MOM_TERMS": 88, "P_TRANSPOSE": 16,
 }
 SEED = 20261202
 

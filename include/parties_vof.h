@@ -212,6 +212,7 @@ enum pv_timer {
     PV_T_FFT_ROWS_INV,     /* inverse real FFT + <r,z>           */
     PV_T_CH_RHS,           /* fused explicit CH right-hand side  */
     PV_T_MOM_TERMS,        /* fused momentum right-hand side     */
+    PV_T_P_TRANSPOSE,      /* spectrum transposes of the FFT-PCG */
     PV_NTIMERS
 };
 int pv_timers_enable(pv_bag *bag, int on);

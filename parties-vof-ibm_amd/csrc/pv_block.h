@@ -1,0 +1,55 @@
+// pv_block.h -- block-level helpers of the fused solver kernels (pv_pcg.hip,
+// pv_cg.hip): 256-thread blocks that each own a band of rows.
+//
+// Scalars of a Krylov iteration (alpha, beta, stop tests) are never written
+// by one block and read by another inside one kernel.  Each block stores one
+// partial sum per reduction; the next kernel's blocks all re-add those few
+// hundred partials in the same order, so every block holds bit-identical
+// scalars and no one-block scalar kernel sits between the sweeps.
+#ifndef PV_BLOCK_H
+#define PV_BLOCK_H
+#include <hip/hip_runtime.h>
+
+#define NTH 256
+
+// Sum of n partials, same order in every block; all threads get the result.
+__device__ __forceinline__ double block_sum(const double *part, int n, double *sh)
+{
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int b = t; b < n; b += NTH) s += part[b];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+// Block total of v (wave shuffles, then the 4 wave sums in order) -> part[slot].
+__device__ __forceinline__ void block_store_partial(double v, double *part, int slot, double *sh)
+{
+    const int t = threadIdx.x;
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    if ((t & 63) == 0) sh[t >> 6] = v;
+    __syncthreads();
+    if (t == 0) {
+        double s = sh[0];
+        for (int w = 1; w < NTH / 64; w++) s += sh[w];
+        part[slot] = s;
+    }
+    __syncthreads();
+}
+
+// Workgroups are dealt to the 8 XCDs round robin (block b runs on XCD b % 8).
+// Give each XCD a contiguous run of bands so the rows a band's stencil shares
+// with its neighbours stay in that XCD's L2.  Identity unless nblocks % 8 == 0.
+__device__ __forceinline__ int xcd_band(int b, int nblocks)
+{
+    if (nblocks & 7) return b;
+    return (b & 7) * (nblocks >> 3) + (b >> 3);
+}
+#endif

@@ -70,6 +70,16 @@ typedef struct pvk_fft {
     double *spec;             /* NXM * NYM spectral scratch, row-major */
 } pvk_fft;
 
+/* Work set of the fused FFT-preconditioned PCG (csrc/pv_pcg.hip). */
+typedef struct pvk_pcg {
+    double *phi, *r, *z, *d[2], *Ad;  /* fields, cell (0,0); d double-buffered */
+    const double *rhs, *rho;
+    double *specT;                    /* NXM * NYM transposed spectrum */
+    double *dad, *rr, *rz[2], *mean;  /* per-block partial sums */
+    int ndad, nrr, nrz, nmean;        /* partial counts */
+    int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
+} pvk_pcg;
+
 /* ---- ghost fills (kind codes in pv_kernels.hip k_ghost) ---- */
 void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s);          /* VOF_DIFFUSE_set_boundary_values */
 void pvk_halo_cell(const pvk_grid *g, double *f, hipStream_t s);
@@ -177,6 +187,13 @@ int  pvk_fft_rows_blocks(const pvk_grid *g, const pvk_fft *F);
 /* stage 0: rows forward, 1: columns, 2: rows inverse (+ <in, out> partials) */
 void pvk_fft_stage(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, int stage, const double *in,
                    double *out, int mean_free, double *part, hipStream_t s);
+
+/* fused FFT-PCG (csrc/pv_pcg.hip): start (mean removal, r0, forward rows)
+ * and the six per-iteration stages; n = -1 preconditions the start residual */
+void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, double tol,
+                   hipStream_t s);
+void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
+                   double tol, int maxit, hipStream_t s);
 
 /* ---- time step ---- */
 void pvk_cfl(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s);

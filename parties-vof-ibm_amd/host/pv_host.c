@@ -163,6 +163,10 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     *out = NULL;
     if (!prm || prm->NXM < 8 || prm->NYM < 8)
         return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
+    if (prm->precision != PV_PREC_FP64)
+        return pv_set_error(PV_EINVAL, "only precision = PV_PREC_FP64 is implemented");
+    if (prm->pressure_solver != PV_PSOLVE_JACOBI_PCG && prm->pressure_solver != PV_PSOLVE_FFT_PCG)
+        return pv_set_error(PV_EINVAL, "unknown pressure_solver %d", prm->pressure_solver);
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
         return pv_set_error(PV_ENODEV, "no HIP device %d (count %d)", device, ndev);
     HIPCHK(hipSetDevice(device));
@@ -378,7 +382,9 @@ int pv_fft_create(pv_bag *b)
     double idx2, idy2;
     memset(&b->fft, 0, sizeof b->fft);
     b->fftmem = NULL;
-    if (lx < 3 || ly < 3) return PV_OK;          /* FFT-PCG unavailable for this size */
+    /* FFT-PCG unavailable for this size: one row of N/2 complex doubles must
+     * fit the 32 KiB per-block LDS budget of the row transforms */
+    if (lx < 3 || ly < 3 || nx > 4096 || ny > 4096) return PV_OK;
     b->fft.logmx = lx - 1;
     b->fft.logmy = ly - 1;
     /* transforms per block: R * N/2 complex doubles of LDS, kept <= 32 KiB */
@@ -424,6 +430,35 @@ int pv_fft_create(pv_bag *b)
     free(h);
     b->fftmem = d;
     q = d;
+    /* fused PCG work set: spectrum transpose + five partial buffers */
+    {
+        int band = (ny >= 512) ? 2 : 1, rrow, rdct, nband, nrow, np;
+        double *pm;
+        rrow = 4096 / nx; if (rrow > 8) rrow = 8; if (rrow < 1) rrow = 1;
+        if (nx >= 1024 && rrow > 2) rrow = 2;
+        rdct = 4096 / ny; if (rdct > 8) rdct = 8; if (rdct < 1) rdct = 1;
+        while (rdct > 1 && rdct * (ny / 4 + 1) > 8 * 256) rdct /= 2;
+        if (ny >= 1024 && rdct > 2) rdct = 2;
+        nband = (ny + band - 1) / band;
+        nrow = (ny + rrow - 1) / rrow;
+        np = (nband > nrow) ? nband : nrow;
+        if (hipMalloc((void **)&pm, sizeof(double) * ((size_t)nx * ny + 5 * (size_t)np)) != hipSuccess)
+            return pv_set_error(PV_ENOMEM, "PCG work allocation failed");
+        b->pcgmem = pm;
+        memset(&b->pcg, 0, sizeof b->pcg);
+        b->pcg.specT = pm;
+        b->pcg.dad = pm + (size_t)nx * ny;
+        b->pcg.rr = b->pcg.dad + np;
+        b->pcg.rz[0] = b->pcg.rr + np;
+        b->pcg.rz[1] = b->pcg.rz[0] + np;
+        b->pcg.mean = b->pcg.rz[1] + np;
+        b->pcg.ndad = nband; b->pcg.nmean = nband;
+        b->pcg.nrr = nrow; b->pcg.nrz = nrow;
+        b->pcg.band = band; b->pcg.rrow = rrow; b->pcg.rdct = rdct;
+        b->pcg.phi = b->f[PV_F_PHI]; b->pcg.r = b->f[PV_F_P_RES]; b->pcg.z = b->f[PV_FI_Z];
+        b->pcg.d[0] = b->f[PV_F_P_D]; b->pcg.d[1] = b->f[PV_FI_D2]; b->pcg.Ad = b->f[PV_F_P_AD];
+        b->pcg.rhs = b->f[PV_F_P_RHS]; b->pcg.rho = b->f[PV_F_RHO];
+    }
     b->fft.twx = q; q += n_twx;
     b->fft.twy = q; q += n_twy;
     b->fft.rtx = q; q += n_rtx;
@@ -438,6 +473,8 @@ int pv_fft_create(pv_bag *b)
 
 void pv_fft_destroy(pv_bag *b)
 {
+    if (b->pcgmem) hipFree(b->pcgmem);
+    b->pcgmem = NULL;
     if (b->fftmem) hipFree(b->fftmem);
     b->fftmem = NULL;
     b->fft.ok = 0;

@@ -12,6 +12,7 @@ enum pv_internal_field {
     PV_FI_DG_V,                /* same, v */
     PV_FI_TMP,                 /* scratch */
     PV_FI_Z,                   /* preconditioned residual of the pressure PCG */
+    PV_FI_D2,                  /* second direction buffer of the fused pressure PCG */
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
@@ -55,6 +56,8 @@ struct pv_bag {
     /* FFT-x / DCT-y preconditioner */
     pvk_fft fft;
     double *fftmem;
+    pvk_pcg pcg;
+    double *pcgmem;
 };
 
 int  pv_set_error(int code, const char *fmt, ...);

@@ -407,60 +407,38 @@ static int p_solve_jacobi(pv_bag *b)
 }
 
 /* z = L0^+ r with the <r, z> partials, each transform stage timed. */
-static void fft_precond(pv_bag *b, const double *r, double *z)
+/* PCG with the FFT-x / DCT-y preconditioner; relative-residual stop.
+ * Fused form (csrc/pv_pcg.hip): six launches per iteration, all scalars
+ * formed redundantly per block from the previous kernel's partials. */
+static const int pcg_timer[6] = { PV_T_P_APPLY, PV_T_P_UPDATE, PV_T_P_TRANSPOSE, PV_T_FFT_COLS,
+                                  PV_T_P_TRANSPOSE, PV_T_FFT_ROWS_INV };
+
+static void pcg_iteration(pv_bag *b, int n, double tol, int maxit)
 {
-    pv_tbegin(b, PV_T_FFT_ROWS_FWD);
-    pvk_fft_stage(G(b), &b->fft, b->d_sc, 0, r, z, 1, b->rd.part, S(b));
-    pv_tend(b, PV_T_FFT_ROWS_FWD);
-    pv_tbegin(b, PV_T_FFT_COLS);
-    pvk_fft_stage(G(b), &b->fft, b->d_sc, 1, r, z, 1, b->rd.part, S(b));
-    pv_tend(b, PV_T_FFT_COLS);
-    pv_tbegin(b, PV_T_FFT_ROWS_INV);
-    pvk_fft_stage(G(b), &b->fft, b->d_sc, 2, r, z, 1, b->rd.part, S(b));
-    pv_tend(b, PV_T_FFT_ROWS_INV);
+    int st;
+    for (st = (n < 0) ? 2 : 0; st < 6; st++) {
+        pv_tbegin(b, pcg_timer[st]);
+        pvk_pcg_stage(G(b), &b->fft, b->d_sc, &b->pcg, st, n, tol, maxit, S(b));
+        pv_tend(b, pcg_timer[st]);
+    }
 }
 
-/* PCG with the FFT-x / DCT-y preconditioner; relative-residual stop. */
 static int p_solve_fft_pcg(pv_bag *b)
 {
-    double *phi = FD(b, PV_F_PHI), *rhs = FD(b, PV_F_P_RHS), *r = FD(b, PV_F_P_RES), *d = FD(b, PV_F_P_D);
-    double *Ad = FD(b, PV_F_P_AD), *z = FD(b, PV_FI_Z);
-    int maxit = b->ph.p_max_iter, chunk = b->p_chunk, total = 0;
-    int nbr = pvk_fft_rows_blocks(G(b), &b->fft);
+    int maxit = b->ph.p_max_iter, chunk = b->p_chunk, n = 0;
     double tol = b->ph.p_rel_tol;
     if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
-    pvk_p_mean(G(b), rhs, &b->rd, S(b));
-    pvk_p_mean_finalize(G(b), b->d_sc, &b->rd, S(b));
-    pvk_p_cg_init2(G(b), rhs, NULL, phi, r, d, 0, 1, b->d_sc, &b->rd, S(b));
-    pvk_p_scalar2(G(b), b->d_sc, 0, 1, maxit, tol, &b->rd, 0, S(b));
-    fft_precond(b, r, z);
-    pvk_p_scalar2(G(b), b->d_sc, 4, 1, maxit, tol, &b->rd, nbr, S(b));
-    pv_tbegin(b, PV_T_P_DUPDATE);
-    pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
-    pv_tend(b, PV_T_P_DUPDATE);
+    pv_tbegin(b, PV_T_P_UPDATE);
+    pvk_pcg_start(G(b), &b->fft, b->d_sc, &b->pcg, tol, S(b));
+    pv_tend(b, PV_T_P_UPDATE);
+    pcg_iteration(b, -1, tol, maxit);
     for (;;) {
         int k;
-        for (k = 0; k < chunk; k++) {
-            pvk_p_bc(G(b), d, S(b));
-            pv_tbegin(b, PV_T_P_APPLY);
-            pvk_p_cg_apply(G(b), b->d_sc, FD(b, PV_F_RHO), d, Ad, &b->rd, S(b));
-            pv_tend(b, PV_T_P_APPLY);
-            pvk_p_scalar2(G(b), b->d_sc, 1, 1, maxit, tol, &b->rd, 0, S(b));
-            pv_tbegin(b, PV_T_P_UPDATE);
-            pvk_p_cg_update(G(b), b->d_sc, phi, d, r, Ad, NULL, 0, &b->rd, S(b));
-            pv_tend(b, PV_T_P_UPDATE);
-            pvk_p_scalar2(G(b), b->d_sc, 2, 1, maxit, tol, &b->rd, 0, S(b));
-            fft_precond(b, r, z);
-            pvk_p_scalar2(G(b), b->d_sc, 3, 1, maxit, tol, &b->rd, nbr, S(b));
-            pv_tbegin(b, PV_T_P_DUPDATE);
-            pvk_p_cg_dupdate(G(b), b->d_sc, z, NULL, 0, d, S(b));
-            pv_tend(b, PV_T_P_DUPDATE);
-        }
+        for (k = 0; k < chunk && n < maxit; k++, n++) pcg_iteration(b, n, tol, maxit);
         LAUNCHCHK();
-        total += chunk;
         RC(pv_read_scalars(b));
         if (b->h_sc->p_done) break;
-        if (total > maxit + 16) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
+        if (n >= maxit) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
         chunk = 4;
     }
     b->p_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 4;
