@@ -215,6 +215,9 @@ enum pv_timer {
     PV_T_P_TRANSPOSE,      /* spectrum transposes of the FFT-PCG */
     PV_NTIMERS
 };
+/* on = 0 disables; on = k >= 1 times every k-th launch of each timed kernel
+ * with a pair of HIP events on the library's stream (k > 1 keeps the event
+ * records from perturbing the timed work). */
 int pv_timers_enable(pv_bag *bag, int on);
 int pv_timers_reset(pv_bag *bag);
 /* total milliseconds and launch counts per timer (arrays of PV_NTIMERS) */

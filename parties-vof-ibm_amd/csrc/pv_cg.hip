@@ -1,0 +1,437 @@
+// pv_cg.hip -- fused Krylov loops of the Cahn-Hilliard biharmonic solve
+// (VOF_DIFFUSE.c:339-429) and of the momentum Jacobi-PCG (msolve_cg.c:525-722).
+//
+// Each iteration is two launches of 256-thread blocks, each block owning a
+// band of rows (pv_block.h):
+//   K1 apply : stop test and beta from the previous partials; the new search
+//              direction p = z + beta p_old evaluated on the fly at every
+//              stencil point through the ghost-fill index maps (so no ghost
+//              fill kernel runs); A p; partial <p, A p>.
+//   K2 update: alpha; x += alpha p; r -= alpha A p; partials of the next
+//              residual products.
+// A scalar that an earlier kernel already formed (||b||, the previous <r,z>)
+// is stored by block 0 in a parity-indexed slot that only later kernels read,
+// so a kernel re-adds at most two partial sets.  The arithmetic of every
+// update is the reference's, in its order; only reduction order differs.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include "pv_kernels.h"
+#include "pv_stencil.h"
+#include "pv_block.h"
+
+#define CPT 4   // cells per thread per pass: loads of CPT cells issue before any store
+
+// ---------------------------------------------------------------------------
+// Cahn-Hilliard
+// ---------------------------------------------------------------------------
+// CH ghost fill as an index map: x periodic cell image, y zero gradient
+// (VOF_DIFFUSE.c:571-683; all ghost layers equal the first interior value).
+__device__ __forceinline__ int chy(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
+
+// L(L f) of bilap_at with f given by an accessor that applies the CH map.
+template <class F>
+__device__ __forceinline__ double bilap_fn(const pvk_grid &g, const F &f, int i, int j)
+{
+    const int NXM = g.NXM, NYM = g.NYM;
+    auto lapf = [&](int a, int b) {
+        return lap_vals(g, a, b, f(a, b), f(a + 1, b), f(a - 1, b), f(a, b + 1), f(a, b - 1));
+    };
+    double L0 = lapf(i, j);
+    double LE = lapf((i + 1 == NXM) ? 0 : i + 1, j);
+    double LW = lapf((i == 0) ? NXM - 1 : i - 1, j);
+    double LN = (j + 1 == NYM) ? L0 : lapf(i, j + 1);
+    double LS = (j == 0) ? L0 : lapf(i, j - 1);
+    return lap_vals(g, i, j, L0, LE, LW, LN, LS);
+}
+
+// Explicit CH right-hand side and CG start (the body of k_ch_rhs_init) in
+// band form: rc -> rn, r = p = b - A C; partials ||b||^2, ||r||^2.
+__global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc,
+                                                     const double *__restrict__ C, const double *__restrict__ u,
+                                                     const double *__restrict__ v, double *__restrict__ rn,
+                                                     double *__restrict__ r, double *__restrict__ p, pvk_ccg w)
+{
+    __shared__ double sh[NTH];
+    const int P = g.P, NXM = g.NXM;
+    const int band = xcd_band(blockIdx.x, gridDim.x);
+    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+    double sbb = 0.0, srr = 0.0;
+    for (int c = threadIdx.x; c < ncell; c += NTH) {
+        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+        double rc = weno_rhs_at(g, ph, C, u, v, i, j);
+        rc += diffusion_at(g, ph, C, i, j);
+        double cc = AT(C, i, j);
+        double rv = (stage == 0) ? a_dt * cc + c_GAMB[0] * rc
+                                 : a_dt * cc + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
+        double L2 = bilap_at(g, C, i, j);
+        double b = rv - ph.bih * L2;
+        double Ap = a_dt * cc + ph.bih * L2;
+        double rr = b - Ap;
+        AT(rn, i, j) = rc;
+        AT(r, i, j) = rr;
+        AT(p, i, j) = rr;
+        sbb += ch_w(g, i, j, b, b);
+        srr += ch_w(g, i, j, rr, rr);
+    }
+    block_store_partial(sbb, w.bb, band, sh);
+    block_store_partial(srr, w.rr[0], band, sh);
+}
+
+// CG start from a given b (b already holds rhs - (Cn^2/Pe) L^2 C).
+__global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc,
+                                                   const double *__restrict__ C, const double *__restrict__ bv,
+                                                   double *__restrict__ r, double *__restrict__ p, pvk_ccg w)
+{
+    __shared__ double sh[NTH];
+    const int P = g.P, NXM = g.NXM;
+    const int band = xcd_band(blockIdx.x, gridDim.x);
+    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+    double sbb = 0.0, srr = 0.0;
+    for (int c = threadIdx.x; c < ncell; c += NTH) {
+        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+        double cc = AT(C, i, j);
+        double Ap = a_dt * cc + ph.bih * bilap_at(g, C, i, j);
+        double bb = AT(bv, i, j);
+        double rr = bb - Ap;
+        AT(r, i, j) = rr;
+        AT(p, i, j) = rr;
+        sbb += ch_w(g, i, j, bb, bb);
+        srr += ch_w(g, i, j, rr, rr);
+    }
+    block_store_partial(sbb, w.bb, band, sh);
+    block_store_partial(srr, w.rr[0], band, sh);
+}
+
+// K1: stop test on ||r_n|| (VOF_DIFFUSE.c:403-414), beta, p_n = r + beta p_{n-1}
+// (:416-422), Ap = a_dt p + (Cn^2/Pe) L^2 p (:318-337), partial <p, Ap>.
+__global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_ccg w,
+                                                  const double *__restrict__ r, const double *__restrict__ pold,
+                                                  double *__restrict__ pnew, double *__restrict__ Ap, int n,
+                                                  int maxit, double tol)
+{
+    __shared__ double sh[NTH];
+    if (sc->ch_done) return;
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    const int band = xcd_band(blockIdx.x, gridDim.x);
+    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    const double rr = 2.0 * block_sum(w.rr[n & 1], w.nb, sh);
+    double bnorm;
+    if (n == 0) {
+        bnorm = sqrt(2.0 * block_sum(w.bb, w.nb, sh));
+        if (bnorm < 1e-30) bnorm = 1.0;
+        if (lead) sc->ch_bnorm = bnorm;
+    } else {
+        bnorm = sc->ch_bnorm;
+    }
+    const double rel = sqrt(rr) / bnorm;
+    double beta = 0.0;
+    bool stop = (rel < tol) || (n == 0 && maxit <= 0);
+    if (!stop && n > 0) {
+        beta = rr / (sc->ch_rzb[(n + 1) & 1] + 1e-30);
+        stop = (n >= maxit);
+    }
+    if (lead) {
+        sc->ch_rzb[n & 1] = rr;
+        if (stop) {
+            sc->ch_rel = rel;
+            sc->ch_iter = n;
+            sc->ch_done = (rel < tol || maxit <= 0) ? 1 : 3;
+        }
+    }
+    if (stop) return;
+
+    const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
+    auto pv = [&](int a, int b) {
+        const int ia = wrapc(a, NXM), jb = chy(b, NYM);
+        return (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
+    };
+    const int j0 = band * w.band, j1 = min(NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    double acc = 0.0;
+    for (int base = 0; base < ncell; base += NTH * CPT) {
+        double pc[CPT], ap[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            const int c = base + k * NTH + threadIdx.x;
+            pc[k] = 0.0; ap[k] = 0.0;
+            if (c < ncell) {
+                const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+                pc[k] = pv(i, j);
+                ap[k] = a_dt * pc[k] + ph.bih * bilap_fn(g, pv, i, j);
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            const int c = base + k * NTH + threadIdx.x;
+            if (c < ncell) {
+                const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+                AT(pnew, i, j) = pc[k];
+                AT(Ap, i, j) = ap[k];
+                acc += ch_w(g, i, j, pc[k], ap[k]);
+            }
+        }
+    }
+    block_store_partial(acc, w.pap, band, sh);
+}
+
+// K2: alpha = <r,r> / <p,Ap> (VOF_DIFFUSE.c:384-390); x += alpha p;
+// r -= alpha Ap (:390-405); partial <r, r> of the next iteration.
+__global__ void __launch_bounds__(NTH) k_ccg_update(pvk_grid g, pvk_scal *sc, pvk_ccg w, double *__restrict__ x,
+                                                   const double *__restrict__ p, double *__restrict__ r,
+                                                   const double *__restrict__ Ap, int n)
+{
+    __shared__ double sh[NTH];
+    if (sc->ch_done) return;
+    const int P = g.P, NXM = g.NXM;
+    const int band = xcd_band(blockIdx.x, gridDim.x);
+    const double pAp = 2.0 * block_sum(w.pap, w.nb, sh);
+    if (fabs(pAp) < 1e-30) {
+        if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 2; sc->ch_iter = n; }
+        return;
+    }
+    const double alpha = sc->ch_rzb[n & 1] / pAp;
+    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < ncell; c += NTH) {
+        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+        AT(x, i, j) += alpha * AT(p, i, j);
+        double rr = AT(r, i, j) - alpha * AT(Ap, i, j);
+        AT(r, i, j) = rr;
+        acc += ch_w(g, i, j, rr, rr);
+    }
+    block_store_partial(acc, w.rr[(n + 1) & 1], band, sh);
+}
+
+extern "C" void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
+                             const double *C, const double *u, const double *v, double *rn, const double *b,
+                             double *r, double *p, hipStream_t s)
+{
+    if (b)
+        hipLaunchKernelGGL(k_ccg_init_b, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, b, r, p, *w);
+    else
+        hipLaunchKernelGGL(k_ccg_init_rhs, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p, *w);
+}
+
+extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
+                             int half, int n, double *x, double *r, double *Ap, hipStream_t s)
+{
+    if (half == 0)
+        hipLaunchKernelGGL(k_ccg_apply, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, (const double *)r,
+                           (const double *)w->p[(n + 1) & 1], w->p[n & 1], Ap, n, ph->ch_iter_max, ph->ch_tol);
+    else
+        hipLaunchKernelGGL(k_ccg_update, dim3(w->nb), dim3(NTH), 0, s, *g, sc, *w, x, (const double *)w->p[n & 1], r,
+                           (const double *)Ap, n);
+}
+
+// ---------------------------------------------------------------------------
+// Momentum (u and v solved in the same launches; blocks [0, nb[0]) take u)
+// ---------------------------------------------------------------------------
+// Unknown ranges (msolve_cg.c:525-722): u at i = 1..NXM, j = 0..NYM-1; v at
+// i = 0..NXM-1, j = 1..NYM-1.  Ghost maps of the CG-type boundary update
+// (Velocity.c:2427-2805): u is x periodic with column 0 = column NXM and
+// ghost rows -1 / NYM holding minus rows 0 / NYM-1; v is x periodic (cell
+// image) and its wall rows 0 and NYM are never written by the solve.
+struct mom_band {
+    int comp, band, j0, j1, i0, ncell;
+};
+__device__ __forceinline__ mom_band mom_locate(const pvk_grid &g, const pvk_mcg &w)
+{
+    mom_band m;
+    const int vb = xcd_band(blockIdx.x, gridDim.x);
+    m.comp = (vb < w.nb[0]) ? 0 : 1;
+    m.band = m.comp ? vb - w.nb[0] : vb;
+    const int jlo = m.comp ? 1 : 0, jhi = g.NYM;     // rows [jlo, NYM)
+    m.j0 = jlo + m.band * w.band;
+    m.j1 = min(jhi, m.j0 + w.band);
+    m.i0 = m.comp ? 0 : 1;
+    m.ncell = (m.j1 - m.j0) * g.NXM;
+    return m;
+}
+
+// r = rhs - A x; dg = diag + EPS; d = r / dg; partial <r, d> (msolve_cg.c:627-645)
+__global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w)
+{
+    __shared__ double sh[NTH];
+    const int P = g.P, NXM = g.NXM;
+    const mom_band m = mom_locate(g, w);
+    const int comp = m.comp ? 'v' : 'u';
+    if (m.band == 0 && threadIdx.x == 0) { sc->m_done[m.comp] = (w.maxit <= 0); sc->m_iter[m.comp] = 0; }
+    const double alpha_k = c_BET[stage], dt = sc->dt;
+    const double *x = w.x[m.comp], *rhs = w.rhs[m.comp];
+    double *r = w.r[m.comp], *dg = w.dg[m.comp], *d = w.d[m.comp][0], *Ad = w.Ad[m.comp];
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < m.ncell; c += NTH) {
+        const int j = m.j0 + c / NXM, i = m.i0 + c - (c / NXM) * NXM;
+        double ad = mom_op(g, ph, alpha_k, dt, w.rho, w.mu, x, comp, i, j);
+        double rr = AT(rhs, i, j) - ad;
+        double dgv = mom_diag(g, ph, alpha_k, dt, w.rho, w.mu, comp, i, j) + 1e-14;
+        double dd = rr / dgv;
+        AT(Ad, i, j) = ad;
+        AT(r, i, j) = rr;
+        AT(dg, i, j) = dgv;
+        AT(d, i, j) = dd;
+        acc += rr * dd;
+    }
+    block_store_partial(acc, w.rz[m.comp][0], m.band, sh);
+}
+
+// The Jacobi-PCG operator of mom_op with the five x values supplied.
+__device__ __forceinline__ double mom_op_vals(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                              const double *rho, const double *mu, int comp, int i, int j, double xc,
+                                              double xE, double xW, double xN, double xS)
+{
+    const int P = g.P;
+    const double iRe = ph.iRe;
+    double op = 0.0;
+    if (comp == 0) {
+        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
+        double factor_time = rho_face / (alpha_k * dt);
+        double Ax_val = factor_time * xc;
+        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
+        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
+        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
+        double flux_xp = mu_xp * (xE - xc) * g.idx_u[i];
+        double flux_xm = mu_xm * (xc - xW) * g.idx_u[i - 1];
+        double flux_yp = mu_yp * (xN - xc) * g.idy_c[j];
+        double flux_ym = mu_ym * (xc - xS) * g.idy_c[j - 1];
+        op += iRe * ((flux_xp - flux_xm) * g.idx_c[i - 1]);
+        op += iRe * (flux_yp - flux_ym) * g.idy_v[j];
+        op += 0.0;
+        return Ax_val - op;
+    }
+    double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
+    double factor_time = rho_face / (alpha_k * dt);
+    double Ax_val = factor_time * xc;
+    double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
+    double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
+    double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
+    double flux_xp = mu_xp * (xE - xc) * g.idx_c[i];
+    double flux_xm = mu_xm * (xc - xW) * g.idx_c[i - 1];
+    double flux_yp = mu_yp * (xN - xc) * g.idy_v[j];
+    double flux_ym = mu_ym * (xc - xS) * g.idy_v[j - 1];
+    op += iRe * ((flux_xp - flux_xm) * g.idx_u[i]);
+    op += iRe * (flux_yp - flux_ym) * g.idy_c[j - 1];
+    return Ax_val - op;
+}
+
+// K1: stop test sqrt(r.r / N) < CG_ETOL (msolve_cg.c:691-704), beta =
+// rz_new / (rz + EPS), d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
+// the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
+__global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w,
+                                                  int n)
+{
+    __shared__ double sh[NTH];
+    const mom_band m = mom_locate(g, w);
+    const int c = m.comp;
+    if (sc->m_done[c]) return;
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    const double EPS = 1e-14;
+    const bool lead = (m.band == 0 && threadIdx.x == 0);
+    const double rz = block_sum(w.rz[c][n & 1], w.nb[c], sh);
+    double beta = 0.0;
+    if (n > 0) {
+        const double rr = block_sum(w.rr[c], w.nb[c], sh);
+        const double rms = sqrt(w.in_tot[c] * rr);
+        bool stop = rms < w.tol;
+        if (!stop) {
+            beta = rz / (sc->m_rzb[c][(n + 1) & 1] + EPS);
+            stop = (n >= w.maxit);
+        }
+        if (stop) {
+            if (lead) {
+                sc->m_rms[c] = rms;
+                sc->m_iter[c] = n;
+                sc->m_done[c] = (rms < w.tol) ? 1 : 3;
+            }
+            return;
+        }
+    }
+    if (lead) sc->m_rzb[c][n & 1] = rz;
+    const double *r = w.r[c], *dg = w.dg[c], *dold = w.d[c][(n + 1) & 1], *dwall = w.d[c][0];
+    double *dnew = w.d[c][n & 1], *Ad = w.Ad[c];
+    const double alpha_k = c_BET[stage], dt = sc->dt;
+    // d at (a, b) after the CG-type ghost fill of the new direction
+    auto dv = [&](int a, int b) -> double {
+        if (c == 0) {
+            const int ia = wrapu(a, NXM);
+            const int jb = b < 0 ? 0 : (b >= NYM ? NYM - 1 : b);
+            double val = (n == 0) ? AT(dwall, ia, jb) : AT(r, ia, jb) / AT(dg, ia, jb) + beta * AT(dold, ia, jb);
+            return (b < 0 || b >= NYM) ? -val : val;
+        }
+        const int ia = wrapc(a, NXM);
+        if (b <= 0 || b >= NYM) return AT(dwall, ia, b);
+        return (n == 0) ? AT(dwall, ia, b) : AT(r, ia, b) / AT(dg, ia, b) + beta * AT(dold, ia, b);
+    };
+    double acc = 0.0;
+    for (int base = 0; base < m.ncell; base += NTH * CPT) {
+        double dc[CPT], ad[CPT];
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            const int q = base + k * NTH + threadIdx.x;
+            dc[k] = 0.0; ad[k] = 0.0;
+            if (q < m.ncell) {
+                const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+                dc[k] = dv(i, j);
+                ad[k] = mom_op_vals(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, dc[k], dv(i + 1, j), dv(i - 1, j),
+                                    dv(i, j + 1), dv(i, j - 1));
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < CPT; k++) {
+            const int q = base + k * NTH + threadIdx.x;
+            if (q < m.ncell) {
+                const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+                if (n > 0) AT(dnew, i, j) = dc[k];
+                AT(Ad, i, j) = ad[k];
+                acc += dc[k] * ad[k];
+            }
+        }
+    }
+    block_store_partial(acc, w.dad[c], m.band, sh);
+}
+
+// K2: alpha = rz / (dAd + EPS) (msolve_cg.c:674); x += alpha d; r -= alpha Ad;
+// partials of r^2 / dg and r^2 (:676-690).
+__global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pvk_mcg w, int n)
+{
+    __shared__ double sh[NTH];
+    const mom_band m = mom_locate(g, w);
+    const int c = m.comp;
+    if (sc->m_done[c]) return;
+    const int P = g.P, NXM = g.NXM;
+    const double EPS = 1e-14;
+    const double dAd = block_sum(w.dad[c], w.nb[c], sh);
+    const double alpha = sc->m_rzb[c][n & 1] / (dAd + EPS);
+    if (m.band == 0 && threadIdx.x == 0) sc->m_iter[c] = n + 1;
+    double *x = w.x[c], *r = w.r[c];
+    const double *d = w.d[c][n & 1], *Ad = w.Ad[c], *dg = w.dg[c];
+    double a0 = 0.0, a1 = 0.0;
+    for (int q = threadIdx.x; q < m.ncell; q += NTH) {
+        const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+        AT(x, i, j) += alpha * AT(d, i, j);
+        double ri = AT(r, i, j) - alpha * AT(Ad, i, j);
+        AT(r, i, j) = ri;
+        a0 += ri * ri / AT(dg, i, j);
+        a1 += ri * ri;
+    }
+    block_store_partial(a0, w.rz[c][(n + 1) & 1], m.band, sh);
+    block_store_partial(a1, w.rr[c], m.band, sh);
+}
+
+extern "C" void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
+                             hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mcg_init, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w);
+}
+
+extern "C" void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
+                             int half, int n, hipStream_t s)
+{
+    if (half == 0)
+        hipLaunchKernelGGL(k_mcg_apply, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
+    else
+        hipLaunchKernelGGL(k_mcg_update, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
+}

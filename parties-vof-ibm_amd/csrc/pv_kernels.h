@@ -21,10 +21,12 @@ extern "C" {
 typedef struct pvk_scal {
     double dt, dt_old, time;
     /* Cahn-Hilliard biharmonic CG (VOF_DIFFUSE.c:339-429) */
-    double ch_rz, ch_alpha, ch_beta, ch_bnorm, ch_rel;
-    int    ch_iter, ch_done, ch_maxit, pad0;
+    double ch_rzb[2];        /* <r,r> of iterations n (slot n & 1) and n - 1 */
+    double ch_bnorm, ch_rel;
+    int    ch_iter, ch_done;
     /* momentum Jacobi-PCG per component 0 = u, 1 = v (msolve_cg.c:525-722) */
-    double m_rz[2], m_alpha[2], m_beta[2], m_rms[2];
+    double m_rzb[2][2];      /* [component][n & 1]: <r, z> of iterations n and n - 1 */
+    double m_rms[2];
     int    m_iter[2], m_done[2];
     /* pressure PCG (psolve_cg.c:130-293 or FFT-PCG) */
     double p_gamma, p_alpha, p_beta, p_res, p_bnorm, p_mean;
@@ -67,6 +69,7 @@ typedef struct pvk_fft {
     const double *rtx, *rty;  /* (cos, sin)(2 pi k / N), k <= N/2 */
     const double *dcty;       /* (cos, sin)(pi k / (2 NYM)), k < NYM */
     const double *lamx, *lamy;/* 1-D eigenvalue magnitudes */
+    const double *wx, *wy;    /* (cos, -sin)(2 pi t / M), t < M: exp(-2 pi i t / M) */
     double *spec;             /* NXM * NYM spectral scratch, row-major */
 } pvk_fft;
 
@@ -79,6 +82,25 @@ typedef struct pvk_pcg {
     int ndad, nrr, nrz, nmean;        /* partial counts */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
 } pvk_pcg;
+
+/* Work sets of the fused CG loops (csrc/pv_cg.hip).  Partial-sum arrays
+ * hold one double per band. */
+typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
+    double *p[2];                    /* search direction, double-buffered */
+    double *bb, *rr[2], *pap;        /* partials: ||b||^2, ||r||^2 (by parity), <p, Ap> */
+    int nb, band;                    /* bands (= blocks) and rows per band */
+} pvk_ccg;
+
+typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
+    double *x[2], *r[2], *dg[2], *Ad[2];
+    double *d[2][2];                 /* [component][n & 1]; d[c][0] is the public field */
+    const double *rhs[2];
+    const double *rho, *mu;
+    double *rz[2][2], *dad[2], *rr[2];
+    int nb[2], band;                 /* nb[c] = 0 leaves component c out */
+    int maxit;
+    double tol, in_tot[2];           /* CG_ETOL; 1 / number of unknowns */
+} pvk_mcg;
 
 /* ---- ghost fills (kind codes in pv_kernels.hip k_ghost) ---- */
 void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s);          /* VOF_DIFFUSE_set_boundary_values */
@@ -98,21 +120,16 @@ void pvk_weno_advect(const pvk_grid *g, const pvk_phys *ph, const double *C, con
                      const double *v, double *rhs, hipStream_t s);
 void pvk_explicit_diffusion(const pvk_grid *g, const pvk_phys *ph, const double *C, double *rhs,
                             hipStream_t s);
-void pvk_ch_rhs_init(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
-                     const double *C, const double *u, const double *v, const double *rn,
-                     double *rc, double *r, double *p, const pvk_red *rd, hipStream_t s);
-void pvk_ch_cg_init_from_b(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
-                           const double *C, double *b, double *r, double *p, const pvk_red *rd,
-                           hipStream_t s);
-void pvk_ch_finalize_init(const pvk_grid *g, pvk_scal *sc, int maxit, double tol, const pvk_red *rd,
-                          hipStream_t s);
-void pvk_ch_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const double *p,
-                  double *Ap, const pvk_red *rd, hipStream_t s);
-void pvk_ch_alpha(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s);
-void pvk_ch_update(const pvk_grid *g, pvk_scal *sc, double *x, const double *p, double *r,
-                   const double *Ap, const pvk_red *rd, hipStream_t s);
-void pvk_ch_beta(const pvk_grid *g, pvk_scal *sc, double tol, const pvk_red *rd, hipStream_t s);
-void pvk_ch_pupdate(const pvk_grid *g, pvk_scal *sc, const double *r, double *p, hipStream_t s);
+/* b -= (Cn^2/Pe) L^2 C (standalone biharmonic solve) */
+void pvk_ch_sub_bih(const pvk_grid *g, const pvk_phys *ph, const double *C, double *b, hipStream_t s);
+/* fused biharmonic CG (csrc/pv_cg.hip): start from the explicit CH
+ * right-hand side (b == NULL: WENO5 + diffusion + RK3 assembly) or from a
+ * given b; then iterations n = 0, 1, ...: half 0 = apply, half 1 = update */
+void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
+                  const double *C, const double *u, const double *v, double *rn, const double *b,
+                  double *r, double *p, hipStream_t s);
+void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
+                  int half, int n, double *x, double *r, double *Ap, hipStream_t s);
 void pvk_clamp_props(const pvk_grid *g, const pvk_phys *ph, double *C, double *G, double *rho,
                      double *mu, const pvk_red *rd, pvk_scal *sc, hipStream_t s);
 void pvk_phase_props(const pvk_grid *g, const pvk_phys *ph, const double *C, double *G, double *rho,
@@ -134,19 +151,11 @@ void pvk_mom_force(const pvk_grid *g, const pvk_phys *ph, int comp, const double
                    double *rhs, int which, hipStream_t s);
 void pvk_mom_matvec(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc, int comp,
                     const double *x, const double *rho, const double *mu, double *Ax, hipStream_t s);
-void pvk_mom_cg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
-                     const double *x, const double *rhs, const double *rho, const double *mu,
-                     double *r, double *d, double *Ad, double *dg, const pvk_red *rd, hipStream_t s);
-void pvk_mom_cg_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
-                      const double *d, const double *rho, const double *mu, double *Ad,
-                      const pvk_red *rd, hipStream_t s);
-void pvk_mom_cg_update(const pvk_grid *g, pvk_scal *sc, int comp, double *x, const double *d,
-                       double *r, const double *Ad, const double *dg, const pvk_red *rd,
-                       hipStream_t s);
-void pvk_mom_cg_dupdate(const pvk_grid *g, pvk_scal *sc, int comp, const double *r,
-                        const double *dg, double *d, hipStream_t s);
-void pvk_mom_scalar(const pvk_grid *g, pvk_scal *sc, int comp, int which, int maxit, double tol,
-                    const pvk_red *rd, hipStream_t s);
+/* fused momentum Jacobi-PCG for the components with nb[c] > 0 */
+void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
+                  hipStream_t s);
+void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
+                  int half, int n, hipStream_t s);
 
 /* ---- pressure ---- */
 void pvk_p_rhs2(const pvk_grid *g, int stage, const pvk_scal *sc, const double *u, const double *v,

@@ -273,197 +273,15 @@ extern "C" void pvk_explicit_diffusion(const pvk_grid *g, const pvk_phys *ph, co
     hipLaunchKernelGGL(k_explicit_diffusion, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, rhs);
 }
 
-// Explicit CH right-hand side and start of the biharmonic CG in one sweep:
-//   rc = -div(uC) + L S(C) / Pe                         (VOF_DIFFUSE.c:1284-1296)
-//   b  = a_dt C + (g/b) rc [+ (z/b) rn] - (Cn^2/Pe) L^2 C (:1298-1317, :1047-1057)
-//   r  = p = b - (a_dt C + (Cn^2/Pe) L^2 C)               (:356-364)
-// rc overwrites rn in place (history shift ch_rhs_n <- rc, :1357-1358).
-__global__ void k_ch_rhs_init(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *C,
-                              const double *u, const double *v, double *rn, double *r, double *p,
-                              double *part, int nb)
-{
-    CELL_IJ
-    double vals[2] = {0.0, 0.0};
-    if (in) {
-        const double dt = sc->dt;
-        const double a_dt = 1.0 / (dt * c_BET[stage]);
-        double rc = weno_rhs_at(g, ph, C, u, v, i, j);
-        rc += diffusion_at(g, ph, C, i, j);
-        double c = AT(C, i, j);
-        double rv = (stage == 0) ? a_dt * c + c_GAMB[0] * rc
-                                 : a_dt * c + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
-        double L2 = bilap_at(g, C, i, j);
-        double b = rv - ph.bih * L2;
-        double Ap = a_dt * c + ph.bih * L2;
-        double rr = b - Ap;
-        AT(rn, i, j) = rc;
-        AT(r, i, j) = rr;
-        AT(p, i, j) = rr;
-        vals[0] = ch_w(g, i, j, b, b);
-        vals[1] = ch_w(g, i, j, rr, rr);
-    }
-    block_partials<2>(vals, part, nb);
-}
-extern "C" void pvk_ch_rhs_init(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
-                                const double *C, const double *u, const double *v, const double *rn,
-                                double *rc, double *r, double *p, const pvk_red *rd, hipStream_t s)
-{
-    (void)rc;
-    hipLaunchKernelGGL(k_ch_rhs_init, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, C, u, v,
-                       (double *)rn, r, p, rd->part, rd->nblocks);
-}
-
-// Biharmonic CG start from a given b (b already holds rhs - (Cn^2/Pe) L^2 C).
-__global__ void k_ch_cg_init_b(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *C,
-                               const double *b, double *r, double *p, double *part, int nb)
-{
-    CELL_IJ
-    double vals[2] = {0.0, 0.0};
-    if (in) {
-        const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
-        double c = AT(C, i, j);
-        double Ap = a_dt * c + ph.bih * bilap_at(g, C, i, j);
-        double bb = AT(b, i, j);
-        double rr = bb - Ap;
-        AT(r, i, j) = rr;
-        AT(p, i, j) = rr;
-        vals[0] = ch_w(g, i, j, bb, bb);
-        vals[1] = ch_w(g, i, j, rr, rr);
-    }
-    block_partials<2>(vals, part, nb);
-}
-
 // b -= (Cn^2/Pe) L^2 C on the interior (VOF_DIFFUSE.c:1047-1057)
 __global__ void k_ch_sub_bih(pvk_grid g, pvk_phys ph, const double *C, double *b)
 {
     CELL_IJ
     if (in) AT(b, i, j) -= ph.bih * bilap_at(g, C, i, j);
 }
-
-extern "C" void pvk_ch_cg_init_from_b(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_scal *sc,
-                                      const double *C, double *b, double *r, double *p, const pvk_red *rd,
-                                      hipStream_t s)
+extern "C" void pvk_ch_sub_bih(const pvk_grid *g, const pvk_phys *ph, const double *C, double *b, hipStream_t s)
 {
     hipLaunchKernelGGL(k_ch_sub_bih, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, b);
-    hipLaunchKernelGGL(k_ch_cg_init_b, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, C, b, r, p,
-                       rd->part, rd->nblocks);
-}
-
-// CG bookkeeping after the start sweep (VOF_DIFFUSE.c:366-381)
-__global__ void k_ch_finalize_init(pvk_scal *sc, int maxit, double tol, const double *part, int nb)
-{
-    double bb = 2.0 * final_sum(part, nb, 0, false);
-    double rr = 2.0 * final_sum(part, nb, 1, false);
-    if (threadIdx.x == 0) {
-        double bnorm = sqrt(bb);
-        if (bnorm < 1e-30) bnorm = 1.0;
-        sc->ch_bnorm = bnorm;
-        sc->ch_rz = rr;
-        sc->ch_rel = sqrt(rr) / bnorm;
-        sc->ch_iter = 0;
-        sc->ch_maxit = maxit;
-        sc->ch_done = (sc->ch_rel < tol || maxit <= 0) ? 1 : 0;
-        sc->red[0] = bb;
-        sc->red[1] = rr;
-    }
-}
-extern "C" void pvk_ch_finalize_init(const pvk_grid *g, pvk_scal *sc, int maxit, double tol, const pvk_red *rd,
-                                     hipStream_t s)
-{
-    (void)g;
-    hipLaunchKernelGGL(k_ch_finalize_init, dim3(1), dim3(256), 0, s, sc, maxit, tol, rd->part, rd->nblocks);
-}
-
-// Ap = a_dt p + (Cn^2/Pe) L^2 p, partial <p, Ap> (VOF_DIFFUSE.c:318-337, :384-386)
-__global__ void k_ch_apply(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, const double *p,
-                           double *Ap, double *part, int nb)
-{
-    if (sc->ch_done) return;
-    CELL_IJ
-    double vals[1] = {0.0};
-    if (in) {
-        const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
-        double pc = AT(p, i, j);
-        double ap = a_dt * pc + ph.bih * bilap_at(g, p, i, j);
-        AT(Ap, i, j) = ap;
-        vals[0] = ch_w(g, i, j, pc, ap);
-    }
-    block_partials<1>(vals, part, nb);
-}
-extern "C" void pvk_ch_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const double *p,
-                             double *Ap, const pvk_red *rd, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ch_apply, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, p, Ap, rd->part,
-                       rd->nblocks);
-}
-
-__global__ void k_ch_alpha(pvk_scal *sc, const double *part, int nb)
-{
-    if (sc->ch_done) return;
-    double pAp = 2.0 * final_sum(part, nb, 0, false);
-    if (threadIdx.x == 0) {
-        if (fabs(pAp) < 1e-30) sc->ch_done = 2;
-        else sc->ch_alpha = sc->ch_rz / pAp;
-    }
-}
-extern "C" void pvk_ch_alpha(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s)
-{
-    (void)g;
-    hipLaunchKernelGGL(k_ch_alpha, dim3(1), dim3(256), 0, s, sc, rd->part, rd->nblocks);
-}
-
-// x += alpha p; r -= alpha Ap; partial <r, r> (VOF_DIFFUSE.c:390-405)
-__global__ void k_ch_update(pvk_grid g, const pvk_scal *sc, double *x, const double *p, double *r,
-                            const double *Ap, double *part, int nb)
-{
-    if (sc->ch_done) return;
-    CELL_IJ
-    double vals[1] = {0.0};
-    if (in) {
-        const double alpha = sc->ch_alpha;
-        AT(x, i, j) += alpha * AT(p, i, j);
-        double rr = AT(r, i, j) - alpha * AT(Ap, i, j);
-        AT(r, i, j) = rr;
-        vals[0] = ch_w(g, i, j, rr, rr);
-    }
-    block_partials<1>(vals, part, nb);
-}
-extern "C" void pvk_ch_update(const pvk_grid *g, pvk_scal *sc, double *x, const double *p, double *r,
-                              const double *Ap, const pvk_red *rd, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ch_update, grid2d(g), dim3(BX, BY), 0, s, *g, sc, x, p, r, Ap, rd->part, rd->nblocks);
-}
-
-// rel = ||r|| / ||b||, stop test, beta (VOF_DIFFUSE.c:403-414)
-__global__ void k_ch_beta(pvk_scal *sc, double tol, const double *part, int nb)
-{
-    if (sc->ch_done) return;
-    double rr = 2.0 * final_sum(part, nb, 0, false);
-    if (threadIdx.x == 0) {
-        sc->ch_iter += 1;
-        sc->ch_rel = sqrt(rr) / sc->ch_bnorm;
-        if (sc->ch_rel < tol) { sc->ch_done = 1; return; }
-        sc->ch_beta = rr / (sc->ch_rz + 1e-30);
-        sc->ch_rz = rr;
-        if (sc->ch_iter >= sc->ch_maxit) sc->ch_done = 3;
-    }
-}
-extern "C" void pvk_ch_beta(const pvk_grid *g, pvk_scal *sc, double tol, const pvk_red *rd, hipStream_t s)
-{
-    (void)g;
-    hipLaunchKernelGGL(k_ch_beta, dim3(1), dim3(256), 0, s, sc, tol, rd->part, rd->nblocks);
-}
-
-// p = z + beta p with z = r (VOF_DIFFUSE.c:416-422)
-__global__ void k_ch_pupdate(pvk_grid g, const pvk_scal *sc, const double *r, double *p)
-{
-    if (sc->ch_done) return;
-    CELL_IJ
-    if (in) AT(p, i, j) = AT(r, i, j) + sc->ch_beta * AT(p, i, j);
-}
-extern "C" void pvk_ch_pupdate(const pvk_grid *g, pvk_scal *sc, const double *r, double *p, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_ch_pupdate, grid2d(g), dim3(BX, BY), 0, s, *g, sc, r, p);
 }
 
 // diffuse_bound_liquid_fraction + update_phase_cache + density/viscosity
@@ -767,135 +585,6 @@ extern "C" void pvk_mom_matvec(const pvk_grid *g, const pvk_phys *ph, int stage,
                                const double *x, const double *rho, const double *mu, double *Ax, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mom_matvec, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, x, rho, mu, Ax);
-}
-
-// r = rhs - A x; dg = diag + EPS; d = r / dg; partial <r, d> (msolve_cg.c:627-645)
-__global__ void k_mom_cg_init(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp, const double *x,
-                              const double *rhs, const double *rho, const double *mu, double *r, double *d,
-                              double *Ad, double *dg, double *part, int nb)
-{
-    MOM_IJ
-    double vals[1] = {0.0};
-    if (act) {
-        const double alpha_k = c_BET[stage], dt = sc->dt;
-        double ad = mom_op(g, ph, alpha_k, dt, rho, mu, x, comp, ii, j);
-        double rr = AT(rhs, ii, j) - ad;
-        double dgv = mom_diag(g, ph, alpha_k, dt, rho, mu, comp, ii, j) + 1e-14;
-        double dd = rr / dgv;
-        AT(Ad, ii, j) = ad;
-        AT(r, ii, j) = rr;
-        AT(dg, ii, j) = dgv;
-        AT(d, ii, j) = dd;
-        vals[0] = rr * dd;
-    }
-    block_partials<1>(vals, part, nb);
-}
-extern "C" void pvk_mom_cg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
-                                const double *x, const double *rhs, const double *rho, const double *mu, double *r,
-                                double *d, double *Ad, double *dg, const pvk_red *rd, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mom_cg_init, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, x, rhs, rho, mu,
-                       r, d, Ad, dg, rd->part, rd->nblocks);
-}
-
-// Ad = A d; partial <d, Ad> (msolve_cg.c:665-672)
-__global__ void k_mom_cg_apply(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp, const double *d,
-                               const double *rho, const double *mu, double *Ad, double *part, int nb)
-{
-    const int c = (comp == 'u') ? 0 : 1;
-    if (sc->m_done[c]) return;
-    MOM_IJ
-    double vals[1] = {0.0};
-    if (act) {
-        double ad = mom_op(g, ph, c_BET[stage], sc->dt, rho, mu, d, comp, ii, j);
-        AT(Ad, ii, j) = ad;
-        vals[0] = AT(d, ii, j) * ad;
-    }
-    block_partials<1>(vals, part, nb);
-}
-extern "C" void pvk_mom_cg_apply(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, int comp,
-                                 const double *d, const double *rho, const double *mu, double *Ad, const pvk_red *rd,
-                                 hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mom_cg_apply, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, d, rho, mu, Ad,
-                       rd->part, rd->nblocks);
-}
-
-// x += alpha d; r -= alpha Ad; partials of r^2/dg and r^2 (msolve_cg.c:676-690)
-__global__ void k_mom_cg_update(pvk_grid g, const pvk_scal *sc, int comp, double *x, const double *d, double *r,
-                                const double *Ad, const double *dg, double *part, int nb)
-{
-    const int c = (comp == 'u') ? 0 : 1;
-    if (sc->m_done[c]) return;
-    MOM_IJ
-    double vals[2] = {0.0, 0.0};
-    if (act) {
-        const double alpha = sc->m_alpha[c];
-        AT(x, ii, j) += alpha * AT(d, ii, j);
-        double ri = AT(r, ii, j) - alpha * AT(Ad, ii, j);
-        AT(r, ii, j) = ri;
-        vals[0] = ri * ri / AT(dg, ii, j);
-        vals[1] = ri * ri;
-    }
-    block_partials<2>(vals, part, nb);
-}
-extern "C" void pvk_mom_cg_update(const pvk_grid *g, pvk_scal *sc, int comp, double *x, const double *d, double *r,
-                                  const double *Ad, const double *dg, const pvk_red *rd, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mom_cg_update, grid2d(g), dim3(BX, BY), 0, s, *g, sc, comp, x, d, r, Ad, dg, rd->part,
-                       rd->nblocks);
-}
-
-// d = r / dg + beta d (msolve_cg.c:705-713)
-__global__ void k_mom_cg_dupdate(pvk_grid g, const pvk_scal *sc, int comp, const double *r, const double *dg,
-                                 double *d)
-{
-    const int c = (comp == 'u') ? 0 : 1;
-    if (sc->m_done[c]) return;
-    MOM_IJ
-    if (act) AT(d, ii, j) = AT(r, ii, j) / AT(dg, ii, j) + sc->m_beta[c] * AT(d, ii, j);
-}
-extern "C" void pvk_mom_cg_dupdate(const pvk_grid *g, pvk_scal *sc, int comp, const double *r, const double *dg,
-                                   double *d, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_mom_cg_dupdate, grid2d(g), dim3(BX, BY), 0, s, *g, sc, comp, r, dg, d);
-}
-
-// which 0: rz after init; 1: alpha; 2: stop test + beta (msolve_cg.c:645-704)
-__global__ void k_mom_scalar(pvk_scal *sc, int comp, int which, int maxit, double tol, double in_tot,
-                             const double *part, int nb)
-{
-    const int c = (comp == 'u') ? 0 : 1;
-    const double EPS = 1e-14;
-    if (which == 0) {
-        double rz = final_sum(part, nb, 0, false);
-        if (threadIdx.x == 0) { sc->m_rz[c] = rz; sc->m_iter[c] = 0; sc->m_done[c] = (maxit <= 0); }
-        return;
-    }
-    if (sc->m_done[c]) return;
-    if (which == 1) {
-        double dAd = final_sum(part, nb, 0, false);
-        if (threadIdx.x == 0) { sc->m_alpha[c] = sc->m_rz[c] / (dAd + EPS); sc->m_iter[c] += 1; }
-        return;
-    }
-    double rz_new = final_sum(part, nb, 0, false);
-    double rr = final_sum(part, nb, 1, false);
-    if (threadIdx.x == 0) {
-        double rms = sqrt(in_tot * rr);
-        sc->m_rms[c] = rms;
-        if (rms < tol) { sc->m_done[c] = 1; return; }
-        sc->m_beta[c] = rz_new / (sc->m_rz[c] + EPS);
-        sc->m_rz[c] = rz_new;
-        if (sc->m_iter[c] >= maxit) sc->m_done[c] = 3;
-    }
-}
-extern "C" void pvk_mom_scalar(const pvk_grid *g, pvk_scal *sc, int comp, int which, int maxit, double tol,
-                               const pvk_red *rd, hipStream_t s)
-{
-    double in_tot = (comp == 'u') ? 1.0 / (double)((g->NXM) * (g->NYM) * 1)
-                                  : 1.0 / (double)((g->NXM) * (g->NYM - 1) * 1);
-    hipLaunchKernelGGL(k_mom_scalar, dim3(1), dim3(256), 0, s, sc, comp, which, maxit, tol, in_tot, rd->part,
-                       rd->nblocks);
 }
 
 // ---------------------------------------------------------------------------

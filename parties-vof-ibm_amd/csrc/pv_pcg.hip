@@ -21,66 +21,15 @@
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "pv_kernels.h"
+#include "pv_block.h"
+#include "pv_fftk.h"
 
 #define AT(a, i, j) (a)[(ptrdiff_t)(j) * P + (i)]
-#define NTH 256
 
 struct cplx2 { double x, y; };
 
-// Sum of n partials, same order in every block; all threads get the result.
-__device__ double block_sum(const double *part, int n, double *sh)
-{
-    const int t = threadIdx.x;
-    double s = 0.0;
-    for (int b = t; b < n; b += NTH) s += part[b];
-    sh[t] = s;
-    __syncthreads();
-    for (int w = NTH / 2; w > 0; w >>= 1) {
-        if (t < w) sh[t] += sh[t + w];
-        __syncthreads();
-    }
-    double r = sh[0];
-    __syncthreads();
-    return r;
-}
-
-__device__ void block_store_partial(double v, double *part, double *sh)
-{
-    const int t = threadIdx.x;
-    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    if ((t & 63) == 0) sh[t >> 6] = v;
-    __syncthreads();
-    if (t == 0) {
-        double s = sh[0];
-        for (int w = 1; w < NTH / 64; w++) s += sh[w];
-        part[blockIdx.x] = s;
-    }
-    __syncthreads();
-}
-
-__device__ __forceinline__ unsigned brev2(unsigned m, int logM) { return logM ? (__brev(m) >> (32 - logM)) : 0u; }
-
-// radix-2 DIT over R sequences of length M (bit-reversed input) in LDS
-__device__ void fft_lds(double *sr, double *si, int R, int M, int logM, const double *tw, bool inverse)
-{
-    const int nb = R * (M >> 1);
-    for (int s = 0; s < logM; s++) {
-        const int half = 1 << s, tstep = (M >> 1) >> s;
-        for (int b = threadIdx.x; b < nb; b += NTH) {
-            int q = b / (M >> 1), bb = b - q * (M >> 1);
-            int grp = bb >> s, pos = bb & (half - 1);
-            int i0 = q * M + grp * 2 * half + pos, i1 = i0 + half;
-            double c = tw[2 * (pos * tstep)], sn = tw[2 * (pos * tstep) + 1];
-            double wr = c, wi = inverse ? sn : -sn;
-            double xr = sr[i1], xi = si[i1];
-            double tr = wr * xr - wi * xi, ti = wr * xi + wi * xr;
-            double ar = sr[i0], ai = si[i0];
-            sr[i1] = ar - tr; si[i1] = ai - ti;
-            sr[i0] = ar + tr; si[i0] = ai + ti;
-        }
-        __syncthreads();
-    }
-}
+// threads per transform of length M in the Stockham FFT (pv_fftk.h)
+__device__ __forceinline__ int sk_threads(int M) { return M >= 8 ? M / 8 : 1; }
 
 // X_k, k = 0..M, of a real length-2M sequence from Z = FFT_M(x_{2m} + i x_{2m+1})
 __device__ __forceinline__ cplx2 rpost(const double *sr, const double *si, int base, int M, int k, const double *rt)
@@ -151,7 +100,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
         AT(w.Ad, i, j) = ad;
         acc += d0 * ad;
     }
-    block_store_partial(acc, w.dad, sh);
+    block_store_partial(acc, w.dad, blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -191,11 +140,13 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
             AT(w.r, i + 1, j) = b;
             acc += a * a + b * b;
         }
-        int dd = q * M + brev2(m, logM);
-        sr[dd] = a; si[dd] = b;
+        sr[q * M + m] = a; si[q * M + m] = b;
     }
-    block_store_partial(acc, w.rr, sh);
-    fft_lds(sr, si, R, M, logM, F.twx, false);
+    block_store_partial(acc, w.rr, blockIdx.x, sh);
+    {
+        const int T = sk_threads(M), q = threadIdx.x / T;
+        sk_fft<false>(sr + q * M, si + q * M, M, logM, F.wx, threadIdx.x - q * T, T);
+    }
     const double scale = 1.0 / (double)N;
     for (int t = threadIdx.x; t < R * (M + 1); t += NTH) {
         int q = t / (M + 1), k = t - q * (M + 1), j = j0 + q;
@@ -266,11 +217,11 @@ __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_sca
             a = row[vmap2(2 * m, N)];
             b = row[vmap2(2 * m + 1, N)];
         }
-        int d = q * M + brev2(m, logM);
-        sr[d] = a; si[d] = b;
+        sr[q * M + m] = a; si[q * M + m] = b;
     }
     __syncthreads();
-    fft_lds(sr, si, R, M, logM, F.twy, false);
+    const int T = sk_threads(M), qq = threadIdx.x / T, lane = threadIdx.x - qq * T;
+    sk_fft<false>(sr + qq * M, si + qq * M, M, logM, F.wy, lane, T);
     const int npair = M / 2 + 1;
     cplx2 Ya[KST], Yb[KST];
     int cnt = 0;
@@ -299,12 +250,12 @@ __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_sca
     cnt = 0;
     for (int t = threadIdx.x; t < R * npair && cnt < KST; t += NTH, cnt++) {
         int q = t / npair, k = t - q * npair;
-        if (k < M) { int d = q * M + brev2(k, logM); sr[d] = Ya[cnt].x; si[d] = Ya[cnt].y; }
+        if (k < M) { sr[q * M + k] = Ya[cnt].x; si[q * M + k] = Ya[cnt].y; }
         int k2 = M - k;
-        if (k2 < M && k2 != k) { int d = q * M + brev2(k2, logM); sr[d] = Yb[cnt].x; si[d] = Yb[cnt].y; }
+        if (k2 < M && k2 != k) { sr[q * M + k2] = Yb[cnt].x; si[q * M + k2] = Yb[cnt].y; }
     }
     __syncthreads();
-    fft_lds(sr, si, R, M, logM, F.twy, true);
+    sk_fft<true>(sr + qq * M, si + qq * M, M, logM, F.wy, lane, T);
     const double invN = 1.0 / (double)N;
     for (int t = threadIdx.x; t < R * M; t += NTH) {
         int q = t / M, m = t - q * M, i = i0 + q;
@@ -333,11 +284,13 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
             const double *row = F.spec + (size_t)j * N;
             Y = rpre(hcget(row, N, k), hcget(row, N, M - k), k, F.rtx);
         }
-        int d = q * M + brev2(k, logM);
-        sr[d] = Y.x; si[d] = Y.y;
+        sr[q * M + k] = Y.x; si[q * M + k] = Y.y;
     }
     __syncthreads();
-    fft_lds(sr, si, R, M, logM, F.twx, true);
+    {
+        const int T = sk_threads(M), q = threadIdx.x / T;
+        sk_fft<true>(sr + q * M, si + q * M, M, logM, F.wx, threadIdx.x - q * T, T);
+    }
     double acc = 0.0;
     for (int t = threadIdx.x; t < R * M; t += NTH) {
         int q = t / M, m = t - q * M, j = j0 + q;
@@ -347,7 +300,7 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
         AT(w.z, 2 * m + 1, j) = b;
         acc += AT(w.r, 2 * m, j) * a + AT(w.r, 2 * m + 1, j) * b;
     }
-    block_store_partial(acc, w.rz[(n + 1) & 1], sh);
+    block_store_partial(acc, w.rz[(n + 1) & 1], blockIdx.x, sh);
 }
 
 // partial sums of rhs (mean removal before the solve)
@@ -362,7 +315,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_mean(pvk_grid g, pvk_scal *sc, pvk_
         int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
         acc += AT(w.rhs, i, j);
     }
-    block_store_partial(acc, w.mean, sh);
+    block_store_partial(acc, w.mean, blockIdx.x, sh);
 }
 
 static size_t lds_rows(int R, int N) { return (size_t)R * (size_t)(N / 2) * 2 * sizeof(double); }

@@ -240,6 +240,10 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     b->time = 0.0; b->dt = b->dt_old = b->prm.default_dt; b->stage = 0; b->ntime = 1; b->time_nonzero = 0;
     b->ch_chunk = 8; b->mom_chunk[0] = b->mom_chunk[1] = 4; b->p_chunk = 16;
     {
+        int rc = pv_cg_create(b);
+        if (rc) return rc;
+    }
+    {
         int rc = pv_fft_create(b);
         if (rc) return rc;
     }
@@ -251,6 +255,50 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     return PV_OK;
 }
 
+/* Work sets of the fused biharmonic and momentum CG loops: one band of rows
+ * per 256-thread block, one partial-sum slot per band and reduction. */
+int pv_cg_create(pv_bag *b)
+{
+    const int nx = b->NXM, ny = b->NYM;
+    const int band = 1;
+    int nbc = (ny + band - 1) / band, nbu = nbc, nbv = (ny - 1 + band - 1) / band, c;
+    size_t n = 4 * (size_t)nbc + 4 * (size_t)nbu + 4 * (size_t)nbv;
+    double *q;
+    HIPCHK(hipMalloc((void **)&b->cgmem, sizeof(double) * n));
+    HIPCHK(hipMemset(b->cgmem, 0, sizeof(double) * n));
+    q = b->cgmem;
+    memset(&b->ccg, 0, sizeof b->ccg);
+    b->ccg.nb = nbc; b->ccg.band = band;
+    b->ccg.p[0] = b->f[PV_F_CH_DIR]; b->ccg.p[1] = b->f[PV_FI_CHP2];
+    b->ccg.bb = q; q += nbc;
+    b->ccg.rr[0] = q; q += nbc;
+    b->ccg.rr[1] = q; q += nbc;
+    b->ccg.pap = q; q += nbc;
+    memset(&b->mcg, 0, sizeof b->mcg);
+    b->mcg.band = band;
+    b->mcg.rho = b->f[PV_F_RHO]; b->mcg.mu = b->f[PV_F_MU];
+    b->mcg.maxit = b->ph.CG_MAXIT; b->mcg.tol = b->ph.CG_ETOL;
+    /* RMS over the unknowns (msolve_cg.c:691): NXM * NYM for u, NXM * (NYM - 1) for v */
+    b->mcg.in_tot[0] = 1.0 / (double)(nx * ny);
+    b->mcg.in_tot[1] = 1.0 / (double)(nx * (ny - 1));
+    for (c = 0; c < 2; c++) {
+        int nbk = c ? nbv : nbu;
+        b->mcg.x[c] = b->f[c ? PV_F_V : PV_F_U];
+        b->mcg.r[c] = b->f[c ? PV_F_V_R : PV_F_U_R];
+        b->mcg.dg[c] = b->f[c ? PV_FI_DG_V : PV_FI_DG_U];
+        b->mcg.Ad[c] = b->f[c ? PV_F_V_AD : PV_F_U_AD];
+        b->mcg.d[c][0] = b->f[c ? PV_F_V_D : PV_F_U_D];
+        b->mcg.d[c][1] = b->f[c ? PV_FI_DV2 : PV_FI_DU2];
+        b->mcg.rhs[c] = b->f[c ? PV_F_V_RHS : PV_F_U_RHS];
+        b->mcg.rz[c][0] = q; q += nbk;
+        b->mcg.rz[c][1] = q; q += nbk;
+        b->mcg.dad[c] = q; q += nbk;
+        b->mcg.rr[c] = q; q += nbk;
+        b->mcg.nb[c] = nbk;
+    }
+    return PV_OK;
+}
+
 int pv_destroy(pv_bag *b)
 {
     if (!b) return PV_OK;
@@ -258,6 +306,7 @@ int pv_destroy(pv_bag *b)
     hipStreamSynchronize(b->stream);
     if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
+    hipFree(b->cgmem);
     hipFree(b->slabmem);
     hipFree(b->metricmem);
     hipFree(b->rd.part);
@@ -377,7 +426,7 @@ int pv_fft_create(pv_bag *b)
 {
     const double PI_ = 3.14159265358979323846;
     int nx = b->NXM, ny = b->NYM, lx = ilog2_exact(nx), ly = ilog2_exact(ny), k;
-    size_t n_twx, n_twy, n_rtx, n_rty, n_dct, total;
+    size_t n_twx, n_twy, n_rtx, n_rty, n_dct, n_wx, n_wy, total;
     double *h, *d, *q;
     double idx2, idy2;
     memset(&b->fft, 0, sizeof b->fft);
@@ -395,7 +444,8 @@ int pv_fft_create(pv_bag *b)
     n_twx = 2 * (size_t)(nx / 4 > 0 ? nx / 4 : 1); n_twy = 2 * (size_t)(ny / 4 > 0 ? ny / 4 : 1);
     n_rtx = 2 * (size_t)(nx / 2 + 1); n_rty = 2 * (size_t)(ny / 2 + 1);
     n_dct = 2 * (size_t)ny;
-    total = n_twx + n_twy + n_rtx + n_rty + n_dct + nx + ny;
+    n_wx = 2 * (size_t)(nx / 2); n_wy = 2 * (size_t)(ny / 2);
+    total = n_twx + n_twy + n_rtx + n_rty + n_dct + nx + ny + n_wx + n_wy;
     h = (double *)calloc(total, sizeof(double));
     if (!h) return pv_set_error(PV_ENOMEM, "FFT table allocation failed");
     q = h;
@@ -419,6 +469,11 @@ int pv_fft_create(pv_bag *b)
     }
     q += nx;
     for (k = 0; k < ny; k++) q[k] = 2.0 * (1 - cos(PI_ * k / ny)) * idy2;
+    q += ny;
+    /* Stockham twiddles of the half-length complex transforms */
+    for (k = 0; k < nx / 2; k++) { double a = 2.0 * PI_ * k / (nx / 2); q[2 * k] = cos(a); q[2 * k + 1] = -sin(a); }
+    q += n_wx;
+    for (k = 0; k < ny / 2; k++) { double a = 2.0 * PI_ * k / (ny / 2); q[2 * k] = cos(a); q[2 * k + 1] = -sin(a); }
     if (hipMalloc((void **)&d, sizeof(double) * (total + (size_t)nx * ny)) != hipSuccess) {
         free(h);
         return pv_set_error(PV_ENOMEM, "FFT table device allocation failed");
@@ -432,13 +487,12 @@ int pv_fft_create(pv_bag *b)
     q = d;
     /* fused PCG work set: spectrum transpose + five partial buffers */
     {
+        /* row transforms use M/8 threads each (M = N/2, Stockham radix 8) in
+         * 256-thread blocks: 256 / (M/8) rows per block, 32 KiB of LDS */
         int band = (ny >= 512) ? 2 : 1, rrow, rdct, nband, nrow, np;
         double *pm;
-        rrow = 4096 / nx; if (rrow > 8) rrow = 8; if (rrow < 1) rrow = 1;
-        if (nx >= 1024 && rrow > 2) rrow = 2;
-        rdct = 4096 / ny; if (rdct > 8) rdct = 8; if (rdct < 1) rdct = 1;
-        while (rdct > 1 && rdct * (ny / 4 + 1) > 8 * 256) rdct /= 2;
-        if (ny >= 1024 && rdct > 2) rdct = 2;
+        rrow = 256 / ((nx / 2 >= 8) ? nx / 16 : 1);
+        rdct = 256 / ((ny / 2 >= 8) ? ny / 16 : 1);
         nband = (ny + band - 1) / band;
         nrow = (ny + rrow - 1) / rrow;
         np = (nband > nrow) ? nband : nrow;
@@ -466,6 +520,8 @@ int pv_fft_create(pv_bag *b)
     b->fft.dcty = q; q += n_dct;
     b->fft.lamx = q; q += nx;
     b->fft.lamy = q; q += ny;
+    b->fft.wx = q; q += n_wx;
+    b->fft.wy = q; q += n_wy;
     b->fft.spec = q;
     b->fft.ok = 1;
     return PV_OK;
@@ -486,6 +542,8 @@ void pv_fft_destroy(pv_bag *b)
 void pv_tbegin(pv_bag *b, int t)
 {
     if (!b->timers_on) return;
+    b->tactive[t] = (b->tseen[t]++ % b->timers_on) == 0;
+    if (!b->tactive[t]) return;
     if (b->tpending[t] >= PV_TPOOL) {
         hipStreamSynchronize(b->stream);
         pv_timers_flush(b);
@@ -495,7 +553,7 @@ void pv_tbegin(pv_bag *b, int t)
 
 void pv_tend(pv_bag *b, int t)
 {
-    if (!b->timers_on) return;
+    if (!b->timers_on || !b->tactive[t]) return;
     hipEventRecord(b->tev[t][1][b->tpending[t]], b->stream);
     b->tpending[t]++;
 }
@@ -525,8 +583,9 @@ int pv_timers_enable(pv_bag *b, int on)
     if (on && !b->timers_on) {
         for (t = 0; t < PV_NTIMERS; t++)
             for (k = 0; k < PV_TPOOL; k++) {
-                HIPCHK(hipEventCreate(&b->tev[t][0][k]));
-                HIPCHK(hipEventCreate(&b->tev[t][1][k]));
+                /* timing-only events: no system-scope fence (cache writeback) per record */
+                HIPCHK(hipEventCreateWithFlags(&b->tev[t][0][k], hipEventDisableSystemFence));
+                HIPCHK(hipEventCreateWithFlags(&b->tev[t][1][k], hipEventDisableSystemFence));
             }
     } else if (!on && b->timers_on) {
         pv_timers_flush(b);
@@ -536,7 +595,8 @@ int pv_timers_enable(pv_bag *b, int on)
                 hipEventDestroy(b->tev[t][1][k]);
             }
     }
-    b->timers_on = on ? 1 : 0;
+    b->timers_on = (on > 0) ? on : 0;
+    memset(b->tseen, 0, sizeof b->tseen);
     return PV_OK;
 }
 

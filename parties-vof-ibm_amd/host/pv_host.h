@@ -13,10 +13,13 @@ enum pv_internal_field {
     PV_FI_TMP,                 /* scratch */
     PV_FI_Z,                   /* preconditioned residual of the pressure PCG */
     PV_FI_D2,                  /* second direction buffer of the fused pressure PCG */
+    PV_FI_CHP2,                /* second direction buffer of the biharmonic CG */
+    PV_FI_DU2,                 /* second direction buffers of the momentum PCG */
+    PV_FI_DV2,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
-#define PV_TPOOL 64                /* event pairs per timer between syncs */
+#define PV_TPOOL 256                /* event pairs per timer between syncs */
 
 typedef struct pv_metrics {
     int nx, ny;
@@ -48,7 +51,9 @@ struct pv_bag {
     pv_stats st;
     int ch_chunk, mom_chunk[2], p_chunk;
     /* kernel timers */
-    int timers_on;
+    int timers_on;             /* 0 off, else sample period: time every k-th launch per timer */
+    long long tseen[PV_NTIMERS];
+    int tactive[PV_NTIMERS];
     hipEvent_t tev[PV_NTIMERS][2][PV_TPOOL];
     int tpending[PV_NTIMERS];
     double tms[PV_NTIMERS];
@@ -58,6 +63,9 @@ struct pv_bag {
     double *fftmem;
     pvk_pcg pcg;
     double *pcgmem;
+    pvk_ccg ccg;                   /* fused biharmonic CG work set */
+    pvk_mcg mcg;                   /* fused momentum PCG work set */
+    double *cgmem;
 };
 
 int  pv_set_error(int code, const char *fmt, ...);
@@ -68,6 +76,7 @@ int  pv_sync_scalars_to_device(pv_bag *b);
 int  pv_read_scalars(pv_bag *b);
 double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
+int  pv_cg_create(pv_bag *b);
 void pv_tbegin(pv_bag *b, int t);
 void pv_tend(pv_bag *b, int t);
 int  pv_timers_flush(pv_bag *b);

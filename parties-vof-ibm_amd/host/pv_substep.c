@@ -41,33 +41,25 @@ static int begin(pv_bag *b)
  * has been enqueued; x = C_L, r = CH_RES, p = CH_DIR, Ap = LAP_C. */
 static int ch_cg_loop(pv_bag *b, int stage)
 {
-    double *x = FD(b, PV_F_C_L), *r = FD(b, PV_F_CH_RES), *p = FD(b, PV_F_CH_DIR), *Ap = FD(b, PV_F_LAP_C);
-    double tol = b->ph.ch_tol;
-    int chunk = b->ch_chunk, total = 0;
-    pvk_bc_ch(G(b), p, S(b));
+    double *x = FD(b, PV_F_C_L), *r = FD(b, PV_F_CH_RES), *Ap = FD(b, PV_F_LAP_C);
+    int chunk = b->ch_chunk, n = 0, maxn = b->ph.ch_iter_max + 1;
     for (;;) {
         int k;
-        for (k = 0; k < chunk; k++) {
+        for (k = 0; k < chunk && n <= maxn; k++, n++) {
             pv_tbegin(b, PV_T_CH_APPLY);
-            pvk_ch_apply(G(b), &b->ph, stage, b->d_sc, p, Ap, &b->rd, S(b));
+            pvk_ccg_iter(G(b), &b->ph, stage, b->d_sc, &b->ccg, 0, n, x, r, Ap, S(b));
             pv_tend(b, PV_T_CH_APPLY);
-            pvk_ch_alpha(G(b), b->d_sc, &b->rd, S(b));
             pv_tbegin(b, PV_T_CH_UPDATE);
-            pvk_ch_update(G(b), b->d_sc, x, p, r, Ap, &b->rd, S(b));
+            pvk_ccg_iter(G(b), &b->ph, stage, b->d_sc, &b->ccg, 1, n, x, r, Ap, S(b));
             pv_tend(b, PV_T_CH_UPDATE);
-            pvk_ch_beta(G(b), b->d_sc, tol, &b->rd, S(b));
-            pvk_ch_pupdate(G(b), b->d_sc, r, p, S(b));
-            pvk_bc_ch(G(b), p, S(b));
         }
         LAUNCHCHK();
-        total += chunk;
         RC(pv_read_scalars(b));
         if (b->h_sc->ch_done) break;
-        if (total > b->ph.ch_iter_max + 8)
-            return pv_set_error(PV_EHIP, "biharmonic CG did not stop (device flag)");
+        if (n > maxn) return pv_set_error(PV_EHIP, "biharmonic CG did not stop (device flag)");
         chunk = 2;
     }
-    b->ch_chunk = (b->h_sc->ch_iter > 2) ? b->h_sc->ch_iter - 1 : 2;
+    b->ch_chunk = (b->h_sc->ch_iter > 1) ? b->h_sc->ch_iter + 1 : 2;
     b->st.n_ch[stage] = b->h_sc->ch_iter;
     b->st.ch_res[stage] = b->h_sc->ch_rel;
     pvk_bc_ch(G(b), x, S(b));
@@ -80,10 +72,9 @@ static int ch_step(pv_bag *b)
 {
     int st = b->stage;
     pv_tbegin(b, PV_T_CH_RHS);
-    pvk_ch_rhs_init(G(b), &b->ph, st, b->d_sc, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
-                    FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), &b->rd, S(b));
+    pvk_ccg_init(G(b), &b->ph, st, b->d_sc, &b->ccg, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
+                 FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), S(b));
     pv_tend(b, PV_T_CH_RHS);
-    pvk_ch_finalize_init(G(b), b->d_sc, b->ph.ch_iter_max, b->ph.ch_tol, &b->rd, S(b));
     LAUNCHCHK();
     RC(ch_cg_loop(b, st));
     pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
@@ -156,9 +147,9 @@ int pv_VOF_DIFFUSE_solve_implicit_biharmonic(pv_bag *b, int rhs, int *iters, dou
 {
     RC(begin(b));
     pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
-    pvk_ch_cg_init_from_b(G(b), &b->ph, b->stage, b->d_sc, FD(b, PV_F_C_L), FD(b, rhs), FD(b, PV_F_CH_RES),
-                          FD(b, PV_F_CH_DIR), &b->rd, S(b));
-    pvk_ch_finalize_init(G(b), b->d_sc, b->ph.ch_iter_max, b->ph.ch_tol, &b->rd, S(b));
+    pvk_ch_sub_bih(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, rhs), S(b));
+    pvk_ccg_init(G(b), &b->ph, b->stage, b->d_sc, &b->ccg, FD(b, PV_F_C_L), NULL, NULL, NULL, FD(b, rhs),
+                 FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), S(b));
     LAUNCHCHK();
     RC(ch_cg_loop(b, b->stage));
     if (iters) *iters = b->h_sc->ch_iter;
@@ -286,40 +277,40 @@ int pv_matVec(pv_bag *b, int out, int in, int component)
 }
 
 /* Velocity_solve_cg (lsolver/msolve_cg.c:525-722) */
-static int mom_solve(pv_bag *b, int comp)
+/* Velocity_solve_cg (msolve_cg.c:525-722) for the components in mask
+ * (1 = u, 2 = v): both run in the same launches and stop independently. */
+static int mom_solve(pv_bag *b, int mask)
 {
-    int u = (comp == 'u'), c = u ? 0 : 1, st = b->stage, chunk = b->mom_chunk[c], total = 0;
-    double *x = FD(b, u ? PV_F_U : PV_F_V), *rhs = FD(b, u ? PV_F_U_RHS : PV_F_V_RHS);
-    double *r = FD(b, u ? PV_F_U_R : PV_F_V_R), *d = FD(b, u ? PV_F_U_D : PV_F_V_D);
-    double *Ad = FD(b, u ? PV_F_U_AD : PV_F_V_AD), *dg = FD(b, u ? PV_FI_DG_U : PV_FI_DG_V);
-    int kind = u ? 4 : 6;
-    pvk_mom_cg_init(G(b), &b->ph, st, b->d_sc, comp, x, rhs, FD(b, PV_F_RHO), FD(b, PV_F_MU), r, d, Ad, dg,
-                    &b->rd, S(b));
-    pvk_mom_scalar(G(b), b->d_sc, comp, 0, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
+    int st = b->stage, n = 0, maxn = b->ph.CG_MAXIT + 1, c, chunk;
+    pvk_mcg w = b->mcg;
+    for (c = 0; c < 2; c++)
+        if (!(mask & (1 << c))) w.nb[c] = 0;
+    chunk = (mask & 1) ? b->mom_chunk[0] : b->mom_chunk[1];
+    if ((mask & 2) && b->mom_chunk[1] > chunk) chunk = b->mom_chunk[1];
+    pvk_mcg_init(G(b), &b->ph, st, b->d_sc, &w, S(b));
     for (;;) {
-        int k;
-        for (k = 0; k < chunk; k++) {
-            pvk_ghost_multi(G(b), d, NULL, NULL, NULL, 1, kind, S(b));
+        int k, done = 1;
+        for (k = 0; k < chunk && n <= maxn; k++, n++) {
             pv_tbegin(b, PV_T_MOM_APPLY);
-            pvk_mom_cg_apply(G(b), &b->ph, st, b->d_sc, comp, d, FD(b, PV_F_RHO), FD(b, PV_F_MU), Ad, &b->rd,
-                             S(b));
+            pvk_mcg_iter(G(b), &b->ph, st, b->d_sc, &w, 0, n, S(b));
             pv_tend(b, PV_T_MOM_APPLY);
-            pvk_mom_scalar(G(b), b->d_sc, comp, 1, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
             pv_tbegin(b, PV_T_MOM_UPDATE);
-            pvk_mom_cg_update(G(b), b->d_sc, comp, x, d, r, Ad, dg, &b->rd, S(b));
+            pvk_mcg_iter(G(b), &b->ph, st, b->d_sc, &w, 1, n, S(b));
             pv_tend(b, PV_T_MOM_UPDATE);
-            pvk_mom_scalar(G(b), b->d_sc, comp, 2, b->ph.CG_MAXIT, b->ph.CG_ETOL, &b->rd, S(b));
-            pvk_mom_cg_dupdate(G(b), b->d_sc, comp, r, dg, d, S(b));
         }
         LAUNCHCHK();
-        total += chunk;
         RC(pv_read_scalars(b));
-        if (b->h_sc->m_done[c]) break;
-        if (total > b->ph.CG_MAXIT + 8) return pv_set_error(PV_EHIP, "momentum CG did not stop (device flag)");
+        for (c = 0; c < 2; c++)
+            if ((mask & (1 << c)) && !b->h_sc->m_done[c]) done = 0;
+        if (done) break;
+        if (n > maxn) return pv_set_error(PV_EHIP, "momentum CG did not stop (device flag)");
         chunk = 2;
     }
-    b->mom_chunk[c] = (b->h_sc->m_iter[c] > 2) ? b->h_sc->m_iter[c] - 1 : 2;
-    if (u) b->st.n_u[st] = b->h_sc->m_iter[c]; else b->st.n_v[st] = b->h_sc->m_iter[c];
+    for (c = 0; c < 2; c++) {
+        if (!(mask & (1 << c))) continue;
+        b->mom_chunk[c] = (b->h_sc->m_iter[c] > 1) ? b->h_sc->m_iter[c] + 1 : 2;
+        if (c == 0) b->st.n_u[st] = b->h_sc->m_iter[0]; else b->st.n_v[st] = b->h_sc->m_iter[1];
+    }
     return PV_OK;
 }
 
@@ -327,7 +318,7 @@ int pv_Velocity_solve_cg(pv_bag *b, int component, int *iters)
 {
     RC(begin(b));
     if (component != 'u' && component != 'v') return pv_set_error(PV_EINVAL, "component must be 'u' or 'v'");
-    RC(mom_solve(b, component));
+    RC(mom_solve(b, component == 'u' ? 1 : 2));
     if (iters) *iters = b->h_sc->m_iter[component == 'u' ? 0 : 1];
     return PV_OK;
 }
@@ -547,8 +538,7 @@ int pv_Temporal_int_all_the_equations(pv_bag *b)
     pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
                    FD(b, PV_F_FSY_NEW), S(b));
     LAUNCHCHK();
-    RC(mom_solve(b, 'u'));
-    RC(mom_solve(b, 'v'));
+    RC(mom_solve(b, 3));
     pvk_vel_bc(G(b), FD(b, PV_F_U), 'u', PV_VEL_TYPE_NORMAL, S(b));
     pvk_vel_bc(G(b), FD(b, PV_F_V), 'v', PV_VEL_TYPE_NORMAL, S(b));
     LAUNCHCHK();
