@@ -58,8 +58,8 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, i
     if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
     double sbb = 0.0, srr = 0.0;
-    for (int c = threadIdx.x; c < ncell; c += NTH) {
-        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+    for (int j = j0; j < j1; j++)
+    for (int i = threadIdx.x; i < NXM; i += NTH) {
         double rc = weno_rhs_at(g, ph, C, u, v, i, j);
         rc += diffusion_at(g, ph, C, i, j);
         double cc = AT(C, i, j);
@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int
     if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
     double sbb = 0.0, srr = 0.0;
-    for (int c = threadIdx.x; c < ncell; c += NTH) {
-        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+    for (int j = j0; j < j1; j++)
+    for (int i = threadIdx.x; i < NXM; i += NTH) {
         double cc = AT(C, i, j);
         double Ap = a_dt * cc + ph.bih * bilap_at(g, C, i, j);
         double bb = AT(bv, i, j);
@@ -149,25 +149,24 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
         const int ia = wrapc(a, NXM), jb = chy(b, NYM);
         return (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
     };
-    const int j0 = band * w.band, j1 = min(NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    const int j0 = band * w.band, j1 = min(NYM, j0 + w.band);
     double acc = 0.0;
-    for (int base = 0; base < ncell; base += NTH * CPT) {
+    for (int j = j0; j < j1; j++)
+    for (int base = 0; base < NXM; base += NTH * CPT) {
         double pc[CPT], ap[CPT];
 #pragma unroll
         for (int k = 0; k < CPT; k++) {
-            const int c = base + k * NTH + threadIdx.x;
+            const int i = base + k * NTH + (int)threadIdx.x;
             pc[k] = 0.0; ap[k] = 0.0;
-            if (c < ncell) {
-                const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+            if (i < NXM) {
                 pc[k] = pv(i, j);
                 ap[k] = a_dt * pc[k] + ph.bih * bilap_fn(g, pv, i, j);
             }
         }
 #pragma unroll
         for (int k = 0; k < CPT; k++) {
-            const int c = base + k * NTH + threadIdx.x;
-            if (c < ncell) {
-                const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+            const int i = base + k * NTH + (int)threadIdx.x;
+            if (i < NXM) {
                 AT(pnew, i, j) = pc[k];
                 AT(Ap, i, j) = ap[k];
                 acc += ch_w(g, i, j, pc[k], ap[k]);
@@ -193,10 +192,10 @@ __global__ void __launch_bounds__(NTH) k_ccg_update(pvk_grid g, pvk_scal *sc, pv
         return;
     }
     const double alpha = sc->ch_rzb[n & 1] / pAp;
-    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
+    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band);
     double acc = 0.0;
-    for (int c = threadIdx.x; c < ncell; c += NTH) {
-        const int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
+    for (int j = j0; j < j1; j++)
+    for (int i = threadIdx.x; i < NXM; i += NTH) {
         AT(x, i, j) += alpha * AT(p, i, j);
         double rr = AT(r, i, j) - alpha * AT(Ap, i, j);
         AT(r, i, j) = rr;
@@ -263,8 +262,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
     const double *x = w.x[m.comp], *rhs = w.rhs[m.comp];
     double *r = w.r[m.comp], *dg = w.dg[m.comp], *d = w.d[m.comp][0], *Ad = w.Ad[m.comp];
     double acc = 0.0;
-    for (int c = threadIdx.x; c < m.ncell; c += NTH) {
-        const int j = m.j0 + c / NXM, i = m.i0 + c - (c / NXM) * NXM;
+    for (int j = m.j0; j < m.j1; j++)
+    for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) {
         double ad = mom_op(g, ph, alpha_k, dt, w.rho, w.mu, x, comp, i, j);
         double rr = AT(rhs, i, j) - ad;
         double dgv = mom_diag(g, ph, alpha_k, dt, w.rho, w.mu, comp, i, j) + 1e-14;
@@ -366,14 +365,14 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         return (n == 0) ? AT(dwall, ia, b) : AT(r, ia, b) / AT(dg, ia, b) + beta * AT(dold, ia, b);
     };
     double acc = 0.0;
-    for (int base = 0; base < m.ncell; base += NTH * CPT) {
+    for (int j = m.j0; j < m.j1; j++)
+    for (int base = 0; base < NXM; base += NTH * CPT) {
         double dc[CPT], ad[CPT];
 #pragma unroll
         for (int k = 0; k < CPT; k++) {
-            const int q = base + k * NTH + threadIdx.x;
+            const int q = base + k * NTH + (int)threadIdx.x, i = m.i0 + q;
             dc[k] = 0.0; ad[k] = 0.0;
-            if (q < m.ncell) {
-                const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+            if (q < NXM) {
                 dc[k] = dv(i, j);
                 ad[k] = mom_op_vals(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, dc[k], dv(i + 1, j), dv(i - 1, j),
                                     dv(i, j + 1), dv(i, j - 1));
@@ -381,9 +380,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         }
 #pragma unroll
         for (int k = 0; k < CPT; k++) {
-            const int q = base + k * NTH + threadIdx.x;
-            if (q < m.ncell) {
-                const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+            const int q = base + k * NTH + (int)threadIdx.x, i = m.i0 + q;
+            if (q < NXM) {
                 if (n > 0) AT(dnew, i, j) = dc[k];
                 AT(Ad, i, j) = ad[k];
                 acc += dc[k] * ad[k];
@@ -409,8 +407,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pv
     double *x = w.x[c], *r = w.r[c];
     const double *d = w.d[c][n & 1], *Ad = w.Ad[c], *dg = w.dg[c];
     double a0 = 0.0, a1 = 0.0;
-    for (int q = threadIdx.x; q < m.ncell; q += NTH) {
-        const int j = m.j0 + q / NXM, i = m.i0 + q - (q / NXM) * NXM;
+    for (int j = m.j0; j < m.j1; j++)
+    for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) {
         AT(x, i, j) += alpha * AT(d, i, j);
         double ri = AT(r, i, j) - alpha * AT(Ad, i, j);
         AT(r, i, j) = ri;

@@ -1,7 +1,7 @@
 // pv_fftk.h -- radix-8/4/2 Stockham FFT of length M (a power of two, 4..2048)
 // held in LDS as separate real and imaginary arrays.
 //
-// Pass with radix R and sub-transform size Ns (Ns = 1, R, R*R', ... ): the
+// Pass with radix R and sub-transform size Ns (Ns = 1, R, R*R', ...): the
 // butterfly bb < M/R reads x[bb + r M/R], r < R, multiplies element r by
 // W^(k r), k = bb mod Ns, W = exp(-+2 pi i / (Ns R)), takes an R-point DFT in
 // registers and writes y[(bb / Ns) Ns R + k + r Ns].  Input and output are
@@ -80,18 +80,21 @@ __host__ __device__ __forceinline__ void sk_dft8(double *vr, double *vi)
     vr[7] = e3r - w3r; vi[7] = e3i - w3i;
 }
 
-// Read butterfly bb of a radix-R pass, twiddle, DFT: v[0..R).
-template <int R, bool INV>
-__host__ __device__ __forceinline__ void sk_load(const double *sr, const double *si, int bb, int M, int Ns,
+// Read butterfly bb of a radix-R pass (R = 2^LR), twiddle, DFT: v[0..R).
+// All sizes are powers of two and passed as base-2 logarithms, so index
+// arithmetic is shifts and masks.
+template <int LR, bool INV>
+__host__ __device__ __forceinline__ void sk_load(const double *sr, const double *si, int bb, int logM, int logNs,
                                                  const double *tw, double *vr, double *vi)
 {
-    const int stride = M / R, k = bb & (Ns - 1), step = k * (M / (Ns * R));
+    constexpr int R = 1 << LR;
+    const int stride = 1 << (logM - LR), k = bb & ((1 << logNs) - 1), step = k << (logM - logNs - LR);
 #pragma unroll
     for (int r = 0; r < R; r++) {
         vr[r] = sr[bb + r * stride];
         vi[r] = si[bb + r * stride];
     }
-    if (Ns > 1) {
+    if (logNs > 0) {
 #pragma unroll
         for (int r = 1; r < R; r++) {
             const int t = r * step;
@@ -106,11 +109,13 @@ __host__ __device__ __forceinline__ void sk_load(const double *sr, const double 
     else sk_dft2<INV>(vr, vi, 0, 1);
 }
 
-template <int R>
-__host__ __device__ __forceinline__ void sk_store(double *sr, double *si, int bb, int Ns, const double *vr,
+template <int LR>
+__host__ __device__ __forceinline__ void sk_store(double *sr, double *si, int bb, int logNs, const double *vr,
                                                   const double *vi)
 {
-    const int base = (bb / Ns) * Ns * R + (bb & (Ns - 1));
+    constexpr int R = 1 << LR;
+    const int Ns = 1 << logNs;
+    const int base = ((bb >> logNs) << (logNs + LR)) + (bb & (Ns - 1));
 #pragma unroll
     for (int r = 0; r < R; r++) {
         sr[base + r * Ns] = vr[r];
@@ -120,33 +125,35 @@ __host__ __device__ __forceinline__ void sk_store(double *sr, double *si, int bb
 
 // One radix-R pass over a transform whose T threads have lane ids 0..T-1;
 // every thread of the block calls it (barriers inside).  bb >= M / R idle.
-template <int R, bool INV>
-__device__ __forceinline__ void sk_pass(double *sr, double *si, int M, int Ns, const double *tw, int lane, int T)
+template <int LR, bool INV>
+__device__ __forceinline__ void sk_pass(double *sr, double *si, int logM, int logNs, const double *tw, int lane,
+                                        int T)
 {
-    constexpr int U = 8 / R;
+    constexpr int U = 8 >> LR;
+    const int nbf = 1 << (logM - LR);
     double vr[8], vi[8];
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int bb = lane + u * T;
-        if (bb < M / R) sk_load<R, INV>(sr, si, bb, M, Ns, tw, vr + u * R, vi + u * R);
+        if (bb < nbf) sk_load<LR, INV>(sr, si, bb, logM, logNs, tw, vr + (u << LR), vi + (u << LR));
     }
     __syncthreads();
 #pragma unroll
     for (int u = 0; u < U; u++) {
         const int bb = lane + u * T;
-        if (bb < M / R) sk_store<R>(sr, si, bb, Ns, vr + u * R, vi + u * R);
+        if (bb < nbf) sk_store<LR>(sr, si, bb, logNs, vr + (u << LR), vi + (u << LR));
     }
     __syncthreads();
 }
 
-// Whole transform (logM = log2 M); the caller has synchronised after filling
-// sr / si and reads the result after this returns.
+// Whole transform of length M = 2^logM; the caller has synchronised after
+// filling sr / si and reads the result after this returns.
 template <bool INV>
-__device__ __forceinline__ void sk_fft(double *sr, double *si, int M, int logM, const double *tw, int lane, int T)
+__device__ __forceinline__ void sk_fft(double *sr, double *si, int logM, const double *tw, int lane, int T)
 {
-    int Ns = 1, rem = logM;
-    for (; rem >= 3; rem -= 3, Ns *= 8) sk_pass<8, INV>(sr, si, M, Ns, tw, lane, T);
-    if (rem == 2) sk_pass<4, INV>(sr, si, M, Ns, tw, lane, T);
-    else if (rem == 1) sk_pass<2, INV>(sr, si, M, Ns, tw, lane, T);
+    int logNs = 0, rem = logM;
+    for (; rem >= 3; rem -= 3, logNs += 3) sk_pass<3, INV>(sr, si, logM, logNs, tw, lane, T);
+    if (rem == 2) sk_pass<2, INV>(sr, si, logM, logNs, tw, lane, T);
+    else if (rem == 1) sk_pass<1, INV>(sr, si, logM, logNs, tw, lane, T);
 }
 #endif

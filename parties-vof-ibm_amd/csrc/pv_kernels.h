@@ -76,6 +76,7 @@ typedef struct pvk_fft {
 /* Work set of the fused FFT-preconditioned PCG (csrc/pv_pcg.hip). */
 typedef struct pvk_pcg {
     double *phi, *r, *z, *d[2], *Ad;  /* fields, cell (0,0); d double-buffered */
+    double *bx, *by;                  /* operator face coefficients (per solve) */
     const double *rhs, *rho;
     double *specT;                    /* NXM * NYM transposed spectrum */
     double *dad, *rr, *rz[2], *mean;  /* per-block partial sums */

@@ -28,8 +28,9 @@
 
 struct cplx2 { double x, y; };
 
-// threads per transform of length M in the Stockham FFT (pv_fftk.h)
-__device__ __forceinline__ int sk_threads(int M) { return M >= 8 ? M / 8 : 1; }
+// log2 of the threads per transform of length 2^logM in the Stockham FFT
+// (pv_fftk.h): M / 8 threads, at least one
+__device__ __forceinline__ int sk_logT(int logM) { return logM >= 3 ? logM - 3 : 0; }
 
 // X_k, k = 0..M, of a real length-2M sequence from Z = FFT_M(x_{2m} + i x_{2m+1})
 __device__ __forceinline__ cplx2 rpost(const double *sr, const double *si, int base, int M, int k, const double *rt)
@@ -68,8 +69,43 @@ __device__ __forceinline__ cplx2 hcget(const double *row, int N, int k)
     return X;
 }
 
+// Every FFT kernel handles R = 2048 / M rows of M complex points per block
+// (T = M / 8 threads per transform, 256 threads), so each thread owns
+// IT = R * M / 256 = 8 items of the row sweeps; the sweeps load all of a
+// thread's items before storing any, so the loads are in flight together.
+#define IT 8
+#define CPT 4
+
+// Stockham twiddles (M complex) staged in LDS behind the 2 R M row values.
+__device__ __forceinline__ double *stage_twiddles(double *lds, int RM, int M, const double *tw)
+{
+    double *t = lds + 2 * RM;
+    for (int k = threadIdx.x; k < 2 * M; k += NTH) t[k] = tw[k];
+    return t;
+}
+
 // ---------------------------------------------------------------------------
-// K1: d_n = z + beta d_{n-1}, Ad = A d_n, partial <d_n, Ad>
+// Face coefficients of the operator, formed once per solve (rho is fixed):
+// bx(i,j) = 2 / (rho_i + rho_{i+1}) idx^2 for i in [-1, NXM), by(i,j) the
+// same along y for j in [-1, NYM).  Addition commutes exactly, so
+// bx(i-1, j) is bit for bit the west coefficient of cell (i, j).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(NTH) k_pcg_faces(pvk_grid g, pvk_pcg w)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    const int j = (int)blockIdx.x - 1;
+    const double idx2 = g.idx_c[1] * g.idx_c[1], idy2 = g.idy_c[1] * g.idy_c[1];
+    const double *__restrict__ rho = w.rho;
+    for (int i = (int)threadIdx.x - 1; i < NXM; i += NTH) {
+        const double r0 = AT(rho, i, j);
+        AT(w.bx, i, j) = (2.0 / (r0 + AT(rho, i + 1, j))) * idx2;
+        AT(w.by, i, j) = (2.0 / (r0 + AT(rho, i, j + 1))) * idy2;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// K1: d_n = z + beta d_{n-1}, Ad = A d_n, partial <d_n, Ad>.  One block per
+// row j: the y neighbours and row clamps are block-uniform.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk_pcg w, int n)
 {
@@ -78,27 +114,40 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
     const double gam = block_sum(w.rz[n & 1], w.nrz, sh);
     const double beta = (n == 0) ? 0.0 : gam / block_sum(w.rz[(n + 1) & 1], w.nrz, sh);
-    const double *dold = w.d[(n + 1) & 1];
-    double *dnew = w.d[n & 1];
-    const double idx2 = g.idx_c[1] * g.idx_c[1], idy2 = g.idy_c[1] * g.idy_c[1];
-    const int j0 = blockIdx.x * w.band, j1 = min(NYM, j0 + w.band);
+    const double *__restrict__ z = w.z;
+    const double *__restrict__ dold = w.d[(n + 1) & 1];
+    const double *__restrict__ bx = w.bx, *__restrict__ by = w.by;
+    double *__restrict__ dnew = w.d[n & 1];
+    double *__restrict__ Ad = w.Ad;
+    const int j = blockIdx.x;
+    const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
     double acc = 0.0;
-    for (int c = threadIdx.x; c < (j1 - j0) * NXM; c += NTH) {
-        int j = j0 + c / NXM, i = c - (c / NXM) * NXM;
-        int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
-        int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
-#define DN(ii, jj) (n == 0 ? AT(w.z, ii, jj) : AT(w.z, ii, jj) + beta * AT(dold, ii, jj))
-        double d0 = DN(i, j), dE = DN(ie, j), dW = DN(iw, j), dN = DN(i, jn), dS = DN(i, js);
+    for (int base = 0; base < NXM; base += NTH * CPT) {
+        double dc[CPT], ad[CPT];
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+            const int i = base + u * NTH + (int)threadIdx.x;
+            dc[u] = 0.0; ad[u] = 0.0;
+            if (i < NXM) {
+                const int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
+#define DN(ii, jj) (n == 0 ? AT(z, ii, jj) : AT(z, ii, jj) + beta * AT(dold, ii, jj))
+                const double d0 = DN(i, j), dE = DN(ie, j), dW = DN(iw, j), dN = DN(i, jn), dS = DN(i, js);
 #undef DN
-        double r0 = AT(w.rho, i, j);
-        double cxp = (2.0 / (r0 + AT(w.rho, i + 1, j))) * idx2;
-        double cxm = (2.0 / (r0 + AT(w.rho, i - 1, j))) * idx2;
-        double cyp = (2.0 / (r0 + AT(w.rho, i, j + 1))) * idy2;
-        double cym = (2.0 / (r0 + AT(w.rho, i, j - 1))) * idy2;
-        double ad = cxp * (dE - d0) - cxm * (d0 - dW) + cyp * (dN - d0) - cym * (d0 - dS);
-        AT(dnew, i, j) = d0;
-        AT(w.Ad, i, j) = ad;
-        acc += d0 * ad;
+                const double cxp = AT(bx, i, j), cxm = AT(bx, i - 1, j);
+                const double cyp = AT(by, i, j), cym = AT(by, i, j - 1);
+                dc[u] = d0;
+                ad[u] = cxp * (dE - d0) - cxm * (d0 - dW) + cyp * (dN - d0) - cym * (d0 - dS);
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < CPT; u++) {
+            const int i = base + u * NTH + (int)threadIdx.x;
+            if (i < NXM) {
+                AT(dnew, i, j) = dc[u];
+                AT(Ad, i, j) = ad[u];
+                acc += dc[u] * ad[u];
+            }
+        }
     }
     block_store_partial(acc, w.dad, blockIdx.x, sh);
 }
@@ -112,49 +161,67 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int P = g.P, N = g.NXM, M = N / 2, logM = F.logmx, R = w.rrow;
-    double *sr = lds, *si = lds + R * M;
+    const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
+    double *sr = lds, *si = lds + RM;
+    double *tw = stage_twiddles(lds, RM, M, F.wx);
     double alpha = 0.0, mean = 0.0;
     if (n >= 0) alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad, w.ndad, sh);
     else mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
-    const double *d = (n >= 0) ? w.d[n & 1] : (const double *)0;
+    const double *__restrict__ d = (n >= 0) ? w.d[n & 1] : w.rhs;
+    const double *__restrict__ Ad = w.Ad, *__restrict__ rhs = w.rhs;
+    double *__restrict__ phi = w.phi, *__restrict__ r = w.r;
     const int j0 = blockIdx.x * R;
-    double acc = 0.0;
-    for (int t = threadIdx.x; t < R * M; t += NTH) {
-        int q = t / M, m = t - q * M, j = j0 + q;
-        double a = 0.0, b = 0.0;
-        if (j < g.NYM) {
-            int i = 2 * m;
+    double xa[IT], xb[IT], ea[IT], eb[IT], ra[IT], rb[IT];
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
+        xa[u] = xb[u] = ea[u] = eb[u] = ra[u] = rb[u] = 0.0;
+        if (t < RM && j < g.NYM) {
             if (n >= 0) {
-                AT(w.phi, i, j) += alpha * AT(d, i, j);
-                AT(w.phi, i + 1, j) += alpha * AT(d, i + 1, j);
-                a = AT(w.r, i, j) - alpha * AT(w.Ad, i, j);
-                b = AT(w.r, i + 1, j) - alpha * AT(w.Ad, i + 1, j);
+                xa[u] = AT(phi, i, j); xb[u] = AT(phi, i + 1, j);
+                ea[u] = AT(d, i, j);   eb[u] = AT(d, i + 1, j);
+                ra[u] = AT(r, i, j) - alpha * AT(Ad, i, j);
+                rb[u] = AT(r, i + 1, j) - alpha * AT(Ad, i + 1, j);
             } else {
-                AT(w.phi, i, j) = 0.0;
-                AT(w.phi, i + 1, j) = 0.0;
-                a = AT(w.rhs, i, j) - mean;
-                b = AT(w.rhs, i + 1, j) - mean;
+                ra[u] = AT(rhs, i, j) - mean;
+                rb[u] = AT(rhs, i + 1, j) - mean;
             }
-            AT(w.r, i, j) = a;
-            AT(w.r, i + 1, j) = b;
-            acc += a * a + b * b;
         }
-        sr[q * M + m] = a; si[q * M + m] = b;
+    }
+    double acc = 0.0;
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
+        if (t < RM && j < g.NYM) {
+            AT(phi, i, j) = (n >= 0) ? xa[u] + alpha * ea[u] : 0.0;
+            AT(phi, i + 1, j) = (n >= 0) ? xb[u] + alpha * eb[u] : 0.0;
+            AT(r, i, j) = ra[u];
+            AT(r, i + 1, j) = rb[u];
+            acc += ra[u] * ra[u] + rb[u] * rb[u];
+        }
+        if (t < RM) { sr[t] = ra[u]; si[t] = rb[u]; }
     }
     block_store_partial(acc, w.rr, blockIdx.x, sh);
     {
-        const int T = sk_threads(M), q = threadIdx.x / T;
-        sk_fft<false>(sr + q * M, si + q * M, M, logM, F.wx, threadIdx.x - q * T, T);
+        const int lT = sk_logT(logM), q = threadIdx.x >> lT;
+        sk_fft<false>(sr + q * M, si + q * M, logM, tw, threadIdx.x & ((1 << lT) - 1), 1 << lT);
     }
     const double scale = 1.0 / (double)N;
-    for (int t = threadIdx.x; t < R * (M + 1); t += NTH) {
-        int q = t / (M + 1), k = t - q * (M + 1), j = j0 + q;
-        if (j >= g.NYM) continue;
-        cplx2 X = rpost(sr, si, q * M, M, k, F.rtx);
-        double *row = F.spec + (size_t)j * N;
-        row[k] = X.x * scale;
-        if (k > 0 && k < M) row[N - k] = X.y * scale;
+    // X_k, k < M (and X_M for each row below): halfcomplex row of spec
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logM, k = t & (M - 1), j = j0 + q;
+        if (t < RM && j < g.NYM) {
+            cplx2 X = rpost(sr, si, q * M, M, k, F.rtx);
+            double *row = F.spec + (size_t)j * N;
+            row[k] = X.x * scale;
+            if (k > 0) row[N - k] = X.y * scale;
+        }
+    }
+    if ((int)threadIdx.x < R && j0 + (int)threadIdx.x < g.NYM) {
+        const int q = threadIdx.x;
+        cplx2 X = rpost(sr, si, q * M, M, M, F.rtx);
+        F.spec[(size_t)(j0 + q) * N + M] = X.x * scale;
     }
 }
 
@@ -162,8 +229,9 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
 // K3 / K5: 32 x 32 tiled transpose of an (rows x cols) row-major array;
 // with check != 0 the stop test of iteration n runs first (all blocks).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, const double *in, double *out, int rows,
-                                                  int cols, int check, int n, double tol, int maxit)
+__global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, const double *__restrict__ in,
+                                                  double *__restrict__ out, int rows, int cols, int check, int n,
+                                                  double tol, int maxit)
 {
     __shared__ double tile[32][33];
     __shared__ double sh[NTH];
@@ -184,85 +252,119 @@ __global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, cons
     }
     const int tx = threadIdx.x & 31, ty = threadIdx.x >> 5;   // 32 x 8
     const int c0 = blockIdx.x * 32, r0 = blockIdx.y * 32;
-    for (int k = 0; k < 32; k += 8) {
-        int r = r0 + ty + k, c = c0 + tx;
-        if (r < rows && c < cols) tile[ty + k][tx] = in[(size_t)r * cols + c];
+    double v[4];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int r = r0 + ty + 8 * k, c = c0 + tx;
+        v[k] = (r < rows && c < cols) ? in[(size_t)r * cols + c] : 0.0;
     }
+#pragma unroll
+    for (int k = 0; k < 4; k++) tile[ty + 8 * k][tx] = v[k];
     __syncthreads();
-    for (int k = 0; k < 32; k += 8) {
-        int r = c0 + ty + k, c = r0 + tx;      // out is cols x rows
-        if (r < cols && c < rows) out[(size_t)r * rows + c] = tile[tx][ty + k];
+#pragma unroll
+    for (int k = 0; k < 4; k++) {
+        int r = c0 + ty + 8 * k, c = r0 + tx;      // out is cols x rows
+        if (r < cols && c < rows) out[(size_t)r * rows + c] = tile[tx][ty + 8 * k];
     }
 }
 
 // ---------------------------------------------------------------------------
 // K4: per row of specT (one halfcomplex x-index i, length NYM along y):
-// DCT-II, multiply by -1/(lamx_i + lamy_k) ((0,0) -> 0), DCT-III
+// DCT-II, multiply by -1/(lamx_i + lamy_k) ((0,0) -> 0), DCT-III.
+// DCT-II of x_n as a real FFT of v_nn = x_vmap(nn) (evens ascending, odds
+// descending); the row is read and written contiguously and permuted in LDS.
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ int vmap2(int nn, int N) { return (nn < N / 2) ? 2 * nn : 2 * (N - 1 - nn) + 1; }
+// LDS slot (sr or si, index) of row element n under the even/odd reordering
+__device__ __forceinline__ int dct_slot(int n, int N)
+{
+    const int nn = (n & 1) ? N - 1 - (n >> 1) : (n >> 1);
+    return nn;                       // complex index nn >> 1, part nn & 1
+}
 
-#define KST 8
 __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, double *specT)
 {
     extern __shared__ double lds[];
     if (sc && sc->p_done) return;
-    const int N = g.NYM, M = N / 2, logM = F.logmy, R = w.rdct, NX = g.NXM;
-    double *sr = lds, *si = lds + R * M;
+    const int logM = F.logmy, M = 1 << logM, logN = logM + 1, N = 2 * M, R = w.rdct, NX = g.NXM, RM = R * M;
+    double *sr = lds, *si = lds + RM;
+    double *tw = stage_twiddles(lds, RM, M, F.wy);
     const int i0 = blockIdx.x * R;
-    for (int t = threadIdx.x; t < R * M; t += NTH) {
-        int q = t / M, m = t - q * M, i = i0 + q;
-        double a = 0.0, b = 0.0;
-        if (i < NX) {
-            const double *row = specT + (size_t)i * N;
-            a = row[vmap2(2 * m, N)];
-            b = row[vmap2(2 * m + 1, N)];
+    // R rows of N reals = 2 RM values: 2 IT per thread, contiguous per row
+    {
+        double v[2 * IT];
+#pragma unroll
+        for (int u = 0; u < 2 * IT; u++) {
+            const int t = threadIdx.x + u * NTH, q = t >> logN, nidx = t & (N - 1), i = i0 + q;
+            v[u] = (t < 2 * RM && i < NX) ? specT[(size_t)i * N + nidx] : 0.0;
         }
-        sr[q * M + m] = a; si[q * M + m] = b;
+#pragma unroll
+        for (int u = 0; u < 2 * IT; u++) {
+            const int t = threadIdx.x + u * NTH, q = t >> logN, nidx = t & (N - 1);
+            if (t < 2 * RM) {
+                const int nn = dct_slot(nidx, N);
+                ((nn & 1) ? si : sr)[q * M + (nn >> 1)] = v[u];
+            }
+        }
     }
     __syncthreads();
-    const int T = sk_threads(M), qq = threadIdx.x / T, lane = threadIdx.x - qq * T;
-    sk_fft<false>(sr + qq * M, si + qq * M, M, logM, F.wy, lane, T);
-    const int npair = M / 2 + 1;
-    cplx2 Ya[KST], Yb[KST];
-    int cnt = 0;
-    for (int t = threadIdx.x; t < R * npair && cnt < KST; t += NTH, cnt++) {
-        int q = t / npair, k = t - q * npair, i = i0 + q;
-        int kk[2] = {k, M - k};
+    const int lT = sk_logT(logM), T = 1 << lT, qq = threadIdx.x >> lT, lane = threadIdx.x & (T - 1);
+    sk_fft<false>(sr + qq * M, si + qq * M, logM, tw, lane, T);
+    // spectral step on the pairs (k, M - k): k < M/2 as IT/2 items per
+    // thread, k = M/2 by threads t < R
+    const int logH = logM - 1, H = M >> 1;
+    cplx2 Ya[IT / 2 + 1], Yb[IT / 2 + 1];
+    auto pair = [&](int q, int k, cplx2 &ya, cplx2 &yb) {
+        const int i = i0 + q;
+        const int kk[2] = {k, M - k};
         cplx2 Xp[2];
+        const double lx = (i < NX) ? F.lamx[i] : 1.0;
+#pragma unroll
         for (int h = 0; h < 2; h++) {
-            int kv = kk[h];
+            const int kv = kk[h];
             cplx2 V = rpost(sr, si, q * M, M, kv, F.rty);
             double c = F.dcty[2 * kv], s = F.dcty[2 * kv + 1];
             double Ck = c * V.x + s * V.y;
             double Sk = c * V.y - s * V.x;
             double CNk = -Sk;
-            double lx = (i < NX) ? F.lamx[i] : 1.0;
             double gk = (kv == 0 && i == 0) ? 0.0 : -1.0 / (lx + F.lamy[kv]);
             double gN = (kv == 0) ? 0.0 : -1.0 / (lx + F.lamy[N - kv]);
             double Cp = Ck * gk, CNp = (kv == 0) ? 0.0 : CNk * gN;
             Xp[h].x = c * Cp + s * CNp;
             Xp[h].y = s * Cp - c * CNp;
         }
-        Ya[cnt] = rpre(Xp[0], Xp[1], k, F.rty);
-        Yb[cnt] = rpre(Xp[1], Xp[0], M - k, F.rty);
+        ya = rpre(Xp[0], Xp[1], k, F.rty);
+        yb = rpre(Xp[1], Xp[0], M - k, F.rty);
+    };
+#pragma unroll
+    for (int u = 0; u < IT / 2; u++) {
+        const int t = threadIdx.x + u * NTH;
+        if (t < R * H) pair(t >> logH, t & (H - 1), Ya[u], Yb[u]);
+    }
+    if ((int)threadIdx.x < R) pair(threadIdx.x, H, Ya[IT / 2], Yb[IT / 2]);
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < IT / 2; u++) {
+        const int t = threadIdx.x + u * NTH;
+        if (t < R * H) {
+            const int q = t >> logH, k = t & (H - 1);
+            sr[q * M + k] = Ya[u].x; si[q * M + k] = Ya[u].y;
+            if (k > 0) { sr[q * M + M - k] = Yb[u].x; si[q * M + M - k] = Yb[u].y; }
+        }
+    }
+    if ((int)threadIdx.x < R) {
+        const int q = threadIdx.x;
+        sr[q * M + H] = Ya[IT / 2].x; si[q * M + H] = Ya[IT / 2].y;
     }
     __syncthreads();
-    cnt = 0;
-    for (int t = threadIdx.x; t < R * npair && cnt < KST; t += NTH, cnt++) {
-        int q = t / npair, k = t - q * npair;
-        if (k < M) { sr[q * M + k] = Ya[cnt].x; si[q * M + k] = Ya[cnt].y; }
-        int k2 = M - k;
-        if (k2 < M && k2 != k) { sr[q * M + k2] = Yb[cnt].x; si[q * M + k2] = Yb[cnt].y; }
-    }
-    __syncthreads();
-    sk_fft<true>(sr + qq * M, si + qq * M, M, logM, F.wy, lane, T);
+    sk_fft<true>(sr + qq * M, si + qq * M, logM, tw, lane, T);
     const double invN = 1.0 / (double)N;
-    for (int t = threadIdx.x; t < R * M; t += NTH) {
-        int q = t / M, m = t - q * M, i = i0 + q;
-        if (i >= NX) continue;
-        double *row = specT + (size_t)i * N;
-        row[vmap2(2 * m, N)] = sr[q * M + m] * invN;
-        row[vmap2(2 * m + 1, N)] = si[q * M + m] * invN;
+#pragma unroll
+    for (int u = 0; u < 2 * IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logN, nidx = t & (N - 1), i = i0 + q;
+        if (t < 2 * RM && i < NX) {
+            const int nn = dct_slot(nidx, N);
+            specT[(size_t)i * N + nidx] = ((nn & 1) ? si : sr)[q * M + (nn >> 1)] * invN;
+        }
     }
 }
 
@@ -274,31 +376,51 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int P = g.P, N = g.NXM, M = N / 2, logM = F.logmx, R = w.rrow;
-    double *sr = lds, *si = lds + R * M;
+    const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
+    double *sr = lds, *si = lds + RM;
+    double *tw = stage_twiddles(lds, RM, M, F.wx);
+    const double *__restrict__ r = w.r;
+    double *__restrict__ z = w.z;
     const int j0 = blockIdx.x * R;
-    for (int t = threadIdx.x; t < R * M; t += NTH) {
-        int q = t / M, k = t - q * M, j = j0 + q;
-        cplx2 Y = {0.0, 0.0};
-        if (j < g.NYM) {
-            const double *row = F.spec + (size_t)j * N;
-            Y = rpre(hcget(row, N, k), hcget(row, N, M - k), k, F.rtx);
+    {
+        cplx2 Y[IT];
+#pragma unroll
+        for (int u = 0; u < IT; u++) {
+            const int t = threadIdx.x + u * NTH, q = t >> logM, k = t & (M - 1), j = j0 + q;
+            Y[u].x = 0.0; Y[u].y = 0.0;
+            if (t < RM && j < g.NYM) {
+                const double *row = F.spec + (size_t)j * N;
+                Y[u] = rpre(hcget(row, N, k), hcget(row, N, M - k), k, F.rtx);
+            }
         }
-        sr[q * M + k] = Y.x; si[q * M + k] = Y.y;
+#pragma unroll
+        for (int u = 0; u < IT; u++) {
+            const int t = threadIdx.x + u * NTH;
+            if (t < RM) { sr[t] = Y[u].x; si[t] = Y[u].y; }
+        }
     }
     __syncthreads();
     {
-        const int T = sk_threads(M), q = threadIdx.x / T;
-        sk_fft<true>(sr + q * M, si + q * M, M, logM, F.wx, threadIdx.x - q * T, T);
+        const int lT = sk_logT(logM), q = threadIdx.x >> lT;
+        sk_fft<true>(sr + q * M, si + q * M, logM, tw, threadIdx.x & ((1 << lT) - 1), 1 << lT);
+    }
+    double ra[IT], rb[IT];
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q;
+        ra[u] = rb[u] = 0.0;
+        if (t < RM && j < g.NYM) { ra[u] = AT(r, 2 * m, j); rb[u] = AT(r, 2 * m + 1, j); }
     }
     double acc = 0.0;
-    for (int t = threadIdx.x; t < R * M; t += NTH) {
-        int q = t / M, m = t - q * M, j = j0 + q;
-        if (j >= g.NYM) continue;
-        double a = sr[q * M + m], b = si[q * M + m];
-        AT(w.z, 2 * m, j) = a;
-        AT(w.z, 2 * m + 1, j) = b;
-        acc += AT(w.r, 2 * m, j) * a + AT(w.r, 2 * m + 1, j) * b;
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q;
+        if (t < RM && j < g.NYM) {
+            const double a = sr[t], b = si[t];
+            AT(z, 2 * m, j) = a;
+            AT(z, 2 * m + 1, j) = b;
+            acc += ra[u] * a + rb[u] * b;
+        }
     }
     block_store_partial(acc, w.rz[(n + 1) & 1], blockIdx.x, sh);
 }
@@ -318,12 +440,14 @@ __global__ void __launch_bounds__(NTH) k_pcg_mean(pvk_grid g, pvk_scal *sc, pvk_
     block_store_partial(acc, w.mean, blockIdx.x, sh);
 }
 
-static size_t lds_rows(int R, int N) { return (size_t)R * (size_t)(N / 2) * 2 * sizeof(double); }
+// R rows of N/2 complex values + the N/2 complex twiddles
+static size_t lds_rows(int R, int N) { return ((size_t)R * (size_t)(N / 2) * 2 + (size_t)N) * sizeof(double); }
 
 extern "C" void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, double tol,
                               hipStream_t s)
 {
     int nband = (g->NYM + w->band - 1) / w->band, nrow = (g->NYM + w->rrow - 1) / w->rrow;
+    hipLaunchKernelGGL(k_pcg_faces, dim3(g->NYM + 1), dim3(NTH), 0, s, *g, *w);
     hipLaunchKernelGGL(k_pcg_mean, dim3(nband), dim3(NTH), 0, s, *g, sc, *w);
     hipLaunchKernelGGL(k_pcg_update_fwd, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, -1);
     (void)tol;
@@ -338,7 +462,7 @@ extern "C" void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
     int nband = (g->NYM + w->band - 1) / w->band, nrow = (g->NYM + w->rrow - 1) / w->rrow;
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(k_pcg_apply, dim3(nband), dim3(NTH), 0, s, *g, sc, *w, n);
+        hipLaunchKernelGGL(k_pcg_apply, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, *w, n);
         break;
     case 1:
         hipLaunchKernelGGL(k_pcg_update_fwd, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);

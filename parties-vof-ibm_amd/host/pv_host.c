@@ -495,7 +495,9 @@ int pv_fft_create(pv_bag *b)
         rdct = 256 / ((ny / 2 >= 8) ? ny / 16 : 1);
         nband = (ny + band - 1) / band;
         nrow = (ny + rrow - 1) / rrow;
-        np = (nband > nrow) ? nband : nrow;
+        np = ny;                       /* the operator kernel stores one partial per row */
+        if (nband > np) np = nband;
+        if (nrow > np) np = nrow;
         if (hipMalloc((void **)&pm, sizeof(double) * ((size_t)nx * ny + 5 * (size_t)np)) != hipSuccess)
             return pv_set_error(PV_ENOMEM, "PCG work allocation failed");
         b->pcgmem = pm;
@@ -506,12 +508,13 @@ int pv_fft_create(pv_bag *b)
         b->pcg.rz[0] = b->pcg.rr + np;
         b->pcg.rz[1] = b->pcg.rz[0] + np;
         b->pcg.mean = b->pcg.rz[1] + np;
-        b->pcg.ndad = nband; b->pcg.nmean = nband;
+        b->pcg.ndad = ny; b->pcg.nmean = nband;
         b->pcg.nrr = nrow; b->pcg.nrz = nrow;
         b->pcg.band = band; b->pcg.rrow = rrow; b->pcg.rdct = rdct;
         b->pcg.phi = b->f[PV_F_PHI]; b->pcg.r = b->f[PV_F_P_RES]; b->pcg.z = b->f[PV_FI_Z];
         b->pcg.d[0] = b->f[PV_F_P_D]; b->pcg.d[1] = b->f[PV_FI_D2]; b->pcg.Ad = b->f[PV_F_P_AD];
         b->pcg.rhs = b->f[PV_F_P_RHS]; b->pcg.rho = b->f[PV_F_RHO];
+        b->pcg.bx = b->f[PV_FI_PBX]; b->pcg.by = b->f[PV_FI_PBY];
     }
     b->fft.twx = q; q += n_twx;
     b->fft.twy = q; q += n_twy;

@@ -16,6 +16,8 @@ enum pv_internal_field {
     PV_FI_CHP2,                /* second direction buffer of the biharmonic CG */
     PV_FI_DU2,                 /* second direction buffers of the momentum PCG */
     PV_FI_DV2,
+    PV_FI_PBX,                 /* pressure operator face coefficients, x and y */
+    PV_FI_PBY,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
