@@ -94,6 +94,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024)
     ap.add_argument("--solver", choices=["fft", "jacobi"], default="fft")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
+                    help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-n", type=int, default=1024)
     ap.add_argument("--timer-period", type=int, default=16,
@@ -107,7 +109,8 @@ def main():
 
     lib = pvof.load_lib()
     solver = pvof.PV_PSOLVE_FFT_PCG if args.solver == "fft" else pvof.PV_PSOLVE_JACOBI_PCG
-    p = workload_params(args.n, solver)
+    prec = pvof.PV_PREC_MIXED if args.precision == "mixed" else pvof.PV_PREC_FP64
+    p = workload_params(args.n, solver, prec)
     F = pvof.ic(lib, "bubbles", p, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
     s = pvof.Solver(p, lib, device=0)
     s.init_from_F(F)
@@ -145,7 +148,8 @@ def main():
     iters = {key: float(np.mean([sum(st[key]) for st in stats])) for key in ("n_ch", "n_u", "n_v", "n_p")}
 
     out = {
-        "metric": "Mcell-steps/s at 1024^2, plain FP64 (FFT-PCG pressure)",
+        "metric": "Mcell-steps/s at 1024^2, plain FP64 (FFT-PCG pressure)" if prec == pvof.PV_PREC_FP64
+                  else "Mcell-steps/s at 1024^2, mixed FP32/FP64 (FFT-PCG pressure)",
         "value": round(value, 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -155,11 +159,11 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if prec == pvof.PV_PREC_FP64 else "f32+f64",
         "data": "synthetic (seeded bubble field generated in-library)",
         "config": {"workload": f"planar {p.NXM}x{p.NYM} bubbly two-phase RK3 step, rho2/rho1=1e-3, "
                                f"{'FFT-PCG' if solver else 'Jacobi-PCG'} pressure",
-                   "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver,
+                   "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": round(path_gbs / HBM_PEAK_GBS, 4),
         "path_bytes_per_cell_step": round(bytes_step, 1),

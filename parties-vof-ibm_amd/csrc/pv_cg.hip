@@ -46,6 +46,7 @@ __device__ __forceinline__ double bilap_fn(const pvk_grid &g, const F &f, int i,
 
 // Explicit CH right-hand side and CG start (the body of k_ch_rhs_init) in
 // band form: rc -> rn, r = p = b - A C; partials ||b||^2, ||r||^2.
+template <class R>
 __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc,
                                                      const double *__restrict__ C, const double *__restrict__ u,
                                                      const double *__restrict__ v, double *__restrict__ rn,
@@ -60,8 +61,8 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, i
     double sbb = 0.0, srr = 0.0;
     for (int j = j0; j < j1; j++)
     for (int i = threadIdx.x; i < NXM; i += NTH) {
-        double rc = weno_rhs_at(g, ph, C, u, v, i, j);
-        rc += diffusion_at(g, ph, C, i, j);
+        double rc = weno_rhs_at<R>(g, ph, C, u, v, i, j);
+        rc += diffusion_at<R>(g, ph, C, i, j);
         double cc = AT(C, i, j);
         double rv = (stage == 0) ? a_dt * cc + c_GAMB[0] * rc
                                  : a_dt * cc + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
@@ -210,8 +211,12 @@ extern "C" void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, p
 {
     if (b)
         hipLaunchKernelGGL(k_ccg_init_b, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, b, r, p, *w);
+    else if (ph->precision)
+        hipLaunchKernelGGL(k_ccg_init_rhs<float>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p,
+                           *w);
     else
-        hipLaunchKernelGGL(k_ccg_init_rhs, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p, *w);
+        hipLaunchKernelGGL(k_ccg_init_rhs<double>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r,
+                           p, *w);
 }
 
 extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
@@ -251,6 +256,7 @@ __device__ __forceinline__ mom_band mom_locate(const pvk_grid &g, const pvk_mcg 
 }
 
 // r = rhs - A x; dg = diag + EPS; d = r / dg; partial <r, d> (msolve_cg.c:627-645)
+template <class R>
 __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w)
 {
     __shared__ double sh[NTH];
@@ -264,9 +270,9 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
     double acc = 0.0;
     for (int j = m.j0; j < m.j1; j++)
     for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) {
-        double ad = mom_op(g, ph, alpha_k, dt, w.rho, w.mu, x, comp, i, j);
+        double ad = (double)mom_op<R>(g, ph, alpha_k, dt, w.rho, w.mu, x, comp, i, j);
         double rr = AT(rhs, i, j) - ad;
-        double dgv = mom_diag(g, ph, alpha_k, dt, w.rho, w.mu, comp, i, j) + 1e-14;
+        double dgv = (double)mom_diag<R>(g, ph, alpha_k, dt, w.rho, w.mu, comp, i, j) + 1e-14;
         double dd = rr / dgv;
         AT(Ad, i, j) = ad;
         AT(r, i, j) = rr;
@@ -277,48 +283,10 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
     block_store_partial(acc, w.rz[m.comp][0], m.band, sh);
 }
 
-// The Jacobi-PCG operator of mom_op with the five x values supplied.
-__device__ __forceinline__ double mom_op_vals(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
-                                              const double *rho, const double *mu, int comp, int i, int j, double xc,
-                                              double xE, double xW, double xN, double xS)
-{
-    const int P = g.P;
-    const double iRe = ph.iRe;
-    double op = 0.0;
-    if (comp == 0) {
-        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
-        double factor_time = rho_face / (alpha_k * dt);
-        double Ax_val = factor_time * xc;
-        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
-        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
-        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
-        double flux_xp = mu_xp * (xE - xc) * g.idx_u[i];
-        double flux_xm = mu_xm * (xc - xW) * g.idx_u[i - 1];
-        double flux_yp = mu_yp * (xN - xc) * g.idy_c[j];
-        double flux_ym = mu_ym * (xc - xS) * g.idy_c[j - 1];
-        op += iRe * ((flux_xp - flux_xm) * g.idx_c[i - 1]);
-        op += iRe * (flux_yp - flux_ym) * g.idy_v[j];
-        op += 0.0;
-        return Ax_val - op;
-    }
-    double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
-    double factor_time = rho_face / (alpha_k * dt);
-    double Ax_val = factor_time * xc;
-    double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
-    double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
-    double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
-    double flux_xp = mu_xp * (xE - xc) * g.idx_c[i];
-    double flux_xm = mu_xm * (xc - xW) * g.idx_c[i - 1];
-    double flux_yp = mu_yp * (xN - xc) * g.idy_v[j];
-    double flux_ym = mu_ym * (xc - xS) * g.idy_v[j - 1];
-    op += iRe * ((flux_xp - flux_xm) * g.idx_u[i]);
-    op += iRe * (flux_yp - flux_ym) * g.idy_c[j - 1];
-    return Ax_val - op;
-}
-
 // K1: stop test sqrt(r.r / N) < CG_ETOL (msolve_cg.c:691-704), beta =
 // rz_new / (rz + EPS), d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
 // the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
+template <class R>
 __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w,
                                                   int n)
 {
@@ -374,8 +342,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
             dc[k] = 0.0; ad[k] = 0.0;
             if (q < NXM) {
                 dc[k] = dv(i, j);
-                ad[k] = mom_op_vals(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, dc[k], dv(i + 1, j), dv(i - 1, j),
-                                    dv(i, j + 1), dv(i, j - 1));
+                ad[k] = (double)mom_op_vals<R>(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, R(dc[k]), R(dv(i + 1, j)),
+                                               R(dv(i - 1, j)), R(dv(i, j + 1)), R(dv(i, j - 1)));
             }
         }
 #pragma unroll
@@ -422,14 +390,19 @@ __global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pv
 extern "C" void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                              hipStream_t s)
 {
-    hipLaunchKernelGGL(k_mcg_init, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w);
+    if (ph->precision)
+        hipLaunchKernelGGL(k_mcg_init<float>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w);
+    else
+        hipLaunchKernelGGL(k_mcg_init<double>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w);
 }
 
 extern "C" void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                              int half, int n, hipStream_t s)
 {
-    if (half == 0)
-        hipLaunchKernelGGL(k_mcg_apply, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
+    if (half == 0 && ph->precision)
+        hipLaunchKernelGGL(k_mcg_apply<float>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
+    else if (half == 0)
+        hipLaunchKernelGGL(k_mcg_apply<double>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
     else
         hipLaunchKernelGGL(k_mcg_update, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
 }

@@ -341,55 +341,61 @@ extern "C" void pvk_phase_props(const pvk_grid *g, const pvk_phys *ph, const dou
 // psi_LG (VOF_DIFFUSE.c:907-929) and the potential-form capillary force on
 // u and v faces (VOF_DIFFUSE.c:1363-1445).  Thread i handles u-face
 // iu = (i == 0 ? NXM : i) (unknowns 1..NXM) and v-face (i, j), j >= 1.
-__device__ __forceinline__ double psi_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
+template <class R = double>
+__device__ __forceinline__ R psi_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
 {
     const int P = g.P;
-    double c = AT(C, i, j);
-    return c * c * c - 1.5 * c * c + 0.5 * c - ph.Cn * ph.Cn * lap_at(g, C, i, j);
+    R c = R(AT(C, i, j));
+    return c * c * c - R(1.5) * c * c + R(0.5) * c - R(ph.Cn) * R(ph.Cn) * lap_at<R>(g, C, i, j);
 }
 
+// The band test reads the FP64 fields; psi and the force are formed in R.
+template <class R>
 __global__ void k_psi_fsigma(pvk_grid g, pvk_phys ph, const double *C, const double *G, double *psi_out,
                              double *fx, double *fy)
 {
     CELL_IJ
     if (!in) return;
-    const double scale = 2.0 * 6.0 * sqrt(2.0) / (ph.Cn * ph.We + 1e-30);
+    const R scale = R(2.0 * 6.0 * sqrt(2.0) / (ph.Cn * ph.We + 1e-30));
     const double cs = 0.0;
-    double psi_c = psi_at(g, ph, C, i, j);
-    AT(psi_out, i, j) = psi_c;
+    R psi_c = psi_at<R>(g, ph, C, i, j);
+    AT(psi_out, i, j) = (double)psi_c;
     {   // u face iu, between cells iu-1 and iu
         int iu = (i == 0) ? g.NXM : i;
-        double fval = 0.0;
+        R fval = R(0.0);
         if (j < g.NYM) {
             double cl = 0.5 * (AT(C, iu, j) + AT(C, iu - 1, j));
             double cg = 0.5 * (AT(G, iu, j) + AT(G, iu - 1, j));
             if (!(cl < 0.005 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05)) {
-                double psi_e = (iu == g.NXM) ? psi_at(g, ph, C, 0, j) : psi_c;
-                double psi_w = psi_at(g, ph, C, iu - 1, j);
-                double psi_f = 0.5 * (psi_e + psi_w);
-                double gradC = (AT(C, iu, j) - AT(C, iu - 1, j)) * g.idx_c[iu - 1];
+                R psi_e = (iu == g.NXM) ? psi_at<R>(g, ph, C, 0, j) : psi_c;
+                R psi_w = psi_at<R>(g, ph, C, iu - 1, j);
+                R psi_f = R(0.5) * (psi_e + psi_w);
+                R gradC = (R(AT(C, iu, j)) - R(AT(C, iu - 1, j))) * R(g.idx_c[iu - 1]);
                 fval = scale * psi_f * gradC;
             }
         }
-        AT(fx, iu, j) = fval;
+        AT(fx, iu, j) = (double)fval;
     }
     if (j >= 1) {   // v face (i, j), between cells j-1 and j
-        double fval = 0.0;
+        R fval = R(0.0);
         double cl = 0.5 * (AT(C, i, j) + AT(C, i, j - 1));
         double cg = 0.5 * (AT(G, i, j) + AT(G, i, j - 1));
         if (!(cl < 0.005 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05)) {
-            double psi_f = 0.5 * (psi_c + psi_at(g, ph, C, i, j - 1));
-            double gradC = (AT(C, i, j) - AT(C, i, j - 1)) * g.idy_c[j - 1];
+            R psi_f = R(0.5) * (psi_c + psi_at<R>(g, ph, C, i, j - 1));
+            R gradC = (R(AT(C, i, j)) - R(AT(C, i, j - 1))) * R(g.idy_c[j - 1]);
             fval = scale * psi_f * gradC;
         }
-        AT(fy, i, j) = fval;
+        AT(fy, i, j) = (double)fval;
     }
 }
 extern "C" void pvk_psi_fsigma(const pvk_grid *g, const pvk_phys *ph, const double *C, const double *G,
                                double *lap, double *psi, double *fx, double *fy, hipStream_t s)
 {
     (void)lap;
-    hipLaunchKernelGGL(k_psi_fsigma, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, G, psi, fx, fy);
+    if (ph->precision)
+        hipLaunchKernelGGL(k_psi_fsigma<float>, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, G, psi, fx, fy);
+    else
+        hipLaunchKernelGGL(k_psi_fsigma<double>, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, C, G, psi, fx, fy);
 }
 
 // ---------------------------------------------------------------------------
@@ -416,87 +422,90 @@ extern "C" void pvk_cell_center(const pvk_grid *g, const double *u, const double
 
 // Explicit, implicit-diagonal and explicit-viscous terms for u at face
 // (iu, j) (Velocity.c:366-762, VOF + VOF_DIFFUSE branches, planar).
+template <class R = double>
 __device__ __forceinline__ void u_terms(const pvk_grid &g, const pvk_phys &ph, const double *u, const double *v,
-                                        const double *mu, int i, int j, double &ex, double &im, double &ve)
+                                        const double *mu, int i, int j, R &ex, R &im, R &ve)
 {
     const int P = g.P;
-    const double iRe = ph.iRe;
-    double muE = AT(mu, i, j), muW = AT(mu, i - 1, j);
-    double mu_yp = 2.0 * (0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) *
-                   (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) /
-                   ((0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) +
-                    (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) + 1e-12);
-    double mu_ym = 2.0 * (0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) *
-                   (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i, j - 1))) /
-                   ((0.5 * (AT(mu, i - 1, j) + AT(mu, i, j))) +
-                    (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i, j - 1))) + 1e-12);
-    double uc = AT(u, i, j);
-    double dudxE = (AT(u, i + 1, j) - uc) * g.idx_u[i];
-    double dudxW = (uc - AT(u, i - 1, j)) * g.idx_u[i - 1];
-    double dudyN = (AT(u, i, j + 1) - uc) * g.idy_c[j];
-    double dudyS = (uc - AT(u, i, j - 1)) * g.idy_c[j - 1];
-    double d2udy2 = iRe * (mu_yp * dudyN - mu_ym * dudyS) * g.idy_v[j];
-    double dvdxN = (AT(v, i, j + 1) - AT(v, i - 1, j + 1)) * g.idx_c[i - 1];
-    double dvdxS = (AT(v, i, j) - AT(v, i - 1, j)) * g.idx_c[i - 1];
-    double d2udx2 = iRe * ((muE * dudxE - muW * dudxW) * g.idx_c[i - 1]);
-    double ddydvdx = iRe * (mu_yp * dvdxN - mu_ym * dvdxS) * g.idy_v[j];
-    double vCN = 0.5 * (AT(v, i, j + 1) + AT(v, i - 1, j + 1));
-    double vCS = 0.5 * (AT(v, i, j) + AT(v, i - 1, j));
-    double ubE = 0.5 * (uc + AT(u, i + 1, j));            // u_bc[i]
-    double ubW = 0.5 * (AT(u, i - 1, j) + uc);            // u_bc[i-1]
-    double uuE = ubE * ubE, uuW = ubW * ubW;
-    double uvN = 0.5 * (uc + AT(u, i, j + 1)) * vCN;
-    double uvS = 0.5 * (uc + AT(u, i, j - 1)) * vCS;
-    double duudx = (uuE - uuW) * g.idx_c[i - 1];
-    double duvdy = (uvN - uvS) * g.idy_v[j];
-    ex = -(duudx + duvdy + 0.0);
-    ve = d2udx2 + ddydvdx + 0.0;
-    im = d2udx2 + d2udy2 + 0.0 + 0.0;
+    const R iRe = R(ph.iRe);
+    R muE = R(AT(mu, i, j)), muW = R(AT(mu, i - 1, j));
+    R mu_yp = R(2.0) * (R(0.5) * (R(AT(mu, i - 1, j + 1)) + R(AT(mu, i, j + 1)))) *
+                   (R(0.5) * (R(AT(mu, i - 1, j)) + R(AT(mu, i, j)))) /
+                   ((R(0.5) * (R(AT(mu, i - 1, j + 1)) + R(AT(mu, i, j + 1)))) +
+                    (R(0.5) * (R(AT(mu, i - 1, j)) + R(AT(mu, i, j)))) + R(1e-12));
+    R mu_ym = R(2.0) * (R(0.5) * (R(AT(mu, i - 1, j)) + R(AT(mu, i, j)))) *
+                   (R(0.5) * (R(AT(mu, i - 1, j - 1)) + R(AT(mu, i, j - 1)))) /
+                   ((R(0.5) * (R(AT(mu, i - 1, j)) + R(AT(mu, i, j)))) +
+                    (R(0.5) * (R(AT(mu, i - 1, j - 1)) + R(AT(mu, i, j - 1)))) + R(1e-12));
+    R uc = R(AT(u, i, j));
+    R dudxE = (R(AT(u, i + 1, j)) - uc) * R(g.idx_u[i]);
+    R dudxW = (uc - R(AT(u, i - 1, j))) * R(g.idx_u[i - 1]);
+    R dudyN = (R(AT(u, i, j + 1)) - uc) * R(g.idy_c[j]);
+    R dudyS = (uc - R(AT(u, i, j - 1))) * R(g.idy_c[j - 1]);
+    R d2udy2 = iRe * (mu_yp * dudyN - mu_ym * dudyS) * R(g.idy_v[j]);
+    R dvdxN = (R(AT(v, i, j + 1)) - R(AT(v, i - 1, j + 1))) * R(g.idx_c[i - 1]);
+    R dvdxS = (R(AT(v, i, j)) - R(AT(v, i - 1, j))) * R(g.idx_c[i - 1]);
+    R d2udx2 = iRe * ((muE * dudxE - muW * dudxW) * R(g.idx_c[i - 1]));
+    R ddydvdx = iRe * (mu_yp * dvdxN - mu_ym * dvdxS) * R(g.idy_v[j]);
+    R vCN = R(0.5) * (R(AT(v, i, j + 1)) + R(AT(v, i - 1, j + 1)));
+    R vCS = R(0.5) * (R(AT(v, i, j)) + R(AT(v, i - 1, j)));
+    R ubE = R(0.5) * (uc + R(AT(u, i + 1, j)));            // u_bc[i]
+    R ubW = R(0.5) * (R(AT(u, i - 1, j)) + uc);            // u_bc[i-1]
+    R uuE = ubE * ubE, uuW = ubW * ubW;
+    R uvN = R(0.5) * (uc + R(AT(u, i, j + 1))) * vCN;
+    R uvS = R(0.5) * (uc + R(AT(u, i, j - 1))) * vCS;
+    R duudx = (uuE - uuW) * R(g.idx_c[i - 1]);
+    R duvdy = (uvN - uvS) * R(g.idy_v[j]);
+    ex = -(duudx + duvdy + R(0.0));
+    ve = d2udx2 + ddydvdx + R(0.0);
+    im = d2udx2 + d2udy2 + R(0.0) + R(0.0);
 }
 
 // Same for v at face (i, j) (Velocity.c:771-1144).
+template <class R = double>
 __device__ __forceinline__ void v_terms(const pvk_grid &g, const pvk_phys &ph, const double *u, const double *v,
-                                        const double *mu, int i, int j, double &ex, double &im, double &ve)
+                                        const double *mu, int i, int j, R &ex, R &im, R &ve)
 {
     const int P = g.P;
-    const double iRe = ph.iRe;
-    double mu_xp = 2.0 * (0.5 * (AT(mu, i + 1, j - 1) + AT(mu, i + 1, j))) *
-                   (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) /
-                   ((0.5 * (AT(mu, i + 1, j - 1) + AT(mu, i + 1, j))) +
-                    (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) + 1e-12);
-    double mu_xm = 2.0 * (0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) *
-                   (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i - 1, j))) /
-                   ((0.5 * (AT(mu, i, j - 1) + AT(mu, i, j))) +
-                    (0.5 * (AT(mu, i - 1, j - 1) + AT(mu, i - 1, j))) + 1e-12);
-    double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
-    double vc = AT(v, i, j);
-    double dvdxE = (AT(v, i + 1, j) - vc) * g.idx_c[i];
-    double dvdxW = (vc - AT(v, i - 1, j)) * g.idx_c[i - 1];
-    double dvdyN = (AT(v, i, j + 1) - vc) * g.idy_v[j];
-    double dvdyS = (vc - AT(v, i, j - 1)) * g.idy_v[j - 1];
-    double d2vdx2 = iRe * ((mu_xp * dvdxE - mu_xm * dvdxW) * g.idx_u[i]);
-    double d2vdy2 = iRe * (mu_yp * dvdyN - mu_ym * dvdyS) * g.idy_c[j - 1];
-    double dudyE = (AT(u, i + 1, j) - AT(u, i + 1, j - 1)) * g.idy_c[j - 1];
-    double dudyW = (AT(u, i, j) - AT(u, i, j - 1)) * g.idy_c[j - 1];
-    double ddxdudy = iRe * ((mu_xp * dudyE - mu_xm * dudyW) * g.idx_u[i]);
-    double uCE = 0.5 * (AT(u, i + 1, j) + AT(u, i + 1, j - 1));
-    double uCW = 0.5 * (AT(u, i, j) + AT(u, i, j - 1));
-    double vuE = 0.5 * (vc + AT(v, i + 1, j)) * uCE;
-    double vuW = 0.5 * (vc + AT(v, i - 1, j)) * uCW;
-    double vbN = 0.5 * (vc + AT(v, i, j + 1));            // v_bc[j]
-    double vbS = 0.5 * (AT(v, i, j - 1) + vc);            // v_bc[j-1]
-    double vvN = vbN * vbN, vvS = vbS * vbS;
-    double dvudx = (vuE - vuW) * g.idx_u[i];
-    double dvvdy = (vvN - vvS) * g.idy_c[j - 1];
-    ex = -(dvudx + dvvdy + 0.0);
-    ve = ddxdudy + d2vdy2 + 0.0;
-    im = d2vdx2 + d2vdy2 + 0.0;
+    const R iRe = R(ph.iRe);
+    R mu_xp = R(2.0) * (R(0.5) * (R(AT(mu, i + 1, j - 1)) + R(AT(mu, i + 1, j)))) *
+                   (R(0.5) * (R(AT(mu, i, j - 1)) + R(AT(mu, i, j)))) /
+                   ((R(0.5) * (R(AT(mu, i + 1, j - 1)) + R(AT(mu, i + 1, j)))) +
+                    (R(0.5) * (R(AT(mu, i, j - 1)) + R(AT(mu, i, j)))) + R(1e-12));
+    R mu_xm = R(2.0) * (R(0.5) * (R(AT(mu, i, j - 1)) + R(AT(mu, i, j)))) *
+                   (R(0.5) * (R(AT(mu, i - 1, j - 1)) + R(AT(mu, i - 1, j)))) /
+                   ((R(0.5) * (R(AT(mu, i, j - 1)) + R(AT(mu, i, j)))) +
+                    (R(0.5) * (R(AT(mu, i - 1, j - 1)) + R(AT(mu, i - 1, j)))) + R(1e-12));
+    R mu_yp = R(AT(mu, i, j)), mu_ym = R(AT(mu, i, j - 1));
+    R vc = R(AT(v, i, j));
+    R dvdxE = (R(AT(v, i + 1, j)) - vc) * R(g.idx_c[i]);
+    R dvdxW = (vc - R(AT(v, i - 1, j))) * R(g.idx_c[i - 1]);
+    R dvdyN = (R(AT(v, i, j + 1)) - vc) * R(g.idy_v[j]);
+    R dvdyS = (vc - R(AT(v, i, j - 1))) * R(g.idy_v[j - 1]);
+    R d2vdx2 = iRe * ((mu_xp * dvdxE - mu_xm * dvdxW) * R(g.idx_u[i]));
+    R d2vdy2 = iRe * (mu_yp * dvdyN - mu_ym * dvdyS) * R(g.idy_c[j - 1]);
+    R dudyE = (R(AT(u, i + 1, j)) - R(AT(u, i + 1, j - 1))) * R(g.idy_c[j - 1]);
+    R dudyW = (R(AT(u, i, j)) - R(AT(u, i, j - 1))) * R(g.idy_c[j - 1]);
+    R ddxdudy = iRe * ((mu_xp * dudyE - mu_xm * dudyW) * R(g.idx_u[i]));
+    R uCE = R(0.5) * (R(AT(u, i + 1, j)) + R(AT(u, i + 1, j - 1)));
+    R uCW = R(0.5) * (R(AT(u, i, j)) + R(AT(u, i, j - 1)));
+    R vuE = R(0.5) * (vc + R(AT(v, i + 1, j))) * uCE;
+    R vuW = R(0.5) * (vc + R(AT(v, i - 1, j))) * uCW;
+    R vbN = R(0.5) * (vc + R(AT(v, i, j + 1)));            // v_bc[j]
+    R vbS = R(0.5) * (R(AT(v, i, j - 1)) + vc);            // v_bc[j-1]
+    R vvN = vbN * vbN, vvS = vbS * vbS;
+    R dvudx = (vuE - vuW) * R(g.idx_u[i]);
+    R dvvdy = (vvN - vvS) * R(g.idy_c[j - 1]);
+    ex = -(dvudx + dvvdy + R(0.0));
+    ve = ddxdudy + d2vdy2 + R(0.0);
+    im = d2vdx2 + d2vdy2 + R(0.0);
 }
 
 // Fused momentum right-hand side per face (Velocity.c:1494-1947) plus the
 // lagged capillary force (VOF_SurfaceTension.c:1983-2041) and gravity
 // (:2075-2154).  mode bit 1: store ex/im/ve; bit 2: assemble rhs and shift
 // the explicit history; bit 4: add f_sigma_old and gravity.
+template <class R>
 __global__ void k_mom_terms(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *sc, int comp,
                             const double *u, const double *v, const double *mu, const double *rho,
                             const double *p, double *eold, double *vold, const double *fold, double *rhs,
@@ -507,9 +516,10 @@ __global__ void k_mom_terms(pvk_grid g, pvk_phys ph, int stage, const pvk_scal *
     int ii = i;
     if (comp == 'u') ii = (i == 0) ? g.NXM : i;
     else if (j == 0) return;
-    double ex, im, ve;
-    if (comp == 'u') u_terms(g, ph, u, v, mu, ii, j, ex, im, ve);
-    else v_terms(g, ph, u, v, mu, ii, j, ex, im, ve);
+    R exr, imr, ver;
+    if (comp == 'u') u_terms<R>(g, ph, u, v, mu, ii, j, exr, imr, ver);
+    else v_terms<R>(g, ph, u, v, mu, ii, j, exr, imr, ver);
+    const double ex = (double)exr, im = (double)imr, ve = (double)ver;
     if (mode & 1) { AT(exo, ii, j) = ex; AT(imo, ii, j) = im; AT(veo, ii, j) = ve; }
     if (mode & 2) {
         const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
@@ -543,9 +553,13 @@ extern "C" void pvk_mom_terms(const pvk_grid *g, const pvk_phys *ph, int stage, 
                               double *ex_out, double *im_out, double *ve_out, int write_terms, int add_force,
                               hipStream_t s)
 {
-    int mode = write_terms;
-    hipLaunchKernelGGL(k_mom_terms, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, u, v, mu, rho, p,
-                       eold, vold, fold, rhs, ex_out, im_out, ve_out, mode | (add_force ? 4 : 0));
+    int mode = write_terms | (add_force ? 4 : 0);
+    if (ph->precision)
+        hipLaunchKernelGGL(k_mom_terms<float>, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, u, v, mu, rho,
+                           p, eold, vold, fold, rhs, ex_out, im_out, ve_out, mode);
+    else
+        hipLaunchKernelGGL(k_mom_terms<double>, grid2d(g), dim3(BX, BY), 0, s, *g, *ph, stage, sc, comp, u, v, mu, rho,
+                           p, eold, vold, fold, rhs, ex_out, im_out, ve_out, mode);
 }
 
 // Force terms alone (VOF_apply_f_sigma_old / Velocity_add_gravity_2_RHS).

@@ -24,27 +24,29 @@ __device__ __forceinline__ int wrapc(int i, int n) { return i < 0 ? i + n : (i >
 // u-type image (Communication.c:162-169): unknowns 1..n, column 0 = column n.
 __device__ __forceinline__ int wrapu(int i, int n) { return i <= 0 ? i + n : (i > n ? i - n : i); }
 
-// TwodOps_scalar_laplacian, planar (TwodOps.c:159-199)
-__device__ __forceinline__ double lap_at(const pvk_grid &g, const double *f, int i, int j)
+// Arithmetic type R: double for the FP64 path, float for the mixed path's
+// FP32 stencils.  Inputs are FP64 fields converted on load; every literal is
+// written R(...) so a float instantiation never promotes to double.  With
+// R = double each expression is the FP64 one, operand for operand.
+
+// Same formula as lap_at applied to five already-evaluated values.
+template <class R = double>
+__device__ __forceinline__ R lap_vals(const pvk_grid &g, int i, int j, R c, R E, R W, R N, R S)
 {
-    const int P = g.P;
-    double c = AT(f, i, j);
-    double dfdxE = (AT(f, i + 1, j) - c) * g.idx_c[i];
-    double dfdxW = (c - AT(f, i - 1, j)) * ((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]);
-    double dfdyN = (AT(f, i, j + 1) - c) * g.idy_c[j];
-    double dfdyS = (c - AT(f, i, j - 1)) * ((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]);
-    return (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
+    R dfdxE = (E - c) * R(g.idx_c[i]);
+    R dfdxW = (c - W) * R((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]);
+    R dfdyN = (N - c) * R(g.idy_c[j]);
+    R dfdyS = (c - S) * R((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]);
+    return (dfdxE - dfdxW) * R(g.idx_u[i]) + (dfdyN - dfdyS) * R(g.idy_v[j]);
 }
 
-// Same formula applied to five already-evaluated centre/neighbour values.
-__device__ __forceinline__ double lap_vals(const pvk_grid &g, int i, int j, double c, double E,
-                                           double W, double N, double S)
+// TwodOps_scalar_laplacian, planar (TwodOps.c:159-199)
+template <class R = double>
+__device__ __forceinline__ R lap_at(const pvk_grid &g, const double *f, int i, int j)
 {
-    double dfdxE = (E - c) * g.idx_c[i];
-    double dfdxW = (c - W) * ((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]);
-    double dfdyN = (N - c) * g.idy_c[j];
-    double dfdyS = (c - S) * ((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]);
-    return (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
+    const int P = g.P;
+    return lap_vals<R>(g, i, j, R(AT(f, i, j)), R(AT(f, i + 1, j)), R(AT(f, i - 1, j)), R(AT(f, i, j + 1)),
+                       R(AT(f, i, j - 1)));
 }
 
 // L(L f) at interior (i,j) where the inner Laplacian obeys the CH boundary
@@ -62,63 +64,76 @@ __device__ __forceinline__ double bilap_at(const pvk_grid &g, const double *f, i
 }
 
 // S(c) = 0.5 c - 1.5 c^2 + c^3 - c cs (1 - c - cs), cs = 0 (VOF_DIFFUSE.c:865-885)
-__device__ __forceinline__ double bulkS(double c)
+template <class R = double>
+__device__ __forceinline__ R bulkS(R c)
 {
-    const double cs = 0.0;
-    return 0.5 * c - 1.5 * c * c + c * c * c - c * cs * (1.0 - c - cs);
+    const R cs = R(0.0);
+    return R(0.5) * c - R(1.5) * c * c + c * c * c - c * cs * (R(1.0) - c - cs);
 }
 
 // diffuse_weno5_left (VOF_DIFFUSE.c:431-453); pow(x, 2.0) evaluated as x*x
-__device__ __forceinline__ double weno5(double v0, double v1, double v2, double v3, double v4)
+template <class R = double>
+__device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4)
 {
-    const double eps = 1e-6;
-    double q0 = (2.0 * v0 - 7.0 * v1 + 11.0 * v2) / 6.0;
-    double q1 = (-v1 + 5.0 * v2 + 2.0 * v3) / 6.0;
-    double q2 = (2.0 * v2 + 5.0 * v3 - v4) / 6.0;
-    double t0 = v0 - 2.0 * v1 + v2, t1 = v0 - 4.0 * v1 + 3.0 * v2;
-    double t2 = v1 - 2.0 * v2 + v3, t3 = v1 - v3;
-    double t4 = v2 - 2.0 * v3 + v4, t5 = 3.0 * v2 - 4.0 * v3 + v4;
-    double b0 = 13.0 / 12.0 * (t0 * t0) + 0.25 * (t1 * t1);
-    double b1 = 13.0 / 12.0 * (t2 * t2) + 0.25 * (t3 * t3);
-    double b2 = 13.0 / 12.0 * (t4 * t4) + 0.25 * (t5 * t5);
-    double a0 = 0.1 / ((eps + b0) * (eps + b0));
-    double a1 = 0.6 / ((eps + b1) * (eps + b1));
-    double a2 = 0.3 / ((eps + b2) * (eps + b2));
-    double asum = a0 + a1 + a2;
+    const R eps = R(1e-6);
+    R q0 = (R(2.0) * v0 - R(7.0) * v1 + R(11.0) * v2) / R(6.0);
+    R q1 = (-v1 + R(5.0) * v2 + R(2.0) * v3) / R(6.0);
+    R q2 = (R(2.0) * v2 + R(5.0) * v3 - v4) / R(6.0);
+    R t0 = v0 - R(2.0) * v1 + v2, t1 = v0 - R(4.0) * v1 + R(3.0) * v2;
+    R t2 = v1 - R(2.0) * v2 + v3, t3 = v1 - v3;
+    R t4 = v2 - R(2.0) * v3 + v4, t5 = R(3.0) * v2 - R(4.0) * v3 + v4;
+    R b0 = R(13.0 / 12.0) * (t0 * t0) + R(0.25) * (t1 * t1);
+    R b1 = R(13.0 / 12.0) * (t2 * t2) + R(0.25) * (t3 * t3);
+    R b2 = R(13.0 / 12.0) * (t4 * t4) + R(0.25) * (t5 * t5);
+    R a0 = R(0.1) / ((eps + b0) * (eps + b0));
+    R a1 = R(0.6) / ((eps + b1) * (eps + b1));
+    R a2 = R(0.3) / ((eps + b2) * (eps + b2));
+    R asum = a0 + a1 + a2;
     return (a0 * q0 + a1 * q1 + a2 * q2) / asum;
 }
 
 // diffuse_face_state_x / _y (VOF_DIFFUSE.c:455-483); face i sits between i-1 and i
-__device__ __forceinline__ double face_x(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
+template <class R = double>
+__device__ __forceinline__ R face_x(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
 {
     const int P = g.P;
     if (wo == 5) {
-        if (vel >= 0.0) return weno5(AT(C, i - 3, j), AT(C, i - 2, j), AT(C, i - 1, j), AT(C, i, j), AT(C, i + 1, j));
-        return weno5(AT(C, i + 2, j), AT(C, i + 1, j), AT(C, i, j), AT(C, i - 1, j), AT(C, i - 2, j));
+        if (vel >= 0.0)
+            return weno5<R>(R(AT(C, i - 3, j)), R(AT(C, i - 2, j)), R(AT(C, i - 1, j)), R(AT(C, i, j)),
+                            R(AT(C, i + 1, j)));
+        return weno5<R>(R(AT(C, i + 2, j)), R(AT(C, i + 1, j)), R(AT(C, i, j)), R(AT(C, i - 1, j)),
+                        R(AT(C, i - 2, j)));
     }
-    return (vel >= 0.0) ? AT(C, i - 1, j) : AT(C, i, j);
+    return R((vel >= 0.0) ? AT(C, i - 1, j) : AT(C, i, j));
 }
-__device__ __forceinline__ double face_y(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
+template <class R = double>
+__device__ __forceinline__ R face_y(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
 {
     const int P = g.P;
     if (wo == 5) {
-        if (vel >= 0.0) return weno5(AT(C, i, j - 3), AT(C, i, j - 2), AT(C, i, j - 1), AT(C, i, j), AT(C, i, j + 1));
-        return weno5(AT(C, i, j + 2), AT(C, i, j + 1), AT(C, i, j), AT(C, i, j - 1), AT(C, i, j - 2));
+        if (vel >= 0.0)
+            return weno5<R>(R(AT(C, i, j - 3)), R(AT(C, i, j - 2)), R(AT(C, i, j - 1)), R(AT(C, i, j)),
+                            R(AT(C, i, j + 1)));
+        return weno5<R>(R(AT(C, i, j + 2)), R(AT(C, i, j + 1)), R(AT(C, i, j)), R(AT(C, i, j - 1)),
+                        R(AT(C, i, j - 2)));
     }
-    return (vel >= 0.0) ? AT(C, i, j - 1) : AT(C, i, j);
+    return R((vel >= 0.0) ? AT(C, i, j - 1) : AT(C, i, j));
 }
 
-// -(div(u C)) at interior cell (i,j): VOF_DIFFUSE.c:931-996 + TwodOps.c:201-229
+// -(div(u C)) at interior cell (i,j): VOF_DIFFUSE.c:931-996 + TwodOps.c:201-229.
+// The face states are reconstructed in R; fluxes u C_face and their
+// divergence are always accumulated in FP64 (conservative flux form).
+template <class R = double>
 __device__ __forceinline__ double weno_rhs_at(const pvk_grid &g, const pvk_phys &ph, const double *C,
                                               const double *u, const double *v, int i, int j)
 {
     const int P = g.P;
     double uw = AT(u, i, j), ue = AT(u, i + 1, j);
     double vs = AT(v, i, j), vn = AT(v, i, j + 1);
-    double fxW = uw * face_x(g, C, i, j, uw, ph.weno_order);
-    double fxE = ue * face_x(g, C, i + 1, j, ue, ph.weno_order);
-    double fyS = vs * face_y(g, C, i, j, vs, ph.weno_order);
-    double fyN = vn * face_y(g, C, i, j + 1, vn, ph.weno_order);
+    double fxW = uw * (double)face_x<R>(g, C, i, j, uw, ph.weno_order);
+    double fxE = ue * (double)face_x<R>(g, C, i + 1, j, ue, ph.weno_order);
+    double fyS = vs * (double)face_y<R>(g, C, i, j, vs, ph.weno_order);
+    double fyN = vn * (double)face_y<R>(g, C, i, j + 1, vn, ph.weno_order);
     double div_x = (fxE - fxW) / g.dx_c[i];
     double div_y = (fyN - fyS) / g.dy_c[j];
     double div = div_x + div_y;
@@ -126,12 +141,21 @@ __device__ __forceinline__ double weno_rhs_at(const pvk_grid &g, const pvk_phys 
 }
 
 // L(S(C)) / Pe at interior (i,j) (VOF_DIFFUSE.c:998-1016); C ghosts valid.
+// S and the face gradients are formed in R; the flux differences are FP64,
+// so a face flux enters its two cells as the same double and the sum of the
+// term over the domain telescopes to FP64 round-off (phase volume kept).
+template <class R = double>
 __device__ __forceinline__ double diffusion_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
 {
     const int P = g.P;
-    double S0 = bulkS(AT(C, i, j));
-    double lapS = lap_vals(g, i, j, S0, bulkS(AT(C, i + 1, j)), bulkS(AT(C, i - 1, j)),
-                           bulkS(AT(C, i, j + 1)), bulkS(AT(C, i, j - 1)));
+    const R S0 = bulkS<R>(R(AT(C, i, j)));
+    const R SE = bulkS<R>(R(AT(C, i + 1, j))), SW = bulkS<R>(R(AT(C, i - 1, j)));
+    const R SN = bulkS<R>(R(AT(C, i, j + 1))), SS = bulkS<R>(R(AT(C, i, j - 1)));
+    const double dfdxE = (double)((SE - S0) * R(g.idx_c[i]));
+    const double dfdxW = (double)((S0 - SW) * R((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]));
+    const double dfdyN = (double)((SN - S0) * R(g.idy_c[j]));
+    const double dfdyS = (double)((S0 - SS) * R((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]));
+    const double lapS = (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
     return lapS / ph.Pe_CH;
 }
 
@@ -148,72 +172,85 @@ __device__ __forceinline__ double ch_w(const pvk_grid &g, int i, int j, double a
     if (ix && jy) s += ((g.dx_u[g.NXM] * g.dy_v[g.NYM] * g.slab) * a) * b;
     return s;
 }
-__device__ __forceinline__ double harm(double a, double b) { return 2.0 * (a) * (b) / ((a) + (b) + 1e-12); }
-__device__ __forceinline__ double avg2(double a, double b) { return 0.5 * ((a) + (b)); }
+template <class R = double>
+__device__ __forceinline__ R harm(R a, R b) { return R(2.0) * (a) * (b) / ((a) + (b) + R(1e-12)); }
+template <class R = double>
+__device__ __forceinline__ R avg2(R a, R b) { return R(0.5) * ((a) + (b)); }
 
 // Jacobi diagonal (lsolver/msolve_cg.c:103-249) and operator (:255-493).
-__device__ __forceinline__ double mom_diag(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
-                                           const double *rho, const double *mu, int comp, int i, int j)
+template <class R = double>
+__device__ __forceinline__ R mom_diag(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                      const double *rho, const double *mu, int comp, int i, int j)
 {
     const int P = g.P;
-    const double iRe = ph.iRe;
-    double rho_face, d_time, dx, dy;
+    const R iRe = R(ph.iRe);
+    R rho_face, d_time, dx, dy;
     if (comp == 'u') {
-        rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
-        d_time = rho_face / (alpha_k * dt);
-        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
-        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
-        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
-        dx = iRe * ((mu_xp * g.idx_u[i] + mu_xm * g.idx_u[i - 1]) * g.idx_c[i - 1]);
-        dy = iRe * (mu_yp * g.idy_c[j] + mu_ym * g.idy_c[j - 1]) * g.idy_v[j];
+        rho_face = R(0.5) * (R(AT(rho, i, j)) + R(AT(rho, i - 1, j)));
+        d_time = rho_face / (R(alpha_k) * R(dt));
+        R mu_xp = R(AT(mu, i, j)), mu_xm = R(AT(mu, i - 1, j));
+        R mu_yp = harm<R>(avg2<R>(R(AT(mu, i - 1, j + 1)), R(AT(mu, i, j + 1))), avg2<R>(R(AT(mu, i - 1, j)), R(AT(mu, i, j))));
+        R mu_ym = harm<R>(avg2<R>(R(AT(mu, i - 1, j)), R(AT(mu, i, j))), avg2<R>(R(AT(mu, i - 1, j - 1)), R(AT(mu, i, j - 1))));
+        dx = iRe * ((mu_xp * R(g.idx_u[i]) + mu_xm * R(g.idx_u[i - 1])) * R(g.idx_c[i - 1]));
+        dy = iRe * (mu_yp * R(g.idy_c[j]) + mu_ym * R(g.idy_c[j - 1])) * R(g.idy_v[j]);
     } else {
-        rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
-        d_time = rho_face / (alpha_k * dt);
-        double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
-        double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
-        double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
-        dx = iRe * ((mu_xp * g.idx_c[i] + mu_xm * g.idx_c[i - 1]) * g.idx_u[i]);
-        dy = iRe * (mu_yp * g.idy_v[j] + mu_ym * g.idy_v[j - 1]) * g.idy_c[j - 1];
+        rho_face = R(0.5) * (R(AT(rho, i, j)) + R(AT(rho, i, j - 1)));
+        d_time = rho_face / (R(alpha_k) * R(dt));
+        R mu_yp = R(AT(mu, i, j)), mu_ym = R(AT(mu, i, j - 1));
+        R mu_xp = harm<R>(avg2<R>(R(AT(mu, i + 1, j - 1)), R(AT(mu, i + 1, j))), avg2<R>(R(AT(mu, i, j - 1)), R(AT(mu, i, j))));
+        R mu_xm = harm<R>(avg2<R>(R(AT(mu, i, j - 1)), R(AT(mu, i, j))), avg2<R>(R(AT(mu, i - 1, j - 1)), R(AT(mu, i - 1, j))));
+        dx = iRe * ((mu_xp * R(g.idx_c[i]) + mu_xm * R(g.idx_c[i - 1])) * R(g.idx_u[i]));
+        dy = iRe * (mu_yp * R(g.idy_v[j]) + mu_ym * R(g.idy_v[j - 1])) * R(g.idy_c[j - 1]);
     }
-    return d_time + dx + dy + 0.0 + 0.0;
+    return d_time + dx + dy + R(0.0) + R(0.0);
 }
 
-__device__ __forceinline__ double mom_op(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
-                                         const double *rho, const double *mu, const double *x, int comp, int i,
-                                         int j)
+// The operator with the five x values supplied (centre, E, W, N, S).
+template <class R = double>
+__device__ __forceinline__ R mom_op_vals(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                         const double *rho, const double *mu, int comp, int i, int j, R xc, R xE,
+                                         R xW, R xN, R xS)
 {
     const int P = g.P;
-    const double iRe = ph.iRe;
-    double xc = AT(x, i, j), op = 0.0;
-    if (comp == 'u') {
-        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
-        double factor_time = rho_face / (alpha_k * dt);
-        double Ax_val = factor_time * xc;
-        double mu_xp = AT(mu, i, j), mu_xm = AT(mu, i - 1, j);
-        double mu_yp = harm(avg2(AT(mu, i - 1, j + 1), AT(mu, i, j + 1)), avg2(AT(mu, i - 1, j), AT(mu, i, j)));
-        double mu_ym = harm(avg2(AT(mu, i - 1, j), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i, j - 1)));
-        double flux_xp = mu_xp * (AT(x, i + 1, j) - xc) * g.idx_u[i];
-        double flux_xm = mu_xm * (xc - AT(x, i - 1, j)) * g.idx_u[i - 1];
-        double flux_yp = mu_yp * (AT(x, i, j + 1) - xc) * g.idy_c[j];
-        double flux_ym = mu_ym * (xc - AT(x, i, j - 1)) * g.idy_c[j - 1];
-        op += iRe * ((flux_xp - flux_xm) * g.idx_c[i - 1]);
-        op += iRe * (flux_yp - flux_ym) * g.idy_v[j];
-        op += 0.0;
-        return Ax_val - op;
-    } else {
-        double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
-        double factor_time = rho_face / (alpha_k * dt);
-        double Ax_val = factor_time * xc;
-        double mu_yp = AT(mu, i, j), mu_ym = AT(mu, i, j - 1);
-        double mu_xp = harm(avg2(AT(mu, i + 1, j - 1), AT(mu, i + 1, j)), avg2(AT(mu, i, j - 1), AT(mu, i, j)));
-        double mu_xm = harm(avg2(AT(mu, i, j - 1), AT(mu, i, j)), avg2(AT(mu, i - 1, j - 1), AT(mu, i - 1, j)));
-        double flux_xp = mu_xp * (AT(x, i + 1, j) - xc) * g.idx_c[i];
-        double flux_xm = mu_xm * (xc - AT(x, i - 1, j)) * g.idx_c[i - 1];
-        double flux_yp = mu_yp * (AT(x, i, j + 1) - xc) * g.idy_v[j];
-        double flux_ym = mu_ym * (xc - AT(x, i, j - 1)) * g.idy_v[j - 1];
-        op += iRe * ((flux_xp - flux_xm) * g.idx_u[i]);
-        op += iRe * (flux_yp - flux_ym) * g.idy_c[j - 1];
+    const R iRe = R(ph.iRe);
+    R op = R(0.0);
+    if (comp == 0) {
+        R rho_face = R(0.5) * (R(AT(rho, i, j)) + R(AT(rho, i - 1, j)));
+        R factor_time = rho_face / (R(alpha_k) * R(dt));
+        R Ax_val = factor_time * xc;
+        R mu_xp = R(AT(mu, i, j)), mu_xm = R(AT(mu, i - 1, j));
+        R mu_yp = harm<R>(avg2<R>(R(AT(mu, i - 1, j + 1)), R(AT(mu, i, j + 1))), avg2<R>(R(AT(mu, i - 1, j)), R(AT(mu, i, j))));
+        R mu_ym = harm<R>(avg2<R>(R(AT(mu, i - 1, j)), R(AT(mu, i, j))), avg2<R>(R(AT(mu, i - 1, j - 1)), R(AT(mu, i, j - 1))));
+        R flux_xp = mu_xp * (xE - xc) * R(g.idx_u[i]);
+        R flux_xm = mu_xm * (xc - xW) * R(g.idx_u[i - 1]);
+        R flux_yp = mu_yp * (xN - xc) * R(g.idy_c[j]);
+        R flux_ym = mu_ym * (xc - xS) * R(g.idy_c[j - 1]);
+        op += iRe * ((flux_xp - flux_xm) * R(g.idx_c[i - 1]));
+        op += iRe * (flux_yp - flux_ym) * R(g.idy_v[j]);
+        op += R(0.0);
         return Ax_val - op;
     }
+    R rho_face = R(0.5) * (R(AT(rho, i, j)) + R(AT(rho, i, j - 1)));
+    R factor_time = rho_face / (R(alpha_k) * R(dt));
+    R Ax_val = factor_time * xc;
+    R mu_yp = R(AT(mu, i, j)), mu_ym = R(AT(mu, i, j - 1));
+    R mu_xp = harm<R>(avg2<R>(R(AT(mu, i + 1, j - 1)), R(AT(mu, i + 1, j))), avg2<R>(R(AT(mu, i, j - 1)), R(AT(mu, i, j))));
+    R mu_xm = harm<R>(avg2<R>(R(AT(mu, i, j - 1)), R(AT(mu, i, j))), avg2<R>(R(AT(mu, i - 1, j - 1)), R(AT(mu, i - 1, j))));
+    R flux_xp = mu_xp * (xE - xc) * R(g.idx_c[i]);
+    R flux_xm = mu_xm * (xc - xW) * R(g.idx_c[i - 1]);
+    R flux_yp = mu_yp * (xN - xc) * R(g.idy_v[j]);
+    R flux_ym = mu_ym * (xc - xS) * R(g.idy_v[j - 1]);
+    op += iRe * ((flux_xp - flux_xm) * R(g.idx_u[i]));
+    op += iRe * (flux_yp - flux_ym) * R(g.idy_c[j - 1]);
+    return Ax_val - op;
+}
+
+template <class R = double>
+__device__ __forceinline__ R mom_op(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                    const double *rho, const double *mu, const double *x, int comp, int i, int j)
+{
+    const int P = g.P;
+    return mom_op_vals<R>(g, ph, alpha_k, dt, rho, mu, comp == 'u' ? 0 : 1, i, j, R(AT(x, i, j)),
+                          R(AT(x, i + 1, j)), R(AT(x, i - 1, j)), R(AT(x, i, j + 1)), R(AT(x, i, j - 1)));
 }
 #endif

@@ -163,8 +163,8 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     *out = NULL;
     if (!prm || prm->NXM < 8 || prm->NYM < 8)
         return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
-    if (prm->precision != PV_PREC_FP64)
-        return pv_set_error(PV_EINVAL, "only precision = PV_PREC_FP64 is implemented");
+    if (prm->precision != PV_PREC_FP64 && prm->precision != PV_PREC_MIXED)
+        return pv_set_error(PV_EINVAL, "unknown precision %d", prm->precision);
     if (prm->pressure_solver != PV_PSOLVE_JACOBI_PCG && prm->pressure_solver != PV_PSOLVE_FFT_PCG)
         return pv_set_error(PV_EINVAL, "unknown pressure_solver %d", prm->pressure_solver);
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)

@@ -39,7 +39,7 @@ def test_no_gpu_create_fails_cleanly():
     assert lib.pv_last_error()
 
 
-@pytest.mark.parametrize("kw", [dict(precision=pvof.PV_PREC_MIXED), dict(pressure_solver=7),
+@pytest.mark.parametrize("kw", [dict(precision=5), dict(pressure_solver=7),
                                 dict(NXM=4)])
 def test_create_rejects_unsupported_params(kw):
     """Unsupported settings fail with PV_EINVAL before any device is touched."""
