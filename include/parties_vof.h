@@ -225,6 +225,22 @@ int pv_timers_read(pv_bag *bag, double *ms, long long *count);
 /* Block until all work queued on the handle's stream has finished. */
 int pv_synchronize(pv_bag *bag);
 
+/* ---- learned pressure preconditioner network --------------------------
+ * No counterpart in the reference: the 6-layer 3x3 BF16-MFMA network of the
+ * learned FFT-FCG pressure solve.  Parameters are PV_CNN_NPARAM floats in the
+ * layout documented in csrc/pv_cnn.hip.  Batches are B images of H x W
+ * pixels (multiples of 16): in3[B][H][W][3] -> out[B][H][W]. */
+#define PV_CNN_NPARAM 38177
+int pv_cnn_init(pv_bag *bag, unsigned long long seed);
+int pv_cnn_set_params(pv_bag *bag, const float *host);
+int pv_cnn_get_params(pv_bag *bag, float *host);
+/* forward; with tgt != NULL also the loss mean((out - tgt)^2) over pixels at
+ * least `border` from every edge and its gradient (to grad, if not NULL, and
+ * kept on the device for pv_cnn_adam_step) */
+int pv_cnn_eval(pv_bag *bag, const float *in3, const float *tgt, float *out, float *grad, double *loss,
+                int B, int H, int W, int periodic_x, int border);
+int pv_cnn_adam_step(pv_bag *bag, float lr, int t);
+
 /* ---- initial volume fraction F on the G range (host only, no device) ----
  * 6 x 6 sub-sampled tanh profile of width 2 sqrt(2) Cn, as in
  * IO/VoF_Init.c:877-953 / :1032-1100 (F = 1 liquid, 0 gas).  The seeded

@@ -103,6 +103,19 @@ typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double tol, in_tot[2];           /* CG_ETOL; 1 / number of unknowns */
 } pvk_mcg;
 
+/* Learned preconditioner network (csrc/pv_cnn.hip): parameters, Adam state
+ * and the activation / gradient tensors of the largest batch seen so far. */
+#define PVK_CNN_NPARAM 38177
+typedef struct pvk_cnn {
+    float *w, *g, *m, *v;            /* FP32 parameters, gradient, Adam moments */
+    unsigned short *wb, *wt;         /* BF16 32 -> 32 weights: forward, flipped transpose */
+    unsigned short *act[5];          /* BF16 NHWC activations of layers 1..5 */
+    unsigned short *gz[2];           /* BF16 NHWC gradients w.r.t. pre-activations */
+    float *gout;                     /* dLoss / d out */
+    float *part, *lpart;             /* per-block partials: gradients, loss */
+    size_t cap_pix;                  /* B * H * W the tensors hold */
+} pvk_cnn;
+
 /* ---- ghost fills (kind codes in pv_kernels.hip k_ghost) ---- */
 void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s);          /* VOF_DIFFUSE_set_boundary_values */
 void pvk_halo_cell(const pvk_grid *g, double *f, hipStream_t s);
@@ -204,6 +217,13 @@ void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_
                    hipStream_t s);
 void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
                    double tol, int maxit, hipStream_t s);
+
+/* learned preconditioner network (csrc/pv_cnn.hip) */
+void pvk_cnn_pack(const pvk_cnn *c, hipStream_t s);
+void pvk_cnn_forward(const pvk_cnn *c, const float *in3, float *out, int B, int H, int W, int px, hipStream_t s);
+void pvk_cnn_backward(const pvk_cnn *c, const float *in3, const float *out, const float *tgt, int B, int H, int W,
+                      int px, int border, hipStream_t s);
+void pvk_cnn_adam(const pvk_cnn *c, float lr, int t, hipStream_t s);
 
 /* ---- time step ---- */
 void pvk_cfl(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd, hipStream_t s);

@@ -306,6 +306,7 @@ int pv_destroy(pv_bag *b)
     hipStreamSynchronize(b->stream);
     if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
+    pv_cnn_free(b);
     hipFree(b->cgmem);
     hipFree(b->slabmem);
     hipFree(b->metricmem);

@@ -66,6 +66,10 @@ struct pv_bag {
     pvk_pcg pcg;
     double *pcgmem;
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
+    pvk_cnn cnn;                   /* learned preconditioner network (pv_cnn_init) */
+    int cnn_ready;
+    float *cnn_io;                 /* device staging of batch inputs / targets / outputs */
+    size_t cnn_io_cap;
     pvk_mcg mcg;                   /* fused momentum PCG work set */
     double *cgmem;
 };
@@ -79,6 +83,7 @@ int  pv_read_scalars(pv_bag *b);
 double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
 int  pv_cg_create(pv_bag *b);
+void pv_cnn_free(pv_bag *b);
 void pv_tbegin(pv_bag *b, int t);
 void pv_tend(pv_bag *b, int t);
 int  pv_timers_flush(pv_bag *b);

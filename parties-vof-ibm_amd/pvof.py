@@ -146,6 +146,12 @@ def load_lib(path=LIB_PATH):
     lib.pv_timers_enable.argtypes = [vp, C.c_int]
     lib.pv_timers_reset.argtypes = [vp]
     lib.pv_timers_read.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
+    fp = C.POINTER(C.c_float)
+    lib.pv_cnn_init.argtypes = [vp, C.c_uint64]
+    lib.pv_cnn_set_params.argtypes = [vp, fp]
+    lib.pv_cnn_get_params.argtypes = [vp, fp]
+    lib.pv_cnn_eval.argtypes = [vp, fp, fp, fp, fp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
+    lib.pv_cnn_adam_step.argtypes = [vp, C.c_float, C.c_int]
     for name in ("pv_ic_rising_bubble", "pv_ic_rayleigh_taylor"):
         getattr(lib, name).argtypes = [C.POINTER(Params), dp]
     lib.pv_ic_multimode_rt.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_double, C.c_double, dp]
@@ -207,6 +213,50 @@ class Solver:
 
     def call(self, name, *args):
         return self.check(getattr(self.lib, "pv_" + name)(self.h, *args))
+
+
+PV_CNN_NPARAM = 38177
+
+
+def _fptr(a):
+    assert a.dtype == np.float32 and a.flags.c_contiguous
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class Network:
+    """Host view of the learned preconditioner network of one Solver."""
+
+    def __init__(self, solver, seed=1):
+        self.s = solver
+        solver.check(solver.lib.pv_cnn_init(solver.h, seed))
+
+    def params(self):
+        a = np.zeros(PV_CNN_NPARAM, dtype=np.float32)
+        self.s.check(self.s.lib.pv_cnn_get_params(self.s.h, _fptr(a)))
+        return a
+
+    def set_params(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float32)
+        self.s.check(self.s.lib.pv_cnn_set_params(self.s.h, _fptr(a)))
+
+    def eval(self, x, tgt=None, px=0, border=0, want_grad=False):
+        """x [B][H][W][3] -> out [B][H][W]; with tgt also (loss, gradient)."""
+        x = np.ascontiguousarray(x, dtype=np.float32)
+        B, H, W, _ = x.shape
+        out = np.zeros((B, H, W), dtype=np.float32)
+        loss = C.c_double(0.0)
+        grad = np.zeros(PV_CNN_NPARAM, dtype=np.float32) if want_grad else None
+        t = None if tgt is None else np.ascontiguousarray(tgt, dtype=np.float32)
+        null = C.POINTER(C.c_float)()
+        self.s.check(self.s.lib.pv_cnn_eval(self.s.h, _fptr(x), _fptr(t) if t is not None else null, _fptr(out),
+                                            _fptr(grad) if grad is not None else null, C.byref(loss), B, H, W,
+                                            px, border))
+        if tgt is None:
+            return out
+        return out, loss.value, grad
+
+    def adam_step(self, lr, t):
+        self.s.check(self.s.lib.pv_cnn_adam_step(self.s.h, lr, t))
 
 
 def ic(lib, kind, params, **kw):
