@@ -87,13 +87,39 @@ def cpu_baseline(params, F):
                       f"of the serial C oracle: {dt:.1f} s, scaled by 3 substeps per step"}
 
 
+def train_network(lib, args, prec):
+    """Train the preconditioner network (untimed) on the plain solver's own
+    pressure solves of a held-out seed of the same workload."""
+    t0 = time.perf_counter()
+    pt = workload_params(args.n, pvof.PV_PSOLVE_FFT_PCG, prec)
+    Ft = pvof.ic(lib, "bubbles", pt, seed=SEED + 1, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
+    st = pvof.Solver(pt, lib, device=0)
+    st.init_from_F(Ft)
+    net = pvof.Network(st, seed=3)
+    net.collect([0, 8, 32, 96], crops_per_snap=400, crop_size=64, max_samples=20000, seed=5)
+    st.call("Temporal_int_rk3", args.train_collect_steps)
+    nsamp = net.collect_end()
+    t1 = time.perf_counter()
+    l0, l1 = net.train(args.train_steps, batch=32, lr=1e-3, seed=11)
+    _, l2 = net.train(max(1, args.train_steps // 2), batch=32, lr=3e-4, seed=12)
+    prm = net.params()
+    st.close()
+    return {"params": prm, "samples": nsamp, "collect_s": round(t1 - t0, 2),
+            "train_s": round(time.perf_counter() - t1, 2), "train_steps": args.train_steps * 3 // 2,
+            "loss_first": l0, "loss_last": l2, "held_out_seed": SEED + 1}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--solver", choices=["fft", "jacobi"], default="fft")
+    ap.add_argument("--solver", choices=["fft", "jacobi", "learned", "learned_fcg"], default="fft",
+                    help="learned: network warm start + FFT-PCG; learned_fcg: + network-corrected "
+                         "preconditioner in flexible CG (network trained first, untimed, on a held-out seed)")
+    ap.add_argument("--train-collect-steps", type=int, default=4)
+    ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
                     help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -108,23 +134,33 @@ def main():
         sys.exit(2)
 
     lib = pvof.load_lib()
-    solver = pvof.PV_PSOLVE_FFT_PCG if args.solver == "fft" else pvof.PV_PSOLVE_JACOBI_PCG
+    solver = {"fft": pvof.PV_PSOLVE_FFT_PCG, "jacobi": pvof.PV_PSOLVE_JACOBI_PCG,
+              "learned": pvof.PV_PSOLVE_LEARNED, "learned_fcg": pvof.PV_PSOLVE_LEARNED_FCG}[args.solver]
     prec = pvof.PV_PREC_MIXED if args.precision == "mixed" else pvof.PV_PREC_FP64
     p = workload_params(args.n, solver, prec)
+    learned = None
+    if args.solver.startswith("learned"):
+        learned = train_network(lib, args, prec)
     F = pvof.ic(lib, "bubbles", p, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
     s = pvof.Solver(p, lib, device=0)
     s.init_from_F(F)
+    if learned is not None:
+        pvof.Network(s).set_params(learned.pop("params"))
     if args.warmup:
         s.call("Temporal_int_rk3", args.warmup)
     s.call("synchronize")
     if not args.no_timers:
         s.call("timers_enable", max(1, args.timer_period))
         s.call("timers_reset")
-    stats = []
+    stats, starts = [], []
     t0 = time.perf_counter()
     for _ in range(args.steps):
         s.call("Temporal_int_rk3", 1)
         stats.append(s.stats())
+        if learned is not None:
+            r = C.c_double(0.0)
+            lib.pv_learn_start_residual(s.h, C.byref(r))
+            starts.append(r.value)
     s.call("synchronize")
     t1 = time.perf_counter()
     elapsed = t1 - t0
@@ -132,7 +168,7 @@ def main():
     cnt = (C.c_longlong * len(TIMERS))()
     if not args.no_timers:
         s.check(lib.pv_timers_read(s.h, ms, cnt))
-    bpc = BYTES_PER_CELL_FFT if args.solver == "fft" else BYTES_PER_CELL
+    bpc = BYTES_PER_CELL if args.solver == "jacobi" else BYTES_PER_CELL_FFT
 
     cells = p.NXM * p.NYM
     value = cells * args.steps / elapsed / 1e6
@@ -148,8 +184,10 @@ def main():
     iters = {key: float(np.mean([sum(st[key]) for st in stats])) for key in ("n_ch", "n_u", "n_v", "n_p")}
 
     out = {
-        "metric": "Mcell-steps/s at 1024^2, plain FP64 (FFT-PCG pressure)" if prec == pvof.PV_PREC_FP64
-                  else "Mcell-steps/s at 1024^2, mixed FP32/FP64 (FFT-PCG pressure)",
+        "metric": "Mcell-steps/s at 1024^2, " + ("plain FP64" if prec == pvof.PV_PREC_FP64 else "mixed FP32/FP64")
+                  + {"fft": " (FFT-PCG pressure)", "jacobi": " (Jacobi-PCG pressure)",
+                     "learned": " (learned warm start + FFT-PCG pressure)",
+                     "learned_fcg": " (learned warm start + learned-preconditioned FCG pressure)"}[args.solver],
         "value": round(value, 3),
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
@@ -172,6 +210,10 @@ def main():
         "kernels": kernels,
         "cpu_baseline": None,
     }
+    if learned is not None:
+        learned["start_residual_last_step"] = starts[-1] if starts else None
+        learned["fallbacks"] = int(sum(sum(st["p_fallback"]) for st in stats))
+        out["learned"] = learned
     if kernels:
         dom = max(kernels, key=lambda n: kernels[n]["est_total_ms"])
         achieved = kernels[dom]["GBs"]

@@ -46,6 +46,10 @@ extern "C" {
 
 /* ---- pressure solver selection ---------------------------------------- */
 #define PV_PSOLVE_JACOBI_PCG 0   /* psolve_cg.c:130-293, the reference VOF path */
+#define PV_PSOLVE_LEARNED    2   /* FFT-PCG from the learned network's warm start (no
+                                    counterpart in the reference; needs a trained network) */
+#define PV_PSOLVE_LEARNED_FCG 3  /* learned warm start + learned-preconditioned flexible
+                                    CG, falling back to FFT-PCG when the residual stalls */
 #define PV_PSOLVE_FFT_PCG    1   /* PCG preconditioned by the FFT-x/Thomas-y solve
                                     of psolve_fft.c:1536-1802, relative-residual stop */
 
@@ -240,6 +244,41 @@ int pv_cnn_get_params(pv_bag *bag, float *host);
 int pv_cnn_eval(pv_bag *bag, const float *in3, const float *tgt, float *out, float *grad, double *loss,
                 int B, int H, int W, int periodic_x, int border);
 int pv_cnn_adam_step(pv_bag *bag, float lr, int t);
+
+/* ---- launch graphs, autotuning and batches of cases ---------------------
+ * No counterpart in the reference.  pv_set_graphs replays the pressure
+ * FFT-PCG from hipGraphs of `chunk` iterations (captured on first use);
+ * pv_autotune times eager launches and graph chunks 8, 16, 32 on the current
+ * pressure solve and keeps the fastest (the trajectory is unchanged).  A
+ * batch holds independent cases of any size (one handle and stream each);
+ * pv_batch_rk3 advances all of them, one host thread per case, so their
+ * kernels overlap on the GPU; each case computes exactly what it computes
+ * alone. */
+typedef struct pv_batch pv_batch;
+int pv_set_graphs(pv_bag *bag, int on, int chunk);
+int pv_autotune(pv_bag *bag, int *chunk);
+int pv_batch_create(const pv_params *prms, int ncases, int device, pv_batch **out);
+int pv_batch_destroy(pv_batch *bt);
+int pv_batch_size(const pv_batch *bt);
+pv_bag *pv_batch_case(pv_batch *bt, int k);
+int pv_batch_rk3(pv_batch *bt, int nsteps);
+int pv_batch_autotune(pv_batch *bt, int *chunks);
+
+/* ---- learned pressure solve ----------------------------------------------
+ * No counterpart in the reference.  Training data come from the plain
+ * FFT-PCG solves of the handle's own time steps: while collecting, each
+ * pressure solve stores at the given PCG iterations the features of
+ * (r_k, M_fft r_k, rho) and the target of the network's correction, cut into
+ * crop_size^2 crops (crops_per_snap per snapshot, at most max_samples). */
+int pv_set_pressure_solver(pv_bag *bag, int solver);
+int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
+                           int max_samples, unsigned long long seed);
+int pv_learn_collect_end(pv_bag *bag, int *nsamples);
+/* Adam on random batches of the collected crops; loss of the first and last batch */
+int pv_learn_train(pv_bag *bag, int steps, int batch, float lr, unsigned long long seed, double *loss_first,
+                   double *loss_last);
+/* ||b - A phi0|| / ||b|| of the last learned warm start (>= 1: it was discarded) */
+int pv_learn_start_residual(pv_bag *bag, double *res);
 
 /* ---- initial volume fraction F on the G range (host only, no device) ----
  * 6 x 6 sub-sampled tanh profile of width 2 sqrt(2) Cn, as in

@@ -79,8 +79,10 @@ typedef struct pvk_pcg {
     double *bx, *by;                  /* operator face coefficients (per solve) */
     const double *rhs, *rho;
     double *specT;                    /* NXM * NYM transposed spectrum */
-    double *dad, *rr, *rz[2], *mean;  /* per-block partial sums */
+    double *dad[2], *rr, *rz[2], *mean;  /* per-block partial sums; dad by iteration parity */
+    double *zad;                      /* <A d_n, z_{n+1}> partials of the flexible (learned) form */
     int ndad, nrr, nrz, nmean;        /* partial counts */
+    int flex;                         /* 1: flexible CG, beta = -<z_{n+1}, A d_n> / <d_n, A d_n> */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
 } pvk_pcg;
 
@@ -217,6 +219,25 @@ void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_
                    hipStream_t s);
 void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
                    double tol, int maxit, hipStream_t s);
+
+/* out = A x with the pressure operator of the fused PCG (faces from its start) */
+void pvk_pcg_opx(const pvk_grid *g, const pvk_pcg *w, const double *x, double *out, hipStream_t s);
+
+/* learned-solve glue (csrc/pv_learn.hip); sc->red[5..7] = s_r, s_z, mean(e) */
+void pvk_learn_stats(const pvk_grid *g, pvk_scal *sc, const double *r, const double *z, const double *a,
+                     const double *b, double *part, hipStream_t s);
+void pvk_learn_feat(const pvk_grid *g, const pvk_scal *sc, const double *r, const double *z, const double *rho,
+                    float *in3, hipStream_t s);
+void pvk_learn_combine(const pvk_grid *g, const pvk_scal *sc, const double *z, const float *c, double *out,
+                       hipStream_t s);
+void pvk_learn_combine_dots(const pvk_grid *g, const pvk_scal *sc, double *z, const float *c, const double *r,
+                            const double *Ad, double *rzp, double *zadp, hipStream_t s);
+void pvk_learn_target(const pvk_grid *g, const pvk_scal *sc, const double *phistar, const double *phik,
+                      const double *z, float *tgt, hipStream_t s);
+void pvk_learn_crop(const float *in3, const float *tgt, int NXM, int cs, const int *org, int n, int base,
+                    float *ds_in, float *ds_tgt, hipStream_t s);
+void pvk_learn_gather(const float *ds_in, const float *ds_tgt, int cs, const int *idx, int B, float *bin,
+                      float *btgt, hipStream_t s);
 
 /* learned preconditioner network (csrc/pv_cnn.hip) */
 void pvk_cnn_pack(const pvk_cnn *c, hipStream_t s);

@@ -112,8 +112,11 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
-    const double gam = block_sum(w.rz[n & 1], w.nrz, sh);
-    const double beta = (n == 0) ? 0.0 : gam / block_sum(w.rz[(n + 1) & 1], w.nrz, sh);
+    double beta = 0.0;
+    if (n > 0 && w.flex)      // flexible CG: A-orthogonal to the previous direction only
+        beta = -block_sum(w.zad, w.nrz, sh) / block_sum(w.dad[(n + 1) & 1], w.ndad, sh);
+    else if (n > 0)           // Fletcher-Reeves: gamma_n / gamma_{n-1}
+        beta = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.rz[(n + 1) & 1], w.nrz, sh);
     const double *__restrict__ z = w.z;
     const double *__restrict__ dold = w.d[(n + 1) & 1];
     const double *__restrict__ bx = w.bx, *__restrict__ by = w.by;
@@ -149,7 +152,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
             }
         }
     }
-    block_store_partial(acc, w.dad, blockIdx.x, sh);
+    block_store_partial(acc, w.dad[n & 1], blockIdx.x, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -165,7 +168,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);
     double alpha = 0.0, mean = 0.0;
-    if (n >= 0) alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad, w.ndad, sh);
+    if (n >= 0) alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
     else mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
     const double *__restrict__ d = (n >= 0) ? w.d[n & 1] : w.rhs;
     const double *__restrict__ Ad = w.Ad, *__restrict__ rhs = w.rhs;
@@ -182,6 +185,10 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
                 ea[u] = AT(d, i, j);   eb[u] = AT(d, i + 1, j);
                 ra[u] = AT(r, i, j) - alpha * AT(Ad, i, j);
                 rb[u] = AT(r, i + 1, j) - alpha * AT(Ad, i + 1, j);
+            } else if (n == -2) {               // warm restart: Ad holds A phi0
+                xa[u] = AT(phi, i, j); xb[u] = AT(phi, i + 1, j);
+                ra[u] = AT(rhs, i, j) - mean - AT(Ad, i, j);
+                rb[u] = AT(rhs, i + 1, j) - mean - AT(Ad, i + 1, j);
             } else {
                 ra[u] = AT(rhs, i, j) - mean;
                 rb[u] = AT(rhs, i + 1, j) - mean;
@@ -193,8 +200,8 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
     for (int u = 0; u < IT; u++) {
         const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
         if (t < RM && j < g.NYM) {
-            AT(phi, i, j) = (n >= 0) ? xa[u] + alpha * ea[u] : 0.0;
-            AT(phi, i + 1, j) = (n >= 0) ? xb[u] + alpha * eb[u] : 0.0;
+            AT(phi, i, j) = (n >= 0) ? xa[u] + alpha * ea[u] : (n == -2 ? xa[u] : 0.0);
+            AT(phi, i + 1, j) = (n >= 0) ? xb[u] + alpha * eb[u] : (n == -2 ? xb[u] : 0.0);
             AT(r, i, j) = ra[u];
             AT(r, i + 1, j) = rb[u];
             acc += ra[u] * ra[u] + rb[u] * rb[u];
@@ -239,13 +246,15 @@ __global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, cons
     if (sc->p_done) return;
     if (check) {
         double rr = block_sum(w.rr, w.nrr, sh);
-        double bn = (n < 0) ? ((sqrt(rr) == 0.0) ? 1.0 : sqrt(rr)) : sc->p_bnorm;
+        // n = -1: cold start, ||r0|| = ||b - mean|| is the reference norm;
+        // n = -2: warm restart, the cold start's norm is kept
+        double bn = (n == -1) ? ((sqrt(rr) == 0.0) ? 1.0 : sqrt(rr)) : sc->p_bnorm;
         double rel = sqrt(rr) / bn;
         bool done = (rel <= tol) || (n + 1 >= maxit && n >= 0);
         if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) {
-            if (n < 0) sc->p_bnorm = bn;
+            if (n == -1) sc->p_bnorm = bn;
             sc->p_res = rel;
-            sc->p_iter = n + 1;
+            sc->p_iter = (n < 0) ? 0 : n + 1;
             if (done) sc->p_done = (rel <= tol) ? 1 : 3;
         }
         if (done) return;
@@ -422,7 +431,26 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
             acc += ra[u] * a + rb[u] * b;
         }
     }
-    block_store_partial(acc, w.rz[(n + 1) & 1], blockIdx.x, sh);
+    block_store_partial(acc, w.rz[(n < 0) ? 0 : ((n + 1) & 1)], blockIdx.x, sh);
+}
+
+// out = A x (the operator of k_pcg_apply on a given field; faces formed by
+// the start of the solve)
+__global__ void __launch_bounds__(NTH) k_pcg_opx(pvk_grid g, pvk_pcg w, const double *__restrict__ x,
+                                                double *__restrict__ out)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM, j = blockIdx.x;
+    const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
+    for (int i = threadIdx.x; i < NXM; i += NTH) {
+        const int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
+        const double d0 = AT(x, i, j);
+        AT(out, i, j) = AT(w.bx, i, j) * (AT(x, ie, j) - d0) - AT(w.bx, i - 1, j) * (d0 - AT(x, iw, j)) +
+                        AT(w.by, i, j) * (AT(x, i, jn) - d0) - AT(w.by, i, j - 1) * (d0 - AT(x, i, js));
+    }
+}
+extern "C" void pvk_pcg_opx(const pvk_grid *g, const pvk_pcg *w, const double *x, double *out, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_opx, dim3(g->NYM), dim3(NTH), 0, s, *g, *w, x, out);
 }
 
 // partial sums of rhs (mean removal before the solve)

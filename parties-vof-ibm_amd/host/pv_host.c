@@ -129,6 +129,10 @@ void pv_host_metrics_free(pv_metrics *m)
 
 static double *fld(pv_bag *b, int f) { return b->f[f]; }
 
+/* doubles of one field allocation, and its cell (0,0) given the allocation base */
+size_t pv_field_elems(const pv_bag *b) { return b->alloc_elems; }
+double *pv_field_origin(const pv_bag *b, double *base) { return base + OFF0 + 3 * (size_t)b->P + 3; }
+
 static void fill_phys(pv_bag *b)
 {
     pv_params *p = &b->prm;
@@ -165,7 +169,7 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
         return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
     if (prm->precision != PV_PREC_FP64 && prm->precision != PV_PREC_MIXED)
         return pv_set_error(PV_EINVAL, "unknown precision %d", prm->precision);
-    if (prm->pressure_solver != PV_PSOLVE_JACOBI_PCG && prm->pressure_solver != PV_PSOLVE_FFT_PCG)
+    if (prm->pressure_solver < PV_PSOLVE_JACOBI_PCG || prm->pressure_solver > PV_PSOLVE_LEARNED_FCG)
         return pv_set_error(PV_EINVAL, "unknown pressure_solver %d", prm->pressure_solver);
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
         return pv_set_error(PV_ENODEV, "no HIP device %d (count %d)", device, ndev);
@@ -306,6 +310,8 @@ int pv_destroy(pv_bag *b)
     hipStreamSynchronize(b->stream);
     if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
+    pv_learn_free(b);
+    pv_graphs_free(b);
     pv_cnn_free(b);
     hipFree(b->cgmem);
     hipFree(b->slabmem);
@@ -499,13 +505,15 @@ int pv_fft_create(pv_bag *b)
         np = ny;                       /* the operator kernel stores one partial per row */
         if (nband > np) np = nband;
         if (nrow > np) np = nrow;
-        if (hipMalloc((void **)&pm, sizeof(double) * ((size_t)nx * ny + 5 * (size_t)np)) != hipSuccess)
+        if (hipMalloc((void **)&pm, sizeof(double) * ((size_t)nx * ny + 7 * (size_t)np)) != hipSuccess)
             return pv_set_error(PV_ENOMEM, "PCG work allocation failed");
         b->pcgmem = pm;
         memset(&b->pcg, 0, sizeof b->pcg);
         b->pcg.specT = pm;
-        b->pcg.dad = pm + (size_t)nx * ny;
-        b->pcg.rr = b->pcg.dad + np;
+        b->pcg.dad[0] = pm + (size_t)nx * ny;
+        b->pcg.dad[1] = b->pcg.dad[0] + np;
+        b->pcg.zad = b->pcg.dad[1] + np;
+        b->pcg.rr = b->pcg.zad + np;
         b->pcg.rz[0] = b->pcg.rr + np;
         b->pcg.rz[1] = b->pcg.rz[0] + np;
         b->pcg.mean = b->pcg.rz[1] + np;

@@ -30,6 +30,20 @@ typedef struct pv_metrics {
     double *dy_v, *dy_c, *idy_v, *idy_c;
 } pv_metrics;
 
+/* learned-solve training collector (host/pv_learn_host.c) */
+#define PV_LEARN_MAXSNAP 8
+typedef struct pv_learn {
+    int collecting, nsnap, snap_iter[PV_LEARN_MAXSNAP];
+    int cs, crops_per_snap, ds_cap, ds_n;
+    unsigned long long rng;
+    double *snapmem, *sphi[PV_LEARN_MAXSNAP], *sz[PV_LEARN_MAXSNAP];
+    float *feat;                   /* [nsnap][NYM][NXM][3] */
+    float *tgt, *cout;             /* [NYM][NXM] target / network output */
+    double *part;                  /* 3 NYM row partials */
+    float *ds_in, *ds_tgt;         /* dataset crops [n][cs][cs][3], [n][cs][cs] */
+    int *d_org, *d_idx;
+} pv_learn;
+
 struct pv_bag {
     pv_params prm;
     double slab;
@@ -72,6 +86,11 @@ struct pv_bag {
     size_t cnn_io_cap;
     pvk_mcg mcg;                   /* fused momentum PCG work set */
     double *cgmem;
+    pv_learn *learn;               /* collector / learned-solve state, or NULL */
+    int use_graphs, graph_chunk;   /* pressure FFT-PCG chunks replayed from hipGraphs */
+    hipGraphExec_t *pgraph;        /* one executable graph per chunk index */
+    int npgraph;
+    double learn_start_res;        /* relative residual after the last warm start */
 };
 
 int  pv_set_error(int code, const char *fmt, ...);
@@ -84,6 +103,14 @@ double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
 int  pv_cg_create(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
+int  pv_learn_snapshot(pv_bag *b, int si);
+int  pv_learn_harvest(pv_bag *b, int nsnap);
+int  pv_learn_warm(pv_bag *b, double *phi);
+int  pv_learn_correct(pv_bag *b, double *rzp);
+void pv_learn_free(pv_bag *b);
+void pv_graphs_free(pv_bag *b);
+size_t pv_field_elems(const pv_bag *b);
+double *pv_field_origin(const pv_bag *b, double *base);
 void pv_tbegin(pv_bag *b, int t);
 void pv_tend(pv_bag *b, int t);
 int  pv_timers_flush(pv_bag *b);

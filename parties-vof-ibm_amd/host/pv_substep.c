@@ -404,25 +404,77 @@ static int p_solve_jacobi(pv_bag *b)
 static const int pcg_timer[6] = { PV_T_P_APPLY, PV_T_P_UPDATE, PV_T_P_TRANSPOSE, PV_T_FFT_COLS,
                                   PV_T_P_TRANSPOSE, PV_T_FFT_ROWS_INV };
 
-static void pcg_iteration(pv_bag *b, int n, double tol, int maxit)
+static void pcg_iteration_w(pv_bag *b, const pvk_pcg *w, int n, double tol, int maxit)
 {
     int st;
     for (st = (n < 0) ? 2 : 0; st < 6; st++) {
         pv_tbegin(b, pcg_timer[st]);
-        pvk_pcg_stage(G(b), &b->fft, b->d_sc, &b->pcg, st, n, tol, maxit, S(b));
+        pvk_pcg_stage(G(b), &b->fft, b->d_sc, w, st, n, tol, maxit, S(b));
         pv_tend(b, pcg_timer[st]);
     }
 }
 
-static int p_solve_fft_pcg(pv_bag *b)
+static void pcg_iteration(pv_bag *b, int n, double tol, int maxit) { pcg_iteration_w(b, &b->pcg, n, tol, maxit); }
+
+/* start of the FFT-PCG: phi = 0, r = rhs - mean, z = M_fft r, ||b|| */
+static void pcg_cold_start(pv_bag *b, double tol, int maxit)
 {
-    int maxit = b->ph.p_max_iter, chunk = b->p_chunk, n = 0;
-    double tol = b->ph.p_rel_tol;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
     pv_tbegin(b, PV_T_P_UPDATE);
     pvk_pcg_start(G(b), &b->fft, b->d_sc, &b->pcg, tol, S(b));
     pv_tend(b, PV_T_P_UPDATE);
     pcg_iteration(b, -1, tol, maxit);
+}
+
+/* Iterations 0, 1, ... replayed from hipGraphs: graph kc holds iterations
+ * [kc C, (kc + 1) C), captured on first use (kernel arguments are fixed per
+ * handle, and the kernels stop themselves on the device flag). */
+static int pcg_loop_graphs(pv_bag *b, double tol, int maxit)
+{
+    const int C = b->graph_chunk;
+    int kc = 0, n = 0;
+    for (;;) {
+        if (kc >= b->npgraph) {
+            int cap = (kc + 1) * 2, k;
+            hipGraphExec_t *ng = (hipGraphExec_t *)realloc(b->pgraph, cap * sizeof(hipGraphExec_t));
+            if (!ng) return pv_set_error(PV_ENOMEM, "graph table allocation failed");
+            for (k = b->npgraph; k < cap; k++) ng[k] = NULL;
+            b->pgraph = ng;
+            b->npgraph = cap;
+        }
+        if (!b->pgraph[kc]) {
+            hipGraph_t gr;
+            int k;
+            HIPCHK(hipStreamBeginCapture(S(b), hipStreamCaptureModeThreadLocal));
+            for (k = 0; k < C && kc * C + k < maxit; k++) pcg_iteration(b, kc * C + k, tol, maxit);
+            HIPCHK(hipStreamEndCapture(S(b), &gr));
+            HIPCHK(hipGraphInstantiate(&b->pgraph[kc], gr, NULL, NULL, 0));
+            HIPCHK(hipGraphDestroy(gr));
+        }
+        HIPCHK(hipGraphLaunch(b->pgraph[kc], S(b)));
+        n = (kc + 1) * C;
+        kc++;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (n >= maxit) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
+    }
+    return PV_OK;
+}
+
+void pv_graphs_free(pv_bag *b)
+{
+    int k;
+    for (k = 0; k < b->npgraph; k++)
+        if (b->pgraph[k]) hipGraphExecDestroy(b->pgraph[k]);
+    free(b->pgraph);
+    b->pgraph = NULL;
+    b->npgraph = 0;
+}
+
+/* iterations n0, n0 + 1, ... until the device stop flag */
+static int pcg_loop(pv_bag *b, int n0, double tol, int maxit)
+{
+    int chunk = b->p_chunk, n = n0;
+    if (b->use_graphs && n0 == 0 && !b->timers_on) return pcg_loop_graphs(b, tol, maxit);
     for (;;) {
         int k;
         for (k = 0; k < chunk && n < maxit; k++, n++) pcg_iteration(b, n, tol, maxit);
@@ -436,10 +488,134 @@ static int p_solve_fft_pcg(pv_bag *b)
     return PV_OK;
 }
 
+static int p_solve_fft_pcg(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter;
+    double tol = b->ph.p_rel_tol;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    pcg_cold_start(b, tol, maxit);
+    return pcg_loop(b, 0, tol, maxit);
+}
+
+/* The plain FFT-PCG with training snapshots (phi_k, z_k, network features)
+ * at the collector's iterations; crops are harvested after convergence. */
+static int p_solve_collect(pv_bag *b)
+{
+    pv_learn *L = b->learn;
+    int maxit = b->ph.p_max_iter, n = 0, si = 0;
+    double tol = b->ph.p_rel_tol;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    pcg_cold_start(b, tol, maxit);
+    for (;;) {
+        int next;
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (si < L->nsnap && n == L->snap_iter[si]) RC(pv_learn_snapshot(b, si++));
+        next = (si < L->nsnap) ? L->snap_iter[si] : maxit;
+        if (n >= maxit) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
+        for (; n < next && n < maxit; n++) pcg_iteration(b, n, tol, maxit);
+    }
+    return pv_learn_harvest(b, si);
+}
+
+/* Learned solve: the network's warm start phi0 = z + s_z c(r0, z0, rho), kept
+ * only if its residual is below ||b|| (else the cold start), then the
+ * FFT-PCG iterations. */
+static int p_solve_learned(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter;
+    double tol = b->ph.p_rel_tol;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    pcg_cold_start(b, tol, maxit);
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (b->h_sc->p_done) return PV_OK;
+    RC(pv_learn_warm(b, FD(b, PV_F_PHI)));
+    pvk_pcg_opx(G(b), &b->pcg, FD(b, PV_F_PHI), b->pcg.Ad, S(b));
+    pvk_pcg_stage(G(b), &b->fft, b->d_sc, &b->pcg, 1, -2, tol, maxit, S(b));
+    pcg_iteration(b, -2, tol, maxit);
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    b->learn_start_res = b->h_sc->p_res;
+    if (!b->h_sc->p_done && b->h_sc->p_res >= 1.0) {       /* warm start judged worse: cold start */
+        b->st.p_fallback[b->stage] += 1;
+        pcg_cold_start(b, tol, maxit);
+    }
+    if (b->h_sc->p_done) return PV_OK;
+    return pcg_loop(b, 0, tol, maxit);
+}
+
+/* warm restart of the FFT-PCG from the current phi: r = rhs - mean - A phi */
+static int pcg_warm_restart(pv_bag *b, double tol, int maxit)
+{
+    pvk_pcg_opx(G(b), &b->pcg, FD(b, PV_F_PHI), b->pcg.Ad, S(b));
+    pvk_pcg_stage(G(b), &b->fft, b->d_sc, &b->pcg, 1, -2, tol, maxit, S(b));
+    pcg_iteration(b, -2, tol, maxit);
+    LAUNCHCHK();
+    return pv_read_scalars(b);
+}
+
+/* Learned flexible CG: the network's warm start, then z = M_fft r + s_z c(r,
+ * M_fft r, rho) in every iteration with beta = -<z_{n+1}, A d_n> / <d_n, A d_n>.
+ * If the residual stalls (less than a factor 2 over 24 iterations) the solve
+ * restarts as the plain FFT-PCG from the current iterate. */
+static int p_solve_learned_fcg(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter, n = 0, since = 0, done_fcg;
+    double tol = b->ph.p_rel_tol, best;
+    pvk_pcg w;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    pcg_cold_start(b, tol, maxit);
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (b->h_sc->p_done) return PV_OK;
+    RC(pv_learn_warm(b, FD(b, PV_F_PHI)));
+    RC(pcg_warm_restart(b, tol, maxit));
+    b->learn_start_res = b->h_sc->p_res;
+    if (!b->h_sc->p_done && b->h_sc->p_res >= 1.0) {
+        b->st.p_fallback[b->stage] += 1;
+        pcg_cold_start(b, tol, maxit);
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+    }
+    if (b->h_sc->p_done) return PV_OK;
+    w = b->pcg;
+    w.flex = 1;
+    w.nrz = b->NYM;
+    RC(pv_learn_correct(b, w.rz[0]));
+    best = b->h_sc->p_res;
+    for (;;) {
+        int k;
+        for (k = 0; k < 8 && n < maxit; k++, n++) {
+            pcg_iteration_w(b, &w, n, tol, maxit);
+            RC(pv_learn_correct(b, w.rz[(n + 1) & 1]));
+        }
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) return PV_OK;
+        if (n >= maxit) return pv_set_error(PV_EHIP, "learned FCG did not stop (device flag)");
+        since += 8;
+        if (b->h_sc->p_res < 0.5 * best) { best = b->h_sc->p_res; since = 0; }
+        if (since >= 24) break;                                 /* stalled */
+    }
+    done_fcg = b->h_sc->p_iter;
+    b->st.p_fallback[b->stage] += 1;
+    RC(pcg_warm_restart(b, tol, maxit));
+    if (!b->h_sc->p_done) RC(pcg_loop(b, 0, tol, maxit));
+    b->st.n_p[b->stage] += done_fcg;
+    return PV_OK;
+}
+
 /* Pressure_solve_cg + the mean-removal gauge (psolve_cg.c:255-279) */
 static int p_solve(pv_bag *b)
 {
-    if (b->prm.pressure_solver == PV_PSOLVE_FFT_PCG) RC(p_solve_fft_pcg(b));
+    if (b->prm.pressure_solver == PV_PSOLVE_FFT_PCG)
+        RC((b->learn && b->learn->collecting) ? p_solve_collect(b) : p_solve_fft_pcg(b));
+    else if (b->prm.pressure_solver == PV_PSOLVE_LEARNED) RC(p_solve_learned(b));
+    else if (b->prm.pressure_solver == PV_PSOLVE_LEARNED_FCG) RC(p_solve_learned_fcg(b));
     else RC(p_solve_jacobi(b));
     if (b->h_sc->p_done != 4) {
         pvk_p_mean(G(b), FD(b, PV_F_PHI), &b->rd, S(b));
@@ -608,5 +784,49 @@ int pv_init_from_F(pv_bag *b, const double *F_host)
     RC(pv_sync_scalars_to_device(b));
     RC(pv_Velocity_cell_center(b));
     HIPCHK(hipStreamSynchronize(S(b)));
+    return PV_OK;
+}
+
+/* Replay the pressure FFT-PCG from hipGraphs of `chunk` iterations (on != 0)
+ * or launch it eagerly.  Graphs are dropped when the setting changes. */
+int pv_set_graphs(pv_bag *b, int on, int chunk)
+{
+    RC(begin(b));
+    if (on && (chunk < 1 || chunk > 256)) return pv_set_error(PV_EINVAL, "graph chunk must be in [1, 256]");
+    HIPCHK(hipStreamSynchronize(S(b)));
+    if (!on || chunk != b->graph_chunk) pv_graphs_free(b);
+    b->use_graphs = on ? 1 : 0;
+    if (on) b->graph_chunk = chunk;
+    return PV_OK;
+}
+
+/* Pick the fastest of eager launches and graph chunks 8, 16, 32 for the
+ * pressure solve of the current state (the solve restarts from phi = 0 and
+ * its right-hand side is left as it is, so the trajectory is unchanged). */
+int pv_autotune(pv_bag *b, int *choice)
+{
+    static const int chunks[4] = {0, 8, 16, 32};
+    double best = 1e300;
+    int k, rep, pick = 0, its;
+    RC(begin(b));
+    if (b->prm.pressure_solver != PV_PSOLVE_FFT_PCG) return pv_set_error(PV_EINVAL, "autotune is for FFT-PCG");
+    for (k = 0; k < 4; k++) {
+        hipEvent_t e0, e1;
+        float ms = 0.0f;
+        RC(pv_set_graphs(b, chunks[k] > 0, chunks[k] > 0 ? chunks[k] : 16));
+        RC(pv_Pressure_solve_cg(b, &its));                   /* warm-up, captures graphs */
+        HIPCHK(hipEventCreate(&e0));
+        HIPCHK(hipEventCreate(&e1));
+        HIPCHK(hipEventRecord(e0, S(b)));
+        for (rep = 0; rep < 2; rep++) RC(pv_Pressure_solve_cg(b, &its));
+        HIPCHK(hipEventRecord(e1, S(b)));
+        HIPCHK(hipEventSynchronize(e1));
+        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+        hipEventDestroy(e0);
+        hipEventDestroy(e1);
+        if (ms < best) { best = ms; pick = k; }
+    }
+    RC(pv_set_graphs(b, chunks[pick] > 0, chunks[pick] > 0 ? chunks[pick] : 16));
+    if (choice) *choice = chunks[pick];
     return PV_OK;
 }

@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(HERE, "_build", "libparties_vof.so")
 
 PV_PSOLVE_JACOBI_PCG = 0
 PV_PSOLVE_FFT_PCG = 1
+PV_PSOLVE_LEARNED = 2
+PV_PSOLVE_LEARNED_FCG = 3
 PV_PREC_FP64 = 0
 PV_PREC_MIXED = 1
 PV_VEL_TYPE_NORMAL = 0
@@ -152,6 +154,21 @@ def load_lib(path=LIB_PATH):
     lib.pv_cnn_get_params.argtypes = [vp, fp]
     lib.pv_cnn_eval.argtypes = [vp, fp, fp, fp, fp, dp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]
     lib.pv_cnn_adam_step.argtypes = [vp, C.c_float, C.c_int]
+    ip = C.POINTER(C.c_int)
+    lib.pv_set_pressure_solver.argtypes = [vp, C.c_int]
+    lib.pv_learn_collect_begin.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
+    lib.pv_learn_collect_end.argtypes = [vp, ip]
+    lib.pv_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, dp, dp]
+    lib.pv_learn_start_residual.argtypes = [vp, dp]
+    lib.pv_set_graphs.argtypes = [vp, C.c_int, C.c_int]
+    lib.pv_autotune.argtypes = [vp, ip]
+    lib.pv_batch_create.argtypes = [C.POINTER(Params), C.c_int, C.c_int, C.POINTER(vp)]
+    lib.pv_batch_destroy.argtypes = [vp]
+    lib.pv_batch_size.argtypes = [vp]
+    lib.pv_batch_case.argtypes = [vp, C.c_int]
+    lib.pv_batch_case.restype = vp
+    lib.pv_batch_rk3.argtypes = [vp, C.c_int]
+    lib.pv_batch_autotune.argtypes = [vp, ip]
     for name in ("pv_ic_rising_bubble", "pv_ic_rayleigh_taylor"):
         getattr(lib, name).argtypes = [C.POINTER(Params), dp]
     lib.pv_ic_multimode_rt.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_double, C.c_double, dp]
@@ -257,6 +274,73 @@ class Network:
 
     def adam_step(self, lr, t):
         self.s.check(self.s.lib.pv_cnn_adam_step(self.s.h, lr, t))
+
+    def collect(self, snap_iters, crops_per_snap=32, crop_size=64, max_samples=20000, seed=1):
+        """Start collecting training crops from the handle's plain FFT-PCG solves."""
+        it = (C.c_int * len(snap_iters))(*snap_iters)
+        self.s.check(self.s.lib.pv_learn_collect_begin(self.s.h, it, len(snap_iters), crops_per_snap, crop_size,
+                                                       max_samples, seed))
+
+    def collect_end(self):
+        n = C.c_int(0)
+        self.s.check(self.s.lib.pv_learn_collect_end(self.s.h, C.byref(n)))
+        return n.value
+
+    def train(self, steps, batch=32, lr=1e-3, seed=1):
+        l0, l1 = C.c_double(0.0), C.c_double(0.0)
+        self.s.check(self.s.lib.pv_learn_train(self.s.h, steps, batch, lr, seed, C.byref(l0), C.byref(l1)))
+        return l0.value, l1.value
+
+
+class Batch:
+    """Independent cases of any size advanced together (one handle and stream each)."""
+
+    def __init__(self, params_list, lib=None, device=0):
+        self.lib = lib or load_lib()
+        arr = (Params * len(params_list))(*params_list)
+        self.h = C.c_void_p()
+        rc = self.lib.pv_batch_create(arr, len(params_list), device, C.byref(self.h))
+        if rc != 0:
+            raise RuntimeError(f"parties_vof error {rc}: {self.lib.pv_last_error().decode()}")
+        self.cases = [_CaseView(self.lib, self.lib.pv_batch_case(self.h, k), params_list[k])
+                      for k in range(len(params_list))]
+
+    def check(self, rc):
+        if rc != 0:
+            raise RuntimeError(f"parties_vof error {rc}: {self.lib.pv_last_error().decode()}")
+
+    def rk3(self, nsteps):
+        self.check(self.lib.pv_batch_rk3(self.h, nsteps))
+
+    def autotune(self):
+        ch = (C.c_int * len(self.cases))()
+        self.check(self.lib.pv_batch_autotune(self.h, ch))
+        return list(ch)
+
+    def close(self):
+        if self.h:
+            self.lib.pv_batch_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class _CaseView(Solver):
+    """A Solver view of one batch case (owned by the batch, not destroyed here)."""
+
+    def __init__(self, lib, handle, params):
+        self.lib = lib
+        self.h = C.c_void_p(handle)
+        self.params = Params()
+        self.check(self.lib.pv_get_params(self.h, C.byref(self.params)))
+        self.shape = shape_of(self.params)
+
+    def close(self):
+        self.h = C.c_void_p()
 
 
 def ic(lib, kind, params, **kw):
