@@ -33,7 +33,7 @@ import pvof  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDATE", "P_DUPDATE",
-          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE"]
+          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE", "MG_VCYCLE"]
 # algorithmic HBM bytes per cell and launch (each distinct array read or
 # written once; stencil neighbours come from cache)
 BYTES_PER_CELL = {

@@ -52,6 +52,13 @@ extern "C" {
                                     CG, falling back to FFT-PCG when the residual stalls */
 #define PV_PSOLVE_FFT_PCG    1   /* PCG preconditioned by the FFT-x/Thomas-y solve
                                     of psolve_fft.c:1536-1802, relative-residual stop */
+#define PV_PSOLVE_MG_PCG     4   /* the same CG, start, gauge and stop test, preconditioned
+                                    by one geometric multigrid V-cycle on the variable face
+                                    coefficients (no counterpart in the reference): NXM and
+                                    NYM must halve together down to <= 1024 cells and on to
+                                    a last level of <= 64 cells */
+#define PV_PSOLVE_LEARNED_MG 5   /* MG-PCG from the learned network's warm start, kept
+                                    only when its energy is below the cold start's */
 
 /* ---- arithmetic selection ---------------------------------------------- */
 #define PV_PREC_FP64  0          /* all FP64 (the plain path) */
@@ -217,6 +224,7 @@ enum pv_timer {
     PV_T_CH_RHS,           /* fused explicit CH right-hand side  */
     PV_T_MOM_TERMS,        /* fused momentum right-hand side     */
     PV_T_P_TRANSPOSE,      /* spectrum transposes of the FFT-PCG */
+    PV_T_MG_VCYCLE,        /* multigrid V-cycle, all levels      */
     PV_NTIMERS
 };
 /* on = 0 disables; on = k >= 1 times every k-th launch of each timed kernel
@@ -271,6 +279,9 @@ int pv_batch_autotune(pv_batch *bt, int *chunks);
  * (r_k, M_fft r_k, rho) and the target of the network's correction, cut into
  * crop_size^2 crops (crops_per_snap per snapshot, at most max_samples). */
 int pv_set_pressure_solver(pv_bag *bag, int solver);
+/* Gauss-Seidel sweeps before and after the coarse correction of the
+ * multigrid V-cycle (1..4, default 2). */
+int pv_set_mg_sweeps(pv_bag *bag, int nu);
 int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
                            int max_samples, unsigned long long seed);
 int pv_learn_collect_end(pv_bag *bag, int *nsamples);

@@ -220,6 +220,34 @@ void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_
 void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
                    double tol, int maxit, hipStream_t s);
 
+/* sc->red[slot] = sum of the <r, z> partials rz[rzs] */
+void pvk_pcg_gamma(pvk_scal *sc, const pvk_pcg *w, int rzs, int slot, hipStream_t s);
+void pvk_pcg_prep(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, hipStream_t s);
+void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
+
+/* ---- multigrid-preconditioned CG (csrc/pv_mg.hip) ----
+ * Level 0 is the pressure grid (x = z, f = r, faces = the PCG faces, pitch
+ * P); levels 1.. are compact (pitch nx).  Levels c0.. (<= PVK_MG_LDS_CELLS
+ * cells) run inside one workgroup; the last level (<= PVK_MG_BOTTOM cells) is
+ * solved with the grounded inverse G. */
+#define PVK_MG_MAXLEV 16
+#define PVK_MG_LDS_CELLS 1024
+#define PVK_MG_BOTTOM 64
+typedef struct pvk_mglev {
+    int nx, ny, pitch;
+    double *x, *f, *bx, *by;
+} pvk_mglev;
+typedef struct pvk_mg {
+    int nlev, c0, nu;                 /* levels, first in-LDS level, sweeps per side */
+    pvk_mglev L[PVK_MG_MAXLEV];
+    double *G;                        /* bottom inverse, nb x nb */
+    int ldsx;                         /* LDS offsets: doubles per in-LDS array set */
+} pvk_mg;
+void pvk_mg_setup(const pvk_mg *m, hipStream_t s);
+void pvk_mg_vcycle(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s);
+int  pvk_mg_nrz(const pvk_mg *m);
+void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s);
+void pvk_mg_update(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
 /* out = A x with the pressure operator of the fused PCG (faces from its start) */
 void pvk_pcg_opx(const pvk_grid *g, const pvk_pcg *w, const double *x, double *out, hipStream_t s);
 
@@ -228,12 +256,15 @@ void pvk_learn_stats(const pvk_grid *g, pvk_scal *sc, const double *r, const dou
                      const double *b, double *part, hipStream_t s);
 void pvk_learn_feat(const pvk_grid *g, const pvk_scal *sc, const double *r, const double *z, const double *rho,
                     float *in3, hipStream_t s);
-void pvk_learn_combine(const pvk_grid *g, const pvk_scal *sc, const double *z, const float *c, double *out,
-                       hipStream_t s);
-void pvk_learn_combine_dots(const pvk_grid *g, const pvk_scal *sc, double *z, const float *c, const double *r,
-                            const double *Ad, double *rzp, double *zadp, hipStream_t s);
+
+
 void pvk_learn_target(const pvk_grid *g, const pvk_scal *sc, const double *phistar, const double *phik,
-                      const double *z, float *tgt, hipStream_t s);
+                      const double *r, double sr, float *tgt, hipStream_t s);
+void pvk_learn_addr(const pvk_grid *g, const pvk_scal *sc, const double *r, const float *c, double *out,
+                    hipStream_t s);
+void pvk_learn_dots(const pvk_grid *g, const double *z, const double *r, const double *Ad, double *rzp, double *zadp,
+                    hipStream_t s);
+void pvk_learn_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s);
 void pvk_learn_crop(const float *in3, const float *tgt, int NXM, int cs, const int *org, int n, int base,
                     float *ds_in, float *ds_tgt, hipStream_t s);
 void pvk_learn_gather(const float *ds_in, const float *ds_tgt, int cs, const int *idx, int B, float *bin,

@@ -5,9 +5,12 @@
 // For a residual r with FFT-preconditioned z = M_fft r the network input at
 // cell (i, j) is (r / s_r, z / s_z, log10(rho) / 1.5 + 1), s = RMS over the
 // cells, as an NHWC image of NYM rows and NXM columns (x periodic).  The
-// learned preconditioner returns z + s_z * c with c the network output; the
-// training target is (e - mean(e) - z) / s_z where e = A^-1 r is known from
-// a converged solve.
+// network predicts a source correction c: the learned approximation of
+// A^-1 r is M_fft(r + s_r c).  With e = A^-1 r known from a converged solve
+// the training target is (L e - r) / s_r, L the constant-coefficient
+// Laplacian that M_fft inverts; it is nonzero only where the density differs
+// from the liquid's (L - A = div((1 - beta) grad)), a local quantity, and
+// M_fft damps grid-scale noise of the network output as 1/k^2.
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "pv_kernels.h"
@@ -61,29 +64,26 @@ __global__ void __launch_bounds__(NTH) k_learn_feat(pvk_grid g, const pvk_scal *
     }
 }
 
-// out = z + s_z * c   (out may alias z)
-__global__ void __launch_bounds__(NTH) k_learn_combine(pvk_grid g, const pvk_scal *sc, const double *z,
-                                                      const float *c, double *out)
+// out = r + s_r c: the right-hand side whose constant-density solve M_fft
+// gives the learned approximation of A^-1 r
+__global__ void __launch_bounds__(NTH) k_learn_addr(pvk_grid g, const pvk_scal *sc, const double *r, const float *c,
+                                                   double *out)
 {
     const int P = g.P, j = blockIdx.x;
-    const double s = sc->red[6];
+    const double s = sc->red[5];
     for (int i = threadIdx.x; i < g.NXM; i += NTH)
-        AT(out, i, j) = AT(z, i, j) + s * (double)c[(size_t)j * g.NXM + i];
+        AT(out, i, j) = AT(r, i, j) + s * (double)c[(size_t)j * g.NXM + i];
 }
 
-// learned preconditioner output z <- z + s_z c with the flexible-CG partials
-// <r, z> -> rzp and <Ad, z> -> zadp (one block and one partial per row)
-__global__ void __launch_bounds__(NTH) k_learn_combine_dots(pvk_grid g, const pvk_scal *sc, double *z, const float *c,
-                                                           const double *r, const double *Ad, double *rzp,
-                                                           double *zadp)
+// flexible-CG partials <r, z> -> rzp and <Ad, z> -> zadp, one per row
+__global__ void __launch_bounds__(NTH) k_learn_dots(pvk_grid g, const double *z, const double *r, const double *Ad,
+                                                   double *rzp, double *zadp)
 {
     __shared__ double sh[NTH];
     const int P = g.P, j = blockIdx.x;
-    const double s = sc->red[6];
     double a0 = 0.0, a1 = 0.0;
     for (int i = threadIdx.x; i < g.NXM; i += NTH) {
-        const double v = AT(z, i, j) + s * (double)c[(size_t)j * g.NXM + i];
-        AT(z, i, j) = v;
+        const double v = AT(z, i, j);
         a0 += AT(r, i, j) * v;
         a1 += AT(Ad, i, j) * v;
     }
@@ -91,15 +91,29 @@ __global__ void __launch_bounds__(NTH) k_learn_combine_dots(pvk_grid g, const pv
     block_store_partial(a1, zadp, j, sh);
 }
 
-// target t = (e - mean(e) - z) / s_z with e = phistar - phik
-__global__ void __launch_bounds__(NTH) k_learn_target(pvk_grid g, const pvk_scal *sc, const double *phistar,
-                                                     const double *phik, const double *z, float *tgt)
+__global__ void __launch_bounds__(NTH) k_learn_copy(pvk_grid g, const double *src, double *dst)
 {
     const int P = g.P, j = blockIdx.x;
-    const double iz = (sc->red[6] > 0.0) ? 1.0 / sc->red[6] : 0.0, em = sc->red[7];
-    for (int i = threadIdx.x; i < g.NXM; i += NTH)
-        tgt[(size_t)j * g.NXM + i] =
-            (float)((AT(phistar, i, j) - (phik ? AT(phik, i, j) : 0.0) - em - AT(z, i, j)) * iz);
+    for (int i = threadIdx.x; i < g.NXM; i += NTH) AT(dst, i, j) = AT(src, i, j);
+}
+
+// target t = (L e - r) / s_r, e = phistar - phik, L the constant-coefficient
+// 5-point Laplacian that M_fft inverts (x periodic, zero-gradient y walls):
+// the source correction that makes M_fft(r + s_r t) = e.
+__global__ void __launch_bounds__(NTH) k_learn_target(pvk_grid g, const double *phistar, const double *phik,
+                                                     const double *r, double sr, float *tgt)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM, j = blockIdx.x;
+    const double idx2 = g.idx_c[1] * g.idx_c[1], idy2 = g.idy_c[1] * g.idy_c[1], is = (sr > 0.0) ? 1.0 / sr : 0.0;
+    const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
+    for (int i = threadIdx.x; i < NXM; i += NTH) {
+        const int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
+#define EV(ii, jj) (AT(phistar, ii, jj) - AT(phik, ii, jj))
+        const double e0 = EV(i, j);
+        const double Le = (EV(ie, j) - 2.0 * e0 + EV(iw, j)) * idx2 + (EV(i, jn) - 2.0 * e0 + EV(i, js)) * idy2;
+#undef EV
+        tgt[(size_t)j * NXM + i] = (float)((Le - AT(r, i, j)) * is);
+    }
 }
 
 // crop n of side cs at origin org[2n], org[2n+1] (x, y) -> dataset slot base + n
@@ -141,20 +155,25 @@ extern "C" void pvk_learn_feat(const pvk_grid *g, const pvk_scal *sc, const doub
 {
     hipLaunchKernelGGL(k_learn_feat, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, r, z, rho, in3);
 }
-extern "C" void pvk_learn_combine(const pvk_grid *g, const pvk_scal *sc, const double *z, const float *c,
-                                  double *out, hipStream_t s)
+extern "C" void pvk_learn_addr(const pvk_grid *g, const pvk_scal *sc, const double *r, const float *c, double *out,
+                               hipStream_t s)
 {
-    hipLaunchKernelGGL(k_learn_combine, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, z, c, out);
+    hipLaunchKernelGGL(k_learn_addr, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, r, c, out);
 }
-extern "C" void pvk_learn_combine_dots(const pvk_grid *g, const pvk_scal *sc, double *z, const float *c,
-                                       const double *r, const double *Ad, double *rzp, double *zadp, hipStream_t s)
+extern "C" void pvk_learn_dots(const pvk_grid *g, const double *z, const double *r, const double *Ad, double *rzp,
+                               double *zadp, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_learn_combine_dots, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, z, c, r, Ad, rzp, zadp);
+    hipLaunchKernelGGL(k_learn_dots, dim3(g->NYM), dim3(NTH), 0, s, *g, z, r, Ad, rzp, zadp);
+}
+extern "C" void pvk_learn_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_learn_copy, dim3(g->NYM), dim3(NTH), 0, s, *g, src, dst);
 }
 extern "C" void pvk_learn_target(const pvk_grid *g, const pvk_scal *sc, const double *phistar, const double *phik,
-                                 const double *z, float *tgt, hipStream_t s)
+                                 const double *r, double sr, float *tgt, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_learn_target, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, phistar, phik, z, tgt);
+    (void)sc;
+    hipLaunchKernelGGL(k_learn_target, dim3(g->NYM), dim3(NTH), 0, s, *g, phistar, phik, r, sr, tgt);
 }
 extern "C" void pvk_learn_crop(const float *in3, const float *tgt, int NXM, int cs, const int *org, int n,
                                int base, float *ds_in, float *ds_tgt, hipStream_t s)

@@ -185,6 +185,9 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
                 ea[u] = AT(d, i, j);   eb[u] = AT(d, i + 1, j);
                 ra[u] = AT(r, i, j) - alpha * AT(Ad, i, j);
                 rb[u] = AT(r, i + 1, j) - alpha * AT(Ad, i + 1, j);
+            } else if (n == -3) {               // M_fft of the current r, no update
+                ra[u] = AT(r, i, j);
+                rb[u] = AT(r, i + 1, j);
             } else if (n == -2) {               // warm restart: Ad holds A phi0
                 xa[u] = AT(phi, i, j); xb[u] = AT(phi, i + 1, j);
                 ra[u] = AT(rhs, i, j) - mean - AT(Ad, i, j);
@@ -199,13 +202,13 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
 #pragma unroll
     for (int u = 0; u < IT; u++) {
         const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
-        if (t < RM && j < g.NYM) {
+        if (t < RM && j < g.NYM && n != -3) {
             AT(phi, i, j) = (n >= 0) ? xa[u] + alpha * ea[u] : (n == -2 ? xa[u] : 0.0);
             AT(phi, i + 1, j) = (n >= 0) ? xb[u] + alpha * eb[u] : (n == -2 ? xb[u] : 0.0);
             AT(r, i, j) = ra[u];
             AT(r, i + 1, j) = rb[u];
-            acc += ra[u] * ra[u] + rb[u] * rb[u];
         }
+        if (t < RM && j < g.NYM) acc += ra[u] * ra[u] + rb[u] * rb[u];
         if (t < RM) { sr[t] = ra[u]; si[t] = rb[u]; }
     }
     block_store_partial(acc, w.rr, blockIdx.x, sh);
@@ -244,7 +247,7 @@ __global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, cons
     __shared__ double sh[NTH];
     // a stop flag set by an earlier iteration leaves <r,r> stale: no recount
     if (sc->p_done) return;
-    if (check) {
+    if (check && n != -3) {                 // n = -3: a plain M_fft solve, no stop test
         double rr = block_sum(w.rr, w.nrr, sh);
         // n = -1: cold start, ||r0|| = ||b - mean|| is the reference norm;
         // n = -2: warm restart, the cold start's norm is kept
@@ -434,6 +437,19 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
     block_store_partial(acc, w.rz[(n < 0) ? 0 : ((n + 1) & 1)], blockIdx.x, sh);
 }
 
+// sc->red[slot] = sum of the <r, z> partials of slot rzs (the energy-norm
+// proxy <r, M r> that judges a warm start)
+__global__ void __launch_bounds__(NTH) k_pcg_gamma(pvk_scal *sc, pvk_pcg w, int rzs, int slot)
+{
+    __shared__ double sh[NTH];
+    const double v = block_sum(w.rz[rzs], w.nrz, sh);
+    if (threadIdx.x == 0) sc->red[slot] = v;
+}
+extern "C" void pvk_pcg_gamma(pvk_scal *sc, const pvk_pcg *w, int rzs, int slot, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_gamma, dim3(1), dim3(NTH), 0, s, sc, *w, rzs, slot);
+}
+
 // out = A x (the operator of k_pcg_apply on a given field; faces formed by
 // the start of the solve)
 __global__ void __launch_bounds__(NTH) k_pcg_opx(pvk_grid g, pvk_pcg w, const double *__restrict__ x,
@@ -479,6 +495,20 @@ extern "C" void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
     hipLaunchKernelGGL(k_pcg_mean, dim3(nband), dim3(NTH), 0, s, *g, sc, *w);
     hipLaunchKernelGGL(k_pcg_update_fwd, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, -1);
     (void)tol;
+}
+
+// face coefficients + partial sums of rhs (resets the stop flag): the start
+// of a solve for preconditioners other than the FFT one
+extern "C" void pvk_pcg_prep(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_faces, dim3(g->NYM + 1), dim3(NTH), 0, s, *g, *w);
+    hipLaunchKernelGGL(k_pcg_mean, dim3((g->NYM + w->band - 1) / w->band), dim3(NTH), 0, s, *g, sc, *w);
+}
+
+// K1 alone (d_n = z + beta d_{n-1}, Ad, <d_n, Ad>), for any preconditioner
+extern "C" void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_apply, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, *w, n);
 }
 
 // stage: 0 = K1 apply, 1 = K2 update + forward rows, 2 = K3 check + transpose,

@@ -114,6 +114,7 @@ int pv_cnn_set_params(pv_bag *b, const float *host)
 {
     if (!b || !b->cnn_ready) return pv_set_error(PV_EINVAL, "network not initialised (pv_cnn_init)");
     HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));   /* work in flight may still read the weights */
     HIPCHK(hipMemcpy(b->cnn.w, host, PVK_CNN_NPARAM * sizeof(float), hipMemcpyHostToDevice));
     HIPCHK(hipMemset(b->cnn.m, 0, PVK_CNN_NPARAM * sizeof(float)));
     HIPCHK(hipMemset(b->cnn.v, 0, PVK_CNN_NPARAM * sizeof(float)));

@@ -169,7 +169,7 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
         return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
     if (prm->precision != PV_PREC_FP64 && prm->precision != PV_PREC_MIXED)
         return pv_set_error(PV_EINVAL, "unknown precision %d", prm->precision);
-    if (prm->pressure_solver < PV_PSOLVE_JACOBI_PCG || prm->pressure_solver > PV_PSOLVE_LEARNED_FCG)
+    if (prm->pressure_solver < PV_PSOLVE_JACOBI_PCG || prm->pressure_solver > PV_PSOLVE_LEARNED_MG)
         return pv_set_error(PV_EINVAL, "unknown pressure_solver %d", prm->pressure_solver);
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
         return pv_set_error(PV_ENODEV, "no HIP device %d (count %d)", device, ndev);
@@ -250,6 +250,13 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     {
         int rc = pv_fft_create(b);
         if (rc) return rc;
+        if ((rc = pv_mg_create(b))) return rc;
+        if ((b->prm.pressure_solver == PV_PSOLVE_MG_PCG || b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG) &&
+            !b->mg_ok) {
+            pv_destroy(b);
+            return pv_set_error(PV_EINVAL, "MG-PCG: %d x %d does not halve to <= %d cells and a last level "
+                                "of <= %d cells", prm->NXM, prm->NYM, PVK_MG_LDS_CELLS, PVK_MG_BOTTOM);
+        }
     }
     {
         int rc = pv_sync_scalars_to_device(b);
@@ -310,6 +317,7 @@ int pv_destroy(pv_bag *b)
     hipStreamSynchronize(b->stream);
     if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
+    pv_mg_destroy(b);
     pv_learn_free(b);
     pv_graphs_free(b);
     pv_cnn_free(b);

@@ -18,6 +18,7 @@ enum pv_internal_field {
     PV_FI_DV2,
     PV_FI_PBX,                 /* pressure operator face coefficients, x and y */
     PV_FI_PBY,
+    PV_FI_LTMP,                /* right-hand side of the learned preconditioner's FFT solve */
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
@@ -36,7 +37,8 @@ typedef struct pv_learn {
     int collecting, nsnap, snap_iter[PV_LEARN_MAXSNAP];
     int cs, crops_per_snap, ds_cap, ds_n;
     unsigned long long rng;
-    double *snapmem, *sphi[PV_LEARN_MAXSNAP], *sz[PV_LEARN_MAXSNAP];
+    double *snapmem, *sphi[PV_LEARN_MAXSNAP], *sz[PV_LEARN_MAXSNAP], *sr[PV_LEARN_MAXSNAP];
+    double sig_r[PV_LEARN_MAXSNAP];  /* RMS of r_k at each snapshot */
     float *feat;                   /* [nsnap][NYM][NXM][3] */
     float *tgt, *cout;             /* [NYM][NXM] target / network output */
     double *part;                  /* 3 NYM row partials */
@@ -79,6 +81,10 @@ struct pv_bag {
     double *fftmem;
     pvk_pcg pcg;
     double *pcgmem;
+    pvk_mg mg;                     /* multigrid preconditioner (mg_ok) */
+    pvk_pcg mgw;                   /* CG work set of the MG-PCG: fields + its own partials */
+    double *mgmem;
+    int mg_ok, mg_chunk;
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
     pvk_cnn cnn;                   /* learned preconditioner network (pv_cnn_init) */
     int cnn_ready;
@@ -102,11 +108,12 @@ int  pv_read_scalars(pv_bag *b);
 double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
 int  pv_cg_create(pv_bag *b);
+int  pv_mg_create(pv_bag *b);
+void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
 int  pv_learn_snapshot(pv_bag *b, int si);
 int  pv_learn_harvest(pv_bag *b, int nsnap);
-int  pv_learn_warm(pv_bag *b, double *phi);
-int  pv_learn_correct(pv_bag *b, double *rzp);
+int  pv_learn_source(pv_bag *b, double *out);
 void pv_learn_free(pv_bag *b);
 void pv_graphs_free(pv_bag *b);
 size_t pv_field_elems(const pv_bag *b);

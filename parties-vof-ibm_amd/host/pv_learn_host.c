@@ -3,10 +3,12 @@
  * pressure solve (no counterpart in the reference).
  *
  * Data: while collecting, every plain FFT-PCG pressure solve stores at the
- * chosen iterations k the iterate phi_k, the FFT-preconditioned residual z_k
- * and the network features of (r_k, z_k, rho).  After convergence (phi*),
- * e_k = phi* - phi_k solves A e_k = r_k; the target (e_k - mean - z_k) / s_z
- * and the features are cut into random crops of side cs (kept in HBM).
+ * chosen iterations k the iterate phi_k, the residual r_k, the
+ * FFT-preconditioned z_k and the network features of (r_k, z_k, rho).
+ * After convergence (phi*), e_k = phi* - phi_k solves A e_k = r_k; the target
+ * is the source correction (L e_k - r_k) / s_r (L: the constant-coefficient
+ * Laplacian, M_fft = L^-1), cut with the features into random crops of side
+ * cs (kept in HBM).
  * Training: Adam on random batches of crops, loss on the crop interior
  * (6 pixels from the edges: the network's receptive-field radius).
  */
@@ -62,7 +64,7 @@ static int learn_alloc(pv_bag *b)
 int pv_set_pressure_solver(pv_bag *b, int solver)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");
-    if (solver < PV_PSOLVE_JACOBI_PCG || solver > PV_PSOLVE_LEARNED_FCG)
+    if (solver < PV_PSOLVE_JACOBI_PCG || solver > PV_PSOLVE_LEARNED_MG)
         return pv_set_error(PV_EINVAL, "unknown pressure_solver %d", solver);
     b->prm.pressure_solver = solver;
     return PV_OK;
@@ -87,10 +89,11 @@ int pv_learn_collect_begin(pv_bag *b, const int *snap_iters, int nsnap, int crop
     hipFree(L->snapmem); hipFree(L->feat); hipFree(L->ds_in); hipFree(L->ds_tgt); hipFree(L->d_org);
     L->snapmem = NULL; L->feat = NULL; L->ds_in = NULL; L->ds_tgt = NULL; L->d_org = NULL;
     fe = pv_field_elems(b);
-    HIPCHK(hipMalloc((void **)&L->snapmem, 2 * (size_t)nsnap * fe * sizeof(double)));
+    HIPCHK(hipMalloc((void **)&L->snapmem, 3 * (size_t)nsnap * fe * sizeof(double)));
     for (k = 0; k < nsnap; k++) {
-        L->sphi[k] = pv_field_origin(b, L->snapmem + (2 * (size_t)k) * fe);
-        L->sz[k] = pv_field_origin(b, L->snapmem + (2 * (size_t)k + 1) * fe);
+        L->sphi[k] = pv_field_origin(b, L->snapmem + (3 * (size_t)k) * fe);
+        L->sz[k] = pv_field_origin(b, L->snapmem + (3 * (size_t)k + 1) * fe);
+        L->sr[k] = pv_field_origin(b, L->snapmem + (3 * (size_t)k + 2) * fe);
         L->snap_iter[k] = snap_iters[k];
     }
     HIPCHK(hipMalloc((void **)&L->feat, (size_t)nsnap * npix * 3 * sizeof(float)));
@@ -116,20 +119,20 @@ int pv_learn_collect_end(pv_bag *b, int *nsamples)
     return PV_OK;
 }
 
-/* snapshot si of the running FFT-PCG: phi_k, z_k and the features of (r_k, z_k) */
+/* snapshot si of the running FFT-PCG: phi_k, z_k, r_k, s_r and the features
+ * of (r_k, z_k) */
 int pv_learn_snapshot(pv_bag *b, int si)
 {
     pv_learn *L = b->learn;
-    size_t npix = (size_t)b->NXM * b->NYM, bytes = pv_field_elems(b) * sizeof(double);
-    double *base_phi = L->sphi[si] - (pv_field_origin(b, L->snapmem) - L->snapmem);
-    double *base_z = L->sz[si] - (pv_field_origin(b, L->snapmem) - L->snapmem);
-    double *src_phi = b->f[PV_F_PHI] - (pv_field_origin(b, L->snapmem) - L->snapmem);
-    double *src_z = b->pcg.z - (pv_field_origin(b, L->snapmem) - L->snapmem);
-    HIPCHK(hipMemcpyAsync(base_phi, src_phi, bytes, hipMemcpyDeviceToDevice, b->stream));
-    HIPCHK(hipMemcpyAsync(base_z, src_z, bytes, hipMemcpyDeviceToDevice, b->stream));
+    size_t npix = (size_t)b->NXM * b->NYM;
+    pvk_learn_copy(&b->g, b->f[PV_F_PHI], L->sphi[si], b->stream);
+    pvk_learn_copy(&b->g, b->pcg.z, L->sz[si], b->stream);
+    pvk_learn_copy(&b->g, b->pcg.r, L->sr[si], b->stream);
     pvk_learn_stats(&b->g, b->d_sc, b->pcg.r, b->pcg.z, NULL, NULL, L->part, b->stream);
     pvk_learn_feat(&b->g, b->d_sc, b->pcg.r, b->pcg.z, b->f[PV_F_RHO], L->feat + (size_t)si * npix * 3, b->stream);
     HIPCHK(hipGetLastError());
+    if (pv_read_scalars(b)) return PV_EHIP;
+    L->sig_r[si] = b->h_sc->red[5];
     return PV_OK;
 }
 
@@ -144,8 +147,7 @@ int pv_learn_harvest(pv_bag *b, int nsnap)
     for (si = 0; si < nsnap && L->ds_n < L->ds_cap; si++) {
         int n = L->crops_per_snap;
         if (n > L->ds_cap - L->ds_n) n = L->ds_cap - L->ds_n;
-        pvk_learn_stats(&b->g, b->d_sc, NULL, L->sz[si], b->f[PV_F_PHI], L->sphi[si], L->part, b->stream);
-        pvk_learn_target(&b->g, b->d_sc, b->f[PV_F_PHI], L->sphi[si], L->sz[si], L->tgt, b->stream);
+        pvk_learn_target(&b->g, b->d_sc, b->f[PV_F_PHI], L->sphi[si], L->sr[si], L->sig_r[si], L->tgt, b->stream);
         for (k = 0; k < n; k++) {
             org[2 * k] = (int)(sm64(&L->rng) % (unsigned long long)(b->NXM - L->cs + 1));
             org[2 * k + 1] = (int)(sm64(&L->rng) % (unsigned long long)(b->NYM - L->cs + 1));
@@ -179,9 +181,12 @@ int pv_learn_train(pv_bag *b, int steps, int batch, float lr, unsigned long long
     HIPCHK(hipSetDevice(b->device));
     cs2 = (size_t)L->cs * L->cs;
     npix = (size_t)batch * cs2;
+    /* every step's sample indices are drawn up front and uploaded once, in
+     * stream order (the stream does not synchronise with the null stream) */
+    HIPCHK(hipStreamSynchronize(b->stream));
     hipFree(L->d_idx);
     L->d_idx = NULL;
-    HIPCHK(hipMalloc((void **)&L->d_idx, batch * sizeof(int)));
+    HIPCHK(hipMalloc((void **)&L->d_idx, (size_t)steps * batch * sizeof(int)));
     if (b->cnn_io_cap < npix * 5) {
         hipFree(b->cnn_io);
         b->cnn_io = NULL;
@@ -198,22 +203,25 @@ int pv_learn_train(pv_bag *b, int steps, int batch, float lr, unsigned long long
         RC(k);
     }
     bin = b->cnn_io; bout = bin + 3 * npix; btgt = bout + npix;
-    idx = (int *)malloc(batch * sizeof(int));
+    idx = (int *)malloc((size_t)steps * batch * sizeof(int));
     nblk = (int)(cs2 / 256) * batch;
     lp = (float *)malloc(nblk * sizeof(float));
     if (!idx || !lp) { free(idx); free(lp); return pv_set_error(PV_ENOMEM, "host allocation failed"); }
+    for (k = 0; k < steps * batch; k++) idx[k] = (int)(sm64(&s) % (unsigned long long)L->ds_n);
+    if (hipMemcpy(L->d_idx, idx, (size_t)steps * batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
+        free(idx); free(lp);
+        return pv_set_error(PV_EHIP, "batch index upload failed");
+    }
     for (t = 1; t <= steps; t++) {
-        for (k = 0; k < batch; k++) idx[k] = (int)(sm64(&s) % (unsigned long long)L->ds_n);
-        if (hipMemcpy(L->d_idx, idx, batch * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) {
-            free(idx); free(lp);
-            return pv_set_error(PV_EHIP, "batch index upload failed");
-        }
-        pvk_learn_gather(L->ds_in, L->ds_tgt, L->cs, L->d_idx, batch, bin, btgt, b->stream);
+        pvk_learn_gather(L->ds_in, L->ds_tgt, L->cs, L->d_idx + (size_t)(t - 1) * batch, batch, bin, btgt,
+                         b->stream);
         pvk_cnn_forward(&b->cnn, bin, bout, batch, L->cs, L->cs, 0, b->stream);
         pvk_cnn_backward(&b->cnn, bin, bout, btgt, batch, L->cs, L->cs, 0, CNN_BORDER, b->stream);
         if (t == 1 || t == steps) {
             double sum = 0.0;
-            if (hipMemcpy(lp, b->cnn.lpart, nblk * sizeof(float), hipMemcpyDeviceToHost) != hipSuccess) {
+            if (hipMemcpyAsync(lp, b->cnn.lpart, nblk * sizeof(float), hipMemcpyDeviceToHost, b->stream) !=
+                    hipSuccess ||
+                hipStreamSynchronize(b->stream) != hipSuccess) {
                 free(idx); free(lp);
                 return pv_set_error(PV_EHIP, "loss readback failed");
             }
@@ -230,8 +238,10 @@ int pv_learn_train(pv_bag *b, int steps, int batch, float lr, unsigned long long
     return PV_OK;
 }
 
-/* phi = z + s_z c(r, z, rho) for the current FFT-PCG start state (r, z) */
-int pv_learn_warm(pv_bag *b, double *phi)
+/* out = r + s_r c(r, z, rho) for the current (r, z = M_fft r) of the solve:
+ * the right-hand side whose M_fft solve is the learned approximation of
+ * A^-1 r */
+int pv_learn_source(pv_bag *b, double *out)
 {
     pv_learn *L;
     size_t npix = (size_t)b->NXM * b->NYM;
@@ -239,38 +249,18 @@ int pv_learn_warm(pv_bag *b, double *phi)
     RC(learn_alloc(b));
     L = b->learn;
     if (b->cnn.cap_pix < npix) {
-        float *tmp = (float *)calloc(16 * 16 * 3, sizeof(float));
+        float *big = (float *)calloc(npix * 3, sizeof(float));
         int rc;
-        if (!tmp) return pv_set_error(PV_ENOMEM, "host allocation failed");
-        /* grow the activation tensors to the grid through a small evaluation */
-        rc = pv_cnn_eval(b, tmp, NULL, NULL, NULL, NULL, 1, 16, 16, 0, 0);
-        free(tmp);
+        if (!big) return pv_set_error(PV_ENOMEM, "host allocation failed");
+        /* grows the activation tensors to the grid (a forward on zeros) */
+        rc = pv_cnn_eval(b, big, NULL, NULL, NULL, NULL, 1, b->NYM, b->NXM, 1, 0);
+        free(big);
         RC(rc);
-        if (b->cnn.cap_pix < npix) {
-            float *big = (float *)calloc(npix * 3, sizeof(float));
-            if (!big) return pv_set_error(PV_ENOMEM, "host allocation failed");
-            rc = pv_cnn_eval(b, big, NULL, NULL, NULL, NULL, 1, b->NYM, b->NXM, 1, 0);
-            free(big);
-            RC(rc);
-        }
     }
     pvk_learn_stats(&b->g, b->d_sc, b->pcg.r, b->pcg.z, NULL, NULL, L->part, b->stream);
     pvk_learn_feat(&b->g, b->d_sc, b->pcg.r, b->pcg.z, b->f[PV_F_RHO], L->feat, b->stream);
     pvk_cnn_forward(&b->cnn, L->feat, L->cout, 1, b->NYM, b->NXM, 1, b->stream);
-    pvk_learn_combine(&b->g, b->d_sc, b->pcg.z, L->cout, phi, b->stream);
-    HIPCHK(hipGetLastError());
-    return PV_OK;
-}
-
-/* learned preconditioner on the current (r, z = M_fft r): z <- z + s_z c and
- * the flexible-CG partials <r, z> -> rzp, <A d, z> -> zad (one per row) */
-int pv_learn_correct(pv_bag *b, double *rzp)
-{
-    pv_learn *L = b->learn;
-    pvk_learn_stats(&b->g, b->d_sc, b->pcg.r, b->pcg.z, NULL, NULL, L->part, b->stream);
-    pvk_learn_feat(&b->g, b->d_sc, b->pcg.r, b->pcg.z, b->f[PV_F_RHO], L->feat, b->stream);
-    pvk_cnn_forward(&b->cnn, L->feat, L->cout, 1, b->NYM, b->NXM, 1, b->stream);
-    pvk_learn_combine_dots(&b->g, b->d_sc, b->pcg.z, L->cout, b->pcg.r, b->pcg.Ad, rzp, b->pcg.zad, b->stream);
+    pvk_learn_addr(&b->g, b->d_sc, b->pcg.r, L->cout, out, b->stream);
     HIPCHK(hipGetLastError());
     return PV_OK;
 }

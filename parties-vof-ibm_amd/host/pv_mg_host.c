@@ -1,0 +1,121 @@
+/*
+ * pv_mg_host.c -- level hierarchy and work set of the multigrid-preconditioned
+ * pressure CG (no counterpart in the reference; its pressure CG is
+ * preconditioned by the FFT solve of psolve_fft.c or by Jacobi).
+ *
+ * Level 0 is the pressure grid itself (z, r and the face coefficients of the
+ * CG, device pitch P).  Levels halve both sizes while the level has more than
+ * PVK_MG_LDS_CELLS cells (the first level at or below that size, c0, and all
+ * levels after it run in one workgroup from LDS), then on to a last level of
+ * at most PVK_MG_BOTTOM cells.  Every level before the last must have even
+ * sizes; a grid that does not halve that way has no MG-PCG (mg_ok = 0).
+ */
+#include "pv_host.h"
+#include <stdlib.h>
+#include <string.h>
+
+int pv_mg_create(pv_bag *b)
+{
+    pvk_mg *m = &b->mg;
+    int lx[PVK_MG_MAXLEV], ly[PVK_MG_MAXLEV], nl = 1, c0, l, nrz, ny = b->NYM;
+    size_t cells = 0, lds = 0, total;
+    double *q;
+    memset(m, 0, sizeof *m);
+    memset(&b->mgw, 0, sizeof b->mgw);
+    b->mgmem = NULL;
+    b->mg_ok = 0;
+    b->mg_chunk = 4;
+    lx[0] = b->NXM;
+    ly[0] = b->NYM;
+    while (lx[nl - 1] * ly[nl - 1] > PVK_MG_LDS_CELLS) {
+        if ((lx[nl - 1] & 1) || (ly[nl - 1] & 1) || nl == PVK_MG_MAXLEV) return PV_OK;
+        lx[nl] = lx[nl - 1] / 2;
+        ly[nl] = ly[nl - 1] / 2;
+        nl++;
+    }
+    c0 = nl - 1;
+    while (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM) {
+        if ((lx[nl - 1] & 1) || (ly[nl - 1] & 1) || nl == PVK_MG_MAXLEV) return PV_OK;
+        lx[nl] = lx[nl - 1] / 2;
+        ly[nl] = ly[nl - 1] / 2;
+        nl++;
+    }
+    /* the smoothing levels (all but the last) pair their cells along x */
+    for (l = 0; l + 1 < nl; l++)
+        if (lx[l] & 1) return PV_OK;
+    m->nlev = nl;
+    m->c0 = c0;
+    m->nu = 2;
+    for (l = 1; l < nl; l++) cells += (size_t)lx[l] * ly[l];
+    for (l = c0; l < nl; l++) lds += 4 * (size_t)lx[l] * ly[l];
+    m->ldsx = (int)lds;
+    m->L[0].nx = lx[0];
+    m->L[0].ny = ly[0];
+    nrz = pvk_mg_nrz(m);
+    total = 4 * cells + (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 4 * (size_t)ny + 2 * (size_t)nrz;
+    if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
+        b->mgmem = NULL;
+        return pv_set_error(PV_ENOMEM, "MG-PCG work allocation failed");
+    }
+    q = b->mgmem;
+    m->L[0].pitch = b->g.P;
+    m->L[0].x = b->f[PV_FI_Z];
+    m->L[0].f = b->f[PV_F_P_RES];
+    m->L[0].bx = b->f[PV_FI_PBX];
+    m->L[0].by = b->f[PV_FI_PBY];
+    for (l = 1; l < nl; l++) {
+        size_t nc = (size_t)lx[l] * ly[l];
+        m->L[l].nx = lx[l];
+        m->L[l].ny = ly[l];
+        m->L[l].pitch = lx[l];
+        m->L[l].x = q; q += nc;
+        m->L[l].f = q; q += nc;
+        m->L[l].bx = q; q += nc;
+        m->L[l].by = q; q += nc;
+    }
+    m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
+    {
+        pvk_pcg *w = &b->mgw;
+        w->phi = b->f[PV_F_PHI];
+        w->r = b->f[PV_F_P_RES];
+        w->z = b->f[PV_FI_Z];
+        w->d[0] = b->f[PV_F_P_D];
+        w->d[1] = b->f[PV_FI_D2];
+        w->Ad = b->f[PV_F_P_AD];
+        w->bx = b->f[PV_FI_PBX];
+        w->by = b->f[PV_FI_PBY];
+        w->rhs = b->f[PV_F_P_RHS];
+        w->rho = b->f[PV_F_RHO];
+        w->dad[0] = q; q += ny;
+        w->dad[1] = q; q += ny;
+        w->rr = q; q += ny;
+        w->mean = q; q += ny;
+        w->rz[0] = q; q += nrz;
+        w->rz[1] = q; q += nrz;
+        w->ndad = ny;
+        w->nrr = ny;
+        w->nrz = nrz;
+        w->nmean = ny;
+        w->band = 1;
+        w->rrow = 1;
+        w->rdct = 1;
+        w->flex = 0;
+    }
+    b->mg_ok = 1;
+    return PV_OK;
+}
+
+void pv_mg_destroy(pv_bag *b)
+{
+    if (b->mgmem) hipFree(b->mgmem);
+    b->mgmem = NULL;
+    b->mg_ok = 0;
+}
+
+int pv_set_mg_sweeps(pv_bag *b, int nu)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (nu < 1 || nu > 4) return pv_set_error(PV_EINVAL, "MG sweeps must be 1..4 (got %d)", nu);
+    b->mg.nu = nu;
+    return PV_OK;
+}

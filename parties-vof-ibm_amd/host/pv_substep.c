@@ -497,6 +497,104 @@ static int p_solve_fft_pcg(pv_bag *b)
     return pcg_loop(b, 0, tol, maxit);
 }
 
+/* ---- MG-PCG: the CG above with one multigrid V-cycle as preconditioner ---- */
+static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
+{
+    pv_tbegin(b, PV_T_MG_VCYCLE);
+    pvk_mg_vcycle(&b->mg, b->d_sc, &b->mgw, n, tol, maxit, S(b));
+    pv_tend(b, PV_T_MG_VCYCLE);
+}
+
+static void mg_iteration(pv_bag *b, int n, double tol, int maxit)
+{
+    pv_tbegin(b, PV_T_P_APPLY);
+    pvk_pcg_apply(G(b), b->d_sc, &b->mgw, n, S(b));
+    pv_tend(b, PV_T_P_APPLY);
+    pv_tbegin(b, PV_T_P_UPDATE);
+    pvk_mg_update(G(b), b->d_sc, &b->mgw, n, S(b));
+    pv_tend(b, PV_T_P_UPDATE);
+    mg_vcycle(b, n, tol, maxit);
+}
+
+/* faces, coarse levels, r = rhs - mean, phi = 0, z = V(r), stop test -1 */
+static void mg_cold_start(pv_bag *b, double tol, int maxit)
+{
+    pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
+    pvk_mg_setup(&b->mg, S(b));
+    pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
+    mg_vcycle(b, -1, tol, maxit);
+}
+
+/* iterations n0, n0 + 1, ... until the device stop flag; the first check
+ * comes after the previous solve's iteration count less two */
+static int mg_loop(pv_bag *b, int n0, double tol, int maxit)
+{
+    int chunk = b->mg_chunk, n = n0;
+    for (;;) {
+        int k;
+        for (k = 0; k < chunk && n < maxit; k++, n++) mg_iteration(b, n, tol, maxit);
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (n >= maxit) return pv_set_error(PV_EHIP, "pressure MG-PCG did not stop (device flag)");
+        chunk = 2;
+    }
+    b->mg_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 2;
+    return PV_OK;
+}
+
+static int p_solve_mg(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter;
+    double tol = b->ph.p_rel_tol;
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    mg_cold_start(b, tol, maxit);
+    return mg_loop(b, 0, tol, maxit);
+}
+
+static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit);
+
+/* MG-PCG from the learned warm start phi0 = M_fft(b + s c(b, M_fft b, rho)),
+ * judged by the energy |<r, V r>| (V the V-cycle, close to A^-1) of its
+ * residual against the cold start's: kept if smaller, else the cold start is
+ * restored.  The network features are those it was trained on (M_fft, not
+ * V), so the FFT tables are needed too. */
+static int p_solve_learned_mg(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter;
+    double tol = b->ph.p_rel_tol, gw, gc;
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (power-of-two sizes)");
+    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    mg_cold_start(b, tol, maxit);
+    pvk_pcg_gamma(b->d_sc, &b->mgw, 0, 4, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (b->h_sc->p_done) return PV_OK;
+    fft_solve_field(b, b->pcg.r, tol, maxit);               /* z = M_fft r0: the features' second input */
+    RC(pv_learn_source(b, FD(b, PV_FI_LTMP)));
+    fft_solve_field(b, FD(b, PV_FI_LTMP), tol, maxit);
+    pvk_learn_copy(G(b), b->pcg.z, FD(b, PV_F_PHI), S(b));
+    pvk_pcg_opx(G(b), &b->mgw, FD(b, PV_F_PHI), b->mgw.Ad, S(b));
+    pvk_mg_start(G(b), b->d_sc, &b->mgw, 1, S(b));
+    mg_vcycle(b, -2, tol, maxit);
+    pvk_pcg_gamma(b->d_sc, &b->mgw, 0, 3, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    gw = fabs(b->h_sc->red[3]);
+    gc = fabs(b->h_sc->red[4]);
+    b->learn_start_res = (gc > 0.0) ? gw / gc : 0.0;
+    if (!b->h_sc->p_done && gw >= gc) {
+        b->st.p_fallback[b->stage] += 1;
+        pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
+        mg_vcycle(b, -1, tol, maxit);
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+    }
+    if (b->h_sc->p_done) return PV_OK;
+    return mg_loop(b, 0, tol, maxit);
+}
+
 /* The plain FFT-PCG with training snapshots (phi_k, z_k, network features)
  * at the collector's iterations; crops are harvested after convergence. */
 static int p_solve_collect(pv_bag *b)
@@ -519,32 +617,13 @@ static int p_solve_collect(pv_bag *b)
     return pv_learn_harvest(b, si);
 }
 
-/* Learned solve: the network's warm start phi0 = z + s_z c(r0, z0, rho), kept
- * only if its residual is below ||b|| (else the cold start), then the
- * FFT-PCG iterations. */
-static int p_solve_learned(pv_bag *b)
+/* z = M_fft src (the FFT stages on a given field; no PCG update, no stop test) */
+static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit)
 {
-    int maxit = b->ph.p_max_iter;
-    double tol = b->ph.p_rel_tol;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
-    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
-    pcg_cold_start(b, tol, maxit);
-    LAUNCHCHK();
-    RC(pv_read_scalars(b));
-    if (b->h_sc->p_done) return PV_OK;
-    RC(pv_learn_warm(b, FD(b, PV_F_PHI)));
-    pvk_pcg_opx(G(b), &b->pcg, FD(b, PV_F_PHI), b->pcg.Ad, S(b));
-    pvk_pcg_stage(G(b), &b->fft, b->d_sc, &b->pcg, 1, -2, tol, maxit, S(b));
-    pcg_iteration(b, -2, tol, maxit);
-    LAUNCHCHK();
-    RC(pv_read_scalars(b));
-    b->learn_start_res = b->h_sc->p_res;
-    if (!b->h_sc->p_done && b->h_sc->p_res >= 1.0) {       /* warm start judged worse: cold start */
-        b->st.p_fallback[b->stage] += 1;
-        pcg_cold_start(b, tol, maxit);
-    }
-    if (b->h_sc->p_done) return PV_OK;
-    return pcg_loop(b, 0, tol, maxit);
+    pvk_pcg w = b->pcg;
+    w.r = src;
+    pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 1, -3, tol, maxit, S(b));
+    pcg_iteration_w(b, &w, -3, tol, maxit);
 }
 
 /* warm restart of the FFT-PCG from the current phi: r = rhs - mean - A phi */
@@ -557,10 +636,55 @@ static int pcg_warm_restart(pv_bag *b, double tol, int maxit)
     return pv_read_scalars(b);
 }
 
-/* Learned flexible CG: the network's warm start, then z = M_fft r + s_z c(r,
- * M_fft r, rho) in every iteration with beta = -<z_{n+1}, A d_n> / <d_n, A d_n>.
- * If the residual stalls (less than a factor 2 over 24 iterations) the solve
- * restarts as the plain FFT-PCG from the current iterate. */
+/* The learned warm start phi0 = M_fft(b + s c(b, M_fft b, rho)).  It is
+ * judged by the energy-norm proxy <r, M_fft r> of its residual against the
+ * cold start's (CG converges in the A-norm): kept if smaller, else the cold
+ * start is restored.  Leaves the solve ready for iteration 0. */
+static int learned_warm_start(pv_bag *b, double tol, int maxit)
+{
+    double gw, gc;
+    pcg_cold_start(b, tol, maxit);
+    pvk_pcg_gamma(b->d_sc, &b->pcg, 0, 4, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    if (b->h_sc->p_done) return PV_OK;
+    RC(pv_learn_source(b, FD(b, PV_FI_LTMP)));
+    fft_solve_field(b, FD(b, PV_FI_LTMP), tol, maxit);
+    pvk_learn_copy(G(b), b->pcg.z, FD(b, PV_F_PHI), S(b));
+    RC(pcg_warm_restart(b, tol, maxit));
+    pvk_pcg_gamma(b->d_sc, &b->pcg, 0, 3, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    /* M_fft is negative definite: the energies are -<r, M_fft r> */
+    gw = fabs(b->h_sc->red[3]);
+    gc = fabs(b->h_sc->red[4]);
+    b->learn_start_res = (gc > 0.0) ? gw / gc : 0.0;
+    if (!b->h_sc->p_done && gw >= gc) {
+        b->st.p_fallback[b->stage] += 1;
+        pcg_cold_start(b, tol, maxit);
+        LAUNCHCHK();
+        RC(pv_read_scalars(b));
+    }
+    return PV_OK;
+}
+
+/* Learned solve: the learned warm start, then the FFT-PCG iterations. */
+static int p_solve_learned(pv_bag *b)
+{
+    int maxit = b->ph.p_max_iter;
+    double tol = b->ph.p_rel_tol;
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    RC(learned_warm_start(b, tol, maxit));
+    if (b->h_sc->p_done) return PV_OK;
+    return pcg_loop(b, 0, tol, maxit);
+}
+
+/* Learned flexible CG: the learned warm start, then in every iteration the
+ * learned preconditioner z = M_fft(r + s c(r, M_fft r, rho)) with beta =
+ * -<z_{n+1}, A d_n> / <d_n, A d_n>.  If the residual stalls (less than a
+ * factor 2 over 24 iterations) the solve restarts as the plain FFT-PCG from
+ * the current iterate. */
 static int p_solve_learned_fcg(pv_bag *b)
 {
     int maxit = b->ph.p_max_iter, n = 0, since = 0, done_fcg;
@@ -568,30 +692,24 @@ static int p_solve_learned_fcg(pv_bag *b)
     pvk_pcg w;
     if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
     if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
-    pcg_cold_start(b, tol, maxit);
-    LAUNCHCHK();
-    RC(pv_read_scalars(b));
-    if (b->h_sc->p_done) return PV_OK;
-    RC(pv_learn_warm(b, FD(b, PV_F_PHI)));
-    RC(pcg_warm_restart(b, tol, maxit));
-    b->learn_start_res = b->h_sc->p_res;
-    if (!b->h_sc->p_done && b->h_sc->p_res >= 1.0) {
-        b->st.p_fallback[b->stage] += 1;
-        pcg_cold_start(b, tol, maxit);
-        LAUNCHCHK();
-        RC(pv_read_scalars(b));
-    }
+    RC(learned_warm_start(b, tol, maxit));
     if (b->h_sc->p_done) return PV_OK;
     w = b->pcg;
     w.flex = 1;
     w.nrz = b->NYM;
-    RC(pv_learn_correct(b, w.rz[0]));
+#define LEARNED_Z(slot)                                                                    \
+    do {                                                                                   \
+        RC(pv_learn_source(b, FD(b, PV_FI_LTMP)));                                         \
+        fft_solve_field(b, FD(b, PV_FI_LTMP), tol, maxit);                                 \
+        pvk_learn_dots(G(b), b->pcg.z, b->pcg.r, b->pcg.Ad, w.rz[slot], w.zad, S(b));      \
+    } while (0)
+    LEARNED_Z(0);
     best = b->h_sc->p_res;
     for (;;) {
         int k;
         for (k = 0; k < 8 && n < maxit; k++, n++) {
             pcg_iteration_w(b, &w, n, tol, maxit);
-            RC(pv_learn_correct(b, w.rz[(n + 1) & 1]));
+            LEARNED_Z((n + 1) & 1);
         }
         LAUNCHCHK();
         RC(pv_read_scalars(b));
@@ -601,6 +719,7 @@ static int p_solve_learned_fcg(pv_bag *b)
         if (b->h_sc->p_res < 0.5 * best) { best = b->h_sc->p_res; since = 0; }
         if (since >= 24) break;                                 /* stalled */
     }
+#undef LEARNED_Z
     done_fcg = b->h_sc->p_iter;
     b->st.p_fallback[b->stage] += 1;
     RC(pcg_warm_restart(b, tol, maxit));
@@ -616,6 +735,8 @@ static int p_solve(pv_bag *b)
         RC((b->learn && b->learn->collecting) ? p_solve_collect(b) : p_solve_fft_pcg(b));
     else if (b->prm.pressure_solver == PV_PSOLVE_LEARNED) RC(p_solve_learned(b));
     else if (b->prm.pressure_solver == PV_PSOLVE_LEARNED_FCG) RC(p_solve_learned_fcg(b));
+    else if (b->prm.pressure_solver == PV_PSOLVE_MG_PCG) RC(p_solve_mg(b));
+    else if (b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG) RC(p_solve_learned_mg(b));
     else RC(p_solve_jacobi(b));
     if (b->h_sc->p_done != 4) {
         pvk_p_mean(G(b), FD(b, PV_F_PHI), &b->rd, S(b));

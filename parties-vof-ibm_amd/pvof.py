@@ -18,6 +18,8 @@ PV_PSOLVE_JACOBI_PCG = 0
 PV_PSOLVE_FFT_PCG = 1
 PV_PSOLVE_LEARNED = 2
 PV_PSOLVE_LEARNED_FCG = 3
+PV_PSOLVE_MG_PCG = 4
+PV_PSOLVE_LEARNED_MG = 5
 PV_PREC_FP64 = 0
 PV_PREC_MIXED = 1
 PV_VEL_TYPE_NORMAL = 0
@@ -156,6 +158,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_cnn_adam_step.argtypes = [vp, C.c_float, C.c_int]
     ip = C.POINTER(C.c_int)
     lib.pv_set_pressure_solver.argtypes = [vp, C.c_int]
+    lib.pv_set_mg_sweeps.argtypes = [vp, C.c_int]
     lib.pv_learn_collect_begin.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
     lib.pv_learn_collect_end.argtypes = [vp, ip]
     lib.pv_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, dp, dp]
