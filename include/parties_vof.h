@@ -200,6 +200,10 @@ int pv_Pressure_operator_variableCoeff(pv_bag *bag, int out, int in);        /* 
 int pv_Pressure_apply_BCs(pv_bag *bag, int field);                           /* lsolver/psolve_cg.c:358-529 + cell halo */
 int pv_Pressure_project_velocity(pv_bag *bag);                               /* Pressure.c:218-464 */
 int pv_Pressure_compute_velocity_divergence(pv_bag *bag, double *div_max);   /* Pressure.c:475-597 */
+/* No counterpart in the reference: SCRATCH0 = V(P_RES), one V-cycle of the
+ * MG-PCG preconditioner on the face coefficients of the current RHO field
+ * (for tests of the preconditioner as a linear operator). */
+int pv_mg_vcycle_apply(pv_bag *bag);
 /* Constant-coefficient Poisson solve, FFT in x (R2HC, 1/NXM scale, modified
  * wavenumbers) and Thomas in y with the (0,0) mode pinned at j = 0:
  * lsolver/psolve_fft.c:17-207, :467-591, :1536-1802 (z of length 1). */

@@ -81,7 +81,7 @@ __global__ void __launch_bounds__(NTH) k_mg_smooth(pvk_mglev L, pvk_mglev C, int
 {
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    if (check && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
+    if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
     const int ip = blockIdx.x * TXP + (threadIdx.x & (TXP - 1));
     const int j = blockIdx.y * TYR + (threadIdx.x >> 6);
     const int p = L.pitch;
@@ -339,7 +339,7 @@ __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 __global__ void __launch_bounds__(NTH) k_mg_check(pvk_scal *sc, pvk_pcg w, int n, double tol, int maxit)
 {
     __shared__ double sh[NTH];
-    if (sc->p_done) return;
+    if (sc->p_done || n == -3) return;
     mg_stop_test(sc, w, n, tol, maxit, sh);
 }
 __global__ void __launch_bounds__(NTH) k_mg_dot(pvk_mglev L, const pvk_scal *sc, pvk_pcg w, int slot)
@@ -416,8 +416,8 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, hipStream_t s)
     hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(NTH), 0, s, m->L[m->nlev - 1], m->G);
 }
 
-// z = V(r) for CG iteration n (n < 0: the start), with the stop test of n
-// first and <r, z> partials into w->rz[(n < 0) ? 0 : (n + 1) & 1]
+// z = V(r) for CG iteration n (n < 0: the start; n = -3: no stop test),
+// with the stop test of n first and <r, z> partials into w->rz[(n < 0) ? 0 : (n + 1) & 1]
 extern "C" void pvk_mg_vcycle(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                               hipStream_t s)
 {

@@ -768,6 +768,18 @@ int pv_Pressure_project_velocity(pv_bag *b)
     return PV_OK;
 }
 
+int pv_mg_vcycle_apply(pv_bag *b)
+{
+    RC(begin(b));
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
+    pvk_mg_setup(&b->mg, S(b));
+    pvk_mg_vcycle(&b->mg, b->d_sc, &b->mgw, -3, 0.0, 1, S(b));
+    pvk_learn_copy(G(b), b->mgw.z, FD(b, PV_F_SCRATCH0), S(b));
+    LAUNCHCHK();
+    return pv_read_scalars(b);
+}
+
 int pv_Pressure_compute_velocity_divergence(pv_bag *b, double *div_max)
 {
     RC(begin(b));

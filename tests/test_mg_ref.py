@@ -1,0 +1,80 @@
+"""The NumPy restatement of the multigrid preconditioner (tests/mg_ref.py),
+pinned by what the mathematics fixes rather than by retyping it:
+
+* one V-cycle is a symmetric operator (pre- and post-smoothing are adjoint,
+  restriction is the transpose of prolongation, the last level is exact), so
+  CG preconditioned by it is plain CG;
+* on a grid that is its own last level the V-cycle is the exact inverse of A
+  on mean-free right-hand sides;
+* MG-PCG solves the bubbly variable-density problem in a handful of
+  iterations, far fewer than Jacobi-PCG, and to the right answer.
+Each check fails on one of the planted mistakes of mg_ref (mutant=...).
+"""
+import numpy as np
+import pytest
+
+import mg_ref as M
+
+
+def problem(n, seed=3):
+    rho = M.bubbles(n, n, seed)
+    bx, by = M.faces(rho, float(n * n), float(n * n))
+    return rho, bx, by
+
+
+def asymmetry(bx, by, mutant=None, seed=1):
+    n = bx.shape[0]
+    rng = np.random.default_rng(seed)
+    a = rng.standard_normal((n, n))
+    b = rng.standard_normal((n, n))
+    a -= a.mean()
+    b -= b.mean()
+    lv, G = M.build(bx, by, mutant)
+    va, vb = M.vcycle(lv, G, a, mutant=mutant), M.vcycle(lv, G, b, mutant=mutant)
+    return abs((va * b).sum() - (a * vb).sum()) / abs((va * b).sum())
+
+
+def test_vcycle_is_symmetric():
+    _, bx, by = problem(64)
+    assert asymmetry(bx, by) < 1e-12
+    assert asymmetry(bx, by, "post_red_first") > 1e-6
+
+
+def test_last_level_alone_is_the_exact_inverse():
+    _, bx, by = problem(8)
+    lv, G = M.build(bx, by)
+    assert len(lv) == 1
+    f = np.random.default_rng(2).standard_normal((8, 8))
+    x = M.vcycle(lv, G, f)
+    assert abs(x.mean()) < 1e-14 * np.abs(x).max()
+    r = M.apply_a(bx, by, x) - (f - f.mean())
+    assert np.abs(r).max() < 1e-12 * np.abs(f).max()
+
+
+def test_level_sizes():
+    assert M.level_sizes(1024, 1024)[-1] == (8, 8)
+    assert M.level_sizes(96, 96)[-1] == (6, 6)
+    assert M.level_sizes(384, 768)[-1] == (3, 6)
+    with pytest.raises(ValueError):
+        M.level_sizes(100, 100)
+
+
+@pytest.mark.parametrize("mutant", [None, "restrict_mean", "coarse_sum"])
+def test_mg_pcg_iterations_and_answer(mutant):
+    n = 64
+    _, bx, by = problem(n)
+    rng = np.random.default_rng(4)
+    xs = rng.standard_normal((n, n))
+    xs -= xs.mean()
+    b = M.apply_a(bx, by, xs)
+    lv, G = M.build(bx, by, mutant)
+    x, it = M.pcg(bx, by, b, lambda r: M.vcycle(lv, G, r, mutant=mutant), tol=1e-12)
+    diag = bx + np.roll(bx, 1, 1)
+    diag[:-1] += by[:-1]
+    diag[1:] += by[:-1]
+    _, itj = M.pcg(bx, by, b, lambda r: -r / diag, tol=1e-12)
+    if mutant is None:
+        assert it <= 12 and itj > 10 * it
+        assert np.abs(x - x.mean() - xs).max() < 1e-8 * np.abs(xs).max()
+    else:
+        assert it > 2 * 12
