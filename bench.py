@@ -5,12 +5,19 @@ Workload (one GPU): 1024 x 1024 planar cells on [0,1]^2, x periodic, no-slip
 walls in y, 16-64 seeded gas bubbles of radius 0.01-0.08 (seed 20261202,
 splitmix64), density ratio 1e-3, viscosity ratio 1e-2, Re = 100, We = 200,
 Ri = 1, gravity -y, Cn = 0.004, Pe_CH = 1/Cn, CFL 0.2, default_dt 1e-5.
-Pressure: FFT-x/DCT-y preconditioned PCG to ||r||/||b|| <= 1e-10.  The data are
-synthetic (generated in-library); all fields are resident in HBM before the
-timed region starts.
+Every pressure solve stops at ||r|| / ||b|| <= 1e-10.  The data are synthetic
+(generated in-library); all fields are resident in HBM before the timed
+region starts.
 
 A "step" is one full RK3 step (three substeps of Temporal_int_all_the_equations
 including CFL).  value = NXM * NYM * steps / seconds / 1e6 (Mcell-steps/s).
+
+Default run: the learned, mixed-precision form (network warm start +
+multigrid-preconditioned CG pressure, FP32 stencils with FP64 fluxes, solves
+and dots; the network is trained first, untimed, on a held-out seed), then on
+the same GPU the plain FP64 path (FFT-preconditioned CG, the reference
+algorithm) and the same form without the network (ablation), each reported
+with its own Mcell-steps/s.  --solver / --precision time one configuration.
 
 The roofline block reports the kernel with the largest share of device time,
 timed live with HIP events on the library's stream over the timed region.
@@ -33,7 +40,8 @@ import pvof  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDATE", "P_DUPDATE",
-          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE", "MG_VCYCLE"]
+          "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE",
+          "MG_PRE0", "MG_COARSE", "MG_POST0"]
 # algorithmic HBM bytes per cell and launch (each distinct array read or
 # written once; stencil neighbours come from cache)
 BYTES_PER_CELL = {
@@ -43,8 +51,15 @@ BYTES_PER_CELL = {
     "CH_RHS": 56, "MOM_TERMS": 88, "P_TRANSPOSE": 16,
 }
 # the fused FFT-PCG kernels behind the P_APPLY / P_UPDATE timers also form the
-# new direction (z, d_old, rho -> d, Ad) and the forward x-FFT (-> spectrum)
-BYTES_PER_CELL_FFT = dict(BYTES_PER_CELL, P_APPLY=40, P_UPDATE=56)
+# new direction (z, d_old, faces -> d, Ad) and the forward x-FFT (-> spectrum)
+BYTES_PER_CELL_FFT = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=56)
+# MG-PCG: the same direction kernel, phi/r update (phi, d, r, Ad -> phi, r),
+# level-0 pre-smoothing + restriction (f, faces -> x, coarse f) and
+# correction + post-smoothing (x, coarse x, f, faces -> x)
+BYTES_PER_CELL_MG = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=48, MG_PRE0=34, MG_POST0=42)
+# per pressure iteration, all of its kernels (MG: level 0 + 1/3 of the level-0
+# smoothing bytes for the coarser levels)
+BYTES_PER_ITER = {"fft": 176, "mg": 197, "jacobi": 112}
 SEED = 20261202
 
 
@@ -57,14 +72,26 @@ def workload_params(n=1024, solver=pvof.PV_PSOLVE_FFT_PCG, precision=pvof.PV_PRE
         precision=precision)
 
 
-def step_bytes_per_cell(st):
+def step_bytes_per_cell(st, family):
     """Algorithmic bytes per cell for one RK3 step from the solver counts
-    (fixed 563 B per substep + 88 per biharmonic, 136 per momentum and 152 per
-    FFT-preconditioned pressure iteration)."""
+    (fixed 563 B per substep + 88 per biharmonic, 136 per momentum and
+    BYTES_PER_ITER per pressure iteration)."""
     total = 0
     for k in range(3):
-        total += 563 + 88 * st["n_ch"][k] + 136 * (st["n_u"][k] + st["n_v"][k]) + 152 * st["n_p"][k]
+        total += 563 + 88 * st["n_ch"][k] + 136 * (st["n_u"][k] + st["n_v"][k]) + BYTES_PER_ITER[family] * st["n_p"][k]
     return total
+
+
+def family_of(solver):
+    return "jacobi" if solver == "jacobi" else ("mg" if solver in ("mg", "learned_mg") else "fft")
+
+
+SOLVERS = {"fft": pvof.PV_PSOLVE_FFT_PCG, "jacobi": pvof.PV_PSOLVE_JACOBI_PCG, "mg": pvof.PV_PSOLVE_MG_PCG,
+           "learned": pvof.PV_PSOLVE_LEARNED, "learned_fcg": pvof.PV_PSOLVE_LEARNED_FCG,
+           "learned_mg": pvof.PV_PSOLVE_LEARNED_MG}
+LABEL = {"fft": "FFT-PCG", "jacobi": "Jacobi-PCG", "mg": "MG-PCG", "learned": "learned warm start + FFT-PCG",
+         "learned_fcg": "learned warm start + learned-preconditioned FCG",
+         "learned_mg": "learned warm start + MG-PCG"}
 
 
 def cpu_baseline(params, F):
@@ -109,21 +136,84 @@ def train_network(lib, args, prec):
             "loss_first": l0, "loss_last": l2, "held_out_seed": SEED + 1}
 
 
+def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=True):
+    """Time `steps` RK3 steps of one configuration from the seeded field F
+    (after `warmup` untimed steps); returns the measurements."""
+    prec = pvof.PV_PREC_MIXED if precision == "mixed" else pvof.PV_PREC_FP64
+    p = workload_params(args.n, SOLVERS[solver], prec)
+    s = pvof.Solver(p, lib, device=0)
+    s.init_from_F(F)
+    if net_params is not None:
+        pvof.Network(s).set_params(net_params)
+    if warmup:
+        s.call("Temporal_int_rk3", warmup)
+    s.call("synchronize")
+    if timers:
+        s.call("timers_enable", max(1, args.timer_period))
+        s.call("timers_reset")
+    stats, starts = [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s.call("Temporal_int_rk3", 1)
+        stats.append(s.stats())
+        if net_params is not None:
+            r = C.c_double(0.0)
+            lib.pv_learn_start_residual(s.h, C.byref(r))
+            starts.append(r.value)
+    s.call("synchronize")
+    elapsed = time.perf_counter() - t0
+    ms = (C.c_double * len(TIMERS))()
+    cnt = (C.c_longlong * len(TIMERS))()
+    if timers:
+        s.check(lib.pv_timers_read(s.h, ms, cnt))
+    fam = family_of(solver)
+    bpc = {"jacobi": BYTES_PER_CELL, "fft": BYTES_PER_CELL_FFT, "mg": BYTES_PER_CELL_MG}[fam]
+    cells = p.NXM * p.NYM
+    kernels = {}
+    for k, name in enumerate(TIMERS):
+        if cnt[k]:
+            avg_ms = ms[k] / cnt[k]
+            kernels[name] = {"timed_launches": int(cnt[k]), "avg_us": round(1e3 * avg_ms, 3),
+                             "est_total_ms": round(ms[k] * max(1, args.timer_period), 3),
+                             "GBs": (round(bpc[name] * cells / (avg_ms * 1e-3) / 1e9, 1) if name in bpc else None)}
+    bytes_step = sum(step_bytes_per_cell(st, fam) for st in stats) / len(stats)
+    out = {"solver": solver, "precision": precision, "steps": steps, "warmup": warmup,
+           "value": round(cells * steps / elapsed / 1e6, 3), "ms_per_step": round(1e3 * elapsed / steps, 3),
+           "iterations_per_step": {key: float(np.mean([sum(st[key]) for st in stats]))
+                                   for key in ("n_ch", "n_u", "n_v", "n_p")},
+           "max_p_res": float(max(max(st["p_res"]) for st in stats)),
+           "path_hbm_fraction": round(bytes_step * cells * steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
+           "path_bytes_per_cell_step": round(bytes_step, 1), "kernels": kernels, "_bpc": bpc, "_cells": cells,
+           "_params": p}
+    if net_params is not None:
+        out["start_energy_ratio_last_step"] = starts[-1] if starts else None
+        out["fallbacks"] = int(sum(sum(st["p_fallback"]) for st in stats))
+    s.close()
+    return out
+
+
+def public(r):
+    return {k: v for k, v in r.items() if not k.startswith("_") and k != "kernels"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--n", type=int, default=1024)
-    ap.add_argument("--solver", choices=["fft", "jacobi", "learned", "learned_fcg"], default="fft",
-                    help="learned: network warm start + FFT-PCG; learned_fcg: + network-corrected "
-                         "preconditioner in flexible CG (network trained first, untimed, on a held-out seed)")
+    ap.add_argument("--solver", choices=sorted(SOLVERS), default="learned_mg",
+                    help="pressure solver of the timed configuration (learned*: network trained first, "
+                         "untimed, on a held-out seed)")
+    ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
+                    help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
+    ap.add_argument("--no-compare", action="store_true",
+                    help="skip the plain FP64 FFT-PCG run and the no-network ablation")
+    ap.add_argument("--plain-steps", type=int, default=10)
     ap.add_argument("--train-collect-steps", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=3000)
-    ap.add_argument("--precision", choices=["fp64", "mixed"], default="fp64",
-                    help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--cpu-n", type=int, default=1024)
+    ap.add_argument("--cpu-n", type=int, default=512)
     ap.add_argument("--timer-period", type=int, default=16,
                     help="time every k-th launch of each kernel class with HIP events")
     ap.add_argument("--no-timers", action="store_true")
@@ -134,97 +224,69 @@ def main():
         sys.exit(2)
 
     lib = pvof.load_lib()
-    solver = {"fft": pvof.PV_PSOLVE_FFT_PCG, "jacobi": pvof.PV_PSOLVE_JACOBI_PCG,
-              "learned": pvof.PV_PSOLVE_LEARNED, "learned_fcg": pvof.PV_PSOLVE_LEARNED_FCG}[args.solver]
     prec = pvof.PV_PREC_MIXED if args.precision == "mixed" else pvof.PV_PREC_FP64
-    p = workload_params(args.n, solver, prec)
     learned = None
     if args.solver.startswith("learned"):
         learned = train_network(lib, args, prec)
-    F = pvof.ic(lib, "bubbles", p, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
-    s = pvof.Solver(p, lib, device=0)
-    s.init_from_F(F)
-    if learned is not None:
-        pvof.Network(s).set_params(learned.pop("params"))
-    if args.warmup:
-        s.call("Temporal_int_rk3", args.warmup)
-    s.call("synchronize")
-    if not args.no_timers:
-        s.call("timers_enable", max(1, args.timer_period))
-        s.call("timers_reset")
-    stats, starts = [], []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        s.call("Temporal_int_rk3", 1)
-        stats.append(s.stats())
-        if learned is not None:
-            r = C.c_double(0.0)
-            lib.pv_learn_start_residual(s.h, C.byref(r))
-            starts.append(r.value)
-    s.call("synchronize")
-    t1 = time.perf_counter()
-    elapsed = t1 - t0
-    ms = (C.c_double * len(TIMERS))()
-    cnt = (C.c_longlong * len(TIMERS))()
-    if not args.no_timers:
-        s.check(lib.pv_timers_read(s.h, ms, cnt))
-    bpc = BYTES_PER_CELL if args.solver == "jacobi" else BYTES_PER_CELL_FFT
+    p0 = workload_params(args.n, SOLVERS[args.solver], prec)
+    F = pvof.ic(lib, "bubbles", p0, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
+    main_run = run(lib, args, args.solver, args.precision, args.steps, args.warmup, F,
+                   learned.pop("params") if learned is not None else None, timers=not args.no_timers)
+    p = main_run["_params"]
+    compare = {}
+    if not args.no_compare:
+        plain = run(lib, args, "fft", "fp64", args.plain_steps, 1, F, timers=False)
+        compare["plain_fp64"] = public(plain)
+        compare["speedup_vs_plain_fp64"] = round(main_run["value"] / plain["value"], 3)
+        if args.solver.startswith("learned"):
+            base = {"learned_mg": "mg", "learned": "fft", "learned_fcg": "fft"}[args.solver]
+            abl = run(lib, args, base, args.precision, args.steps, args.warmup, F, timers=False)
+            compare["ablation_no_network"] = public(abl)
+            compare["network_gain"] = round(main_run["value"] / abl["value"], 3)
 
-    cells = p.NXM * p.NYM
-    value = cells * args.steps / elapsed / 1e6
-    bytes_step = sum(step_bytes_per_cell(st) for st in stats) / len(stats)
-    path_gbs = bytes_step * cells * args.steps / elapsed / 1e9
-    kernels = {}
-    for k, name in enumerate(TIMERS):
-        if cnt[k]:
-            avg_ms = ms[k] / cnt[k]
-            kernels[name] = {"timed_launches": int(cnt[k]), "avg_us": round(1e3 * avg_ms, 3),
-                             "est_total_ms": round(ms[k] * max(1, args.timer_period), 3),
-                             "GBs": round(bpc[name] * cells / (avg_ms * 1e-3) / 1e9, 1)}
-    iters = {key: float(np.mean([sum(st[key]) for st in stats])) for key in ("n_ch", "n_u", "n_v", "n_p")}
-
+    form = LABEL[args.solver] + (", mixed FP32/FP64" if args.precision == "mixed" else ", FP64")
     out = {
-        "metric": "Mcell-steps/s at 1024^2, " + ("plain FP64" if prec == pvof.PV_PREC_FP64 else "mixed FP32/FP64")
-                  + {"fft": " (FFT-PCG pressure)", "jacobi": " (Jacobi-PCG pressure)",
-                     "learned": " (learned warm start + FFT-PCG pressure)",
-                     "learned_fcg": " (learned warm start + learned-preconditioned FCG pressure)"}[args.solver],
-        "value": round(value, 3),
+        "metric": f"Mcell-steps/s at {p.NXM}^2: {form} (plain FP64 FFT-PCG path in compare.plain_fp64)",
+        "value": main_run["value"],
         "unit": "Mcell-steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(1e3 * elapsed / args.steps, 3),
+        "ms_per_step": main_run["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if prec == pvof.PV_PREC_FP64 else "f32+f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic (seeded bubble field generated in-library)",
         "config": {"workload": f"planar {p.NXM}x{p.NYM} bubbly two-phase RK3 step, rho2/rho1=1e-3, "
-                               f"{'FFT-PCG' if solver else 'Jacobi-PCG'} pressure",
+                               f"{LABEL[args.solver]} pressure to 1e-10 relative residual",
                    "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
-        "path_hbm_fraction": round(path_gbs / HBM_PEAK_GBS, 4),
-        "path_bytes_per_cell_step": round(bytes_step, 1),
-        "iterations_per_step": iters,
+        "path_hbm_fraction": main_run["path_hbm_fraction"],
+        "path_bytes_per_cell_step": main_run["path_bytes_per_cell_step"],
+        "iterations_per_step": main_run["iterations_per_step"],
+        "max_p_res": main_run["max_p_res"],
         "roofline": None,
-        "kernels": kernels,
+        "kernels": main_run["kernels"],
+        "compare": compare,
         "cpu_baseline": None,
     }
     if learned is not None:
-        learned["start_residual_last_step"] = starts[-1] if starts else None
-        learned["fallbacks"] = int(sum(sum(st["p_fallback"]) for st in stats))
+        learned["start_energy_ratio_last_step"] = main_run.get("start_energy_ratio_last_step")
+        learned["fallbacks"] = main_run.get("fallbacks")
         out["learned"] = learned
-    if kernels:
-        dom = max(kernels, key=lambda n: kernels[n]["est_total_ms"])
-        achieved = kernels[dom]["GBs"]
+    kernels, bpc, cells = main_run["kernels"], main_run["_bpc"], main_run["_cells"]
+    timed = {k: v for k, v in kernels.items() if v["GBs"] is not None}
+    if timed:
+        dom = max(timed, key=lambda n: timed[n]["est_total_ms"])
+        achieved = timed[dom]["GBs"]
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
                            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                            "bytes_per_cell": bpc[dom], "cells_per_launch": cells,
-                           "avg_us": kernels[dom]["avg_us"], "timer_period": args.timer_period}
+                           "avg_us": timed[dom]["avg_us"], "timer_period": args.timer_period}
     if not args.no_cpu_baseline:
-        pc = workload_params(args.cpu_n, solver)
-        Fc = F if args.cpu_n == args.n else pvof.ic(lib, "bubbles", pc, seed=SEED, nmin=16, nmax=64,
-                                                   rmin=0.01, rmax=0.08, gas_inside=1)
+        pc = workload_params(args.cpu_n, pvof.PV_PSOLVE_FFT_PCG)
+        Fc = pvof.ic(lib, "bubbles", pc, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
         out["cpu_baseline"] = cpu_baseline(pc, Fc)
     print(json.dumps(out))
 

@@ -55,8 +55,9 @@ extern "C" {
 #define PV_PSOLVE_MG_PCG     4   /* the same CG, start, gauge and stop test, preconditioned
                                     by one geometric multigrid V-cycle on the variable face
                                     coefficients (no counterpart in the reference): NXM and
-                                    NYM must halve together down to <= 1024 cells and on to
-                                    a last level of <= 64 cells */
+                                    NYM must halve together down to <= 1024 cells; halving
+                                    goes on while both are even, to a last level of at most
+                                    64 cells */
 #define PV_PSOLVE_LEARNED_MG 5   /* MG-PCG from the learned network's warm start, kept
                                     only when its energy is below the cold start's */
 
@@ -228,7 +229,9 @@ enum pv_timer {
     PV_T_CH_RHS,           /* fused explicit CH right-hand side  */
     PV_T_MOM_TERMS,        /* fused momentum right-hand side     */
     PV_T_P_TRANSPOSE,      /* spectrum transposes of the FFT-PCG */
-    PV_T_MG_VCYCLE,        /* multigrid V-cycle, all levels      */
+    PV_T_MG_PRE0,          /* multigrid pre-smoothing + restriction, level 0 */
+    PV_T_MG_COARSE,        /* multigrid levels 1 .. (both ways)  */
+    PV_T_MG_POST0,         /* correction + post-smoothing + <r,z>, level 0 */
     PV_NTIMERS
 };
 /* on = 0 disables; on = k >= 1 times every k-th launch of each timed kernel
@@ -284,7 +287,12 @@ int pv_batch_autotune(pv_batch *bt, int *chunks);
  * crop_size^2 crops (crops_per_snap per snapshot, at most max_samples). */
 int pv_set_pressure_solver(pv_bag *bag, int solver);
 /* Gauss-Seidel sweeps before and after the coarse correction of the
- * multigrid V-cycle (1..4, default 2). */
+ * multigrid V-cycle (1 or 2, default 2). */
+/* Start of the MG-PCG solves (PV_PSOLVE_MG_PCG, PV_PSOLVE_LEARNED_MG):
+ * mode 0 = from phi = 0 (as the reference); mode 1 = also try the pressure
+ * correction of the same RK3 stage of the previous step, kept only when the
+ * energy |<r, V r>| of its residual is below the cold start's. */
+int pv_set_pressure_warm_start(pv_bag *bag, int mode);
 int pv_set_mg_sweeps(pv_bag *bag, int nu);
 int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
                            int max_samples, unsigned long long seed);

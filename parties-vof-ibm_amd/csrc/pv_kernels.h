@@ -228,11 +228,13 @@ void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hip
 /* ---- multigrid-preconditioned CG (csrc/pv_mg.hip) ----
  * Level 0 is the pressure grid (x = z, f = r, faces = the PCG faces, pitch
  * P); levels 1.. are compact (pitch nx).  Levels c0.. (<= PVK_MG_LDS_CELLS
- * cells) run inside one workgroup; the last level (<= PVK_MG_BOTTOM cells) is
- * solved with the grounded inverse G. */
+ * cells) run inside one workgroup; the last level (halving stops at
+ * <= PVK_MG_BOTTOM_MIN cells or odd sizes; <= PVK_MG_BOTTOM cells) is solved
+ * with the grounded inverse G. */
 #define PVK_MG_MAXLEV 16
 #define PVK_MG_LDS_CELLS 1024
 #define PVK_MG_BOTTOM 64
+#define PVK_MG_BOTTOM_MIN 64
 typedef struct pvk_mglev {
     int nx, ny, pitch;
     double *x, *f, *bx, *by;
@@ -244,7 +246,10 @@ typedef struct pvk_mg {
     int ldsx;                         /* LDS offsets: doubles per in-LDS array set */
 } pvk_mg;
 void pvk_mg_setup(const pvk_mg *m, hipStream_t s);
-void pvk_mg_vcycle(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s);
+void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
+                hipStream_t s);
+void pvk_mg_coarse(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s);
+void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
 int  pvk_mg_nrz(const pvk_mg *m);
 void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s);
 void pvk_mg_update(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
@@ -265,6 +270,7 @@ void pvk_learn_addr(const pvk_grid *g, const pvk_scal *sc, const double *r, cons
 void pvk_learn_dots(const pvk_grid *g, const double *z, const double *r, const double *Ad, double *rzp, double *zadp,
                     hipStream_t s);
 void pvk_learn_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s);
+void pvk_learn_axpy(const pvk_grid *g, const double *src, double *dst, hipStream_t s);   /* dst += src */
 void pvk_learn_crop(const float *in3, const float *tgt, int NXM, int cs, const int *org, int n, int base,
                     float *ds_in, float *ds_tgt, hipStream_t s);
 void pvk_learn_gather(const float *ds_in, const float *ds_tgt, int cs, const int *idx, int B, float *bin,

@@ -97,6 +97,12 @@ __global__ void __launch_bounds__(NTH) k_learn_copy(pvk_grid g, const double *sr
     for (int i = threadIdx.x; i < g.NXM; i += NTH) AT(dst, i, j) = AT(src, i, j);
 }
 
+__global__ void __launch_bounds__(NTH) k_learn_axpy(pvk_grid g, const double *src, double *dst)
+{
+    const int P = g.P, j = blockIdx.x;
+    for (int i = threadIdx.x; i < g.NXM; i += NTH) AT(dst, i, j) += AT(src, i, j);
+}
+
 // target t = (L e - r) / s_r, e = phistar - phik, L the constant-coefficient
 // 5-point Laplacian that M_fft inverts (x periodic, zero-gradient y walls):
 // the source correction that makes M_fft(r + s_r t) = e.
@@ -168,6 +174,10 @@ extern "C" void pvk_learn_dots(const pvk_grid *g, const double *z, const double 
 extern "C" void pvk_learn_copy(const pvk_grid *g, const double *src, double *dst, hipStream_t s)
 {
     hipLaunchKernelGGL(k_learn_copy, dim3(g->NYM), dim3(NTH), 0, s, *g, src, dst);
+}
+extern "C" void pvk_learn_axpy(const pvk_grid *g, const double *src, double *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_learn_axpy, dim3(g->NYM), dim3(NTH), 0, s, *g, src, dst);
 }
 extern "C" void pvk_learn_target(const pvk_grid *g, const pvk_scal *sc, const double *phistar, const double *phik,
                                  const double *r, double sr, float *tgt, hipStream_t s)

@@ -9,17 +9,17 @@
 // Operator, as k_pcg_apply: (A x)_ij = bx_i (x_E - x) - bx_{i-1} (x - x_W)
 //   + by_j (x_N - x) - by_{j-1} (x - x_S), periodic in x, zero-gradient walls.
 // V-cycle (cell-centred, 2 x 2 aggregation; symmetric, so CG stays CG):
-//   smoother      red-black Gauss-Seidel, nu sweeps: (red, black) before the
-//                 coarse correction, (black, red) after it;
+//   smoother      red-black Gauss-Seidel, nu = 1 or 2 sweeps: (red, black)
+//                 before the coarse correction, (black, red) after it;
 //   restriction   sum of the four fine residuals;
 //   prolongation  piecewise constant;
 //   coarse faces  half the sum of the two fine faces across the coarse face;
 //   levels of at most PVK_MG_LDS_CELLS cells run in one workgroup from LDS;
 //   the last level (<= PVK_MG_BOTTOM cells) is solved with the inverse of its
 //   operator grounded at cell 0, both sides projected to zero mean.
-// A Gauss-Seidel value depends only on the neighbours, so the first red sweep
-// starts from x = 0 without clearing x, and the coarse correction is added on
-// the fly where the first black post-sweep reads the red neighbours.
+// Each level above PVK_MG_LDS_CELLS cells is two launches: pre-smoothing +
+// restriction, and correction + post-smoothing, each on LDS tiles with a
+// halo (so f, the faces and x cross HBM once per launch).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "pv_kernels.h"
@@ -42,17 +42,6 @@ __device__ __forceinline__ double gs_value(const pvk_mglev &L, int i, int j, dou
     return (ce * xe + cw * xw + cn * xn + cs * xs - f) / (ce + cw + cn + cs);
 }
 
-// (A x)(i, j) on a level
-__device__ __forceinline__ double a_x(const pvk_mglev &L, int i, int j)
-{
-    const int p = L.pitch, ie = (i + 1 == L.nx) ? 0 : i + 1, iw = (i == 0) ? L.nx - 1 : i - 1;
-    const double x0 = ld(L.x, p, i, j);
-    double v = ld(L.bx, p, i, j) * (ld(L.x, p, ie, j) - x0) - ld(L.bx, p, iw, j) * (x0 - ld(L.x, p, iw, j));
-    if (j + 1 < L.ny) v += ld(L.by, p, i, j) * (ld(L.x, p, i, j + 1) - x0);
-    if (j > 0) v -= ld(L.by, p, i, j - 1) * (x0 - ld(L.x, p, i, j - 1));
-    return v;
-}
-
 // stop test of CG iteration n on the <r, r> partials (as k_transpose):
 // n = -1 sets the reference norm, n = -2 keeps it (warm restart)
 __device__ void mg_stop_test(pvk_scal *sc, const pvk_pcg &w, int n, double tol, int maxit, double *sh)
@@ -69,64 +58,163 @@ __device__ void mg_stop_test(pvk_scal *sc, const pvk_pcg &w, int n, double tol, 
     }
 }
 
-// One half-sweep on the cells of colour col ((i + j) & 1 == col), one thread
-// per horizontal cell pair.  MODE 0: first red sweep from x = 0; MODE 1:
-// Gauss-Seidel; MODE 2: first black sweep after the coarse correction (red
-// neighbours read as x + xc(parent)).  check: block (0, 0) runs the stop test
-// of iteration n first (the other blocks go on; later kernels see the flag).
-// dot: partial <f, x> of the level after the sweep into w.rz[slot].
-template <int MODE>
-__global__ void __launch_bounds__(NTH) k_mg_smooth(pvk_mglev L, pvk_mglev C, int col, pvk_scal *sc, pvk_pcg w,
-                                                  int n, double tol, int maxit, int check, int dot, int slot)
+// ---------------------------------------------------------------------------
+// Fused smoothing of one level (levels with more than PVK_MG_LDS_CELLS cells).
+// A block owns a core tile of TX x MG_TY cells (TX = 64, or 32 on levels at
+// most 256 wide, for more blocks) and works on it extended by
+// a halo of 2 nu cells (periodic in x, cut at the walls in y): half-sweep k
+// updates the cells of its colour at distance >= k from the extended edge,
+// so after 2 nu half-sweeps the core (and the ring the residual needs) holds
+// exactly what sweeping the whole level would give.  The face coefficients
+// and f stay in registers, x in LDS.
+// ---------------------------------------------------------------------------
+#define MG_TY 16
+
+template <int NU, int TX> struct MgTile {
+    static constexpr int H = 2 * NU;                  // halo
+    static constexpr int W = TX + 2 * H;              // extended width
+    static constexpr int HT = MG_TY + 2 * H;          // extended height
+    static constexpr int NC = W * HT;
+    static constexpr int K = (NC + NTH - 1) / NTH;    // cells per thread
+};
+
+// f, the four face coefficients and 1 / (sum of them) of the extended-tile
+// cells of this thread
+template <int NU, int TX>
+__device__ __forceinline__ void tile_coeffs(const pvk_mglev &L, int x0, int y0, double *f, double *ce, double *cw,
+                                            double *cn, double *cs, double *id)
 {
+    using T = MgTile<NU, TX>;
+    const int p = L.pitch, nx = L.nx, ny = L.ny;
+#pragma unroll
+    for (int u = 0; u < T::K; u++) {
+        const int c = threadIdx.x + u * NTH, ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
+        f[u] = ce[u] = cw[u] = cn[u] = cs[u] = id[u] = 0.0;
+        if (c < T::NC && gj >= 0 && gj < ny) {
+            int gi = (x0 + lx) % nx;
+            if (gi < 0) gi += nx;
+            const int giw = (gi == 0) ? nx - 1 : gi - 1;
+            f[u] = ld(L.f, p, gi, gj);
+            ce[u] = ld(L.bx, p, gi, gj);
+            cw[u] = ld(L.bx, p, giw, gj);
+            cn[u] = (gj + 1 < ny) ? ld(L.by, p, gi, gj) : 0.0;
+            cs[u] = (gj > 0) ? ld(L.by, p, gi, gj - 1) : 0.0;
+            id[u] = 1.0 / (ce[u] + cw[u] + cn[u] + cs[u]);
+        }
+    }
+}
+
+// half-sweep k of colour col on the cells at distance >= k from the edge
+template <int NU, int TX>
+__device__ __forceinline__ void tile_sweep(double *xs, int x0, int y0, int ny, int k, int col, const double *f,
+                                           const double *ce, const double *cw, const double *cn, const double *cs,
+                                           const double *id)
+{
+    using T = MgTile<NU, TX>;
+    double v[T::K];
+    bool act[T::K];
+#pragma unroll
+    for (int u = 0; u < T::K; u++) {
+        const int c = threadIdx.x + u * NTH, ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
+        act[u] = c < T::NC && gj >= 0 && gj < ny && (((x0 + lx + gj) & 1) == col) && lx >= k && lx < T::W - k &&
+                 ly >= k && ly < T::HT - k;
+        v[u] = 0.0;
+        if (act[u]) {
+            const double xe = (k > 0) ? xs[c + 1] : 0.0, xw = (k > 0) ? xs[c - 1] : 0.0;
+            const double xn = (k > 0) ? xs[c + T::W] : 0.0, xq = (k > 0) ? xs[c - T::W] : 0.0;
+            v[u] = (ce[u] * xe + cw[u] * xw + cn[u] * xn + cs[u] * xq - f[u]) * id[u];
+        }
+    }
+    // a colour's cells read only the other colour: no barrier before writing
+#pragma unroll
+    for (int u = 0; u < T::K; u++)
+        if (act[u]) xs[threadIdx.x + u * NTH] = v[u];
+    __syncthreads();
+}
+
+// pre-smoothing from x = 0 and restriction of the residual; the stop test of
+// iteration n in block (0, 0) of level 0 (check)
+template <int NU, int TX>
+__global__ void __launch_bounds__(NTH) k_mg_pre(pvk_mglev L, pvk_mglev C, pvk_scal *sc, pvk_pcg w, int n,
+                                               double tol, int maxit, int check)
+{
+    using T = MgTile<NU, TX>;
+    __shared__ double xs[T::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
-    const int ip = blockIdx.x * TXP + (threadIdx.x & (TXP - 1));
-    const int j = blockIdx.y * TYR + (threadIdx.x >> 6);
-    const int p = L.pitch;
-    double acc = 0.0;
-    if (2 * ip < L.nx && j < L.ny) {
-        const int i = 2 * ip + ((j + col) & 1);
-        const double f = ld(L.f, p, i, j);
-        double v;
-        if (MODE == 0) {
-            v = gs_value(L, i, j, 0.0, 0.0, 0.0, 0.0, f);
-        } else {
-            const int ie = (i + 1 == L.nx) ? 0 : i + 1, iw = (i == 0) ? L.nx - 1 : i - 1;
-            double xe = ld(L.x, p, ie, j), xw = ld(L.x, p, iw, j);
-            double xn = (j + 1 < L.ny) ? ld(L.x, p, i, j + 1) : 0.0;
-            double xs = (j > 0) ? ld(L.x, p, i, j - 1) : 0.0;
-            if (MODE == 2) {
-                const int cp = C.pitch;
-                xe += ld(C.x, cp, ie >> 1, j >> 1);
-                xw += ld(C.x, cp, iw >> 1, j >> 1);
-                if (j + 1 < L.ny) xn += ld(C.x, cp, i >> 1, (j + 1) >> 1);
-                if (j > 0) xs += ld(C.x, cp, i >> 1, (j - 1) >> 1);
-            }
-            v = gs_value(L, i, j, xe, xw, xn, xs, f);
+    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * MG_TY - T::H;
+    double f[T::K], ce[T::K], cw[T::K], cn[T::K], cs[T::K], id[T::K];
+    tile_coeffs<NU, TX>(L, x0, y0, f, ce, cw, cn, cs, id);
+    for (int c = threadIdx.x; c < T::NC; c += NTH) xs[c] = 0.0;
+    __syncthreads();
+    for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, TX>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
+    // restriction: one coarse cell per thread (TX/2 x MG_TY/2 <= NTH)
+    if ((int)threadIdx.x < (TX / 2) * (MG_TY / 2)) {
+        const int I = threadIdx.x % (TX / 2), J = threadIdx.x / (TX / 2);
+        const int gI = blockIdx.x * (TX / 2) + I, gJ = blockIdx.y * (MG_TY / 2) + J;
+        if (gI < C.nx && gJ < C.ny) {
+            const int p = L.pitch, nx = L.nx, ny = L.ny;
+            double s = 0.0;
+#pragma unroll
+            for (int dj = 0; dj < 2; dj++)
+#pragma unroll
+                for (int di = 0; di < 2; di++) {
+                    const int gi = 2 * gI + di, gj = 2 * gJ + dj, giw = (gi == 0) ? nx - 1 : gi - 1;
+                    const int c = (T::H + 2 * J + dj) * T::W + T::H + 2 * I + di;
+                    const double x0c = xs[c];
+                    double ax = ld(L.bx, p, gi, gj) * (xs[c + 1] - x0c) - ld(L.bx, p, giw, gj) * (x0c - xs[c - 1]);
+                    if (gj + 1 < ny) ax += ld(L.by, p, gi, gj) * (xs[c + T::W] - x0c);
+                    if (gj > 0) ax -= ld(L.by, p, gi, gj - 1) * (x0c - xs[c - T::W]);
+                    s += ld(L.f, p, gi, gj) - ax;
+                }
+            C.f[(ptrdiff_t)gJ * C.pitch + gI] = s;
         }
-        L.x[(ptrdiff_t)j * p + i] = v;
-        if (dot) acc = f * v + ld(L.f, p, i ^ 1, j) * ld(L.x, p, i ^ 1, j);
     }
-    if (dot) block_store_partial(acc, w.rz[slot], blockIdx.y * gridDim.x + blockIdx.x, sh);
+    // the core of x for the post-smoothing
+    for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
+        const int ly = c / TX, lx = c - ly * TX;
+        const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
+        if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + T::H) * T::W + lx + T::H];
+    }
 }
 
-// coarse right-hand side: sum of the four fine residuals f - A x
-__global__ void __launch_bounds__(NTH) k_mg_restrict(pvk_mglev L, pvk_mglev C, const pvk_scal *sc)
+// coarse correction x += xc(parent) and post-smoothing (black first);
+// dot: partial <f, x> of the core into w.rz[slot]
+template <int NU, int TX>
+__global__ void __launch_bounds__(NTH) k_mg_post(pvk_mglev L, pvk_mglev C, const pvk_scal *sc, pvk_pcg w, int dot,
+                                                int slot)
 {
+    using T = MgTile<NU, TX>;
+    __shared__ double xs[T::NC];
+    __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int I = blockIdx.x * TXP + (threadIdx.x & (TXP - 1)), J = blockIdx.y * TYR + (threadIdx.x >> 6);
-    if (I >= C.nx || J >= C.ny) return;
-    double s = 0.0;
-#pragma unroll
-    for (int dj = 0; dj < 2; dj++)
-#pragma unroll
-        for (int di = 0; di < 2; di++) {
-            const int i = 2 * I + di, j = 2 * J + dj;
-            s += ld(L.f, L.pitch, i, j) - a_x(L, i, j);
+    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * MG_TY - T::H;
+    double f[T::K], ce[T::K], cw[T::K], cn[T::K], cs[T::K], id[T::K];
+    tile_coeffs<NU, TX>(L, x0, y0, f, ce, cw, cn, cs, id);
+    for (int c = threadIdx.x; c < T::NC; c += NTH) {
+        const int ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
+        double v = 0.0;
+        if (gj >= 0 && gj < L.ny) {
+            int gi = (x0 + lx) % L.nx;
+            if (gi < 0) gi += L.nx;
+            v = ld(L.x, L.pitch, gi, gj) + ld(C.x, C.pitch, gi >> 1, gj >> 1);
         }
-    C.f[(ptrdiff_t)J * C.pitch + I] = s;
+        xs[c] = v;
+    }
+    __syncthreads();
+    for (int k = 1; k <= 2 * NU; k++) tile_sweep<NU, TX>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
+    double acc = 0.0;
+    for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
+        const int ly = c / TX, lx = c - ly * TX;
+        const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
+        if (gi < L.nx && gj < L.ny) {
+            const double v = xs[(ly + T::H) * T::W + lx + T::H];
+            L.x[(ptrdiff_t)gj * L.pitch + gi] = v;
+            if (dot) acc += ld(L.f, L.pitch, gi, gj) * v;
+        }
+    }
+    if (dot) block_store_partial(acc, w.rz[slot], blockIdx.y * gridDim.x + blockIdx.x, sh);
 }
 
 // coarse face coefficients from the fine ones
@@ -140,43 +228,52 @@ __global__ void __launch_bounds__(NTH) k_mg_faces(pvk_mglev L, pvk_mglev C)
 }
 
 // inverse of the bottom operator -A_B + s e0 e0^T (s = its (0,0) entry):
-// symmetric positive definite, in-place Gauss-Jordan without pivoting
-__global__ void __launch_bounds__(NTH) k_mg_bottom(pvk_mglev B, double *G)
+// symmetric positive definite, in-place Gauss-Jordan without pivoting; one
+// workgroup of 1024 threads, thread (r, q) owns row r, columns q + 16 t
+#define BT 1024
+__global__ void __launch_bounds__(BT) k_mg_bottom(pvk_mglev B, double *G)
 {
     __shared__ double M[PVK_MG_BOTTOM][PVK_MG_BOTTOM + 1];
-    __shared__ double fac[PVK_MG_BOTTOM];
-    __shared__ double piv;
     const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch;
-    for (int e = threadIdx.x; e < nb * nb; e += NTH) M[e / nb][e % nb] = 0.0;
+    const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
+    for (int c = q; c < PVK_MG_BOTTOM; c += 16) M[r][c] = 0.0;
     __syncthreads();
-    if ((int)threadIdx.x < nb) {
-        const int c = threadIdx.x, j = c / nx, i = c - j * nx;
+    if (q == 0 && r < nb) {
+        const int j = r / nx, i = r - j * nx;
         const int ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
         const double ce = ld(B.bx, p, i, j), cw = ld(B.bx, p, iw, j);
         const double cn = (j + 1 < ny) ? ld(B.by, p, i, j) : 0.0, cs = (j > 0) ? ld(B.by, p, i, j - 1) : 0.0;
-        M[c][c] += ce + cw + cn + cs;
-        M[c][j * nx + ie] -= ce;
-        M[c][j * nx + iw] -= cw;
-        if (j + 1 < ny) M[c][(j + 1) * nx + i] -= cn;
-        if (j > 0) M[c][(j - 1) * nx + i] -= cs;
+        M[r][r] += ce + cw + cn + cs;
+        M[r][j * nx + ie] -= ce;
+        M[r][j * nx + iw] -= cw;
+        if (j + 1 < ny) M[r][(j + 1) * nx + i] -= cn;
+        if (j > 0) M[r][(j - 1) * nx + i] -= cs;
     }
     __syncthreads();
     if (threadIdx.x == 0) M[0][0] += M[0][0];
     __syncthreads();
     for (int k = 0; k < nb; k++) {
-        if (threadIdx.x == 0) piv = 1.0 / M[k][k];
+        // read phase: the pivot row and this row's factor, as they stand
+        const double piv = 1.0 / M[k][k], fr = M[r][k];
+        double mk[PVK_MG_BOTTOM / 16], mr[PVK_MG_BOTTOM / 16];
+#pragma unroll
+        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+            const int c = q + 16 * t;
+            mk[t] = (c == k) ? piv : M[k][c] * piv;           // new pivot row
+            mr[t] = M[r][c];
+        }
         __syncthreads();
-        for (int c = threadIdx.x; c < nb; c += NTH) M[k][c] = ((c == k) ? 1.0 : M[k][c]) * piv;
-        __syncthreads();
-        for (int r = threadIdx.x; r < nb; r += NTH) fac[r] = (r == k) ? 0.0 : M[r][k];
-        __syncthreads();
-        for (int e = threadIdx.x; e < nb * nb; e += NTH) {
-            const int r = e / nb, c = e - r * nb;
-            if (r != k) M[r][c] = ((c == k) ? 0.0 : M[r][c]) - fac[r] * M[k][c];
+        if (r < nb) {
+#pragma unroll
+            for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+                const int c = q + 16 * t;
+                if (c < nb) M[r][c] = (r == k) ? mk[t] : ((c == k) ? 0.0 : mr[t]) - fr * mk[t];
+            }
         }
         __syncthreads();
     }
-    for (int e = threadIdx.x; e < nb * nb; e += NTH) G[e] = M[e / nb][e % nb];
+    if (r < nb)
+        for (int c = q; c < nb; c += 16) G[r * nb + c] = M[r][c];
 }
 
 // ---------------------------------------------------------------------------
@@ -252,47 +349,55 @@ __device__ void lds_restrict(const LdsLev &L, const LdsLev &C)
     __syncthreads();
 }
 
+// level l of the in-LDS levels: x, f, bx, by of each level in turn
+__device__ __forceinline__ LdsLev lds_level(const pvk_mg &m, double *lds, int l)
+{
+    double *q = lds;
+    for (int k = m.c0; k < l; k++) q += 4 * m.L[k].nx * m.L[k].ny;
+    const int nc = m.L[l].nx * m.L[l].ny;
+    LdsLev v;
+    v.nx = m.L[l].nx;
+    v.ny = m.L[l].ny;
+    v.x = q;
+    v.f = q + nc;
+    v.bx = q + 2 * nc;
+    v.by = q + 3 * nc;
+    return v;
+}
+
 __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 {
     extern __shared__ double lds[];
     __shared__ double red[CT / 64];
     __shared__ double mean;
     if (sc->p_done) return;
-    LdsLev lv[PVK_MG_MAXLEV];
-    {
-        double *q = lds;
-        for (int l = m.c0; l < m.nlev; l++) {
-            const int nc = m.L[l].nx * m.L[l].ny;
-            lv[l].nx = m.L[l].nx; lv[l].ny = m.L[l].ny;
-            lv[l].x = q; lv[l].f = q + nc; lv[l].bx = q + 2 * nc; lv[l].by = q + 3 * nc;
-            q += 4 * nc;
-        }
-    }
     // faces of every in-LDS level and the right-hand side of the first
     for (int l = m.c0; l < m.nlev; l++) {
         const pvk_mglev &G = m.L[l];
+        const LdsLev Ll = lds_level(m, lds, l);
         const int hx = G.nx;
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
             const int j = q / hx, i = q - j * hx;
-            lv[l].bx[q] = ld(G.bx, G.pitch, i, j);
-            lv[l].by[q] = ld(G.by, G.pitch, i, j);
-            if (l == m.c0) lv[l].f[q] = ld(G.f, G.pitch, i, j);
+            Ll.bx[q] = ld(G.bx, G.pitch, i, j);
+            Ll.by[q] = ld(G.by, G.pitch, i, j);
+            if (l == m.c0) Ll.f[q] = ld(G.f, G.pitch, i, j);
         }
     }
     __syncthreads();
     const int nb = m.nlev - 1;
     for (int l = m.c0; l < nb; l++) {
-        lds_sweep(lv[l], lv[l], 0, 0);
-        lds_sweep(lv[l], lv[l], 1, 1);
+        const LdsLev Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        lds_sweep(Ll, Ll, 0, 0);
+        lds_sweep(Ll, Ll, 1, 1);
         for (int s = 1; s < m.nu; s++) {
-            lds_sweep(lv[l], lv[l], 0, 1);
-            lds_sweep(lv[l], lv[l], 1, 1);
+            lds_sweep(Ll, Ll, 0, 1);
+            lds_sweep(Ll, Ll, 1, 1);
         }
-        lds_restrict(lv[l], lv[l + 1]);
+        lds_restrict(Ll, Lc);
     }
     // bottom: x = -P G P f, P the projection to zero mean
     {
-        const LdsLev &B = lv[nb];
+        const LdsLev B = lds_level(m, lds, nb);
         const int n = B.nx * B.ny;
         if (threadIdx.x == 0) {
             double s = 0.0;
@@ -318,18 +423,20 @@ __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
         __syncthreads();
     }
     for (int l = nb - 1; l >= m.c0; l--) {
-        lds_sweep(lv[l], lv[l + 1], 1, 2);
-        lds_sweep(lv[l], lv[l], 0, 1);
+        const LdsLev Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        lds_sweep(Ll, Lc, 1, 2);
+        lds_sweep(Ll, Ll, 0, 1);
         for (int s = 1; s < m.nu; s++) {
-            lds_sweep(lv[l], lv[l], 1, 1);
-            lds_sweep(lv[l], lv[l], 0, 1);
+            lds_sweep(Ll, Ll, 1, 1);
+            lds_sweep(Ll, Ll, 0, 1);
         }
     }
     {
         const pvk_mglev &G = m.L[m.c0];
+        const LdsLev top = lds_level(m, lds, m.c0);
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
             const int j = q / G.nx, i = q - j * G.nx;
-            G.x[(ptrdiff_t)j * G.pitch + i] = lv[m.c0].x[q];
+            G.x[(ptrdiff_t)j * G.pitch + i] = top.x[q];
         }
     }
     (void)red;
@@ -397,14 +504,20 @@ __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *s
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-static dim3 pair_grid(const pvk_mglev &L) { return dim3((L.nx / 2 + TXP - 1) / TXP, (L.ny + TYR - 1) / TYR); }
+// tile width by level: 64 cells, 32 on levels at most 256 wide (more blocks)
+static int tile_tx(const pvk_mglev &L) { return (L.nx > 256) ? 64 : 32; }
+static dim3 tile_grid(const pvk_mglev &L)
+{
+    const int tx = tile_tx(L);
+    return dim3((L.nx + tx - 1) / tx, (L.ny + MG_TY - 1) / MG_TY);
+}
 static dim3 cell_grid(const pvk_mglev &L) { return dim3((L.nx + TXP - 1) / TXP, (L.ny + TYR - 1) / TYR); }
 static size_t coarse_lds(const pvk_mg *m) { return (size_t)m->ldsx * sizeof(double); }
 
 extern "C" int pvk_mg_nrz(const pvk_mg *m)
 {
     if (m->c0 == 0) return 1;
-    const dim3 gr = pair_grid(m->L[0]);
+    const dim3 gr = tile_grid(m->L[0]);
     return (int)(gr.x * gr.y);
 }
 
@@ -413,45 +526,46 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, hipStream_t s)
 {
     for (int l = 0; l + 1 < m->nlev; l++)
         hipLaunchKernelGGL(k_mg_faces, cell_grid(m->L[l + 1]), dim3(NTH), 0, s, m->L[l], m->L[l + 1]);
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(NTH), 0, s, m->L[m->nlev - 1], m->G);
+    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(BT), 0, s, m->L[m->nlev - 1], m->G);
 }
 
-// z = V(r) for CG iteration n (n < 0: the start; n = -3: no stop test),
-// with the stop test of n first and <r, z> partials into w->rz[(n < 0) ? 0 : (n + 1) & 1]
-extern "C" void pvk_mg_vcycle(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
+// One V-cycle z = V(r) for CG iteration n is, in order: pvk_mg_pre for
+// levels 0 .. c0-1, pvk_mg_coarse, pvk_mg_post for levels c0-1 .. 0.  The
+// stop test of n runs first (n = -3: none); <r, z> partials go to
+// w->rz[(n < 0) ? 0 : (n + 1) & 1].
+static int rz_slot(int n) { return (n < 0) ? 0 : ((n + 1) & 1); }
+
+#define MG_LAUNCH(KERNEL, L, ...)                                                                          \
+    do {                                                                                                   \
+        const dim3 gr_ = tile_grid(L);                                                                     \
+        if (m->nu == 1 && tile_tx(L) == 64)                                                                \
+            hipLaunchKernelGGL((KERNEL<1, 64>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
+        else if (m->nu == 1)                                                                               \
+            hipLaunchKernelGGL((KERNEL<1, 32>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
+        else if (tile_tx(L) == 64)                                                                         \
+            hipLaunchKernelGGL((KERNEL<2, 64>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
+        else                                                                                               \
+            hipLaunchKernelGGL((KERNEL<2, 32>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
+    } while (0)
+
+extern "C" void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
+                           hipStream_t s)
+{
+    MG_LAUNCH(k_mg_pre, m->L[l], m->L[l], m->L[l + 1], sc, *w, n, tol, maxit, l == 0);
+}
+
+extern "C" void pvk_mg_coarse(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                               hipStream_t s)
 {
-    const int slot = (n < 0) ? 0 : ((n + 1) & 1);
-    if (m->c0 == 0) {
-        hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
-        hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CT), coarse_lds(m), s, *m, sc);
-        hipLaunchKernelGGL(k_mg_dot, dim3(1), dim3(NTH), 0, s, m->L[0], sc, *w, slot);
-        return;
-    }
-    for (int l = 0; l < m->c0; l++) {
-        const pvk_mglev &L = m->L[l], &C = m->L[l + 1];
-        const dim3 gr = pair_grid(L);
-        hipLaunchKernelGGL(k_mg_smooth<0>, gr, dim3(NTH), 0, s, L, C, 0, sc, *w, n, tol, maxit, l == 0, 0, 0);
-        hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 1, sc, *w, n, tol, maxit, 0, 0, 0);
-        for (int k = 1; k < m->nu; k++) {
-            hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 0, sc, *w, n, tol, maxit, 0, 0, 0);
-            hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 1, sc, *w, n, tol, maxit, 0, 0, 0);
-        }
-        hipLaunchKernelGGL(k_mg_restrict, cell_grid(C), dim3(NTH), 0, s, L, C, (const pvk_scal *)sc);
-    }
+    if (m->c0 == 0) hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
     hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CT), coarse_lds(m), s, *m, sc);
-    for (int l = m->c0 - 1; l >= 0; l--) {
-        const pvk_mglev &L = m->L[l], &C = m->L[l + 1];
-        const dim3 gr = pair_grid(L);
-        const int last = (m->nu == 1) && l == 0;
-        hipLaunchKernelGGL(k_mg_smooth<2>, gr, dim3(NTH), 0, s, L, C, 1, sc, *w, n, tol, maxit, 0, 0, 0);
-        hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 0, sc, *w, n, tol, maxit, 0, last, slot);
-        for (int k = 1; k < m->nu; k++) {
-            const int dl = (k + 1 == m->nu) && l == 0;
-            hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 1, sc, *w, n, tol, maxit, 0, 0, 0);
-            hipLaunchKernelGGL(k_mg_smooth<1>, gr, dim3(NTH), 0, s, L, C, 0, sc, *w, n, tol, maxit, 0, dl, slot);
-        }
-    }
+    if (m->c0 == 0) hipLaunchKernelGGL(k_mg_dot, dim3(1), dim3(NTH), 0, s, m->L[0], (const pvk_scal *)sc, *w,
+                                       rz_slot(n));
+}
+
+extern "C" void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
+{
+    MG_LAUNCH(k_mg_post, m->L[l], m->L[l], m->L[l + 1], (const pvk_scal *)sc, *w, l == 0, rz_slot(n));
 }
 
 extern "C" void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s)

@@ -19,6 +19,9 @@ enum pv_internal_field {
     PV_FI_PBX,                 /* pressure operator face coefficients, x and y */
     PV_FI_PBY,
     PV_FI_LTMP,                /* right-hand side of the learned preconditioner's FFT solve */
+    PV_FI_PHI0,                /* pressure correction of the last solve of RK3 stage 0, 1, 2 */
+    PV_FI_PHI1,
+    PV_FI_PHI2,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
@@ -85,6 +88,9 @@ struct pv_bag {
     pvk_pcg mgw;                   /* CG work set of the MG-PCG: fields + its own partials */
     double *mgmem;
     int mg_ok, mg_chunk;
+    int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
+    int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
+    int net_losses, net_skip;      /* consecutive losses of the network's start; solves left to skip it */
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
     pvk_cnn cnn;                   /* learned preconditioner network (pv_cnn_init) */
     int cnn_ready;

@@ -6,9 +6,10 @@
  * Level 0 is the pressure grid itself (z, r and the face coefficients of the
  * CG, device pitch P).  Levels halve both sizes while the level has more than
  * PVK_MG_LDS_CELLS cells (the first level at or below that size, c0, and all
- * levels after it run in one workgroup from LDS), then on to a last level of
- * at most PVK_MG_BOTTOM cells.  Every level before the last must have even
- * sizes; a grid that does not halve that way has no MG-PCG (mg_ok = 0).
+ * levels after it run in one workgroup from LDS), then on while both sizes
+ * are even and the level has more than PVK_MG_BOTTOM_MIN cells; the last
+ * level must have at most PVK_MG_BOTTOM cells.  A grid that does not halve
+ * that way has no MG-PCG (mg_ok = 0).
  */
 #include "pv_host.h"
 #include <stdlib.h>
@@ -34,12 +35,13 @@ int pv_mg_create(pv_bag *b)
         nl++;
     }
     c0 = nl - 1;
-    while (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM) {
-        if ((lx[nl - 1] & 1) || (ly[nl - 1] & 1) || nl == PVK_MG_MAXLEV) return PV_OK;
+    while (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM_MIN && !(lx[nl - 1] & 1) && !(ly[nl - 1] & 1) &&
+           nl < PVK_MG_MAXLEV) {
         lx[nl] = lx[nl - 1] / 2;
         ly[nl] = ly[nl - 1] / 2;
         nl++;
     }
+    if (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM) return PV_OK;
     /* the smoothing levels (all but the last) pair their cells along x */
     for (l = 0; l + 1 < nl; l++)
         if (lx[l] & 1) return PV_OK;
@@ -112,10 +114,19 @@ void pv_mg_destroy(pv_bag *b)
     b->mg_ok = 0;
 }
 
+int pv_set_pressure_warm_start(pv_bag *b, int mode)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (mode < 0 || mode > 1) return pv_set_error(PV_EINVAL, "warm start mode must be 0 or 1 (got %d)", mode);
+    b->p_warm = mode;
+    b->have_prev[0] = b->have_prev[1] = b->have_prev[2] = 0;
+    return PV_OK;
+}
+
 int pv_set_mg_sweeps(pv_bag *b, int nu)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");
-    if (nu < 1 || nu > 4) return pv_set_error(PV_EINVAL, "MG sweeps must be 1..4 (got %d)", nu);
+    if (nu < 1 || nu > 2) return pv_set_error(PV_EINVAL, "MG sweeps must be 1 or 2 (got %d)", nu);
     b->mg.nu = nu;
     return PV_OK;
 }

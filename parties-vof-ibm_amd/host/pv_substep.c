@@ -498,11 +498,26 @@ static int p_solve_fft_pcg(pv_bag *b)
 }
 
 /* ---- MG-PCG: the CG above with one multigrid V-cycle as preconditioner ---- */
+/* z = V(r) for CG iteration n (the stop test of n first; n = -3: none) */
 static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
 {
-    pv_tbegin(b, PV_T_MG_VCYCLE);
-    pvk_mg_vcycle(&b->mg, b->d_sc, &b->mgw, n, tol, maxit, S(b));
-    pv_tend(b, PV_T_MG_VCYCLE);
+    const pvk_mg *m = &b->mg;
+    int l;
+    if (m->c0 > 0) {
+        pv_tbegin(b, PV_T_MG_PRE0);
+        pvk_mg_pre(m, 0, b->d_sc, &b->mgw, n, tol, maxit, S(b));
+        pv_tend(b, PV_T_MG_PRE0);
+    }
+    pv_tbegin(b, PV_T_MG_COARSE);
+    for (l = 1; l < m->c0; l++) pvk_mg_pre(m, l, b->d_sc, &b->mgw, n, tol, maxit, S(b));
+    pvk_mg_coarse(m, b->d_sc, &b->mgw, n, tol, maxit, S(b));
+    for (l = m->c0 - 1; l >= 1; l--) pvk_mg_post(m, l, b->d_sc, &b->mgw, n, S(b));
+    pv_tend(b, PV_T_MG_COARSE);
+    if (m->c0 > 0) {
+        pv_tbegin(b, PV_T_MG_POST0);
+        pvk_mg_post(m, 0, b->d_sc, &b->mgw, n, S(b));
+        pv_tend(b, PV_T_MG_POST0);
+    }
 }
 
 static void mg_iteration(pv_bag *b, int n, double tol, int maxit)
@@ -543,57 +558,103 @@ static int mg_loop(pv_bag *b, int n0, double tol, int maxit)
     return PV_OK;
 }
 
-static int p_solve_mg(pv_bag *b)
+/* energy |<r, V r>| of the current MG start (after its V-cycle) */
+static int mg_energy(pv_bag *b, double *e)
 {
-    int maxit = b->ph.p_max_iter;
-    double tol = b->ph.p_rel_tol;
-    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
-    mg_cold_start(b, tol, maxit);
-    return mg_loop(b, 0, tol, maxit);
+    pvk_pcg_gamma(b->d_sc, &b->mgw, 0, 3, S(b));
+    LAUNCHCHK();
+    RC(pv_read_scalars(b));
+    *e = fabs(b->h_sc->red[3]);
+    return PV_OK;
+}
+
+/* restart the MG-PCG from the phi in PHI: r = rhs - mean - A phi, z = V r */
+static void mg_warm_restart(pv_bag *b, double tol, int maxit)
+{
+    pvk_pcg_opx(G(b), &b->mgw, FD(b, PV_F_PHI), b->mgw.Ad, S(b));
+    pvk_mg_start(G(b), b->d_sc, &b->mgw, 1, S(b));
+    mg_vcycle(b, -2, tol, maxit);
 }
 
 static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit);
 
-/* MG-PCG from the learned warm start phi0 = M_fft(b + s c(b, M_fft b, rho)),
- * judged by the energy |<r, V r>| (V the V-cycle, close to A^-1) of its
- * residual against the cold start's: kept if smaller, else the cold start is
- * restored.  The network features are those it was trained on (M_fft, not
- * V), so the FFT tables are needed too. */
-static int p_solve_learned_mg(pv_bag *b)
+/* MG-PCG.  The start is the best of up to three candidates by the energy
+ * |<r, V r>| of their residuals (V the V-cycle, close to A^-1, so this is
+ * close to the A-norm of the error): phi = 0 (always); with p_warm the last
+ * correction of the same RK3 stage; with `learned` the better of those plus
+ * the network's correction M_fft(r + s c(r, M_fft r, rho)) (the features it
+ * was trained on, so the FFT tables are needed).  learn_start_res is the
+ * chosen start's energy relative to the cold start's; a candidate that loses
+ * counts as a fallback.  After 3 losses in a row the network's candidate is
+ * not formed for the next 16 solves (then tried again), so a network that
+ * does not help costs one try in 19 solves. */
+static int mg_solve(pv_bag *b, int learned)
 {
-    int maxit = b->ph.p_max_iter;
-    double tol = b->ph.p_rel_tol, gw, gc;
+    int maxit = b->ph.p_max_iter, st = b->stage, best = 0;
+    double tol = b->ph.p_rel_tol, ec, eb, e;
     if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (power-of-two sizes)");
-    if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    if (learned && !b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (power-of-two sizes)");
+    if (learned && !b->cnn_ready)
+        return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
     mg_cold_start(b, tol, maxit);
-    pvk_pcg_gamma(b->d_sc, &b->mgw, 0, 4, S(b));
-    LAUNCHCHK();
-    RC(pv_read_scalars(b));
+    b->learn_start_res = 1.0;
+    if (!b->p_warm && !learned) return mg_loop(b, 0, tol, maxit);
+    RC(mg_energy(b, &ec));
+    eb = ec;
     if (b->h_sc->p_done) return PV_OK;
-    fft_solve_field(b, b->pcg.r, tol, maxit);               /* z = M_fft r0: the features' second input */
-    RC(pv_learn_source(b, FD(b, PV_FI_LTMP)));
-    fft_solve_field(b, FD(b, PV_FI_LTMP), tol, maxit);
-    pvk_learn_copy(G(b), b->pcg.z, FD(b, PV_F_PHI), S(b));
-    pvk_pcg_opx(G(b), &b->mgw, FD(b, PV_F_PHI), b->mgw.Ad, S(b));
-    pvk_mg_start(G(b), b->d_sc, &b->mgw, 1, S(b));
-    mg_vcycle(b, -2, tol, maxit);
-    pvk_pcg_gamma(b->d_sc, &b->mgw, 0, 3, S(b));
-    LAUNCHCHK();
-    RC(pv_read_scalars(b));
-    gw = fabs(b->h_sc->red[3]);
-    gc = fabs(b->h_sc->red[4]);
-    b->learn_start_res = (gc > 0.0) ? gw / gc : 0.0;
-    if (!b->h_sc->p_done && gw >= gc) {
-        b->st.p_fallback[b->stage] += 1;
-        pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
-        mg_vcycle(b, -1, tol, maxit);
+    if (b->p_warm && b->have_prev[st]) {
+        pvk_learn_copy(G(b), FD(b, PV_FI_PHI0 + st), FD(b, PV_F_PHI), S(b));
+        mg_warm_restart(b, tol, maxit);
+        RC(mg_energy(b, &e));
+        if (e < eb) { eb = e; best = 1; }
+        else b->st.p_fallback[st] += 1;
+    }
+    if (learned && b->net_skip > 0) {
+        b->net_skip--;
+        learned = 0;
+    }
+    if (learned && !b->h_sc->p_done) {
+        if (best == 0 && b->p_warm && b->have_prev[st]) {   /* back to the cold state */
+            pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
+            mg_vcycle(b, -1, tol, maxit);
+        }
+        fft_solve_field(b, b->mgw.r, tol, maxit);            /* z = M_fft r: the second feature */
+        RC(pv_learn_source(b, FD(b, PV_FI_LTMP)));
+        fft_solve_field(b, FD(b, PV_FI_LTMP), tol, maxit);
+        pvk_learn_axpy(G(b), b->pcg.z, FD(b, PV_F_PHI), S(b));
+        mg_warm_restart(b, tol, maxit);
+        RC(mg_energy(b, &e));
+        if (e < eb) {
+            eb = e;
+            best = 2;
+            b->net_losses = 0;
+        } else {
+            b->st.p_fallback[st] += 1;
+            if (++b->net_losses >= 3) {
+                b->net_losses = 0;
+                b->net_skip = 16;
+            }
+        }
+    }
+    if (!b->h_sc->p_done && ((learned && best != 2) || (!learned && best == 0 && b->p_warm && b->have_prev[st]))) {
+        /* the winner is not the state in place: rebuild it */
+        if (best == 0) {
+            pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
+            mg_vcycle(b, -1, tol, maxit);
+        } else {
+            pvk_learn_copy(G(b), FD(b, PV_FI_PHI0 + st), FD(b, PV_F_PHI), S(b));
+            mg_warm_restart(b, tol, maxit);
+        }
         LAUNCHCHK();
         RC(pv_read_scalars(b));
     }
+    b->learn_start_res = (ec > 0.0) ? eb / ec : 0.0;
     if (b->h_sc->p_done) return PV_OK;
     return mg_loop(b, 0, tol, maxit);
 }
+
+static int p_solve_mg(pv_bag *b) { return mg_solve(b, 0); }
+static int p_solve_learned_mg(pv_bag *b) { return mg_solve(b, 1); }
 
 /* The plain FFT-PCG with training snapshots (phi_k, z_k, network features)
  * at the collector's iterations; crops are harvested after convergence. */
@@ -746,6 +807,10 @@ static int p_solve(pv_bag *b)
     }
     b->st.n_p[b->stage] += b->h_sc->p_iter;
     b->st.p_res[b->stage] = b->h_sc->p_res;
+    if (b->p_warm) {
+        pvk_learn_copy(G(b), FD(b, PV_F_PHI), FD(b, PV_FI_PHI0 + b->stage), S(b));
+        b->have_prev[b->stage] = 1;
+    }
     return PV_OK;
 }
 
@@ -774,7 +839,7 @@ int pv_mg_vcycle_apply(pv_bag *b)
     if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
     pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
     pvk_mg_setup(&b->mg, S(b));
-    pvk_mg_vcycle(&b->mg, b->d_sc, &b->mgw, -3, 0.0, 1, S(b));
+    mg_vcycle(b, -3, 0.0, 1);
     pvk_learn_copy(G(b), b->mgw.z, FD(b, PV_F_SCRATCH0), S(b));
     LAUNCHCHK();
     return pv_read_scalars(b);
@@ -813,6 +878,7 @@ static int pressure_passes(pv_bag *b)
     int passes = 0, st = b->stage;
     double div;
     b->st.n_p[st] = 0;
+    b->st.p_fallback[st] = 0;
     do {
         pvk_fill(FD(b, PV_F_PHI) - 3 * (size_t)b->P - 3, (size_t)b->P * b->NYT, 0.0, S(b));
         pvk_p_rhs2(G(b), st, b->d_sc, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_F_RHO), FD(b, PV_F_P_RHS),

@@ -6,8 +6,9 @@ The operator is the variable-coefficient pressure operator of the PCG
 psolve_cg.c:63-101), x periodic, zero-gradient walls in y:
     (A x)_ij = bx_ij (x_E - x) - bx_(i-1)j (x - x_W) + by_ij (x_N - x) - by_i(j-1) (x - x_S).
 One V-cycle V(f), arrays indexed [j][i]:
-  * levels halve both sizes while a level has more than 1024 cells, then on
-    to a last level of at most 64 cells; coarse faces are half the sum of the
+  * levels halve both sizes while a level has more than 1024 cells, then
+    while both sizes are even and the level has more than 64 cells; the last
+    level must have at most 64 cells; coarse faces are half the sum of the
     two fine faces across the coarse face;
   * nu red-black Gauss-Seidel sweeps (red = (i + j) even first) from x = 0,
     restriction by the sum of the four fine residuals, the coarse V-cycle,
@@ -20,6 +21,7 @@ catch it): "post_red_first", "restrict_mean", "coarse_sum".
 import numpy as np
 
 LDS_CELLS = 1024
+BOTTOM_MIN = 64
 BOTTOM = 64
 
 
@@ -41,11 +43,10 @@ def apply_a(bx, by, x):
 
 def level_sizes(nx, ny):
     sizes = [(nx, ny)]
-    while sizes[-1][0] * sizes[-1][1] > BOTTOM:
-        a, b = sizes[-1]
-        if a % 2 or b % 2:
-            raise ValueError(f"{nx} x {ny} does not halve to a last level of <= {BOTTOM} cells")
-        sizes.append((a // 2, b // 2))
+    while sizes[-1][0] * sizes[-1][1] > BOTTOM_MIN and sizes[-1][0] % 2 == 0 and sizes[-1][1] % 2 == 0:
+        sizes.append((sizes[-1][0] // 2, sizes[-1][1] // 2))
+    if sizes[-1][0] * sizes[-1][1] > BOTTOM:
+        raise ValueError(f"{nx} x {ny} does not halve to a last level of <= {BOTTOM} cells")
     return sizes
 
 

@@ -109,3 +109,13 @@ def test_multimode_rt_ic():
     col = f[:, 0]
     yint = 4.0 * np.argmin(np.abs(col - 0.5)) / 256
     assert abs(yint - 2.0) <= 0.4 + 4.0 / 256
+
+
+def test_bench_timer_names_match_header():
+    """bench.py reads PV_NTIMERS totals into an array sized by its name list."""
+    import re
+    import bench
+    hdr = open(os.path.join(ROOT, "include", "parties_vof.h")).read()
+    body = hdr[hdr.index("enum pv_timer {"):hdr.index("PV_NTIMERS")]
+    names = re.findall(r"PV_T_([A-Z0-9_]+)", body)
+    assert names == bench.TIMERS

@@ -3,10 +3,12 @@ without it (ablation), on the GPU.
 
 The network is trained on the GPU by the project's own kernels on crops of
 the plain FFT-PCG solves of one seeded bubble field, then evaluated on a
-held-out seed: the same RK3 steps are run from the same state with the plain
-FFT-PCG, the learned warm start, and the learned flexible CG.  All three
-solve to the same relative residual, so the pressure fields agree to the
-solve tolerance; the test records the pressure iterations of each.
+held-out seed: the same RK3 steps are run from the same state with each
+pressure solver with and without the network (FFT-PCG / learned warm start /
+learned flexible CG, and MG-PCG / learned warm start + MG-PCG).  All solve to
+the same relative residual, so the pressure fields agree to the solve
+tolerance; the test records the pressure iterations of each, and that a warm
+start is kept only when its residual energy is below the cold start's.
 """
 import ctypes as C
 import os
@@ -46,9 +48,9 @@ def run_mode(lib, p, prm, solver, steps=3):
     s = pvof.Solver(p, lib)
     s.init_from_F(F)
     s.call("Temporal_int_rk3", 2)
-    if solver != pvof.PV_PSOLVE_FFT_PCG:
+    if solver not in (pvof.PV_PSOLVE_FFT_PCG, pvof.PV_PSOLVE_MG_PCG):
         pvof.Network(s).set_params(prm)
-        s.check(lib.pv_set_pressure_solver(s.h, solver))
+    s.check(lib.pv_set_pressure_solver(s.h, solver))
     iters, starts, fb = [], [], 0
     for _ in range(steps):
         s.call("Temporal_int_rk3", 1)
@@ -71,10 +73,12 @@ def test_training_reduces_loss(trained):
 def test_learned_modes_vs_plain(trained):
     lib, p, prm, _, _, _ = trained
     base, it0, _, _ = run_mode(lib, p, prm, pvof.PV_PSOLVE_FFT_PCG)
+    _, itm, _, _ = run_mode(lib, p, prm, pvof.PV_PSOLVE_MG_PCG)
     R = regions(p)
-    report = [f"plain FFT-PCG iterations per step {it0}"]
+    report = [f"plain FFT-PCG iterations per step {it0}", f"MG-PCG iterations per step {itm}"]
     for name, solver in (("learned warm start", pvof.PV_PSOLVE_LEARNED),
-                         ("learned FCG", pvof.PV_PSOLVE_LEARNED_FCG)):
+                         ("learned FCG", pvof.PV_PSOLVE_LEARNED_FCG),
+                         ("learned warm start + MG-PCG", pvof.PV_PSOLVE_LEARNED_MG)):
         s, it, starts, fb = run_mode(lib, p, prm, solver)
         report.append(f"{name}: iterations {it}, start residuals {[round(x, 4) for x in starts]}, "
                       f"fallbacks {fb}")

@@ -103,3 +103,32 @@ def test_rk3_steps_match_fft_pcg(lib):
     R = regions(pm)
     for f in ("P", "U", "V", "C_L"):
         assert relerr(out["mg"][0].get(f)[R["cell"]], out["fft"][0].get(f)[R["cell"]]) < 1e-9, f
+
+
+def test_temporal_warm_start(lib):
+    """Starting each solve from the same RK3 stage's last correction (kept
+    only when its residual energy is lower) changes the fields only below the
+    solve tolerance and never costs iterations over the cold start."""
+    n = 128
+    pm = params(n, n)
+    F = pvof.ic(lib, "bubbles", pm, seed=223, nmin=8, nmax=24, rmin=0.02, rmax=0.1, gas_inside=1)
+    out = {}
+    for warm in (0, 1):
+        s = pvof.Solver(pm, lib)
+        s.init_from_F(F)
+        s.check(lib.pv_set_pressure_warm_start(s.h, warm))
+        iters, ratio = 0, C.c_double(0.0)
+        for _ in range(4):
+            s.call("Temporal_int_rk3", 1)
+            st = s.stats()
+            assert max(st["p_res"]) <= 1e-10
+            iters += sum(st["n_p"])
+        lib.pv_learn_start_residual(s.h, C.byref(ratio))
+        out[warm] = (s, iters, ratio.value)
+    print(f"\nMG-PCG iterations over 4 steps: cold {out[0][1]}, warm {out[1][1]} "
+          f"(last start energy ratio {out[1][2]:.3f})")
+    assert out[1][1] <= out[0][1]
+    assert 0.0 < out[1][2] < 1.0
+    R = regions(pm)
+    for f in ("P", "U", "V", "C_L"):
+        assert relerr(out[1][0].get(f)[R["cell"]], out[0][0].get(f)[R["cell"]]) < 1e-9, f

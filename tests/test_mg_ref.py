@@ -55,6 +55,7 @@ def test_level_sizes():
     assert M.level_sizes(1024, 1024)[-1] == (8, 8)
     assert M.level_sizes(96, 96)[-1] == (6, 6)
     assert M.level_sizes(384, 768)[-1] == (3, 6)
+    assert M.level_sizes(32, 64)[-1] == (4, 8)
     with pytest.raises(ValueError):
         M.level_sizes(100, 100)
 
