@@ -14,7 +14,9 @@ including CFL).  value = NXM * NYM * steps / seconds / 1e6 (Mcell-steps/s).
 
 Default run: the learned, mixed-precision form (network warm start +
 multigrid-preconditioned CG pressure, FP32 stencils with FP64 fluxes, solves
-and dots; the network is trained first, untimed, on a held-out seed), then on
+and dots; each solve starts from the best of phi = 0, the same RK3 stage's
+last correction and the network's correction by residual energy; the network
+is trained first, untimed, on a held-out seed), then on
 the same GPU the plain FP64 path (FFT-preconditioned CG, the reference
 algorithm) and the same form without the network (ablation), each reported
 with its own Mcell-steps/s.  --solver / --precision time one configuration.
@@ -45,7 +47,7 @@ TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDA
 # algorithmic HBM bytes per cell and launch (each distinct array read or
 # written once; stencil neighbours come from cache)
 BYTES_PER_CELL = {
-    "CH_APPLY": 16, "CH_UPDATE": 48, "MOM_APPLY": 32, "MOM_UPDATE": 56,
+    "CH_APPLY": 32, "CH_UPDATE": 48, "MOM_APPLY": 32, "MOM_UPDATE": 56,
     "P_APPLY": 24, "P_UPDATE": 48, "P_DUPDATE": 24,
     "FFT_ROWS_FWD": 16, "FFT_COLS": 16, "FFT_ROWS_INV": 24,
     "CH_RHS": 56, "MOM_TERMS": 88, "P_TRANSPOSE": 16,
@@ -145,6 +147,8 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     s.init_from_F(F)
     if net_params is not None:
         pvof.Network(s).set_params(net_params)
+    if family_of(solver) == "mg":
+        s.check(lib.pv_set_pressure_warm_start(s.h, args.warm_start))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -207,6 +211,8 @@ def main():
                          "untimed, on a held-out seed)")
     ap.add_argument("--precision", choices=["fp64", "mixed"], default="mixed",
                     help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
+    ap.add_argument("--warm-start", type=int, choices=[0, 1], default=1,
+                    help="MG-PCG solvers: 1 = also try the same RK3 stage's last correction as the start")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain FP64 FFT-PCG run and the no-network ablation")
     ap.add_argument("--plain-steps", type=int, default=10)
@@ -261,6 +267,7 @@ def main():
         "config": {"workload": f"planar {p.NXM}x{p.NYM} bubbly two-phase RK3 step, rho2/rho1=1e-3, "
                                f"{LABEL[args.solver]} pressure to 1e-10 relative residual",
                    "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
+                   "warm_start": args.warm_start if family_of(args.solver) == "mg" else None,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": main_run["path_hbm_fraction"],
         "path_bytes_per_cell_step": main_run["path_bytes_per_cell_step"],

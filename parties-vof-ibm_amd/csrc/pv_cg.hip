@@ -28,22 +28,6 @@
 // (VOF_DIFFUSE.c:571-683; all ghost layers equal the first interior value).
 __device__ __forceinline__ int chy(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
 
-// L(L f) of bilap_at with f given by an accessor that applies the CH map.
-template <class F>
-__device__ __forceinline__ double bilap_fn(const pvk_grid &g, const F &f, int i, int j)
-{
-    const int NXM = g.NXM, NYM = g.NYM;
-    auto lapf = [&](int a, int b) {
-        return lap_vals(g, a, b, f(a, b), f(a + 1, b), f(a - 1, b), f(a, b + 1), f(a, b - 1));
-    };
-    double L0 = lapf(i, j);
-    double LE = lapf((i + 1 == NXM) ? 0 : i + 1, j);
-    double LW = lapf((i == 0) ? NXM - 1 : i - 1, j);
-    double LN = (j + 1 == NYM) ? L0 : lapf(i, j + 1);
-    double LS = (j == 0) ? L0 : lapf(i, j - 1);
-    return lap_vals(g, i, j, L0, LE, LW, LN, LS);
-}
-
 // Explicit CH right-hand side and CG start (the body of k_ch_rhs_init) in
 // band form: rc -> rn, r = p = b - A C; partials ||b||^2, ||r||^2.
 template <class R>
@@ -108,16 +92,27 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int
 }
 
 // K1: stop test on ||r_n|| (VOF_DIFFUSE.c:403-414), beta, p_n = r + beta p_{n-1}
-// (:416-422), Ap = a_dt p + (Cn^2/Pe) L^2 p (:318-337), partial <p, Ap>.
+// (:416-422), Ap = a_dt p + (Cn^2/Pe) L^2 p (:318-337), per-row partials of
+// <p, Ap>.  A block owns CHT_Y
+// rows and walks them in chunks of CHT_X columns: p on the chunk with a
+// two-cell halo (CH ghost maps) and the inner Laplacian with a one-cell halo
+// are formed once in LDS, so each p value is formed once per chunk instead of
+// once per stencil use; every value is formed by the arithmetic of bilap_at
+// (inner Laplacian with the CH ghost rules, then the outer one).
+#define CHT_X 64
+#define CHT_Y 4
+__device__ __forceinline__ int pmod(int i, int n) { return ((i % n) + n) % n; }
+
 __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_ccg w,
                                                   const double *__restrict__ r, const double *__restrict__ pold,
                                                   double *__restrict__ pnew, double *__restrict__ Ap, int n,
                                                   int maxit, double tol)
 {
     __shared__ double sh[NTH];
+    __shared__ double sp[CHT_Y + 4][CHT_X + 4];
+    __shared__ double sl[CHT_Y + 2][CHT_X + 2];
     if (sc->ch_done) return;
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
-    const int band = xcd_band(blockIdx.x, gridDim.x);
     const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
     const double rr = 2.0 * block_sum(w.rr[n & 1], w.nb, sh);
     double bnorm;
@@ -146,35 +141,39 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
     if (stop) return;
 
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
-    auto pv = [&](int a, int b) {
-        const int ia = wrapc(a, NXM), jb = chy(b, NYM);
-        return (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
-    };
-    const int j0 = band * w.band, j1 = min(NYM, j0 + w.band);
+    const int j0 = xcd_band(blockIdx.x, gridDim.x) * CHT_Y;
+    const int tr = threadIdx.x >> 6, tc = threadIdx.x & 63, j = j0 + tr;
     double acc = 0.0;
-    for (int j = j0; j < j1; j++)
-    for (int base = 0; base < NXM; base += NTH * CPT) {
-        double pc[CPT], ap[CPT];
-#pragma unroll
-        for (int k = 0; k < CPT; k++) {
-            const int i = base + k * NTH + (int)threadIdx.x;
-            pc[k] = 0.0; ap[k] = 0.0;
-            if (i < NXM) {
-                pc[k] = pv(i, j);
-                ap[k] = a_dt * pc[k] + ph.bih * bilap_fn(g, pv, i, j);
-            }
+    for (int c0 = 0; c0 < NXM; c0 += CHT_X) {
+        for (int q = threadIdx.x; q < (CHT_Y + 4) * (CHT_X + 4); q += NTH) {
+            const int a = q / (CHT_X + 4), b = q - a * (CHT_X + 4);
+            const int ia = pmod(c0 + b - 2, NXM), jb = chy(j0 + a - 2, NYM);
+            sp[a][b] = (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
         }
-#pragma unroll
-        for (int k = 0; k < CPT; k++) {
-            const int i = base + k * NTH + (int)threadIdx.x;
-            if (i < NXM) {
-                AT(pnew, i, j) = pc[k];
-                AT(Ap, i, j) = ap[k];
-                acc += ch_w(g, i, j, pc[k], ap[k]);
-            }
+        __syncthreads();
+        for (int q = threadIdx.x; q < (CHT_Y + 2) * (CHT_X + 2); q += NTH) {
+            const int a = q / (CHT_X + 2), b = q - a * (CHT_X + 2), jb = j0 + a - 1;
+            if (jb >= 0 && jb < NYM)
+                sl[a][b] = lap_vals(g, pmod(c0 + b - 1, NXM), jb, sp[a + 1][b + 1], sp[a + 1][b + 2], sp[a + 1][b],
+                                    sp[a + 2][b + 1], sp[a][b + 1]);
         }
+        __syncthreads();
+        const int i = c0 + tc;
+        if (i < NXM && j < NYM) {
+            const double L0 = sl[tr + 1][tc + 1];
+            const double LN = (j + 1 == NYM) ? L0 : sl[tr + 2][tc + 1];
+            const double LS = (j == 0) ? L0 : sl[tr][tc + 1];
+            const double pc = sp[tr + 2][tc + 2];
+            const double ap = a_dt * pc + ph.bih * lap_vals(g, i, j, L0, sl[tr + 1][tc + 2], sl[tr + 1][tc], LN, LS);
+            AT(pnew, i, j) = pc;
+            AT(Ap, i, j) = ap;
+            acc += ch_w(g, i, j, pc, ap);
+        }
+        __syncthreads();
     }
-    block_store_partial(acc, w.pap, band, sh);
+    // one wavefront per row: the row's partial
+    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
+    if (tc == 0 && j < NYM) w.pap[j] = acc;
 }
 
 // K2: alpha = <r,r> / <p,Ap> (VOF_DIFFUSE.c:384-390); x += alpha p;
@@ -223,7 +222,8 @@ extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, p
                              int half, int n, double *x, double *r, double *Ap, hipStream_t s)
 {
     if (half == 0)
-        hipLaunchKernelGGL(k_ccg_apply, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, (const double *)r,
+        hipLaunchKernelGGL(k_ccg_apply, dim3((g->NYM + CHT_Y - 1) / CHT_Y), dim3(NTH), 0, s, *g, *ph, stage, sc, *w,
+                           (const double *)r,
                            (const double *)w->p[(n + 1) & 1], w->p[n & 1], Ap, n, ph->ch_iter_max, ph->ch_tol);
     else
         hipLaunchKernelGGL(k_ccg_update, dim3(w->nb), dim3(NTH), 0, s, *g, sc, *w, x, (const double *)w->p[n & 1], r,
