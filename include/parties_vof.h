@@ -293,6 +293,10 @@ int pv_set_pressure_solver(pv_bag *bag, int solver);
  * correction of the same RK3 stage of the previous step, kept only when the
  * energy |<r, V r>| of its residual is below the cold start's. */
 int pv_set_pressure_warm_start(pv_bag *bag, int mode);
+/* Arithmetic of the multigrid V-cycle inside the FP64 CG: 0 = FP64 (default
+ * for PV_PREC_FP64 handles), 1 = FP32 (default for PV_PREC_MIXED handles).
+ * The CG vectors, dots, stop test and solution stay FP64 either way. */
+int pv_set_mg_precision(pv_bag *bag, int fp32);
 int pv_set_mg_sweeps(pv_bag *bag, int nu);
 int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
                            int max_samples, unsigned long long seed);

@@ -92,16 +92,21 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int
 }
 
 // K1: stop test on ||r_n|| (VOF_DIFFUSE.c:403-414), beta, p_n = r + beta p_{n-1}
-// (:416-422), Ap = a_dt p + (Cn^2/Pe) L^2 p (:318-337), per-row partials of
-// <p, Ap>.  A block owns CHT_Y
-// rows and walks them in chunks of CHT_X columns: p on the chunk with a
-// two-cell halo (CH ghost maps) and the inner Laplacian with a one-cell halo
-// are formed once in LDS, so each p value is formed once per chunk instead of
-// once per stencil use; every value is formed by the arithmetic of bilap_at
+// (:416-422), Ap = a_dt p + (Cn^2/Pe) L^2 p (:318-337), partials of
+// <p, Ap> (one per block).  A block owns a tile of
+// CHT_X x CHT_Y cells: p on the tile with a two-cell halo (CH ghost maps) and
+// the inner Laplacian with a one-cell halo are formed once in LDS, so each p
+// value is formed once per tile instead of once per stencil use; every value is formed by the arithmetic of bilap_at
 // (inner Laplacian with the CH ghost rules, then the outer one).
-#define CHT_X 64
-#define CHT_Y 4
-__device__ __forceinline__ int pmod(int i, int n) { return ((i % n) + n) % n; }
+#define CHT_X 128
+#define CHT_Y 8
+// column index of the x-periodic CH map for i in [-NXM, 3 NXM)
+__device__ __forceinline__ int chx(int i, int n)
+{
+    while (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
+}
 
 __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_ccg w,
                                                   const double *__restrict__ r, const double *__restrict__ pold,
@@ -113,7 +118,7 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
     __shared__ double sl[CHT_Y + 2][CHT_X + 2];
     if (sc->ch_done) return;
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
-    const bool lead = (blockIdx.x == 0 && threadIdx.x == 0);
+    const bool lead = (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0);
     const double rr = 2.0 * block_sum(w.rr[n & 1], w.nb, sh);
     double bnorm;
     if (n == 0) {
@@ -141,24 +146,23 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
     if (stop) return;
 
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
-    const int j0 = xcd_band(blockIdx.x, gridDim.x) * CHT_Y;
-    const int tr = threadIdx.x >> 6, tc = threadIdx.x & 63, j = j0 + tr;
+    const int c0 = blockIdx.x * CHT_X, j0 = blockIdx.y * CHT_Y;
+    for (int q = threadIdx.x; q < (CHT_Y + 4) * (CHT_X + 4); q += NTH) {
+        const int a = q / (CHT_X + 4), b = q - a * (CHT_X + 4);
+        const int ia = chx(c0 + b - 2, NXM), jb = chy(j0 + a - 2, NYM);
+        sp[a][b] = (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < (CHT_Y + 2) * (CHT_X + 2); q += NTH) {
+        const int a = q / (CHT_X + 2), b = q - a * (CHT_X + 2), jb = j0 + a - 1;
+        if (jb >= 0 && jb < NYM)
+            sl[a][b] = lap_vals(g, chx(c0 + b - 1, NXM), jb, sp[a + 1][b + 1], sp[a + 1][b + 2], sp[a + 1][b],
+                                sp[a + 2][b + 1], sp[a][b + 1]);
+    }
+    __syncthreads();
     double acc = 0.0;
-    for (int c0 = 0; c0 < NXM; c0 += CHT_X) {
-        for (int q = threadIdx.x; q < (CHT_Y + 4) * (CHT_X + 4); q += NTH) {
-            const int a = q / (CHT_X + 4), b = q - a * (CHT_X + 4);
-            const int ia = pmod(c0 + b - 2, NXM), jb = chy(j0 + a - 2, NYM);
-            sp[a][b] = (n == 0) ? AT(r, ia, jb) : AT(r, ia, jb) + beta * AT(pold, ia, jb);
-        }
-        __syncthreads();
-        for (int q = threadIdx.x; q < (CHT_Y + 2) * (CHT_X + 2); q += NTH) {
-            const int a = q / (CHT_X + 2), b = q - a * (CHT_X + 2), jb = j0 + a - 1;
-            if (jb >= 0 && jb < NYM)
-                sl[a][b] = lap_vals(g, pmod(c0 + b - 1, NXM), jb, sp[a + 1][b + 1], sp[a + 1][b + 2], sp[a + 1][b],
-                                    sp[a + 2][b + 1], sp[a][b + 1]);
-        }
-        __syncthreads();
-        const int i = c0 + tc;
+    for (int q = threadIdx.x; q < CHT_X * CHT_Y; q += NTH) {
+        const int tr = q / CHT_X, tc = q - tr * CHT_X, i = c0 + tc, j = j0 + tr;
         if (i < NXM && j < NYM) {
             const double L0 = sl[tr + 1][tc + 1];
             const double LN = (j + 1 == NYM) ? L0 : sl[tr + 2][tc + 1];
@@ -169,11 +173,8 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
             AT(Ap, i, j) = ap;
             acc += ch_w(g, i, j, pc, ap);
         }
-        __syncthreads();
     }
-    // one wavefront per row: the row's partial
-    for (int off = 32; off > 0; off >>= 1) acc += __shfl_down(acc, off, 64);
-    if (tc == 0 && j < NYM) w.pap[j] = acc;
+    block_store_partial(acc, w.pap, blockIdx.y * gridDim.x + blockIdx.x, sh);
 }
 
 // K2: alpha = <r,r> / <p,Ap> (VOF_DIFFUSE.c:384-390); x += alpha p;
@@ -186,7 +187,7 @@ __global__ void __launch_bounds__(NTH) k_ccg_update(pvk_grid g, pvk_scal *sc, pv
     if (sc->ch_done) return;
     const int P = g.P, NXM = g.NXM;
     const int band = xcd_band(blockIdx.x, gridDim.x);
-    const double pAp = 2.0 * block_sum(w.pap, w.nb, sh);
+    const double pAp = 2.0 * block_sum(w.pap, w.npap, sh);
     if (fabs(pAp) < 1e-30) {
         if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 2; sc->ch_iter = n; }
         return;
@@ -222,8 +223,8 @@ extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, p
                              int half, int n, double *x, double *r, double *Ap, hipStream_t s)
 {
     if (half == 0)
-        hipLaunchKernelGGL(k_ccg_apply, dim3((g->NYM + CHT_Y - 1) / CHT_Y), dim3(NTH), 0, s, *g, *ph, stage, sc, *w,
-                           (const double *)r,
+        hipLaunchKernelGGL(k_ccg_apply, dim3((g->NXM + CHT_X - 1) / CHT_X, (g->NYM + CHT_Y - 1) / CHT_Y), dim3(NTH),
+                           0, s, *g, *ph, stage, sc, *w, (const double *)r,
                            (const double *)w->p[(n + 1) & 1], w->p[n & 1], Ap, n, ph->ch_iter_max, ph->ch_tol);
     else
         hipLaunchKernelGGL(k_ccg_update, dim3(w->nb), dim3(NTH), 0, s, *g, sc, *w, x, (const double *)w->p[n & 1], r,

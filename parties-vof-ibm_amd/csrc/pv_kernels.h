@@ -92,6 +92,7 @@ typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
     double *p[2];                    /* search direction, double-buffered */
     double *bb, *rr[2], *pap;        /* partials: ||b||^2, ||r||^2 (by parity), <p, Ap> */
     int nb, band;                    /* bands (= blocks) and rows per band */
+    int npap;                        /* <p, Ap> partials: one per 128 x 8 tile */
 } pvk_ccg;
 
 typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
@@ -226,31 +227,38 @@ void pvk_pcg_prep(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, hipStream_t
 void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
 
 /* ---- multigrid-preconditioned CG (csrc/pv_mg.hip) ----
- * Level 0 is the pressure grid (x = z, f = r, faces = the PCG faces, pitch
- * P); levels 1.. are compact (pitch nx).  Levels c0.. (<= PVK_MG_LDS_CELLS
- * cells) run inside one workgroup; the last level (halving stops at
- * <= PVK_MG_BOTTOM_MIN cells or odd sizes; <= PVK_MG_BOTTOM cells) is solved
- * with the grounded inverse G. */
+ * Level 0 is the pressure grid; levels 1.. are compact (pitch nx).  The
+ * V-cycle's arrays are of its type (FP64, or FP32 with fp32 = 1).  FP64:
+ * level 0 x = z, f = r, faces = the PCG faces (pitch P).  FP32: level 0 x and
+ * faces are FP32 copies (pitch nx), r is read from r0 and z written to z0
+ * (pitch p0).  Levels c0.. (<= PVK_MG_LDS_CELLS cells) run inside one
+ * workgroup; the last level (halving stops at <= PVK_MG_BOTTOM_MIN cells or
+ * odd sizes; <= PVK_MG_BOTTOM cells) is solved with the grounded inverse G. */
 #define PVK_MG_MAXLEV 16
-#define PVK_MG_LDS_CELLS 1024
+#define PVK_MG_LDS_CELLS 4096
 #define PVK_MG_BOTTOM 64
 #define PVK_MG_BOTTOM_MIN 64
 typedef struct pvk_mglev {
     int nx, ny, pitch;
-    double *x, *f, *bx, *by;
+    void *x, *f, *bx, *by;
 } pvk_mglev;
 typedef struct pvk_mg {
-    int nlev, c0, nu;                 /* levels, first in-LDS level, sweeps per side */
+    int nlev, c0, nu, fp32;           /* levels, first in-LDS level, sweeps per side, FP32 V-cycle */
+    int topf;                         /* faces of level c0 copied to LDS too (when they fit) */
     pvk_mglev L[PVK_MG_MAXLEV];
+    const double *r0;                 /* FP32: level-0 right-hand side (r), pitch p0 */
+    double *z0;                       /* FP32: level-0 result (z), pitch p0 */
+    int p0;
     double *G;                        /* bottom inverse, nb x nb */
-    int ldsx;                         /* LDS offsets: doubles per in-LDS array set */
 } pvk_mg;
-void pvk_mg_setup(const pvk_mg *m, hipStream_t s);
+void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);
+int  pvk_mg_nrz(const pvk_mg *m);
+size_t pvk_mg_lds_bytes(const pvk_mg *m);
 void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                 hipStream_t s);
-void pvk_mg_coarse(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s);
+void pvk_mg_coarse(const pvk_mg *m, const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, double tol,
+                   int maxit, hipStream_t s);
 void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
-int  pvk_mg_nrz(const pvk_mg *m);
 void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s);
 void pvk_mg_update(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
 /* out = A x with the pressure operator of the fused PCG (faces from its start) */

@@ -14,32 +14,50 @@
 //   restriction   sum of the four fine residuals;
 //   prolongation  piecewise constant;
 //   coarse faces  half the sum of the two fine faces across the coarse face;
-//   levels of at most PVK_MG_LDS_CELLS cells run in one workgroup from LDS;
 //   the last level (<= PVK_MG_BOTTOM cells) is solved with the inverse of its
 //   operator grounded at cell 0, both sides projected to zero mean.
-// Each level above PVK_MG_LDS_CELLS cells is two launches: pre-smoothing +
-// restriction, and correction + post-smoothing, each on LDS tiles with a
-// halo (so f, the faces and x cross HBM once per launch).
+// Each level above PVK_MG_LDS_CELLS cells is two launches (pre-smoothing +
+// restriction; correction + post-smoothing) on LDS tiles with a halo, so f,
+// the faces and x cross HBM once per launch; the levels below run in one
+// workgroup from LDS.  The V-cycle stores and computes in T = double, or in
+// T = float inside the FP64 CG (the mixed form: level 0 then reads r and
+// writes z in FP64, its own x and faces are FP32).
 #include <hip/hip_runtime.h>
 #include <math.h>
 #include "pv_kernels.h"
 #include "pv_block.h"
 
 #define AT(a, i, j) (a)[(ptrdiff_t)(j) * P + (i)]
-#define TXP 64      // cell pairs per block row
+#define TXP 64      // cells per block row of the per-cell kernels
 #define TYR 4       // rows per block; TXP * TYR = NTH
 
-__device__ __forceinline__ double ld(const double *a, int pitch, int i, int j) { return a[(ptrdiff_t)j * pitch + i]; }
-
-// Gauss-Seidel value of cell (i, j): (sum_k c_k x_k - f) / sum_k c_k
-__device__ __forceinline__ double gs_value(const pvk_mglev &L, int i, int j, double xe, double xw, double xn,
-                                           double xs, double f)
+// a level with arrays of the V-cycle's type
+template <typename T> struct Lev {
+    int nx, ny, pitch;
+    T *x, *f, *bx, *by;
+};
+template <typename T> __host__ __device__ static Lev<T> lev(const pvk_mglev &l)
 {
-    const int p = L.pitch, iw = (i == 0) ? L.nx - 1 : i - 1;
-    const double ce = ld(L.bx, p, i, j), cw = ld(L.bx, p, iw, j);
-    const double cn = (j + 1 < L.ny) ? ld(L.by, p, i, j) : 0.0;
-    const double cs = (j > 0) ? ld(L.by, p, i, j - 1) : 0.0;
-    return (ce * xe + cw * xw + cn * xn + cs * xs - f) / (ce + cw + cn + cs);
+    Lev<T> v;
+    v.nx = l.nx;
+    v.ny = l.ny;
+    v.pitch = l.pitch;
+    v.x = (T *)l.x;
+    v.f = (T *)l.f;
+    v.bx = (T *)l.bx;
+    v.by = (T *)l.by;
+    return v;
+}
+
+template <typename T> __device__ __forceinline__ T ld(const T *a, int pitch, int i, int j)
+{
+    return a[(ptrdiff_t)j * pitch + i];
+}
+
+// f of level L at (i, j); level 0 of the FP32 V-cycle reads the FP64 r0
+template <typename T> __device__ __forceinline__ T fget(const Lev<T> &L, const double *r0, int p0, int i, int j)
+{
+    return r0 ? (T)r0[(ptrdiff_t)j * p0 + i] : ld(L.f, L.pitch, i, j);
 }
 
 // stop test of CG iteration n on the <r, r> partials (as k_transpose):
@@ -61,12 +79,12 @@ __device__ void mg_stop_test(pvk_scal *sc, const pvk_pcg &w, int n, double tol, 
 // ---------------------------------------------------------------------------
 // Fused smoothing of one level (levels with more than PVK_MG_LDS_CELLS cells).
 // A block owns a core tile of TX x MG_TY cells (TX = 64, or 32 on levels at
-// most 256 wide, for more blocks) and works on it extended by
-// a halo of 2 nu cells (periodic in x, cut at the walls in y): half-sweep k
-// updates the cells of its colour at distance >= k from the extended edge,
-// so after 2 nu half-sweeps the core (and the ring the residual needs) holds
-// exactly what sweeping the whole level would give.  The face coefficients
-// and f stay in registers, x in LDS.
+// most 256 wide, for more blocks) and works on it extended by a halo of 2 nu
+// cells (periodic in x, cut at the walls in y): half-sweep k updates the
+// cells of its colour at distance >= k from the extended edge, so after
+// 2 nu half-sweeps the core (and the ring the residual needs) holds exactly
+// what sweeping the whole level would give.  f, the face coefficients and
+// 1 / (their sum) stay in registers, x in LDS.
 // ---------------------------------------------------------------------------
 #define MG_TY 16
 
@@ -78,75 +96,72 @@ template <int NU, int TX> struct MgTile {
     static constexpr int K = (NC + NTH - 1) / NTH;    // cells per thread
 };
 
-// f, the four face coefficients and 1 / (sum of them) of the extended-tile
-// cells of this thread
-template <int NU, int TX>
-__device__ __forceinline__ void tile_coeffs(const pvk_mglev &L, int x0, int y0, double *f, double *ce, double *cw,
-                                            double *cn, double *cs, double *id)
+template <int NU, int TX, typename T>
+__device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, int x0, int y0, T *f, T *ce,
+                                            T *cw, T *cn, T *cs, T *id)
 {
-    using T = MgTile<NU, TX>;
+    using TT = MgTile<NU, TX>;
     const int p = L.pitch, nx = L.nx, ny = L.ny;
 #pragma unroll
-    for (int u = 0; u < T::K; u++) {
-        const int c = threadIdx.x + u * NTH, ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
-        f[u] = ce[u] = cw[u] = cn[u] = cs[u] = id[u] = 0.0;
-        if (c < T::NC && gj >= 0 && gj < ny) {
+    for (int u = 0; u < TT::K; u++) {
+        const int c = threadIdx.x + u * NTH, ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
+        f[u] = ce[u] = cw[u] = cn[u] = cs[u] = id[u] = T(0);
+        if (c < TT::NC && gj >= 0 && gj < ny) {
             int gi = (x0 + lx) % nx;
             if (gi < 0) gi += nx;
             const int giw = (gi == 0) ? nx - 1 : gi - 1;
-            f[u] = ld(L.f, p, gi, gj);
+            f[u] = fget(L, r0, p0, gi, gj);
             ce[u] = ld(L.bx, p, gi, gj);
             cw[u] = ld(L.bx, p, giw, gj);
-            cn[u] = (gj + 1 < ny) ? ld(L.by, p, gi, gj) : 0.0;
-            cs[u] = (gj > 0) ? ld(L.by, p, gi, gj - 1) : 0.0;
-            id[u] = 1.0 / (ce[u] + cw[u] + cn[u] + cs[u]);
+            cn[u] = (gj + 1 < ny) ? ld(L.by, p, gi, gj) : T(0);
+            cs[u] = (gj > 0) ? ld(L.by, p, gi, gj - 1) : T(0);
+            id[u] = T(1) / (ce[u] + cw[u] + cn[u] + cs[u]);
         }
     }
 }
 
 // half-sweep k of colour col on the cells at distance >= k from the edge
-template <int NU, int TX>
-__device__ __forceinline__ void tile_sweep(double *xs, int x0, int y0, int ny, int k, int col, const double *f,
-                                           const double *ce, const double *cw, const double *cn, const double *cs,
-                                           const double *id)
+template <int NU, int TX, typename T>
+__device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k, int col, const T *f, const T *ce,
+                                           const T *cw, const T *cn, const T *cs, const T *id)
 {
-    using T = MgTile<NU, TX>;
-    double v[T::K];
-    bool act[T::K];
+    using TT = MgTile<NU, TX>;
+    T v[TT::K];
+    bool act[TT::K];
 #pragma unroll
-    for (int u = 0; u < T::K; u++) {
-        const int c = threadIdx.x + u * NTH, ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
-        act[u] = c < T::NC && gj >= 0 && gj < ny && (((x0 + lx + gj) & 1) == col) && lx >= k && lx < T::W - k &&
-                 ly >= k && ly < T::HT - k;
-        v[u] = 0.0;
+    for (int u = 0; u < TT::K; u++) {
+        const int c = threadIdx.x + u * NTH, ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
+        act[u] = c < TT::NC && gj >= 0 && gj < ny && (((x0 + lx + gj) & 1) == col) && lx >= k && lx < TT::W - k &&
+                 ly >= k && ly < TT::HT - k;
+        v[u] = T(0);
         if (act[u]) {
-            const double xe = (k > 0) ? xs[c + 1] : 0.0, xw = (k > 0) ? xs[c - 1] : 0.0;
-            const double xn = (k > 0) ? xs[c + T::W] : 0.0, xq = (k > 0) ? xs[c - T::W] : 0.0;
+            const T xe = (k > 0) ? xs[c + 1] : T(0), xw = (k > 0) ? xs[c - 1] : T(0);
+            const T xn = (k > 0) ? xs[c + TT::W] : T(0), xq = (k > 0) ? xs[c - TT::W] : T(0);
             v[u] = (ce[u] * xe + cw[u] * xw + cn[u] * xn + cs[u] * xq - f[u]) * id[u];
         }
     }
     // a colour's cells read only the other colour: no barrier before writing
 #pragma unroll
-    for (int u = 0; u < T::K; u++)
+    for (int u = 0; u < TT::K; u++)
         if (act[u]) xs[threadIdx.x + u * NTH] = v[u];
     __syncthreads();
 }
 
 // pre-smoothing from x = 0 and restriction of the residual; the stop test of
 // iteration n in block (0, 0) of level 0 (check)
-template <int NU, int TX>
-__global__ void __launch_bounds__(NTH) k_mg_pre(pvk_mglev L, pvk_mglev C, pvk_scal *sc, pvk_pcg w, int n,
-                                               double tol, int maxit, int check)
+template <int NU, int TX, typename T>
+__global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, int p0, pvk_scal *sc,
+                                               pvk_pcg w, int n, double tol, int maxit, int check)
 {
-    using T = MgTile<NU, TX>;
-    __shared__ double xs[T::NC];
+    using TT = MgTile<NU, TX>;
+    __shared__ T xs[TT::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
-    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * MG_TY - T::H;
-    double f[T::K], ce[T::K], cw[T::K], cn[T::K], cs[T::K], id[T::K];
-    tile_coeffs<NU, TX>(L, x0, y0, f, ce, cw, cn, cs, id);
-    for (int c = threadIdx.x; c < T::NC; c += NTH) xs[c] = 0.0;
+    const int x0 = blockIdx.x * TX - TT::H, y0 = blockIdx.y * MG_TY - TT::H;
+    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
+    tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
     __syncthreads();
     for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, TX>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
     // restriction: one coarse cell per thread (TX/2 x MG_TY/2 <= NTH)
@@ -155,18 +170,18 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(pvk_mglev L, pvk_mglev C, pvk_sc
         const int gI = blockIdx.x * (TX / 2) + I, gJ = blockIdx.y * (MG_TY / 2) + J;
         if (gI < C.nx && gJ < C.ny) {
             const int p = L.pitch, nx = L.nx, ny = L.ny;
-            double s = 0.0;
+            T s = T(0);
 #pragma unroll
             for (int dj = 0; dj < 2; dj++)
 #pragma unroll
                 for (int di = 0; di < 2; di++) {
                     const int gi = 2 * gI + di, gj = 2 * gJ + dj, giw = (gi == 0) ? nx - 1 : gi - 1;
-                    const int c = (T::H + 2 * J + dj) * T::W + T::H + 2 * I + di;
-                    const double x0c = xs[c];
-                    double ax = ld(L.bx, p, gi, gj) * (xs[c + 1] - x0c) - ld(L.bx, p, giw, gj) * (x0c - xs[c - 1]);
-                    if (gj + 1 < ny) ax += ld(L.by, p, gi, gj) * (xs[c + T::W] - x0c);
-                    if (gj > 0) ax -= ld(L.by, p, gi, gj - 1) * (x0c - xs[c - T::W]);
-                    s += ld(L.f, p, gi, gj) - ax;
+                    const int c = (TT::H + 2 * J + dj) * TT::W + TT::H + 2 * I + di;
+                    const T x0c = xs[c];
+                    T ax = ld(L.bx, p, gi, gj) * (xs[c + 1] - x0c) - ld(L.bx, p, giw, gj) * (x0c - xs[c - 1]);
+                    if (gj + 1 < ny) ax += ld(L.by, p, gi, gj) * (xs[c + TT::W] - x0c);
+                    if (gj > 0) ax -= ld(L.by, p, gi, gj - 1) * (x0c - xs[c - TT::W]);
+                    s += fget(L, r0, p0, gi, gj) - ax;
                 }
             C.f[(ptrdiff_t)gJ * C.pitch + gI] = s;
         }
@@ -175,26 +190,27 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(pvk_mglev L, pvk_mglev C, pvk_sc
     for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
         const int ly = c / TX, lx = c - ly * TX;
         const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
-        if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + T::H) * T::W + lx + T::H];
+        if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
     }
 }
 
-// coarse correction x += xc(parent) and post-smoothing (black first);
-// dot: partial <f, x> of the core into w.rz[slot]
-template <int NU, int TX>
-__global__ void __launch_bounds__(NTH) k_mg_post(pvk_mglev L, pvk_mglev C, const pvk_scal *sc, pvk_pcg w, int dot,
-                                                int slot)
+// coarse correction x += xc(parent) and post-smoothing (black first); the
+// result goes to x, or to the FP64 z0 on level 0 of the FP32 V-cycle.
+// dot: partial <f, x> of the core (FP64, with r0 on that level) into w.rz[slot]
+template <int NU, int TX, typename T>
+__global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const double *r0, double *z0, int p0,
+                                                const pvk_scal *sc, pvk_pcg w, int dot, int slot)
 {
-    using T = MgTile<NU, TX>;
-    __shared__ double xs[T::NC];
+    using TT = MgTile<NU, TX>;
+    __shared__ T xs[TT::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int x0 = blockIdx.x * TX - T::H, y0 = blockIdx.y * MG_TY - T::H;
-    double f[T::K], ce[T::K], cw[T::K], cn[T::K], cs[T::K], id[T::K];
-    tile_coeffs<NU, TX>(L, x0, y0, f, ce, cw, cn, cs, id);
-    for (int c = threadIdx.x; c < T::NC; c += NTH) {
-        const int ly = c / T::W, lx = c - ly * T::W, gj = y0 + ly;
-        double v = 0.0;
+    const int x0 = blockIdx.x * TX - TT::H, y0 = blockIdx.y * MG_TY - TT::H;
+    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
+    tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    for (int c = threadIdx.x; c < TT::NC; c += NTH) {
+        const int ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
+        T v = T(0);
         if (gj >= 0 && gj < L.ny) {
             int gi = (x0 + lx) % L.nx;
             if (gi < 0) gi += L.nx;
@@ -209,29 +225,43 @@ __global__ void __launch_bounds__(NTH) k_mg_post(pvk_mglev L, pvk_mglev C, const
         const int ly = c / TX, lx = c - ly * TX;
         const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
         if (gi < L.nx && gj < L.ny) {
-            const double v = xs[(ly + T::H) * T::W + lx + T::H];
-            L.x[(ptrdiff_t)gj * L.pitch + gi] = v;
-            if (dot) acc += ld(L.f, L.pitch, gi, gj) * v;
+            const T v = xs[(ly + TT::H) * TT::W + lx + TT::H];
+            if (z0) z0[(ptrdiff_t)gj * p0 + gi] = (double)v;
+            else L.x[(ptrdiff_t)gj * L.pitch + gi] = v;
+            if (dot) acc += (r0 ? r0[(ptrdiff_t)gj * p0 + gi] : (double)ld(L.f, L.pitch, gi, gj)) * (double)v;
         }
     }
     if (dot) block_store_partial(acc, w.rz[slot], blockIdx.y * gridDim.x + blockIdx.x, sh);
 }
 
-// coarse face coefficients from the fine ones
-__global__ void __launch_bounds__(NTH) k_mg_faces(pvk_mglev L, pvk_mglev C)
+// coarse face coefficients from the fine ones (the fine level may be the
+// FP64 level 0 of the FP32 V-cycle)
+template <typename TF, typename T> __global__ void __launch_bounds__(NTH) k_mg_faces(Lev<TF> L, Lev<T> C)
 {
     const int I = blockIdx.x * TXP + (threadIdx.x & (TXP - 1)), J = blockIdx.y * TYR + (threadIdx.x >> 6);
     if (I >= C.nx || J >= C.ny) return;
     const int p = L.pitch;
-    C.bx[(ptrdiff_t)J * C.pitch + I] = 0.5 * (ld(L.bx, p, 2 * I + 1, 2 * J) + ld(L.bx, p, 2 * I + 1, 2 * J + 1));
-    C.by[(ptrdiff_t)J * C.pitch + I] = 0.5 * (ld(L.by, p, 2 * I, 2 * J + 1) + ld(L.by, p, 2 * I + 1, 2 * J + 1));
+    C.bx[(ptrdiff_t)J * C.pitch + I] = (T)(0.5 * ((double)ld(L.bx, p, 2 * I + 1, 2 * J) +
+                                                  (double)ld(L.bx, p, 2 * I + 1, 2 * J + 1)));
+    C.by[(ptrdiff_t)J * C.pitch + I] = (T)(0.5 * ((double)ld(L.by, p, 2 * I, 2 * J + 1) +
+                                                  (double)ld(L.by, p, 2 * I + 1, 2 * J + 1)));
 }
 
-// inverse of the bottom operator -A_B + s e0 e0^T (s = its (0,0) entry):
-// symmetric positive definite, in-place Gauss-Jordan without pivoting; one
-// workgroup of 1024 threads, thread (r, q) owns row r, columns q + 16 t
+// FP32 copies of the level-0 faces
+__global__ void __launch_bounds__(NTH) k_mg_faces0(Lev<double> L, Lev<float> C)
+{
+    const int I = blockIdx.x * TXP + (threadIdx.x & (TXP - 1)), J = blockIdx.y * TYR + (threadIdx.x >> 6);
+    if (I >= C.nx || J >= C.ny) return;
+    C.bx[(ptrdiff_t)J * C.pitch + I] = (float)ld(L.bx, L.pitch, I, J);
+    C.by[(ptrdiff_t)J * C.pitch + I] = (float)ld(L.by, L.pitch, I, J);
+}
+
+// inverse of the bottom operator -A_B + s e0 e0^T (s = its (0,0) entry),
+// in FP64: symmetric positive definite, in-place Gauss-Jordan without
+// pivoting; one workgroup of 1024 threads, thread (r, q) owns row r, columns
+// q + 16 t
 #define BT 1024
-__global__ void __launch_bounds__(BT) k_mg_bottom(pvk_mglev B, double *G)
+template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B, double *G)
 {
     __shared__ double M[PVK_MG_BOTTOM][PVK_MG_BOTTOM + 1];
     const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch;
@@ -242,7 +272,8 @@ __global__ void __launch_bounds__(BT) k_mg_bottom(pvk_mglev B, double *G)
         const int j = r / nx, i = r - j * nx;
         const int ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
         const double ce = ld(B.bx, p, i, j), cw = ld(B.bx, p, iw, j);
-        const double cn = (j + 1 < ny) ? ld(B.by, p, i, j) : 0.0, cs = (j > 0) ? ld(B.by, p, i, j - 1) : 0.0;
+        const double cn = (j + 1 < ny) ? (double)ld(B.by, p, i, j) : 0.0;
+        const double cs = (j > 0) ? (double)ld(B.by, p, i, j - 1) : 0.0;
         M[r][r] += ce + cw + cn + cs;
         M[r][j * nx + ie] -= ce;
         M[r][j * nx + iw] -= cw;
@@ -277,39 +308,43 @@ __global__ void __launch_bounds__(BT) k_mg_bottom(pvk_mglev B, double *G)
 }
 
 // ---------------------------------------------------------------------------
-// Levels c0 .. nlev-1 in one workgroup of 1024 threads, from LDS.
+// Levels c0 .. nlev-1 in one workgroup of 1024 threads.  x and f of every
+// level live in LDS, and the faces of every level below c0; those of level
+// c0 are copied too when they fit (topf), else read from its arrays.
 // ---------------------------------------------------------------------------
 #define CT 1024
 
-struct LdsLev {
-    int nx, ny;
-    double *x, *f, *bx, *by;     // compact (pitch nx), in LDS
+template <typename T> struct LdsLev {
+    int nx, ny, fp;         // cells; pitch of bx, by
+    T *x, *f;               // LDS, compact
+    const T *bx, *by;       // LDS (pitch nx) or the level's arrays
 };
 
-__device__ __forceinline__ double lgs(const LdsLev &L, int i, int j, double xe, double xw, double xn, double xs)
+template <typename T>
+__device__ __forceinline__ T lgs(const LdsLev<T> &L, int i, int j, T xe, T xw, T xn, T xs)
 {
-    const int nx = L.nx, iw = (i == 0) ? nx - 1 : i - 1;
-    const double ce = L.bx[j * nx + i], cw = L.bx[j * nx + iw];
-    const double cn = (j + 1 < L.ny) ? L.by[j * nx + i] : 0.0;
-    const double cs = (j > 0) ? L.by[(j - 1) * nx + i] : 0.0;
+    const int nx = L.nx, iw = (i == 0) ? nx - 1 : i - 1, fp = L.fp;
+    const T ce = L.bx[j * fp + i], cw = L.bx[j * fp + iw];
+    const T cn = (j + 1 < L.ny) ? L.by[j * fp + i] : T(0);
+    const T cs = (j > 0) ? L.by[(j - 1) * fp + i] : T(0);
     return (ce * xe + cw * xw + cn * xn + cs * xs - L.f[j * nx + i]) / (ce + cw + cn + cs);
 }
 
 // half-sweep of colour col; mode 0: from x = 0, 1: Gauss-Seidel, 2: first
 // post-sweep with the red neighbours read as x + xc(parent)
-__device__ void lds_sweep(const LdsLev &L, const LdsLev &C, int col, int mode)
+template <typename T> __device__ void lds_sweep(const LdsLev<T> &L, const LdsLev<T> &C, int col, int mode)
 {
     const int hx = L.nx >> 1, npair = hx * L.ny, nx = L.nx;
     for (int q = threadIdx.x; q < npair; q += CT) {
         const int j = q / hx, i = 2 * (q - j * hx) + ((j + col) & 1);
-        double v;
+        T v;
         if (mode == 0) {
-            v = lgs(L, i, j, 0.0, 0.0, 0.0, 0.0);
+            v = lgs(L, i, j, T(0), T(0), T(0), T(0));
         } else {
             const int ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-            double xe = L.x[j * nx + ie], xw = L.x[j * nx + iw];
-            double xn = (j + 1 < L.ny) ? L.x[(j + 1) * nx + i] : 0.0;
-            double xs = (j > 0) ? L.x[(j - 1) * nx + i] : 0.0;
+            T xe = L.x[j * nx + ie], xw = L.x[j * nx + iw];
+            T xn = (j + 1 < L.ny) ? L.x[(j + 1) * nx + i] : T(0);
+            T xs = (j > 0) ? L.x[(j - 1) * nx + i] : T(0);
             if (mode == 2) {
                 const int cx = C.nx;
                 xe += C.x[(j >> 1) * cx + (ie >> 1)];
@@ -324,21 +359,21 @@ __device__ void lds_sweep(const LdsLev &L, const LdsLev &C, int col, int mode)
     __syncthreads();
 }
 
-__device__ __forceinline__ double lds_ax(const LdsLev &L, int i, int j)
+template <typename T> __device__ __forceinline__ T lds_ax(const LdsLev<T> &L, int i, int j)
 {
-    const int nx = L.nx, ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-    const double x0 = L.x[j * nx + i];
-    double v = L.bx[j * nx + i] * (L.x[j * nx + ie] - x0) - L.bx[j * nx + iw] * (x0 - L.x[j * nx + iw]);
-    if (j + 1 < L.ny) v += L.by[j * nx + i] * (L.x[(j + 1) * nx + i] - x0);
-    if (j > 0) v -= L.by[(j - 1) * nx + i] * (x0 - L.x[(j - 1) * nx + i]);
+    const int nx = L.nx, fp = L.fp, ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
+    const T x0 = L.x[j * nx + i];
+    T v = L.bx[j * fp + i] * (L.x[j * nx + ie] - x0) - L.bx[j * fp + iw] * (x0 - L.x[j * nx + iw]);
+    if (j + 1 < L.ny) v += L.by[j * fp + i] * (L.x[(j + 1) * nx + i] - x0);
+    if (j > 0) v -= L.by[(j - 1) * fp + i] * (x0 - L.x[(j - 1) * nx + i]);
     return v;
 }
 
-__device__ void lds_restrict(const LdsLev &L, const LdsLev &C)
+template <typename T> __device__ void lds_restrict(const LdsLev<T> &L, const LdsLev<T> &C)
 {
     for (int q = threadIdx.x; q < C.nx * C.ny; q += CT) {
         const int J = q / C.nx, I = q - J * C.nx;
-        double s = 0.0;
+        T s = T(0);
         for (int dj = 0; dj < 2; dj++)
             for (int di = 0; di < 2; di++) {
                 const int i = 2 * I + di, j = 2 * J + dj;
@@ -349,44 +384,53 @@ __device__ void lds_restrict(const LdsLev &L, const LdsLev &C)
     __syncthreads();
 }
 
-// level l of the in-LDS levels: x, f, bx, by of each level in turn
-__device__ __forceinline__ LdsLev lds_level(const pvk_mg &m, double *lds, int l)
+// in-LDS level l: x, f, bx, by (level c0 without topf: x, f)
+template <typename T> __device__ __forceinline__ LdsLev<T> lds_level(const pvk_mg &m, T *lds, int l)
 {
-    double *q = lds;
-    for (int k = m.c0; k < l; k++) q += 4 * m.L[k].nx * m.L[k].ny;
+    T *q = lds;
+    for (int k = m.c0; k < l; k++) q += ((k == m.c0 && !m.topf) ? 2 : 4) * m.L[k].nx * m.L[k].ny;
     const int nc = m.L[l].nx * m.L[l].ny;
-    LdsLev v;
+    LdsLev<T> v;
     v.nx = m.L[l].nx;
     v.ny = m.L[l].ny;
     v.x = q;
     v.f = q + nc;
-    v.bx = q + 2 * nc;
-    v.by = q + 3 * nc;
+    if (l == m.c0 && !m.topf) {
+        v.fp = m.L[l].pitch;
+        v.bx = (const T *)m.L[l].bx;
+        v.by = (const T *)m.L[l].by;
+    } else {
+        v.fp = v.nx;
+        v.bx = q + 2 * nc;
+        v.by = q + 3 * nc;
+    }
     return v;
 }
 
-__global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
+template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 {
-    extern __shared__ double lds[];
-    __shared__ double red[CT / 64];
+    extern __shared__ double lds_raw[];
     __shared__ double mean;
+    __shared__ double vbuf[PVK_MG_BOTTOM];
+    T *lds = (T *)lds_raw;
     if (sc->p_done) return;
-    // faces of every in-LDS level and the right-hand side of the first
+    // f of level c0, and the faces of the in-LDS levels below it
     for (int l = m.c0; l < m.nlev; l++) {
-        const pvk_mglev &G = m.L[l];
-        const LdsLev Ll = lds_level(m, lds, l);
-        const int hx = G.nx;
+        const Lev<T> G = lev<T>(m.L[l]);
+        const LdsLev<T> Ll = lds_level(m, lds, l);
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
-            const int j = q / hx, i = q - j * hx;
-            Ll.bx[q] = ld(G.bx, G.pitch, i, j);
-            Ll.by[q] = ld(G.by, G.pitch, i, j);
-            if (l == m.c0) Ll.f[q] = ld(G.f, G.pitch, i, j);
+            const int j = q / G.nx, i = q - j * G.nx;
+            if (l == m.c0) Ll.f[q] = (l == 0 && m.r0) ? (T)m.r0[(ptrdiff_t)j * m.p0 + i] : ld(G.f, G.pitch, i, j);
+            if (l != m.c0 || m.topf) {
+                ((T *)Ll.bx)[q] = ld(G.bx, G.pitch, i, j);
+                ((T *)Ll.by)[q] = ld(G.by, G.pitch, i, j);
+            }
         }
     }
     __syncthreads();
     const int nb = m.nlev - 1;
     for (int l = m.c0; l < nb; l++) {
-        const LdsLev Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
         lds_sweep(Ll, Ll, 0, 0);
         lds_sweep(Ll, Ll, 1, 1);
         for (int s = 1; s < m.nu; s++) {
@@ -395,35 +439,34 @@ __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
         }
         lds_restrict(Ll, Lc);
     }
-    // bottom: x = -P G P f, P the projection to zero mean
+    // bottom: x = -P G P f, P the projection to zero mean (in FP64)
     {
-        const LdsLev B = lds_level(m, lds, nb);
+        const LdsLev<T> B = lds_level(m, lds, nb);
         const int n = B.nx * B.ny;
         if (threadIdx.x == 0) {
             double s = 0.0;
-            for (int k = 0; k < n; k++) s += B.f[k];
+            for (int k = 0; k < n; k++) s += (double)B.f[k];
             mean = s / n;
         }
         __syncthreads();
-        double v = 0.0;
         if ((int)threadIdx.x < n) {
             const double *g = m.G + (size_t)threadIdx.x * n;
-            for (int k = 0; k < n; k++) v -= g[k] * (B.f[k] - mean);
+            double v = 0.0;
+            for (int k = 0; k < n; k++) v -= g[k] * ((double)B.f[k] - mean);
+            vbuf[threadIdx.x] = v;
         }
-        __syncthreads();
-        if ((int)threadIdx.x < n) B.x[threadIdx.x] = v;
         __syncthreads();
         if (threadIdx.x == 0) {
             double s = 0.0;
-            for (int k = 0; k < n; k++) s += B.x[k];
+            for (int k = 0; k < n; k++) s += vbuf[k];
             mean = s / n;
         }
         __syncthreads();
-        if ((int)threadIdx.x < n) B.x[threadIdx.x] -= mean;
+        if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
         __syncthreads();
     }
     for (int l = nb - 1; l >= m.c0; l--) {
-        const LdsLev Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
         lds_sweep(Ll, Lc, 1, 2);
         lds_sweep(Ll, Ll, 0, 1);
         for (int s = 1; s < m.nu; s++) {
@@ -432,14 +475,14 @@ __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
         }
     }
     {
-        const pvk_mglev &G = m.L[m.c0];
-        const LdsLev top = lds_level(m, lds, m.c0);
+        const Lev<T> G = lev<T>(m.L[m.c0]);
+        const LdsLev<T> top = lds_level(m, lds, m.c0);
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
             const int j = q / G.nx, i = q - j * G.nx;
-            G.x[(ptrdiff_t)j * G.pitch + i] = top.x[q];
+            if (m.c0 == 0 && m.z0) m.z0[(ptrdiff_t)j * m.p0 + i] = (double)top.x[q];
+            else G.x[(ptrdiff_t)j * G.pitch + i] = top.x[q];
         }
     }
-    (void)red;
 }
 
 // stop test alone, and <r, z> alone (grids small enough for one workgroup)
@@ -449,14 +492,15 @@ __global__ void __launch_bounds__(NTH) k_mg_check(pvk_scal *sc, pvk_pcg w, int n
     if (sc->p_done || n == -3) return;
     mg_stop_test(sc, w, n, tol, maxit, sh);
 }
-__global__ void __launch_bounds__(NTH) k_mg_dot(pvk_mglev L, const pvk_scal *sc, pvk_pcg w, int slot)
+__global__ void __launch_bounds__(NTH) k_mg_dot(const pvk_scal *sc, pvk_pcg w, pvk_grid g, int slot)
 {
     __shared__ double sh[NTH];
     if (sc->p_done) return;
+    const int P = g.P;
     double acc = 0.0;
-    for (int q = threadIdx.x; q < L.nx * L.ny; q += NTH) {
-        const int j = q / L.nx, i = q - j * L.nx;
-        acc += ld(L.f, L.pitch, i, j) * ld(L.x, L.pitch, i, j);
+    for (int q = threadIdx.x; q < g.NXM * g.NYM; q += NTH) {
+        const int j = q / g.NXM, i = q - j * g.NXM;
+        acc += AT(w.r, i, j) * AT(w.z, i, j);
     }
     block_store_partial(acc, w.rz[slot], 0, sh);
 }
@@ -512,7 +556,14 @@ static dim3 tile_grid(const pvk_mglev &L)
     return dim3((L.nx + tx - 1) / tx, (L.ny + MG_TY - 1) / MG_TY);
 }
 static dim3 cell_grid(const pvk_mglev &L) { return dim3((L.nx + TXP - 1) / TXP, (L.ny + TYR - 1) / TYR); }
-static size_t coarse_lds(const pvk_mg *m) { return (size_t)m->ldsx * sizeof(double); }
+
+// LDS bytes of the one-workgroup kernel
+extern "C" size_t pvk_mg_lds_bytes(const pvk_mg *m)
+{
+    size_t n = 0;
+    for (int l = m->c0; l < m->nlev; l++) n += ((l == m->c0 && !m->topf) ? 2 : 4) * (size_t)m->L[l].nx * m->L[l].ny;
+    return n * (m->fp32 ? sizeof(float) : sizeof(double));
+}
 
 extern "C" int pvk_mg_nrz(const pvk_mg *m)
 {
@@ -521,12 +572,32 @@ extern "C" int pvk_mg_nrz(const pvk_mg *m)
     return (int)(gr.x * gr.y);
 }
 
-// per solve: coarse faces of every level and the bottom inverse
-extern "C" void pvk_mg_setup(const pvk_mg *m, hipStream_t s)
+// per solve: coarse faces of every level (FP32: and the level-0 copies) and
+// the bottom inverse, from the FP64 level-0 faces bx0, by0 (pitch pf)
+extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s)
 {
-    for (int l = 0; l + 1 < m->nlev; l++)
-        hipLaunchKernelGGL(k_mg_faces, cell_grid(m->L[l + 1]), dim3(NTH), 0, s, m->L[l], m->L[l + 1]);
-    hipLaunchKernelGGL(k_mg_bottom, dim3(1), dim3(BT), 0, s, m->L[m->nlev - 1], m->G);
+    Lev<double> F0;
+    F0.nx = m->L[0].nx;
+    F0.ny = m->L[0].ny;
+    F0.pitch = pf;
+    F0.x = F0.f = nullptr;
+    F0.bx = (double *)bx0;
+    F0.by = (double *)by0;
+    if (m->fp32) {
+        hipLaunchKernelGGL(k_mg_faces0, cell_grid(m->L[0]), dim3(NTH), 0, s, F0, lev<float>(m->L[0]));
+        if (m->nlev > 1)
+            hipLaunchKernelGGL((k_mg_faces<double, float>), cell_grid(m->L[1]), dim3(NTH), 0, s, F0,
+                               lev<float>(m->L[1]));
+        for (int l = 1; l + 1 < m->nlev; l++)
+            hipLaunchKernelGGL((k_mg_faces<float, float>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
+                               lev<float>(m->L[l]), lev<float>(m->L[l + 1]));
+        hipLaunchKernelGGL(k_mg_bottom<float>, dim3(1), dim3(BT), 0, s, lev<float>(m->L[m->nlev - 1]), m->G);
+    } else {
+        for (int l = 0; l + 1 < m->nlev; l++)
+            hipLaunchKernelGGL((k_mg_faces<double, double>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
+                               lev<double>(m->L[l]), lev<double>(m->L[l + 1]));
+        hipLaunchKernelGGL(k_mg_bottom<double>, dim3(1), dim3(BT), 0, s, lev<double>(m->L[m->nlev - 1]), m->G);
+    }
 }
 
 // One V-cycle z = V(r) for CG iteration n is, in order: pvk_mg_pre for
@@ -535,37 +606,74 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, hipStream_t s)
 // w->rz[(n < 0) ? 0 : (n + 1) & 1].
 static int rz_slot(int n) { return (n < 0) ? 0 : ((n + 1) & 1); }
 
-#define MG_LAUNCH(KERNEL, L, ...)                                                                          \
-    do {                                                                                                   \
-        const dim3 gr_ = tile_grid(L);                                                                     \
-        if (m->nu == 1 && tile_tx(L) == 64)                                                                \
-            hipLaunchKernelGGL((KERNEL<1, 64>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
-        else if (m->nu == 1)                                                                               \
-            hipLaunchKernelGGL((KERNEL<1, 32>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
-        else if (tile_tx(L) == 64)                                                                         \
-            hipLaunchKernelGGL((KERNEL<2, 64>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
-        else                                                                                               \
-            hipLaunchKernelGGL((KERNEL<2, 32>), gr_, dim3(NTH), 0, s, __VA_ARGS__);                        \
-    } while (0)
+template <typename T>
+static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
+                  hipStream_t s)
+{
+    const dim3 gr = tile_grid(m->L[l]);
+    const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
+    const double *r0 = (l == 0) ? m->r0 : nullptr;
+    const int wide = tile_tx(m->L[l]) == 64, chk = (l == 0);
+    if (m->nu == 1 && wide)
+        hipLaunchKernelGGL((k_mg_pre<1, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    else if (m->nu == 1)
+        hipLaunchKernelGGL((k_mg_pre<1, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    else if (wide)
+        hipLaunchKernelGGL((k_mg_pre<2, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    else
+        hipLaunchKernelGGL((k_mg_pre<2, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+}
+
+template <typename T> static void post_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
+{
+    const dim3 gr = tile_grid(m->L[l]);
+    const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
+    const double *r0 = (l == 0) ? m->r0 : nullptr;
+    double *z0 = (l == 0) ? m->z0 : nullptr;
+    const pvk_scal *cs = sc;
+    const int wide = tile_tx(m->L[l]) == 64, slot = rz_slot(n), dot = (l == 0);
+    if (m->nu == 1 && wide)
+        hipLaunchKernelGGL((k_mg_post<1, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+    else if (m->nu == 1)
+        hipLaunchKernelGGL((k_mg_post<1, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+    else if (wide)
+        hipLaunchKernelGGL((k_mg_post<2, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+    else
+        hipLaunchKernelGGL((k_mg_post<2, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+}
 
 extern "C" void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                            hipStream_t s)
 {
-    MG_LAUNCH(k_mg_pre, m->L[l], m->L[l], m->L[l + 1], sc, *w, n, tol, maxit, l == 0);
+    if (m->fp32) pre_t<float>(m, l, sc, w, n, tol, maxit, s);
+    else pre_t<double>(m, l, sc, w, n, tol, maxit, s);
 }
 
-extern "C" void pvk_mg_coarse(const pvk_mg *m, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
-                              hipStream_t s)
+extern "C" void pvk_mg_coarse(const pvk_mg *m, const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n,
+                              double tol, int maxit, hipStream_t s)
 {
+    // above 64 KiB of dynamic LDS the kernel must be allowed it (gfx950: up to
+    // 160 KiB per workgroup); a refusal leaves the launch to report the error
+    static size_t allowed[2] = {65536, 65536};
+    const size_t need = pvk_mg_lds_bytes(m);
+    if (need > allowed[m->fp32]) {
+        const void *k = m->fp32 ? (const void *)k_mg_coarse<float> : (const void *)k_mg_coarse<double>;
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess)
+            allowed[m->fp32] = need;
+        else
+            (void)hipGetLastError();
+    }
     if (m->c0 == 0) hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
-    hipLaunchKernelGGL(k_mg_coarse, dim3(1), dim3(CT), coarse_lds(m), s, *m, sc);
-    if (m->c0 == 0) hipLaunchKernelGGL(k_mg_dot, dim3(1), dim3(NTH), 0, s, m->L[0], (const pvk_scal *)sc, *w,
-                                       rz_slot(n));
+    if (m->fp32) hipLaunchKernelGGL(k_mg_coarse<float>, dim3(1), dim3(CT), need, s, *m, sc);
+    else hipLaunchKernelGGL(k_mg_coarse<double>, dim3(1), dim3(CT), need, s, *m, sc);
+    if (m->c0 == 0)
+        hipLaunchKernelGGL(k_mg_dot, dim3(1), dim3(NTH), 0, s, (const pvk_scal *)sc, *w, *g, rz_slot(n));
 }
 
 extern "C" void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
 {
-    MG_LAUNCH(k_mg_post, m->L[l], m->L[l], m->L[l + 1], (const pvk_scal *)sc, *w, l == 0, rz_slot(n));
+    if (m->fp32) post_t<float>(m, l, sc, w, n, s);
+    else post_t<double>(m, l, sc, w, n, s);
 }
 
 extern "C" void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s)

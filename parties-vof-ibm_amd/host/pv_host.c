@@ -273,7 +273,8 @@ int pv_cg_create(pv_bag *b)
     const int nx = b->NXM, ny = b->NYM;
     const int band = 1;
     int nbc = (ny + band - 1) / band, nbu = nbc, nbv = (ny - 1 + band - 1) / band, c;
-    size_t n = 4 * (size_t)nbc + 4 * (size_t)nbu + 4 * (size_t)nbv;
+    int npap = ((ny + 7) / 8) * ((nx + 127) / 128);
+    size_t n = 3 * (size_t)nbc + (size_t)npap + 4 * (size_t)nbu + 4 * (size_t)nbv;
     double *q;
     HIPCHK(hipMalloc((void **)&b->cgmem, sizeof(double) * n));
     HIPCHK(hipMemset(b->cgmem, 0, sizeof(double) * n));
@@ -284,7 +285,8 @@ int pv_cg_create(pv_bag *b)
     b->ccg.bb = q; q += nbc;
     b->ccg.rr[0] = q; q += nbc;
     b->ccg.rr[1] = q; q += nbc;
-    b->ccg.pap = q; q += nbc;
+    b->ccg.pap = q; q += npap;
+    b->ccg.npap = npap;
     memset(&b->mcg, 0, sizeof b->mcg);
     b->mcg.band = band;
     b->mcg.rho = b->f[PV_F_RHO]; b->mcg.mu = b->f[PV_F_MU];

@@ -84,9 +84,11 @@ struct pv_bag {
     double *fftmem;
     pvk_pcg pcg;
     double *pcgmem;
-    pvk_mg mg;                     /* multigrid preconditioner (mg_ok) */
+    pvk_mg mg, mgf;                /* multigrid preconditioner, FP64 and FP32 V-cycle (mg_ok) */
+    pvk_mg *mgp;                   /* the one in use */
     pvk_pcg mgw;                   /* CG work set of the MG-PCG: fields + its own partials */
     double *mgmem;
+    float *mgfmem;
     int mg_ok, mg_chunk;
     int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
@@ -102,6 +104,10 @@ struct pv_bag {
     int use_graphs, graph_chunk;   /* pressure FFT-PCG chunks replayed from hipGraphs */
     hipGraphExec_t *pgraph;        /* one executable graph per chunk index */
     int npgraph;
+    hipGraphExec_t *mggraph;       /* the same for MG-PCG iterations */
+    int nmggraph;
+    const pvk_mg *mggraph_key;     /* hierarchy and sweeps the MG graphs were captured with */
+    int mggraph_nu;
     double learn_start_res;        /* relative residual after the last warm start */
 };
 
@@ -115,6 +121,7 @@ double *pv_dev_field(pv_bag *b, int f);
 int  pv_fft_create(pv_bag *b);
 int  pv_cg_create(pv_bag *b);
 int  pv_mg_create(pv_bag *b);
+int  pv_set_mg_precision(pv_bag *b, int fp32);
 void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
 int  pv_learn_snapshot(pv_bag *b, int si);

@@ -10,6 +10,11 @@
  * are even and the level has more than PVK_MG_BOTTOM_MIN cells; the last
  * level must have at most PVK_MG_BOTTOM cells.  A grid that does not halve
  * that way has no MG-PCG (mg_ok = 0).
+ *
+ * Two hierarchies of the same sizes are kept: mg (FP64 V-cycle) and mgf
+ * (FP32 V-cycle inside the FP64 CG; its level 0 keeps FP32 copies of x and
+ * the faces and reads r / writes z in FP64).  Handles of mixed precision use
+ * mgf unless pv_set_mg_precision says otherwise.
  */
 #include "pv_host.h"
 #include <stdlib.h>
@@ -17,17 +22,21 @@
 
 int pv_mg_create(pv_bag *b)
 {
-    pvk_mg *m = &b->mg;
-    int lx[PVK_MG_MAXLEV], ly[PVK_MG_MAXLEV], nl = 1, c0, l, nrz, ny = b->NYM;
-    size_t cells = 0, lds = 0, total;
+    pvk_mg *m = &b->mg, *mf = &b->mgf;
+    int lx[PVK_MG_MAXLEV], ly[PVK_MG_MAXLEV], nl = 1, c0, l, nrz, nx = b->NXM, ny = b->NYM;
+    size_t cells = 0, total, totalf, nc0 = (size_t)nx * ny;
     double *q;
+    float *qf;
     memset(m, 0, sizeof *m);
+    memset(mf, 0, sizeof *mf);
     memset(&b->mgw, 0, sizeof b->mgw);
     b->mgmem = NULL;
+    b->mgfmem = NULL;
     b->mg_ok = 0;
     b->mg_chunk = 4;
-    lx[0] = b->NXM;
-    ly[0] = b->NYM;
+    b->mgp = m;
+    lx[0] = nx;
+    ly[0] = ny;
     while (lx[nl - 1] * ly[nl - 1] > PVK_MG_LDS_CELLS) {
         if ((lx[nl - 1] & 1) || (ly[nl - 1] & 1) || nl == PVK_MG_MAXLEV) return PV_OK;
         lx[nl] = lx[nl - 1] / 2;
@@ -45,20 +54,28 @@ int pv_mg_create(pv_bag *b)
     /* the smoothing levels (all but the last) pair their cells along x */
     for (l = 0; l + 1 < nl; l++)
         if (lx[l] & 1) return PV_OK;
-    m->nlev = nl;
-    m->c0 = c0;
-    m->nu = 2;
+    m->nlev = mf->nlev = nl;
+    m->c0 = mf->c0 = c0;
+    m->nu = mf->nu = 2;
+    mf->fp32 = 1;
+    for (l = 0; l < nl; l++) {
+        m->L[l].nx = mf->L[l].nx = lx[l];
+        m->L[l].ny = mf->L[l].ny = ly[l];
+        m->L[l].pitch = mf->L[l].pitch = lx[l];
+    }
     for (l = 1; l < nl; l++) cells += (size_t)lx[l] * ly[l];
-    for (l = c0; l < nl; l++) lds += 4 * (size_t)lx[l] * ly[l];
-    m->ldsx = (int)lds;
-    m->L[0].nx = lx[0];
-    m->L[0].ny = ly[0];
     nrz = pvk_mg_nrz(m);
-    total = 4 * cells + (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 4 * (size_t)ny + 2 * (size_t)nrz;
+    total = 4 * cells + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 4 * (size_t)ny + 2 * (size_t)nrz;
+    totalf = 4 * cells + 3 * nc0;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
         return pv_set_error(PV_ENOMEM, "MG-PCG work allocation failed");
     }
+    if (hipMalloc((void **)&b->mgfmem, totalf * sizeof(float)) != hipSuccess) {
+        b->mgfmem = NULL;
+        return pv_set_error(PV_ENOMEM, "MG-PCG work allocation failed");
+    }
+    /* FP64 V-cycle: level 0 is the CG's z, r and faces */
     q = b->mgmem;
     m->L[0].pitch = b->g.P;
     m->L[0].x = b->f[PV_FI_Z];
@@ -66,16 +83,31 @@ int pv_mg_create(pv_bag *b)
     m->L[0].bx = b->f[PV_FI_PBX];
     m->L[0].by = b->f[PV_FI_PBY];
     for (l = 1; l < nl; l++) {
-        size_t nc = (size_t)lx[l] * ly[l];
-        m->L[l].nx = lx[l];
-        m->L[l].ny = ly[l];
-        m->L[l].pitch = lx[l];
-        m->L[l].x = q; q += nc;
-        m->L[l].f = q; q += nc;
-        m->L[l].bx = q; q += nc;
-        m->L[l].by = q; q += nc;
+        size_t n = (size_t)lx[l] * ly[l];
+        m->L[l].x = q; q += n;
+        m->L[l].f = q; q += n;
+        m->L[l].bx = q; q += n;
+        m->L[l].by = q; q += n;
     }
     m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
+    m->p0 = b->g.P;
+    /* FP32 V-cycle: level 0 has its own x and face copies; r in, z out */
+    qf = b->mgfmem;
+    mf->L[0].x = qf; qf += nc0;
+    mf->L[0].f = NULL;
+    mf->L[0].bx = qf; qf += nc0;
+    mf->L[0].by = qf; qf += nc0;
+    for (l = 1; l < nl; l++) {
+        size_t n = (size_t)lx[l] * ly[l];
+        mf->L[l].x = qf; qf += n;
+        mf->L[l].f = qf; qf += n;
+        mf->L[l].bx = qf; qf += n;
+        mf->L[l].by = qf; qf += n;
+    }
+    mf->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
+    mf->r0 = b->f[PV_F_P_RES];
+    mf->z0 = b->f[PV_FI_Z];
+    mf->p0 = b->g.P;
     {
         pvk_pcg *w = &b->mgw;
         w->phi = b->f[PV_F_PHI];
@@ -103,6 +135,11 @@ int pv_mg_create(pv_bag *b)
         w->rdct = 1;
         w->flex = 0;
     }
+    /* the faces of level c0 go to LDS too when the whole set fits in 144 KiB */
+    m->topf = mf->topf = 1;
+    if (pvk_mg_lds_bytes(m) > 144 * 1024) m->topf = 0;
+    if (pvk_mg_lds_bytes(mf) > 144 * 1024) mf->topf = 0;
+    b->mgp = (b->prm.precision == PV_PREC_MIXED) ? mf : m;
     b->mg_ok = 1;
     return PV_OK;
 }
@@ -110,8 +147,18 @@ int pv_mg_create(pv_bag *b)
 void pv_mg_destroy(pv_bag *b)
 {
     if (b->mgmem) hipFree(b->mgmem);
+    if (b->mgfmem) hipFree(b->mgfmem);
     b->mgmem = NULL;
+    b->mgfmem = NULL;
     b->mg_ok = 0;
+}
+
+int pv_set_mg_precision(pv_bag *b, int fp32)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (fp32 < 0 || fp32 > 1) return pv_set_error(PV_EINVAL, "MG precision must be 0 (FP64) or 1 (FP32)");
+    b->mgp = fp32 ? &b->mgf : &b->mg;
+    return PV_OK;
 }
 
 int pv_set_pressure_warm_start(pv_bag *b, int mode)
@@ -127,6 +174,6 @@ int pv_set_mg_sweeps(pv_bag *b, int nu)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");
     if (nu < 1 || nu > 2) return pv_set_error(PV_EINVAL, "MG sweeps must be 1 or 2 (got %d)", nu);
-    b->mg.nu = nu;
+    b->mg.nu = b->mgf.nu = nu;
     return PV_OK;
 }

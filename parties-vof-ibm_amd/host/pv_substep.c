@@ -460,6 +460,17 @@ static int pcg_loop_graphs(pv_bag *b, double tol, int maxit)
     return PV_OK;
 }
 
+static void mg_graphs_free(pv_bag *b)
+{
+    int k;
+    for (k = 0; k < b->nmggraph; k++)
+        if (b->mggraph[k]) hipGraphExecDestroy(b->mggraph[k]);
+    free(b->mggraph);
+    b->mggraph = NULL;
+    b->nmggraph = 0;
+    b->mggraph_key = NULL;
+}
+
 void pv_graphs_free(pv_bag *b)
 {
     int k;
@@ -468,12 +479,13 @@ void pv_graphs_free(pv_bag *b)
     free(b->pgraph);
     b->pgraph = NULL;
     b->npgraph = 0;
+    mg_graphs_free(b);
 }
 
 /* iterations n0, n0 + 1, ... until the device stop flag */
 static int pcg_loop(pv_bag *b, int n0, double tol, int maxit)
 {
-    int chunk = b->p_chunk, n = n0;
+    int chunk = (b->p_chunk - n0 > 4) ? b->p_chunk - n0 : 4, n = n0;
     if (b->use_graphs && n0 == 0 && !b->timers_on) return pcg_loop_graphs(b, tol, maxit);
     for (;;) {
         int k;
@@ -501,7 +513,7 @@ static int p_solve_fft_pcg(pv_bag *b)
 /* z = V(r) for CG iteration n (the stop test of n first; n = -3: none) */
 static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
 {
-    const pvk_mg *m = &b->mg;
+    const pvk_mg *m = b->mgp;
     int l;
     if (m->c0 > 0) {
         pv_tbegin(b, PV_T_MG_PRE0);
@@ -510,7 +522,7 @@ static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
     }
     pv_tbegin(b, PV_T_MG_COARSE);
     for (l = 1; l < m->c0; l++) pvk_mg_pre(m, l, b->d_sc, &b->mgw, n, tol, maxit, S(b));
-    pvk_mg_coarse(m, b->d_sc, &b->mgw, n, tol, maxit, S(b));
+    pvk_mg_coarse(m, G(b), b->d_sc, &b->mgw, n, tol, maxit, S(b));
     for (l = m->c0 - 1; l >= 1; l--) pvk_mg_post(m, l, b->d_sc, &b->mgw, n, S(b));
     pv_tend(b, PV_T_MG_COARSE);
     if (m->c0 > 0) {
@@ -535,9 +547,49 @@ static void mg_iteration(pv_bag *b, int n, double tol, int maxit)
 static void mg_cold_start(pv_bag *b, double tol, int maxit)
 {
     pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
-    pvk_mg_setup(&b->mg, S(b));
+    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
     pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
     mg_vcycle(b, -1, tol, maxit);
+}
+
+/* MG-PCG iterations 0, 1, ... replayed from hipGraphs: graph kc holds
+ * iterations [kc C, (kc + 1) C), captured on first use for the hierarchy
+ * and sweep count in use (the kernels stop themselves on the device flag). */
+static int mg_loop_graphs(pv_bag *b, double tol, int maxit)
+{
+    const int C = b->graph_chunk;
+    int kc = 0, n = 0;
+    if (b->mggraph_key != b->mgp || b->mggraph_nu != b->mgp->nu) {
+        mg_graphs_free(b);
+        b->mggraph_key = b->mgp;
+        b->mggraph_nu = b->mgp->nu;
+    }
+    for (;;) {
+        if (kc >= b->nmggraph) {
+            int cap = (kc + 1) * 2, k;
+            hipGraphExec_t *ng = (hipGraphExec_t *)realloc(b->mggraph, cap * sizeof(hipGraphExec_t));
+            if (!ng) return pv_set_error(PV_ENOMEM, "graph table allocation failed");
+            for (k = b->nmggraph; k < cap; k++) ng[k] = NULL;
+            b->mggraph = ng;
+            b->nmggraph = cap;
+        }
+        if (!b->mggraph[kc]) {
+            hipGraph_t gr;
+            int k;
+            HIPCHK(hipStreamBeginCapture(S(b), hipStreamCaptureModeThreadLocal));
+            for (k = 0; k < C && kc * C + k < maxit; k++) mg_iteration(b, kc * C + k, tol, maxit);
+            HIPCHK(hipStreamEndCapture(S(b), &gr));
+            HIPCHK(hipGraphInstantiate(&b->mggraph[kc], gr, NULL, NULL, 0));
+            HIPCHK(hipGraphDestroy(gr));
+        }
+        HIPCHK(hipGraphLaunch(b->mggraph[kc], S(b)));
+        n = (kc + 1) * C;
+        kc++;
+        RC(pv_read_scalars(b));
+        if (b->h_sc->p_done) break;
+        if (n >= maxit) return pv_set_error(PV_EHIP, "pressure MG-PCG did not stop (device flag)");
+    }
+    return PV_OK;
 }
 
 /* iterations n0, n0 + 1, ... until the device stop flag; the first check
@@ -545,6 +597,7 @@ static void mg_cold_start(pv_bag *b, double tol, int maxit)
 static int mg_loop(pv_bag *b, int n0, double tol, int maxit)
 {
     int chunk = b->mg_chunk, n = n0;
+    if (b->use_graphs && n0 == 0 && !b->timers_on) return mg_loop_graphs(b, tol, maxit);
     for (;;) {
         int k;
         for (k = 0; k < chunk && n < maxit; k++, n++) mg_iteration(b, n, tol, maxit);
@@ -671,8 +724,12 @@ static int p_solve_collect(pv_bag *b)
         RC(pv_read_scalars(b));
         if (b->h_sc->p_done) break;
         if (si < L->nsnap && n == L->snap_iter[si]) RC(pv_learn_snapshot(b, si++));
-        next = (si < L->nsnap) ? L->snap_iter[si] : maxit;
         if (n >= maxit) return pv_set_error(PV_EHIP, "pressure FFT-PCG did not stop (device flag)");
+        if (si >= L->nsnap) {                  /* no snapshot left: the checked loop */
+            RC(pcg_loop(b, n, tol, maxit));
+            break;
+        }
+        next = L->snap_iter[si];
         for (; n < next && n < maxit; n++) pcg_iteration(b, n, tol, maxit);
     }
     return pv_learn_harvest(b, si);
@@ -838,7 +895,7 @@ int pv_mg_vcycle_apply(pv_bag *b)
     RC(begin(b));
     if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
     pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
-    pvk_mg_setup(&b->mg, S(b));
+    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
     mg_vcycle(b, -3, 0.0, 1);
     pvk_learn_copy(G(b), b->mgw.z, FD(b, PV_F_SCRATCH0), S(b));
     LAUNCHCHK();

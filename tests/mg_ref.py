@@ -6,7 +6,7 @@ The operator is the variable-coefficient pressure operator of the PCG
 psolve_cg.c:63-101), x periodic, zero-gradient walls in y:
     (A x)_ij = bx_ij (x_E - x) - bx_(i-1)j (x - x_W) + by_ij (x_N - x) - by_i(j-1) (x - x_S).
 One V-cycle V(f), arrays indexed [j][i]:
-  * levels halve both sizes while a level has more than 1024 cells, then
+  * levels halve both sizes while a level has more than 4096 cells, then
     while both sizes are even and the level has more than 64 cells; the last
     level must have at most 64 cells; coarse faces are half the sum of the
     two fine faces across the coarse face;
@@ -20,7 +20,7 @@ catch it): "post_red_first", "restrict_mean", "coarse_sum".
 """
 import numpy as np
 
-LDS_CELLS = 1024
+LDS_CELLS = 4096
 BOTTOM_MIN = 64
 BOTTOM = 64
 

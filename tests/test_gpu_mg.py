@@ -2,8 +2,9 @@
 
 * One V-cycle of the GPU (through the pv_mg_vcycle_apply hook) against the
   NumPy restatement tests/mg_ref.py on bubbly densities, on grids whose
-  levels all run in one workgroup (8 x 8, 32 x 32), on grids with both kinds
-  of level (64 x 128, 96 x 96, 256 x 256) and for 1 and 2 sweeps.
+  levels all run in one workgroup (8 x 8, 32 x 32, 64 x 64), on grids with
+  both kinds of level (64 x 128, 96 x 96, 256 x 256), for 1 and 2 sweeps,
+  with the FP64 V-cycle (to 1e-11) and the FP32 one (to 1e-5).
 * Pressure solves on uneven sizes (where the FFT preconditioner does not
   exist): the GPU's PHI has the stated relative residual under the NumPy
   operator, zero mean, and took few iterations.
@@ -42,12 +43,14 @@ def canonical(p, a):
     return out
 
 
-@pytest.mark.parametrize("nx,ny", [(8, 8), (32, 32), (64, 128), (96, 96), (256, 256)])
+@pytest.mark.parametrize("nx,ny", [(8, 8), (32, 32), (64, 64), (64, 128), (96, 96), (256, 256)])
 @pytest.mark.parametrize("nu", [1, 2])
-def test_vcycle_matches_reference(lib, nx, ny, nu):
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_vcycle_matches_reference(lib, nx, ny, nu, fp32):
     p = params(nx, ny)
     s = pvof.Solver(p, lib)
     s.check(lib.pv_set_mg_sweeps(s.h, nu))
+    s.check(lib.pv_set_mg_precision(s.h, fp32))
     rho = mg_ref.bubbles(nx, ny, seed=nx + ny, nb=6)
     f = np.random.default_rng(nx * ny).standard_normal((ny, nx))
     f -= f.mean()
@@ -58,14 +61,16 @@ def test_vcycle_matches_reference(lib, nx, ny, nu):
     bx, by = mg_ref.faces(rho, float(nx * nx), float(nx * nx))
     lv, G = mg_ref.build(bx, by)
     zr = mg_ref.vcycle(lv, G, f, nu)
-    print(f"\n{nx}x{ny} nu={nu} levels={len(lv)} relerr={relerr(z, zr):.2e}")
-    assert relerr(z, zr) < 1e-11
+    print(f"\n{nx}x{ny} nu={nu} fp32={fp32} levels={len(lv)} relerr={relerr(z, zr):.2e}")
+    assert relerr(z, zr) < (1e-5 if fp32 else 1e-11)
 
 
 @pytest.mark.parametrize("nx,ny", [(96, 96), (192, 96), (256, 512), (384, 768)])
-def test_pressure_solve_residual(lib, nx, ny):
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_pressure_solve_residual(lib, nx, ny, fp32):
     p = params(nx, ny)
     s = pvof.Solver(p, lib)
+    s.check(lib.pv_set_mg_precision(s.h, fp32))
     rho = mg_ref.bubbles(nx, ny, seed=5, nb=10)
     rng = np.random.default_rng(6)
     b = rng.standard_normal((ny, nx))
@@ -77,7 +82,7 @@ def test_pressure_solve_residual(lib, nx, ny):
     bx, by = mg_ref.faces(rho, float(nx * nx), float(nx * nx))
     bm = b - b.mean()
     res = np.linalg.norm(mg_ref.apply_a(bx, by, phi) - bm) / np.linalg.norm(bm)
-    print(f"\n{nx}x{ny}: {it.value} iterations, relative residual {res:.2e}")
+    print(f"\n{nx}x{ny} fp32={fp32}: {it.value} iterations, relative residual {res:.2e}")
     assert res <= 1.05e-10
     assert abs(phi.mean()) < 1e-12 * np.abs(phi).max()
     assert 0 < it.value <= 30
@@ -88,8 +93,10 @@ def test_rk3_steps_match_fft_pcg(lib):
     pm = params(n, n)
     F = pvof.ic(lib, "bubbles", pm, seed=222, nmin=8, nmax=24, rmin=0.02, rmax=0.1, gas_inside=1)
     out = {}
-    for name, solver in (("fft", pvof.PV_PSOLVE_FFT_PCG), ("mg", pvof.PV_PSOLVE_MG_PCG)):
+    for name, solver, fp32 in (("fft", pvof.PV_PSOLVE_FFT_PCG, 0), ("mg", pvof.PV_PSOLVE_MG_PCG, 0),
+                               ("mg32", pvof.PV_PSOLVE_MG_PCG, 1)):
         s = pvof.Solver(params(n, n, solver), lib)
+        s.check(lib.pv_set_mg_precision(s.h, fp32))
         s.init_from_F(F)
         iters = []
         for _ in range(3):
@@ -98,11 +105,14 @@ def test_rk3_steps_match_fft_pcg(lib):
             assert max(st["p_res"]) <= 1e-10
             iters += list(st["n_p"])
         out[name] = (s, iters)
-    print(f"\npressure iterations per substep: FFT-PCG {out['fft'][1]}, MG-PCG {out['mg'][1]}")
+    print(f"\npressure iterations per substep: FFT-PCG {out['fft'][1]}, MG-PCG {out['mg'][1]}, "
+          f"MG-PCG with the FP32 V-cycle {out['mg32'][1]}")
     assert max(out["mg"][1]) * 5 < min(out["fft"][1])
+    assert max(out["mg32"][1]) * 5 < min(out["fft"][1])
     R = regions(pm)
-    for f in ("P", "U", "V", "C_L"):
-        assert relerr(out["mg"][0].get(f)[R["cell"]], out["fft"][0].get(f)[R["cell"]]) < 1e-9, f
+    for name in ("mg", "mg32"):
+        for f in ("P", "U", "V", "C_L"):
+            assert relerr(out[name][0].get(f)[R["cell"]], out["fft"][0].get(f)[R["cell"]]) < 1e-9, (name, f)
 
 
 def test_temporal_warm_start(lib):
