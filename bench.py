@@ -61,9 +61,18 @@ BYTES_PER_CELL_FFT = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=56)
 # level-0 pre-smoothing + restriction (f, faces -> x, coarse f) and
 # correction + post-smoothing (x, coarse x, f, faces -> x)
 BYTES_PER_CELL_MG = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=48, MG_PRE0=34, MG_POST0=42)
+# the FP32 V-cycle (mixed precision): level 0 reads r (FP64) and FP32 faces,
+# writes FP32 x (pre) and FP64 z (post)
+BYTES_PER_CELL_MG32 = dict(BYTES_PER_CELL_MG, MG_PRE0=21, MG_POST0=29)
 # per pressure iteration, all of its kernels (MG: level 0 + 1/3 of the level-0
 # smoothing bytes for the coarser levels)
-BYTES_PER_ITER = {"fft": 176, "mg": 197, "jacobi": 112}
+BYTES_PER_ITER = {"fft": 176, "mg": 197, "mg32": 163, "jacobi": 112}
+# HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
+# the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
+# the factors a 256 MiB device copy gives in the same run); the summary is
+# profiles/mg_pcg_mixed_1024_pmc_traffic.md
+TRAFFIC_MB_MG32_1024 = {"MG_POST0": 29.7, "MG_PRE0": 22.1, "P_APPLY": 48.3, "P_UPDATE": 48.3,
+                        "CH_APPLY": 26.7, "CH_UPDATE": 39.0, "MOM_APPLY": 82.2, "MOM_UPDATE": 86.1}
 SEED = 20261202
 
 
@@ -173,7 +182,10 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     if timers:
         s.check(lib.pv_timers_read(s.h, ms, cnt))
     fam = family_of(solver)
-    bpc = {"jacobi": BYTES_PER_CELL, "fft": BYTES_PER_CELL_FFT, "mg": BYTES_PER_CELL_MG}[fam]
+    if fam == "mg" and precision == "mixed":      # mixed handles run the FP32 V-cycle
+        fam = "mg32"
+    bpc = {"jacobi": BYTES_PER_CELL, "fft": BYTES_PER_CELL_FFT, "mg": BYTES_PER_CELL_MG,
+           "mg32": BYTES_PER_CELL_MG32}[fam]
     cells = p.NXM * p.NYM
     kernels = {}
     for k, name in enumerate(TIMERS):
@@ -190,7 +202,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
            "max_p_res": float(max(max(st["p_res"]) for st in stats)),
            "path_hbm_fraction": round(bytes_step * cells * steps / elapsed / 1e9 / HBM_PEAK_GBS, 4),
            "path_bytes_per_cell_step": round(bytes_step, 1), "kernels": kernels, "_bpc": bpc, "_cells": cells,
-           "_params": p}
+           "_params": p, "_family": fam}
     if net_params is not None:
         out["start_energy_ratio_last_step"] = starts[-1] if starts else None
         out["fallbacks"] = int(sum(sum(st["p_fallback"]) for st in stats))
@@ -227,6 +239,9 @@ def main():
     ap.add_argument("--timer-period", type=int, default=16,
                     help="time every k-th launch of each kernel class with HIP events")
     ap.add_argument("--no-timers", action="store_true")
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="first copy this many MiB device to device three times (a kernel of known traffic, "
+                         "to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in the same run)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
@@ -234,6 +249,14 @@ def main():
         sys.exit(2)
 
     lib = pvof.load_lib()
+    if args.copy_mib > 0:
+        import torch
+        src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        del src, dst
     prec = pvof.PV_PREC_MIXED if args.precision == "mixed" else pvof.PV_PREC_FP64
     learned = None
     if args.solver.startswith("learned"):
@@ -295,8 +318,12 @@ def main():
     if timed:
         dom = max(timed, key=lambda n: timed[n]["est_total_ms"])
         achieved = timed[dom]["GBs"]
+        traffic = None
+        if main_run["_family"] == "mg32" and cells == 1024 * 1024 and dom in TRAFFIC_MB_MG32_1024:
+            traffic = TRAFFIC_MB_MG32_1024[dom] * 1e6
         out["roofline"] = {"bound": "hbm", "kernel": dom, "achieved": achieved, "peak": HBM_PEAK_GBS,
-                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                           "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+                           "traffic_unit": "bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE, calibrated)",
                            "bytes_per_cell": bpc[dom], "cells_per_launch": cells,
                            "avg_us": timed[dom]["avg_us"], "timer_period": args.timer_period}
     if not args.no_cpu_baseline:

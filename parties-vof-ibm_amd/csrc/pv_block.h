@@ -45,11 +45,13 @@ __device__ __forceinline__ void block_store_partial(double v, double *part, int 
 }
 
 // Workgroups are dealt to the 8 XCDs round robin (block b runs on XCD b % 8).
-// Give each XCD a contiguous run of bands so the rows a band's stencil shares
-// with its neighbours stay in that XCD's L2.  Identity unless nblocks % 8 == 0.
+// Give each XCD a contiguous run of bands (or tiles, in row-major order) so
+// the data a band's stencil shares with its neighbours stays in that XCD's
+// L2: XCD x owns q + (x < rem) consecutive indices from x q + min(x, rem),
+// q = nblocks / 8, rem = nblocks % 8 (a bijection on [0, nblocks)).
 __device__ __forceinline__ int xcd_band(int b, int nblocks)
 {
-    if (nblocks & 7) return b;
-    return (b & 7) * (nblocks >> 3) + (b >> 3);
+    const int x = b & 7, q = nblocks >> 3, rem = nblocks & 7;
+    return x * q + (x < rem ? x : rem) + (b >> 3);
 }
 #endif

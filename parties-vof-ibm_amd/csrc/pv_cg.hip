@@ -146,7 +146,9 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
     if (stop) return;
 
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
-    const int c0 = blockIdx.x * CHT_X, j0 = blockIdx.y * CHT_Y;
+    const int t = xcd_band(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+    const int c0 = tx * CHT_X, j0 = ty * CHT_Y;
     for (int q = threadIdx.x; q < (CHT_Y + 4) * (CHT_X + 4); q += NTH) {
         const int a = q / (CHT_X + 4), b = q - a * (CHT_X + 4);
         const int ia = chx(c0 + b - 2, NXM), jb = chy(j0 + a - 2, NYM);
@@ -174,7 +176,7 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
             acc += ch_w(g, i, j, pc, ap);
         }
     }
-    block_store_partial(acc, w.pap, blockIdx.y * gridDim.x + blockIdx.x, sh);
+    block_store_partial(acc, w.pap, t, sh);
 }
 
 // K2: alpha = <r,r> / <p,Ap> (VOF_DIFFUSE.c:384-390); x += alpha p;

@@ -120,6 +120,14 @@ __device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, i
     }
 }
 
+// tile of this block, XCD-aware (consecutive tiles of a run share an L2)
+__device__ __forceinline__ void tile_of(int &bx, int &by)
+{
+    const int t = xcd_band(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    by = t / gridDim.x;
+    bx = t - by * gridDim.x;
+}
+
 // half-sweep k of colour col on the cells at distance >= k from the edge
 template <int NU, int TX, typename T>
 __device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k, int col, const T *f, const T *ce,
@@ -158,7 +166,9 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
-    const int x0 = blockIdx.x * TX - TT::H, y0 = blockIdx.y * MG_TY - TT::H;
+    int bx, by;
+    tile_of(bx, by);
+    const int x0 = bx * TX - TT::H, y0 = by * MG_TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
     tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
@@ -167,7 +177,7 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
     // restriction: one coarse cell per thread (TX/2 x MG_TY/2 <= NTH)
     if ((int)threadIdx.x < (TX / 2) * (MG_TY / 2)) {
         const int I = threadIdx.x % (TX / 2), J = threadIdx.x / (TX / 2);
-        const int gI = blockIdx.x * (TX / 2) + I, gJ = blockIdx.y * (MG_TY / 2) + J;
+        const int gI = bx * (TX / 2) + I, gJ = by * (MG_TY / 2) + J;
         if (gI < C.nx && gJ < C.ny) {
             const int p = L.pitch, nx = L.nx, ny = L.ny;
             T s = T(0);
@@ -189,7 +199,7 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
     // the core of x for the post-smoothing
     for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
         const int ly = c / TX, lx = c - ly * TX;
-        const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
+        const int gi = bx * TX + lx, gj = by * MG_TY + ly;
         if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
     }
 }
@@ -205,7 +215,9 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
     __shared__ T xs[TT::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int x0 = blockIdx.x * TX - TT::H, y0 = blockIdx.y * MG_TY - TT::H;
+    int bx, by;
+    tile_of(bx, by);
+    const int x0 = bx * TX - TT::H, y0 = by * MG_TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
     tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) {
@@ -223,7 +235,7 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
     double acc = 0.0;
     for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
         const int ly = c / TX, lx = c - ly * TX;
-        const int gi = blockIdx.x * TX + lx, gj = blockIdx.y * MG_TY + ly;
+        const int gi = bx * TX + lx, gj = by * MG_TY + ly;
         if (gi < L.nx && gj < L.ny) {
             const T v = xs[(ly + TT::H) * TT::W + lx + TT::H];
             if (z0) z0[(ptrdiff_t)gj * p0 + gi] = (double)v;
@@ -231,7 +243,7 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
             if (dot) acc += (r0 ? r0[(ptrdiff_t)gj * p0 + gi] : (double)ld(L.f, L.pitch, gi, gj)) * (double)v;
         }
     }
-    if (dot) block_store_partial(acc, w.rz[slot], blockIdx.y * gridDim.x + blockIdx.x, sh);
+    if (dot) block_store_partial(acc, w.rz[slot], by * gridDim.x + bx, sh);
 }
 
 // coarse face coefficients from the fine ones (the fine level may be the
@@ -513,7 +525,7 @@ __global__ void __launch_bounds__(NTH) k_mg_dot(const pvk_scal *sc, pvk_pcg w, p
 __global__ void __launch_bounds__(NTH) k_mg_start(pvk_grid g, const pvk_scal *sc, pvk_pcg w, int warm)
 {
     __shared__ double sh[NTH];
-    const int P = g.P, j = blockIdx.x;
+    const int P = g.P, j = xcd_band(blockIdx.x, gridDim.x);
     const double mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
     double acc = 0.0;
     for (int i = threadIdx.x; i < g.NXM; i += NTH) {
@@ -532,7 +544,7 @@ __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *s
 {
     __shared__ double sh[NTH];
     if (sc->p_done) return;
-    const int P = g.P, j = blockIdx.x;
+    const int P = g.P, j = xcd_band(blockIdx.x, gridDim.x);
     const double alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
     const double *d = w.d[n & 1];
     double acc = 0.0;

@@ -122,7 +122,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
     const double *__restrict__ bx = w.bx, *__restrict__ by = w.by;
     double *__restrict__ dnew = w.d[n & 1];
     double *__restrict__ Ad = w.Ad;
-    const int j = blockIdx.x;
+    const int j = xcd_band(blockIdx.x, gridDim.x);
     const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
     double acc = 0.0;
     for (int base = 0; base < NXM; base += NTH * CPT) {
@@ -152,7 +152,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
             }
         }
     }
-    block_store_partial(acc, w.dad[n & 1], blockIdx.x, sh);
+    block_store_partial(acc, w.dad[n & 1], j, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -455,7 +455,7 @@ extern "C" void pvk_pcg_gamma(pvk_scal *sc, const pvk_pcg *w, int rzs, int slot,
 __global__ void __launch_bounds__(NTH) k_pcg_opx(pvk_grid g, pvk_pcg w, const double *__restrict__ x,
                                                 double *__restrict__ out)
 {
-    const int P = g.P, NXM = g.NXM, NYM = g.NYM, j = blockIdx.x;
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM, j = xcd_band(blockIdx.x, gridDim.x);
     const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
     for (int i = threadIdx.x; i < NXM; i += NTH) {
         const int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
