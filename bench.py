@@ -160,6 +160,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
         pvof.Network(s).set_params(net_params)
     if family_of(solver) == "mg":
         s.check(lib.pv_set_pressure_warm_start(s.h, args.warm_start))
+        s.check(lib.pv_set_ch_solver(s.h, args.ch_spectral))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -227,6 +228,8 @@ def main():
                     help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--warm-start", type=int, choices=[0, 1], default=1,
                     help="MG-PCG solvers: 1 = also try the same RK3 stage's last correction as the start")
+    ap.add_argument("--ch-spectral", type=int, choices=[0, 1], default=1,
+                    help="MG-PCG forms: 1 = start the biharmonic CG from its exact spectral solve")
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain FP64 FFT-PCG run and the no-network ablation")
     ap.add_argument("--plain-steps", type=int, default=10)
@@ -299,6 +302,7 @@ def main():
                                f"{LABEL[args.solver]} pressure to 1e-10 relative residual",
                    "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
                    "warm_start": args.warm_start if family_of(args.solver) == "mg" else None,
+                   "ch_spectral": args.ch_spectral if family_of(args.solver) == "mg" else None,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": main_run["path_hbm_fraction"],
         "path_bytes_per_cell_step": main_run["path_bytes_per_cell_step"],

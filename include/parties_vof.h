@@ -297,6 +297,13 @@ int pv_set_pressure_warm_start(pv_bag *bag, int mode);
  * for PV_PREC_FP64 handles), 1 = FP32 (default for PV_PREC_MIXED handles).
  * The CG vectors, dots, stop test and solution stay FP64 either way. */
 int pv_set_mg_precision(pv_bag *bag, int fp32);
+/* Cahn-Hilliard implicit solve (no counterpart in the reference, which runs
+ * the CG of VOF_DIFFUSE.c:318-429 from C): 0 = that CG (default); 1 = the CG
+ * started from the exact solution of a_dt x + (Cn^2/Pe) L^2 x = b, which the
+ * constant-coefficient operator allows on the uniform grid through the
+ * FFT-x / DCT-y transforms of the pressure preconditioner (power-of-two
+ * sizes); the CG's own stop test accepts it, normally at iteration 0. */
+int pv_set_ch_solver(pv_bag *bag, int spectral);
 int pv_set_mg_sweeps(pv_bag *bag, int nu);
 int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
                            int max_samples, unsigned long long seed);

@@ -34,7 +34,8 @@ template <class R>
 __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc,
                                                      const double *__restrict__ C, const double *__restrict__ u,
                                                      const double *__restrict__ v, double *__restrict__ rn,
-                                                     double *__restrict__ r, double *__restrict__ p, pvk_ccg w)
+                                                     double *__restrict__ r, double *__restrict__ p, pvk_ccg w,
+                                                     double *__restrict__ bout)
 {
     __shared__ double sh[NTH];
     const int P = g.P, NXM = g.NXM;
@@ -54,6 +55,7 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, i
         double b = rv - ph.bih * L2;
         double Ap = a_dt * cc + ph.bih * L2;
         double rr = b - Ap;
+        if (bout) AT(bout, i, j) = b;
         AT(rn, i, j) = rc;
         AT(r, i, j) = rr;
         AT(p, i, j) = rr;
@@ -209,16 +211,16 @@ __global__ void __launch_bounds__(NTH) k_ccg_update(pvk_grid g, pvk_scal *sc, pv
 
 extern "C" void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
                              const double *C, const double *u, const double *v, double *rn, const double *b,
-                             double *r, double *p, hipStream_t s)
+                             double *r, double *p, double *bout, hipStream_t s)
 {
     if (b)
         hipLaunchKernelGGL(k_ccg_init_b, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, b, r, p, *w);
     else if (ph->precision)
         hipLaunchKernelGGL(k_ccg_init_rhs<float>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p,
-                           *w);
+                           *w, bout);
     else
         hipLaunchKernelGGL(k_ccg_init_rhs<double>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r,
-                           p, *w);
+                           p, *w, bout);
 }
 
 extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,

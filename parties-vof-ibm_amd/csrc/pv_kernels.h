@@ -84,6 +84,10 @@ typedef struct pvk_pcg {
     int ndad, nrr, nrz, nmean;        /* partial counts */
     int flex;                         /* 1: flexible CG, beta = -<z_{n+1}, A d_n> / <d_n, A d_n> */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
+    double chb;                       /* > 0: the FFT stages solve (a_dt + chb L^2) z = r (Cahn-Hilliard,
+                                         a_dt of RK stage chstage) instead of L z = r, and run even
+                                         after the pressure solve's stop flag */
+    int chstage;
 } pvk_pcg;
 
 /* Work sets of the fused CG loops (csrc/pv_cg.hip).  Partial-sum arrays
@@ -142,9 +146,11 @@ void pvk_ch_sub_bih(const pvk_grid *g, const pvk_phys *ph, const double *C, doub
 /* fused biharmonic CG (csrc/pv_cg.hip): start from the explicit CH
  * right-hand side (b == NULL: WENO5 + diffusion + RK3 assembly) or from a
  * given b; then iterations n = 0, 1, ...: half 0 = apply, half 1 = update */
+/* b == NULL: the explicit right-hand side and the CG start from C (bout, if
+ * not NULL, receives b); else r = p = b - A C for a given b */
 void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
                   const double *C, const double *u, const double *v, double *rn, const double *b,
-                  double *r, double *p, hipStream_t s);
+                  double *r, double *p, double *bout, hipStream_t s);
 void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,
                   int half, int n, double *x, double *r, double *Ap, hipStream_t s);
 void pvk_clamp_props(const pvk_grid *g, const pvk_phys *ph, double *C, double *G, double *rho,

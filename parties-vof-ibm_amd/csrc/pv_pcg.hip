@@ -23,6 +23,7 @@
 #include "pv_kernels.h"
 #include "pv_block.h"
 #include "pv_fftk.h"
+#include "pv_stencil.h"
 
 #define AT(a, i, j) (a)[(ptrdiff_t)(j) * P + (i)]
 
@@ -163,7 +164,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
 {
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
-    if (sc->p_done) return;
+    if (w.chb == 0.0 && sc->p_done) return;
     const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);
@@ -246,7 +247,7 @@ __global__ void __launch_bounds__(NTH) k_transpose(pvk_scal *sc, pvk_pcg w, cons
     __shared__ double tile[32][33];
     __shared__ double sh[NTH];
     // a stop flag set by an earlier iteration leaves <r,r> stale: no recount
-    if (sc->p_done) return;
+    if (w.chb == 0.0 && sc->p_done) return;
     if (check && n != -3) {                 // n = -3: a plain M_fft solve, no stop test
         double rr = block_sum(w.rr, w.nrr, sh);
         // n = -1: cold start, ||r0|| = ||b - mean|| is the reference norm;
@@ -296,7 +297,8 @@ __device__ __forceinline__ int dct_slot(int n, int N)
 __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, double *specT)
 {
     extern __shared__ double lds[];
-    if (sc && sc->p_done) return;
+    if (w.chb == 0.0 && sc && sc->p_done) return;
+    const double a_ch = (w.chb > 0.0) ? 1.0 / (sc->dt * c_BET[w.chstage]) : 0.0;
     const int logM = F.logmy, M = 1 << logM, logN = logM + 1, N = 2 * M, R = w.rdct, NX = g.NXM, RM = R * M;
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wy);
@@ -338,8 +340,19 @@ __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_sca
             double Ck = c * V.x + s * V.y;
             double Sk = c * V.y - s * V.x;
             double CNk = -Sk;
-            double gk = (kv == 0 && i == 0) ? 0.0 : -1.0 / (lx + F.lamy[kv]);
-            double gN = (kv == 0) ? 0.0 : -1.0 / (lx + F.lamy[N - kv]);
+            double gk, gN;
+            if (w.chb > 0.0) {          // (a_dt + bih lambda^2)^-1, no null mode
+                const double lk = lx + F.lamy[kv];
+                gk = 1.0 / (a_ch + w.chb * lk * lk);
+                gN = 0.0;
+                if (kv != 0) {
+                    const double lN = lx + F.lamy[N - kv];
+                    gN = 1.0 / (a_ch + w.chb * lN * lN);
+                }
+            } else {
+                gk = (kv == 0 && i == 0) ? 0.0 : -1.0 / (lx + F.lamy[kv]);
+                gN = (kv == 0) ? 0.0 : -1.0 / (lx + F.lamy[N - kv]);
+            }
             double Cp = Ck * gk, CNp = (kv == 0) ? 0.0 : CNk * gN;
             Xp[h].x = c * Cp + s * CNp;
             Xp[h].y = s * Cp - c * CNp;
@@ -387,7 +400,7 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
 {
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
-    if (sc->p_done) return;
+    if (w.chb == 0.0 && sc->p_done) return;
     const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);

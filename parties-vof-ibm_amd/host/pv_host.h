@@ -91,6 +91,7 @@ struct pv_bag {
     float *mgfmem;
     int mg_ok, mg_chunk;
     int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
+    int ch_spectral;               /* 1: biharmonic CG started from the exact spectral solve */
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
     int net_losses, net_skip;      /* consecutive losses of the network's start; solves left to skip it */
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
@@ -122,6 +123,7 @@ int  pv_fft_create(pv_bag *b);
 int  pv_cg_create(pv_bag *b);
 int  pv_mg_create(pv_bag *b);
 int  pv_set_mg_precision(pv_bag *b, int fp32);
+int  pv_set_ch_solver(pv_bag *b, int spectral);
 void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
 int  pv_learn_snapshot(pv_bag *b, int si);

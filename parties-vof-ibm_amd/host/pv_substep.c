@@ -67,14 +67,22 @@ static int ch_cg_loop(pv_bag *b, int stage)
     return PV_OK;
 }
 
-/* VOF_DIFFUSE_step (VOF_DIFFUSE.c:1262-1359) + VOF_update_density_viscosity */
+static void ch_spectral_start(pv_bag *b, int st);
+
+/* VOF_DIFFUSE_step (VOF_DIFFUSE.c:1262-1359) + VOF_update_density_viscosity.
+ * With ch_spectral the CG starts from C + (a_dt + bih L^2)^-1 (b - A C),
+ * the exact solution on the uniform grid (the CH operator is diagonal in the
+ * FFT-x / DCT-y basis of the pressure preconditioner); the CG's stop test
+ * then accepts it at iteration 0, or iterates on if it is not exact. */
 static int ch_step(pv_bag *b)
 {
-    int st = b->stage;
+    int st = b->stage, spec = b->ch_spectral && b->fft.ok;
     pv_tbegin(b, PV_T_CH_RHS);
     pvk_ccg_init(G(b), &b->ph, st, b->d_sc, &b->ccg, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
-                 FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), S(b));
+                 FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR),
+                 spec ? FD(b, PV_FI_TMP) : NULL, S(b));
     pv_tend(b, PV_T_CH_RHS);
+    if (spec) ch_spectral_start(b, st);
     LAUNCHCHK();
     RC(ch_cg_loop(b, st));
     pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
@@ -149,7 +157,7 @@ int pv_VOF_DIFFUSE_solve_implicit_biharmonic(pv_bag *b, int rhs, int *iters, dou
     pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
     pvk_ch_sub_bih(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, rhs), S(b));
     pvk_ccg_init(G(b), &b->ph, b->stage, b->d_sc, &b->ccg, FD(b, PV_F_C_L), NULL, NULL, NULL, FD(b, rhs),
-                 FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), S(b));
+                 FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), NULL, S(b));
     LAUNCHCHK();
     RC(ch_cg_loop(b, b->stage));
     if (iters) *iters = b->h_sc->ch_iter;
@@ -742,6 +750,23 @@ static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit)
     w.r = src;
     pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 1, -3, tol, maxit, S(b));
     pcg_iteration_w(b, &w, -3, tol, maxit);
+}
+
+/* CH spectral start: C += (a_dt + bih L^2)^-1 r with r = b - A C (the FFT
+ * stages of the pressure PCG with the CH divide), then r = b - A C again from
+ * the kept b (PV_FI_TMP) for the CG's stop test */
+static void ch_spectral_start(pv_bag *b, int st)
+{
+    pvk_pcg w = b->pcg;
+    w.r = FD(b, PV_F_CH_RES);
+    w.chb = b->ph.bih;
+    w.chstage = st;
+    pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 1, -3, 0.0, 1, S(b));
+    pcg_iteration_w(b, &w, -3, 0.0, 1);
+    pvk_learn_axpy(G(b), w.z, FD(b, PV_F_C_L), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_ccg_init(G(b), &b->ph, st, b->d_sc, &b->ccg, FD(b, PV_F_C_L), NULL, NULL, NULL, FD(b, PV_FI_TMP),
+                 FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR), NULL, S(b));
 }
 
 /* warm restart of the FFT-PCG from the current phi: r = rhs - mean - A phi */
