@@ -78,29 +78,30 @@ __device__ void mg_stop_test(pvk_scal *sc, const pvk_pcg &w, int n, double tol, 
 
 // ---------------------------------------------------------------------------
 // Fused smoothing of one level (levels with more than PVK_MG_LDS_CELLS cells).
-// A block owns a core tile of TX x MG_TY cells (TX = 64, or 32 on levels at
-// most 256 wide, for more blocks) and works on it extended by a halo of 2 nu
+// A block owns a core tile of TX x TY cells (64 x 16 on levels wider than
+// 512, 32 x 16 up to 512, 32 x 8 up to 256: smaller tiles, more blocks, on
+// the levels whose time is latency) and works on it extended by a halo of 2 nu
 // cells (periodic in x, cut at the walls in y): half-sweep k updates the
 // cells of its colour at distance >= k from the extended edge, so after
 // 2 nu half-sweeps the core (and the ring the residual needs) holds exactly
 // what sweeping the whole level would give.  f, the face coefficients and
 // 1 / (their sum) stay in registers, x in LDS.
 // ---------------------------------------------------------------------------
-#define MG_TY 16
-
-template <int NU, int TX> struct MgTile {
+template <int NU, int SH> struct MgTile {
+    static constexpr int TX = (SH == 0) ? 64 : 32;    // core tile
+    static constexpr int TY = (SH == 2) ? 8 : 16;
     static constexpr int H = 2 * NU;                  // halo
     static constexpr int W = TX + 2 * H;              // extended width
-    static constexpr int HT = MG_TY + 2 * H;          // extended height
+    static constexpr int HT = TY + 2 * H;          // extended height
     static constexpr int NC = W * HT;
     static constexpr int K = (NC + NTH - 1) / NTH;    // cells per thread
 };
 
-template <int NU, int TX, typename T>
+template <int NU, int SH, typename T>
 __device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, int x0, int y0, T *f, T *ce,
                                             T *cw, T *cn, T *cs, T *id)
 {
-    using TT = MgTile<NU, TX>;
+    using TT = MgTile<NU, SH>;
     const int p = L.pitch, nx = L.nx, ny = L.ny;
 #pragma unroll
     for (int u = 0; u < TT::K; u++) {
@@ -129,11 +130,11 @@ __device__ __forceinline__ void tile_of(int &bx, int &by)
 }
 
 // half-sweep k of colour col on the cells at distance >= k from the edge
-template <int NU, int TX, typename T>
+template <int NU, int SH, typename T>
 __device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k, int col, const T *f, const T *ce,
                                            const T *cw, const T *cn, const T *cs, const T *id)
 {
-    using TT = MgTile<NU, TX>;
+    using TT = MgTile<NU, SH>;
     T v[TT::K];
     bool act[TT::K];
 #pragma unroll
@@ -157,27 +158,27 @@ __device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k,
 
 // pre-smoothing from x = 0 and restriction of the residual; the stop test of
 // iteration n in block (0, 0) of level 0 (check)
-template <int NU, int TX, typename T>
+template <int NU, int SH, typename T>
 __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, int p0, pvk_scal *sc,
                                                pvk_pcg w, int n, double tol, int maxit, int check)
 {
-    using TT = MgTile<NU, TX>;
+    using TT = MgTile<NU, SH>;
     __shared__ T xs[TT::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
     int bx, by;
     tile_of(bx, by);
-    const int x0 = bx * TX - TT::H, y0 = by * MG_TY - TT::H;
+    const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
-    tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    tile_coeffs<NU, SH>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
     __syncthreads();
-    for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, TX>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
-    // restriction: one coarse cell per thread (TX/2 x MG_TY/2 <= NTH)
-    if ((int)threadIdx.x < (TX / 2) * (MG_TY / 2)) {
-        const int I = threadIdx.x % (TX / 2), J = threadIdx.x / (TX / 2);
-        const int gI = bx * (TX / 2) + I, gJ = by * (MG_TY / 2) + J;
+    for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, SH>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
+    // restriction: one coarse cell per thread (TT::TX/2 x TT::TY/2 <= NTH)
+    if ((int)threadIdx.x < (TT::TX / 2) * (TT::TY / 2)) {
+        const int I = threadIdx.x % (TT::TX / 2), J = threadIdx.x / (TT::TX / 2);
+        const int gI = bx * (TT::TX / 2) + I, gJ = by * (TT::TY / 2) + J;
         if (gI < C.nx && gJ < C.ny) {
             const int p = L.pitch, nx = L.nx, ny = L.ny;
             T s = T(0);
@@ -197,9 +198,9 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
         }
     }
     // the core of x for the post-smoothing
-    for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
-        const int ly = c / TX, lx = c - ly * TX;
-        const int gi = bx * TX + lx, gj = by * MG_TY + ly;
+    for (int c = threadIdx.x; c < TT::TX * TT::TY; c += NTH) {
+        const int ly = c / TT::TX, lx = c - ly * TT::TX;
+        const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
         if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
     }
 }
@@ -207,19 +208,19 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
 // coarse correction x += xc(parent) and post-smoothing (black first); the
 // result goes to x, or to the FP64 z0 on level 0 of the FP32 V-cycle.
 // dot: partial <f, x> of the core (FP64, with r0 on that level) into w.rz[slot]
-template <int NU, int TX, typename T>
+template <int NU, int SH, typename T>
 __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const double *r0, double *z0, int p0,
                                                 const pvk_scal *sc, pvk_pcg w, int dot, int slot)
 {
-    using TT = MgTile<NU, TX>;
+    using TT = MgTile<NU, SH>;
     __shared__ T xs[TT::NC];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     int bx, by;
     tile_of(bx, by);
-    const int x0 = bx * TX - TT::H, y0 = by * MG_TY - TT::H;
+    const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
-    tile_coeffs<NU, TX>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    tile_coeffs<NU, SH>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) {
         const int ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
         T v = T(0);
@@ -231,11 +232,11 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
         xs[c] = v;
     }
     __syncthreads();
-    for (int k = 1; k <= 2 * NU; k++) tile_sweep<NU, TX>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
+    for (int k = 1; k <= 2 * NU; k++) tile_sweep<NU, SH>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
     double acc = 0.0;
-    for (int c = threadIdx.x; c < TX * MG_TY; c += NTH) {
-        const int ly = c / TX, lx = c - ly * TX;
-        const int gi = bx * TX + lx, gj = by * MG_TY + ly;
+    for (int c = threadIdx.x; c < TT::TX * TT::TY; c += NTH) {
+        const int ly = c / TT::TX, lx = c - ly * TT::TX;
+        const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
         if (gi < L.nx && gj < L.ny) {
             const T v = xs[(ly + TT::H) * TT::W + lx + TT::H];
             if (z0) z0[(ptrdiff_t)gj * p0 + gi] = (double)v;
@@ -328,9 +329,18 @@ template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B
 
 template <typename T> struct LdsLev {
     int nx, ny, fp;         // cells; pitch of bx, by
+    int lnx;                // log2(nx) if nx is a power of two, else -1
     T *x, *f;               // LDS, compact
     const T *bx, *by;       // LDS (pitch nx) or the level's arrays
 };
+
+// q / n and q % n, by shift and mask when n = 2^l (l >= 0)
+__device__ __forceinline__ void divmod(int q, int n, int l, int &d, int &r)
+{
+    d = (l >= 0) ? (q >> l) : q / n;
+    r = q - d * n;
+}
+__device__ __forceinline__ int ilog2_or_neg(int n) { return (n > 0 && (n & (n - 1)) == 0) ? __ffs(n) - 1 : -1; }
 
 template <typename T>
 __device__ __forceinline__ T lgs(const LdsLev<T> &L, int i, int j, T xe, T xw, T xn, T xs)
@@ -346,9 +356,11 @@ __device__ __forceinline__ T lgs(const LdsLev<T> &L, int i, int j, T xe, T xw, T
 // post-sweep with the red neighbours read as x + xc(parent)
 template <typename T> __device__ void lds_sweep(const LdsLev<T> &L, const LdsLev<T> &C, int col, int mode)
 {
-    const int hx = L.nx >> 1, npair = hx * L.ny, nx = L.nx;
+    const int hx = L.nx >> 1, npair = hx * L.ny, nx = L.nx, lhx = (L.lnx > 0) ? L.lnx - 1 : -1;
     for (int q = threadIdx.x; q < npair; q += CT) {
-        const int j = q / hx, i = 2 * (q - j * hx) + ((j + col) & 1);
+        int j, ip;
+        divmod(q, hx, lhx, j, ip);
+        const int i = 2 * ip + ((j + col) & 1);
         T v;
         if (mode == 0) {
             v = lgs(L, i, j, T(0), T(0), T(0), T(0));
@@ -384,7 +396,8 @@ template <typename T> __device__ __forceinline__ T lds_ax(const LdsLev<T> &L, in
 template <typename T> __device__ void lds_restrict(const LdsLev<T> &L, const LdsLev<T> &C)
 {
     for (int q = threadIdx.x; q < C.nx * C.ny; q += CT) {
-        const int J = q / C.nx, I = q - J * C.nx;
+        int J, I;
+        divmod(q, C.nx, C.lnx, J, I);
         T s = T(0);
         for (int dj = 0; dj < 2; dj++)
             for (int di = 0; di < 2; di++) {
@@ -405,6 +418,7 @@ template <typename T> __device__ __forceinline__ LdsLev<T> lds_level(const pvk_m
     LdsLev<T> v;
     v.nx = m.L[l].nx;
     v.ny = m.L[l].ny;
+    v.lnx = ilog2_or_neg(v.nx);
     v.x = q;
     v.f = q + nc;
     if (l == m.c0 && !m.topf) {
@@ -431,7 +445,8 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         const Lev<T> G = lev<T>(m.L[l]);
         const LdsLev<T> Ll = lds_level(m, lds, l);
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
-            const int j = q / G.nx, i = q - j * G.nx;
+            int j, i;
+            divmod(q, G.nx, Ll.lnx, j, i);
             if (l == m.c0) Ll.f[q] = (l == 0 && m.r0) ? (T)m.r0[(ptrdiff_t)j * m.p0 + i] : ld(G.f, G.pitch, i, j);
             if (l != m.c0 || m.topf) {
                 ((T *)Ll.bx)[q] = ld(G.bx, G.pitch, i, j);
@@ -490,7 +505,8 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         const Lev<T> G = lev<T>(m.L[m.c0]);
         const LdsLev<T> top = lds_level(m, lds, m.c0);
         for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
-            const int j = q / G.nx, i = q - j * G.nx;
+            int j, i;
+            divmod(q, G.nx, top.lnx, j, i);
             if (m.c0 == 0 && m.z0) m.z0[(ptrdiff_t)j * m.p0 + i] = (double)top.x[q];
             else G.x[(ptrdiff_t)j * G.pitch + i] = top.x[q];
         }
@@ -560,12 +576,12 @@ __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *s
 // ---------------------------------------------------------------------------
 // launchers
 // ---------------------------------------------------------------------------
-// tile width by level: 64 cells, 32 on levels at most 256 wide (more blocks)
-static int tile_tx(const pvk_mglev &L) { return (L.nx > 256) ? 64 : 32; }
+// tile shape by level width: 0 = 64 x 16, 1 = 32 x 16, 2 = 32 x 8
+static int tile_shape(const pvk_mglev &L) { return (L.nx > 512) ? 0 : (L.nx > 256 ? 1 : 2); }
 static dim3 tile_grid(const pvk_mglev &L)
 {
-    const int tx = tile_tx(L);
-    return dim3((L.nx + tx - 1) / tx, (L.ny + MG_TY - 1) / MG_TY);
+    const int sh = tile_shape(L), tx = sh ? 32 : 64, ty = (sh == 2) ? 8 : 16;
+    return dim3((L.nx + tx - 1) / tx, (L.ny + ty - 1) / ty);
 }
 static dim3 cell_grid(const pvk_mglev &L) { return dim3((L.nx + TXP - 1) / TXP, (L.ny + TYR - 1) / TYR); }
 
@@ -618,6 +634,13 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *b
 // w->rz[(n < 0) ? 0 : (n + 1) & 1].
 static int rz_slot(int n) { return (n < 0) ? 0 : ((n + 1) & 1); }
 
+#define MG_SHAPES(KERNEL, NU, ...)                                                                         \
+    do {                                                                                                   \
+        if (sh == 0) hipLaunchKernelGGL((KERNEL<NU, 0, T>), gr, dim3(NTH), 0, s, __VA_ARGS__);             \
+        else if (sh == 1) hipLaunchKernelGGL((KERNEL<NU, 1, T>), gr, dim3(NTH), 0, s, __VA_ARGS__);        \
+        else hipLaunchKernelGGL((KERNEL<NU, 2, T>), gr, dim3(NTH), 0, s, __VA_ARGS__);                     \
+    } while (0)
+
 template <typename T>
 static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                   hipStream_t s)
@@ -625,15 +648,9 @@ static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n,
     const dim3 gr = tile_grid(m->L[l]);
     const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
     const double *r0 = (l == 0) ? m->r0 : nullptr;
-    const int wide = tile_tx(m->L[l]) == 64, chk = (l == 0);
-    if (m->nu == 1 && wide)
-        hipLaunchKernelGGL((k_mg_pre<1, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
-    else if (m->nu == 1)
-        hipLaunchKernelGGL((k_mg_pre<1, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
-    else if (wide)
-        hipLaunchKernelGGL((k_mg_pre<2, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
-    else
-        hipLaunchKernelGGL((k_mg_pre<2, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    const int sh = tile_shape(m->L[l]), chk = (l == 0);
+    if (m->nu == 1) MG_SHAPES(k_mg_pre, 1, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    else MG_SHAPES(k_mg_pre, 2, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
 }
 
 template <typename T> static void post_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
@@ -643,15 +660,9 @@ template <typename T> static void post_t(const pvk_mg *m, int l, pvk_scal *sc, c
     const double *r0 = (l == 0) ? m->r0 : nullptr;
     double *z0 = (l == 0) ? m->z0 : nullptr;
     const pvk_scal *cs = sc;
-    const int wide = tile_tx(m->L[l]) == 64, slot = rz_slot(n), dot = (l == 0);
-    if (m->nu == 1 && wide)
-        hipLaunchKernelGGL((k_mg_post<1, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
-    else if (m->nu == 1)
-        hipLaunchKernelGGL((k_mg_post<1, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
-    else if (wide)
-        hipLaunchKernelGGL((k_mg_post<2, 64, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
-    else
-        hipLaunchKernelGGL((k_mg_post<2, 32, T>), gr, dim3(NTH), 0, s, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+    const int sh = tile_shape(m->L[l]), slot = rz_slot(n), dot = (l == 0);
+    if (m->nu == 1) MG_SHAPES(k_mg_post, 1, L, C, r0, z0, m->p0, cs, *w, dot, slot);
+    else MG_SHAPES(k_mg_post, 2, L, C, r0, z0, m->p0, cs, *w, dot, slot);
 }
 
 extern "C" void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
