@@ -466,27 +466,29 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         }
         lds_restrict(Ll, Lc);
     }
-    // bottom: x = -P G P f, P the projection to zero mean (in FP64)
+    // bottom: x = -P G P f, P the projection to zero mean (in FP64); the
+    // means in the first wavefront, row r of G by the 16 lanes 16 r .. 16 r + 15
     {
         const LdsLev<T> B = lds_level(m, lds, nb);
-        const int n = B.nx * B.ny;
-        if (threadIdx.x == 0) {
+        const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
+        if (threadIdx.x < 64) {
             double s = 0.0;
-            for (int k = 0; k < n; k++) s += (double)B.f[k];
-            mean = s / n;
+            for (int k = threadIdx.x; k < n; k += 64) s += (double)B.f[k];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (threadIdx.x == 0) mean = s / n;
         }
         __syncthreads();
-        if ((int)threadIdx.x < n) {
-            const double *g = m.G + (size_t)threadIdx.x * n;
-            double v = 0.0;
-            for (int k = 0; k < n; k++) v -= g[k] * ((double)B.f[k] - mean);
-            vbuf[threadIdx.x] = v;
-        }
+        double v = 0.0;
+        if (r < n)
+            for (int c = q; c < n; c += 16) v -= m.G[(size_t)r * n + c] * ((double)B.f[c] - mean);
+        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+        if (q == 0 && r < n) vbuf[r] = v;
         __syncthreads();
-        if (threadIdx.x == 0) {
+        if (threadIdx.x < 64) {
             double s = 0.0;
-            for (int k = 0; k < n; k++) s += vbuf[k];
-            mean = s / n;
+            for (int k = threadIdx.x; k < n; k += 64) s += vbuf[k];
+            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+            if (threadIdx.x == 0) mean = s / n;
         }
         __syncthreads();
         if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
