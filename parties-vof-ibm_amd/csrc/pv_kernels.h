@@ -81,6 +81,8 @@ typedef struct pvk_pcg {
     double *specT;                    /* NXM * NYM transposed spectrum */
     double *dad[2], *rr, *rz[2], *mean;  /* per-block partial sums; dad by iteration parity */
     double *zad;                      /* <A d_n, z_{n+1}> partials of the flexible (learned) form */
+    double *sad;                      /* non-NULL: partials of sum(A d), one per row, so the residual
+                                         update can keep r at zero mean (MG-PCG) */
     int ndad, nrr, nrz, nmean;        /* partial counts */
     int flex;                         /* 1: flexible CG, beta = -<z_{n+1}, A d_n> / <d_n, A d_n> */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
@@ -246,7 +248,10 @@ void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hip
 #define PVK_MG_BOTTOM_MIN 64
 typedef struct pvk_mglev {
     int nx, ny, pitch;
-    void *x, *f, *bx, *by;
+    void *x, *f, *bx, *by;            /* result x, right-hand side, faces */
+    void *xp;                         /* tiled levels: x after the pre-smoothing, kept apart from x so
+                                         that no block of the post kernel reads, as its halo, cells
+                                         another block of the same launch is overwriting */
 } pvk_mglev;
 typedef struct pvk_mg {
     int nlev, c0, nu, fp32;           /* levels, first in-LDS level, sweeps per side, FP32 V-cycle */

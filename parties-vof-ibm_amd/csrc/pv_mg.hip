@@ -34,7 +34,7 @@
 // a level with arrays of the V-cycle's type
 template <typename T> struct Lev {
     int nx, ny, pitch;
-    T *x, *f, *bx, *by;
+    T *x, *f, *bx, *by, *xp;
 };
 template <typename T> __host__ __device__ static Lev<T> lev(const pvk_mglev &l)
 {
@@ -46,6 +46,7 @@ template <typename T> __host__ __device__ static Lev<T> lev(const pvk_mglev &l)
     v.f = (T *)l.f;
     v.bx = (T *)l.bx;
     v.by = (T *)l.by;
+    v.xp = (T *)l.xp;
     return v;
 }
 
@@ -197,16 +198,21 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
             C.f[(ptrdiff_t)gJ * C.pitch + gI] = s;
         }
     }
-    // the core of x for the post-smoothing
+    // the core of x for the post-smoothing (to xp: the post kernel reads it
+    // with halos while it writes x)
     for (int c = threadIdx.x; c < TT::TX * TT::TY; c += NTH) {
         const int ly = c / TT::TX, lx = c - ly * TT::TX;
         const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
-        if (gi < L.nx && gj < L.ny) L.x[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
+        if (gi < L.nx && gj < L.ny) L.xp[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
     }
 }
 
-// coarse correction x += xc(parent) and post-smoothing (black first); the
-// result goes to x, or to the FP64 z0 on level 0 of the FP32 V-cycle.
+// coarse correction xp + xc(parent) and post-smoothing (black first); the
+// result goes to x, or to the FP64 z0 on level 0 of the FP32 V-cycle.  The
+// input (xp) and the output are separate arrays: a block reads its halo from
+// the neighbouring tiles' cores, which their blocks overwrite, and when the
+// level has more tiles than the GPU holds at once those blocks may already
+// have finished.
 // dot: partial <f, x> of the core (FP64, with r0 on that level) into w.rz[slot]
 template <int NU, int SH, typename T>
 __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const double *r0, double *z0, int p0,
@@ -227,7 +233,7 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
         if (gj >= 0 && gj < L.ny) {
             int gi = (x0 + lx) % L.nx;
             if (gi < 0) gi += L.nx;
-            v = ld(L.x, L.pitch, gi, gj) + ld(C.x, C.pitch, gi >> 1, gj >> 1);
+            v = ld(L.xp, L.pitch, gi, gj) + ld(C.x, C.pitch, gi >> 1, gj >> 1);
         }
         xs[c] = v;
     }
@@ -538,17 +544,26 @@ __global__ void __launch_bounds__(NTH) k_mg_dot(const pvk_scal *sc, pvk_pcg w, p
 // ---------------------------------------------------------------------------
 // CG vector kernels, one block per row
 // ---------------------------------------------------------------------------
-// start: r = rhs - mean (phi = 0), or with warm != 0 r = rhs - mean - Ad
+// The pressure operator is singular (constants are its null space) and the
+// V-cycle, unlike the FFT solve, does not drop the mean of its input: a mean
+// left in r by rounding in A d is a part of the system no iteration can
+// reduce, and once ||r|| comes down to it the CG stalls.  So the residual is
+// kept at zero mean: every A d is used less its mean (sum(A d) is zero in
+// exact arithmetic; the partials come from the kernel that forms A d).
+//
+// start: r = rhs - mean (phi = 0), or with warm != 0 r = rhs - mean - (Ad - mean(Ad))
 // (Ad = A phi0, phi kept); partial <r, r>
 __global__ void __launch_bounds__(NTH) k_mg_start(pvk_grid g, const pvk_scal *sc, pvk_pcg w, int warm)
 {
     __shared__ double sh[NTH];
     const int P = g.P, j = xcd_band(blockIdx.x, gridDim.x);
-    const double mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
+    const double ncell = (double)g.NXM * (double)g.NYM;
+    const double mean = block_sum(w.mean, w.nmean, sh) / ncell;
+    const double mad = (warm && w.sad) ? block_sum(w.sad, w.ndad, sh) / ncell : 0.0;
     double acc = 0.0;
     for (int i = threadIdx.x; i < g.NXM; i += NTH) {
         double r = AT(w.rhs, i, j) - mean;
-        if (warm) r -= AT(w.Ad, i, j);
+        if (warm) r -= AT(w.Ad, i, j) - mad;
         else AT(w.phi, i, j) = 0.0;
         AT(w.r, i, j) = r;
         acc += r * r;
@@ -557,18 +572,20 @@ __global__ void __launch_bounds__(NTH) k_mg_start(pvk_grid g, const pvk_scal *sc
     (void)sc;
 }
 
-// alpha = <r, z> / <d, A d>; phi += alpha d; r -= alpha A d; partial <r, r>
+// alpha = <r, z> / <d, A d>; phi += alpha d; r -= alpha (A d - mean(A d));
+// partial <r, r>
 __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *sc, pvk_pcg w, int n)
 {
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     const int P = g.P, j = xcd_band(blockIdx.x, gridDim.x);
     const double alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
+    const double mad = w.sad ? block_sum(w.sad, w.ndad, sh) / ((double)g.NXM * (double)g.NYM) : 0.0;
     const double *d = w.d[n & 1];
     double acc = 0.0;
     for (int i = threadIdx.x; i < g.NXM; i += NTH) {
         AT(w.phi, i, j) += alpha * AT(d, i, j);
-        const double r = AT(w.r, i, j) - alpha * AT(w.Ad, i, j);
+        const double r = AT(w.r, i, j) - alpha * (AT(w.Ad, i, j) - mad);
         AT(w.r, i, j) = r;
         acc += r * r;
     }

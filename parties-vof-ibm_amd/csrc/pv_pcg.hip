@@ -125,7 +125,7 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
     double *__restrict__ Ad = w.Ad;
     const int j = xcd_band(blockIdx.x, gridDim.x);
     const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
-    double acc = 0.0;
+    double acc = 0.0, sacc = 0.0;
     for (int base = 0; base < NXM; base += NTH * CPT) {
         double dc[CPT], ad[CPT];
 #pragma unroll
@@ -150,10 +150,12 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
                 AT(dnew, i, j) = dc[u];
                 AT(Ad, i, j) = ad[u];
                 acc += dc[u] * ad[u];
+                sacc += ad[u];
             }
         }
     }
     block_store_partial(acc, w.dad[n & 1], j, sh);
+    if (w.sad) block_store_partial(sacc, w.sad, j, sh);
 }
 
 // ---------------------------------------------------------------------------
@@ -468,14 +470,19 @@ extern "C" void pvk_pcg_gamma(pvk_scal *sc, const pvk_pcg *w, int rzs, int slot,
 __global__ void __launch_bounds__(NTH) k_pcg_opx(pvk_grid g, pvk_pcg w, const double *__restrict__ x,
                                                 double *__restrict__ out)
 {
+    __shared__ double sh[NTH];
     const int P = g.P, NXM = g.NXM, NYM = g.NYM, j = xcd_band(blockIdx.x, gridDim.x);
     const int jn = (j + 1 == NYM) ? j : j + 1, js = (j == 0) ? 0 : j - 1;
+    double sacc = 0.0;
     for (int i = threadIdx.x; i < NXM; i += NTH) {
         const int ie = (i + 1 == NXM) ? 0 : i + 1, iw = (i == 0) ? NXM - 1 : i - 1;
         const double d0 = AT(x, i, j);
-        AT(out, i, j) = AT(w.bx, i, j) * (AT(x, ie, j) - d0) - AT(w.bx, i - 1, j) * (d0 - AT(x, iw, j)) +
-                        AT(w.by, i, j) * (AT(x, i, jn) - d0) - AT(w.by, i, j - 1) * (d0 - AT(x, i, js));
+        const double v = AT(w.bx, i, j) * (AT(x, ie, j) - d0) - AT(w.bx, i - 1, j) * (d0 - AT(x, iw, j)) +
+                         AT(w.by, i, j) * (AT(x, i, jn) - d0) - AT(w.by, i, j - 1) * (d0 - AT(x, i, js));
+        AT(out, i, j) = v;
+        sacc += v;
     }
+    if (w.sad) block_store_partial(sacc, w.sad, j, sh);
 }
 extern "C" void pvk_pcg_opx(const pvk_grid *g, const pvk_pcg *w, const double *x, double *out, hipStream_t s)
 {

@@ -12,9 +12,11 @@
  * that way has no MG-PCG (mg_ok = 0).
  *
  * Two hierarchies of the same sizes are kept: mg (FP64 V-cycle) and mgf
- * (FP32 V-cycle inside the FP64 CG; its level 0 keeps FP32 copies of x and
- * the faces and reads r / writes z in FP64).  Handles of mixed precision use
- * mgf unless pv_set_mg_precision says otherwise.
+ * (FP32 V-cycle inside the FP64 CG; its level 0 keeps FP32 copies of the
+ * pre-smoothed x and the faces and reads r / writes z in FP64).  Handles of
+ * mixed precision use mgf unless pv_set_mg_precision says otherwise.  Every
+ * level has a pre-smoothed x (xp) apart from its result x.  Level 0's pitch is
+ * the field pitch P, the others' their width.
  */
 #include "pv_host.h"
 #include <stdlib.h>
@@ -65,8 +67,9 @@ int pv_mg_create(pv_bag *b)
     }
     for (l = 1; l < nl; l++) cells += (size_t)lx[l] * ly[l];
     nrz = pvk_mg_nrz(m);
-    total = 4 * cells + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 4 * (size_t)ny + 2 * (size_t)nrz;
-    totalf = 4 * cells + 3 * nc0;
+    total = 5 * cells + (size_t)b->g.P * ny + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 5 * (size_t)ny +
+            2 * (size_t)nrz;
+    totalf = 5 * cells + 3 * nc0;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
         return pv_set_error(PV_ENOMEM, "MG-PCG work allocation failed");
@@ -82,18 +85,21 @@ int pv_mg_create(pv_bag *b)
     m->L[0].f = b->f[PV_F_P_RES];
     m->L[0].bx = b->f[PV_FI_PBX];
     m->L[0].by = b->f[PV_FI_PBY];
+    m->L[0].xp = q; q += (size_t)b->g.P * ny;
     for (l = 1; l < nl; l++) {
         size_t n = (size_t)lx[l] * ly[l];
         m->L[l].x = q; q += n;
         m->L[l].f = q; q += n;
         m->L[l].bx = q; q += n;
         m->L[l].by = q; q += n;
+        m->L[l].xp = q; q += n;
     }
     m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     m->p0 = b->g.P;
     /* FP32 V-cycle: level 0 has its own x and face copies; r in, z out */
     qf = b->mgfmem;
-    mf->L[0].x = qf; qf += nc0;
+    mf->L[0].x = NULL;                 /* the result goes to z0 */
+    mf->L[0].xp = qf; qf += nc0;
     mf->L[0].f = NULL;
     mf->L[0].bx = qf; qf += nc0;
     mf->L[0].by = qf; qf += nc0;
@@ -103,6 +109,7 @@ int pv_mg_create(pv_bag *b)
         mf->L[l].f = qf; qf += n;
         mf->L[l].bx = qf; qf += n;
         mf->L[l].by = qf; qf += n;
+        mf->L[l].xp = qf; qf += n;
     }
     mf->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     mf->r0 = b->f[PV_F_P_RES];
@@ -124,6 +131,7 @@ int pv_mg_create(pv_bag *b)
         w->dad[1] = q; q += ny;
         w->rr = q; q += ny;
         w->mean = q; q += ny;
+        w->sad = q; q += ny;
         w->rz[0] = q; q += nrz;
         w->rz[1] = q; q += nrz;
         w->ndad = ny;
