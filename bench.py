@@ -211,6 +211,55 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     return out
 
 
+def mixed_error(lib, args):
+    """Per-field error of the fast mixed form (FP32 stencils and V-cycle, FP64
+    fluxes / solves / dots, spectral CH start, temporal pressure start) against
+    the FP64 path (FP64 stencils and V-cycle, the reference CH CG, cold
+    pressure starts), both MG-PCG, stepped side by side from the same seeded
+    field; errors at checkpoints up to --mixed-error-steps."""
+    n, steps = args.n, args.mixed_error_steps
+    if args.ic == "droplets":          # 32 liquid droplets of radius 0.02-0.06 in gas
+        kw = dict(nmin=32, nmax=32, rmin=0.02, rmax=0.06, gas_inside=0)
+    else:
+        kw = dict(nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
+    runs, secs = {}, {}
+    for name, prec in (("fp64", pvof.PV_PREC_FP64), ("mixed", pvof.PV_PREC_MIXED)):
+        p = workload_params(n, pvof.PV_PSOLVE_MG_PCG, prec)
+        F = pvof.ic(lib, "bubbles", p, seed=SEED, **kw)
+        s = pvof.Solver(p, lib, device=0)
+        s.init_from_F(F)
+        if name == "mixed":
+            s.check(lib.pv_set_ch_solver(s.h, 1))
+            s.check(lib.pv_set_pressure_warm_start(s.h, 1))
+        runs[name], secs[name] = s, 0.0
+    R = (slice(3, 3 + n), slice(3, 3 + n))
+    marks = sorted({m for m in (10, 50, 100, 250, steps) if m <= steps})
+    checkpoints, done = [], 0
+    for m in marks:
+        for name, s in runs.items():
+            t0 = time.perf_counter()
+            for _ in range(m - done):
+                s.call("Temporal_int_rk3", 1)
+            s.call("synchronize")
+            secs[name] += time.perf_counter() - t0
+        done = m
+        fields = {}
+        for f in ("C_L", "U", "V", "P"):
+            a, b = runs["mixed"].get(f)[R], runs["fp64"].get(f)[R]
+            d = a - b
+            fields[f] = {"rel_L2": float(np.sqrt((d * d).sum() / max((b * b).sum(), 1e-300))),
+                         "rel_Linf": float(np.abs(d).max() / max(np.abs(b).max(), 1e-300))}
+        cl = [runs[k].get("C_L")[R].sum() for k in ("fp64", "mixed")]
+        t = [runs[k].stats()["time"] for k in ("fp64", "mixed")]
+        checkpoints.append({"steps": m, "fields": fields, "time": t,
+                            "phase_volume_rel_diff": float(abs(cl[1] - cl[0]) / abs(cl[0]))})
+        print(f"mixed_error: step {m}: " + ", ".join(f"{f} {v['rel_L2']:.2e}" for f, v in fields.items()),
+              file=sys.stderr, flush=True)
+    return {"n": n, "steps": steps, "ic": args.ic, "checkpoints": checkpoints,
+            "fp64_ms_per_step": round(1e3 * secs["fp64"] / steps, 2),
+            "mixed_ms_per_step": round(1e3 * secs["mixed"] / steps, 2)}
+
+
 def public(r):
     return {k: v for k, v in r.items() if not k.startswith("_") and k != "kernels"}
 
@@ -242,6 +291,10 @@ def main():
     ap.add_argument("--timer-period", type=int, default=16,
                     help="time every k-th launch of each kernel class with HIP events")
     ap.add_argument("--no-timers", action="store_true")
+    ap.add_argument("--mixed-error-steps", type=int, default=0,
+                    help="only report the per-field error of the mixed form against the FP64 path after this "
+                         "many steps at --n (with --ic)")
+    ap.add_argument("--ic", choices=["bubbles", "droplets"], default="bubbles")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="first copy this many MiB device to device three times (a kernel of known traffic, "
                          "to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in the same run)")
@@ -252,6 +305,9 @@ def main():
         sys.exit(2)
 
     lib = pvof.load_lib()
+    if args.mixed_error_steps > 0:
+        print(json.dumps({"mixed_error": mixed_error(lib, args)}))
+        return
     if args.copy_mib > 0:
         import torch
         src = torch.ones(args.copy_mib * (1 << 20) // 8, dtype=torch.float64, device="cuda")
