@@ -69,10 +69,10 @@ BYTES_PER_CELL_MG32 = dict(BYTES_PER_CELL_MG, MG_PRE0=21, MG_POST0=29)
 BYTES_PER_ITER = {"fft": 176, "mg": 197, "mg32": 163, "jacobi": 112}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
-# the factors a 256 MiB device copy gives in the same run); the summary is
+# the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_POST0": 29.7, "MG_PRE0": 22.1, "P_APPLY": 48.3, "P_UPDATE": 48.3,
-                        "CH_APPLY": 26.7, "CH_UPDATE": 39.0, "MOM_APPLY": 82.2, "MOM_UPDATE": 86.1}
+TRAFFIC_MB_MG32_1024 = {"MG_POST0": 32.1, "MG_PRE0": 23.4, "P_APPLY": 51.5, "P_UPDATE": 51.2,
+                        "MOM_APPLY": 121.9, "MOM_UPDATE": 119.6, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 
 
@@ -260,6 +260,10 @@ def mixed_error(lib, args):
             "mixed_ms_per_step": round(1e3 * secs["mixed"] / steps, 2)}
 
 
+def note(msg):
+    print(f"bench.py: {msg}", file=sys.stderr, flush=True)
+
+
 def public(r):
     return {k: v for k, v in r.items() if not k.startswith("_") and k != "kernels"}
 
@@ -296,8 +300,8 @@ def main():
                          "many steps at --n (with --ic)")
     ap.add_argument("--ic", choices=["bubbles", "droplets"], default="bubbles")
     ap.add_argument("--copy-mib", type=int, default=0,
-                    help="first copy this many MiB device to device three times (a kernel of known traffic, "
-                         "to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in the same run)")
+                    help="only copy this many MiB device to device three times and exit: a kernel of known "
+                         "traffic, to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in a run of its own")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs one subdomain on one GPU; --gpus {args.gpus} is not supported "
@@ -315,27 +319,33 @@ def main():
         for _ in range(3):
             dst.copy_(src)
         torch.cuda.synchronize()
-        del src, dst
+        print(json.dumps({"copy_bytes_each_way": src.numel() * 8, "copies": 3}))
+        return
     prec = pvof.PV_PREC_MIXED if args.precision == "mixed" else pvof.PV_PREC_FP64
     learned = None
     if args.solver.startswith("learned"):
+        note("training the preconditioner network")
         learned = train_network(lib, args, prec)
     p0 = workload_params(args.n, SOLVERS[args.solver], prec)
     F = pvof.ic(lib, "bubbles", p0, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
     net = learned.pop("params") if learned is not None else None
     # the timed run records no events; a separate short run times the kernels
+    note(f"timed run: {args.solver}, {args.precision}, {args.steps} steps after {args.warmup}")
     main_run = run(lib, args, args.solver, args.precision, args.steps, args.warmup, F, net, timers=False)
     if not args.no_timers:
+        note(f"kernel timing run: {args.kernel_steps} steps")
         prof = run(lib, args, args.solver, args.precision, args.kernel_steps, 2, F, net, timers=True)
         main_run["kernels"], main_run["_bpc"] = prof["kernels"], prof["_bpc"]
     p = main_run["_params"]
     compare = {}
     if not args.no_compare:
+        note(f"plain FP64 FFT-PCG run: {args.plain_steps} steps")
         plain = run(lib, args, "fft", "fp64", args.plain_steps, 1, F, timers=False)
         compare["plain_fp64"] = public(plain)
         compare["speedup_vs_plain_fp64"] = round(main_run["value"] / plain["value"], 3)
         if args.solver.startswith("learned"):
             base = {"learned_mg": "mg", "learned": "fft", "learned_fcg": "fft"}[args.solver]
+            note(f"ablation run without the network: {base}")
             abl = run(lib, args, base, args.precision, args.steps, args.warmup, F, timers=False)
             compare["ablation_no_network"] = public(abl)
             compare["network_gain"] = round(main_run["value"] / abl["value"], 3)
@@ -387,6 +397,7 @@ def main():
                            "bytes_per_cell": bpc[dom], "cells_per_launch": cells,
                            "avg_us": timed[dom]["avg_us"], "timer_period": args.timer_period}
     if not args.no_cpu_baseline:
+        note(f"CPU baseline at {args.cpu_n}^2")
         pc = workload_params(args.cpu_n, pvof.PV_PSOLVE_FFT_PCG)
         Fc = pvof.ic(lib, "bubbles", pc, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
         out["cpu_baseline"] = cpu_baseline(pc, Fc)
