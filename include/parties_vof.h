@@ -279,15 +279,13 @@ pv_bag *pv_batch_case(pv_batch *bt, int k);
 int pv_batch_rk3(pv_batch *bt, int nsteps);
 int pv_batch_autotune(pv_batch *bt, int *chunks);
 
-/* ---- learned pressure solve ----------------------------------------------
- * No counterpart in the reference.  Training data come from the plain
- * FFT-PCG solves of the handle's own time steps: while collecting, each
- * pressure solve stores at the given PCG iterations the features of
- * (r_k, M_fft r_k, rho) and the target of the network's correction, cut into
- * crop_size^2 crops (crops_per_snap per snapshot, at most max_samples). */
+/* ---- pressure and Cahn-Hilliard solver options ---------------------------
+ * No counterpart in the reference. */
+/* Pressure solver of the handle (PV_PSOLVE_*), as pv_params.pressure_solver. */
 int pv_set_pressure_solver(pv_bag *bag, int solver);
 /* Gauss-Seidel sweeps before and after the coarse correction of the
  * multigrid V-cycle (1 or 2, default 2). */
+int pv_set_mg_sweeps(pv_bag *bag, int nu);
 /* Start of the MG-PCG solves (PV_PSOLVE_MG_PCG, PV_PSOLVE_LEARNED_MG):
  * mode 0 = from phi = 0 (as the reference); mode 1 = also try the pressure
  * correction of the same RK3 stage of the previous step, kept only when the
@@ -297,21 +295,29 @@ int pv_set_pressure_warm_start(pv_bag *bag, int mode);
  * for PV_PREC_FP64 handles), 1 = FP32 (default for PV_PREC_MIXED handles).
  * The CG vectors, dots, stop test and solution stay FP64 either way. */
 int pv_set_mg_precision(pv_bag *bag, int fp32);
-/* Cahn-Hilliard implicit solve (no counterpart in the reference, which runs
- * the CG of VOF_DIFFUSE.c:318-429 from C): 0 = that CG (default); 1 = the CG
- * started from the exact solution of a_dt x + (Cn^2/Pe) L^2 x = b, which the
+/* Cahn-Hilliard implicit solve (the reference runs the CG of
+ * VOF_DIFFUSE.c:318-429 from C): 0 = that CG (default); 1 = the CG started
+ * from the exact solution of a_dt x + (Cn^2/Pe) L^2 x = b, which the
  * constant-coefficient operator allows on the uniform grid through the
  * FFT-x / DCT-y transforms of the pressure preconditioner (power-of-two
  * sizes); the CG's own stop test accepts it, normally at iteration 0. */
 int pv_set_ch_solver(pv_bag *bag, int spectral);
-int pv_set_mg_sweeps(pv_bag *bag, int nu);
+
+/* ---- learned pressure solve ----------------------------------------------
+ * No counterpart in the reference.  Training data come from the plain
+ * FFT-PCG solves of the handle's own time steps: while collecting, each
+ * pressure solve stores at the given PCG iterations the features of
+ * (r_k, M_fft r_k, rho) and the target of the network's correction, cut into
+ * crop_size^2 crops (crops_per_snap per snapshot, at most max_samples). */
 int pv_learn_collect_begin(pv_bag *bag, const int *snap_iters, int nsnap, int crops_per_snap, int crop_size,
                            int max_samples, unsigned long long seed);
 int pv_learn_collect_end(pv_bag *bag, int *nsamples);
 /* Adam on random batches of the collected crops; loss of the first and last batch */
 int pv_learn_train(pv_bag *bag, int steps, int batch, float lr, unsigned long long seed, double *loss_first,
                    double *loss_last);
-/* ||b - A phi0|| / ||b|| of the last learned warm start (>= 1: it was discarded) */
+/* Energy of the last MG-PCG start relative to the cold start's (learned MG
+ * solver), or ||b - A phi0|| / ||b|| of the last learned warm start (FFT
+ * forms); >= 1: the candidate was discarded */
 int pv_learn_start_residual(pv_bag *bag, double *res);
 
 /* ---- initial volume fraction F on the G range (host only, no device) ----
