@@ -246,6 +246,7 @@ void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hip
 #define PVK_MG_LDS_CELLS 4096
 #define PVK_MG_BOTTOM 64
 #define PVK_MG_BOTTOM_MIN 64
+#define PVK_MG_LOW 3                  /* most levels below the first in-LDS level */
 typedef struct pvk_mglev {
     int nx, ny, pitch;
     void *x, *f, *bx, *by;            /* result x, right-hand side, faces */

@@ -329,9 +329,15 @@ template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B
 // ---------------------------------------------------------------------------
 // Levels c0 .. nlev-1 in one workgroup of 1024 threads.  x and f of every
 // level live in LDS, and the faces of every level below c0; those of level
-// c0 are copied too when they fit (topf), else read from its arrays.
+// c0 are copied too when they fit (topf), else read from its arrays.  The
+// kernel runs on one CU, so its time is instruction issue: each thread reads
+// the coefficients and neighbour indices of its cells once per level and
+// pass into registers, and a Gauss-Seidel update is then four LDS reads, the
+// weighted sum and one store.
 // ---------------------------------------------------------------------------
 #define CT 1024
+#define MG_LOW PVK_MG_LOW   // 4096 -> 1024 -> 256 -> 64 cells
+static_assert(PVK_MG_LDS_CELLS <= 4 * CT, "level c0 is loaded as at most 4 cells a thread");
 
 template <typename T> struct LdsLev {
     int nx, ny, fp;         // cells; pitch of bx, by
@@ -348,71 +354,143 @@ __device__ __forceinline__ void divmod(int q, int n, int l, int &d, int &r)
 }
 __device__ __forceinline__ int ilog2_or_neg(int n) { return (n > 0 && (n & (n - 1)) == 0) ? __ffs(n) - 1 : -1; }
 
-template <typename T>
-__device__ __forceinline__ T lgs(const LdsLev<T> &L, int i, int j, T xe, T xw, T xn, T xs)
-{
-    const int nx = L.nx, iw = (i == 0) ? nx - 1 : i - 1, fp = L.fp;
-    const T ce = L.bx[j * fp + i], cw = L.bx[j * fp + iw];
-    const T cn = (j + 1 < L.ny) ? L.by[j * fp + i] : T(0);
-    const T cs = (j > 0) ? L.by[(j - 1) * fp + i] : T(0);
-    return (ce * xe + cw * xw + cn * xn + cs * xs - L.f[j * nx + i]) / (ce + cw + cn + cs);
-}
+// a / b of the Gauss-Seidel update: FP64 divides; FP32 multiplies by the
+// hardware reciprocal (1 ulp), which keeps the V-cycle a fixed symmetric
+// operator (both sweep directions use the same value)
+template <typename T> __device__ __forceinline__ T gs_div(T a, T b) { return a / b; }
+template <> __device__ __forceinline__ float gs_div<float>(float a, float b) { return a * __builtin_amdgcn_rcpf(b); }
 
-// half-sweep of colour col; mode 0: from x = 0, 1: Gauss-Seidel, 2: first
-// post-sweep with the red neighbours read as x + xc(parent)
-template <typename T> __device__ void lds_sweep(const LdsLev<T> &L, const LdsLev<T> &C, int col, int mode)
-{
-    const int hx = L.nx >> 1, npair = hx * L.ny, nx = L.nx, lhx = (L.lnx > 0) ? L.lnx - 1 : -1;
-    for (int q = threadIdx.x; q < npair; q += CT) {
-        int j, ip;
-        divmod(q, hx, lhx, j, ip);
-        const int i = 2 * ip + ((j + col) & 1);
-        T v;
-        if (mode == 0) {
-            v = lgs(L, i, j, T(0), T(0), T(0), T(0));
-        } else {
-            const int ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-            T xe = L.x[j * nx + ie], xw = L.x[j * nx + iw];
-            T xn = (j + 1 < L.ny) ? L.x[(j + 1) * nx + i] : T(0);
-            T xs = (j > 0) ? L.x[(j - 1) * nx + i] : T(0);
-            if (mode == 2) {
-                const int cx = C.nx;
-                xe += C.x[(j >> 1) * cx + (ie >> 1)];
-                xw += C.x[(j >> 1) * cx + (iw >> 1)];
-                if (j + 1 < L.ny) xn += C.x[((j + 1) >> 1) * cx + (i >> 1)];
-                if (j > 0) xs += C.x[((j - 1) >> 1) * cx + (i >> 1)];
-            }
-            v = lgs(L, i, j, xe, xw, xn, xs);
-        }
-        L.x[j * nx + i] = v;
-    }
-    __syncthreads();
-}
+// A cell of an in-LDS level as one thread smooths it: the LDS indices of the
+// cell and of its four neighbours, the faces, 1 / (their sum) and f, read
+// once per level and pass and kept in registers for all the half-sweeps of
+// that pass.  A neighbour beyond a y wall has face 0 and is pointed at the
+// neighbour on the other side (a smoothing level has at least two rows):
+// a cell of the other colour, so already written when it is read (a cell of
+// the same colour may still hold stale LDS, and 0 * NaN is NaN).
+template <typename T> struct GsCell {
+    int s, e, w, n, so;
+    T ce, cw, cn, cs, id, f;
+};
 
-template <typename T> __device__ __forceinline__ T lds_ax(const LdsLev<T> &L, int i, int j)
+template <typename T> __device__ __forceinline__ void gs_cell(const LdsLev<T> &L, int i, int j, GsCell<T> &c)
 {
     const int nx = L.nx, fp = L.fp, ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
-    const T x0 = L.x[j * nx + i];
-    T v = L.bx[j * fp + i] * (L.x[j * nx + ie] - x0) - L.bx[j * fp + iw] * (x0 - L.x[j * nx + iw]);
-    if (j + 1 < L.ny) v += L.by[j * fp + i] * (L.x[(j + 1) * nx + i] - x0);
-    if (j > 0) v -= L.by[(j - 1) * fp + i] * (x0 - L.x[(j - 1) * nx + i]);
-    return v;
+    c.s = j * nx + i;
+    c.e = j * nx + ie;
+    c.w = j * nx + iw;
+    c.n = (j + 1 < L.ny) ? c.s + nx : c.s - nx;
+    c.so = (j > 0) ? c.s - nx : c.s + nx;
+    c.ce = L.bx[j * fp + i];
+    c.cw = L.bx[j * fp + iw];
+    c.cn = (j + 1 < L.ny) ? L.by[j * fp + i] : T(0);
+    c.cs = (j > 0) ? L.by[(j - 1) * fp + i] : T(0);
+    c.id = gs_div(T(1), c.ce + c.cw + c.cn + c.cs);
+    c.f = L.f[c.s];
 }
 
-template <typename T> __device__ void lds_restrict(const LdsLev<T> &L, const LdsLev<T> &C)
+// Gauss-Seidel update of the cell from its neighbours' current x
+template <typename T> __device__ __forceinline__ void gs_update(T *x, const GsCell<T> &c)
 {
+    x[c.s] = (c.ce * x[c.e] + c.cw * x[c.w] + c.cn * x[c.n] + c.cs * x[c.so] - c.f) * c.id;
+}
+
+// residual f - A x at the cell
+template <typename T> __device__ __forceinline__ T gs_res(const T *x, const GsCell<T> &c)
+{
+    const T x0 = x[c.s];
+    return c.f - (c.ce * (x[c.e] - x0) - c.cw * (x0 - x[c.w]) + c.cn * (x[c.n] - x0) - c.cs * (x0 - x[c.so]));
+}
+
+// The cells of in-LDS level L pair along x: pair p = (ip, j) holds cells
+// 2 ip and 2 ip + 1 of row j, one red ((i + j) even) and one black.  Thread
+// t owns pairs t and t + CT (a level has at most 2 CT pairs).
+template <typename T> struct GsPairs {
+    GsCell<T> r[2], b[2];
+    int p[2];        // pair index, or -1
+};
+
+template <typename T> __device__ __forceinline__ void gs_pairs(const LdsLev<T> &L, GsPairs<T> &P)
+{
+    const int hx = L.nx >> 1, npair = hx * L.ny, lhx = (L.lnx > 0) ? L.lnx - 1 : -1;
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        const int q = threadIdx.x + k * CT;
+        P.p[k] = -1;
+        if (q < npair) {
+            int j, ip;
+            divmod(q, hx, lhx, j, ip);
+            P.p[k] = q;
+            gs_cell(L, 2 * ip + (j & 1), j, P.r[k]);
+            gs_cell(L, 2 * ip + 1 - (j & 1), j, P.b[k]);
+        }
+    }
+}
+
+// Pre-smoothing of in-LDS level L (red from x = 0, black, then nu - 1 more
+// red-black pairs) and restriction: C.f = the sum of the four fine
+// residuals, through rs (one residual sum per fine pair).
+template <typename T> __device__ void lds_pre(const LdsLev<T> &L, const LdsLev<T> &C, T *rs, int nu)
+{
+    GsPairs<T> P;
+    gs_pairs(L, P);
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (P.p[k] >= 0) L.x[P.r[k].s] = -P.r[k].f * P.r[k].id;
+    __syncthreads();
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (P.p[k] >= 0) gs_update(L.x, P.b[k]);
+    __syncthreads();
+    for (int sw = 1; sw < nu; sw++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (P.p[k] >= 0) gs_update(L.x, P.r[k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (P.p[k] >= 0) gs_update(L.x, P.b[k]);
+        __syncthreads();
+    }
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (P.p[k] >= 0) rs[P.p[k]] = gs_res(L.x, P.r[k]) + gs_res(L.x, P.b[k]);
+    __syncthreads();
+    const int hx = L.nx >> 1;
     for (int q = threadIdx.x; q < C.nx * C.ny; q += CT) {
         int J, I;
         divmod(q, C.nx, C.lnx, J, I);
-        T s = T(0);
-        for (int dj = 0; dj < 2; dj++)
-            for (int di = 0; di < 2; di++) {
-                const int i = 2 * I + di, j = 2 * J + dj;
-                s += L.f[j * L.nx + i] - lds_ax(L, i, j);
-            }
-        C.f[q] = s;
+        C.f[q] = rs[2 * J * hx + I] + rs[(2 * J + 1) * hx + I];
     }
     __syncthreads();
+}
+
+// Coarse correction x += xc(parent) and post-smoothing (black first) of
+// in-LDS level L; C holds the coarse result.
+template <typename T> __device__ void lds_post(const LdsLev<T> &L, const LdsLev<T> &C, int nu)
+{
+    GsPairs<T> P;
+    gs_pairs(L, P);
+    const int hx = L.nx >> 1, lhx = (L.lnx > 0) ? L.lnx - 1 : -1;
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (P.p[k] >= 0) {
+            int j, ip;
+            divmod(P.p[k], hx, lhx, j, ip);
+            const T xc = C.x[(j >> 1) * C.nx + ip];
+            L.x[P.r[k].s] += xc;
+            L.x[P.b[k].s] += xc;
+        }
+    __syncthreads();
+    for (int sw = 0; sw < nu; sw++) {
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (P.p[k] >= 0) gs_update(L.x, P.b[k]);
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < 2; k++)
+            if (P.p[k] >= 0) gs_update(L.x, P.r[k]);
+        __syncthreads();
+    }
 }
 
 // in-LDS level l: x, f, bx, by (level c0 without topf: x, f)
@@ -439,39 +517,90 @@ template <typename T> __device__ __forceinline__ LdsLev<T> lds_level(const pvk_m
     return v;
 }
 
+// residual sums of the restriction (one per fine pair of the largest
+// smoothing level, c0), after the levels
+template <typename T> __device__ __forceinline__ T *lds_scratch(const pvk_mg &m, T *lds)
+{
+    T *q = lds;
+    for (int k = m.c0; k < m.nlev; k++) q += ((k == m.c0 && !m.topf) ? 2 : 4) * m.L[k].nx * m.L[k].ny;
+    return q;
+}
+
 template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 {
     extern __shared__ double lds_raw[];
     __shared__ double mean;
     __shared__ double vbuf[PVK_MG_BOTTOM];
     T *lds = (T *)lds_raw;
-    if (sc->p_done) return;
-    // f of level c0, and the faces of the in-LDS levels below it
-    for (int l = m.c0; l < m.nlev; l++) {
-        const Lev<T> G = lev<T>(m.L[l]);
-        const LdsLev<T> Ll = lds_level(m, lds, l);
-        for (int q = threadIdx.x; q < G.nx * G.ny; q += CT) {
+    const int nb = m.nlev - 1;
+    // The loads below are valid whatever the stop flag says, so they are
+    // issued before it is read: one memory round trip instead of two.
+    // This thread's entries of the bottom inverse (row threadIdx / 16,
+    // columns threadIdx % 16 + 16 t), loaded first: their latency hides
+    // behind the smoothing.
+    const int nbot = m.L[nb].nx * m.L[nb].ny, br = threadIdx.x >> 4, bq = threadIdx.x & 15;
+    double gv[PVK_MG_BOTTOM / 16];
+#pragma unroll
+    for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+        const int c = bq + 16 * t;
+        gv[t] = (br < nbot && c < nbot) ? m.G[(size_t)br * nbot + c] : 0.0;
+    }
+    // f of level c0 (and its faces with topf), the faces of the levels below
+    // it: every global load of this thread is issued before the first LDS
+    // store, so the loads wait for memory once, not once per level and pass.
+    // Level c0 has at most 4 CT cells, each level below at most CT, and there
+    // are at most MG_LOW of those (pv_mg_create).
+    T f0[4], bx0[4], by0[4], lbx[MG_LOW], lby[MG_LOW];
+    const Lev<T> G0 = lev<T>(m.L[m.c0]);
+    const int n0 = G0.nx * G0.ny, l0 = ilog2_or_neg(G0.nx);
+#pragma unroll
+    for (int u = 0; u < 4; u++) {
+        const int q = threadIdx.x + u * CT;
+        f0[u] = bx0[u] = by0[u] = T(0);
+        if (q < n0) {
             int j, i;
-            divmod(q, G.nx, Ll.lnx, j, i);
-            if (l == m.c0) Ll.f[q] = (l == 0 && m.r0) ? (T)m.r0[(ptrdiff_t)j * m.p0 + i] : ld(G.f, G.pitch, i, j);
-            if (l != m.c0 || m.topf) {
-                ((T *)Ll.bx)[q] = ld(G.bx, G.pitch, i, j);
-                ((T *)Ll.by)[q] = ld(G.by, G.pitch, i, j);
+            divmod(q, G0.nx, l0, j, i);
+            f0[u] = (m.c0 == 0 && m.r0) ? (T)m.r0[(ptrdiff_t)j * m.p0 + i] : ld(G0.f, G0.pitch, i, j);
+            if (m.topf) {
+                bx0[u] = ld(G0.bx, G0.pitch, i, j);
+                by0[u] = ld(G0.by, G0.pitch, i, j);
+            }
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < MG_LOW; k++) {
+        const Lev<T> G = lev<T>(m.L[min(m.c0 + 1 + k, nb)]);
+        const bool in = (m.c0 + 1 + k <= nb) && (int)threadIdx.x < G.nx * G.ny;
+        lbx[k] = in ? G.bx[threadIdx.x] : T(0);      // levels below c0: pitch = width
+        lby[k] = in ? G.by[threadIdx.x] : T(0);
+    }
+    if (sc->p_done) return;
+    {
+        const LdsLev<T> L0 = lds_level(m, lds, m.c0);
+#pragma unroll
+        for (int u = 0; u < 4; u++) {
+            const int q = threadIdx.x + u * CT;
+            if (q < n0) {
+                L0.f[q] = f0[u];
+                if (m.topf) {
+                    ((T *)L0.bx)[q] = bx0[u];
+                    ((T *)L0.by)[q] = by0[u];
+                }
+            }
+        }
+#pragma unroll
+        for (int k = 0; k < MG_LOW; k++) {
+            if (m.c0 + 1 + k > nb) break;
+            const LdsLev<T> Ll = lds_level(m, lds, m.c0 + 1 + k);
+            if ((int)threadIdx.x < Ll.nx * Ll.ny) {
+                ((T *)Ll.bx)[threadIdx.x] = lbx[k];
+                ((T *)Ll.by)[threadIdx.x] = lby[k];
             }
         }
     }
     __syncthreads();
-    const int nb = m.nlev - 1;
-    for (int l = m.c0; l < nb; l++) {
-        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
-        lds_sweep(Ll, Ll, 0, 0);
-        lds_sweep(Ll, Ll, 1, 1);
-        for (int s = 1; s < m.nu; s++) {
-            lds_sweep(Ll, Ll, 0, 1);
-            lds_sweep(Ll, Ll, 1, 1);
-        }
-        lds_restrict(Ll, Lc);
-    }
+    T *rs = lds_scratch(m, lds);
+    for (int l = m.c0; l < nb; l++) lds_pre(lds_level(m, lds, l), lds_level(m, lds, l + 1), rs, m.nu);
     // bottom: x = -P G P f, P the projection to zero mean (in FP64); the
     // means in the first wavefront, row r of G by the 16 lanes 16 r .. 16 r + 15
     {
@@ -485,8 +614,11 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         }
         __syncthreads();
         double v = 0.0;
-        if (r < n)
-            for (int c = q; c < n; c += 16) v -= m.G[(size_t)r * n + c] * ((double)B.f[c] - mean);
+#pragma unroll
+        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+            const int c = q + 16 * t;
+            if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mean);
+        }
         for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
         if (q == 0 && r < n) vbuf[r] = v;
         __syncthreads();
@@ -500,15 +632,7 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
         __syncthreads();
     }
-    for (int l = nb - 1; l >= m.c0; l--) {
-        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
-        lds_sweep(Ll, Lc, 1, 2);
-        lds_sweep(Ll, Ll, 0, 1);
-        for (int s = 1; s < m.nu; s++) {
-            lds_sweep(Ll, Ll, 1, 1);
-            lds_sweep(Ll, Ll, 0, 1);
-        }
-    }
+    for (int l = nb - 1; l >= m.c0; l--) lds_post(lds_level(m, lds, l), lds_level(m, lds, l + 1), m.nu);
     {
         const Lev<T> G = lev<T>(m.L[m.c0]);
         const LdsLev<T> top = lds_level(m, lds, m.c0);
@@ -609,6 +733,7 @@ extern "C" size_t pvk_mg_lds_bytes(const pvk_mg *m)
 {
     size_t n = 0;
     for (int l = m->c0; l < m->nlev; l++) n += ((l == m->c0 && !m->topf) ? 2 : 4) * (size_t)m->L[l].nx * m->L[l].ny;
+    if (m->nlev > m->c0 + 1) n += (size_t)(m->L[m->c0].nx / 2) * m->L[m->c0].ny;   // restriction scratch
     return n * (m->fp32 ? sizeof(float) : sizeof(double));
 }
 

@@ -52,7 +52,7 @@ int pv_mg_create(pv_bag *b)
         ly[nl] = ly[nl - 1] / 2;
         nl++;
     }
-    if (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM) return PV_OK;
+    if (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM || nl - 1 - c0 > PVK_MG_LOW) return PV_OK;
     /* the smoothing levels (all but the last) pair their cells along x */
     for (l = 0; l + 1 < nl; l++)
         if (lx[l] & 1) return PV_OK;
