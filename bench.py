@@ -57,21 +57,22 @@ BYTES_PER_CELL = {
 # the fused FFT-PCG kernels behind the P_APPLY / P_UPDATE timers also form the
 # new direction (z, d_old, faces -> d, Ad) and the forward x-FFT (-> spectrum)
 BYTES_PER_CELL_FFT = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=56)
-# MG-PCG: the same direction kernel, phi/r update (phi, d, r, Ad -> phi, r),
-# level-0 pre-smoothing + restriction (f, faces -> x, coarse f) and
-# correction + post-smoothing (x, coarse x, f, faces -> x)
-BYTES_PER_CELL_MG = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=48, MG_PRE0=34, MG_POST0=42)
-# the FP32 V-cycle (mixed precision): level 0 reads r (FP64) and FP32 faces,
-# writes FP32 x (pre) and FP64 z (post)
-BYTES_PER_CELL_MG32 = dict(BYTES_PER_CELL_MG, MG_PRE0=21, MG_POST0=29)
+# MG-PCG: the same direction kernel; the level-0 pre-smoothing also does the
+# CG step (r_n, A d, d, phi, faces -> r_{n+1}, phi, pre-smoothed x, coarse f)
+# and correction + post-smoothing (x, coarse x, r, faces -> z).  P_UPDATE is
+# the separate step kernel of grids without tiled levels.
+BYTES_PER_CELL_MG = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=48, MG_PRE0=74, MG_POST0=42)
+# the FP32 V-cycle (mixed precision): FP32 faces and pre-smoothed x, FP64 r,
+# A d, d, phi and z
+BYTES_PER_CELL_MG32 = dict(BYTES_PER_CELL_MG, MG_PRE0=61, MG_POST0=29)
 # per pressure iteration, all of its kernels (MG: level 0 + 1/3 of the level-0
-# smoothing bytes for the coarser levels)
-BYTES_PER_ITER = {"fft": 176, "mg": 197, "mg32": 163, "jacobi": 112}
+# smoothing bytes, without the CG step, for the coarser levels)
+BYTES_PER_ITER = {"fft": 176, "mg": 189, "mg32": 155, "jacobi": 112}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_POST0": 32.1, "MG_PRE0": 23.4, "P_APPLY": 51.5, "P_UPDATE": 51.2,
+TRAFFIC_MB_MG32_1024 = {"MG_POST0": 32.1, "P_APPLY": 51.5,
                         "MOM_APPLY": 121.9, "MOM_UPDATE": 119.6, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 

@@ -83,6 +83,11 @@ typedef struct pvk_pcg {
     double *zad;                      /* <A d_n, z_{n+1}> partials of the flexible (learned) form */
     double *sad;                      /* non-NULL: partials of sum(A d), one per row, so the residual
                                          update can keep r at zero mean (MG-PCG) */
+    double *r2;                       /* MG-PCG with tiled levels: the residual alternates between r
+                                         (r_0, r_2, ...) and r2 (r_1, r_3, ...): the level-0 smoothing
+                                         forms r_{n+1} from r_n and reads halos of it */
+    double *rrt;                      /* <r_{n+1}, r_{n+1}> partials of those kernels, one per tile */
+    int nrrt;                         /* their count (0: r is updated in place, partials in rr) */
     int ndad, nrr, nrz, nmean;        /* partial counts */
     int flex;                         /* 1: flexible CG, beta = -<z_{n+1}, A d_n> / <d_n, A d_n> */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */

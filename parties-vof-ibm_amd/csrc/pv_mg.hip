@@ -61,11 +61,27 @@ template <typename T> __device__ __forceinline__ T fget(const Lev<T> &L, const d
     return r0 ? (T)r0[(ptrdiff_t)j * p0 + i] : ld(L.f, L.pitch, i, j);
 }
 
+// the CG step of iteration n folded into the level-0 pre-smoothing:
+// r_{n+1} = r_n - alpha (A d_n - mean(A d_n)), phi += alpha d_n
+struct MgStep {
+    const double *Ad, *d;
+    double *phi, *rnew;
+    double alpha, mad;
+    bool on;
+};
+__device__ __forceinline__ double mg_rnew(const MgStep &u, const double *r0, size_t idx)
+{
+    return r0[idx] - u.alpha * (u.Ad[idx] - u.mad);
+}
+
 // stop test of CG iteration n on the <r, r> partials (as k_transpose):
-// n = -1 sets the reference norm, n = -2 keeps it (warm restart)
+// n = -1 sets the reference norm, n = -2 keeps it (warm restart).  The
+// partials of r_{n+1} come from the level-0 pre-smoothing (rrt, per tile)
+// when it forms r_{n+1}, else from the start or update kernels (rr, per row).
 __device__ void mg_stop_test(pvk_scal *sc, const pvk_pcg &w, int n, double tol, int maxit, double *sh)
 {
-    const double rr = block_sum(w.rr, w.nrr, sh);
+    const bool tiles = (n >= 0 && w.nrrt > 0);
+    const double rr = tiles ? block_sum(w.rrt, w.nrrt, sh) : block_sum(w.rr, w.nrr, sh);
     const double bn = (n == -1) ? ((sqrt(rr) == 0.0) ? 1.0 : sqrt(rr)) : sc->p_bnorm;
     const double rel = sqrt(rr) / bn;
     const bool done = (rel <= tol) || (n + 1 >= maxit && n >= 0);
@@ -98,9 +114,21 @@ template <int NU, int SH> struct MgTile {
     static constexpr int K = (NC + NTH - 1) / NTH;    // cells per thread
 };
 
+// is extended-tile cell (lx, ly) in the core, and in the level (x0 + lx the
+// unwrapped column)?
+template <int NU, int SH> __device__ __forceinline__ bool tile_core(int lx, int ly, int x0, int gj, int nx, int ny)
+{
+    using TT = MgTile<NU, SH>;
+    return lx >= TT::H && lx < TT::H + TT::TX && ly >= TT::H && ly < TT::H + TT::TY && x0 + lx < nx && gj < ny;
+}
+
+// f, faces and 1 / (their sum) of this thread's cells of the extended tile.
+// With the CG step on (level 0 of iteration n), f = r_{n+1} is formed from
+// r_n here, and the core cells also store r_{n+1}, update phi and add
+// r_{n+1}^2 to acc: every global access of the step is in this one batch.
 template <int NU, int SH, typename T>
-__device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, int x0, int y0, T *f, T *ce,
-                                            T *cw, T *cn, T *cs, T *id)
+__device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, const MgStep &st, int x0,
+                                            int y0, T *f, T *ce, T *cw, T *cn, T *cs, T *id, double &acc)
 {
     using TT = MgTile<NU, SH>;
     const int p = L.pitch, nx = L.nx, ny = L.ny;
@@ -112,7 +140,18 @@ __device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, i
             int gi = (x0 + lx) % nx;
             if (gi < 0) gi += nx;
             const int giw = (gi == 0) ? nx - 1 : gi - 1;
-            f[u] = fget(L, r0, p0, gi, gj);
+            if (st.on) {
+                const size_t idx = (size_t)gj * p0 + gi;
+                const double rn = mg_rnew(st, r0, idx);
+                f[u] = (T)rn;
+                if (tile_core<NU, SH>(lx, ly, x0, gj, nx, ny)) {
+                    st.rnew[idx] = rn;
+                    st.phi[idx] += st.alpha * st.d[idx];
+                    acc += rn * rn;
+                }
+            } else {
+                f[u] = fget(L, r0, p0, gi, gj);
+            }
             ce[u] = ld(L.bx, p, gi, gj);
             cw[u] = ld(L.bx, p, giw, gj);
             cn[u] = (gj + 1 < ny) ? ld(L.by, p, gi, gj) : T(0);
@@ -158,53 +197,71 @@ __device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k,
 }
 
 // pre-smoothing from x = 0 and restriction of the residual; the stop test of
-// iteration n in block (0, 0) of level 0 (check)
+// iteration n in block (0, 0) (check: the kernel after the one that formed
+// the residual).  On level 0 with step (CG iteration n >= 0) the kernel
+// first does the CG step: f = r_{n+1} is formed on the fly from r_n = r0
+// (tile and halo), and the core writes r_{n+1} to rnew, phi += alpha d and
+// the partial <r_{n+1}, r_{n+1}> of the tile.
 template <int NU, int SH, typename T>
-__global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, int p0, pvk_scal *sc,
-                                               pvk_pcg w, int n, double tol, int maxit, int check)
+__global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, double *rnew, int p0,
+                                               pvk_scal *sc, pvk_pcg w, int n, double tol, int maxit, int check,
+                                               int step, int ncell)
 {
     using TT = MgTile<NU, SH>;
     __shared__ T xs[TT::NC];
+    __shared__ T rs[TT::TX * TT::TY];
     __shared__ double sh[NTH];
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
+    MgStep st;
+    st.on = step && n >= 0;
+    st.Ad = w.Ad;
+    st.d = w.d[n & 1];
+    st.phi = w.phi;
+    st.rnew = rnew;
+    st.alpha = st.mad = 0.0;
+    if (st.on) {
+        st.alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
+        st.mad = block_sum(w.sad, w.ndad, sh) / (double)ncell;
+    }
     int bx, by;
     tile_of(bx, by);
     const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
-    tile_coeffs<NU, SH>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    double acc = 0.0;
+    tile_coeffs<NU, SH>(L, r0, p0, st, x0, y0, f, ce, cw, cn, cs, id, acc);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
     __syncthreads();
     for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, SH>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
+    // residual f - A x of each core cell from the registers (a wall face is
+    // 0 there) to rs, and the core of x for the post-smoothing (to xp: the
+    // post kernel reads it with halos while it writes x)
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {
+        const int c = threadIdx.x + u * NTH, ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
+        if (c < TT::NC && gj >= 0 && tile_core<NU, SH>(lx, ly, x0, gj, L.nx, L.ny)) {
+            const T x0c = xs[c];
+            rs[(ly - TT::H) * TT::TX + lx - TT::H] =
+                f[u] - (ce[u] * (xs[c + 1] - x0c) - cw[u] * (x0c - xs[c - 1]) + cn[u] * (xs[c + TT::W] - x0c) -
+                        cs[u] * (x0c - xs[c - TT::W]));
+            L.xp[(ptrdiff_t)gj * L.pitch + x0 + lx] = x0c;
+        }
+    }
+    __syncthreads();
     // restriction: one coarse cell per thread (TT::TX/2 x TT::TY/2 <= NTH)
     if ((int)threadIdx.x < (TT::TX / 2) * (TT::TY / 2)) {
         const int I = threadIdx.x % (TT::TX / 2), J = threadIdx.x / (TT::TX / 2);
         const int gI = bx * (TT::TX / 2) + I, gJ = by * (TT::TY / 2) + J;
         if (gI < C.nx && gJ < C.ny) {
-            const int p = L.pitch, nx = L.nx, ny = L.ny;
             T s = T(0);
 #pragma unroll
             for (int dj = 0; dj < 2; dj++)
 #pragma unroll
-                for (int di = 0; di < 2; di++) {
-                    const int gi = 2 * gI + di, gj = 2 * gJ + dj, giw = (gi == 0) ? nx - 1 : gi - 1;
-                    const int c = (TT::H + 2 * J + dj) * TT::W + TT::H + 2 * I + di;
-                    const T x0c = xs[c];
-                    T ax = ld(L.bx, p, gi, gj) * (xs[c + 1] - x0c) - ld(L.bx, p, giw, gj) * (x0c - xs[c - 1]);
-                    if (gj + 1 < ny) ax += ld(L.by, p, gi, gj) * (xs[c + TT::W] - x0c);
-                    if (gj > 0) ax -= ld(L.by, p, gi, gj - 1) * (x0c - xs[c - TT::W]);
-                    s += fget(L, r0, p0, gi, gj) - ax;
-                }
+                for (int di = 0; di < 2; di++) s += rs[(2 * J + dj) * TT::TX + 2 * I + di];
             C.f[(ptrdiff_t)gJ * C.pitch + gI] = s;
         }
     }
-    // the core of x for the post-smoothing (to xp: the post kernel reads it
-    // with halos while it writes x)
-    for (int c = threadIdx.x; c < TT::TX * TT::TY; c += NTH) {
-        const int ly = c / TT::TX, lx = c - ly * TT::TX;
-        const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
-        if (gi < L.nx && gj < L.ny) L.xp[(ptrdiff_t)gj * L.pitch + gi] = xs[(ly + TT::H) * TT::W + lx + TT::H];
-    }
+    if (st.on) block_store_partial(acc, w.rrt, by * gridDim.x + bx, sh);
 }
 
 // coarse correction xp + xc(parent) and post-smoothing (black first); the
@@ -226,7 +283,10 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
     tile_of(bx, by);
     const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
     T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
-    tile_coeffs<NU, SH>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+    MgStep st;
+    st.on = false;
+    double unused = 0.0;
+    tile_coeffs<NU, SH>(L, r0, p0, st, x0, y0, f, ce, cw, cn, cs, id, unused);
     for (int c = threadIdx.x; c < TT::NC; c += NTH) {
         const int ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
         T v = T(0);
@@ -774,9 +834,18 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *b
 
 // One V-cycle z = V(r) for CG iteration n is, in order: pvk_mg_pre for
 // levels 0 .. c0-1, pvk_mg_coarse, pvk_mg_post for levels c0-1 .. 0.  The
-// stop test of n runs first (n = -3: none); <r, z> partials go to
-// w->rz[(n < 0) ? 0 : (n + 1) & 1].
+// stop test of n runs in the kernel after the one that formed r_{n+1}
+// (n = -3: none); <r, z> partials go to w->rz[(n < 0) ? 0 : (n + 1) & 1].
+// With tiled levels (c0 > 0) the level-0 pre kernel of iteration n >= 0 does
+// the CG step, r_n -> r_{n+1}, between w->r and w->r2 by parity; without
+// them (c0 = 0) pvk_mg_update does it in place in w->r.
 static int rz_slot(int n) { return (n < 0) ? 0 : ((n + 1) & 1); }
+static double *r_of(const pvk_mg *m, const pvk_pcg *w, int k)     // r_k, k >= 0
+{
+    return (m->c0 > 0 && (k & 1)) ? w->r2 : w->r;
+}
+// the residual a V-cycle of iteration n reads: r_{n+1} (r_0 for n < 0)
+static const double *r_in(const pvk_mg *m, const pvk_pcg *w, int n) { return r_of(m, w, n < 0 ? 0 : n + 1); }
 
 #define MG_SHAPES(KERNEL, NU, ...)                                                                         \
     do {                                                                                                   \
@@ -791,17 +860,22 @@ static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n,
 {
     const dim3 gr = tile_grid(m->L[l]);
     const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
-    const double *r0 = (l == 0) ? m->r0 : nullptr;
-    const int sh = tile_shape(m->L[l]), chk = (l == 0);
-    if (m->nu == 1) MG_SHAPES(k_mg_pre, 1, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
-    else MG_SHAPES(k_mg_pre, 2, L, C, r0, m->p0, sc, *w, n, tol, maxit, chk);
+    const int step = (l == 0 && n >= 0);
+    const double *r0 = (l == 0) ? (step ? r_of(m, w, n) : r_in(m, w, n)) : nullptr;
+    double *rnew = step ? r_of(m, w, n + 1) : nullptr;
+    const int sh = tile_shape(m->L[l]), chk = (l == 1), nc = m->L[0].nx * m->L[0].ny;
+    if (m->nu == 1) MG_SHAPES(k_mg_pre, 1, L, C, r0, rnew, m->p0, sc, *w, n, tol, maxit, chk, step, nc);
+    else MG_SHAPES(k_mg_pre, 2, L, C, r0, rnew, m->p0, sc, *w, n, tol, maxit, chk, step, nc);
+    // levels 0 .. 1 only: the stop test of the residual pre level 0 formed
+    if (l == 0 && m->c0 == 1 && n != -3)
+        hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
 }
 
 template <typename T> static void post_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
 {
     const dim3 gr = tile_grid(m->L[l]);
     const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
-    const double *r0 = (l == 0) ? m->r0 : nullptr;
+    const double *r0 = (l == 0) ? r_in(m, w, n) : nullptr;
     double *z0 = (l == 0) ? m->z0 : nullptr;
     const pvk_scal *cs = sc;
     const int sh = tile_shape(m->L[l]), slot = rz_slot(n), dot = (l == 0);

@@ -67,8 +67,8 @@ int pv_mg_create(pv_bag *b)
     }
     for (l = 1; l < nl; l++) cells += (size_t)lx[l] * ly[l];
     nrz = pvk_mg_nrz(m);
-    total = 5 * cells + (size_t)b->g.P * ny + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 5 * (size_t)ny +
-            2 * (size_t)nrz;
+    total = 5 * cells + 2 * (size_t)b->g.P * ny + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 5 * (size_t)ny +
+            3 * (size_t)nrz;
     totalf = 5 * cells + 3 * nc0;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
@@ -134,6 +134,9 @@ int pv_mg_create(pv_bag *b)
         w->sad = q; q += ny;
         w->rz[0] = q; q += nrz;
         w->rz[1] = q; q += nrz;
+        w->r2 = q; q += (size_t)b->g.P * ny;     /* r_1, r_3, ...: pitch P like r */
+        w->rrt = q; q += nrz;
+        w->nrrt = (c0 > 0) ? nrz : 0;
         w->ndad = ny;
         w->nrr = ny;
         w->nrz = nrz;

@@ -518,7 +518,9 @@ static int p_solve_fft_pcg(pv_bag *b)
 }
 
 /* ---- MG-PCG: the CG above with one multigrid V-cycle as preconditioner ---- */
-/* z = V(r) for CG iteration n (the stop test of n first; n = -3: none) */
+/* z = V(r_{n+1}) for CG iteration n (r_0 for n < 0); with tiled levels the
+ * level-0 pre kernel first does the CG step of n and the next kernel its stop
+ * test (n = -3: no test) */
 static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
 {
     const pvk_mg *m = b->mgp;
@@ -540,14 +542,18 @@ static void mg_vcycle(pv_bag *b, int n, double tol, int maxit)
     }
 }
 
+/* with tiled levels the level-0 pre-smoothing also does the CG step
+ * (phi, r update); on one-workgroup grids pvk_mg_update does it */
 static void mg_iteration(pv_bag *b, int n, double tol, int maxit)
 {
     pv_tbegin(b, PV_T_P_APPLY);
     pvk_pcg_apply(G(b), b->d_sc, &b->mgw, n, S(b));
     pv_tend(b, PV_T_P_APPLY);
-    pv_tbegin(b, PV_T_P_UPDATE);
-    pvk_mg_update(G(b), b->d_sc, &b->mgw, n, S(b));
-    pv_tend(b, PV_T_P_UPDATE);
+    if (b->mgp->c0 == 0) {
+        pv_tbegin(b, PV_T_P_UPDATE);
+        pvk_mg_update(G(b), b->d_sc, &b->mgw, n, S(b));
+        pv_tend(b, PV_T_P_UPDATE);
+    }
     mg_vcycle(b, n, tol, maxit);
 }
 
