@@ -276,6 +276,8 @@ void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, d
 void pvk_mg_coarse(const pvk_mg *m, const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, double tol,
                    int maxit, hipStream_t s);
 void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
+void pvk_mg_check(pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s);
+void pvk_mg_take(const pvk_grid *g, const pvk_scal *sc, const double *src, double *phi, hipStream_t s);
 void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s);
 void pvk_mg_update(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s);
 /* out = A x with the pressure operator of the fused PCG (faces from its start) */

@@ -917,6 +917,25 @@ extern "C" void pvk_mg_post(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg 
     else post_t<double>(m, l, sc, w, n, s);
 }
 
+// phi = src unless the solve has already stopped (a zero right-hand side:
+// phi stays 0)
+__global__ void __launch_bounds__(NTH) k_mg_take(pvk_grid g, const pvk_scal *sc, const double *src, double *phi)
+{
+    if (sc->p_done) return;
+    const int P = g.P, j = blockIdx.x;
+    for (int i = threadIdx.x; i < g.NXM; i += NTH) AT(phi, i, j) = AT(src, i, j);
+}
+
+extern "C" void pvk_mg_check(pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
+}
+
+extern "C" void pvk_mg_take(const pvk_grid *g, const pvk_scal *sc, const double *src, double *phi, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mg_take, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, src, phi);
+}
+
 extern "C" void pvk_mg_start(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int warm, hipStream_t s)
 {
     hipLaunchKernelGGL(k_mg_start, dim3(g->NYM), dim3(NTH), 0, s, *g, (const pvk_scal *)sc, *w, warm);

@@ -93,6 +93,8 @@ struct pv_bag {
     int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
     int ch_spectral;               /* 1: biharmonic CG started from the exact spectral solve */
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
+    int warm_wins[3], warm_solves[3];  /* per stage: the temporal start's consecutive wins, solves since
+                                          it became trusted */
     int net_losses, net_skip;      /* consecutive losses of the network's start; solves left to skip it */
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
     pvk_cnn cnn;                   /* learned preconditioner network (pv_cnn_init) */

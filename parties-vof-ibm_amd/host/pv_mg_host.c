@@ -178,6 +178,8 @@ int pv_set_pressure_warm_start(pv_bag *b, int mode)
     if (mode < 0 || mode > 1) return pv_set_error(PV_EINVAL, "warm start mode must be 0 or 1 (got %d)", mode);
     b->p_warm = mode;
     b->have_prev[0] = b->have_prev[1] = b->have_prev[2] = 0;
+    memset(b->warm_wins, 0, sizeof b->warm_wins);
+    memset(b->warm_solves, 0, sizeof b->warm_solves);
     return PV_OK;
 }
 

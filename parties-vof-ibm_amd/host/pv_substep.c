@@ -654,7 +654,10 @@ static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit);
  * chosen start's energy relative to the cold start's; a candidate that loses
  * counts as a fallback.  After 3 losses in a row the network's candidate is
  * not formed for the next 16 solves (then tried again), so a network that
- * does not help costs one try in 19 solves. */
+ * does not help costs one try in 19 solves.  Once the temporal start has won
+ * 3 evaluations of its stage in a row (and no network candidate is formed),
+ * it is taken without forming the cold candidate, except in every 8th solve,
+ * which evaluates again; such a solve leaves learn_start_res as it was. */
 static int mg_solve(pv_bag *b, int learned)
 {
     int maxit = b->ph.p_max_iter, st = b->stage, best = 0;
@@ -663,6 +666,20 @@ static int mg_solve(pv_bag *b, int learned)
     if (learned && !b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (power-of-two sizes)");
     if (learned && !b->cnn_ready)
         return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
+    if (learned && b->net_skip > 0) {
+        b->net_skip--;
+        learned = 0;
+    }
+    if (b->p_warm && b->have_prev[st] && !learned && b->warm_wins[st] >= 3 && (++b->warm_solves[st] & 7) != 0) {
+        /* trusted temporal start: ||b|| from the cold residual, then the start */
+        pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
+        pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
+        pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
+        pvk_mg_check(b->d_sc, &b->mgw, -1, tol, maxit, S(b));
+        pvk_mg_take(G(b), b->d_sc, FD(b, PV_FI_PHI0 + st), FD(b, PV_F_PHI), S(b));
+        mg_warm_restart(b, tol, maxit);
+        return mg_loop(b, 0, tol, maxit);
+    }
     mg_cold_start(b, tol, maxit);
     b->learn_start_res = 1.0;
     if (!b->p_warm && !learned) return mg_loop(b, 0, tol, maxit);
@@ -673,12 +690,15 @@ static int mg_solve(pv_bag *b, int learned)
         pvk_learn_copy(G(b), FD(b, PV_FI_PHI0 + st), FD(b, PV_F_PHI), S(b));
         mg_warm_restart(b, tol, maxit);
         RC(mg_energy(b, &e));
-        if (e < eb) { eb = e; best = 1; }
-        else b->st.p_fallback[st] += 1;
-    }
-    if (learned && b->net_skip > 0) {
-        b->net_skip--;
-        learned = 0;
+        if (e < eb) {
+            eb = e;
+            best = 1;
+            b->warm_wins[st]++;
+        } else {
+            b->st.p_fallback[st] += 1;
+            b->warm_wins[st] = 0;
+        }
+        b->warm_solves[st] = 0;
     }
     if (learned && !b->h_sc->p_done) {
         if (best == 0 && b->p_warm && b->have_prev[st]) {   /* back to the cold state */
