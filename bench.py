@@ -49,7 +49,7 @@ TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDA
 # algorithmic HBM bytes per cell and launch (each distinct array read or
 # written once; stencil neighbours come from cache)
 BYTES_PER_CELL = {
-    "CH_APPLY": 32, "CH_UPDATE": 48, "MOM_APPLY": 32, "MOM_UPDATE": 56,
+    "CH_APPLY": 32, "CH_UPDATE": 48, "MOM_APPLY": 56, "MOM_UPDATE": 56,
     "P_APPLY": 24, "P_UPDATE": 48, "P_DUPDATE": 24,
     "FFT_ROWS_FWD": 16, "FFT_COLS": 16, "FFT_ROWS_INV": 24,
     "CH_RHS": 56, "MOM_TERMS": 88, "P_TRANSPOSE": 16,
@@ -72,7 +72,7 @@ BYTES_PER_ITER = {"fft": 176, "mg": 189, "mg32": 155, "jacobi": 112}
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_POST0": 32.1, "P_APPLY": 51.5,
+TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.3, "MG_POST0": 32.1, "P_APPLY": 51.5,
                         "MOM_APPLY": 121.9, "MOM_UPDATE": 119.6, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 
