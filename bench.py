@@ -162,6 +162,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     if family_of(solver) == "mg":
         s.check(lib.pv_set_pressure_warm_start(s.h, args.warm_start))
         s.check(lib.pv_set_ch_solver(s.h, args.ch_spectral))
+        s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -285,6 +286,8 @@ def main():
                     help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--warm-start", type=int, choices=[0, 1], default=1,
                     help="MG-PCG solvers: 1 = also try the same RK3 stage's last correction as the start")
+    ap.add_argument("--mg-cycle", type=int, choices=[0, 1, 2, 3, 4], default=3,
+                    help="MG solvers: the first k in-workgroup levels take two coarse corrections (0 = V-cycle)")
     ap.add_argument("--ch-spectral", type=int, choices=[0, 1], default=1,
                     help="MG-PCG forms: 1 = start the biharmonic CG from its exact spectral solve")
     ap.add_argument("--no-compare", action="store_true",
@@ -373,6 +376,7 @@ def main():
                    "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
                    "warm_start": args.warm_start if family_of(args.solver) == "mg" else None,
                    "ch_spectral": args.ch_spectral if family_of(args.solver) == "mg" else None,
+                   "mg_cycle_w_levels": args.mg_cycle if family_of(args.solver) == "mg" else None,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": main_run["path_hbm_fraction"],
         "path_bytes_per_cell_step": main_run["path_bytes_per_cell_step"],

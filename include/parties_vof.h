@@ -286,6 +286,11 @@ int pv_set_pressure_solver(pv_bag *bag, int solver);
 /* Gauss-Seidel sweeps before and after the coarse correction of the
  * multigrid V-cycle (1 or 2, default 2). */
 int pv_set_mg_sweeps(pv_bag *bag, int nu);
+/* Cycle of the multigrid levels that run in one workgroup (64 x 64 cells and
+ * below): the first wlevels of them take two coarse corrections with
+ * smoothing between them (pre, correct, post, pre, correct, post), a
+ * symmetric W there; 0 (default) = V-cycle throughout. */
+int pv_set_mg_cycle(pv_bag *bag, int wlevels);
 /* Start of the MG-PCG solves (PV_PSOLVE_MG_PCG, PV_PSOLVE_LEARNED_MG):
  * mode 0 = from phi = 0 (as the reference); mode 1 = also try the pressure
  * correction of the same RK3 stage of the previous step, kept only when the

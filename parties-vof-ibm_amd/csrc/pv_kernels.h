@@ -262,6 +262,7 @@ typedef struct pvk_mglev {
 typedef struct pvk_mg {
     int nlev, c0, nu, fp32;           /* levels, first in-LDS level, sweeps per side, FP32 V-cycle */
     int topf;                         /* faces of level c0 copied to LDS too (when they fit) */
+    int wmask;                        /* in-LDS levels (bit l) with two coarse corrections (a W there) */
     pvk_mglev L[PVK_MG_MAXLEV];
     const double *r0;                 /* FP32: level-0 right-hand side (r), pitch p0 */
     double *z0;                       /* FP32: level-0 result (z), pitch p0 */

@@ -486,16 +486,19 @@ template <typename T> __device__ __forceinline__ void gs_pairs(const LdsLev<T> &
     }
 }
 
-// Pre-smoothing of in-LDS level L (red from x = 0, black, then nu - 1 more
-// red-black pairs) and restriction: C.f = the sum of the four fine
-// residuals, through rs (one residual sum per fine pair).
-template <typename T> __device__ void lds_pre(const LdsLev<T> &L, const LdsLev<T> &C, T *rs, int nu)
+// Pre-smoothing of in-LDS level L (red, from x = 0 when zero, black, then
+// nu - 1 more red-black pairs) and restriction: C.f = the sum of the four
+// fine residuals, through rs (one residual sum per fine pair).
+template <typename T> __device__ void lds_pre(const LdsLev<T> &L, const LdsLev<T> &C, T *rs, int nu, bool zero)
 {
     GsPairs<T> P;
     gs_pairs(L, P);
 #pragma unroll
     for (int k = 0; k < 2; k++)
-        if (P.p[k] >= 0) L.x[P.r[k].s] = -P.r[k].f * P.r[k].id;
+        if (P.p[k] >= 0) {
+            if (zero) L.x[P.r[k].s] = -P.r[k].f * P.r[k].id;
+            else gs_update(L.x, P.r[k]);
+        }
     __syncthreads();
 #pragma unroll
     for (int k = 0; k < 2; k++)
@@ -660,39 +663,71 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     }
     __syncthreads();
     T *rs = lds_scratch(m, lds);
-    for (int l = m.c0; l < nb; l++) lds_pre(lds_level(m, lds, l), lds_level(m, lds, l + 1), rs, m.nu);
-    // bottom: x = -P G P f, P the projection to zero mean (in FP64); the
-    // means in the first wavefront, row r of G by the 16 lanes 16 r .. 16 r + 15
-    {
-        const LdsLev<T> B = lds_level(m, lds, nb);
-        const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
-        if (threadIdx.x < 64) {
-            double s = 0.0;
-            for (int k = threadIdx.x; k < n; k += 64) s += (double)B.f[k];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (threadIdx.x == 0) mean = s / n;
-        }
-        __syncthreads();
-        double v = 0.0;
+    // The in-LDS levels: a V-cycle, except that the levels in m.wmask take
+    // two coarse corrections, smoothing between them (pre-smooth, correct,
+    // post-smooth, pre-smooth, correct, post-smooth: the error propagation
+    // S^T C S S^T C S stays symmetric in the energy inner product).  One
+    // schedule for all threads: level l, going down or coming back up, and
+    // the levels whose second correction is under way.
+    int l = m.c0;
+    bool down = true;
+    unsigned second = 0u;
+    for (;;) {
+        if (l == nb) {
+            // bottom: x = -P G P f, P the projection to zero mean (in FP64); the
+            // means in the first wavefront, row r of G by the 16 lanes 16 r .. 16 r + 15
+            {
+                const LdsLev<T> B = lds_level(m, lds, nb);
+                const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
+                if (threadIdx.x < 64) {
+                    double s = 0.0;
+                    for (int k = threadIdx.x; k < n; k += 64) s += (double)B.f[k];
+                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+                    if (threadIdx.x == 0) mean = s / n;
+                }
+                __syncthreads();
+                double v = 0.0;
 #pragma unroll
-        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-            const int c = q + 16 * t;
-            if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mean);
+                for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+                    const int c = q + 16 * t;
+                    if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mean);
+                }
+                for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+                if (q == 0 && r < n) vbuf[r] = v;
+                __syncthreads();
+                if (threadIdx.x < 64) {
+                    double s = 0.0;
+                    for (int k = threadIdx.x; k < n; k += 64) s += vbuf[k];
+                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
+                    if (threadIdx.x == 0) mean = s / n;
+                }
+                __syncthreads();
+                if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
+                __syncthreads();
+            }
+            l--;
+            down = false;
+            if (l < m.c0) break;
+            continue;
         }
-        for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
-        if (q == 0 && r < n) vbuf[r] = v;
-        __syncthreads();
-        if (threadIdx.x < 64) {
-            double s = 0.0;
-            for (int k = threadIdx.x; k < n; k += 64) s += vbuf[k];
-            for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-            if (threadIdx.x == 0) mean = s / n;
+        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        if (down) {
+            lds_pre(Ll, Lc, rs, m.nu, true);
+            l++;
+            continue;
         }
-        __syncthreads();
-        if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
-        __syncthreads();
+        lds_post(Ll, Lc, m.nu);
+        if (((m.wmask >> l) & 1) && !((second >> l) & 1u)) {
+            second |= 1u << l;
+            lds_pre(Ll, Lc, rs, m.nu, false);
+            l++;
+            down = true;
+            continue;
+        }
+        second &= ~(1u << l);
+        if (l == m.c0) break;
+        l--;
     }
-    for (int l = nb - 1; l >= m.c0; l--) lds_post(lds_level(m, lds, l), lds_level(m, lds, l + 1), m.nu);
     {
         const Lev<T> G = lev<T>(m.L[m.c0]);
         const LdsLev<T> top = lds_level(m, lds, m.c0);

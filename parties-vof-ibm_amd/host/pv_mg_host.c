@@ -193,6 +193,18 @@ int pv_set_ch_solver(pv_bag *b, int spectral)
     return PV_OK;
 }
 
+int pv_set_mg_cycle(pv_bag *b, int wlevels)
+{
+    int l, mask = 0;
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (wlevels < 0 || wlevels > PVK_MG_LOW + 1)
+        return pv_set_error(PV_EINVAL, "MG W levels must be 0..%d (got %d)", PVK_MG_LOW + 1, wlevels);
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    for (l = b->mg.c0; l < b->mg.c0 + wlevels && l < b->mg.nlev - 1; l++) mask |= 1 << l;
+    b->mg.wmask = b->mgf.wmask = mask;
+    return PV_OK;
+}
+
 int pv_set_mg_sweeps(pv_bag *b, int nu)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");

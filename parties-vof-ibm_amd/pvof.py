@@ -159,6 +159,7 @@ def load_lib(path=LIB_PATH):
     ip = C.POINTER(C.c_int)
     lib.pv_set_pressure_solver.argtypes = [vp, C.c_int]
     lib.pv_set_mg_sweeps.argtypes = [vp, C.c_int]
+    lib.pv_set_mg_cycle.argtypes = [vp, C.c_int]
     lib.pv_mg_vcycle_apply.argtypes = [vp]
     lib.pv_set_pressure_warm_start.argtypes = [vp, C.c_int]
     lib.pv_set_mg_precision.argtypes = [vp, C.c_int]

@@ -12,7 +12,8 @@ One V-cycle V(f), arrays indexed [j][i]:
     two fine faces across the coarse face;
   * nu red-black Gauss-Seidel sweeps (red = (i + j) even first) from x = 0,
     restriction by the sum of the four fine residuals, the coarse V-cycle,
-    piecewise-constant prolongation, nu sweeps black first;
+    piecewise-constant prolongation, nu sweeps black first; on the levels in
+    `wl` that is done twice (the second pre-smoothing from the current x);
   * the last level: x = -P G P f, G the inverse of -A with its (0, 0) entry
     doubled, P the projection to zero mean.
 `mutant` plants one plausible mistake (for the tests that show the checks
@@ -93,26 +94,29 @@ def sweep(bx, by, x, f, colour):
     return x
 
 
-def vcycle(levels, G, f, nu=2, k=0, mutant=None):
+def vcycle(levels, G, f, nu=2, k=0, mutant=None, wl=()):
+    """One cycle; the levels in wl take two coarse corrections with smoothing
+    between them (pre, correct, post, pre, correct, post)."""
     bx, by = levels[k]
     if k == len(levels) - 1:
         fm = f - f.mean()
         x = -(G @ fm.ravel()).reshape(f.shape)
         return x - x.mean()
     x = np.zeros_like(f)
-    for _ in range(nu):
-        x = sweep(bx, by, x, f, 0)
-        x = sweep(bx, by, x, f, 1)
-    r = f - apply_a(bx, by, x)
-    fc = r[0::2, 0::2] + r[1::2, 0::2] + r[0::2, 1::2] + r[1::2, 1::2]
-    if mutant == "restrict_mean":
-        fc = 0.25 * fc
-    xc = vcycle(levels, G, fc, nu, k + 1, mutant)
-    x = x + np.repeat(np.repeat(xc, 2, 0), 2, 1)
     first, second = (0, 1) if mutant == "post_red_first" else (1, 0)
-    for _ in range(nu):
-        x = sweep(bx, by, x, f, first)
-        x = sweep(bx, by, x, f, second)
+    for rep in range(2 if k in wl else 1):
+        for _ in range(nu):
+            x = sweep(bx, by, x, f, 0)
+            x = sweep(bx, by, x, f, 1)
+        r = f - apply_a(bx, by, x)
+        fc = r[0::2, 0::2] + r[1::2, 0::2] + r[0::2, 1::2] + r[1::2, 1::2]
+        if mutant == "restrict_mean":
+            fc = 0.25 * fc
+        xc = vcycle(levels, G, fc, nu, k + 1, mutant, wl)
+        x = x + np.repeat(np.repeat(xc, 2, 0), 2, 1)
+        for _ in range(nu):
+            x = sweep(bx, by, x, f, first)
+            x = sweep(bx, by, x, f, second)
     return x
 
 

@@ -65,6 +65,33 @@ def test_vcycle_matches_reference(lib, nx, ny, nu, fp32):
     assert relerr(z, zr) < (1e-5 if fp32 else 1e-11)
 
 
+@pytest.mark.parametrize("nx,ny", [(64, 64), (64, 128), (256, 256), (1024, 1024)])
+@pytest.mark.parametrize("wlevels", [1, 2, 3])
+@pytest.mark.parametrize("fp32", [0, 1])
+def test_wcycle_matches_reference(lib, nx, ny, wlevels, fp32):
+    """pv_set_mg_cycle: the first wlevels in-workgroup levels take two coarse
+    corrections; the GPU cycle equals the restatement's with those levels."""
+    p = params(nx, ny)
+    s = pvof.Solver(p, lib)
+    s.check(lib.pv_set_mg_precision(s.h, fp32))
+    s.check(lib.pv_set_mg_cycle(s.h, wlevels))
+    rho = mg_ref.bubbles(nx, ny, seed=nx + 2 * ny, nb=8)
+    f = np.random.default_rng(nx + ny).standard_normal((ny, nx))
+    f -= f.mean()
+    s.set("RHO", canonical(p, rho))
+    s.set("P_RES", canonical(p, f))
+    s.call("mg_vcycle_apply")
+    z = s.get("SCRATCH0")[regions(p)["cell"]]
+    bx, by = mg_ref.faces(rho, float(nx * nx), float(nx * nx))
+    lv, G = mg_ref.build(bx, by)
+    sizes = mg_ref.level_sizes(nx, ny)
+    c0 = next(i for i, sz in enumerate(sizes) if sz[0] * sz[1] <= mg_ref.LDS_CELLS)
+    wl = tuple(range(c0, min(c0 + wlevels, len(lv) - 1)))
+    zr = mg_ref.vcycle(lv, G, f, 2, wl=wl)
+    print(f"\n{nx}x{ny} fp32={fp32} W levels {wl} relerr={relerr(z, zr):.2e}")
+    assert relerr(z, zr) < (1e-5 if fp32 else 1e-11)
+
+
 @pytest.mark.parametrize("nx,ny", [(96, 96), (192, 96), (256, 512), (384, 768)])
 @pytest.mark.parametrize("fp32", [0, 1])
 def test_pressure_solve_residual(lib, nx, ny, fp32):
