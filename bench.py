@@ -72,7 +72,7 @@ BYTES_PER_ITER = {"fft": 176, "mg": 189, "mg32": 155, "jacobi": 112}
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.3, "MG_POST0": 32.1, "P_APPLY": 51.5,
+TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.1, "MG_POST0": 32.0, "P_APPLY": 51.5,
                         "MOM_APPLY": 121.9, "MOM_UPDATE": 119.6, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 
@@ -217,11 +217,12 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
 
 
 def mixed_error(lib, args):
-    """Per-field error of the fast mixed form (FP32 stencils and V-cycle, FP64
-    fluxes / solves / dots, spectral CH start, temporal pressure start) against
-    the FP64 path (FP64 stencils and V-cycle, the reference CH CG, cold
-    pressure starts), both MG-PCG, stepped side by side from the same seeded
-    field; errors at checkpoints up to --mixed-error-steps."""
+    """Per-field error of the fast mixed form (FP32 stencils and multigrid
+    cycle, W on --mg-cycle in-workgroup levels, FP64 fluxes / solves / dots,
+    spectral CH start, temporal pressure start) against the FP64 path (FP64
+    stencils and V-cycle, the reference CH CG, cold pressure starts), both
+    MG-PCG, stepped side by side from the same seeded field; errors at
+    checkpoints up to --mixed-error-steps."""
     n, steps = args.n, args.mixed_error_steps
     if args.ic == "droplets":          # 32 liquid droplets of radius 0.02-0.06 in gas
         kw = dict(nmin=32, nmax=32, rmin=0.02, rmax=0.06, gas_inside=0)
@@ -236,6 +237,7 @@ def mixed_error(lib, args):
         if name == "mixed":
             s.check(lib.pv_set_ch_solver(s.h, 1))
             s.check(lib.pv_set_pressure_warm_start(s.h, 1))
+            s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
         runs[name], secs[name] = s, 0.0
     R = (slice(3, 3 + n), slice(3, 3 + n))
     marks = sorted({m for m in (10, 50, 100, 250, steps) if m <= steps})
