@@ -493,13 +493,16 @@ template <typename T> __device__ void lds_pre(const LdsLev<T> &L, const LdsLev<T
 {
     GsPairs<T> P;
     gs_pairs(L, P);
+    // From x = 0 the first red update is -f / diag.  Otherwise (the second
+    // correction of a W) it is left out: the post-smoothing just ended with
+    // a red half-sweep and no black cell has changed since, so it would
+    // reproduce the same red values.
+    if (zero) {
 #pragma unroll
-    for (int k = 0; k < 2; k++)
-        if (P.p[k] >= 0) {
-            if (zero) L.x[P.r[k].s] = -P.r[k].f * P.r[k].id;
-            else gs_update(L.x, P.r[k]);
-        }
-    __syncthreads();
+        for (int k = 0; k < 2; k++)
+            if (P.p[k] >= 0) L.x[P.r[k].s] = -P.r[k].f * P.r[k].id;
+        __syncthreads();
+    }
 #pragma unroll
     for (int k = 0; k < 2; k++)
         if (P.p[k] >= 0) gs_update(L.x, P.b[k]);
@@ -527,24 +530,39 @@ template <typename T> __device__ void lds_pre(const LdsLev<T> &L, const LdsLev<T
     __syncthreads();
 }
 
+// parent (coarse cell) of the cell at LDS index q of level L
+template <typename T> __device__ __forceinline__ int gs_parent(const LdsLev<T> &L, const LdsLev<T> &C, int q)
+{
+    int j, i;
+    divmod(q, L.nx, L.lnx, j, i);
+    return (j >> 1) * C.nx + (i >> 1);
+}
+
 // Coarse correction x += xc(parent) and post-smoothing (black first) of
-// in-LDS level L; C holds the coarse result.
+// in-LDS level L; C holds the coarse result.  The correction rides in the
+// first black half-sweep: its red neighbours are read as x + xc(their
+// parent), the value the separate correction would have stored; the red
+// cells themselves need not be corrected, the red half-sweep that follows
+// overwrites them from the updated black ones.
 template <typename T> __device__ void lds_post(const LdsLev<T> &L, const LdsLev<T> &C, int nu)
 {
     GsPairs<T> P;
     gs_pairs(L, P);
-    const int hx = L.nx >> 1, lhx = (L.lnx > 0) ? L.lnx - 1 : -1;
 #pragma unroll
     for (int k = 0; k < 2; k++)
         if (P.p[k] >= 0) {
-            int j, ip;
-            divmod(P.p[k], hx, lhx, j, ip);
-            const T xc = C.x[(j >> 1) * C.nx + ip];
-            L.x[P.r[k].s] += xc;
-            L.x[P.b[k].s] += xc;
+            const GsCell<T> &c = P.b[k];
+            const T *x = L.x, *xc = C.x;
+            const T xe = x[c.e] + xc[gs_parent(L, C, c.e)], xw = x[c.w] + xc[gs_parent(L, C, c.w)];
+            const T xn = x[c.n] + xc[gs_parent(L, C, c.n)], xs = x[c.so] + xc[gs_parent(L, C, c.so)];
+            L.x[c.s] = (c.ce * xe + c.cw * xw + c.cn * xn + c.cs * xs - c.f) * c.id;
         }
     __syncthreads();
-    for (int sw = 0; sw < nu; sw++) {
+#pragma unroll
+    for (int k = 0; k < 2; k++)
+        if (P.p[k] >= 0) gs_update(L.x, P.r[k]);
+    __syncthreads();
+    for (int sw = 1; sw < nu; sw++) {
 #pragma unroll
         for (int k = 0; k < 2; k++)
             if (P.p[k] >= 0) gs_update(L.x, P.b[k]);
@@ -592,8 +610,8 @@ template <typename T> __device__ __forceinline__ T *lds_scratch(const pvk_mg &m,
 template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 {
     extern __shared__ double lds_raw[];
-    __shared__ double mean;
     __shared__ double vbuf[PVK_MG_BOTTOM];
+    __shared__ LdsLev<T> lv[PVK_MG_LOW + 1];      // levels c0 .. nb
     T *lds = (T *)lds_raw;
     const int nb = m.nlev - 1;
     // The loads below are valid whatever the stop flag says, so they are
@@ -638,6 +656,9 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
         lby[k] = in ? G.by[threadIdx.x] : T(0);
     }
     if (sc->p_done) return;
+    // the level descriptors, formed once: the schedule below reads them from
+    // LDS instead of walking the kernel argument with a run-time index
+    if (threadIdx.x <= (unsigned)(nb - m.c0)) lv[threadIdx.x] = lds_level(m, lds, m.c0 + (int)threadIdx.x);
     {
         const LdsLev<T> L0 = lds_level(m, lds, m.c0);
 #pragma unroll
@@ -674,35 +695,30 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     unsigned second = 0u;
     for (;;) {
         if (l == nb) {
-            // bottom: x = -P G P f, P the projection to zero mean (in FP64); the
-            // means in the first wavefront, row r of G by the 16 lanes 16 r .. 16 r + 15
+            // bottom: x = -P G P f, P the projection to zero mean (in FP64).
+            // Row r of G is applied by the 16 lanes 16 r .. 16 r + 15; every
+            // group of 16 lanes forms the two means itself (same order, same
+            // values), so the bottom needs two workgroup barriers.
             {
-                const LdsLev<T> B = lds_level(m, lds, nb);
+                const LdsLev<T> B = lv[nb - m.c0];
                 const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
-                if (threadIdx.x < 64) {
-                    double s = 0.0;
-                    for (int k = threadIdx.x; k < n; k += 64) s += (double)B.f[k];
-                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-                    if (threadIdx.x == 0) mean = s / n;
-                }
-                __syncthreads();
+                double s = 0.0;
+                for (int k = q; k < n; k += 16) s += (double)B.f[k];
+                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
+                const double mf = s / n;
                 double v = 0.0;
 #pragma unroll
                 for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
                     const int c = q + 16 * t;
-                    if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mean);
+                    if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mf);
                 }
                 for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
                 if (q == 0 && r < n) vbuf[r] = v;
                 __syncthreads();
-                if (threadIdx.x < 64) {
-                    double s = 0.0;
-                    for (int k = threadIdx.x; k < n; k += 64) s += vbuf[k];
-                    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, 64);
-                    if (threadIdx.x == 0) mean = s / n;
-                }
-                __syncthreads();
-                if ((int)threadIdx.x < n) B.x[threadIdx.x] = (T)(vbuf[threadIdx.x] - mean);
+                double sv = 0.0;
+                for (int k = q; k < n; k += 16) sv += vbuf[k];
+                for (int off = 8; off > 0; off >>= 1) sv += __shfl_xor(sv, off, 16);
+                if (q == 0 && r < n) B.x[r] = (T)(v - sv / n);
                 __syncthreads();
             }
             l--;
@@ -710,7 +726,7 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
             if (l < m.c0) break;
             continue;
         }
-        const LdsLev<T> Ll = lds_level(m, lds, l), Lc = lds_level(m, lds, l + 1);
+        const LdsLev<T> Ll = lv[l - m.c0], Lc = lv[l + 1 - m.c0];
         if (down) {
             lds_pre(Ll, Lc, rs, m.nu, true);
             l++;
