@@ -163,6 +163,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
         s.check(lib.pv_set_pressure_warm_start(s.h, args.warm_start))
         s.check(lib.pv_set_ch_solver(s.h, args.ch_spectral))
         s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
+        s.check(lib.pv_set_mg_sweeps(s.h, args.mg_sweeps))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -290,6 +291,8 @@ def main():
                     help="MG-PCG solvers: 1 = also try the same RK3 stage's last correction as the start")
     ap.add_argument("--mg-cycle", type=int, choices=[0, 1, 2, 3, 4], default=3,
                     help="MG solvers: the first k in-workgroup levels take two coarse corrections (0 = V-cycle)")
+    ap.add_argument("--mg-sweeps", type=int, choices=[1, 2], default=2,
+                    help="MG solvers: Gauss-Seidel sweeps before and after each coarse correction")
     ap.add_argument("--ch-spectral", type=int, choices=[0, 1], default=1,
                     help="MG-PCG forms: 1 = start the biharmonic CG from its exact spectral solve")
     ap.add_argument("--no-compare", action="store_true",

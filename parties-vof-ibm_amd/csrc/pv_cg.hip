@@ -291,11 +291,14 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
 // K1: stop test sqrt(r.r / N) < CG_ETOL (msolve_cg.c:691-704), beta =
 // rz_new / (rz + EPS), d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
 // the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
+#define MCH 256                                                  // columns per chunk of k_mcg_apply
+#define MSQ (((PVK_MOM_BAND + 2) * (MCH + 2) + NTH - 1) / NTH)     // staged values per thread
 template <class R>
 __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w,
                                                   int n)
 {
     __shared__ double sh[NTH];
+    __shared__ double sd[PVK_MOM_BAND + 2][MCH + 2];
     const mom_band m = mom_locate(g, w);
     const int c = m.comp;
     if (sc->m_done[c]) return;
@@ -337,29 +340,37 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         if (b <= 0 || b >= NYM) return AT(dwall, ia, b);
         return (n == 0) ? AT(dwall, ia, b) : AT(r, ia, b) / AT(dg, ia, b) + beta * AT(dold, ia, b);
     };
+    // The band (at most PVK_MOM_BAND rows) in chunks of MCH columns: d of the
+    // chunk and a one-cell ring is formed once into LDS (every value as dv
+    // forms it), then A d of the chunk reads its neighbours there.
+    const int nrow = m.j1 - m.j0;
     double acc = 0.0;
-    for (int j = m.j0; j < m.j1; j++)
-    for (int base = 0; base < NXM; base += NTH * CPT) {
-        double dc[CPT], ad[CPT];
+    for (int cb = 0; cb < NXM; cb += MCH) {
+        double sv[MSQ];
 #pragma unroll
-        for (int k = 0; k < CPT; k++) {
-            const int q = base + k * NTH + (int)threadIdx.x, i = m.i0 + q;
-            dc[k] = 0.0; ad[k] = 0.0;
-            if (q < NXM) {
-                dc[k] = dv(i, j);
-                ad[k] = (double)mom_op_vals<R>(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, R(dc[k]), R(dv(i + 1, j)),
-                                               R(dv(i - 1, j)), R(dv(i, j + 1)), R(dv(i, j - 1)));
-            }
+        for (int k = 0; k < MSQ; k++) {               // all loads of the stage in flight together
+            const int q = threadIdx.x + k * NTH, a = q / (MCH + 2), bb = q - a * (MCH + 2);
+            sv[k] = (a < nrow + 2) ? dv(m.i0 + cb - 1 + bb, m.j0 - 1 + a) : 0.0;
         }
 #pragma unroll
-        for (int k = 0; k < CPT; k++) {
-            const int q = base + k * NTH + (int)threadIdx.x, i = m.i0 + q;
-            if (q < NXM) {
-                if (n > 0) AT(dnew, i, j) = dc[k];
-                AT(Ad, i, j) = ad[k];
-                acc += dc[k] * ad[k];
+        for (int k = 0; k < MSQ; k++) {
+            const int q = threadIdx.x + k * NTH, a = q / (MCH + 2), bb = q - a * (MCH + 2);
+            if (a < PVK_MOM_BAND + 2) sd[a][bb] = sv[k];
+        }
+        __syncthreads();
+        for (int q = threadIdx.x; q < nrow * MCH; q += NTH) {
+            const int a = q / MCH, bb = q - a * MCH, j = m.j0 + a, i = m.i0 + cb + bb;
+            if (cb + bb < NXM) {
+                const double dc = sd[a + 1][bb + 1];
+                const double ad = (double)mom_op_vals<R>(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, R(dc),
+                                                         R(sd[a + 1][bb + 2]), R(sd[a + 1][bb]), R(sd[a + 2][bb + 1]),
+                                                         R(sd[a][bb + 1]));
+                if (n > 0) AT(dnew, i, j) = dc;
+                AT(Ad, i, j) = ad;
+                acc += dc * ad;
             }
         }
+        __syncthreads();
     }
     block_store_partial(acc, w.dad[c], m.band, sh);
 }

@@ -106,6 +106,7 @@ typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
     int npap;                        /* <p, Ap> partials: one per 128 x 8 tile */
 } pvk_ccg;
 
+#define PVK_MOM_BAND 4                /* rows per block of the momentum CG kernels */
 typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double *x[2], *r[2], *dg[2], *Ad[2];
     double *d[2][2];                 /* [component][n & 1]; d[c][0] is the public field */

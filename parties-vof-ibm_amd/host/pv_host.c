@@ -267,12 +267,13 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
 }
 
 /* Work sets of the fused biharmonic and momentum CG loops: one band of rows
- * per 256-thread block, one partial-sum slot per band and reduction. */
+ * per 256-thread block (1 row for the biharmonic CG, PVK_MOM_BAND rows for
+ * the momentum CG), one partial-sum slot per band and reduction. */
 int pv_cg_create(pv_bag *b)
 {
     const int nx = b->NXM, ny = b->NYM;
-    const int band = 1;
-    int nbc = (ny + band - 1) / band, nbu = nbc, nbv = (ny - 1 + band - 1) / band, c;
+    const int band = 1, mband = PVK_MOM_BAND;
+    int nbc = (ny + band - 1) / band, nbu = (ny + mband - 1) / mband, nbv = (ny - 1 + mband - 1) / mband, c;
     int npap = ((ny + 7) / 8) * ((nx + 127) / 128);
     size_t n = 3 * (size_t)nbc + (size_t)npap + 4 * (size_t)nbu + 4 * (size_t)nbv;
     double *q;
@@ -288,7 +289,7 @@ int pv_cg_create(pv_bag *b)
     b->ccg.pap = q; q += npap;
     b->ccg.npap = npap;
     memset(&b->mcg, 0, sizeof b->mcg);
-    b->mcg.band = band;
+    b->mcg.band = mband;
     b->mcg.rho = b->f[PV_F_RHO]; b->mcg.mu = b->f[PV_F_MU];
     b->mcg.maxit = b->ph.CG_MAXIT; b->mcg.tol = b->ph.CG_ETOL;
     /* RMS over the unknowns (msolve_cg.c:691): NXM * NYM for u, NXM * (NYM - 1) for v */
