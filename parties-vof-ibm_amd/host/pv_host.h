@@ -96,6 +96,7 @@ struct pv_bag {
     int warm_wins[3], warm_solves[3];  /* per stage: the temporal start's consecutive wins, solves since
                                           it became trusted */
     int net_losses, net_skip;      /* consecutive losses of the network's start; solves left to skip it */
+    int net_backoff;               /* length of the next skip (16, doubling to 256 while it keeps losing) */
     pvk_ccg ccg;                   /* fused biharmonic CG work set */
     pvk_cnn cnn;                   /* learned preconditioner network (pv_cnn_init) */
     int cnn_ready;

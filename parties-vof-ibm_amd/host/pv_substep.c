@@ -653,11 +653,13 @@ static void fft_solve_field(pv_bag *b, double *src, double tol, int maxit);
  * was trained on, so the FFT tables are needed).  learn_start_res is the
  * chosen start's energy relative to the cold start's; a candidate that loses
  * counts as a fallback.  After 3 losses in a row the network's candidate is
- * not formed for the next 16 solves (then tried again), so a network that
- * does not help costs one try in 19 solves.  Once the temporal start has won
- * 3 evaluations of its stage in a row (and no network candidate is formed),
- * it is taken without forming the cold candidate, except in every 8th solve,
- * which evaluates again; such a solve leaves learn_start_res as it was. */
+ * not formed for the next 16 solves (then tried again); each further run of 3
+ * losses doubles that skip, up to 256 solves, and a win resets it to 16, so
+ * a network that does not help costs ever fewer tries.  Once the temporal
+ * start has won 3 evaluations of its stage in a row (and no network
+ * candidate is formed), it is taken without forming the cold candidate,
+ * except in every 8th solve, which evaluates again; such a solve leaves
+ * learn_start_res as it was. */
 static int mg_solve(pv_bag *b, int learned)
 {
     int maxit = b->ph.p_max_iter, st = b->stage, best = 0;
@@ -715,11 +717,14 @@ static int mg_solve(pv_bag *b, int learned)
             eb = e;
             best = 2;
             b->net_losses = 0;
+            b->net_backoff = 16;
         } else {
             b->st.p_fallback[st] += 1;
             if (++b->net_losses >= 3) {
+                if (b->net_backoff < 16) b->net_backoff = 16;
                 b->net_losses = 0;
-                b->net_skip = 16;
+                b->net_skip = b->net_backoff;
+                if (b->net_backoff < 256) b->net_backoff *= 2;
             }
         }
     }
