@@ -72,8 +72,8 @@ BYTES_PER_ITER = {"fft": 176, "mg": 189, "mg32": 155, "jacobi": 112}
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.1, "MG_POST0": 32.0, "P_APPLY": 51.5,
-                        "MOM_APPLY": 121.9, "MOM_UPDATE": 119.6, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
+TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.2, "MG_POST0": 32.0, "P_APPLY": 51.5,
+                        "MOM_APPLY": 122.9, "MOM_UPDATE": 118.9, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 
 

@@ -607,7 +607,7 @@ static int mg_loop_graphs(pv_bag *b, double tol, int maxit)
 }
 
 /* iterations n0, n0 + 1, ... until the device stop flag; the first check
- * comes after the previous solve's iteration count less two */
+ * comes after the previous solve's iteration count plus one */
 static int mg_loop(pv_bag *b, int n0, double tol, int maxit)
 {
     int chunk = b->mg_chunk, n = n0;
@@ -621,7 +621,9 @@ static int mg_loop(pv_bag *b, int n0, double tol, int maxit)
         if (n >= maxit) return pv_set_error(PV_EHIP, "pressure MG-PCG did not stop (device flag)");
         chunk = 2;
     }
-    b->mg_chunk = (b->h_sc->p_iter > 4) ? b->h_sc->p_iter - 2 : 2;
+    /* an iteration launched past the stop costs its early-exit launches, a
+     * second round trip to the host costs more: aim one past the last count */
+    b->mg_chunk = (b->h_sc->p_iter > 1) ? b->h_sc->p_iter + 1 : 2;
     return PV_OK;
 }
 
