@@ -268,6 +268,64 @@ def mixed_error(lib, args):
             "mixed_ms_per_step": round(1e3 * secs["mixed"] / steps, 2)}
 
 
+def batch_bench(lib, args):
+    """Batched cases: --batch-cases cases of sizes drawn (seeded) from the
+    batched configuration's {96^2, 192^2, 256x512, 384x768}, FP64, MG-PCG with
+    the W on the in-workgroup levels and the temporal start, seeded bubbles
+    (the IBM cylinder and wetting of that configuration are not part of this
+    path).  Aggregate Mcell-steps/s of pv_batch_rk3 (one stream and host
+    thread per case) against the same cases stepped one after another."""
+    rng = np.random.default_rng(SEED)
+    shapes = [(96, 96), (192, 192), (256, 512), (384, 768)]
+    picks = [shapes[k] for k in rng.integers(0, len(shapes), args.batch_cases)]
+    params = []
+    for nx, ny in picks:
+        p = workload_params(nx, pvof.PV_PSOLVE_MG_PCG, pvof.PV_PREC_FP64)
+        p.NYM, p.ymax = ny, ny / nx
+        p.Cn, p.Pe_CH = 2.0 / nx, nx / 2.0          # interface two cells wide on every size
+        params.append(p)
+    fields = [pvof.ic(lib, "bubbles", p, seed=SEED + k, nmin=2, nmax=6, rmin=0.05, rmax=0.15, gas_inside=1)
+              for k, p in enumerate(params)]
+
+    def setup(s, F):
+        s.init_from_F(F)
+        s.check(lib.pv_set_pressure_warm_start(s.h, 1))
+        s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
+
+    cells = sum(p.NXM * p.NYM for p in params)
+    bt = pvof.Batch(params, lib)
+    for c, F in zip(bt.cases, fields):
+        setup(c, F)
+    note(f"batch of {len(params)} cases, {cells / 1e6:.2f} Mcells: {args.warmup} + {args.steps} steps")
+    bt.rk3(args.warmup)
+    for c in bt.cases:
+        c.call("synchronize")
+    t0 = time.perf_counter()
+    bt.rk3(args.steps)
+    for c in bt.cases:
+        c.call("synchronize")
+    tb = time.perf_counter() - t0
+    bt.close()
+    note("the same cases one after another")
+    ts = 0.0
+    for p, F in zip(params, fields):
+        s = pvof.Solver(p, lib, device=0)
+        setup(s, F)
+        s.call("Temporal_int_rk3", args.warmup)
+        s.call("synchronize")
+        t0 = time.perf_counter()
+        s.call("Temporal_int_rk3", args.steps)
+        s.call("synchronize")
+        ts += time.perf_counter() - t0
+        s.close()
+    return {"cases": len(params), "sizes": {f"{nx}x{ny}": picks.count((nx, ny)) for nx, ny in shapes},
+            "cells": cells, "steps": args.steps, "warmup": args.warmup,
+            "batch_Mcell_steps_per_s": round(cells * args.steps / tb / 1e6, 3),
+            "one_after_another_Mcell_steps_per_s": round(cells * args.steps / ts / 1e6, 3),
+            "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
+            "mg_cycle_w_levels": args.mg_cycle}
+
+
 def note(msg):
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
@@ -311,6 +369,9 @@ def main():
                     help="only report the per-field error of the mixed form against the FP64 path after this "
                          "many steps at --n (with --ic)")
     ap.add_argument("--ic", choices=["bubbles", "droplets"], default="bubbles")
+    ap.add_argument("--batch-cases", type=int, default=0,
+                    help="only report batched throughput: this many cases of the batched configuration's sizes "
+                         "(--steps, --warmup per case)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device three times and exit: a kernel of known "
                          "traffic, to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in a run of its own")
@@ -323,6 +384,9 @@ def main():
     lib = pvof.load_lib()
     if args.mixed_error_steps > 0:
         print(json.dumps({"mixed_error": mixed_error(lib, args)}))
+        return
+    if args.batch_cases > 0:
+        print(json.dumps({"batch": batch_bench(lib, args)}))
         return
     if args.copy_mib > 0:
         import torch
