@@ -1,0 +1,61 @@
+"""Writes config1_200_steps.npz: the oracle's (oracle/pvo.c) state after 200
+RK3 steps of the small plain configuration (128 x 256 on a 1 x 2 domain, one
+gas bubble of radius 0.25 at (0.5, 0.5), density ratio 1e-3, FP64, FFT-PCG
+pressure solve to 1e-13 relative, CH and momentum solves tight), for the GPU test that checks the plain
+path over the whole run (tests/test_gpu_parity.py::test_config1_200_steps).
+The oracle needs about 11 minutes for it on one core, too long for a test
+run; the fixture keeps every 4th cell of C_L, U, V and P in each direction,
+the maximum magnitude and the L2 norm of each full field, the time and the
+phase volume sum(C_L) before and after.  Run from the repository root:
+    python tests/golden/make_config1_200.py
+"""
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(HERE, ".."))
+sys.path.insert(0, os.path.join(HERE, "..", "..", "parties-vof-ibm_amd"))
+import oracle_lib  # noqa: E402
+import pvof  # noqa: E402
+from helpers import regions  # noqa: E402
+
+STEPS = 200
+FIELDS = (("C_L", "cell"), ("U", "u"), ("V", "v"), ("P", "cell"))
+
+
+def config1():
+    # solves converged far below the comparison bound, so that no stop
+    # decision flipped by round-off can move a field by more than 1e-13
+    return pvof.config1_params(NXM=128, NYM=256, pressure_solver=pvof.PV_PSOLVE_FFT_PCG, p_rel_tol=1e-13,
+                               ch_tol=1e-13, ch_iter_max=1000, CG_ETOL=1e-11, CG_MAXIT=400)
+
+
+def initial_field(lib, p):
+    return pvof.ic(lib, "rising_bubble", p)
+
+
+def main():
+    lib = pvof.load_lib()
+    p = config1()
+    o = oracle_lib.Oracle(p)
+    o.init_from_F(initial_field(lib, p))
+    R = regions(p)
+    vol0 = float(o.get("C_L")[R["cell"]].sum())
+    for k in range(STEPS // 10):
+        assert o.call("Temporal_int_rk3", 10) == 0
+        print(f"oracle: {10 * (k + 1)} steps", flush=True)
+    out = {"steps": np.int64(STEPS), "time": np.float64(o.stats()["time"]), "vol0": np.float64(vol0),
+           "vol": np.float64(o.get("C_L")[R["cell"]].sum())}
+    for f, r in FIELDS:
+        a = o.get(f)[R[r]]
+        out[f + "_sub"] = a[::4, ::4].copy()
+        out[f + "_max"] = np.float64(np.abs(a).max())
+        out[f + "_l2"] = np.float64(np.sqrt((a * a).sum()))
+    np.savez_compressed(os.path.join(HERE, "config1_200_steps.npz"), **out)
+    print("wrote config1_200_steps.npz", flush=True)
+
+
+if __name__ == "__main__":
+    main()

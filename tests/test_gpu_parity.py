@@ -7,6 +7,8 @@ correctly rounded) and x*x on the GPU: 1e-13 relative there.  Iterative
 solves are compared on their converged solutions.
 """
 import ctypes as C
+import os
+import sys
 
 import numpy as np
 import pytest
@@ -234,6 +236,48 @@ def test_trajectory_10_steps(gpu, lib, solver):
     assert abs(so["time"] - ss["time"]) <= 1e-15 * so["time"]
     for fld, r in (("C_L", "cell"), ("U", "u"), ("V", "v"), ("P", "cell")):
         assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-11, fld
+
+
+def test_config1_200_steps(gpu, lib):
+    """The plain FP64 path over the whole small plain run: 128 x 256, one gas
+    bubble of radius 0.25, 200 RK3 steps (FFT-PCG to 1e-13, CH and momentum
+    solves converged far below the bound), against the oracle's state stored
+    in tests/golden/config1_200_steps.npz by tests/golden/make_config1_200.py
+    (the oracle needs about 11 minutes for it): every 4th cell of C_L, u, v
+    and p, and the maximum and L2 norm of each full field, to relative
+    1e-11; every step changes sum(C_L) by the clamp correction only, to the
+    round-off of the sum (N eps); the time agrees to round-off."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_config1_200 as mk
+    g = np.load(os.path.join(here, "golden", "config1_200_steps.npz"))
+    p = mk.config1()
+    s = pvof.Solver(p, lib)
+    s.init_from_F(mk.initial_field(lib, p))
+    R = regions(s.params)
+    meas = (s.params.xmax - s.params.xmin) / p.NXM * (s.params.ymax - s.params.ymin) / p.NYM * \
+        s.params.twod_slab_thickness
+    vol0 = s.get("C_L")[R["cell"]].sum()
+    assert abs(vol0 - g["vol0"]) <= 1e-13 * abs(g["vol0"])
+    # round-off bound of an N-term sum of positive values: N eps
+    bound, drift = p.NXM * p.NYM * np.finfo(float).eps, 0.0
+    for _ in range(int(g["steps"])):
+        m0 = s.get("C_L")[R["cell"]].sum() * meas
+        s.call("Temporal_int_rk3", 1)
+        m1 = s.get("C_L")[R["cell"]].sum() * meas
+        drift = max(drift, abs((m1 - m0) - sum(s.stats()["clamp_delta"])) / m0)
+        assert drift < bound, drift
+    assert abs(s.stats()["time"] - g["time"]) <= 1e-13 * g["time"]
+    report = []
+    for f, r in mk.FIELDS:
+        a = s.get(f)[R[r]]
+        e_sub = np.abs(a[::4, ::4] - g[f + "_sub"]).max() / g[f + "_max"]
+        e_max = abs(np.abs(a).max() - g[f + "_max"]) / g[f + "_max"]
+        e_l2 = abs(np.sqrt((a * a).sum()) - g[f + "_l2"]) / g[f + "_l2"]
+        report.append(f"{f}: sampled Linf {e_sub:.1e}, max {e_max:.1e}, L2 {e_l2:.1e}")
+        assert max(e_sub, e_max, e_l2) < 1e-11, f
+    print("\n" + "\n".join(report) + f"\nlargest per-step phase-volume change beyond the clamp: {drift:.1e} "
+          f"(bound {bound:.1e})")
 
 
 @pytest.mark.parametrize("n", [64, 512, 1024])
