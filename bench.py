@@ -326,6 +326,52 @@ def batch_bench(lib, args):
             "mg_cycle_w_levels": args.mg_cycle}
 
 
+def config2_run(lib, args):
+    """The plain FP64 path at scale: 512 x 1024 cells, heavy fluid above a
+    seeded 8-mode interface, Atwood 0.5, Re 3000, Cn 0.005, viscosity ratio 1,
+    FFT-PCG pressure solve, --config2-steps RK3 steps; square cells (a 1 x 2
+    domain, interface at y = 1), which the uniform-grid pressure right-hand
+    side needs.  Throughput, iteration counts, the phase-volume change less
+    the clamp corrections, the largest divergence and finiteness."""
+    p = workload_params(512, pvof.PV_PSOLVE_FFT_PCG, pvof.PV_PREC_FP64)
+    for k, v in dict(NYM=1024, ymax=2.0, Re=3000.0, rho2=1.0 / 3.0, mu2=1.0, Cn=0.005, Pe_CH=200.0,
+                     default_dt=1e-4).items():
+        setattr(p, k, v)
+    F = pvof.ic(lib, "multimode_rt", p, seed=SEED, modes=8, amp_max=0.05, y0=1.0)
+    s = pvof.Solver(p, lib, device=0)
+    s.init_from_F(F)
+    R = (slice(3, 3 + p.NYM), slice(3, 3 + p.NXM))
+    pp = s.params
+    meas = (pp.xmax - pp.xmin) / p.NXM * (pp.ymax - pp.ymin) / p.NYM * pp.twod_slab_thickness
+    vol0 = s.get("C_L")[R].sum() * meas
+    clamp, iters, div = 0.0, {"n_ch": 0, "n_u": 0, "n_v": 0, "n_p": 0}, 0.0
+    steps, done = args.config2_steps, 0
+    t = 0.0
+    while done < steps:
+        k = min(50, steps - done)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            s.call("Temporal_int_rk3", 1)
+            st = s.stats()
+            clamp += sum(st["clamp_delta"])
+            div = max(div, max(st["div_max"]))
+            for key in iters:
+                iters[key] += sum(st[key])
+        s.call("synchronize")
+        t += time.perf_counter() - t0
+        done += k
+        note(f"config 2: {done} of {steps} steps")
+    vol = s.get("C_L")[R].sum() * meas
+    finite = all(bool(np.isfinite(s.get(f)).all()) for f in ("C_L", "U", "V", "P"))
+    cells = p.NXM * p.NYM
+    return {"NXM": p.NXM, "NYM": p.NYM, "steps": steps, "time": s.stats()["time"],
+            "Mcell_steps_per_s": round(cells * steps / t / 1e6, 3), "ms_per_step": round(1e3 * t / steps, 3),
+            "iterations_per_step": {k: round(v / steps, 2) for k, v in iters.items()},
+            "phase_volume_change_rel": float((vol - vol0) / vol0), "clamp_total_rel": float(clamp / vol0),
+            "phase_volume_change_less_clamp_rel": float(abs((vol - vol0) - clamp) / vol0),
+            "max_divergence": div, "finite": finite, "precision": "fp64", "pressure_solver": "fft"}
+
+
 def note(msg):
     print(f"bench.py: {msg}", file=sys.stderr, flush=True)
 
@@ -369,6 +415,8 @@ def main():
                     help="only report the per-field error of the mixed form against the FP64 path after this "
                          "many steps at --n (with --ic)")
     ap.add_argument("--ic", choices=["bubbles", "droplets"], default="bubbles")
+    ap.add_argument("--config2-steps", type=int, default=0,
+                    help="only run the plain FP64 path at scale (512 x 1024 Rayleigh-Taylor) for this many steps")
     ap.add_argument("--batch-cases", type=int, default=0,
                     help="only report batched throughput: this many cases of the batched configuration's sizes "
                          "(--steps, --warmup per case)")
@@ -387,6 +435,9 @@ def main():
         return
     if args.batch_cases > 0:
         print(json.dumps({"batch": batch_bench(lib, args)}))
+        return
+    if args.config2_steps > 0:
+        print(json.dumps({"config2": config2_run(lib, args)}))
         return
     if args.copy_mib > 0:
         import torch

@@ -280,6 +280,35 @@ def test_config1_200_steps(gpu, lib):
           f"(bound {bound:.1e})")
 
 
+def test_config2_10_steps(gpu, lib):
+    """The plain FP64 path at the scale configuration's size (512 x 1024
+    square cells, 8-mode Rayleigh-Taylor interface, Atwood 0.5, Re 3000)
+    against the oracle's state after 10 RK3 steps stored in
+    tests/golden/config2_10_steps.npz by tests/golden/make_config2_10.py (the
+    oracle needs about 4 minutes for it): every 16th cell of C_L, u, v and p,
+    and the maximum and L2 norm of each full field, to relative 1e-11; the
+    time agrees to round-off."""
+    here = os.path.dirname(os.path.abspath(__file__))
+    sys.path.insert(0, os.path.join(here, "golden"))
+    import make_config2_10 as mk
+    g = np.load(os.path.join(here, "golden", "config2_10_steps.npz"))
+    p = mk.config2()
+    s = pvof.Solver(p, lib)
+    s.init_from_F(mk.initial_field(lib, p))
+    s.call("Temporal_int_rk3", int(g["steps"]))
+    R = regions(s.params)
+    assert abs(s.stats()["time"] - g["time"]) <= 1e-13 * g["time"]
+    report = []
+    for f, r in mk.FIELDS:
+        a = s.get(f)[R[r]]
+        e_sub = np.abs(a[::mk.STRIDE, ::mk.STRIDE] - g[f + "_sub"]).max() / g[f + "_max"]
+        e_max = abs(np.abs(a).max() - g[f + "_max"]) / g[f + "_max"]
+        e_l2 = abs(np.sqrt((a * a).sum()) - g[f + "_l2"]) / g[f + "_l2"]
+        report.append(f"{f}: sampled Linf {e_sub:.1e}, max {e_max:.1e}, L2 {e_l2:.1e}")
+        assert max(e_sub, e_max, e_l2) < 1e-11, f
+    print("\n" + "\n".join(report))
+
+
 @pytest.mark.parametrize("n", [64, 512, 1024])
 def test_fft_solve_sizes(gpu, lib, n):
     """Standalone FFT-x / DCT-y solve against the oracle's FFT + Thomas solve."""
