@@ -331,8 +331,11 @@ def config2_run(lib, args):
     seeded 8-mode interface, Atwood 0.5, Re 3000, Cn 0.005, viscosity ratio 1,
     FFT-PCG pressure solve, --config2-steps RK3 steps; square cells (a 1 x 2
     domain, interface at y = 1), which the uniform-grid pressure right-hand
-    side needs.  Throughput, iteration counts, the phase-volume change less
-    the clamp corrections, the largest divergence and finiteness."""
+    side needs.  Throughput, iteration counts, the phase-volume change, the
+    clamp corrections as the reference reports them (summed over its G range,
+    which includes the duplicated periodic column and the ghost row above the
+    top wall, VOF_DIFFUSE.c:1093-1127), the difference of the two, the
+    largest divergence and finiteness."""
     p = workload_params(512, pvof.PV_PSOLVE_FFT_PCG, pvof.PV_PREC_FP64)
     for k, v in dict(NYM=1024, ymax=2.0, Re=3000.0, rho2=1.0 / 3.0, mu2=1.0, Cn=0.005, Pe_CH=200.0,
                      default_dt=1e-4).items():
