@@ -111,7 +111,7 @@ struct pv_bag {
     hipGraphExec_t *mggraph;       /* the same for MG-PCG iterations */
     int nmggraph;
     const pvk_mg *mggraph_key;     /* hierarchy and sweeps the MG graphs were captured with */
-    int mggraph_nu;
+    int mggraph_nu, mggraph_w;     /* sweeps and W levels the MG graphs were captured with */
     double learn_start_res;        /* relative residual after the last warm start */
 };
 

@@ -567,16 +567,18 @@ static void mg_cold_start(pv_bag *b, double tol, int maxit)
 }
 
 /* MG-PCG iterations 0, 1, ... replayed from hipGraphs: graph kc holds
- * iterations [kc C, (kc + 1) C), captured on first use for the hierarchy
- * and sweep count in use (the kernels stop themselves on the device flag). */
+ * iterations [kc C, (kc + 1) C), captured on first use for the hierarchy,
+ * sweep count and W levels in use (the kernels stop themselves on the device
+ * flag). */
 static int mg_loop_graphs(pv_bag *b, double tol, int maxit)
 {
     const int C = b->graph_chunk;
     int kc = 0, n = 0;
-    if (b->mggraph_key != b->mgp || b->mggraph_nu != b->mgp->nu) {
+    if (b->mggraph_key != b->mgp || b->mggraph_nu != b->mgp->nu || b->mggraph_w != b->mgp->wmask) {
         mg_graphs_free(b);
         b->mggraph_key = b->mgp;
         b->mggraph_nu = b->mgp->nu;
+        b->mggraph_w = b->mgp->wmask;
     }
     for (;;) {
         if (kc >= b->nmggraph) {

@@ -80,3 +80,25 @@ def test_autotune_keeps_the_trajectory(lib):
     b.call("Temporal_int_rk3", 2)
     for f in FIELDS:
         assert np.array_equal(a.get(f), b.get(f)), f
+
+
+def test_mg_graph_replay_follows_cycle_change(lib):
+    """MG-PCG replayed from hipGraphs stays bitwise equal to eager launches
+    when the multigrid cycle changes between steps (graphs captured with the
+    V-cycle are not replayed for the W)."""
+    p = case_params(128, 128)
+    p.pressure_solver = pvof.PV_PSOLVE_MG_PCG
+    F = bubble_field(lib, p, 11)
+    a, b = pvof.Solver(p, lib), pvof.Solver(p, lib)
+    a.init_from_F(F)
+    b.init_from_F(F)
+    b.check(lib.pv_set_graphs(b.h, 1, 8))
+    for step in range(4):
+        if step == 2:
+            for s in (a, b):
+                s.check(lib.pv_set_mg_cycle(s.h, 2))
+        a.call("Temporal_int_rk3", 1)
+        b.call("Temporal_int_rk3", 1)
+        assert a.stats()["n_p"] == b.stats()["n_p"], step
+        for f in FIELDS:
+            assert np.array_equal(a.get(f), b.get(f)), (step, f)
