@@ -282,7 +282,10 @@ def batch_bench(lib, args):
     the W on the in-workgroup levels and the temporal start, seeded bubbles
     (the IBM cylinder and wetting of that configuration are not part of this
     path).  Aggregate Mcell-steps/s of pv_batch_rk3 (one stream and host
-    thread per case) against the same cases stepped one after another."""
+    thread per case) against the same cases stepped one after another.  With
+    --batch-autotune each case's cycle and sweep count come from
+    pv_batch_autotune_mg after the warm-up (untimed; the one-after-another
+    run uses the same choices)."""
     rng = np.random.default_rng(SEED)
     shapes = [(96, 96), (192, 192), (256, 512), (384, 768)]
     picks = [shapes[k] for k in rng.integers(0, len(shapes), args.batch_cases)]
@@ -306,6 +309,10 @@ def batch_bench(lib, args):
         setup(c, F)
     note(f"batch of {len(params)} cases, {cells / 1e6:.2f} Mcells: {args.warmup} + {args.steps} steps")
     bt.rk3(args.warmup)
+    choices = None
+    if args.batch_autotune:
+        note("autotuning every case's multigrid cycle")
+        choices = bt.autotune_mg()
     for c in bt.cases:
         c.call("synchronize")
     t0 = time.perf_counter()
@@ -316,9 +323,12 @@ def batch_bench(lib, args):
     bt.close()
     note("the same cases one after another")
     ts = 0.0
-    for p, F in zip(params, fields):
+    for k, (p, F) in enumerate(zip(params, fields)):
         s = pvof.Solver(p, lib, device=0)
         setup(s, F)
+        if choices is not None:
+            s.check(lib.pv_set_mg_cycle(s.h, choices[k][0]))
+            s.check(lib.pv_set_mg_sweeps(s.h, choices[k][1]))
         s.call("Temporal_int_rk3", args.warmup)
         s.call("synchronize")
         t0 = time.perf_counter()
@@ -331,7 +341,12 @@ def batch_bench(lib, args):
             "batch_Mcell_steps_per_s": round(cells * args.steps / tb / 1e6, 3),
             "one_after_another_Mcell_steps_per_s": round(cells * args.steps / ts / 1e6, 3),
             "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
-            "mg_cycle_w_levels": args.mg_cycle}
+            "mg_cycle_w_levels": args.mg_cycle if choices is None else "autotuned",
+            "mg_choices": None if choices is None else
+            {f"{nx}x{ny}": sorted({f"W{w}/nu{nu}": sum(1 for q, ch in zip(picks, choices) if q == (nx, ny) and
+                                                      ch == (w, nu))
+                                   for q, (w, nu) in zip(picks, choices) if q == (nx, ny)}.items())
+             for nx, ny in shapes}}
 
 
 def config2_run(lib, args):
@@ -431,6 +446,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=0,
                     help="only report batched throughput: this many cases of the batched configuration's sizes "
                          "(--steps, --warmup per case)")
+    ap.add_argument("--batch-autotune", action="store_true",
+                    help="with --batch-cases: pick each case's multigrid cycle and sweeps with "
+                         "pv_batch_autotune_mg after the warm-up")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device three times and exit: a kernel of known "
                          "traffic, to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in a run of its own")

@@ -268,7 +268,12 @@ int pv_cnn_adam_step(pv_bag *bag, float lr, int t);
  * batch holds independent cases of any size (one handle and stream each);
  * pv_batch_rk3 advances all of them, one host thread per case, so their
  * kernels overlap on the GPU; each case computes exactly what it computes
- * alone. */
+ * alone.  pv_autotune_mg times the multigrid cycles (W on 0 .. 3 of the
+ * in-workgroup levels) and sweep counts (1, 2) on cold solves of the
+ * pressure right-hand side in place (call it after a step), keeps the
+ * fastest and leaves the trajectory as those settings give it;
+ * pv_batch_autotune_mg does it for every MG-PCG case of a batch (-1 for the
+ * others). */
 typedef struct pv_batch pv_batch;
 int pv_set_graphs(pv_bag *bag, int on, int chunk);
 int pv_autotune(pv_bag *bag, int *chunk);
@@ -278,6 +283,8 @@ int pv_batch_size(const pv_batch *bt);
 pv_bag *pv_batch_case(pv_batch *bt, int k);
 int pv_batch_rk3(pv_batch *bt, int nsteps);
 int pv_batch_autotune(pv_batch *bt, int *chunks);
+int pv_autotune_mg(pv_bag *bag, int *wlevels, int *sweeps);
+int pv_batch_autotune_mg(pv_batch *bt, int *wlevels, int *sweeps);
 
 /* ---- pressure and Cahn-Hilliard solver options ---------------------------
  * No counterpart in the reference. */

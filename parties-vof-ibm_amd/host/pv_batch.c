@@ -100,6 +100,22 @@ int pv_batch_rk3(pv_batch *bt, int nsteps)
     return rc;
 }
 
+/* pv_autotune_mg on every MG-PCG case, one case after another (each timed
+ * alone); wlevels[k], sweeps[k] = its choice, -1 for a case without MG-PCG */
+int pv_batch_autotune_mg(pv_batch *bt, int *wlevels, int *sweeps)
+{
+    int k, rc;
+    if (!bt) return pv_set_error(PV_EINVAL, "null batch");
+    for (k = 0; k < bt->n; k++) {
+        int w = -1, nu = -1, ps = bt->bags[k]->prm.pressure_solver;
+        if (ps == PV_PSOLVE_MG_PCG || ps == PV_PSOLVE_LEARNED_MG)
+            if ((rc = pv_autotune_mg(bt->bags[k], &w, &nu))) return rc;
+        if (wlevels) wlevels[k] = w;
+        if (sweeps) sweeps[k] = nu;
+    }
+    return PV_OK;
+}
+
 /* pv_autotune on every FFT-PCG case; choices[k] = chosen graph chunk (0 = eager) */
 int pv_batch_autotune(pv_batch *bt, int *choices)
 {

@@ -1146,3 +1146,64 @@ int pv_autotune(pv_bag *b, int *choice)
     if (choice) *choice = chunks[pick];
     return PV_OK;
 }
+
+/* Pick the fastest multigrid cycle (W on 0 .. 3 of the in-workgroup levels)
+ * and sweep count (1, 2) for the MG-PCG pressure solve of the right-hand side
+ * in place (call it after a step).  Each candidate solves from phi = 0 with
+ * no temporal or network start; PHI and the iteration-chunk guess are put
+ * back afterwards and the start bookkeeping is not touched, so the
+ * trajectory is the one the chosen settings give without tuning. */
+int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
+{
+    double best = 1e300, *phi_save = NULL;
+    int w, nu, rep, pick_w = 0, pick_nu = 2, p_warm, chunk, rc = PV_OK;
+    double start_res;
+    size_t bytes;
+    RC(begin(b));
+    if (b->prm.pressure_solver != PV_PSOLVE_MG_PCG && b->prm.pressure_solver != PV_PSOLVE_LEARNED_MG)
+        return pv_set_error(PV_EINVAL, "MG autotune is for the MG-PCG solvers");
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    bytes = sizeof(double) * pv_field_elems(b);
+    if (hipMalloc((void **)&phi_save, bytes) != hipSuccess)
+        return pv_set_error(PV_ENOMEM, "MG autotune: hipMalloc of %zu bytes failed", bytes);
+    if (hipMemcpyAsync(phi_save, FD(b, PV_F_PHI), bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess) {
+        hipFree(phi_save);
+        return pv_set_error(PV_EHIP, "MG autotune: saving PHI failed");
+    }
+    p_warm = b->p_warm;
+    chunk = b->mg_chunk;
+    start_res = b->learn_start_res;
+    b->p_warm = 0;
+    for (w = 0; w <= 3 && rc == PV_OK; w++)
+        for (nu = 1; nu <= 2 && rc == PV_OK; nu++) {
+            hipEvent_t e0 = NULL, e1 = NULL;
+            float ms = 0.0f;
+            if ((rc = pv_set_mg_cycle(b, w)) || (rc = pv_set_mg_sweeps(b, nu))) break;
+            if ((rc = mg_solve(b, 0))) break;                   /* warm-up; captures graphs if on */
+            if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+                rc = pv_set_error(PV_EHIP, "MG autotune: hipEventCreate failed");
+            } else {
+                hipEventRecord(e0, S(b));
+                for (rep = 0; rep < 2 && rc == PV_OK; rep++) rc = mg_solve(b, 0);
+                hipEventRecord(e1, S(b));
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    rc = pv_set_error(PV_EHIP, "MG autotune: event timing failed");
+            }
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            if (rc == PV_OK && ms < best) { best = ms; pick_w = w; pick_nu = nu; }
+        }
+    b->p_warm = p_warm;
+    b->mg_chunk = chunk;
+    b->learn_start_res = start_res;
+    if (hipMemcpyAsync(FD(b, PV_F_PHI), phi_save, bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+        hipStreamSynchronize(S(b)) != hipSuccess)
+        if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "MG autotune: restoring PHI failed");
+    hipFree(phi_save);
+    if (rc) return rc;
+    RC(pv_set_mg_cycle(b, pick_w));
+    RC(pv_set_mg_sweeps(b, pick_nu));
+    if (wlevels) *wlevels = pick_w;
+    if (sweeps) *sweeps = pick_nu;
+    return PV_OK;
+}

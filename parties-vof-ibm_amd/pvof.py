@@ -177,6 +177,8 @@ def load_lib(path=LIB_PATH):
     lib.pv_batch_case.restype = vp
     lib.pv_batch_rk3.argtypes = [vp, C.c_int]
     lib.pv_batch_autotune.argtypes = [vp, ip]
+    lib.pv_autotune_mg.argtypes = [vp, ip, ip]
+    lib.pv_batch_autotune_mg.argtypes = [vp, ip, ip]
     for name in ("pv_ic_rising_bubble", "pv_ic_rayleigh_taylor"):
         getattr(lib, name).argtypes = [C.POINTER(Params), dp]
     lib.pv_ic_multimode_rt.argtypes = [C.POINTER(Params), C.c_uint64, C.c_int, C.c_double, C.c_double, dp]
@@ -324,6 +326,12 @@ class Batch:
         ch = (C.c_int * len(self.cases))()
         self.check(self.lib.pv_batch_autotune(self.h, ch))
         return list(ch)
+
+    def autotune_mg(self):
+        """(W levels, sweeps) chosen per case; (-1, -1) for a case without MG-PCG."""
+        w, nu = (C.c_int * len(self.cases))(), (C.c_int * len(self.cases))()
+        self.check(self.lib.pv_batch_autotune_mg(self.h, w, nu))
+        return list(zip(w, nu))
 
     def close(self):
         if self.h:

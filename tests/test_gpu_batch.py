@@ -102,3 +102,53 @@ def test_mg_graph_replay_follows_cycle_change(lib):
         assert a.stats()["n_p"] == b.stats()["n_p"], step
         for f in FIELDS:
             assert np.array_equal(a.get(f), b.get(f)), (step, f)
+
+
+@pytest.mark.parametrize("warm", [0, 1])
+def test_mg_autotune_keeps_the_trajectory(lib, warm):
+    """pv_autotune_mg between steps: the tuned handle's next steps equal, bit
+    for bit, those of a handle that never tuned and was set to the same cycle
+    and sweep count (with and without the temporal start, whose stored
+    starts and counters the tuning must leave alone)."""
+    p = case_params(128, 128)
+    p.pressure_solver = pvof.PV_PSOLVE_MG_PCG
+    F = bubble_field(lib, p, 13)
+    a, b = pvof.Solver(p, lib), pvof.Solver(p, lib)
+    for s in (a, b):
+        s.init_from_F(F)
+        s.check(lib.pv_set_pressure_warm_start(s.h, warm))
+        s.call("Temporal_int_rk3", 2)
+    w, nu = pvof.C.c_int(-1), pvof.C.c_int(-1)
+    b.check(lib.pv_autotune_mg(b.h, pvof.C.byref(w), pvof.C.byref(nu)))
+    print(f"\nchosen: W on {w.value} levels, {nu.value} sweeps")
+    assert 0 <= w.value <= 3 and nu.value in (1, 2)
+    a.check(lib.pv_set_mg_cycle(a.h, w.value))
+    a.check(lib.pv_set_mg_sweeps(a.h, nu.value))
+    for step in range(2):
+        a.call("Temporal_int_rk3", 1)
+        b.call("Temporal_int_rk3", 1)
+        assert a.stats()["n_p"] == b.stats()["n_p"], step
+        for f in FIELDS:
+            assert np.array_equal(a.get(f), b.get(f)), (step, f)
+
+
+def test_batch_mg_autotune(lib):
+    """Every MG-PCG case of a batch gets a valid choice, other cases (-1, -1);
+    an FFT-PCG handle is refused."""
+    pm = case_params(96, 96)
+    pm.pressure_solver = pvof.PV_PSOLVE_MG_PCG
+    pf = case_params(64, 64)
+    bt = pvof.Batch([pm, pf], lib)
+    for k, c in enumerate(bt.cases):
+        c.init_from_F(bubble_field(lib, c.params, 40 + k))
+    bt.rk3(1)
+    (w0, nu0), (w1, nu1) = bt.autotune_mg()
+    assert 0 <= w0 <= 3 and nu0 in (1, 2)
+    assert (w1, nu1) == (-1, -1)
+    bt.rk3(1)
+    for c in bt.cases:
+        assert max(c.stats()["p_res"]) <= 1e-10
+    bt.close()
+    s = pvof.Solver(pf, lib)
+    assert lib.pv_autotune_mg(s.h, None, None) == -1
+    assert "MG" in lib.pv_last_error().decode()
