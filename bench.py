@@ -302,6 +302,8 @@ def batch_bench(lib, args):
         s.init_from_F(F)
         s.check(lib.pv_set_pressure_warm_start(s.h, 1))
         s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
+        if args.batch_graphs > 0:
+            s.check(lib.pv_set_graphs(s.h, 1, args.batch_graphs))
 
     cells = sum(p.NXM * p.NYM for p in params)
     bt = pvof.Batch(params, lib)
@@ -342,6 +344,7 @@ def batch_bench(lib, args):
             "one_after_another_Mcell_steps_per_s": round(cells * args.steps / ts / 1e6, 3),
             "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
             "mg_cycle_w_levels": args.mg_cycle if choices is None else "autotuned",
+            "graph_chunk": args.batch_graphs or None,
             "mg_choices": None if choices is None else
             {f"{nx}x{ny}": sorted({f"W{w}/nu{nu}": sum(1 for q, ch in zip(picks, choices) if q == (nx, ny) and
                                                       ch == (w, nu))
@@ -449,6 +452,9 @@ def main():
     ap.add_argument("--batch-autotune", action="store_true",
                     help="with --batch-cases: pick each case's multigrid cycle and sweeps with "
                          "pv_batch_autotune_mg after the warm-up")
+    ap.add_argument("--batch-graphs", type=int, default=0,
+                    help="with --batch-cases: replay each case's pressure iterations from hipGraphs of this "
+                         "many iterations (0 = eager launches)")
     ap.add_argument("--copy-mib", type=int, default=0,
                     help="only copy this many MiB device to device three times and exit: a kernel of known "
                          "traffic, to calibrate rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE in a run of its own")
