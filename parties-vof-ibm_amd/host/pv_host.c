@@ -174,10 +174,6 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0)
         return pv_set_error(PV_ENODEV, "no HIP device %d (count %d)", device, ndev);
     HIPCHK(hipSetDevice(device));
-    /* the CG loops wait on the device's stop flag a few times per substep:
-     * spin instead of sleeping so that the next launches follow at once (a
-     * device already in use keeps its mode; that refusal is not an error) */
-    if (hipSetDeviceFlags(hipDeviceScheduleSpin) != hipSuccess) (void)hipGetLastError();
     HIPCHK(hipGetDeviceProperties(&prop, device));
     if (strncmp(prop.gcnArchName, "gfx950", 6) != 0)
         return pv_set_error(PV_ENODEV, "device %d is %s; this library is built for gfx950", device,
