@@ -307,6 +307,7 @@ def batch_bench(lib, args):
 
     cells = sum(p.NXM * p.NYM for p in params)
     bt = pvof.Batch(params, lib)
+    bt.set_threads(args.batch_threads)
     for c, F in zip(bt.cases, fields):
         setup(c, F)
     note(f"batch of {len(params)} cases, {cells / 1e6:.2f} Mcells: {args.warmup} + {args.steps} steps")
@@ -345,6 +346,7 @@ def batch_bench(lib, args):
             "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
             "mg_cycle_w_levels": args.mg_cycle if choices is None else "autotuned",
             "graph_chunk": args.batch_graphs or None,
+            "host_threads": args.batch_threads or len(params),
             "mg_choices": None if choices is None else
             {f"{nx}x{ny}": sorted({f"W{w}/nu{nu}": sum(1 for q, ch in zip(picks, choices) if q == (nx, ny) and
                                                       ch == (w, nu))
@@ -452,6 +454,9 @@ def main():
     ap.add_argument("--batch-autotune", action="store_true",
                     help="with --batch-cases: pick each case's multigrid cycle and sweeps with "
                          "pv_batch_autotune_mg after the warm-up")
+    ap.add_argument("--batch-threads", type=int, default=16,
+                    help="with --batch-cases: host threads sharing the cases (0 = one per case); more "
+                         "threads than the CPUs the process has slow the batch down")
     ap.add_argument("--batch-graphs", type=int, default=0,
                     help="with --batch-cases: replay each case's pressure iterations from hipGraphs of this "
                          "many iterations (0 = eager launches)")

@@ -282,6 +282,9 @@ int pv_batch_destroy(pv_batch *bt);
 int pv_batch_size(const pv_batch *bt);
 pv_bag *pv_batch_case(pv_batch *bt, int k);
 int pv_batch_rk3(pv_batch *bt, int nsteps);
+/* nthreads host threads share the cases (0 = one thread per case, the
+ * default); results are unchanged */
+int pv_batch_set_threads(pv_batch *bt, int nthreads);
 int pv_batch_autotune(pv_batch *bt, int *chunks);
 int pv_autotune_mg(pv_bag *bag, int *wlevels, int *sweeps);
 int pv_batch_autotune_mg(pv_batch *bt, int *wlevels, int *sweeps);

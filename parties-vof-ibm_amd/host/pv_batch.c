@@ -5,7 +5,9 @@
  * Each case is an ordinary handle with its own HIP stream; pv_batch_rk3
  * drives every case from its own host thread, so the kernels of different
  * cases overlap on the device while each case runs exactly the code path it
- * runs alone (its results are bit for bit those of a solo run).
+ * runs alone (its results are bit for bit those of a solo run).  With
+ * pv_batch_set_threads(bt, T), 0 < T < cases, T host threads share the
+ * cases instead: thread t steps cases t, t + T, ... one step each in turn.
  */
 #include "pv_host.h"
 #include <pthread.h>
@@ -14,13 +16,14 @@
 #include <string.h>
 
 struct pv_batch {
-    int n;
+    int n, nthreads;     /* nthreads 0: one host thread per case */
     pv_bag **bags;
 };
 
 typedef struct {
-    pv_bag *bag;
-    int nsteps, rc;
+    pv_bag *bag;         /* one case, all steps (one thread per case) */
+    pv_batch *bt;        /* or cases first, first + stride, ... step by step */
+    int first, stride, nsteps, rc, bad;
     char err[512];
 } batch_job;
 
@@ -68,22 +71,44 @@ pv_bag *pv_batch_case(pv_batch *bt, int k)
 static void *batch_worker(void *arg)
 {
     batch_job *j = (batch_job *)arg;
-    j->rc = pv_Temporal_int_rk3(j->bag, j->nsteps);
-    if (j->rc) snprintf(j->err, sizeof j->err, "%s", pv_last_error());
+    int s, k;
+    if (j->bag) {
+        j->rc = pv_Temporal_int_rk3(j->bag, j->nsteps);
+        if (j->rc) snprintf(j->err, sizeof j->err, "%s", pv_last_error());
+        return NULL;
+    }
+    for (s = 0; s < j->nsteps && !j->rc; s++)
+        for (k = j->first; k < j->bt->n && !j->rc; k += j->stride)
+            if ((j->rc = pv_Temporal_int_rk3(j->bt->bags[k], 1))) {
+                j->bad = k;
+                snprintf(j->err, sizeof j->err, "%s", pv_last_error());
+            }
     return NULL;
+}
+
+int pv_batch_set_threads(pv_batch *bt, int nthreads)
+{
+    if (!bt || nthreads < 0) return pv_set_error(PV_EINVAL, "bad batch arguments");
+    bt->nthreads = nthreads;
+    return PV_OK;
 }
 
 int pv_batch_rk3(pv_batch *bt, int nsteps)
 {
     batch_job *jobs;
     pthread_t *th;
-    int k, rc = PV_OK, started = 0;
+    int k, rc = PV_OK, started = 0, nt;
     if (!bt || nsteps < 0) return pv_set_error(PV_EINVAL, "bad batch arguments");
-    jobs = (batch_job *)calloc(bt->n, sizeof *jobs);
-    th = (pthread_t *)calloc(bt->n, sizeof *th);
+    nt = (bt->nthreads > 0 && bt->nthreads < bt->n) ? bt->nthreads : bt->n;
+    jobs = (batch_job *)calloc(nt, sizeof *jobs);
+    th = (pthread_t *)calloc(nt, sizeof *th);
     if (!jobs || !th) { free(jobs); free(th); return pv_set_error(PV_ENOMEM, "host allocation failed"); }
-    for (k = 0; k < bt->n; k++) {
-        jobs[k].bag = bt->bags[k];
+    for (k = 0; k < nt; k++) {
+        jobs[k].bag = (nt == bt->n) ? bt->bags[k] : NULL;
+        jobs[k].bt = bt;
+        jobs[k].first = k;
+        jobs[k].stride = nt;
+        jobs[k].bad = k;
         jobs[k].nsteps = nsteps;
         if (pthread_create(&th[k], NULL, batch_worker, &jobs[k]) != 0) {
             jobs[k].rc = PV_EINVAL;
@@ -93,8 +118,8 @@ int pv_batch_rk3(pv_batch *bt, int nsteps)
         started++;
     }
     for (k = 0; k < started; k++) pthread_join(th[k], NULL);
-    for (k = 0; k < bt->n; k++)
-        if (jobs[k].rc && rc == PV_OK) rc = pv_set_error(jobs[k].rc, "case %d: %s", k, jobs[k].err);
+    for (k = 0; k < nt; k++)
+        if (jobs[k].rc && rc == PV_OK) rc = pv_set_error(jobs[k].rc, "case %d: %s", jobs[k].bad, jobs[k].err);
     free(jobs);
     free(th);
     return rc;

@@ -176,6 +176,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_batch_case.argtypes = [vp, C.c_int]
     lib.pv_batch_case.restype = vp
     lib.pv_batch_rk3.argtypes = [vp, C.c_int]
+    lib.pv_batch_set_threads.argtypes = [vp, C.c_int]
     lib.pv_batch_autotune.argtypes = [vp, ip]
     lib.pv_autotune_mg.argtypes = [vp, ip, ip]
     lib.pv_batch_autotune_mg.argtypes = [vp, ip, ip]
@@ -321,6 +322,9 @@ class Batch:
 
     def rk3(self, nsteps):
         self.check(self.lib.pv_batch_rk3(self.h, nsteps))
+
+    def set_threads(self, nthreads):
+        self.check(self.lib.pv_batch_set_threads(self.h, nthreads))
 
     def autotune(self):
         ch = (C.c_int * len(self.cases))()

@@ -48,11 +48,14 @@ def test_graph_replay_matches_eager(lib):
             assert np.array_equal(a.get(f), b.get(f)), f
 
 
-def test_batch_cases_match_solo_runs(lib):
+@pytest.mark.parametrize("threads", [0, 2, 4])
+def test_batch_cases_match_solo_runs(lib, threads):
+    """threads: host threads sharing the 6 cases (0 = one per case)."""
     sizes = [(64, 64), (64, 128), (128, 128), (128, 64), (256, 128), (128, 256)]
     params = [case_params(nx, ny) for nx, ny in sizes]
     fields = [bubble_field(lib, p, 100 + k) for k, p in enumerate(params)]
     bt = pvof.Batch(params, lib)
+    bt.set_threads(threads)
     for c, F in zip(bt.cases, fields):
         c.init_from_F(F)
     bt.rk3(3)
