@@ -279,13 +279,13 @@ def mixed_error(lib, args):
 def batch_bench(lib, args):
     """Batched cases: --batch-cases cases of sizes drawn (seeded) from the
     batched configuration's {96^2, 192^2, 256x512, 384x768}, FP64, MG-PCG with
-    the W on the in-workgroup levels and the temporal start, seeded bubbles
+    the temporal start, seeded bubbles
     (the IBM cylinder and wetting of that configuration are not part of this
     path).  Aggregate Mcell-steps/s of pv_batch_rk3 (one stream and host
-    thread per case) against the same cases stepped one after another.  With
-    --batch-autotune each case's cycle and sweep count come from
-    pv_batch_autotune_mg after the warm-up (untimed; the one-after-another
-    run uses the same choices)."""
+    thread per case) against the same cases stepped one after another.  Each
+    case's cycle and sweep count come from pv_batch_autotune_mg after the
+    warm-up (untimed; the one-after-another run uses the same choices) unless
+    --no-batch-autotune is given."""
     rng = np.random.default_rng(SEED)
     shapes = [(96, 96), (192, 192), (256, 512), (384, 768)]
     picks = [shapes[k] for k in rng.integers(0, len(shapes), args.batch_cases)]
@@ -451,9 +451,9 @@ def main():
     ap.add_argument("--batch-cases", type=int, default=0,
                     help="only report batched throughput: this many cases of the batched configuration's sizes "
                          "(--steps, --warmup per case)")
-    ap.add_argument("--batch-autotune", action="store_true",
+    ap.add_argument("--batch-autotune", action=argparse.BooleanOptionalAction, default=True,
                     help="with --batch-cases: pick each case's multigrid cycle and sweeps with "
-                         "pv_batch_autotune_mg after the warm-up")
+                         "pv_batch_autotune_mg after the warm-up (--no-batch-autotune: --mg-cycle on every case)")
     ap.add_argument("--batch-threads", type=int, default=16,
                     help="with --batch-cases: host threads sharing the cases (0 = one per case); more "
                          "threads than the CPUs the process has slow the batch down")
