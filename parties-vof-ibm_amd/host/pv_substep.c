@@ -1156,7 +1156,7 @@ int pv_autotune(pv_bag *b, int *choice)
 int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
 {
     double best = 1e300, *phi_save = NULL;
-    int w, nu, rep, pick_w = 0, pick_nu = 2, p_warm, chunk, rc = PV_OK;
+    int w, nu, rep, pick_w = 0, pick_nu = 2, p_warm, chunk, rc = PV_OK, mask0, nu0;
     double start_res;
     size_t bytes;
     RC(begin(b));
@@ -1173,6 +1173,8 @@ int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
     p_warm = b->p_warm;
     chunk = b->mg_chunk;
     start_res = b->learn_start_res;
+    mask0 = b->mg.wmask;
+    nu0 = b->mg.nu;
     b->p_warm = 0;
     for (w = 0; w <= 3 && rc == PV_OK; w++)
         for (nu = 1; nu <= 2 && rc == PV_OK; nu++) {
@@ -1200,7 +1202,11 @@ int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
         hipStreamSynchronize(S(b)) != hipSuccess)
         if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "MG autotune: restoring PHI failed");
     hipFree(phi_save);
-    if (rc) return rc;
+    if (rc) {                          /* a failed tuning leaves the cycle as it was */
+        b->mg.wmask = b->mgf.wmask = mask0;
+        b->mg.nu = b->mgf.nu = nu0;
+        return rc;
+    }
     RC(pv_set_mg_cycle(b, pick_w));
     RC(pv_set_mg_sweeps(b, pick_nu));
     if (wlevels) *wlevels = pick_w;
