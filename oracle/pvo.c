@@ -19,6 +19,18 @@
  * array, so sums over the G range count it by repeating the k = 0 sequence.
  *
  * Citations are relative to PARTIES/src/.
+ *
+ * Pinning.  The reference holds no golden vectors for this path and cannot be
+ * built here, so every exported routine is checked against what the
+ * mathematics fixes (tests/test_oracle.py, tests/test_oracle_pins.py:
+ * discrete eigenpairs, exactness classes, spectral solves, operator
+ * identities, force balances, the published RK3 and capillary time-step
+ * formulas), and tests/test_oracle_mutants.py plants 41 plausible mistakes
+ * in this file, each of which one of those checks rejects.  Parity unpinned:
+ * the reference's own iteration counts and dt sequence, and the effect of
+ * the G-range sums of VOF_DIFFUSE.c:128-166 over a whole trajectory (the
+ * code follows the reference line by line there, but only a run of the
+ * reference could confirm the numbers).
  */
 #include "pvo.h"
 #include <math.h>

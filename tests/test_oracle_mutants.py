@@ -14,6 +14,7 @@ import pytest
 
 import oracle_lib
 import test_oracle as T
+import test_oracle_pins as Q
 
 ROOT = oracle_lib.ROOT
 
@@ -45,6 +46,82 @@ MUTANTS = [
      T.test_rk3_identities, {}),
     ("dt_limiter_else", "    else dt = MIND(dt, s->prm.max_dt);", "    dt = MIND(dt, s->prm.max_dt);",
      T.test_dt_limiter_truth_table, {}),
+    # momentum, surface tension, density and time step (test_oracle_pins.py)
+    ("f_sigma_scale", "2.0 * 6.0 * sqrt(2.0)", "2.0 * 5.0 * sqrt(2.0)",
+     Q.test_f_sigma_balances_its_pressure_at_constant_potential, {"direction": "x"}),
+    ("f_sigma_band", "if (cl < 0.005 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05) continue;\n"
+                     "            psi_f = 0.5 * (AT(psi, i, j) + AT(psi, i - 1, j));",
+     "if (cl < 0.05 || cl > 0.995 || cg < 0.005 || cg > 0.995 || cs > 0.05) continue;\n"
+     "            psi_f = 0.5 * (AT(psi, i, j) + AT(psi, i - 1, j));",
+     Q.test_f_sigma_on_a_tanh_profile, {"direction": "x"}),
+    ("f_sigma_y_gradient_index", "gradC = (AT(C, i, j) - AT(C, i, j - 1)) * MT(s->m_idy_c, j - 1);",
+     "gradC = (AT(C, i, j + 1) - AT(C, i, j - 1)) * MT(s->m_idy_c, j - 1);",
+     Q.test_f_sigma_on_a_tanh_profile, {"direction": "y"}),
+    ("gravity_factor", "AT(ru, i, j) += 2.0 * rho_f * Ri * gx;", "AT(ru, i, j) += 1.0 * rho_f * Ri * gx;",
+     Q.test_gravity_hydrostatic_balance, {"grav": (3.0, -4.0, 0.0), "rho_kind": "uniform"}),
+    ("gravity_v_face_density", "double rho_f = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));\n"
+                               "            AT(rv, i, j) += 2.0 * rho_f * Ri * gy;",
+     "double rho_f = 0.5 * (AT(rho, i, j) + AT(rho, i, j + 1));\n"
+     "            AT(rv, i, j) += 2.0 * rho_f * Ri * gy;",
+     Q.test_gravity_hydrostatic_balance, {"grav": (0.0, -1.0, 0.0), "rho_kind": "linear_y"}),
+    ("convective_uv_sign", "AT(ex, i, j) = -(duudx + duvdy + duwdz);", "AT(ex, i, j) = -(duudx - duvdy + duwdz);",
+     Q.test_convective_terms_exact_for_linear_velocity, {}),
+    ("convective_v_face_average", "double vuE = 0.5 * (AT(v, i, j) + AT(v, i + 1, j)) * uCE;",
+     "double vuE = 0.5 * (AT(v, i, j) + AT(v, i + 1, j - 1)) * uCE;",
+     Q.test_convective_terms_exact_for_linear_velocity, {}),
+    ("u_rhs_zeta_sign_convective", "double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));\n"
+                                   "            double conv = GAMB[rk] * AT(ex, i, j) + ZETB[rk] * AT(exo, i, j);",
+     "double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));\n"
+     "            double conv = GAMB[rk] * AT(ex, i, j) - ZETB[rk] * AT(exo, i, j);",
+     Q.test_momentum_rhs_low_storage_rk3, {"stage": 1}),
+    ("u_rhs_zeta_sign_viscous", "(GAMB[rk] * AT(ve, i, j) + ZETB[rk] * AT(veo, i, j)) - 2.0 * dpdx;",
+     "(GAMB[rk] * AT(ve, i, j) - ZETB[rk] * AT(veo, i, j)) - 2.0 * dpdx;",
+     Q.test_momentum_rhs_low_storage_rk3, {"stage": 2}),
+    ("v_rhs_pressure_factor", "(GAMB[rk] * AT(ve, i, j) + ZETB[rk] * AT(veo, i, j)) - 2.0 * dpdy;",
+     "(GAMB[rk] * AT(ve, i, j) + ZETB[rk] * AT(veo, i, j)) - 1.0 * dpdy;",
+     Q.test_momentum_rhs_low_storage_rk3, {"stage": 0}),
+    ("rho_from_liquid_fraction", "AT(rho, i, j) = AT(C, i, j) + rho_s * cs + rho_g * AT(G, i, j);",
+     "AT(rho, i, j) = AT(C, i, j) + rho_s * cs + rho_g * AT(C, i, j);",
+     Q.test_density_viscosity_mixture_rule, {}),
+    ("mu_gas_ratio", "AT(mu, i, j) = AT(C, i, j) + mu_s * cs + mu_g * AT(G, i, j);",
+     "AT(mu, i, j) = AT(C, i, j) + mu_s * cs + rho_g * AT(G, i, j);",
+     Q.test_density_viscosity_mixture_rule, {}),
+    ("ch_zeta_on_current_rhs", "AT(rv, i, j) = a_dt * AT(C, i, j) + GAMB[st] * AT(rc, i, j) + ZETB[st] * AT(rn, i, j);",
+     "AT(rv, i, j) = a_dt * AT(C, i, j) + GAMB[st] * AT(rc, i, j) + ZETB[st] * AT(rc, i, j);",
+     Q.test_ch_step_is_low_storage_rk3, {"stage": 2}),
+    ("ch_explicit_biharmonic_sign", "AT(b, i, j) -= bih * AT(psiLG, i, j);", "AT(b, i, j) += bih * AT(psiLG, i, j);",
+     Q.test_biharmonic_step_crank_nicolson_factor, {"k": 1, "m": 1}),
+    ("explicit_diffusion_peclet", "AT(rhs, i, j) = AT(aux, i, j) / s->prm.Pe_CH;",
+     "AT(rhs, i, j) = AT(aux, i, j) * s->prm.Cn;", Q.test_explicit_diffusion_on_a_cosine, {"k": 1}),
+    ("jacobi_diagonal_y_dropped", "return d_time + diag_x + diag_y + diag_z + diag_axisym;",
+     "return d_time + diag_x + diag_z + diag_axisym;", Q.test_jacobi_diagonal_is_operator_diagonal, {"comp": "u"}),
+    ("jacobi_diagonal_v_time_factor", "d_time = rho_face / (alpha_k * s->dt);\n        mu_yp = AT(mu, i, j);",
+     "d_time = 0.5 * rho_face / (alpha_k * s->dt);\n        mu_yp = AT(mu, i, j);",
+     Q.test_jacobi_diagonal_is_operator_diagonal, {"comp": "v"}),
+    ("implicit_viscous_harmonic_mean", "double mu_yp = 2.0 * (0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) *",
+     "double mu_yp = 1.0 * (0.5 * (AT(mu, i - 1, j + 1) + AT(mu, i, j + 1))) *",
+     Q.test_implicit_viscous_term_is_the_cg_operator, {"comp": "u"}),
+    ("implicit_viscous_v_y_metric", "double d2vdy2 = iRe * (mu_yp * dvdyN - mu_ym * dvdyS) * MT(s->m_idy_c, j - 1);",
+     "double d2vdy2 = iRe * (mu_yp * dvdyN - mu_ym * dvdyS) * MT(s->m_idy_c, j - 1) * 0.5;",
+     Q.test_implicit_viscous_uniform_mu_is_laplacian_eigenpair, {"k": 3, "m": 2}),
+    ("transposed_viscous_dropped_cross_term", "AT(ve, i, j) = ddxdudx + ddydvdx + ddzdwdx;",
+     "AT(ve, i, j) = ddxdudx + ddzdwdx;", Q.test_transposed_viscous_uniform_mu_is_grad_div, {}),
+    ("cell_center_wrong_neighbour", "AT(ub, i, j) = 0.5 * (AT(u, i, j) + AT(u, i + 1, j));",
+     "AT(ub, i, j) = 0.5 * (AT(u, i, j) + AT(u, i - 1, j));", Q.test_cell_center_exact_for_linear_velocity, {}),
+    ("capillary_dt_four_pi", "(4.0 * PI_REF));", "(2.0 * PI_REF));", Q.test_dt_capillary_limit_at_rest, {}),
+    ("convective_dt_metric", "double convective = u_cfl * MT(s->m_idx_u, i) + v_cfl * MT(s->m_idy_v, j);",
+     "double convective = u_cfl * MT(s->m_idx_u, i) + v_cfl * MT(s->m_idx_u, j);", Q.test_dt_convective_limit, {}),
+    ("velocity_cg_beta", "beta = lv0 / (rz + EPS);", "beta = lv1 / (rz + EPS);",
+     Q.test_velocity_cg_solves_its_system, {"comp": "u"}),
+    ("pressure_rhs_substep_factor", "double a_dt = 1.0 / (s->dt * 2.0 * BET[s->stage]);",
+     "double a_dt = 1.0 / (s->dt * BET[s->stage]);", Q.test_pressure_rhs_is_scaled_divergence, {}),
+    ("divergence_wrong_neighbour", "double dvdy = (AT(v, i, j + 1) - AT(v, i, j)) * MT(s->m_idy_v, j);",
+     "double dvdy = (AT(v, i, j + 1) - AT(v, i, j - 1)) * MT(s->m_idy_v, j);",
+     Q.test_pressure_rhs_is_scaled_divergence, {}),
+    ("jacobi_pcg_stop_test", "if (sqrt(gamma_new) < tol) break;", "if (sqrt(gamma_new) < 1e3 * tol) break;",
+     Q.test_pressure_solve_satisfies_poisson, {"solver": 0}),
+    ("init_clamp", "C[i] = (f < 0.0) ? 0.0 : ((f > 1.0) ? 1.0 : f);", "C[i] = (f < 0.0) ? 0.0 : ((f > 1.0) ? 0.99 : f);",
+     Q.test_init_from_F_bootstrap, {}),
 ]
 
 
