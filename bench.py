@@ -85,11 +85,15 @@ SEED = 20261202
 METRIC = "Mcell-steps/s at 1024\u00b2, plain FP64 vs learned-PCG mixed form; % of HBM roofline"
 
 
-def workload_params(n=1024, solver=pvof.PV_PSOLVE_FFT_PCG, precision=pvof.PV_PREC_FP64):
+# momentum CG stop: the reference default (IO/default.inp [lsolve] CG_ETOL)
+CG_ETOL = 1e-9
+
+
+def workload_params(n=1024, solver=pvof.PV_PSOLVE_FFT_PCG, precision=pvof.PV_PREC_FP64, cg_etol=None):
     return pvof.default_params(
         NXM=n, NYM=n, xmax=1.0, ymax=1.0, Re=100.0, We=200.0, rho1=1.0, rho2=1e-3, mu1=1.0, mu2=1e-2,
         Cn=0.004, Pe_CH=1.0 / 0.004, cfl=0.2, default_dt=1e-5, max_dt=1e-3,
-        ch_tol=1e-12, ch_iter_max=300, CG_ETOL=1e-7, CG_MAXIT=120,
+        ch_tol=1e-12, ch_iter_max=300, CG_ETOL=CG_ETOL if cg_etol is None else cg_etol, CG_MAXIT=120,
         P_CG_ETOL=1e-7, P_CG_MAXIT=300, pressure_solver=solver, p_rel_tol=1e-10, p_max_iter=3000,
         precision=precision)
 
@@ -116,24 +120,36 @@ LABEL = {"fft": "FFT-PCG", "jacobi": "Jacobi-PCG", "mg": "MG-PCG", "learned": "l
          "learned_mg": "learned warm start + MG-PCG"}
 
 
-def cpu_baseline(params, F):
-    """Time the CPU oracle on the first RK3 substep of the same workload (same
-    initial field, same solver settings) and scale it to whole steps."""
+def cpu_baseline(params, F, min_seconds=10.0, max_steps=3):
+    """Time the C restatement of the reference algorithm (the oracle built with
+    OpenMP over rows, oracle/_build/libpvo_omp.so) on whole RK3 steps of the
+    same workload from the same initial field, on the host cores this process
+    may use (OMP_NUM_THREADS when set): steps until min_seconds have passed
+    (at least one, at most max_steps)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
+    if "OMP_NUM_THREADS" not in os.environ:
+        os.environ["OMP_NUM_THREADS"] = str(len(os.sched_getaffinity(0)))
     import oracle_lib
 
-    o = oracle_lib.Oracle(params)
+    lib = oracle_lib.load_oracle(os.path.join(ROOT, "oracle", "_build", "libpvo_omp.so"))
+    lib.pvo_threads.restype = C.c_int
+    threads = int(lib.pvo_threads())
+    o = oracle_lib.Oracle(params, lib)
     o.init_from_F(F)
-    o.call("set_time_state", 0.0, params.default_dt, params.default_dt, 0)
-    t0 = time.perf_counter()
-    rc = o.call("Temporal_int_all_the_equations")
-    dt = time.perf_counter() - t0
-    if rc != 0:
-        return None
+    steps, elapsed = 0, 0.0
+    while steps < max_steps and (steps == 0 or elapsed < min_seconds):
+        t0 = time.perf_counter()
+        rc = o.call("Temporal_int_rk3", 1)
+        elapsed += time.perf_counter() - t0
+        if rc != 0:
+            return None
+        steps += 1
     cells = params.NXM * params.NYM
-    return {"value": round(cells / (3.0 * dt) / 1e6, 5), "unit": "Mcell-steps/s", "cores": 1, "kind": "port",
-            "sample": f"first RK3 substep ({params.NXM}x{params.NYM}, same initial field and solver settings) "
-                      f"of the serial C oracle: {dt:.1f} s, scaled by 3 substeps per step"}
+    return {"value": round(cells * steps / elapsed / 1e6, 5), "unit": "Mcell-steps/s", "cores": threads,
+            "kind": "port",
+            "sample": f"{steps} whole RK3 step(s) ({params.NXM}x{params.NYM}, same initial field, FFT-PCG pressure "
+                      f"to {params.p_rel_tol:g}, CG_ETOL {params.CG_ETOL:g}) of the C restatement of the reference "
+                      f"algorithm with OpenMP over rows on {threads} host threads: {elapsed:.1f} s"}
 
 
 def train_network(lib, args, prec):
@@ -158,20 +174,24 @@ def train_network(lib, args, prec):
             "loss_first": l0, "loss_last": l2, "held_out_seed": SEED + 1}
 
 
-def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=True):
+def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=True, opts=None):
     """Time `steps` RK3 steps of one configuration from the seeded field F
-    (after `warmup` untimed steps); returns the measurements."""
+    (after `warmup` untimed steps); returns the measurements.  opts overrides
+    the MG forms' settings (warm_start, ch_spectral, mg_cycle, mg_sweeps)."""
     prec = pvof.PV_PREC_MIXED if precision == "mixed" else pvof.PV_PREC_FP64
-    p = workload_params(args.n, SOLVERS[solver], prec)
+    p = workload_params(args.n, SOLVERS[solver], prec, args.cg_etol)
     s = pvof.Solver(p, lib, device=0)
     s.init_from_F(F)
     if net_params is not None:
         pvof.Network(s).set_params(net_params)
+    o = dict(warm_start=args.warm_start, ch_spectral=args.ch_spectral, mg_cycle=args.mg_cycle,
+             mg_sweeps=args.mg_sweeps)
+    o.update(opts or {})
     if family_of(solver) == "mg":
-        s.check(lib.pv_set_pressure_warm_start(s.h, args.warm_start))
-        s.check(lib.pv_set_ch_solver(s.h, args.ch_spectral))
-        s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
-        s.check(lib.pv_set_mg_sweeps(s.h, args.mg_sweeps))
+        s.check(lib.pv_set_pressure_warm_start(s.h, o["warm_start"]))
+        s.check(lib.pv_set_ch_solver(s.h, o["ch_spectral"]))
+        s.check(lib.pv_set_mg_cycle(s.h, o["mg_cycle"]))
+        s.check(lib.pv_set_mg_sweeps(s.h, o["mg_sweeps"]))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -211,6 +231,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
                              "GBs": (round(bpc[name] * cells / (avg_ms * 1e-3) / 1e9, 1) if name in bpc else None)}
     bytes_step = sum(step_bytes_per_cell(st, fam) for st in stats) / len(stats)
     out = {"solver": solver, "precision": precision, "steps": steps, "warmup": warmup,
+           "settings": o if family_of(solver) == "mg" else None,
            "value": round(cells * steps / elapsed / 1e6, 3), "ms_per_step": round(1e3 * elapsed / steps, 3),
            "iterations_per_step": {key: float(np.mean([sum(st[key]) for st in stats]))
                                    for key in ("n_ch", "n_u", "n_v", "n_p")},
@@ -329,10 +350,10 @@ def batch_bench(lib, args):
     for k, (p, F) in enumerate(zip(params, fields)):
         s = pvof.Solver(p, lib, device=0)
         setup(s, F)
-        if choices is not None:
+        s.call("Temporal_int_rk3", args.warmup)
+        if choices is not None:         # as in the batch: tuned after the warm-up
             s.check(lib.pv_set_mg_cycle(s.h, choices[k][0]))
             s.check(lib.pv_set_mg_sweeps(s.h, choices[k][1]))
-        s.call("Temporal_int_rk3", args.warmup)
         s.call("synchronize")
         t0 = time.perf_counter()
         s.call("Temporal_int_rk3", args.steps)
@@ -346,7 +367,7 @@ def batch_bench(lib, args):
             "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
             "mg_cycle_w_levels": args.mg_cycle if choices is None else "autotuned",
             "graph_chunk": args.batch_graphs or None,
-            "host_threads": args.batch_threads or len(params),
+            "host_threads": min(args.batch_threads, len(params)) if args.batch_threads else len(params),
             "mg_choices": None if choices is None else
             {f"{nx}x{ny}": sorted({f"W{w}/nu{nu}": sum(1 for q, ch in zip(picks, choices) if q == (nx, ny) and
                                                       ch == (w, nu))
@@ -433,6 +454,10 @@ def main():
     ap.add_argument("--no-compare", action="store_true",
                     help="skip the plain FP64 FFT-PCG run and the no-network ablation")
     ap.add_argument("--plain-steps", type=int, default=10)
+    ap.add_argument("--no-fp64-mg", action="store_true",
+                    help="skip the FP64 MG-PCG comparison runs")
+    ap.add_argument("--cg-etol", type=float, default=CG_ETOL,
+                    help="momentum CG stop (RMS residual), default the reference's 1e-9")
     ap.add_argument("--train-collect-steps", type=int, default=4)
     ap.add_argument("--train-steps", type=int, default=3000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -516,6 +541,17 @@ def main():
             abl = run(lib, args, base, args.precision, args.steps, args.warmup, F, timers=False)
             compare["ablation_no_network"] = public(abl)
             compare["network_gain"] = round(main_run["value"] / abl["value"], 3)
+        if not args.no_fp64_mg and family_of(args.solver) == "mg":
+            note(f"FP64 MG-PCG runs: {args.plain_steps} steps each")
+            # the best FP64 path: multigrid V-cycle, the reference CH CG, cold starts, no network
+            ref = run(lib, args, "mg", "fp64", args.plain_steps, 1, F, timers=False,
+                      opts=dict(warm_start=0, ch_spectral=0, mg_cycle=0))
+            compare["fp64_mg"] = public(ref)
+            compare["speedup_vs_fp64_mg"] = round(main_run["value"] / ref["value"], 3)
+            # the timed form's algorithm in FP64 throughout (isolates the FP32 stencils and cycle)
+            same = run(lib, args, "mg", "fp64", args.plain_steps, 1, F, timers=False)
+            compare["fp64_mg_same_settings"] = public(same)
+            compare["speedup_vs_fp64_mg_same_settings"] = round(main_run["value"] / same["value"], 3)
 
     form = LABEL[args.solver] + (", mixed FP32/FP64" if args.precision == "mixed" else ", FP64")
     out = {
@@ -569,7 +605,7 @@ def main():
                            "avg_us": timed[dom]["avg_us"], "timer_period": args.timer_period}
     if not args.no_cpu_baseline:
         note(f"CPU baseline at {args.cpu_n}^2")
-        pc = workload_params(args.cpu_n, pvof.PV_PSOLVE_FFT_PCG)
+        pc = workload_params(args.cpu_n, pvof.PV_PSOLVE_FFT_PCG, cg_etol=args.cg_etol)
         Fc = pvof.ic(lib, "bubbles", pc, seed=SEED, nmin=16, nmax=64, rmin=0.01, rmax=0.08, gas_inside=1)
         out["cpu_baseline"] = cpu_baseline(pc, Fc)
     print(json.dumps(out))

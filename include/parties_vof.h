@@ -127,6 +127,10 @@ typedef struct pv_stats {
     double div_max[3];        /* Pressure_compute_velocity_divergence per stage */
     double clamp_delta[3];    /* diffuse_bound_liquid_fraction return value     */
     int    p_fallback[3];     /* learned preconditioner fell back to FFT-PCG    */
+    double clamp_delta_interior[3]; /* the same correction summed over interior cells
+                                       only (the reference's sum also counts the periodic
+                                       image column i = NXM and the ghost row j = NYM);
+                                       no counterpart in the reference */
 } pv_stats;
 
 typedef struct pv_bag pv_bag;

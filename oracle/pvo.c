@@ -38,6 +38,25 @@
 #include <string.h>
 #include <stdio.h>
 
+/* Built with -fopenmp (libpvo_omp.so, the timed CPU baseline) the sweeps run
+ * over rows on the host's cores; without it (libpvo.so, the checker) these
+ * macros are empty and every sum runs in the serial order. */
+#ifdef _OPENMP
+#include <omp.h>
+#define PVO_PRAGMA(x) _Pragma(#x)
+#define PVO_FOR PVO_PRAGMA(omp parallel for schedule(static) private(i))
+#define PVO_FOR_SUM(...) PVO_PRAGMA(omp parallel for schedule(static) private(i) reduction(+ : __VA_ARGS__))
+#define PVO_FOR_COLS PVO_PRAGMA(omp parallel for schedule(static) private(j))
+#define PVO_TID omp_get_thread_num()
+#define PVO_NTHREADS omp_get_max_threads()
+#else
+#define PVO_FOR
+#define PVO_FOR_SUM(...)
+#define PVO_FOR_COLS
+#define PVO_TID 0
+#define PVO_NTHREADS 1
+#endif
+
 #define PI_REF 3.14159265358979323846          /* Include/definitions.h:7 */
 /* Include/definitions.h:198-204 */
 static const double BET[3]  = { 4.0 / 15.0, 1.0 / 15.0, 1.0 / 6.0 };
@@ -70,8 +89,14 @@ struct pvo_state {
     int stage, ntime;
     pv_stats st;
     double *zbuf;              /* psolve_cg.c:164 per-call z */
-    double *wre, *wim, *wrow;  /* FFT workspace (length NXM) */
     double *twc, *tws;         /* cos/sin(2 pi m / NXM), m < NXM */
+    struct pvo_ws *ws;         /* one FFT / Thomas workspace per host thread */
+    int nws;
+    double clamp_int;          /* last clamp correction over interior cells only */
+};
+
+struct pvo_ws {
+    double *wre, *wim, *wrow;  /* FFT workspace (length NXM) */
     double *fre, *fim;         /* radix-2 FFT work (length NXM) */
     double *tsa, *tsb, *tsc, *tsr, *tsu, *tsg; /* Thomas workspace (NYM) */
 };
@@ -148,11 +173,17 @@ pvo_state *pvo_create(const pv_params *prm)
     MA(m_idx_u, s->P); MA(m_idx_c, s->P);
     MA(m_yv, s->NYT); MA(m_yc, s->NYT); MA(m_dy_v, s->NYT); MA(m_dy_c, s->NYT);
     MA(m_idy_v, s->NYT); MA(m_idy_c, s->NYT);
-    MA(zbuf, s->nel); MA(wre, s->NXM); MA(wim, s->NXM); MA(wrow, s->NXM);
-    MA(twc, s->NXM); MA(tws, s->NXM); MA(fre, s->NXM); MA(fim, s->NXM);
-    MA(tsa, s->NYM); MA(tsb, s->NYM); MA(tsc, s->NYM); MA(tsr, s->NYM);
-    MA(tsu, s->NYM); MA(tsg, s->NYM);
+    MA(zbuf, s->nel); MA(twc, s->NXM); MA(tws, s->NXM);
 #undef MA
+    s->nws = PVO_NTHREADS;
+    s->ws = (struct pvo_ws *)calloc((size_t)s->nws, sizeof(struct pvo_ws));
+    for (k = 0; k < s->nws; k++) {
+        struct pvo_ws *w = &s->ws[k];
+#define MW(x, n) w->x = (double *)calloc((size_t)(n), sizeof(double))
+        MW(wre, s->NXM); MW(wim, s->NXM); MW(wrow, s->NXM); MW(fre, s->NXM); MW(fim, s->NXM);
+        MW(tsa, s->NYM); MW(tsb, s->NYM); MW(tsc, s->NYM); MW(tsr, s->NYM); MW(tsu, s->NYM); MW(tsg, s->NYM);
+#undef MW
+    }
     build_metrics(s);
     for (k = 0; k < s->NXM; k++) {
         double ang = 2.0 * PI_REF * (double)k / (double)s->NXM;
@@ -170,12 +201,20 @@ void pvo_destroy(pvo_state *s)
     free(s->m_xu); free(s->m_xc); free(s->m_dx_u); free(s->m_dx_c);
     free(s->m_idx_u); free(s->m_idx_c); free(s->m_yv); free(s->m_yc);
     free(s->m_dy_v); free(s->m_dy_c); free(s->m_idy_v); free(s->m_idy_c);
-    free(s->zbuf); free(s->wre); free(s->wim); free(s->wrow); free(s->twc); free(s->tws); free(s->fre); free(s->fim);
-    free(s->tsa); free(s->tsb); free(s->tsc); free(s->tsr); free(s->tsu); free(s->tsg);
+    free(s->zbuf); free(s->twc); free(s->tws);
+    for (k = 0; k < s->nws; k++) {
+        struct pvo_ws *w = &s->ws[k];
+        free(w->wre); free(w->wim); free(w->wrow); free(w->fre); free(w->fim);
+        free(w->tsa); free(w->tsb); free(w->tsc); free(w->tsr); free(w->tsu); free(w->tsg);
+    }
+    free(s->ws);
     free(s);
 }
 
 void pvo_get_params(const pvo_state *s, pv_params *out) { *out = s->prm; }
+
+/* Host threads of the OpenMP build (libpvo_omp.so); the serial build has one. */
+int pvo_threads(void) { return PVO_NTHREADS; }
 double *pvo_field(pvo_state *s, int field) { return (field >= 0 && field < PV_NFIELDS) ? s->f[field] : NULL; }
 int pvo_nxt(const pvo_state *s) { return s->P; }
 int pvo_nyt(const pvo_state *s) { return s->NYT; }
@@ -304,6 +343,7 @@ static void compute_laplacian(pvo_state *s, double *in, double *lap)
 {
     int P = s->P, i, j;
     bc_ch(s, in);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++)
             AT(lap, i, j) = lap_at(s, in, i, j);
@@ -321,6 +361,7 @@ void pvo_VOF_DIFFUSE_compute_bulk_S(pvo_state *s)
     int P = s->P, i, j;
     double *C = s->f[PV_F_C_L], *S = s->f[PV_F_BULK_S];
     const double cs = 0.0;
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             double c = AT(C, i, j);
@@ -336,6 +377,7 @@ void pvo_VOF_DIFFUSE_compute_psi_LG(pvo_state *s)
     double *C = s->f[PV_F_C_L], *L = s->f[PV_F_LAP_C], *psi = s->f[PV_F_PSI_LG];
     double Cn = s->prm.Cn;
     compute_laplacian(s, C, L);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             double c = AT(C, i, j);
@@ -370,6 +412,7 @@ void pvo_VOF_DIFFUSE_advect_WENO5(pvo_state *s, int rhs_out)
     bc_ch(s, C);
     memset(fx, 0, s->nel * sizeof(double));
     memset(fy, 0, s->nel * sizeof(double));
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             double vel = AT(u, i, j), st;
@@ -391,6 +434,7 @@ void pvo_VOF_DIFFUSE_advect_WENO5(pvo_state *s, int rhs_out)
         }
     halo_u(s, fx);        /* FLUX_X: u-type exchange */
     halo_cell(s, fy);     /* FLUX_Y: cell-type in x */
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             double div_x = (AT(fx, i + 1, j) - AT(fx, i, j)) / MT(s->m_dx_c, i);
@@ -408,6 +452,7 @@ void pvo_VOF_DIFFUSE_explicit_diffusion(pvo_state *s, int rhs_out)
     double *aux = s->f[PV_F_CH_AUX1], *rhs = s->f[rhs_out];
     pvo_VOF_DIFFUSE_compute_bulk_S(s);
     compute_laplacian(s, s->f[PV_F_BULK_S], aux);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++)
             AT(rhs, i, j) = AT(aux, i, j) / s->prm.Pe_CH;
@@ -420,12 +465,14 @@ static double ch_inner(const pvo_state *s, const double *a, const double *b)
 {
     int P = s->P, i, j, k;
     double sum = 0.0;
-    for (k = 0; k < 2; k++)
+    for (k = 0; k < 2; k++) {
+        PVO_FOR_SUM(sum)
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++) {
                 double measure = MT(s->m_dx_u, i) * MT(s->m_dy_v, j) * s->slab;
                 sum += measure * AT(a, i, j) * AT(b, i, j);
             }
+    }
     return sum;
 }
 
@@ -436,6 +483,7 @@ static void apply_biharmonic(pvo_state *s, double *x, double *Ax, double a_dt, d
     double *lap_x = s->f[PV_F_CH_AUX1], *bih_x = s->f[PV_F_PSI_LG];
     compute_laplacian(s, x, lap_x);
     compute_laplacian(s, lap_x, bih_x);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++)
             AT(Ax, i, j) = a_dt * AT(x, i, j) + bih * AT(bih_x, i, j);
@@ -452,6 +500,7 @@ static int biharmonic_cg(pvo_state *s, double *x, double *b, double a_dt, double
     double rhs_norm, rz, rel;
 
     apply_biharmonic(s, x, Ap, a_dt, bih);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             AT(r, i, j) = AT(b, i, j) - AT(Ap, i, j);
@@ -473,6 +522,7 @@ static int biharmonic_cg(pvo_state *s, double *x, double *b, double a_dt, double
         pAp = ch_inner(s, p, Ap);
         if (fabs(pAp) < 1e-30) break;
         alpha = rz / pAp;
+        PVO_FOR
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++) {
                 AT(x, i, j) += alpha * AT(p, i, j);
@@ -484,6 +534,7 @@ static int biharmonic_cg(pvo_state *s, double *x, double *b, double a_dt, double
         if (rel < tol) break;
         rz_new = ch_inner(s, r, z);
         beta = rz_new / (rz + 1e-30);
+        PVO_FOR
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++)
                 AT(p, i, j) = AT(z, i, j) + beta * AT(p, i, j);
@@ -505,6 +556,7 @@ int pvo_VOF_DIFFUSE_solve_implicit_biharmonic(pvo_state *s, int rhs, double *rel
     memcpy(psi, s->f[PV_F_C_L], s->nel * sizeof(double));
     compute_laplacian(s, psi, s->f[PV_F_CH_AUX1]);
     compute_laplacian(s, s->f[PV_F_CH_AUX1], psiLG);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++)
             AT(b, i, j) -= bih * AT(psiLG, i, j);
@@ -522,9 +574,10 @@ double pvo_VOF_DIFFUSE_bound_liquid_fraction(pvo_state *s)
 {
     int P = s->P, i, j, k;
     double *C = s->f[PV_F_C_L], *G = s->f[PV_F_C_G];
-    double delta = 0.0;
+    double delta = 0.0, delta_int = 0.0;
     const double cs = 0.0;
-    for (k = 0; k < 2; k++)
+    for (k = 0; k < 2; k++) {
+        PVO_FOR_SUM(delta, delta_int)
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++) {
                 double old_cl = AT(C, i, j);
@@ -538,6 +591,7 @@ double pvo_VOF_DIFFUSE_bound_liquid_fraction(pvo_state *s)
                     double measure = MT(s->m_dx_u, i) * MT(s->m_dy_v, j) * s->slab;
                     measure /= 2.0;
                     delta += (new_cl - old_cl) * measure;
+                    if (i < s->NXM && j < s->NYM) delta_int += (new_cl - old_cl) * measure;
                 }
                 if (k == 1) {
                     double g = 1.0 - new_cl - cs;
@@ -545,8 +599,15 @@ double pvo_VOF_DIFFUSE_bound_liquid_fraction(pvo_state *s)
                     AT(G, i, j) = (g < 0.0) ? 0.0 : ((g > 1.0) ? 1.0 : g);
                 }
             }
+    }
+    s->clamp_int = delta_int;
     return delta;
 }
+
+/* The last clamp's correction over interior cells only (i < NXM, j < NYM):
+ * the reference's sum above also counts the periodic image column i = NXM
+ * and the ghost row j = NYM, which hold copies of column 0 and row NYM-1. */
+double pvo_clamp_interior(const pvo_state *s) { return s->clamp_int; }
 
 /* VOF_DIFFUSE.c:108-126 (the F mirror is not kept) */
 void pvo_VOF_DIFFUSE_update_phase_cache(pvo_state *s)
@@ -569,15 +630,18 @@ void pvo_VOF_DIFFUSE_step(pvo_state *s)
 
     pvo_VOF_DIFFUSE_advect_WENO5(s, PV_F_CH_AUX2);
     pvo_VOF_DIFFUSE_explicit_diffusion(s, PV_F_CH_AUX1);
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++)
             AT(rc, i, j) += AT(rt, i, j);
     bc_ch(s, rc);
     if (st == 0) {
+        PVO_FOR
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++)
                 AT(rv, i, j) = a_dt * AT(C, i, j) + GAMB[0] * AT(rc, i, j);
     } else {
+        PVO_FOR
         for (j = 0; j < s->NY; j++)
             for (i = 0; i < s->NX; i++)
                 AT(rv, i, j) = a_dt * AT(C, i, j) + GAMB[st] * AT(rc, i, j) + ZETB[st] * AT(rn, i, j);
@@ -585,6 +649,7 @@ void pvo_VOF_DIFFUSE_step(pvo_state *s)
     bc_ch(s, rv);
     pvo_VOF_DIFFUSE_solve_implicit_biharmonic(s, PV_F_CH_RHS_NM1, NULL);
     s->st.clamp_delta[st] = pvo_VOF_DIFFUSE_bound_liquid_fraction(s);
+    s->st.clamp_delta_interior[st] = s->clamp_int;
     bc_ch(s, C);
     pvo_VOF_DIFFUSE_update_phase_cache(s);
     memcpy(s->f[PV_F_CH_RHS_NM1], rn, s->nel * sizeof(double));
@@ -598,6 +663,7 @@ void pvo_VOF_update_density_viscosity(pvo_state *s)
     double *C = s->f[PV_F_C_L], *G = s->f[PV_F_C_G], *rho = s->f[PV_F_RHO], *mu = s->f[PV_F_MU];
     const double rho_g = s->prm.rho2 / s->prm.rho1, mu_g = s->prm.mu2 / s->prm.mu1;
     const double rho_s = 1.0, mu_s = 1.0, cs = 0.0;
+    PVO_FOR
     for (j = 0; j < s->NY; j++)
         for (i = 0; i < s->NX; i++) {
             AT(rho, i, j) = AT(C, i, j) + rho_s * cs + rho_g * AT(G, i, j);
@@ -617,6 +683,7 @@ void pvo_VOF_DIFFUSE_compute_f_sigma(pvo_state *s)
     const double cs = 0.0;
     memset(fx, 0, s->nel * sizeof(double));
     memset(fy, 0, s->nel * sizeof(double));
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) {
             double cl = 0.5 * (AT(C, i, j) + AT(C, i - 1, j));
@@ -627,6 +694,7 @@ void pvo_VOF_DIFFUSE_compute_f_sigma(pvo_state *s)
             gradC = (AT(C, i, j) - AT(C, i - 1, j)) * MT(s->m_idx_c, i - 1);
             AT(fx, i, j) = scale * psi_f * gradC;
         }
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double cl = 0.5 * (AT(C, i, j) + AT(C, i, j - 1));
@@ -645,8 +713,10 @@ void pvo_VOF_apply_f_sigma_old(pvo_state *s)
     int P = s->P, i, j;
     double *ru = s->f[PV_F_U_RHS], *rv = s->f[PV_F_V_RHS];
     double *fx = s->f[PV_F_FSX_OLD], *fy = s->f[PV_F_FSY_OLD];
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) AT(ru, i, j) += AT(fx, i, j);
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) AT(rv, i, j) += AT(fy, i, j);
 }
@@ -669,11 +739,13 @@ void pvo_Velocity_add_gravity_2_RHS(pvo_state *s)
     double Ri = s->prm.richardson, gm, gx = 0, gy = 0;
     gm = sqrt(grav[0] * grav[0] + grav[1] * grav[1]);
     if (gm > 0.0) { gx = grav[0] / gm; gy = grav[1] / gm; }
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) {
             double rho_f = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
             AT(ru, i, j) += 2.0 * rho_f * Ri * gx;
         }
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double rho_f = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
@@ -692,6 +764,7 @@ void pvo_Velocity_cell_center(pvo_state *s)
     double *u = s->f[PV_F_U], *v = s->f[PV_F_V], *ub = s->f[PV_F_U_BC], *vb = s->f[PV_F_V_BC];
     halo_u(s, u);
     halo_cell(s, v);
+    PVO_FOR
     for (j = -3; j < s->NY + 2; j++)
         for (i = -3; i < s->NX + 2; i++) {
             AT(ub, i, j) = 0.5 * (AT(u, i, j) + AT(u, i + 1, j));
@@ -709,6 +782,7 @@ void pvo_Velocity_u_set_implicit_explicit(pvo_state *s)
     double *u = s->f[PV_F_U], *v = s->f[PV_F_V], *ub = s->f[PV_F_U_BC], *mu = s->f[PV_F_MU];
     double *ex = s->f[PV_F_U_EXPL], *im = s->f[PV_F_U_IMPL], *ve = s->f[PV_F_U_VISC];
     double iRe = 1.0 / s->prm.Re;
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) {
             double muE = AT(mu, i, j), muW = AT(mu, i - 1, j);
@@ -753,6 +827,7 @@ void pvo_Velocity_v_set_implicit_explicit(pvo_state *s)
     double *u = s->f[PV_F_U], *v = s->f[PV_F_V], *vb = s->f[PV_F_V_BC], *mu = s->f[PV_F_MU];
     double *ex = s->f[PV_F_V_EXPL], *im = s->f[PV_F_V_IMPL], *ve = s->f[PV_F_V_VISC];
     double iRe = 1.0 / s->prm.Re;
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double mu_xp = 2.0 * (0.5 * (AT(mu, i + 1, j - 1) + AT(mu, i + 1, j))) *
@@ -798,6 +873,7 @@ void pvo_Velocity_u_set_RHS(pvo_state *s)
     double *ve = s->f[PV_F_U_VISC], *veo = s->f[PV_F_U_VISC_OLD], *im = s->f[PV_F_U_IMPL];
     double a_dt = 1.0 / (s->dt * BET[rk]);
     double dp_dx_source = -s->prm.dp_dx;
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) {
             double dpdx = (AT(p, i, j) - AT(p, i - 1, j)) * MT(s->m_idx_c, i - 1);
@@ -820,6 +896,7 @@ void pvo_Velocity_v_set_RHS(pvo_state *s)
     double *rhs = s->f[PV_F_V_RHS], *ex = s->f[PV_F_V_EXPL], *exo = s->f[PV_F_V_EXPL_OLD];
     double *ve = s->f[PV_F_V_VISC], *veo = s->f[PV_F_V_VISC_OLD], *im = s->f[PV_F_V_IMPL];
     double a_dt = 1.0 / (s->dt * BET[rk]);
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double dpdy = (AT(p, i, j) - AT(p, i, j - 1)) * MT(s->m_idy_c, j - 1);
@@ -870,6 +947,7 @@ static void matvec(pvo_state *s, double *Ax, const double *x, int comp)
     const double *rho = s->f[PV_F_RHO], *mu = s->f[PV_F_MU];
     const double iRe = 1.0 / s->prm.Re, alpha_k = BET[s->stage], dt = s->dt;
     if (comp == 'u') {
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 1; i < s->NX; i++) {
                 double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i - 1, j));
@@ -888,6 +966,7 @@ static void matvec(pvo_state *s, double *Ax, const double *x, int comp)
                 AT(Ax, i, j) = Ax_val - op;
             }
     } else {
+        PVO_FOR
         for (j = 1; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) {
                 double rho_face = 0.5 * (AT(rho, i, j) + AT(rho, i, j - 1));
@@ -935,6 +1014,7 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
     else             in_tot = 1.0 / (double)((s->NX - 1) * (s->NY - 2) * 1);
 
     matvec(s, Ad, data, comp);
+    PVO_FOR
     for (j = js; j < je; j++)
         for (i = is; i < ie; i++) {
             double diag;
@@ -942,6 +1022,7 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
             diag = vel_diag(s, i, j, comp, alpha_k);
             AT(d, i, j) = AT(r, i, j) / (diag + EPS);
         }
+    PVO_FOR_SUM(rz)
     for (j = js; j < je; j++)
         for (i = is; i < ie; i++) rz += AT(r, i, j) * AT(d, i, j);
     while (iters < s->prm.CG_MAXIT) {
@@ -949,9 +1030,11 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
         iters++;
         vel_bc(s, d, comp, PV_VEL_TYPE_CG);
         matvec(s, Ad, d, comp);
+        PVO_FOR_SUM(dAd)
         for (j = js; j < je; j++)
             for (i = is; i < ie; i++) dAd += AT(d, i, j) * AT(Ad, i, j);
         alpha = rz / (dAd + EPS);
+        PVO_FOR_SUM(lv0, lv1)
         for (j = js; j < je; j++)
             for (i = is; i < ie; i++) {
                 double ri, diag;
@@ -966,6 +1049,7 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
         if (RMS < s->prm.CG_ETOL) break;
         beta = lv0 / (rz + EPS);
         rz = lv0;
+        PVO_FOR
         for (j = js; j < je; j++)
             for (i = is; i < ie; i++) {
                 double diag = vel_diag(s, i, j, comp, alpha_k);
@@ -987,6 +1071,7 @@ void pvo_Pressure_set_RHS(pvo_state *s)
     double *u = s->f[PV_F_U], *v = s->f[PV_F_V], *rhs = s->f[PV_F_P_RHS];
     double a_dt = 1.0 / (s->dt * 2.0 * BET[s->stage]);
     double ihdt = MT(s->m_idx_u, 1) * a_dt;
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double dudx = (AT(u, i + 1, j) - AT(u, i, j)) * ihdt;
@@ -1002,6 +1087,7 @@ void pvo_Pressure_compute_preconditioner(pvo_state *s)
     double *rho = s->f[PV_F_RHO], *M = s->f[PV_F_M_INV];
     const double idx2 = MT(s->m_idx_c, 1) * MT(s->m_idx_c, 1);
     const double idy2 = MT(s->m_idy_c, 1) * MT(s->m_idy_c, 1);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double r0 = AT(rho, i, j);
@@ -1019,6 +1105,7 @@ static void p_operator(pvo_state *s, double *Aphi, const double *phi)
     const double *rho = s->f[PV_F_RHO];
     const double idx2 = MT(s->m_idx_c, 1) * MT(s->m_idx_c, 1);
     const double idy2 = MT(s->m_idy_c, 1) * MT(s->m_idy_c, 1);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double r0 = AT(rho, i, j), c0 = AT(phi, i, j);
@@ -1042,6 +1129,7 @@ static double p_inner(const pvo_state *s, const double *a, const double *b)
 {
     int P = s->P, i, j;
     double sum = 0.0;
+    PVO_FOR_SUM(sum)
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) sum += AT(a, i, j) * AT(b, i, j);
     return sum;
@@ -1059,8 +1147,10 @@ static int p_solve_jacobi(pvo_state *s, double *final_res)
     p_bc(s, phi);
     halo_cell(s, phi);
     p_operator(s, Ad, phi);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) AT(res, i, j) = AT(rhs, i, j) - AT(Ad, i, j);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             AT(z, i, j) = AT(res, i, j) * AT(M, i, j);
@@ -1075,16 +1165,19 @@ static int p_solve_jacobi(pvo_state *s, double *final_res)
         p_operator(s, Ad, d);
         dAd = p_inner(s, d, Ad);
         alpha = gamma / (dAd + 1e-30);
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) {
                 AT(phi, i, j) += alpha * AT(d, i, j);
                 AT(res, i, j) -= alpha * AT(Ad, i, j);
             }
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) AT(z, i, j) = AT(res, i, j) * AT(M, i, j);
         gamma_new = p_inner(s, res, z);
         if (sqrt(gamma_new) < tol) break;
         beta = gamma_new / (gamma + 1e-30);
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) AT(d, i, j) = AT(z, i, j) + beta * AT(d, i, j);
         gamma = gamma_new;
@@ -1092,9 +1185,11 @@ static int p_solve_jacobi(pvo_state *s, double *final_res)
     {   /* NEED_REFERENCE_PRESSURE gauge, :255-279 */
         double sum = 0.0, avg;
         long cnt = 0;
+        PVO_FOR_SUM(sum, cnt)
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) { sum += AT(phi, i, j); ++cnt; }
         avg = sum / (double)cnt;
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) AT(phi, i, j) -= avg;
         halo_cell(s, phi);
@@ -1107,10 +1202,10 @@ static int p_solve_jacobi(pvo_state *s, double *final_res)
 /* ---- FFT-x / tridiagonal-y constant-coefficient solve (A23) -------- */
 
 /* In-place iterative radix-2 complex FFT of length n (power of two) on
- * s->fre / s->fim; sign = -1 forward, +1 inverse (unnormalised). */
-static void fft_pow2(pvo_state *s, int n, int sign)
+ * w->fre / w->fim; sign = -1 forward, +1 inverse (unnormalised). */
+static void fft_pow2(const pvo_state *s, struct pvo_ws *w, int n, int sign)
 {
-    double *re = s->fre, *im = s->fim;
+    double *re = w->fre, *im = w->fim;
     int i, j, k, len;
     for (i = 1, j = 0; i < n; i++) {
         int bit = n >> 1;
@@ -1136,13 +1231,13 @@ static int is_pow2(int n) { return n > 0 && (n & (n - 1)) == 0; }
 /* Real DFT of length n in FFTW R2HC layout: hc[k] = Re X_k (k <= n/2),
  * hc[n-k] = Im X_k (0 < k < n/2), X_k = sum_m x_m exp(-2 pi i k m / n).
  * Radix-2 FFT for powers of two, direct summation otherwise. */
-static void r2hc(pvo_state *s, const double *x, double *hc, int n)
+static void r2hc(const pvo_state *s, struct pvo_ws *w, const double *x, double *hc, int n)
 {
     int k, m;
     if (is_pow2(n)) {
-        for (m = 0; m < n; m++) { s->fre[m] = x[m]; s->fim[m] = 0.0; }
-        fft_pow2(s, n, -1);
-        for (k = 0; k <= n / 2; k++) { s->wre[k] = s->fre[k]; s->wim[k] = s->fim[k]; }
+        for (m = 0; m < n; m++) { w->fre[m] = x[m]; w->fim[m] = 0.0; }
+        fft_pow2(s, w, n, -1);
+        for (k = 0; k <= n / 2; k++) { w->wre[k] = w->fre[k]; w->wim[k] = w->fim[k]; }
     } else {
         for (k = 0; k <= n / 2; k++) {
             double re = 0.0, im = 0.0;
@@ -1151,26 +1246,26 @@ static void r2hc(pvo_state *s, const double *x, double *hc, int n)
                 re += x[m] * s->twc[km];
                 im -= x[m] * s->tws[km];
             }
-            s->wre[k] = re; s->wim[k] = im;
+            w->wre[k] = re; w->wim[k] = im;
         }
     }
-    for (k = 0; k <= n / 2; k++) hc[k] = s->wre[k];
-    for (k = 1; k < (n + 1) / 2; k++) hc[n - k] = s->wim[k];
+    for (k = 0; k <= n / 2; k++) hc[k] = w->wre[k];
+    for (k = 1; k < (n + 1) / 2; k++) hc[n - k] = w->wim[k];
 }
 
 /* Inverse of r2hc without normalisation (FFTW HC2R). */
-static void hc2r(pvo_state *s, const double *hc, double *x, int n)
+static void hc2r(const pvo_state *s, struct pvo_ws *w, const double *hc, double *x, int n)
 {
     int k, m;
     if (is_pow2(n)) {
-        s->fre[0] = hc[0]; s->fim[0] = 0.0;
-        s->fre[n / 2] = hc[n / 2]; s->fim[n / 2] = 0.0;
+        w->fre[0] = hc[0]; w->fim[0] = 0.0;
+        w->fre[n / 2] = hc[n / 2]; w->fim[n / 2] = 0.0;
         for (k = 1; k < n / 2; k++) {
-            s->fre[k] = hc[k]; s->fim[k] = hc[n - k];
-            s->fre[n - k] = hc[k]; s->fim[n - k] = -hc[n - k];
+            w->fre[k] = hc[k]; w->fim[k] = hc[n - k];
+            w->fre[n - k] = hc[k]; w->fim[n - k] = -hc[n - k];
         }
-        fft_pow2(s, n, +1);
-        for (m = 0; m < n; m++) x[m] = s->fre[m];
+        fft_pow2(s, w, n, +1);
+        for (m = 0; m < n; m++) x[m] = w->fre[m];
         return;
     }
     for (m = 0; m < n; m++) {
@@ -1209,34 +1304,40 @@ static void fft_solve(pvo_state *s, const double *rhs, double *out, int mean_fre
     int P = s->P, NXM = s->NXM, NYM = s->NYM, NY = s->NY, i, j;
     double idx = MT(s->m_idx_u, 0), inx = 1.0 / (double)NXM;
     double *spec = s->zbuf;     /* NYM rows of NXM halfcomplex values */
+    PVO_FOR
     for (j = 0; j < NYM; j++) {
-        for (i = 0; i < NXM; i++) s->wrow[i] = AT(rhs, i, j);
-        r2hc(s, s->wrow, &spec[(size_t)j * NXM], NXM);
+        struct pvo_ws *w = &s->ws[PVO_TID];
+        for (i = 0; i < NXM; i++) w->wrow[i] = AT(rhs, i, j);
+        r2hc(s, w, w->wrow, &spec[(size_t)j * NXM], NXM);
         for (i = 0; i < NXM; i++) spec[(size_t)j * NXM + i] *= inx;
     }
+    PVO_FOR_COLS
     for (i = 0; i < NXM; i++) {
+        struct pvo_ws *w = &s->ws[PVO_TID];
         int wave = (i < NXM / 2) ? i : (NXM - i);
         double mw = 2.0 * (1 - cos(2. * PI_REF * wave / (NXM))) * (idx * idx);
         for (j = 0; j < NYM; j++) {
             double as = (j > 0) ? MT(s->m_idy_v, j) * MT(s->m_idy_c, j - 1) : 0.0;
             double an = (j < NY - 2) ? MT(s->m_idy_v, j) * MT(s->m_idy_c, j) : 0.0;
-            s->tsa[j] = as; s->tsc[j] = an;
-            s->tsb[j] = (-as - an) - 0.0 - mw;
-            s->tsr[j] = spec[(size_t)j * NXM + i];
+            w->tsa[j] = as; w->tsc[j] = an;
+            w->tsb[j] = (-as - an) - 0.0 - mw;
+            w->tsr[j] = spec[(size_t)j * NXM + i];
         }
-        if (i == 0) { s->tsa[0] = 0.0; s->tsc[0] = 0.0; s->tsb[0] = 1.0; s->tsr[0] = 0.0; }
-        tridiag(s->tsa, s->tsb, s->tsc, s->tsr, s->tsu, s->tsg, NYM);
+        if (i == 0) { w->tsa[0] = 0.0; w->tsc[0] = 0.0; w->tsb[0] = 1.0; w->tsr[0] = 0.0; }
+        tridiag(w->tsa, w->tsb, w->tsc, w->tsr, w->tsu, w->tsg, NYM);
         if (i == 0 && mean_free) {
             double m = 0.0;
-            for (j = 0; j < NYM; j++) m += s->tsu[j];
+            for (j = 0; j < NYM; j++) m += w->tsu[j];
             m /= (double)NYM;
-            for (j = 0; j < NYM; j++) s->tsu[j] -= m;
+            for (j = 0; j < NYM; j++) w->tsu[j] -= m;
         }
-        for (j = 0; j < NYM; j++) spec[(size_t)j * NXM + i] = s->tsu[j];
+        for (j = 0; j < NYM; j++) spec[(size_t)j * NXM + i] = w->tsu[j];
     }
+    PVO_FOR
     for (j = 0; j < NYM; j++) {
-        hc2r(s, &spec[(size_t)j * NXM], s->wrow, NXM);
-        for (i = 0; i < NXM; i++) AT(out, i, j) = s->wrow[i];
+        struct pvo_ws *w = &s->ws[PVO_TID];
+        hc2r(s, w, &spec[(size_t)j * NXM], w->wrow, NXM);
+        for (i = 0; i < NXM; i++) AT(out, i, j) = w->wrow[i];
     }
 }
 
@@ -1261,9 +1362,11 @@ static int p_solve_fft_pcg(pvo_state *s, double *final_res)
     double *z = (double *)calloc(s->nel, sizeof(double));
     double bnorm, gamma, rel = 0.0, mean = 0.0;
     long cnt = (long)(s->NX - 1) * (s->NY - 1);
+    PVO_FOR_SUM(mean)
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) mean += AT(rhs, i, j);
     mean /= (double)cnt;
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             AT(res, i, j) = AT(rhs, i, j) - mean;
@@ -1274,6 +1377,7 @@ static int p_solve_fft_pcg(pvo_state *s, double *final_res)
     rel = sqrt(p_inner(s, res, res)) / bnorm;
     if (rel > s->prm.p_rel_tol) {
         fft_solve(s, res, z, 1);
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) AT(d, i, j) = AT(z, i, j);
         gamma = p_inner(s, res, z);
@@ -1284,6 +1388,7 @@ static int p_solve_fft_pcg(pvo_state *s, double *final_res)
             p_operator(s, Ad, d);
             dAd = p_inner(s, d, Ad);
             alpha = gamma / dAd;
+            PVO_FOR
             for (j = 0; j < s->NY - 1; j++)
                 for (i = 0; i < s->NX - 1; i++) {
                     AT(phi, i, j) += alpha * AT(d, i, j);
@@ -1295,6 +1400,7 @@ static int p_solve_fft_pcg(pvo_state *s, double *final_res)
             fft_solve(s, res, z, 1);
             gnew = p_inner(s, res, z);
             beta = gnew / gamma;
+            PVO_FOR
             for (j = 0; j < s->NY - 1; j++)
                 for (i = 0; i < s->NX - 1; i++) AT(d, i, j) = AT(z, i, j) + beta * AT(d, i, j);
             gamma = gnew;
@@ -1302,9 +1408,11 @@ static int p_solve_fft_pcg(pvo_state *s, double *final_res)
     }
     {
         double sum = 0.0, avg;
+        PVO_FOR_SUM(sum)
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) sum += AT(phi, i, j);
         avg = sum / (double)cnt;
+        PVO_FOR
         for (j = 0; j < s->NY - 1; j++)
             for (i = 0; i < s->NX - 1; i++) AT(phi, i, j) -= avg;
         halo_cell(s, phi);
@@ -1334,16 +1442,19 @@ void pvo_Pressure_project_velocity(pvo_state *s)
     double mdt = 2.0 * BET[s->stage] * s->dt;
     p_bc(s, dp);
     halo_cell(s, dp);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) AT(p, i, j) += AT(dp, i, j);
     p_bc(s, p);
     halo_cell(s, p);
+    PVO_FOR
     for (j = 0; j < s->NY - 1; j++)
         for (i = 1; i < s->NX; i++) {
             double inv_rho_face = 2.0 / (AT(rho, i, j) + AT(rho, i - 1, j));
             double idxdt = MT(s->m_idx_c, i - 1) * mdt;
             AT(u, i, j) -= (AT(dp, i, j) - AT(dp, i - 1, j)) * (idxdt * inv_rho_face);
         }
+    PVO_FOR
     for (j = 1; j < s->NY - 1; j++)
         for (i = 0; i < s->NX - 1; i++) {
             double inv_rho_face = 2.0 / (AT(rho, i, j) + AT(rho, i, j - 1));

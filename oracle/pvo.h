@@ -25,6 +25,7 @@ void   pvo_get_params(const pvo_state *s, pv_params *out);
 double *pvo_field(pvo_state *s, int field);        /* canonical layout, owned */
 int    pvo_nxt(const pvo_state *s);                /* NXM + 7 */
 int    pvo_nyt(const pvo_state *s);                /* NYM + 7 */
+int    pvo_threads(void);                            /* host threads of the build (1 without OpenMP) */
 int    pvo_init_from_F(pvo_state *s, const double *F);
 void   pvo_set_time_state(pvo_state *s, double time, double dt, double dt_old, int stage);
 void   pvo_get_stats(const pvo_state *s, pv_stats *out);
@@ -41,6 +42,7 @@ void   pvo_VOF_DIFFUSE_advect_WENO5(pvo_state *s, int rhs_out);
 void   pvo_VOF_DIFFUSE_explicit_diffusion(pvo_state *s, int rhs_out);
 int    pvo_VOF_DIFFUSE_solve_implicit_biharmonic(pvo_state *s, int rhs, double *rel_res);
 double pvo_VOF_DIFFUSE_bound_liquid_fraction(pvo_state *s);
+double pvo_clamp_interior(const pvo_state *s);
 void   pvo_VOF_DIFFUSE_update_phase_cache(pvo_state *s);
 void   pvo_VOF_DIFFUSE_step(pvo_state *s);
 void   pvo_VOF_DIFFUSE_compute_f_sigma(pvo_state *s);

@@ -32,6 +32,7 @@ typedef struct pvk_scal {
     double p_gamma, p_alpha, p_beta, p_res, p_bnorm, p_mean;
     int    p_iter, p_done;
     double clamp_delta, div_max, max_idt, dt_new;
+    double clamp_delta_int;  /* the clamp's correction over interior cells only */
     double red[8];           /* raw reduction results, for tests */
 } pvk_scal;
 

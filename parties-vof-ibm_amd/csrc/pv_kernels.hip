@@ -290,7 +290,7 @@ __global__ void k_clamp_props(pvk_grid g, pvk_phys ph, double *C, double *G, dou
                               int do_clamp, double *part, int nb)
 {
     CELL_IJ
-    double vals[1] = {0.0};
+    double vals[2] = {0.0, 0.0};
     if (in) {
         const double cs = 0.0;
         double old_cl = AT(C, i, j), new_cl = old_cl;
@@ -307,6 +307,7 @@ __global__ void k_clamp_props(pvk_grid g, pvk_phys ph, double *C, double *G, dou
             if (j == g.NYM - 1) s += d * ((g.dx_u[i] * g.dy_v[g.NYM] * g.slab) / 2.0);
             if (i == 0 && j == g.NYM - 1) s += d * ((g.dx_u[g.NXM] * g.dy_v[g.NYM] * g.slab) / 2.0);
             vals[0] = s;
+            vals[1] = d * m;    // the interior cell alone (no periodic image, no ghost row)
             AT(C, i, j) = new_cl;
         }
         double gg = 1.0 - new_cl - cs;
@@ -315,12 +316,16 @@ __global__ void k_clamp_props(pvk_grid g, pvk_phys ph, double *C, double *G, dou
         AT(rho, i, j) = new_cl + 1.0 * cs + ph.rho_g * gg;
         AT(mu, i, j) = new_cl + 1.0 * cs + ph.mu_g * gg;
     }
-    if (do_clamp) block_partials<1>(vals, part, nb);
+    if (do_clamp) block_partials<2>(vals, part, nb);
 }
 __global__ void k_clamp_finalize(pvk_scal *sc, const double *part, int nb)
 {
     double s = 2.0 * final_sum(part, nb, 0, false);
-    if (threadIdx.x == 0) sc->clamp_delta = s;
+    double si = 2.0 * final_sum(part, nb, 1, false);
+    if (threadIdx.x == 0) {
+        sc->clamp_delta = s;
+        sc->clamp_delta_int = si;
+    }
 }
 extern "C" void pvk_clamp_props(const pvk_grid *g, const pvk_phys *ph, double *C, double *G, double *rho,
                                 double *mu, const pvk_red *rd, pvk_scal *sc, hipStream_t s)

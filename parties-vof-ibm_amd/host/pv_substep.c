@@ -192,6 +192,7 @@ int pv_VOF_DIFFUSE_step(pv_bag *b)
     RC(ch_step(b));
     RC(pv_read_scalars(b));
     b->st.clamp_delta[b->stage] = b->h_sc->clamp_delta;
+    b->st.clamp_delta_interior[b->stage] = b->h_sc->clamp_delta_int;
     return PV_OK;
 }
 
@@ -1036,6 +1037,7 @@ int pv_Temporal_int_all_the_equations(pv_bag *b)
     LAUNCHCHK();
     RC(pressure_passes(b));
     b->st.clamp_delta[st] = b->h_sc->clamp_delta;
+    b->st.clamp_delta_interior[st] = b->h_sc->clamp_delta_int;
     RC(pv_VOF_swap_f_sigma(b));
     return PV_OK;
 }

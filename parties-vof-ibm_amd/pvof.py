@@ -71,6 +71,7 @@ class Stats(C.Structure):
         ("ch_res", C.c_double * 3), ("p_res", C.c_double * 3),
         ("div_max", C.c_double * 3), ("clamp_delta", C.c_double * 3),
         ("p_fallback", C.c_int * 3),
+        ("clamp_delta_interior", C.c_double * 3),
     ]
 
     def as_dict(self):

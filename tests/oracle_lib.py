@@ -44,6 +44,8 @@ def load_oracle(path=None):
     lib.pvo_VOF_DIFFUSE_solve_implicit_biharmonic.argtypes = [vp, C.c_int, dp]
     lib.pvo_VOF_DIFFUSE_bound_liquid_fraction.restype = C.c_double
     lib.pvo_VOF_DIFFUSE_bound_liquid_fraction.argtypes = [vp]
+    lib.pvo_clamp_interior.restype = C.c_double
+    lib.pvo_clamp_interior.argtypes = [vp]
     lib.pvo_Pressure_compute_velocity_divergence.restype = C.c_double
     lib.pvo_Pressure_compute_velocity_divergence.argtypes = [vp]
     for name in ("Velocity_solve_cg", "Pressure_solve_cg", "Temporal_int_all_the_equations"):

@@ -636,3 +636,32 @@ def test_init_from_F_bootstrap(olib):
         assert not o.get(f).any(), f
     st = o.stats()
     assert st["time"] == 0.0 and st["dt"] == p.default_dt and st["which_stage"] == 0
+
+
+# ---------------------------------------------------------------- clamp accounting (VOF_DIFFUSE.c:1093-1127)
+def test_clamp_sum_counts_the_periodic_image_and_the_ghost_row(olib):
+    """The reference sums the clamp's correction over its G range, which holds
+    the periodic image column i = NXM (a copy of column 0) and the ghost row
+    j = NYM above the top wall (a copy of row NYM-1).  So the reported
+    correction exceeds the change of the interior volume by exactly the
+    corrections of column 0 and row NYM-1 once more (the corner cell three
+    times more); the interior sum is the change of the interior volume."""
+    o = make(olib)
+    p = o.params
+    g = coords(o)
+    rng = np.random.default_rng(79)
+    o.set("C_L", rng.uniform(-0.3, 1.3, o.shape))
+    o.call("VOF_DIFFUSE_set_boundary_values", pvof.F["C_L"])
+    before = o.get("C_L")
+    dG = o.lib.pvo_VOF_DIFFUSE_bound_liquid_fraction(o.h)
+    dI = o.lib.pvo_clamp_interior(o.h)
+    meas = g["dx"] * g["dy"] * p.twod_slab_thickness
+    d = (np.clip(before, 0, 1) - before) * meas
+    nx, ny = p.NXM, p.NYM
+    interior_ = d[3:3 + ny, 3:3 + nx].sum()
+    col0, row_top, corner = d[3:3 + ny, 3].sum(), d[2 + ny, 3:3 + nx].sum(), d[2 + ny, 3]
+    vol_change = (o.get("C_L") - before)[3:3 + ny, 3:3 + nx].sum() * meas
+    assert abs(dI - interior_) < 1e-15 * nx * ny
+    assert abs(dI - vol_change) < 1e-15 * nx * ny
+    assert abs(dG - (interior_ + col0 + row_top + corner)) < 1e-15 * nx * ny
+    assert abs(dG - dI) > 1e-3 * abs(dI)
