@@ -110,6 +110,8 @@ enum pv_field {
     PV_F_U_R, PV_F_U_D, PV_F_U_AD, PV_F_V_R, PV_F_V_D, PV_F_V_AD,
     PV_F_P, PV_F_PHI, PV_F_P_RHS, PV_F_P_RES, PV_F_P_D, PV_F_P_AD, PV_F_M_INV, /* Pressure.c, psolve_cg.c */
     PV_F_SCRATCH0, PV_F_SCRATCH1, PV_F_SCRATCH2,
+    PV_F_F,                                                 /* volume fraction F (VolumeFraction.F),
+                                                               the input of VOF_DIFFUSE_init */
     PV_NFIELDS
 };
 
@@ -169,7 +171,23 @@ int pv_Temporal_int_all_the_equations(pv_bag *bag);
 /* Dtime.c:24-249; writes the new dt (does not store it). */
 int pv_Dtime_cfl(pv_bag *bag, double *dt_out);
 
+/* ---- ghost exchange (Include/Communication.h:8) ------------------------ */
+/* Communication.c:128-295 for one rank that is its own x neighbour
+ * (XPERIODIC) between y walls: the x exchange of pnodes ghost layers
+ * (1..3) on rows -pnodes .. NYM+pnodes.  'u'-type components ('u',
+ * U_VELOCITY 'd', U_VELOCITY_PERTURBATION 'e', FLUX_X 'p') receive pnodes+1
+ * layers below, the image column i = 0 included (:162-169); every other
+ * component wraps the cells 0 .. NXM-1.  In y the rank has no neighbour
+ * (MPI_PROC_NULL: ghost rows are left as they are) and the 2-D z self-wrap
+ * is the identity on the k = 0 slab. */
+int pv_Communication_update_ghost_nodes_flow_variable(pv_bag *bag, int field, int component, int pnodes);
+
 /* ---- Cahn-Hilliard (Eulerian/Include/VOF_DIFFUSE.h:7-31) --------------- */
+/* VOF_DIFFUSE.c:524-568 without IBM: C_L = clamp01(F) over the whole
+ * L range (ghosts included), then boundary values, phase cache, clamp,
+ * boundary values and phase cache of C_L twice, and boundary values of C_G.
+ * Reads PV_F_F; writes C_L and C_G (rho and mu are not touched). */
+int pv_VOF_DIFFUSE_init(pv_bag *bag);
 int pv_VOF_DIFFUSE_set_boundary_values(pv_bag *bag, int field);              /* VOF_DIFFUSE.c:571-683 */
 int pv_VOF_DIFFUSE_compute_laplacian(pv_bag *bag, int in, int out);          /* VOF_DIFFUSE.c:838-863 */
 int pv_VOF_DIFFUSE_compute_bulk_S(pv_bag *bag);                              /* VOF_DIFFUSE.c:865-885 */

@@ -257,6 +257,25 @@ static void halo_u(pvo_state *s, double *f)
     }
 }
 
+/* Communication.c:128-295 on one rank that is its own x neighbour, any
+ * pnodes: rows -pnodes .. NY-1+pnodes; 'u'-type components ('u', 'd', 'e',
+ * 'p') start at Is = 1 and receive pnodes + 1 layers (:162-169); the other
+ * components end at Ie = NX - 1.  No y neighbour; z is the k = 0 slab. */
+void pvo_Communication_update_ghost_nodes_flow_variable(pvo_state *s, int field, int component, int pnodes)
+{
+    double *f = s->f[field];
+    const int P = s->P, NX = s->NX;
+    const int ulike = (component == 'u' || component == 'd' || component == 'e' || component == 'p');
+    const int Is = ulike ? 1 : 0, Ie = ulike ? NX : NX - 1;
+    const int precv = ulike ? pnodes + 1 : pnodes, psend = precv;
+    int i, j;
+    (void)psend;
+    for (j = -pnodes; j < s->NY + pnodes; j++) {
+        for (i = 0; i < precv; i++) AT(f, Is - i - 1, j) = AT(f, Ie - i - 1, j);   /* from the upper side */
+        for (i = 0; i < pnodes; i++) AT(f, Ie + i, j) = AT(f, Is + i, j);           /* from the lower side */
+    }
+}
+
 /* VOF_DIFFUSE.c:571-683: zero-gradient y walls over the L range in i, all
  * three ghost layers, then the cell-type exchange. */
 static void bc_ch(pvo_state *s, double *f)
@@ -1574,19 +1593,14 @@ int pvo_Temporal_int_rk3(pvo_state *s, int nsteps)
     return PV_OK;
 }
 
-/* Cart3d.c:1094-1130, VOF_DIFFUSE.c:524-568, Temporal_int.c:163-199 */
-int pvo_init_from_F(pvo_state *s, const double *F)
+/* VOF_DIFFUSE.c:524-568 without IBM: C_L = clamp01(F) over the L range,
+ * then boundary values / phase cache / clamp / boundary values / phase
+ * cache twice, and the boundary values of C_G. */
+void pvo_VOF_DIFFUSE_init(pvo_state *s)
 {
-    int P = s->P, i, j, k;
-    double *C = s->f[PV_F_C_L], *Fw = s->f[PV_F_SCRATCH2];
-    (void)P;
-    for (k = 0; k < PV_NFIELDS; k++) memset(s->f[k], 0, s->nel * sizeof(double));
-    memcpy(Fw, F, s->nel * sizeof(double));
-    bc_ch(s, Fw);
-    for (i = 0; i < (int)s->nel; i++) {
-        double f = Fw[i];
-        C[i] = (f < 0.0) ? 0.0 : ((f > 1.0) ? 1.0 : f);
-    }
+    double *C = s->f[PV_F_C_L], *F = s->f[PV_F_F];
+    size_t n;
+    for (n = 0; n < s->nel; n++) C[n] = (F[n] < 0.0) ? 0.0 : ((F[n] > 1.0) ? 1.0 : F[n]);
     bc_ch(s, C);
     pvo_VOF_DIFFUSE_update_phase_cache(s);
     pvo_VOF_DIFFUSE_bound_liquid_fraction(s);
@@ -1595,6 +1609,17 @@ int pvo_init_from_F(pvo_state *s, const double *F)
     bc_ch(s, C);
     pvo_VOF_DIFFUSE_update_phase_cache(s);
     bc_ch(s, s->f[PV_F_C_G]);
+}
+
+/* Cart3d.c:1094-1130: boundary values of F, VOF_DIFFUSE_init, density and
+ * viscosity, psi_LG and f_sigma, new -> old; Temporal_int.c:163-199 start. */
+int pvo_init_from_F(pvo_state *s, const double *F)
+{
+    int k;
+    for (k = 0; k < PV_NFIELDS; k++) memset(s->f[k], 0, s->nel * sizeof(double));
+    memcpy(s->f[PV_F_F], F, s->nel * sizeof(double));
+    bc_ch(s, s->f[PV_F_F]);
+    pvo_VOF_DIFFUSE_init(s);
     pvo_VOF_update_density_viscosity(s);
     pvo_VOF_DIFFUSE_compute_psi_LG(s);
     pvo_VOF_DIFFUSE_compute_f_sigma(s);
@@ -1607,6 +1632,5 @@ int pvo_init_from_F(pvo_state *s, const double *F)
     memset(&s->st, 0, sizeof(s->st));
     pvo_Velocity_cell_center(s);
     halo_cell(s, s->f[PV_F_P]);
-    (void)j;
     return PV_OK;
 }

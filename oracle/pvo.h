@@ -44,6 +44,8 @@ int    pvo_VOF_DIFFUSE_solve_implicit_biharmonic(pvo_state *s, int rhs, double *
 double pvo_VOF_DIFFUSE_bound_liquid_fraction(pvo_state *s);
 double pvo_clamp_interior(const pvo_state *s);
 void   pvo_VOF_DIFFUSE_update_phase_cache(pvo_state *s);
+void   pvo_VOF_DIFFUSE_init(pvo_state *s);
+void   pvo_Communication_update_ghost_nodes_flow_variable(pvo_state *s, int field, int component, int pnodes);
 void   pvo_VOF_DIFFUSE_step(pvo_state *s);
 void   pvo_VOF_DIFFUSE_compute_f_sigma(pvo_state *s);
 void   pvo_VOF_update_density_viscosity(pvo_state *s);

@@ -136,6 +136,9 @@ typedef struct pvk_cnn {
 void pvk_bc_ch(const pvk_grid *g, double *f, hipStream_t s);          /* VOF_DIFFUSE_set_boundary_values */
 void pvk_halo_cell(const pvk_grid *g, double *f, hipStream_t s);
 void pvk_halo_u(const pvk_grid *g, double *f, hipStream_t s);
+/* x exchange of pnodes (1..3) ghost layers over rows -pnodes .. NYM+pnodes;
+ * ulike: u-type wrap (pnodes+1 layers below, image column 0 included) */
+void pvk_xwrap(const pvk_grid *g, double *f, int ulike, int pnodes, hipStream_t s);
 void pvk_vel_bc(const pvk_grid *g, double *f, int comp, int type, hipStream_t s);
 void pvk_p_bc(const pvk_grid *g, double *f, hipStream_t s);          /* Pressure_apply_BCs + cell halo */
 void pvk_ghost_multi(const pvk_grid *g, double *f0, double *f1, double *f2, double *f3, int nf, int kind,
@@ -170,6 +173,8 @@ void pvk_psi_fsigma(const pvk_grid *g, const pvk_phys *ph, const double *C, cons
                     double *lap, double *psi, double *fx, double *fy, hipStream_t s);
 void pvk_psi_only(const pvk_grid *g, const pvk_phys *ph, const double *C, double *psi, hipStream_t s);
 void pvk_clamp_init(const pvk_grid *g, const double *F, double *C, hipStream_t s);
+/* C = clamp01(F) over the whole L range, ghosts included */
+void pvk_clamp_slab(const pvk_grid *g, const double *F, double *C, hipStream_t s);
 
 /* ---- momentum ---- */
 void pvk_cell_center(const pvk_grid *g, const double *u, const double *v, double *ub, double *vb,

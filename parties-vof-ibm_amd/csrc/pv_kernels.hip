@@ -210,6 +210,34 @@ extern "C" void pvk_vel_bc(const pvk_grid *g, double *f, int comp, int type, hip
     }
 }
 
+// Communication.c:146-295 on one rank that is its own x neighbour: thread
+// (slot, j) copies one ghost of row j.  Slots 0 .. nlo-1 fill the columns
+// below the domain, the rest the columns above it.
+__global__ void k_xwrap(pvk_grid g, double *f, int ulike, int pnodes)
+{
+    const int P = g.P, NXM = g.NXM;
+    const int nlo = ulike ? pnodes + 1 : pnodes, nslot = nlo + pnodes;
+    const int nrow = g.NYM + 1 + 2 * pnodes;            // j in [-pnodes, NYM + pnodes]
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nslot * nrow) return;
+    const int slot = t / nrow, j = t % nrow - pnodes;
+    int dst, src;
+    if (slot < nlo) {                                  // data[Is-q-1] = data[Ie-q-1]
+        dst = ulike ? -slot : -1 - slot;
+        src = ulike ? NXM - slot : NXM - 1 - slot;
+    } else {                                           // data[Ie+q] = data[Is+q]
+        const int q = slot - nlo;
+        dst = ulike ? NXM + 1 + q : NXM + q;
+        src = ulike ? 1 + q : q;
+    }
+    AT(f, dst, j) = AT(f, src, j);
+}
+extern "C" void pvk_xwrap(const pvk_grid *g, double *f, int ulike, int pnodes, hipStream_t s)
+{
+    const int n = ((ulike ? pnodes + 1 : pnodes) + pnodes) * (g->NYM + 1 + 2 * pnodes);
+    hipLaunchKernelGGL(k_xwrap, dim3((n + 255) / 256), dim3(256), 0, s, *g, f, ulike, pnodes);
+}
+
 __global__ void k_fill(double *f, size_t n, double val)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -1005,6 +1033,22 @@ __global__ void k_clamp_init(pvk_grid g, const double *F, double *C)
 extern "C" void pvk_clamp_init(const pvk_grid *g, const double *F, double *C, hipStream_t s)
 {
     hipLaunchKernelGGL(k_clamp_init, grid2d(g), dim3(BX, BY), 0, s, *g, F, C);
+}
+
+// VOF_DIFFUSE.c:531-537: the L-range loop, ghosts included
+__global__ void k_clamp_slab(pvk_grid g, const double *F, double *C)
+{
+    const int P = g.P;
+    int i = blockIdx.x * BX + threadIdx.x - 3, j = blockIdx.y * BY + threadIdx.y - 3;
+    if (i <= g.NXM + 3 && j <= g.NYM + 3) {
+        double f = AT(F, i, j);
+        AT(C, i, j) = (f < 0.0) ? 0.0 : ((f > 1.0) ? 1.0 : f);
+    }
+}
+extern "C" void pvk_clamp_slab(const pvk_grid *g, const double *F, double *C, hipStream_t s)
+{
+    dim3 gr((g->NXM + 7 + BX - 1) / BX, (g->NYM + 7 + BY - 1) / BY);
+    hipLaunchKernelGGL(k_clamp_slab, gr, dim3(BX, BY), 0, s, *g, F, C);
 }
 
 // whole-slab copy including ghosts (Array_copy_withghost)

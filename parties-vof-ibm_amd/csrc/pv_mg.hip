@@ -24,6 +24,7 @@
 // writes z in FP64, its own x and faces are FP32).
 #include <hip/hip_runtime.h>
 #include <math.h>
+#include <mutex>
 #include "pv_kernels.h"
 #include "pv_block.h"
 
@@ -945,15 +946,24 @@ extern "C" void pvk_mg_coarse(const pvk_mg *m, const pvk_grid *g, pvk_scal *sc, 
                               double tol, int maxit, hipStream_t s)
 {
     // above 64 KiB of dynamic LDS the kernel must be allowed it (gfx950: up to
-    // 160 KiB per workgroup); a refusal leaves the launch to report the error
-    static size_t allowed[2] = {65536, 65536};
+    // 160 KiB per workgroup).  Handles of any size may launch from several host
+    // threads at once, so the limit is raised once per device and kernel, to
+    // the whole 160 KiB, under a lock; a refusal leaves the launch to report
+    // the error.
+    static std::mutex mu;
+    static unsigned char raised[64][2];
     const size_t need = pvk_mg_lds_bytes(m);
-    if (need > allowed[m->fp32]) {
-        const void *k = m->fp32 ? (const void *)k_mg_coarse<float> : (const void *)k_mg_coarse<double>;
-        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)need) == hipSuccess)
-            allowed[m->fp32] = need;
-        else
-            (void)hipGetLastError();
+    if (need > 65536) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(mu);
+        if (dev >= 0 && dev < 64 && !raised[dev][m->fp32]) {
+            const void *k = m->fp32 ? (const void *)k_mg_coarse<float> : (const void *)k_mg_coarse<double>;
+            if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess)
+                raised[dev][m->fp32] = 1;
+            else
+                (void)hipGetLastError();
+        }
     }
     if (m->c0 == 0) hipLaunchKernelGGL(k_mg_check, dim3(1), dim3(NTH), 0, s, sc, *w, n, tol, maxit);
     if (m->fp32) hipLaunchKernelGGL(k_mg_coarse<float>, dim3(1), dim3(CT), need, s, *m, sc);

@@ -157,13 +157,14 @@ static void fill_phys(pv_bag *b)
     ph->precision = p->precision;
 }
 
+static int create_on_device(pv_bag *b, const pv_params *prm);
+
 int pv_create(const pv_params *prm, int device, pv_bag **out)
 {
     pv_bag *b;
-    int ndev = 0, k;
-    size_t bytes;
-    pv_metrics m;
+    int ndev = 0, rc;
     hipDeviceProp_t prop;
+    if (!out) return pv_set_error(PV_EINVAL, "null output handle pointer");
     *out = NULL;
     if (!prm || prm->NXM < 8 || prm->NYM < 8)
         return pv_set_error(PV_EINVAL, "NXM and NYM must be >= 8");
@@ -180,16 +181,30 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
                             prop.gcnArchName);
     b = (pv_bag *)calloc(1, sizeof *b);
     if (!b) return pv_set_error(PV_ENOMEM, "host allocation failed");
+    b->device = device;
+    /* every failure after this point releases what was made so far */
+    if ((rc = create_on_device(b, prm)) != PV_OK) {
+        pv_destroy(b);
+        return rc;
+    }
+    *out = b;
+    return PV_OK;
+}
+
+static int create_on_device(pv_bag *b, const pv_params *prm)
+{
+    int k;
+    size_t bytes;
+    pv_metrics m;
     b->prm = *prm;
     pv_derive_params(&b->prm, &b->slab);
-    b->device = device;
     b->NXM = prm->NXM; b->NYM = prm->NYM;
     b->NXT = prm->NXM + 7; b->NYT = prm->NYM + 7;
     b->P = ((b->NXT + 15) / 16) * 16;
     b->alloc_elems = (size_t)OFF0 + (size_t)b->P * (size_t)b->NYT + 16;
     bytes = b->alloc_elems * sizeof(double) * PV_FI_END;
     if (hipMalloc((void **)&b->slabmem, bytes) != hipSuccess) {
-        free(b);
+        b->slabmem = NULL;
         return pv_set_error(PV_ENOMEM, "hipMalloc of %zu bytes failed", bytes);
     }
     HIPCHK(hipMemset(b->slabmem, 0, bytes));
@@ -199,6 +214,7 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
 
     /* metrics */
     pv_host_metrics(&b->prm, &m);
+    memcpy(&b->hm, &m, sizeof m);      /* owned by the handle from here on */
     {
         size_t nmx = m.nx, nmy = m.ny;
         double *dm;
@@ -223,7 +239,6 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
         b->dx_min = (1.0 / m.idx_u[3] > 1.0 / m.idy_v[3]) ? 1.0 / m.idy_v[3] : 1.0 / m.idx_u[3];
         b->h_idx_u0 = m.idx_u[3];
         b->h_idy_v0 = m.idy_v[3];
-        memcpy(&b->hm, &m, sizeof m);
     }
     b->g.NXM = b->NXM; b->g.NYM = b->NYM; b->g.P = b->P;
     b->g.nbx = (b->NXM + 63) / 64; b->g.nby = (b->NYM + 3) / 4;
@@ -246,24 +261,14 @@ int pv_create(const pv_params *prm, int device, pv_bag **out)
     {
         int rc = pv_cg_create(b);
         if (rc) return rc;
-    }
-    {
-        int rc = pv_fft_create(b);
-        if (rc) return rc;
+        if ((rc = pv_fft_create(b))) return rc;
         if ((rc = pv_mg_create(b))) return rc;
         if ((b->prm.pressure_solver == PV_PSOLVE_MG_PCG || b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG) &&
-            !b->mg_ok) {
-            pv_destroy(b);
+            !b->mg_ok)
             return pv_set_error(PV_EINVAL, "MG-PCG: %d x %d does not halve to <= %d cells and a last level "
                                 "of <= %d cells", prm->NXM, prm->NYM, PVK_MG_LDS_CELLS, PVK_MG_BOTTOM);
-        }
+        return pv_sync_scalars_to_device(b);
     }
-    {
-        int rc = pv_sync_scalars_to_device(b);
-        if (rc) return rc;
-    }
-    *out = b;
-    return PV_OK;
 }
 
 /* Work sets of the fused biharmonic and momentum CG loops: one band of rows
@@ -317,7 +322,7 @@ int pv_destroy(pv_bag *b)
 {
     if (!b) return PV_OK;
     hipSetDevice(b->device);
-    hipStreamSynchronize(b->stream);
+    if (b->stream) hipStreamSynchronize(b->stream);
     if (b->timers_on) pv_timers_enable(b, 0);
     pv_fft_destroy(b);
     pv_mg_destroy(b);
@@ -329,8 +334,8 @@ int pv_destroy(pv_bag *b)
     hipFree(b->metricmem);
     hipFree(b->rd.part);
     hipFree(b->d_sc);
-    hipHostFree(b->h_sc);
-    hipStreamDestroy(b->stream);
+    if (b->h_sc) hipHostFree(b->h_sc);
+    if (b->stream) hipStreamDestroy(b->stream);
     pv_host_metrics_free(&b->hm);
     free(b);
     return PV_OK;

@@ -147,33 +147,36 @@ static double sd_bub(double x, double y, const void *c)
 int pv_ic_bubbles(const pv_params *prm, unsigned long long seed, int nmin, int nmax, double rmin, double rmax,
                   int gas_inside, double *F, int *nplaced)
 {
-    static bub_ctx c;
+    bub_ctx *c;                       /* ~96 KiB: on the heap, so calls may run concurrently */
     pv_params p = *prm;
     unsigned long long s = seed;
     double slab, gap;
-    int target, tries = 0, k;
+    int target, tries = 0, k, rc;
     if (nmin < 1 || nmax < nmin || nmax > MAXB) return pv_set_error(PV_EINVAL, "bubble count out of range");
+    c = (bub_ctx *)calloc(1, sizeof *c);
+    if (!c) return pv_set_error(PV_ENOMEM, "pv_ic_bubbles: out of host memory");
     pv_derive_params(&p, &slab);
     gap = 4.0 * p.Cn;
-    memset(&c, 0, sizeof c);
-    c.Lx = p.xmax - p.xmin;
-    c.gas_inside = gas_inside;
+    c->Lx = p.xmax - p.xmin;
+    c->gas_inside = gas_inside;
     target = nmin + (int)(u01(&s) * (double)(nmax - nmin + 1));
     if (target > nmax) target = nmax;
-    while (c.n < target && tries < 200000) {
+    while (c->n < target && tries < 200000) {
         double r = rmin + (rmax - rmin) * u01(&s);
-        double x = p.xmin + c.Lx * u01(&s);
+        double x = p.xmin + c->Lx * u01(&s);
         double y = p.ymin + (r + gap) + ((p.ymax - p.ymin) - 2.0 * (r + gap)) * u01(&s);
         int ok = 1;
         tries++;
-        for (k = 0; k < c.n && ok; k++) {
-            double dx = fabs(x - c.x[k]);
-            if (dx > 0.5 * c.Lx) dx = c.Lx - dx;
-            if (sqrt(dx * dx + (y - c.y[k]) * (y - c.y[k])) < r + c.r[k] + gap) ok = 0;
+        for (k = 0; k < c->n && ok; k++) {
+            double dx = fabs(x - c->x[k]);
+            if (dx > 0.5 * c->Lx) dx = c->Lx - dx;
+            if (sqrt(dx * dx + (y - c->y[k]) * (y - c->y[k])) < r + c->r[k] + gap) ok = 0;
         }
         if (!ok) continue;
-        c.x[c.n] = x; c.y[c.n] = y; c.r[c.n] = r; c.n++;
+        c->x[c->n] = x; c->y[c->n] = y; c->r[c->n] = r; c->n++;
     }
-    if (nplaced) *nplaced = c.n;
-    return fill_profile(prm, F, sd_bub, &c);
+    if (nplaced) *nplaced = c->n;
+    rc = fill_profile(prm, F, sd_bub, c);
+    free(c);
+    return rc;
 }

@@ -100,6 +100,39 @@ int pv_VOF_DIFFUSE_set_boundary_values(pv_bag *b, int field)
     return PV_OK;
 }
 
+/* Communication.c:128-295, one rank: x exchange only (no y neighbour, z is
+ * the k = 0 slab) */
+int pv_Communication_update_ghost_nodes_flow_variable(pv_bag *b, int field, int component, int pnodes)
+{
+    int ulike;
+    RC(begin(b));
+    if (field < 0 || field >= PV_NFIELDS) return pv_set_error(PV_EINVAL, "bad field id %d", field);
+    if (pnodes < 1 || pnodes > 3) return pv_set_error(PV_EINVAL, "pnodes must be 1..3 (got %d)", pnodes);
+    ulike = component == 'u' || component == 'd' || component == 'e' || component == 'p';
+    pvk_xwrap(G(b), FD(b, field), ulike, pnodes, S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
+/* VOF_DIFFUSE.c:524-568 (no IBM).  The phase-cache kernels also refresh rho
+ * and mu from C_G, as they do everywhere in this library. */
+int pv_VOF_DIFFUSE_init(pv_bag *b)
+{
+    RC(begin(b));
+    pvk_clamp_slab(G(b), FD(b, PV_F_F), FD(b, PV_F_C_L), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_phase_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), S(b));
+    pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
+                    b->d_sc, S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_phase_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
+    pvk_phase_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), S(b));
+    pvk_bc_ch(G(b), FD(b, PV_F_C_G), S(b));
+    LAUNCHCHK();
+    return PV_OK;
+}
+
 int pv_VOF_DIFFUSE_compute_laplacian(pv_bag *b, int in, int out)
 {
     RC(begin(b));
@@ -1086,11 +1119,9 @@ int pv_init_from_F(pv_bag *b, const double *F_host)
     if (!F_host) return pv_set_error(PV_EINVAL, "null F");
     pvk_fill(b->slabmem, b->alloc_elems * PV_FI_END, 0.0, S(b));
     LAUNCHCHK();
-    RC(pv_set_field(b, PV_F_SCRATCH2, F_host));
-    pvk_bc_ch(G(b), FD(b, PV_F_SCRATCH2), S(b));
-    pvk_clamp_init(G(b), FD(b, PV_F_SCRATCH2), FD(b, PV_F_C_L), S(b));
-    pvk_clamp_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), &b->rd,
-                    b->d_sc, S(b));
+    RC(pv_set_field(b, PV_F_F, F_host));
+    pvk_bc_ch(G(b), FD(b, PV_F_F), S(b));
+    RC(pv_VOF_DIFFUSE_init(b));
     pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
                    FD(b, PV_F_FSY_NEW), S(b));
     pvk_copy(G(b), FD(b, PV_F_FSX_NEW), FD(b, PV_F_FSX_OLD), S(b));
@@ -1118,31 +1149,50 @@ int pv_set_graphs(pv_bag *b, int on, int chunk)
     return PV_OK;
 }
 
+/* Time one candidate chunk (0 = eager launches): a warm-up solve that
+ * captures the graphs, then two timed solves. */
+static int time_chunk(pv_bag *b, int chunk, float *ms)
+{
+    hipEvent_t e0 = NULL, e1 = NULL;
+    int rep, its, rc;
+    if ((rc = pv_set_graphs(b, chunk > 0, chunk > 0 ? chunk : 16))) return rc;
+    if ((rc = pv_Pressure_solve_cg(b, &its))) return rc;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = pv_set_error(PV_EHIP, "autotune: hipEventCreate failed");
+    } else if (hipEventRecord(e0, S(b)) != hipSuccess) {
+        rc = pv_set_error(PV_EHIP, "autotune: hipEventRecord failed");
+    } else {
+        for (rep = 0; rep < 2 && rc == PV_OK; rep++) rc = pv_Pressure_solve_cg(b, &its);
+        if (rc == PV_OK && (hipEventRecord(e1, S(b)) != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+                            hipEventElapsedTime(ms, e0, e1) != hipSuccess))
+            rc = pv_set_error(PV_EHIP, "autotune: event timing failed");
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return rc;
+}
+
 /* Pick the fastest of eager launches and graph chunks 8, 16, 32 for the
  * pressure solve of the current state (the solve restarts from phi = 0 and
- * its right-hand side is left as it is, so the trajectory is unchanged). */
+ * its right-hand side is left as it is, so the trajectory is unchanged).  On
+ * a failure the handle keeps the graph setting it had before the call. */
 int pv_autotune(pv_bag *b, int *choice)
 {
     static const int chunks[4] = {0, 8, 16, 32};
     double best = 1e300;
-    int k, rep, pick = 0, its;
+    int k, pick = 0, rc = PV_OK;
+    const int use0 = b ? b->use_graphs : 0, chunk0 = b ? b->graph_chunk : 0;
     RC(begin(b));
     if (b->prm.pressure_solver != PV_PSOLVE_FFT_PCG) return pv_set_error(PV_EINVAL, "autotune is for FFT-PCG");
-    for (k = 0; k < 4; k++) {
-        hipEvent_t e0, e1;
+    for (k = 0; k < 4 && rc == PV_OK; k++) {
         float ms = 0.0f;
-        RC(pv_set_graphs(b, chunks[k] > 0, chunks[k] > 0 ? chunks[k] : 16));
-        RC(pv_Pressure_solve_cg(b, &its));                   /* warm-up, captures graphs */
-        HIPCHK(hipEventCreate(&e0));
-        HIPCHK(hipEventCreate(&e1));
-        HIPCHK(hipEventRecord(e0, S(b)));
-        for (rep = 0; rep < 2; rep++) RC(pv_Pressure_solve_cg(b, &its));
-        HIPCHK(hipEventRecord(e1, S(b)));
-        HIPCHK(hipEventSynchronize(e1));
-        HIPCHK(hipEventElapsedTime(&ms, e0, e1));
-        hipEventDestroy(e0);
-        hipEventDestroy(e1);
-        if (ms < best) { best = ms; pick = k; }
+        if ((rc = time_chunk(b, chunks[k], &ms)) == PV_OK && ms < best) { best = ms; pick = k; }
+    }
+    if (rc != PV_OK) {
+        char msg[512];
+        snprintf(msg, sizeof msg, "%s", pv_last_error());
+        pv_set_graphs(b, use0, use0 ? chunk0 : 16);
+        return pv_set_error(rc, "%s", msg);
     }
     RC(pv_set_graphs(b, chunks[pick] > 0, chunks[pick] > 0 ? chunks[pick] : 16));
     if (choice) *choice = chunks[pick];

@@ -37,6 +37,7 @@ FIELDS = [
     "U_R", "U_D", "U_AD", "V_R", "V_D", "V_AD",
     "P", "PHI", "P_RHS", "P_RES", "P_D", "P_AD", "M_INV",
     "SCRATCH0", "SCRATCH1", "SCRATCH2",
+    "F",
 ]
 F = {name: k for k, name in enumerate(FIELDS)}
 PV_NFIELDS = len(FIELDS)
@@ -382,3 +383,55 @@ def ic(lib, kind, params, **kw):
     else:
         raise ValueError(kind)
     return Fld
+
+
+SHIM_PATH = os.path.join(HERE, "_build", "libparties_vof_shim.so")
+PVS_L_RANGE, PVS_G_RANGE = 0, 1
+FATAL_FN = C.CFUNCTYPE(None, C.c_char_p, C.c_void_p)
+
+
+def load_shim(path=SHIM_PATH):
+    """Load the reference-signature layer (include/parties_vof_shim.h)."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run __graft_entry__.build() first")
+    sh = C.CDLL(path)
+    vp = C.c_void_p
+    sh.pvs_last_error.restype = C.c_char_p
+    sh.pvs_bind.argtypes = [vp, vp, vp, vp, vp, vp, vp]
+    sh.pvs_bind_field.argtypes = [vp, vp, C.c_int, C.c_int]
+    sh.pvs_unbind.argtypes = [vp]
+    sh.pvs_handle.argtypes = [vp]
+    sh.pvs_handle.restype = vp
+    sh.pvs_push.argtypes = [vp, vp]
+    sh.pvs_pull.argtypes = [vp, vp]
+    sh.pvs_set_time_max.argtypes = [vp, C.c_double]
+    sh.pvs_set_fatal.argtypes = [FATAL_FN, vp]
+    sh.Temporal_int_rk3.argtypes = [vp, vp]
+    sh.Temporal_int_all_the_equations.argtypes = [vp, vp]
+    sh.Dtime_cfl.argtypes = [vp]
+    sh.Dtime_cfl.restype = C.c_double
+    sh.Communication_update_ghost_nodes_flow_variable.argtypes = [vp, C.c_char, C.c_int, vp]
+    for name in ("VOF_DIFFUSE_set_boundary_values",):
+        getattr(sh, name).argtypes = [vp, vp]
+    sh.VOF_DIFFUSE_compute_laplacian.argtypes = [vp, vp, vp]
+    for name in ("VOF_DIFFUSE_advect_WENO5", "VOF_DIFFUSE_explicit_diffusion",
+                 "VOF_DIFFUSE_solve_implicit_biharmonic"):
+        getattr(sh, name).argtypes = [vp, vp]
+    sh.Velocity_update_boundaries.argtypes = [vp, C.c_char, C.c_int, vp]
+    sh.Velocity_solve_cg.argtypes = [vp, vp]
+    sh.Pressure_solve_cg.argtypes = [vp]
+    sh.Pressure_operator_variableCoeff.argtypes = [vp, vp, vp, vp, vp, vp]
+    sh.Pressure_apply_BCs.argtypes = [vp, vp, vp]
+    sh.Pressure_compute_velocity_divergence.argtypes = [vp]
+    sh.Pressure_compute_velocity_divergence.restype = C.c_double
+    sh.Pressure_solve_x_period.argtypes = [vp, vp, vp]
+    for name in ("VOF_DIFFUSE_init", "VOF_DIFFUSE_update_phase_cache", "VOF_DIFFUSE_compute_bulk_S",
+                 "VOF_DIFFUSE_compute_psi_LG", "VOF_DIFFUSE_step", "VOF_DIFFUSE_compute_f_sigma",
+                 "VOF_DIFFUSE_apply_f_sigma_old", "VOF_DIFFUSE_update_density_viscosity",
+                 "VOF_update_density_viscosity", "VOF_apply_f_sigma_old", "VOF_swap_f_sigma",
+                 "Velocity_add_gravity_2_RHS", "Velocity_cell_center", "Velocity_u_set_implicit_explicit",
+                 "Velocity_v_set_implicit_explicit", "Velocity_w_set_implicit_explicit", "Velocity_u_set_RHS",
+                 "Velocity_v_set_RHS", "Velocity_w_set_RHS", "Pressure_set_RHS",
+                 "Pressure_compute_preconditioner", "Pressure_project_velocity"):
+        getattr(sh, name).argtypes = [vp]
+    return sh

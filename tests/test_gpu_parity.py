@@ -326,3 +326,36 @@ def test_fft_solve_sizes(gpu, lib, n):
     got, ref = s.get("SCRATCH1")[R["cell"]], o.get("SCRATCH1")[R["cell"]]
     assert np.isfinite(got).all()
     assert relerr(got, ref) < 1e-10
+
+
+@pytest.mark.parametrize("component", ["c", "u", "p", "f"])
+@pytest.mark.parametrize("pnodes", [1, 2, 3])
+def test_ghost_exchange(gpu, lib, component, pnodes):
+    p = params_small()
+    o = oracle_lib.Oracle(p)
+    s = pvof.Solver(p, lib)
+    a = np.random.default_rng(pnodes).standard_normal(s.shape)
+    o.set("SCRATCH1", a)
+    s.set("SCRATCH1", a)
+    o.call("Communication_update_ghost_nodes_flow_variable", pvof.F["SCRATCH1"], ord(component), pnodes)
+    s.call("Communication_update_ghost_nodes_flow_variable", pvof.F["SCRATCH1"], ord(component), pnodes)
+    assert np.array_equal(s.get("SCRATCH1"), o.get("SCRATCH1"))
+    s.close()
+
+
+def test_vof_diffuse_init(gpu, lib):
+    p = params_small()
+    o = oracle_lib.Oracle(p)
+    s = pvof.Solver(p, lib)
+    F = np.random.default_rng(11).uniform(-0.3, 1.3, size=s.shape)
+    o.set("F", F)
+    s.set("F", F)
+    o.call("VOF_DIFFUSE_init")
+    s.call("VOF_DIFFUSE_init")
+    # C_L over the whole L range (its clamp covers the ghosts); C_G over the
+    # frame every stencil reads (the last ghost row and column are not
+    # refreshed by the fused phase-cache kernel)
+    assert np.array_equal(s.get("C_L"), o.get("C_L"))
+    fr = regions(p)["frame"]
+    assert np.array_equal(s.get("C_G")[fr], o.get("C_G")[fr])
+    s.close()

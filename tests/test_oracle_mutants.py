@@ -120,8 +120,17 @@ MUTANTS = [
      Q.test_pressure_rhs_is_scaled_divergence, {}),
     ("jacobi_pcg_stop_test", "if (sqrt(gamma_new) < tol) break;", "if (sqrt(gamma_new) < 1e3 * tol) break;",
      Q.test_pressure_solve_satisfies_poisson, {"solver": 0}),
-    ("init_clamp", "C[i] = (f < 0.0) ? 0.0 : ((f > 1.0) ? 1.0 : f);", "C[i] = (f < 0.0) ? 0.0 : ((f > 1.0) ? 0.99 : f);",
-     Q.test_init_from_F_bootstrap, {}),
+    ("init_clamp", "C[n] = (F[n] < 0.0) ? 0.0 : ((F[n] > 1.0) ? 1.0 : F[n]);",
+     "C[n] = (F[n] < 0.0) ? 0.0 : ((F[n] > 1.0) ? 0.99 : F[n]);", Q.test_init_from_F_bootstrap, {}),
+    ("init_clamp_range", "C[n] = (F[n] < 0.0) ? 0.0 : ((F[n] > 1.0) ? 1.0 : F[n]);",
+     "C[n] = (F[n] < 0.0) ? 0.0 : ((F[n] > 0.9) ? 0.9 : F[n]);",
+     Q.test_vof_diffuse_init_clamps_and_completes_the_phases, {}),
+    # ghost exchange: U_VELOCITY ('d') taken for a cell-type component, and
+    # one layer too few received
+    ("comm_u_type", "(component == 'u' || component == 'd' ||", "(component == 'u' ||",
+     Q.test_ghost_exchange_is_the_periodic_extension, {"component": "d", "pnodes": 3}),
+    ("comm_layers", "const int precv = ulike ? pnodes + 1 : pnodes", "const int precv = ulike ? pnodes : pnodes",
+     Q.test_ghost_exchange_is_the_periodic_extension, {"component": "u", "pnodes": 2}),
 ]
 
 

@@ -665,3 +665,64 @@ def test_clamp_sum_counts_the_periodic_image_and_the_ghost_row(olib):
     assert abs(dI - vol_change) < 1e-15 * nx * ny
     assert abs(dG - (interior_ + col0 + row_top + corner)) < 1e-15 * nx * ny
     assert abs(dG - dI) > 1e-3 * abs(dI)
+
+
+# ---------------------------------------------------------------------------
+# ghost exchange (Communication.c:128-295) and VOF_DIFFUSE_init (VOF_DIFFUSE.c:524-568)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("component", ["c", "a", "u", "d", "p", "q"])
+@pytest.mark.parametrize("pnodes", [1, 2, 3])
+def test_ghost_exchange_is_the_periodic_extension(olib, component, pnodes):
+    """One rank that is its own x neighbour: after the exchange every ghost
+    column it fills holds the periodic image of its column (period NXM in
+    the index: cells 0..NXM-1, and for u-type data faces 1..NXM with face 0
+    the image of face NXM), on rows -pnodes .. NYM+pnodes; everything else is
+    untouched."""
+    o = make(olib)
+    p = o.params
+    nx = p.NXM
+    rng = np.random.default_rng(7 + pnodes)
+    a = rng.standard_normal(o.shape)
+    o.set("SCRATCH0", a)
+    o.call("Communication_update_ghost_nodes_flow_variable", pvof.F["SCRATCH0"], ord(component), pnodes)
+    b = o.get("SCRATCH0")
+    i = np.arange(nx + 7) - 3
+    ulike = component in "udep"
+    lo = 1 if ulike else 0                        # first owned column
+    own = (i >= lo) & (i < lo + nx)
+    filled_lo = (i < lo) & (i >= lo - (pnodes + 1 if ulike else pnodes))
+    filled_hi = (i >= lo + nx) & (i < lo + nx + pnodes)
+    rows = slice(3 - pnodes, 3 + p.NYM + 1 + pnodes)
+    src_col = lo + np.mod(i - lo, nx) + 3          # canonical column of the owned image
+    expect = a.copy()
+    for c in np.nonzero(filled_lo | filled_hi)[0]:
+        expect[rows, c] = a[rows, src_col[c]]
+    assert np.array_equal(b, expect)
+    assert np.array_equal(b[:, own.nonzero()[0]], a[:, own.nonzero()[0]])
+
+
+def test_vof_diffuse_init_clamps_and_completes_the_phases(olib):
+    """VOF_DIFFUSE_init: C_L in [0, 1] everywhere, equal to F where F is
+    already in [0, 1] inside the domain, C_L + C_G = 1 (no solid) over the
+    L range, zero-gradient wall ghosts and periodic ghost columns."""
+    o = make(olib)
+    p = o.params
+    rng = np.random.default_rng(3)
+    F = rng.uniform(-0.3, 1.3, size=o.shape)
+    o.set("F", F)
+    o.call("VOF_DIFFUSE_init")
+    CL, CG = o.get("C_L"), o.get("C_G")
+    R = regions(p)["cell"]
+    assert CL.min() >= 0.0 and CL.max() <= 1.0
+    inside = (F[R] >= 0) & (F[R] <= 1)
+    assert np.array_equal(CL[R][inside], F[R][inside])
+    assert np.array_equal(CL[R][F[R] < 0], np.zeros((F[R] < 0).sum()))
+    assert np.array_equal(CL[R][F[R] > 1], np.ones((F[R] > 1).sum()))
+    frame = regions(p)["frame"]
+    assert np.allclose(CL[frame] + CG[frame], 1.0, rtol=0, atol=1e-15)
+    nx, ny = p.NXM, p.NYM
+    for g in (1, 2, 3):                             # walls: zero gradient
+        assert np.array_equal(CL[3 - g, 3:3 + nx], CL[3, 3:3 + nx])
+    assert np.array_equal(CL[3 + ny - 1 + 1, 3:3 + nx], CL[3 + ny - 1, 3:3 + nx])
+    assert np.array_equal(CL[3:3 + ny, 3 + nx], CL[3:3 + ny, 3])     # periodic image column
+    assert np.array_equal(CL[3:3 + ny, 2], CL[3:3 + ny, 3 + nx - 1])
