@@ -51,7 +51,8 @@ TIMERS = ["CH_APPLY", "CH_UPDATE", "MOM_APPLY", "MOM_UPDATE", "P_APPLY", "P_UPDA
           "FFT_ROWS_FWD", "FFT_COLS", "FFT_ROWS_INV", "CH_RHS", "MOM_TERMS", "P_TRANSPOSE",
           "MG_PRE0", "MG_COARSE", "MG_POST0"]
 # algorithmic HBM bytes per cell and launch (each distinct array read or
-# written once; stencil neighbours come from cache)
+# written once; stencil neighbours come from cache); MOM_* per velocity
+# component (a launch carries u and v together)
 BYTES_PER_CELL = {
     "CH_APPLY": 32, "CH_UPDATE": 48, "MOM_APPLY": 56, "MOM_UPDATE": 56,
     "P_APPLY": 24, "P_UPDATE": 48, "P_DUPDATE": 24,
@@ -71,7 +72,7 @@ BYTES_PER_CELL_MG = dict(BYTES_PER_CELL, P_APPLY=48, P_UPDATE=48, MG_PRE0=74, MG
 BYTES_PER_CELL_MG32 = dict(BYTES_PER_CELL_MG, MG_PRE0=61, MG_POST0=29)
 # per pressure iteration, all of its kernels (MG: level 0 + 1/3 of the level-0
 # smoothing bytes, without the CG step, for the coarser levels)
-BYTES_PER_ITER = {"fft": 176, "mg": 189, "mg32": 155, "jacobi": 112}
+BYTES_PER_ITER = {"fft": 144, "mg": 189, "mg32": 155, "jacobi": 112}
 # HBM bytes per launch measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE on
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
@@ -222,6 +223,15 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     if not any(sum(st["n_ch"]) for st in stats):
         # spectral CH start: the CH CG launches only take the stop test (no cell work)
         bpc = {k: v for k, v in bpc.items() if k not in ("CH_APPLY", "CH_UPDATE")}
+    # the momentum CG runs u and v in the same launches: a launch moves the
+    # per-component bytes once per component still iterating, on average
+    # (n_u + n_v) / max(n_u, n_v) components (one after the other has stopped)
+    nu = sum(sum(st["n_u"]) for st in stats)
+    nv = sum(sum(st["n_v"]) for st in stats)
+    mlaunch = sum(max(a, b) for st in stats for a, b in zip(st["n_u"], st["n_v"]))
+    if mlaunch:
+        comps = (nu + nv) / mlaunch
+        bpc = dict(bpc, MOM_APPLY=bpc["MOM_APPLY"] * comps, MOM_UPDATE=bpc["MOM_UPDATE"] * comps)
     kernels = {}
     for k, name in enumerate(TIMERS):
         if cnt[k]:

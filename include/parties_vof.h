@@ -160,6 +160,8 @@ int pv_init_from_F(pv_bag *bag, const double *F_host);
 
 /* Time and stage state normally held in Parameters (time, dt, dt_old, which_stage). */
 int pv_set_time_state(pv_bag *bag, double time, double dt, double dt_old, int which_stage);
+/* ntime of Parameters (the step counter a restart file carries, IO/Resume.c:90) */
+int pv_set_step_count(pv_bag *bag, int ntime);
 int pv_get_stats(pv_bag *bag, pv_stats *out);
 
 /* ---- time integration (Include/Temporal.h:4-5) ------------------------- */

@@ -1595,7 +1595,8 @@ int pvo_Temporal_int_rk3(pvo_state *s, int nsteps)
 
 /* VOF_DIFFUSE.c:524-568 without IBM: C_L = clamp01(F) over the L range,
  * then boundary values / phase cache / clamp / boundary values / phase
- * cache twice, and the boundary values of C_G. */
+ * cache twice, and the boundary values of C_G; F ends as the mirror of C_L
+ * that the phase cache keeps (:108-126). */
 void pvo_VOF_DIFFUSE_init(pvo_state *s)
 {
     double *C = s->f[PV_F_C_L], *F = s->f[PV_F_F];
@@ -1609,6 +1610,7 @@ void pvo_VOF_DIFFUSE_init(pvo_state *s)
     bc_ch(s, C);
     pvo_VOF_DIFFUSE_update_phase_cache(s);
     bc_ch(s, s->f[PV_F_C_G]);
+    memcpy(F, C, s->nel * sizeof(double));          /* the phase cache's F mirror (:108-126) */
 }
 
 /* Cart3d.c:1094-1130: boundary values of F, VOF_DIFFUSE_init, density and

@@ -156,4 +156,99 @@ __device__ __forceinline__ void sk_fft(double *sr, double *si, int logM, const d
     if (rem == 2) sk_pass<2, INV>(sr, si, logM, logNs, tw, lane, T);
     else if (rem == 1) sk_pass<1, INV>(sr, si, logM, logNs, tw, lane, T);
 }
+
+// ---------------------------------------------------------------------------
+// Mixed radix: lengths M = 2^a 3^b with 8 | M.  The same Stockham passes with
+// general (not power-of-two) sub-transform sizes Ns: radix 8 while three
+// factors of two remain, then a radix-4 or radix-2 pass, then one radix-3
+// pass per factor of three.  T = M / 8 threads per transform; a radix-R pass
+// has M / R butterflies, so a thread takes up to 8 / R of them (3 for R = 3).
+// Threads whose transform index is out of range (act = false) only take part
+// in the barriers.
+// ---------------------------------------------------------------------------
+
+// 3-point DFT of v[0..2] in place: X_1,2 = a - (b + c) / 2 -+ i (sqrt 3 / 2)(b - c)
+// (forward: exp(-2 pi i / 3); the inverse swaps X_1 and X_2)
+template <bool INV>
+__host__ __device__ __forceinline__ void sk_dft3(double *vr, double *vi)
+{
+    const double h = 0.86602540378443864676;          // sqrt(3) / 2
+    const double sr_ = vr[1] + vr[2], si_ = vi[1] + vi[2];
+    const double tr = vr[0] - 0.5 * sr_, ti = vi[0] - 0.5 * si_;
+    const double dr = h * (vr[1] - vr[2]), di = h * (vi[1] - vi[2]);
+    vr[0] += sr_; vi[0] += si_;
+    if (INV) { vr[1] = tr - di; vi[1] = ti + dr; vr[2] = tr + di; vi[2] = ti - dr; }
+    else     { vr[1] = tr + di; vi[1] = ti - dr; vr[2] = tr - di; vi[2] = ti + dr; }
+}
+
+template <int R, bool INV>
+__host__ __device__ __forceinline__ void sg_load(const double *sr, const double *si, int bb, int M, int Ns,
+                                                 const double *tw, double *vr, double *vi)
+{
+    const int stride = M / R, k = bb % Ns, step = (M / (Ns * R)) * k;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        vr[r] = sr[bb + r * stride];
+        vi[r] = si[bb + r * stride];
+    }
+    if (Ns > 1) {
+#pragma unroll
+        for (int r = 1; r < R; r++) {
+            const int t = r * step;
+            const double c = tw[2 * t], s = INV ? -tw[2 * t + 1] : tw[2 * t + 1];
+            const double xr = vr[r], xi = vi[r];
+            vr[r] = xr * c - xi * s;
+            vi[r] = xr * s + xi * c;
+        }
+    }
+    if (R == 8) sk_dft8<INV>(vr, vi);
+    else if (R == 4) sk_dft4<INV>(vr, vi, 0, 1);
+    else if (R == 3) sk_dft3<INV>(vr, vi);
+    else sk_dft2<INV>(vr, vi, 0, 1);
+}
+
+template <int R>
+__host__ __device__ __forceinline__ void sg_store(double *sr, double *si, int bb, int Ns, const double *vr,
+                                                  const double *vi)
+{
+    const int k = bb % Ns, base = (bb / Ns) * Ns * R + k;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        sr[base + r * Ns] = vr[r];
+        si[base + r * Ns] = vi[r];
+    }
+}
+
+template <int R, bool INV>
+__device__ __forceinline__ void sg_pass(double *sr, double *si, int M, int Ns, const double *tw, int lane, int T,
+                                        bool act)
+{
+    constexpr int U = (R == 3) ? 3 : 8 / R;
+    const int nbf = M / R;
+    double vr[U * R], vi[U * R];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int bb = lane + u * T;
+        if (act && bb < nbf) sg_load<R, INV>(sr, si, bb, M, Ns, tw, vr + u * R, vi + u * R);
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+        const int bb = lane + u * T;
+        if (act && bb < nbf) sg_store<R>(sr, si, bb, Ns, vr + u * R, vi + u * R);
+    }
+    __syncthreads();
+}
+
+// Whole transform of length M = 2^a 3^b (8 | M)
+template <bool INV>
+__device__ __forceinline__ void sg_fft(double *sr, double *si, int M, const double *tw, int lane, int T, bool act)
+{
+    int Ns = 1, two = M;
+    while (two % 3 == 0) two /= 3;                   // the power-of-two part
+    for (; two % 8 == 0; two /= 8, Ns *= 8) sg_pass<8, INV>(sr, si, M, Ns, tw, lane, T, act);
+    if (two == 4) { sg_pass<4, INV>(sr, si, M, Ns, tw, lane, T, act); Ns *= 4; }
+    else if (two == 2) { sg_pass<2, INV>(sr, si, M, Ns, tw, lane, T, act); Ns *= 2; }
+    for (; Ns < M; Ns *= 3) sg_pass<3, INV>(sr, si, M, Ns, tw, lane, T, act);
+}
 #endif

@@ -63,15 +63,22 @@ typedef struct pvk_red {
 
 /* FFT-x / DCT-y constant-coefficient solve tables (csrc/pv_fft.hip). */
 typedef struct pvk_fft {
-    int ok;                   /* 0: sizes not supported (need powers of two) */
-    int logmx, logmy;         /* log2(NXM/2), log2(NYM/2) */
-    int rx, ry;               /* transforms per block: row and column kernels */
-    const double *twx, *twy;  /* (cos, sin)(2 pi t / M), t < M/2, interleaved */
+    int ok;                   /* 0: sizes not supported (NXM must be 2^a 3^b, 16 | NXM) */
+    int mx;                   /* NXM / 2: length of the complex row transforms */
+    int pow2x;                /* 1: mx is a power of two (shift-indexed passes) */
+    int dct_ok;               /* 1: NYM is a power of two (the y transforms of the CH start) */
+    int logmx, logmy;         /* log2(NXM/2), log2(NYM/2) where they are powers of two */
     const double *rtx, *rty;  /* (cos, sin)(2 pi k / N), k <= N/2 */
     const double *dcty;       /* (cos, sin)(pi k / (2 NYM)), k < NYM */
     const double *lamx, *lamy;/* 1-D eigenvalue magnitudes */
     const double *wx, *wy;    /* (cos, -sin)(2 pi t / M), t < M: exp(-2 pi i t / M) */
     double *spec;             /* NXM * NYM spectral scratch, row-major */
+    /* tridiagonal solves along y on the columns of spec (csrc/pv_pcg.hip,
+     * k_tri_cols): row coefficients as_j, an_j (psolve_fft.c:467-591), the
+     * modified wavenumber of each halfcomplex column, and the launch shape
+     * (tri_kw columns x tri_s segments of tri_l rows per block) */
+    const double *tri_as, *tri_an, *tri_mw;
+    int tri_kw, tri_lkw, tri_s, tri_l;
 } pvk_fft;
 
 /* Work set of the fused FFT-preconditioned PCG (csrc/pv_pcg.hip). */
@@ -92,6 +99,9 @@ typedef struct pvk_pcg {
     int ndad, nrr, nrz, nmean;        /* partial counts */
     int flex;                         /* 1: flexible CG, beta = -<z_{n+1}, A d_n> / <d_n, A d_n> */
     int band, rrow, rdct;             /* rows per block: bands, FFT rows, DCT rows */
+    int keep_pin;                     /* 1: the y solve keeps the reference's pin of mode (0,0) at j = 0
+                                         (the standalone solve); 0: it also removes the mean of that
+                                         column (the PCG preconditioner's pseudo-inverse) */
     double chb;                       /* > 0: the FFT stages solve (a_dt + chb L^2) z = r (Cahn-Hilliard,
                                          a_dt of RK stage chstage) instead of L z = r, and run even
                                          after the pressure solve's stop flag */
@@ -220,19 +230,6 @@ void pvk_project(const pvk_grid *g, int stage, const pvk_scal *sc, const double 
 void pvk_divergence(const pvk_grid *g, const double *u, const double *v, const pvk_red *rd,
                     hipStream_t s);
 void pvk_div_finalize(const pvk_grid *g, pvk_scal *sc, const pvk_red *rd, hipStream_t s);
-void pvk_pin_row0(const pvk_grid *g, double *x, hipStream_t s);
-
-/* ---- FFT-x / DCT-y solve (csrc/pv_fft.hip) ---- */
-void pvk_fft_solve(const pvk_grid *g, const pvk_fft *F, const double *in, double *out, int mean_free,
-                   hipStream_t s);
-void pvk_fft_solve_sc(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, const double *in,
-                      double *out, int mean_free, hipStream_t s);
-void pvk_fft_solve_dot(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, const double *in,
-                       double *out, int mean_free, double *part, hipStream_t s);
-int  pvk_fft_rows_blocks(const pvk_grid *g, const pvk_fft *F);
-/* stage 0: rows forward, 1: columns, 2: rows inverse (+ <in, out> partials) */
-void pvk_fft_stage(const pvk_grid *g, const pvk_fft *F, const pvk_scal *sc, int stage, const double *in,
-                   double *out, int mean_free, double *part, hipStream_t s);
 
 /* fused FFT-PCG (csrc/pv_pcg.hip): start (mean removal, r0, forward rows)
  * and the six per-iteration stages; n = -1 preconditions the start residual */
@@ -240,6 +237,8 @@ void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_
                    hipStream_t s);
 void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
                    double tol, int maxit, hipStream_t s);
+/* clear the solve's stop flag (the stages skip their work once it is set) */
+void pvk_pcg_clear_done(pvk_scal *sc, hipStream_t s);
 
 /* sc->red[slot] = sum of the <r, z> partials rz[rzs] */
 void pvk_pcg_gamma(pvk_scal *sc, const pvk_pcg *w, int rzs, int slot, hipStream_t s);

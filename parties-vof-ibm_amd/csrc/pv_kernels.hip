@@ -1079,27 +1079,3 @@ extern "C" void pvk_dot_sc(const pvk_grid *g, const pvk_scal *sc, const double *
     hipLaunchKernelGGL(k_dot_sc, grid2d(g), dim3(BX, BY), 0, s, *g, sc, a, b, rd->part, rd->nblocks);
 }
 
-// Reference pin of the (0,0) mode at j = 0 (psolve_fft.c:1716-1722): for a
-// zero-mean right-hand side the pinned solution is the mean-free one shifted
-// so that the x-mean of row 0 vanishes.
-__global__ void k_pin_row0(pvk_grid g, double *x)
-{
-    __shared__ double sh[256];
-    const int P = g.P;
-    double s = 0.0;
-    for (int i = threadIdx.x; i < g.NXM; i += 256) s += AT(x, i, 0);
-    sh[threadIdx.x] = s;
-    __syncthreads();
-    for (int w = 128; w > 0; w >>= 1) {
-        if (threadIdx.x < w) sh[threadIdx.x] += sh[threadIdx.x + w];
-        __syncthreads();
-    }
-    double mean = sh[0] / (double)g.NXM;
-    __syncthreads();
-    for (int j = 0; j < g.NYM; j++)
-        for (int i = threadIdx.x; i < g.NXM; i += 256) AT(x, i, j) -= mean;
-}
-extern "C" void pvk_pin_row0(const pvk_grid *g, double *x, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pin_row0, dim3(1), dim3(256), 0, s, *g, x);
-}

@@ -36,7 +36,7 @@ __device__ __forceinline__ int sk_logT(int logM) { return logM >= 3 ? logM - 3 :
 // X_k, k = 0..M, of a real length-2M sequence from Z = FFT_M(x_{2m} + i x_{2m+1})
 __device__ __forceinline__ cplx2 rpost(const double *sr, const double *si, int base, int M, int k, const double *rt)
 {
-    int ka = (k == M) ? 0 : k, kb = (M - k) & (M - 1);
+    int ka = (k == M) ? 0 : k, kb = (k == 0 || k == M) ? 0 : M - k;
     double zr = sr[base + ka], zi = si[base + ka];
     double cr = sr[base + kb], ci = -si[base + kb];
     double er = 0.5 * (zr + cr), ei = 0.5 * (zi + ci);
@@ -68,6 +68,27 @@ __device__ __forceinline__ cplx2 hcget(const double *row, int N, int k)
     X.x = row[k];
     X.y = (k == 0 || k == N / 2) ? 0.0 : row[N - k];
     return X;
+}
+
+// row q and entry m of item t of a block's rows of M values
+__device__ __forceinline__ void row_split(int t, int M, int logM, bool p2, int &q, int &m)
+{
+    if (p2) { q = t >> logM; m = t & (M - 1); }
+    else { q = t / M; m = t - q * M; }
+}
+
+// the block's transforms of its R rows: power-of-two lengths by the
+// shift-indexed passes, 2^a 3^b lengths by the mixed-radix ones
+template <bool INV>
+__device__ __forceinline__ void rows_fft(double *sr, double *si, int M, int logM, bool p2, int R, const double *tw)
+{
+    if (p2) {
+        const int lT = sk_logT(logM), q = threadIdx.x >> lT;
+        sk_fft<INV>(sr + q * M, si + q * M, logM, tw, threadIdx.x & ((1 << lT) - 1), 1 << lT);
+    } else {
+        const int T = M >> 3, q = threadIdx.x / T;
+        sg_fft<INV>(sr + q * M, si + q * M, M, tw, threadIdx.x - q * T, T, q < R);
+    }
 }
 
 // Every FFT kernel handles R = 2048 / M rows of M complex points per block
@@ -162,12 +183,14 @@ __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk
 // K2: alpha; phi, r update; <r,r>; forward real FFT of the r rows -> spec
 // (also the start kernel when n < 0: r = rhs - mean, phi = 0)
 // ---------------------------------------------------------------------------
+template <bool P2>
 __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, int n)
 {
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
     if (w.chb == 0.0 && sc->p_done) return;
-    const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
+    const int P = g.P, N = g.NXM, logM = F.logmx, M = F.mx, R = w.rrow, RM = R * M;
+    constexpr bool p2 = P2;     // power-of-two rows: shift-indexed passes
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);
     double alpha = 0.0, mean = 0.0;
@@ -180,7 +203,10 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
     double xa[IT], xb[IT], ea[IT], eb[IT], ra[IT], rb[IT];
 #pragma unroll
     for (int u = 0; u < IT; u++) {
-        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
+        const int t = threadIdx.x + u * NTH;
+        int q, m;
+        row_split(t, M, logM, p2, q, m);
+        const int j = j0 + q, i = 2 * m;
         xa[u] = xb[u] = ea[u] = eb[u] = ra[u] = rb[u] = 0.0;
         if (t < RM && j < g.NYM) {
             if (n >= 0) {
@@ -204,7 +230,10 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < IT; u++) {
-        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q, i = 2 * m;
+        const int t = threadIdx.x + u * NTH;
+        int q, m;
+        row_split(t, M, logM, p2, q, m);
+        const int j = j0 + q, i = 2 * m;
         if (t < RM && j < g.NYM && n != -3) {
             AT(phi, i, j) = (n >= 0) ? xa[u] + alpha * ea[u] : (n == -2 ? xa[u] : 0.0);
             AT(phi, i + 1, j) = (n >= 0) ? xb[u] + alpha * eb[u] : (n == -2 ? xb[u] : 0.0);
@@ -215,15 +244,15 @@ __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, p
         if (t < RM) { sr[t] = ra[u]; si[t] = rb[u]; }
     }
     block_store_partial(acc, w.rr, blockIdx.x, sh);
-    {
-        const int lT = sk_logT(logM), q = threadIdx.x >> lT;
-        sk_fft<false>(sr + q * M, si + q * M, logM, tw, threadIdx.x & ((1 << lT) - 1), 1 << lT);
-    }
+    rows_fft<false>(sr, si, M, logM, p2, R, tw);
     const double scale = 1.0 / (double)N;
     // X_k, k < M (and X_M for each row below): halfcomplex row of spec
 #pragma unroll
     for (int u = 0; u < IT; u++) {
-        const int t = threadIdx.x + u * NTH, q = t >> logM, k = t & (M - 1), j = j0 + q;
+        const int t = threadIdx.x + u * NTH;
+        int q, k;
+        row_split(t, M, logM, p2, q, k);
+        const int j = j0 + q;
         if (t < RM && j < g.NYM) {
             cplx2 X = rpost(sr, si, q * M, M, k, F.rtx);
             double *row = F.spec + (size_t)j * N;
@@ -396,14 +425,221 @@ __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_sca
 }
 
 // ---------------------------------------------------------------------------
+// K4': the y solve of the FFT preconditioner as the reference does it: one
+// tridiagonal system per halfcomplex column i of spec (psolve_fft.c:1536-1802,
+// tridiag :141-163), a_j = as_j, c_j = an_j, b_j = (-as_j - an_j) - mw_i,
+// with row 0 of column 0 replaced by x_0 = 0 (the pin of mode (0, 0)), solved
+// in place on spec[j][i] -- no transposes.
+//
+// Batched partitioned Thomas mapped to the workgroup: lane kk of a row group
+// owns column i0 + kk (consecutive lanes read consecutive doubles of a row),
+// and the tri_s segments of tri_l rows of a column are owned by tri_s
+// threads.  The last row of every segment but the last is a separator.  Each
+// thread eliminates its segment's interior rows (held in registers) with the
+// couplings to the separators above and below removed, and keeps the
+// forward-swept values, the Thomas multipliers and the forward sweep of the
+// first unit vector; the separators' Schur complement is tridiagonal (one
+// equation per separator: its own row with the neighbouring interior rows
+// written in terms of the separators), solved by parallel cyclic reduction
+// in LDS; then each thread back-substitutes its interior rows with the
+// separator values in place.  With mean removal (the PCG's pseudo-inverse)
+// column 0 is shifted to zero mean afterwards.  In the PCG this kernel also
+// runs the stop test of iteration n (every block, from the <r, r> partials).
+// ---------------------------------------------------------------------------
+#define TRI_LMAX 16
+
+__device__ __forceinline__ double tri_sum(const double *part, int n, double *sh, int nt)
+{
+    const int t = threadIdx.x;
+    double s = 0.0;
+    for (int b = t; b < n; b += nt) s += part[b];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = 256; w > 0; w >>= 1) {
+        if (t < w && t + w < nt) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    double r = sh[0];
+    __syncthreads();
+    return r;
+}
+
+__global__ void __launch_bounds__(512) k_tri_cols(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, int n,
+                                                   double tol, int maxit)
+{
+    extern __shared__ double lds[];
+    if (sc->p_done) return;
+    const int nt = F.tri_kw * F.tri_s;
+    if (n != -3) {                          // stop test of iteration n (as the transpose stage does)
+        double rr = tri_sum(w.rr, w.nrr, lds, nt);
+        double bn = (n == -1) ? ((sqrt(rr) == 0.0) ? 1.0 : sqrt(rr)) : sc->p_bnorm;
+        double rel = sqrt(rr) / bn;
+        bool done = (rel <= tol) || (n + 1 >= maxit && n >= 0);
+        if (blockIdx.x == 0 && threadIdx.x == 0) {
+            if (n == -1) sc->p_bnorm = bn;
+            sc->p_res = rel;
+            sc->p_iter = (n < 0) ? 0 : n + 1;
+            if (done) sc->p_done = (rel <= tol) ? 1 : 3;
+        }
+        if (done) return;
+    }
+    const int NX = g.NXM, NY = g.NYM, KW = F.tri_kw, S = F.tri_s, L = F.tri_l;
+    const int kk = threadIdx.x & (KW - 1), s = threadIdx.x >> F.tri_lkw;
+    const int col = blockIdx.x * KW + kk;
+    const bool act = col < NX;
+    const int j0 = s * L, has_sep = s < S - 1;
+    const int nrow = has_sep ? L : NY - (S - 1) * L, nint = has_sep ? nrow - 1 : nrow;
+    const double mw = act ? F.tri_mw[col] : 0.0;
+    double *spec = F.spec;
+    auto A = [&](int j) { return (col == 0 && j == 0) ? 0.0 : F.tri_as[j]; };
+    auto Cc = [&](int j) { return (col == 0 && j == 0) ? 0.0 : F.tri_an[j]; };
+    auto B = [&](int j) { return (col == 0 && j == 0) ? 1.0 : (-F.tri_as[j] - F.tri_an[j]) - 0.0 - mw; };
+    double y[TRI_LMAX], gg[TRI_LMAX];
+    double dsep = 0.0;
+#pragma unroll
+    for (int l = 0; l < TRI_LMAX; l++) {
+        double v = 0.0;
+        if (l < nrow && act) v = spec[(size_t)(j0 + l) * NX + col];
+        if (col == 0 && j0 + l == 0) v = 0.0;
+        y[l] = v;
+        if (has_sep && l == nrow - 1) dsep = v;
+    }
+    // forward sweep over the interior rows; e1 = forward sweep of the first
+    // unit vector, kept only at the last interior row (Vl) and recovered
+    // backwards from the pivots: e1_{l-1} = -e1_l p_l / a_l (a_l != 0 on
+    // every row but j = 0, which is never l >= 1 of a segment)
+    double inv = 1.0 / B(j0), e1 = inv, Yl = 0.0, Vl = 0.0;
+    y[0] *= inv;
+    gg[0] = 0.0;
+    if (nint == 1) { Yl = y[0]; Vl = e1; }
+#pragma unroll
+    for (int l = 1; l < TRI_LMAX; l++) {
+        gg[l] = 0.0;
+        if (l < nint) {
+            const int j = j0 + l;
+            const double a = A(j);
+            gg[l] = Cc(j - 1) * inv;
+            inv = 1.0 / (B(j) - a * gg[l]);
+            y[l] = (y[l] - a * y[l - 1]) * inv;
+            e1 = -a * e1 * inv;
+            if (l == nint - 1) { Yl = y[l]; Vl = e1; }
+        }
+    }
+    const double Wl = inv;                  // 1 / last pivot: forward sweep of the last unit vector
+    // first-row values of the local solutions for d, the first and the last unit vector
+    double Yf = 0.0, Vf = 0.0, Wf = 0.0, ev = Vl;
+#pragma unroll
+    for (int l = TRI_LMAX - 1; l >= 0; l--) {
+        if (l == nint - 1) { Yf = y[l]; Vf = ev; Wf = Wl; }
+        else if (l < nint - 1) {
+            Yf = y[l] - gg[l + 1] * Yf;
+            Vf = ev - gg[l + 1] * Vf;
+            Wf = -gg[l + 1] * Wf;
+        }
+        if (l >= 1 && l < nint) {
+            const int j = j0 + l;
+            const double a = A(j);
+            ev = -ev * (B(j) - a * gg[l]) / a;
+        }
+    }
+    const double atop = (s > 0) ? A(j0) : 0.0;
+    const double cbot = has_sep ? Cc(j0 + nint - 1) : 0.0;
+    // separators' Schur complement: row s couples z_{s-1}, z_s, z_{s+1}
+    const int nsep = S - 1, SK = S * KW, me = s * KW + kk;
+    double *RA0 = lds, *RB0 = lds + SK, *RC0 = lds + 2 * SK, *RD0 = lds + 3 * SK;
+    double *RA1 = lds + 4 * SK, *RB1 = lds + 5 * SK, *RC1 = lds + 6 * SK, *RD1 = lds + 7 * SK;
+    RB1[me] = Yf; RC1[me] = atop * Vf; RD1[me] = cbot * Wf;      // for the separator above this segment
+    __syncthreads();
+    if (s < nsep) {
+        const int js = j0 + nint;
+        const double as_ = A(js), bs_ = B(js), cs_ = Cc(js);
+        const int nb = me + KW;
+        RA0[me] = -as_ * atop * Vl;
+        RB0[me] = bs_ - as_ * cbot * Wl - cs_ * RC1[nb];
+        RC0[me] = -cs_ * RD1[nb];
+        RD0[me] = dsep - as_ * Yl - cs_ * RB1[nb];
+    } else {
+        RA0[me] = 0.0; RB0[me] = 1.0; RC0[me] = 0.0; RD0[me] = 0.0;
+    }
+    __syncthreads();
+    double *ca = RA0, *cb = RB0, *cc = RC0, *cd = RD0, *na = RA1, *nb_ = RB1, *nc = RC1, *nd = RD1;
+    for (int h = 1; h < nsep; h <<= 1) {
+        if (s < nsep) {
+            double a2 = 0.0, b2 = cb[me], c2 = 0.0, d2 = cd[me];
+            if (s - h >= 0) {
+                const int q = me - h * KW;
+                const double al = -ca[me] / cb[q];
+                a2 = al * ca[q]; b2 += al * cc[q]; d2 += al * cd[q];
+            }
+            if (s + h < nsep) {
+                const int q = me + h * KW;
+                const double ga = -cc[me] / cb[q];
+                c2 = ga * cc[q]; b2 += ga * ca[q]; d2 += ga * cd[q];
+            }
+            na[me] = a2; nb_[me] = b2; nc[me] = c2; nd[me] = d2;
+        }
+        __syncthreads();
+        double *t;
+        t = ca; ca = na; na = t;
+        t = cb; cb = nb_; nb_ = t;
+        t = cc; cc = nc; nc = t;
+        t = cd; cd = nd; nd = t;
+    }
+    double *z = na;                          // the other buffer is free now
+    if (s < nsep) z[me] = cd[me] / cb[me];
+    __syncthreads();
+    const double zprev = (s > 0) ? z[me - KW] : 0.0, zsep = has_sep ? z[me] : 0.0;
+    // interior rows from the separators (back substitution in place)
+    double x = 0.0, colsum = 0.0;
+    ev = Vl;
+#pragma unroll
+    for (int l = TRI_LMAX - 1; l >= 0; l--) {
+        if (l < nint) {
+            double t = y[l] - atop * zprev * ev;
+            if (l == nint - 1) { t -= cbot * zsep * Wl; x = t; }
+            else x = t - gg[l + 1] * x;
+            y[l] = x;
+            colsum += x;
+            if (l >= 1) {
+                const int j = j0 + l;
+                const double a = A(j);
+                ev = -ev * (B(j) - a * gg[l]) / a;
+            }
+        }
+    }
+    if (has_sep) colsum += zsep;
+    double shift = 0.0;
+    if (!w.keep_pin && blockIdx.x == 0) {     // block-uniform: column 0 lives in block 0
+        __syncthreads();
+        if (kk == 0) na[s] = colsum;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double m = 0.0;
+            for (int q = 0; q < S; q++) m += na[q];
+            nb_[0] = m / (double)NY;
+        }
+        __syncthreads();
+        if (col == 0) shift = nb_[0];
+    }
+    if (act) {
+#pragma unroll
+        for (int l = 0; l < TRI_LMAX; l++)
+            if (l < nint) spec[(size_t)(j0 + l) * NX + col] = y[l] - shift;
+        if (has_sep) spec[(size_t)(j0 + nint) * NX + col] = zsep - shift;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // K6: z = inverse real FFT of spec rows; partials <r, z> into rz[(n+1)&1]
 // ---------------------------------------------------------------------------
+template <bool P2>
 __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, int n)
 {
     extern __shared__ double lds[];
     __shared__ double sh[NTH];
     if (w.chb == 0.0 && sc->p_done) return;
-    const int P = g.P, N = g.NXM, logM = F.logmx, M = 1 << logM, R = w.rrow, RM = R * M;
+    const int P = g.P, N = g.NXM, logM = F.logmx, M = F.mx, R = w.rrow, RM = R * M;
+    constexpr bool p2 = P2;     // power-of-two rows: shift-indexed passes
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);
     const double *__restrict__ r = w.r;
@@ -413,7 +649,10 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
         cplx2 Y[IT];
 #pragma unroll
         for (int u = 0; u < IT; u++) {
-            const int t = threadIdx.x + u * NTH, q = t >> logM, k = t & (M - 1), j = j0 + q;
+            const int t = threadIdx.x + u * NTH;
+        int q, k;
+        row_split(t, M, logM, p2, q, k);
+        const int j = j0 + q;
             Y[u].x = 0.0; Y[u].y = 0.0;
             if (t < RM && j < g.NYM) {
                 const double *row = F.spec + (size_t)j * N;
@@ -427,21 +666,24 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
         }
     }
     __syncthreads();
-    {
-        const int lT = sk_logT(logM), q = threadIdx.x >> lT;
-        sk_fft<true>(sr + q * M, si + q * M, logM, tw, threadIdx.x & ((1 << lT) - 1), 1 << lT);
-    }
+    rows_fft<true>(sr, si, M, logM, p2, R, tw);
     double ra[IT], rb[IT];
 #pragma unroll
     for (int u = 0; u < IT; u++) {
-        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q;
+        const int t = threadIdx.x + u * NTH;
+        int q, m;
+        row_split(t, M, logM, p2, q, m);
+        const int j = j0 + q;
         ra[u] = rb[u] = 0.0;
         if (t < RM && j < g.NYM) { ra[u] = AT(r, 2 * m, j); rb[u] = AT(r, 2 * m + 1, j); }
     }
     double acc = 0.0;
 #pragma unroll
     for (int u = 0; u < IT; u++) {
-        const int t = threadIdx.x + u * NTH, q = t >> logM, m = t & (M - 1), j = j0 + q;
+        const int t = threadIdx.x + u * NTH;
+        int q, m;
+        row_split(t, M, logM, p2, q, m);
+        const int j = j0 + q;
         if (t < RM && j < g.NYM) {
             const double a = sr[t], b = si[t];
             AT(z, 2 * m, j) = a;
@@ -450,6 +692,12 @@ __global__ void __launch_bounds__(NTH) k_ifft_rows_dot(pvk_grid g, pvk_fft F, pv
         }
     }
     block_store_partial(acc, w.rz[(n < 0) ? 0 : ((n + 1) & 1)], blockIdx.x, sh);
+}
+
+__global__ void k_pcg_clear_done(pvk_scal *sc) { sc->p_done = 0; }
+extern "C" void pvk_pcg_clear_done(pvk_scal *sc, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_clear_done, dim3(1), dim3(1), 0, s, sc);
 }
 
 // sc->red[slot] = sum of the <r, z> partials of slot rzs (the energy-norm
@@ -513,7 +761,8 @@ extern "C" void pvk_pcg_start(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
     int nband = (g->NYM + w->band - 1) / w->band, nrow = (g->NYM + w->rrow - 1) / w->rrow;
     hipLaunchKernelGGL(k_pcg_faces, dim3(g->NYM + 1), dim3(NTH), 0, s, *g, *w);
     hipLaunchKernelGGL(k_pcg_mean, dim3(nband), dim3(NTH), 0, s, *g, sc, *w);
-    hipLaunchKernelGGL(k_pcg_update_fwd, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, -1);
+    if (F->pow2x) hipLaunchKernelGGL(k_pcg_update_fwd<true>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, -1);
+    else hipLaunchKernelGGL(k_pcg_update_fwd<false>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, -1);
     (void)tol;
 }
 
@@ -532,8 +781,9 @@ extern "C" void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w,
 }
 
 // stage: 0 = K1 apply, 1 = K2 update + forward rows, 2 = K3 check + transpose,
-// 3 = K4 column DCTs, 4 = K5 transpose back, 5 = K6 inverse rows + <r, z>.
-// n = -1 runs stages 2..5 on the start residual.
+// 3 = K4 column DCTs, 4 = K5 transpose back, 5 = K6 inverse rows + <r, z>,
+// 6 = K4' check + tridiagonal y solves in place (replaces 2..4 for the
+// pressure).  n = -1 runs stages 2..5 (or 6, 5) on the start residual.
 extern "C" void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc, const pvk_pcg *w, int stage, int n,
                               double tol, int maxit, hipStream_t s)
 {
@@ -543,7 +793,10 @@ extern "C" void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
         hipLaunchKernelGGL(k_pcg_apply, dim3(g->NYM), dim3(NTH), 0, s, *g, sc, *w, n);
         break;
     case 1:
-        hipLaunchKernelGGL(k_pcg_update_fwd, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
+        if (F->pow2x)
+            hipLaunchKernelGGL(k_pcg_update_fwd<true>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
+        else
+            hipLaunchKernelGGL(k_pcg_update_fwd<false>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
         break;
     case 2:
         hipLaunchKernelGGL(k_transpose, dim3((g->NXM + 31) / 32, (g->NYM + 31) / 32), dim3(NTH), 0, s, sc, *w,
@@ -557,8 +810,16 @@ extern "C" void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
         hipLaunchKernelGGL(k_transpose, dim3((g->NYM + 31) / 32, (g->NXM + 31) / 32), dim3(NTH), 0, s, sc, *w,
                            (const double *)w->specT, F->spec, g->NXM, g->NYM, 0, n, tol, maxit);
         break;
+    case 6: {
+        const int nt = F->tri_kw * F->tri_s;
+        hipLaunchKernelGGL(k_tri_cols, dim3((g->NXM + F->tri_kw - 1) / F->tri_kw), dim3(nt),
+                           (size_t)8 * nt * sizeof(double), s, *g, *F, sc, *w, n, tol, maxit);
+    } break;
     default:
-        hipLaunchKernelGGL(k_ifft_rows_dot, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
+        if (F->pow2x)
+            hipLaunchKernelGGL(k_ifft_rows_dot<true>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
+        else
+            hipLaunchKernelGGL(k_ifft_rows_dot<false>, dim3(nrow), dim3(NTH), lds_rows(w->rrow, g->NXM), s, *g, *F, sc, *w, n);
         break;
     }
 }

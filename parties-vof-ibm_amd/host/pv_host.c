@@ -418,6 +418,13 @@ int pv_set_time_state(pv_bag *b, double time, double dt, double dt_old, int whic
     return pv_sync_scalars_to_device(b);
 }
 
+int pv_set_step_count(pv_bag *b, int ntime)
+{
+    if (!b || ntime < 0) return pv_set_error(PV_EINVAL, "bad step count");
+    b->ntime = ntime;
+    return PV_OK;
+}
+
 int pv_get_stats(pv_bag *b, pv_stats *out)
 {
     int rc;
@@ -445,37 +452,41 @@ static int ilog2_exact(int n)
     return l;
 }
 
+/* NXM = 2^a 3^b with 16 | NXM: the row transforms are complex transforms of
+ * length NXM / 2 with 8 | NXM / 2 (M / 8 threads each, mixed radix 8/4/2/3) */
+static int xlen_ok(int n)
+{
+    if (n < 16 || n % 16) return 0;
+    while (n % 2 == 0) n /= 2;
+    while (n % 3 == 0) n /= 3;
+    return n == 1;
+}
+
 int pv_fft_create(pv_bag *b)
 {
     const double PI_ = 3.14159265358979323846;
     int nx = b->NXM, ny = b->NYM, lx = ilog2_exact(nx), ly = ilog2_exact(ny), k;
-    size_t n_twx, n_twy, n_rtx, n_rty, n_dct, n_wx, n_wy, total;
+    size_t n_rtx, n_rty, n_dct, n_wx, n_wy, total;
     double *h, *d, *q;
     double idx2, idy2;
     memset(&b->fft, 0, sizeof b->fft);
     b->fftmem = NULL;
-    /* FFT-PCG unavailable for this size: one row of N/2 complex doubles must
-     * fit the 32 KiB per-block LDS budget of the row transforms */
-    if (lx < 3 || ly < 3 || nx > 4096 || ny > 4096) return PV_OK;
-    b->fft.logmx = lx - 1;
-    b->fft.logmy = ly - 1;
-    /* transforms per block: R * N/2 complex doubles of LDS, kept <= 32 KiB */
-    b->fft.rx = 4096 / nx; if (b->fft.rx < 1) b->fft.rx = 1; if (b->fft.rx > 16) b->fft.rx = 16;
-    b->fft.ry = 4096 / ny; if (b->fft.ry < 1) b->fft.ry = 1; if (b->fft.ry > 16) b->fft.ry = 16;
-    while (b->fft.ry > 1 && b->fft.ry * (ny / 4 + 1) > 8 * 512) b->fft.ry /= 2;
-    if (b->fft.ry * (ny / 4 + 1) > 8 * 512) return PV_OK;
-    n_twx = 2 * (size_t)(nx / 4 > 0 ? nx / 4 : 1); n_twy = 2 * (size_t)(ny / 4 > 0 ? ny / 4 : 1);
+    /* FFT-PCG unavailable for this size: one row of N/2 complex doubles and
+     * its twiddles must fit the LDS budget of the row transforms (N <= 4096) */
+    if (!xlen_ok(nx) || nx > 4096 || ny < 8 || ny > 8192) return PV_OK;
+    b->fft.mx = nx / 2;
+    b->fft.pow2x = (lx >= 4);
+    b->fft.logmx = (lx >= 4) ? lx - 1 : 0;
+    /* the y transforms of the Cahn-Hilliard spectral start: power-of-two NYM */
+    b->fft.dct_ok = (ly >= 3 && ny <= 4096);
+    b->fft.logmy = (ly >= 3) ? ly - 1 : 0;
     n_rtx = 2 * (size_t)(nx / 2 + 1); n_rty = 2 * (size_t)(ny / 2 + 1);
     n_dct = 2 * (size_t)ny;
     n_wx = 2 * (size_t)(nx / 2); n_wy = 2 * (size_t)(ny / 2);
-    total = n_twx + n_twy + n_rtx + n_rty + n_dct + nx + ny + n_wx + n_wy;
+    total = n_rtx + n_rty + n_dct + nx + ny + n_wx + n_wy + 2 * (size_t)ny + nx;
     h = (double *)calloc(total, sizeof(double));
     if (!h) return pv_set_error(PV_ENOMEM, "FFT table allocation failed");
     q = h;
-    for (k = 0; k < (int)(n_twx / 2); k++) { double a = 2.0 * PI_ * k / (nx / 2); q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
-    q += n_twx;
-    for (k = 0; k < (int)(n_twy / 2); k++) { double a = 2.0 * PI_ * k / (ny / 2); q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
-    q += n_twy;
     for (k = 0; k <= nx / 2; k++) { double a = 2.0 * PI_ * k / nx; q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
     q += n_rtx;
     for (k = 0; k <= ny / 2; k++) { double a = 2.0 * PI_ * k / ny; q[2 * k] = cos(a); q[2 * k + 1] = sin(a); }
@@ -497,6 +508,30 @@ int pv_fft_create(pv_bag *b)
     for (k = 0; k < nx / 2; k++) { double a = 2.0 * PI_ * k / (nx / 2); q[2 * k] = cos(a); q[2 * k + 1] = -sin(a); }
     q += n_wx;
     for (k = 0; k < ny / 2; k++) { double a = 2.0 * PI_ * k / (ny / 2); q[2 * k] = cos(a); q[2 * k + 1] = -sin(a); }
+    q += n_wy;
+    /* y tridiagonals of psolve_fft.c:467-591 (Neumann walls: as_0 = 0,
+     * an_{NYM-1} = 0) and the modified wavenumber of each halfcomplex column
+     * (:577-583, idx = idx_u[0]) */
+    for (k = 0; k < ny; k++) q[k] = (k > 0) ? b->hm.idy_v[k + 3] * b->hm.idy_c[k + 2] : 0.0;
+    q += ny;
+    for (k = 0; k < ny; k++) q[k] = (k < ny - 1) ? b->hm.idy_v[k + 3] * b->hm.idy_c[k + 3] : 0.0;
+    q += ny;
+    for (k = 0; k < nx; k++) {
+        int wave = (k < nx / 2) ? k : (nx - k);
+        q[k] = 2.0 * (1 - cos(2. * PI_ * wave / (nx))) * (b->hm.idx_u[3] * b->hm.idx_u[3]);
+    }
+    /* launch shape of the y solves: segments of <= 16 rows (the registers a
+     * thread keeps per row), as many columns per block as fit 512 threads */
+    {
+        int L = (ny < 16) ? ny : 16, S, kw = 64;
+        S = (ny + L - 1) / L;
+        L = (ny + S - 1) / S;
+        S = (ny + L - 1) / L;
+        while (kw > 1 && kw * S > 512) kw >>= 1;
+        if (kw * S > 512) { free(h); return PV_OK; }
+        b->fft.tri_l = L; b->fft.tri_s = S; b->fft.tri_kw = kw;
+        for (b->fft.tri_lkw = 0; (1 << b->fft.tri_lkw) < kw; b->fft.tri_lkw++) {}
+    }
     if (hipMalloc((void **)&d, sizeof(double) * (total + (size_t)nx * ny)) != hipSuccess) {
         free(h);
         return pv_set_error(PV_ENOMEM, "FFT table device allocation failed");
@@ -541,8 +576,6 @@ int pv_fft_create(pv_bag *b)
         b->pcg.rhs = b->f[PV_F_P_RHS]; b->pcg.rho = b->f[PV_F_RHO];
         b->pcg.bx = b->f[PV_FI_PBX]; b->pcg.by = b->f[PV_FI_PBY];
     }
-    b->fft.twx = q; q += n_twx;
-    b->fft.twy = q; q += n_twy;
     b->fft.rtx = q; q += n_rtx;
     b->fft.rty = q; q += n_rty;
     b->fft.dcty = q; q += n_dct;
@@ -550,6 +583,9 @@ int pv_fft_create(pv_bag *b)
     b->fft.lamy = q; q += ny;
     b->fft.wx = q; q += n_wx;
     b->fft.wy = q; q += n_wy;
+    b->fft.tri_as = q; q += ny;
+    b->fft.tri_an = q; q += ny;
+    b->fft.tri_mw = q; q += nx;
     b->fft.spec = q;
     b->fft.ok = 1;
     return PV_OK;

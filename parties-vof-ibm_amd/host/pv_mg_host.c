@@ -187,8 +187,8 @@ int pv_set_ch_solver(pv_bag *b, int spectral)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");
     if (spectral < 0 || spectral > 1) return pv_set_error(PV_EINVAL, "CH solver must be 0 or 1 (got %d)", spectral);
-    if (spectral && !b->fft.ok)
-        return pv_set_error(PV_EINVAL, "spectral CH solve needs the FFT tables (power-of-two NXM, NYM)");
+    if (spectral && !(b->fft.ok && b->fft.dct_ok))
+        return pv_set_error(PV_EINVAL, "spectral CH solve needs the FFT tables and a power-of-two NYM");
     b->ch_spectral = spectral;
     return PV_OK;
 }

@@ -76,7 +76,11 @@ static void ch_spectral_start(pv_bag *b, int st);
  * then accepts it at iteration 0, or iterates on if it is not exact. */
 static int ch_step(pv_bag *b)
 {
-    int st = b->stage, spec = b->ch_spectral && b->fft.ok;
+    int st = b->stage, spec = b->ch_spectral && b->fft.ok && b->fft.dct_ok;
+    /* VOF_DIFFUSE_advect_WENO5 starts from the boundary values of C_L
+     * (VOF_DIFFUSE.c:945): they are current after every clamp of this
+     * library, but not after a caller or a restart has written C_L */
+    pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
     pv_tbegin(b, PV_T_CH_RHS);
     pvk_ccg_init(G(b), &b->ph, st, b->d_sc, &b->ccg, FD(b, PV_F_C_L), FD(b, PV_F_U), FD(b, PV_F_V),
                  FD(b, PV_F_CH_RHS_N), NULL, FD(b, PV_F_CH_RES), FD(b, PV_F_CH_DIR),
@@ -115,7 +119,10 @@ int pv_Communication_update_ghost_nodes_flow_variable(pv_bag *b, int field, int 
 }
 
 /* VOF_DIFFUSE.c:524-568 (no IBM).  The phase-cache kernels also refresh rho
- * and mu from C_G, as they do everywhere in this library. */
+ * and mu from C_G, as they do everywhere in this library; F ends as the copy
+ * of C_L that the reference's phase cache keeps (VOF_DIFFUSE.c:108-126).
+ * The substep does not maintain that mirror: nothing on the path reads F
+ * after start-up, and the output writers write C_L for it. */
 int pv_VOF_DIFFUSE_init(pv_bag *b)
 {
     RC(begin(b));
@@ -129,6 +136,7 @@ int pv_VOF_DIFFUSE_init(pv_bag *b)
     pvk_bc_ch(G(b), FD(b, PV_F_C_L), S(b));
     pvk_phase_props(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), FD(b, PV_F_RHO), FD(b, PV_F_MU), S(b));
     pvk_bc_ch(G(b), FD(b, PV_F_C_G), S(b));
+    pvk_copy(G(b), FD(b, PV_F_C_L), FD(b, PV_F_F), S(b));     /* the phase cache's F mirror (:108-126) */
     LAUNCHCHK();
     return PV_OK;
 }
@@ -439,17 +447,30 @@ static int p_solve_jacobi(pv_bag *b)
     return PV_OK;
 }
 
-/* z = L0^+ r with the <r, z> partials, each transform stage timed. */
-/* PCG with the FFT-x / DCT-y preconditioner; relative-residual stop.
- * Fused form (csrc/pv_pcg.hip): six launches per iteration, all scalars
- * formed redundantly per block from the previous kernel's partials. */
-static const int pcg_timer[6] = { PV_T_P_APPLY, PV_T_P_UPDATE, PV_T_P_TRANSPOSE, PV_T_FFT_COLS,
-                                  PV_T_P_TRANSPOSE, PV_T_FFT_ROWS_INV };
+/* PCG with the FFT-x / tridiagonal-y preconditioner; relative-residual
+ * stop.  Fused form (csrc/pv_pcg.hip): four launches per iteration (operator,
+ * update + forward rows, stop test + y solves in place, inverse rows +
+ * <r, z>), all scalars formed redundantly per block from the previous
+ * kernel's partials.  The Cahn-Hilliard spectral start (chb > 0) needs the
+ * y transforms instead of the tridiagonal solves: transpose, DCT-II /
+ * divide / DCT-III, transpose back. */
+static const int pcg_timer[7] = { PV_T_P_APPLY, PV_T_P_UPDATE, PV_T_P_TRANSPOSE, PV_T_FFT_COLS,
+                                  PV_T_P_TRANSPOSE, PV_T_FFT_ROWS_INV, PV_T_FFT_COLS };
+static const int pcg_tri_stages[4] = { 0, 1, 6, 5 };
 
 static void pcg_iteration_w(pv_bag *b, const pvk_pcg *w, int n, double tol, int maxit)
 {
-    int st;
-    for (st = (n < 0) ? 2 : 0; st < 6; st++) {
+    int k, st;
+    if (w->chb > 0.0) {
+        for (st = (n < 0) ? 2 : 0; st < 6; st++) {
+            pv_tbegin(b, pcg_timer[st]);
+            pvk_pcg_stage(G(b), &b->fft, b->d_sc, w, st, n, tol, maxit, S(b));
+            pv_tend(b, pcg_timer[st]);
+        }
+        return;
+    }
+    for (k = (n < 0) ? 2 : 0; k < 4; k++) {
+        st = pcg_tri_stages[k];
         pv_tbegin(b, pcg_timer[st]);
         pvk_pcg_stage(G(b), &b->fft, b->d_sc, w, st, n, tol, maxit, S(b));
         pv_tend(b, pcg_timer[st]);
@@ -546,7 +567,7 @@ static int p_solve_fft_pcg(pv_bag *b)
 {
     int maxit = b->ph.p_max_iter;
     double tol = b->ph.p_rel_tol;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs NXM = 2^a 3^b, a multiple of 16 and <= 4096, and 8 <= NYM <= 8192");
     pcg_cold_start(b, tol, maxit);
     return pcg_loop(b, 0, tol, maxit);
 }
@@ -703,7 +724,7 @@ static int mg_solve(pv_bag *b, int learned)
     int maxit = b->ph.p_max_iter, st = b->stage, best = 0;
     double tol = b->ph.p_rel_tol, ec, eb, e;
     if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
-    if (learned && !b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (power-of-two sizes)");
+    if (learned && !b->fft.ok) return pv_set_error(PV_EINVAL, "learned MG-PCG needs the FFT solve (NXM = 2^a 3^b, a multiple of 16)");
     if (learned && !b->cnn_ready)
         return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
     if (learned && b->net_skip > 0) {
@@ -793,7 +814,7 @@ static int p_solve_collect(pv_bag *b)
     pv_learn *L = b->learn;
     int maxit = b->ph.p_max_iter, n = 0, si = 0;
     double tol = b->ph.p_rel_tol;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs NXM = 2^a 3^b, a multiple of 16 and <= 4096, and 8 <= NYM <= 8192");
     pcg_cold_start(b, tol, maxit);
     for (;;) {
         int next;
@@ -885,7 +906,7 @@ static int p_solve_learned(pv_bag *b)
 {
     int maxit = b->ph.p_max_iter;
     double tol = b->ph.p_rel_tol;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs NXM = 2^a 3^b, a multiple of 16 and <= 4096, and 8 <= NYM <= 8192");
     if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
     RC(learned_warm_start(b, tol, maxit));
     if (b->h_sc->p_done) return PV_OK;
@@ -902,7 +923,7 @@ static int p_solve_learned_fcg(pv_bag *b)
     int maxit = b->ph.p_max_iter, n = 0, since = 0, done_fcg;
     double tol = b->ph.p_rel_tol, best;
     pvk_pcg w;
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs power-of-two NXM, NYM >= 8");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT-PCG needs NXM = 2^a 3^b, a multiple of 16 and <= 4096, and 8 <= NYM <= 8192");
     if (!b->cnn_ready) return pv_set_error(PV_EINVAL, "learned pressure solve needs a network (pv_cnn_init/set)");
     RC(learned_warm_start(b, tol, maxit));
     if (b->h_sc->p_done) return PV_OK;
@@ -1010,9 +1031,19 @@ int pv_Pressure_compute_velocity_divergence(pv_bag *b, double *div_max)
 int pv_Pressure_solve_fft(pv_bag *b, int rhs_field, int out_field)
 {
     RC(begin(b));
-    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT solve needs power-of-two NXM, NYM >= 8");
-    pvk_fft_solve(G(b), &b->fft, FD(b, rhs_field), FD(b, out_field), 1, S(b));
-    pvk_pin_row0(G(b), FD(b, out_field), S(b));
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "FFT solve needs NXM = 2^a 3^b, a multiple of 16 and <= 4096, and 8 <= NYM <= 8192");
+    if (rhs_field < 0 || rhs_field >= PV_NFIELDS || out_field < 0 || out_field >= PV_NFIELDS)
+        return pv_set_error(PV_EINVAL, "bad field id");
+    {   /* the preconditioner's stages on the given fields, with the reference's pin */
+        pvk_pcg w = b->pcg;
+        w.r = FD(b, rhs_field);
+        w.z = FD(b, out_field);
+        w.keep_pin = 1;
+        pvk_pcg_clear_done(b->d_sc, S(b));
+        pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 1, -3, 0.0, 1, S(b));
+        pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 6, -3, 0.0, 1, S(b));
+        pvk_pcg_stage(G(b), &b->fft, b->d_sc, &w, 5, -3, 0.0, 1, S(b));
+    }
     pvk_halo_cell(G(b), FD(b, out_field), S(b));
     LAUNCHCHK();
     HIPCHK(hipStreamSynchronize(S(b)));

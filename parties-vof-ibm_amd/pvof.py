@@ -140,6 +140,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_fill_field.argtypes = [vp, C.c_int, C.c_double]
     lib.pv_init_from_F.argtypes = [vp, dp]
     lib.pv_set_time_state.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
+    lib.pv_set_step_count.argtypes = [vp, C.c_int]
     lib.pv_get_stats.argtypes = [vp, C.POINTER(Stats)]
     lib.pv_Temporal_int_rk3.argtypes = [vp, C.c_int]
     lib.pv_Dtime_cfl.argtypes = [vp, dp]
@@ -435,3 +436,25 @@ def load_shim(path=SHIM_PATH):
                  "Pressure_compute_preconditioner", "Pressure_project_velocity"):
         getattr(sh, name).argtypes = [vp]
     return sh
+
+
+IO_PATH = os.path.join(HERE, "_build", "libparties_vof_io.so")
+PV_IO_NTIMER = 34
+
+
+class IOCounters(C.Structure):
+    _fields_ = [("noutput", C.c_int), ("noutput_2d", C.c_int), ("output_time", C.c_double),
+                ("output_time_2d", C.c_double), ("timer", C.c_double * PV_IO_NTIMER)]
+
+
+def load_io(path=IO_PATH):
+    """Load the HDF5 Data / Resume module (include/parties_vof_io.h)."""
+    if not os.path.exists(path):
+        raise FileNotFoundError(f"{path} missing: run __graft_entry__.build() first")
+    io = C.CDLL(path)
+    vp = C.c_void_p
+    io.pv_io_last_error.restype = C.c_char_p
+    io.pv_Output_h5_data.argtypes = [vp, C.c_char_p]
+    io.pv_Output_h5_resume.argtypes = [vp, C.c_char_p, C.POINTER(IOCounters)]
+    io.pv_Resume_h5.argtypes = [vp, C.c_char_p, C.c_char_p, C.POINTER(IOCounters)]
+    return io

@@ -49,7 +49,10 @@ def test_spectral_start_matches_cg(lib, precision):
     print(f"\nbiharmonic CG iterations per substep: from C {out[0][1]}, from the spectral solve {out[1][1]}")
     assert max(out[1][1]) <= 1
     R = regions(p)
-    for f, tol in (("C_L", 1e-9), ("U", 1e-7), ("V", 1e-7), ("P", 1e-7)):
+    # C_L to well inside the CH tolerance; u, v, p to the momentum CG's stop
+    # (RMS residual CG_ETOL = 1e-7): the two runs' momentum solves stop at
+    # different iterates once their inputs differ at round-off
+    for f, tol in (("C_L", 1e-9), ("U", 5e-7), ("V", 5e-7), ("P", 5e-7)):
         e = relerr(out[1][0].get(f)[R["cell"]], out[0][0].get(f)[R["cell"]])
         print(f"  {f}: {e:.2e}")
         assert e < tol, f

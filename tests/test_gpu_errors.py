@@ -4,8 +4,8 @@ Every rejected call returns PV_EINVAL (-1) with a message from
 pv_last_error, launches nothing, and leaves the handle usable: an RK3 step
 on it afterwards meets the pressure tolerance.  Size limits: MG-PCG needs a
 grid that halves to <= 4096 cells and on to a last level of <= 64 cells
-(rejected at pv_create), FFT-PCG needs power-of-two sizes (rejected by the
-pressure solve, the only routine that needs the transform).
+(rejected at pv_create), FFT-PCG needs NXM = 2^a 3^b, a multiple of 16
+(rejected by the pressure solve, the only routine that needs the transform).
 """
 import ctypes as C
 
@@ -49,11 +49,14 @@ def test_mg_pcg_rejects_grid_that_does_not_halve(lib):
     s.close()
 
 
-def test_fft_pcg_rejects_non_power_of_two(lib):
-    s = pvof.Solver(params(96, pvof.PV_PSOLVE_FFT_PCG), lib)
+@pytest.mark.parametrize("n", [98, 100, 40])
+def test_fft_pcg_rejects_unsupported_row_lengths(lib, n):
+    """98 = 2 x 7^2 and 100 = 2^2 x 5^2 have other prime factors; 40 = 2^3 x 5
+    also; the transforms take NXM = 2^a 3^b with 16 | NXM."""
+    s = pvof.Solver(params(n, pvof.PV_PSOLVE_FFT_PCG), lib)
     it = C.c_int(-7)
     assert lib.pv_Pressure_solve_cg(s.h, C.byref(it)) == PV_EINVAL
-    assert "power-of-two" in message(lib)
+    assert "2^a 3^b" in message(lib)
     assert it.value == -7
     s.close()
 

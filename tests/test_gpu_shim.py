@@ -18,7 +18,10 @@ pytestmark = pytest.mark.gpu
 
 
 def params_small(**kw):
-    base = dict(ch_tol=1e-13, CG_ETOL=1e-7, CG_MAXIT=400, P_CG_ETOL=1e-12, P_CG_MAXIT=6000)
+    # solves converged far below the compared tolerance, so the GPU and the
+    # oracle stop at the same iterate whatever their summation order
+    base = dict(ch_tol=1e-13, CG_ETOL=1e-12, CG_MAXIT=400, pressure_solver=pvof.PV_PSOLVE_FFT_PCG,
+                p_rel_tol=1e-13, p_max_iter=3000)
     base.update(kw)
     return pvof.config1_params(NXM=32, NYM=64, **base)
 
