@@ -186,13 +186,15 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     if net_params is not None:
         pvof.Network(s).set_params(net_params)
     o = dict(warm_start=args.warm_start, ch_spectral=args.ch_spectral, mg_cycle=args.mg_cycle,
-             mg_sweeps=args.mg_sweeps)
+             mg_sweeps=args.mg_sweeps, mom_warm=args.mom_warm, mom_exact=args.mom_exact)
     o.update(opts or {})
     if family_of(solver) == "mg":
         s.check(lib.pv_set_pressure_warm_start(s.h, o["warm_start"]))
         s.check(lib.pv_set_ch_solver(s.h, o["ch_spectral"]))
         s.check(lib.pv_set_mg_cycle(s.h, o["mg_cycle"]))
         s.check(lib.pv_set_mg_sweeps(s.h, o["mg_sweeps"]))
+        s.check(lib.pv_set_momentum_warm_start(s.h, o["mom_warm"]))
+        s.check(lib.pv_set_momentum_cg_steps(s.h, o["mom_exact"]))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -259,10 +261,11 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
 def mixed_error(lib, args):
     """Per-field error of the fast mixed form (FP32 stencils and multigrid
     cycle, W on --mg-cycle in-workgroup levels, FP64 fluxes / solves / dots,
-    spectral CH start, temporal pressure start) against the FP64 path (FP64
-    stencils and V-cycle, the reference CH CG, cold pressure starts), both
-    MG-PCG, stepped side by side from the same seeded field; errors at
-    checkpoints up to --mixed-error-steps."""
+    spectral CH start, temporal pressure and momentum starts, momentum CG
+    steps without EPS) against the FP64 path (FP64 stencils and V-cycle, the
+    reference CH and momentum CGs, cold starts), both MG-PCG, stepped side by
+    side from the same seeded field; errors at checkpoints up to
+    --mixed-error-steps."""
     n, steps = args.n, args.mixed_error_steps
     if args.ic == "droplets":          # 32 liquid droplets of radius 0.02-0.06 in gas
         kw = dict(nmin=32, nmax=32, rmin=0.02, rmax=0.06, gas_inside=0)
@@ -278,6 +281,8 @@ def mixed_error(lib, args):
             s.check(lib.pv_set_ch_solver(s.h, 1))
             s.check(lib.pv_set_pressure_warm_start(s.h, 1))
             s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
+            s.check(lib.pv_set_momentum_warm_start(s.h, args.mom_warm))
+            s.check(lib.pv_set_momentum_cg_steps(s.h, args.mom_exact))
         runs[name], secs[name] = s, 0.0
     R = (slice(3, 3 + n), slice(3, 3 + n))
     marks = sorted({m for m in (10, 50, 100, 250, steps) if m <= steps})
@@ -459,6 +464,11 @@ def main():
                     help="MG solvers: the first k in-workgroup levels take two coarse corrections (0 = V-cycle)")
     ap.add_argument("--mg-sweeps", type=int, choices=[1, 2], default=2,
                     help="MG solvers: Gauss-Seidel sweeps before and after each coarse correction")
+    ap.add_argument("--mom-warm", type=int, choices=[0, 1], default=1,
+                    help="MG-PCG forms: 1 = start each momentum CG from the same RK3 stage's last increment")
+    ap.add_argument("--mom-exact", type=int, choices=[0, 1], default=1,
+                    help="MG-PCG forms: 1 = momentum CG step lengths without the reference's EPS = 1e-14 "
+                         "(same stop test; the EPS stalls the CG near CG_ETOL)")
     ap.add_argument("--ch-spectral", type=int, choices=[0, 1], default=1,
                     help="MG-PCG forms: 1 = start the biharmonic CG from its exact spectral solve")
     ap.add_argument("--no-compare", action="store_true",
@@ -555,7 +565,7 @@ def main():
             note(f"FP64 MG-PCG runs: {args.plain_steps} steps each")
             # the best FP64 path: multigrid V-cycle, the reference CH CG, cold starts, no network
             ref = run(lib, args, "mg", "fp64", args.plain_steps, 1, F, timers=False,
-                      opts=dict(warm_start=0, ch_spectral=0, mg_cycle=0))
+                      opts=dict(warm_start=0, ch_spectral=0, mg_cycle=0, mom_warm=0, mom_exact=0))
             compare["fp64_mg"] = public(ref)
             compare["speedup_vs_fp64_mg"] = round(main_run["value"] / ref["value"], 3)
             # the timed form's algorithm in FP64 throughout (isolates the FP32 stencils and cycle)
@@ -585,6 +595,8 @@ def main():
                    "NXM": p.NXM, "NYM": p.NYM, "pressure_solver": args.solver, "precision": args.precision,
                    "warm_start": args.warm_start if family_of(args.solver) == "mg" else None,
                    "ch_spectral": args.ch_spectral if family_of(args.solver) == "mg" else None,
+                   "momentum_warm_start": args.mom_warm if family_of(args.solver) == "mg" else None,
+                   "momentum_exact_steps": args.mom_exact if family_of(args.solver) == "mg" else None,
                    "mg_cycle_w_levels": args.mg_cycle if family_of(args.solver) == "mg" else None,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": main_run["path_hbm_fraction"],

@@ -334,6 +334,20 @@ int pv_set_pressure_warm_start(pv_bag *bag, int mode);
  * for PV_PREC_FP64 handles), 1 = FP32 (default for PV_PREC_MIXED handles).
  * The CG vectors, dots, stop test and solution stay FP64 either way. */
 int pv_set_mg_precision(pv_bag *bag, int fp32);
+/* Start of the momentum solves (Velocity_solve_cg, msolve_cg.c:525-722):
+ * mode 0 = from u^{k-1}, as the reference; mode 1 = from u^{k-1} plus the
+ * increment the same RK3 stage's solve made in the previous step, scaled by
+ * the ratio of the time steps (the CG's stop test is unchanged). */
+int pv_set_momentum_warm_start(pv_bag *bag, int mode);
+/* Step lengths of the momentum CG (msolve_cg.c:674 alpha = rz / (dAd + EPS),
+ * :705 beta = rz_new / (rz + EPS), EPS = 1e-14): 0 = as the reference
+ * (default); 1 = without EPS (alpha = rz / dAd, beta = rz_new / rz, a zero
+ * denominator giving a zero step).  Once rz and d.Ad fall to about EPS --
+ * with this operator's diagonal of rho / (alpha_k dt), well before small
+ * velocities meet CG_ETOL -- the added EPS shortens every step and the CG
+ * creeps on until CG_MAXIT; without it the same stop test (RMS residual <
+ * CG_ETOL) is met in a few iterations.  The diagonal keeps its + EPS. */
+int pv_set_momentum_cg_steps(pv_bag *bag, int exact);
 /* Cahn-Hilliard implicit solve (the reference runs the CG of
  * VOF_DIFFUSE.c:318-429 from C): 0 = that CG (default); 1 = the CG started
  * from the exact solution of a_dt x + (Cn^2/Pe) L^2 x = b, which the

@@ -93,6 +93,7 @@ struct pvo_state {
     struct pvo_ws *ws;         /* one FFT / Thomas workspace per host thread */
     int nws;
     double clamp_int;          /* last clamp correction over interior cells only */
+    int mom_exact;             /* 1: momentum CG steps without the EPS denominators */
 };
 
 struct pvo_ws {
@@ -218,6 +219,14 @@ int pvo_threads(void) { return PVO_NTHREADS; }
 double *pvo_field(pvo_state *s, int field) { return (field >= 0 && field < PV_NFIELDS) ? s->f[field] : NULL; }
 int pvo_nxt(const pvo_state *s) { return s->P; }
 int pvo_nyt(const pvo_state *s) { return s->NYT; }
+
+/* The momentum CG's steps without EPS (alpha = rz / dAd, beta = rz_new / rz,
+ * zero when the denominator is zero): the library's pv_set_momentum_cg_steps
+ * variant, not a reference routine. */
+void pvo_set_momentum_cg_steps(pvo_state *s, int exact)
+{
+    s->mom_exact = exact;
+}
 
 void pvo_set_time_state(pvo_state *s, double time, double dt, double dt_old, int stage)
 {
@@ -1052,7 +1061,8 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
         PVO_FOR_SUM(dAd)
         for (j = js; j < je; j++)
             for (i = is; i < ie; i++) dAd += AT(d, i, j) * AT(Ad, i, j);
-        alpha = rz / (dAd + EPS);
+        if (s->mom_exact) alpha = (dAd != 0.0) ? rz / dAd : 0.0;
+        else alpha = rz / (dAd + EPS);
         PVO_FOR_SUM(lv0, lv1)
         for (j = js; j < je; j++)
             for (i = is; i < ie; i++) {
@@ -1066,7 +1076,8 @@ int pvo_Velocity_solve_cg(pvo_state *s, int comp)
             }
         RMS = sqrt(in_tot * lv1);
         if (RMS < s->prm.CG_ETOL) break;
-        beta = lv0 / (rz + EPS);
+        if (s->mom_exact) beta = (rz != 0.0) ? lv0 / rz : 0.0;
+        else beta = lv0 / (rz + EPS);
         rz = lv0;
         PVO_FOR
         for (j = js; j < je; j++)

@@ -28,6 +28,7 @@ int    pvo_nyt(const pvo_state *s);                /* NYM + 7 */
 int    pvo_threads(void);                            /* host threads of the build (1 without OpenMP) */
 int    pvo_init_from_F(pvo_state *s, const double *F);
 void   pvo_set_time_state(pvo_state *s, double time, double dt, double dt_old, int stage);
+void   pvo_set_momentum_cg_steps(pvo_state *s, int exact);
 void   pvo_get_stats(const pvo_state *s, pv_stats *out);
 
 int    pvo_Temporal_int_rk3(pvo_state *s, int nsteps);

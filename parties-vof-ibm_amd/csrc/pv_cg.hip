@@ -289,7 +289,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
 }
 
 // K1: stop test sqrt(r.r / N) < CG_ETOL (msolve_cg.c:691-704), beta =
-// rz_new / (rz + EPS), d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
+// rz_new / (rz + EPS) (w.exact: rz_new / rz, 0 when rz is 0),
+// d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
 // the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
 #define MCH 256                                                  // columns per chunk of k_mcg_apply
 #define MSQ (((PVK_MOM_BAND + 2) * (MCH + 2) + NTH - 1) / NTH)     // staged values per thread
@@ -312,7 +313,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         const double rms = sqrt(w.in_tot[c] * rr);
         bool stop = rms < w.tol;
         if (!stop) {
-            beta = rz / (sc->m_rzb[c][(n + 1) & 1] + EPS);
+            const double rzo = sc->m_rzb[c][(n + 1) & 1];
+            beta = w.exact ? (rzo != 0.0 ? rz / rzo : 0.0) : rz / (rzo + EPS);
             stop = (n >= w.maxit);
         }
         if (stop) {
@@ -375,7 +377,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
     block_store_partial(acc, w.dad[c], m.band, sh);
 }
 
-// K2: alpha = rz / (dAd + EPS) (msolve_cg.c:674); x += alpha d; r -= alpha Ad;
+// K2: alpha = rz / (dAd + EPS) (msolve_cg.c:674; w.exact: rz / dAd, 0 when
+// dAd is 0); x += alpha d; r -= alpha Ad;
 // partials of r^2 / dg and r^2 (:676-690).
 __global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pvk_mcg w, int n)
 {
@@ -386,7 +389,8 @@ __global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pv
     const int P = g.P, NXM = g.NXM;
     const double EPS = 1e-14;
     const double dAd = block_sum(w.dad[c], w.nb[c], sh);
-    const double alpha = sc->m_rzb[c][n & 1] / (dAd + EPS);
+    const double rzn = sc->m_rzb[c][n & 1];
+    const double alpha = w.exact ? (dAd != 0.0 ? rzn / dAd : 0.0) : rzn / (dAd + EPS);
     if (m.band == 0 && threadIdx.x == 0) sc->m_iter[c] = n + 1;
     double *x = w.x[c], *r = w.r[c];
     const double *d = w.d[c][n & 1], *Ad = w.Ad[c], *dg = w.dg[c];

@@ -373,14 +373,33 @@ __global__ void __launch_bounds__(256) k_cnn_l1_back(const bf16 *__restrict__ gz
     }
 }
 
-// g[off + k] = sum over blocks of part[blk * n + k], k < n (fixed order)
-__global__ void k_cnn_reduce(const float *__restrict__ part, int nblk, int n, float *__restrict__ g)
+// g[k] = sum over blocks q of part[q * n + k], k < n, in a fixed order over
+// two launches: row group y (blockIdx.y) sums rows q = y, y + RSPLIT, ...
+// into rpart[y * n + k]; then the RSPLIT group sums are added in order.
+__global__ void k_cnn_reduce_rows(const float *__restrict__ part, int nblk, int n, float *__restrict__ rpart)
+{
+    const int k = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (k >= n) return;
+    float s = 0.0f;
+    for (int q = y; q < nblk; q += PVK_CNN_RSPLIT) s += part[(size_t)q * n + k];
+    rpart[(size_t)y * n + k] = s;
+}
+
+__global__ void k_cnn_reduce_groups(const float *__restrict__ rpart, int n, float *__restrict__ g)
 {
     const int k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     float s = 0.0f;
-    for (int q = 0; q < nblk; q++) s += part[(size_t)q * n + k];
+#pragma unroll 8
+    for (int y = 0; y < PVK_CNN_RSPLIT; y++) s += rpart[(size_t)y * n + k];
     g[k] = s;
+}
+
+static void cnn_reduce(const pvk_cnn *c, int nblk, int n, float *g, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_cnn_reduce_rows, dim3((n + 255) / 256, PVK_CNN_RSPLIT), dim3(256), 0, s,
+                       (const float *)c->part, nblk, n, c->rpart);
+    hipLaunchKernelGGL(k_cnn_reduce_groups, dim3((n + 255) / 256), dim3(256), 0, s, (const float *)c->rpart, n, g);
 }
 
 // Adam (FP32 master weights)
@@ -441,22 +460,19 @@ extern "C" void pvk_cnn_backward(const pvk_cnn *c, const float *in3, const float
                        1.0f / (float)(interior > 0 ? interior : 1));
     hipLaunchKernelGGL(k_cnn_l6_back, gt, dim3(256), 0, s, (const float *)c->gout, (const bf16 *)c->act[4],
                        (const float *)c->w, c->gz[0], c->part, H, W, px);
-    hipLaunchKernelGGL(k_cnn_reduce, dim3((289 + 255) / 256), dim3(256), 0, s, (const float *)c->part, nt, 289,
-                       c->g + 37888);
+    cnn_reduce(c, nt, 289, c->g + 37888, s);
     int cur = 0;
     for (int l = 3; l >= 0; l--) {       // layer l + 2: input act[l], output-side gradient gz[cur]
         hipLaunchKernelGGL(k_cnn_dw, gt, dim3(256), 0, s, (const bf16 *)c->gz[cur], (const bf16 *)c->act[l], c->part,
                            H, W, px);
-        hipLaunchKernelGGL(k_cnn_reduce, dim3((9248 + 255) / 256), dim3(256), 0, s, (const float *)c->part, nt,
-                           9248, c->g + 896 + l * 9248);
+        cnn_reduce(c, nt, 9248, c->g + 896 + l * 9248, s);
         hipLaunchKernelGGL((k_cnn_mid<false, true>), gt, dim3(256), 0, s, (const bf16 *)c->gz[cur],
                            (const bf16 *)(c->wt + l * 9216), (const float *)0, c->gz[cur ^ 1],
                            (const bf16 *)c->act[l], H, W, px);
         cur ^= 1;
     }
     hipLaunchKernelGGL(k_cnn_l1_back, gt, dim3(256), 0, s, (const bf16 *)c->gz[cur], in3, c->part, H, W, px);
-    hipLaunchKernelGGL(k_cnn_reduce, dim3((896 + 255) / 256), dim3(256), 0, s, (const float *)c->part, nt, 896,
-                       c->g);
+    cnn_reduce(c, nt, 896, c->g, s);
 }
 
 extern "C" void pvk_cnn_adam(const pvk_cnn *c, float lr, int t, hipStream_t s)

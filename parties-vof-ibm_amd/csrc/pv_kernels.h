@@ -126,12 +126,16 @@ typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double *rz[2][2], *dad[2], *rr[2];
     int nb[2], band;                 /* nb[c] = 0 leaves component c out */
     int maxit;
+    int exact;                       /* 0: EPS = 1e-14 added to the step denominators (the
+                                        reference); 1: the denominators as they are, a zero
+                                        one giving a zero step */
     double tol, in_tot[2];           /* CG_ETOL; 1 / number of unknowns */
 } pvk_mcg;
 
 /* Learned preconditioner network (csrc/pv_cnn.hip): parameters, Adam state
  * and the activation / gradient tensors of the largest batch seen so far. */
 #define PVK_CNN_NPARAM 38177
+#define PVK_CNN_RSPLIT 32                /* row groups of the gradient reduction */
 typedef struct pvk_cnn {
     float *w, *g, *m, *v;            /* FP32 parameters, gradient, Adam moments */
     unsigned short *wb, *wt;         /* BF16 32 -> 32 weights: forward, flipped transpose */
@@ -139,6 +143,7 @@ typedef struct pvk_cnn {
     unsigned short *gz[2];           /* BF16 NHWC gradients w.r.t. pre-activations */
     float *gout;                     /* dLoss / d out */
     float *part, *lpart;             /* per-block partials: gradients, loss */
+    float *rpart;                    /* PVK_CNN_RSPLIT x 9248 row-group sums of part */
     size_t cap_pix;                  /* B * H * W the tensors hold */
 } pvk_cnn;
 
@@ -183,6 +188,9 @@ void pvk_psi_fsigma(const pvk_grid *g, const pvk_phys *ph, const double *C, cons
                     double *lap, double *psi, double *fx, double *fy, hipStream_t s);
 void pvk_psi_only(const pvk_grid *g, const pvk_phys *ph, const double *C, double *psi, hipStream_t s);
 void pvk_clamp_init(const pvk_grid *g, const double *F, double *C, hipStream_t s);
+/* temporal start of the momentum solves: mode 0 s = x, x += scale * d; mode 1 d = x - s */
+void pvk_mom_warm(const pvk_grid *g, int mode, double scale, double *u, double *v, double *su, double *sv,
+                  double *du, double *dv, hipStream_t s);
 /* C = clamp01(F) over the whole L range, ghosts included */
 void pvk_clamp_slab(const pvk_grid *g, const double *F, double *C, hipStream_t s);
 

@@ -238,6 +238,39 @@ extern "C" void pvk_xwrap(const pvk_grid *g, double *f, int ulike, int pnodes, h
     hipLaunchKernelGGL(k_xwrap, dim3((n + 255) / 256), dim3(256), 0, s, *g, f, ulike, pnodes);
 }
 
+// Temporal start of the momentum solves: mode 0 keeps x in s and adds
+// scale * dlt (the same stage's increment of the previous step, scaled by the
+// ratio of the time steps) on the unknowns of each component (u: i = 1..NXM,
+// j < NYM; v: i < NXM, 1 <= j < NYM), x untouched when scale is 0; mode 1
+// stores the new increment x - s.
+__global__ void k_mom_warm(pvk_grid g, int mode, double scale, double *u, double *v, double *su, double *sv,
+                           double *du, double *dv)
+{
+    const int P = g.P;
+    const int i = blockIdx.x * BX + threadIdx.x, j = blockIdx.y * BY + threadIdx.y;
+    if (j >= g.NYM || i > g.NXM) return;
+    if (i >= 1) {
+        if (mode == 0) {
+            const double x = AT(u, i, j);
+            AT(su, i, j) = x;
+            if (scale != 0.0) AT(u, i, j) = x + scale * AT(du, i, j);
+        } else AT(du, i, j) = AT(u, i, j) - AT(su, i, j);
+    }
+    if (i < g.NXM && j >= 1) {
+        if (mode == 0) {
+            const double x = AT(v, i, j);
+            AT(sv, i, j) = x;
+            if (scale != 0.0) AT(v, i, j) = x + scale * AT(dv, i, j);
+        } else AT(dv, i, j) = AT(v, i, j) - AT(sv, i, j);
+    }
+}
+extern "C" void pvk_mom_warm(const pvk_grid *g, int mode, double scale, double *u, double *v, double *su,
+                             double *sv, double *du, double *dv, hipStream_t s)
+{
+    dim3 gr((g->NXM + 1 + BX - 1) / BX, (g->NYM + BY - 1) / BY);
+    hipLaunchKernelGGL(k_mom_warm, gr, dim3(BX, BY), 0, s, *g, mode, scale, u, v, su, sv, du, dv);
+}
+
 __global__ void k_fill(double *f, size_t n, double val)
 {
     size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

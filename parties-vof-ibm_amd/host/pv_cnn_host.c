@@ -40,7 +40,7 @@ void pv_cnn_free(pv_bag *b)
     hipFree(c->wb); hipFree(c->wt);
     for (k = 0; k < 5; k++) hipFree(c->act[k]);
     hipFree(c->gz[0]); hipFree(c->gz[1]);
-    hipFree(c->gout); hipFree(c->part); hipFree(c->lpart);
+    hipFree(c->gout); hipFree(c->part); hipFree(c->lpart); hipFree(c->rpart);
     hipFree(b->cnn_io);
     memset(c, 0, sizeof *c);
     b->cnn_io = NULL;
@@ -57,6 +57,10 @@ static int cnn_reserve(pv_bag *b, size_t npix)
     if (npix <= c->cap_pix) return PV_OK;
     for (k = 0; k < 5; k++) { hipFree(c->act[k]); c->act[k] = NULL; }
     hipFree(c->gz[0]); hipFree(c->gz[1]); hipFree(c->gout); hipFree(c->part); hipFree(c->lpart);
+    c->gout = c->part = c->lpart = NULL;
+    c->gz[0] = c->gz[1] = NULL;
+    c->cap_pix = 0;
+    if (!c->rpart) HIPCHK(hipMalloc((void **)&c->rpart, (size_t)PVK_CNN_RSPLIT * 9248 * sizeof(float)));
     for (k = 0; k < 5; k++) HIPCHK(hipMalloc((void **)&c->act[k], act));
     HIPCHK(hipMalloc((void **)&c->gz[0], act));
     HIPCHK(hipMalloc((void **)&c->gz[1], act));

@@ -415,6 +415,7 @@ int pv_set_time_state(pv_bag *b, double time, double dt, double dt_old, int whic
     HIPCHK(hipMemcpy(b->h_sc, b->d_sc, sizeof(pvk_scal), hipMemcpyDeviceToHost));
     b->time = time; b->dt = dt; b->dt_old = dt_old; b->stage = which_stage;
     b->time_nonzero = (time != 0.0);
+    b->have_mprev[0] = b->have_mprev[1] = b->have_mprev[2] = 0;   /* increments of another run */
     return pv_sync_scalars_to_device(b);
 }
 

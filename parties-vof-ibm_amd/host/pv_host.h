@@ -22,6 +22,14 @@ enum pv_internal_field {
     PV_FI_PHI0,                /* pressure correction of the last solve of RK3 stage 0, 1, 2 */
     PV_FI_PHI1,
     PV_FI_PHI2,
+    PV_FI_MU0,                 /* momentum solve of RK3 stage 0, 1, 2: the last increment of u ... */
+    PV_FI_MU1,
+    PV_FI_MU2,
+    PV_FI_MV0,                 /* ... and of v */
+    PV_FI_MV1,
+    PV_FI_MV2,
+    PV_FI_MUS,                 /* u and v before the momentum solve */
+    PV_FI_MVS,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
@@ -93,6 +101,10 @@ struct pv_bag {
     int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
     int ch_spectral;               /* 1: biharmonic CG started from the exact spectral solve */
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
+    int m_warm;                    /* 1: start each momentum solve from u + the stage's last increment */
+    int m_exact;                   /* 1: momentum CG steps without the EPS = 1e-14 denominators */
+    int have_mprev[3];             /* PV_FI_MU0 / MV0 + stage hold an increment ... */
+    double mprev_dt[3];            /* ... of a step of this dt */
     int warm_wins[3], warm_solves[3];  /* per stage: the temporal start's consecutive wins, solves since
                                           it became trusted */
     int net_losses, net_skip;      /* consecutive losses of the network's start; solves left to skip it */

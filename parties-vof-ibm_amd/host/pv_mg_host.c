@@ -183,6 +183,23 @@ int pv_set_pressure_warm_start(pv_bag *b, int mode)
     return PV_OK;
 }
 
+int pv_set_momentum_warm_start(pv_bag *b, int mode)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (mode < 0 || mode > 1) return pv_set_error(PV_EINVAL, "momentum warm start must be 0 or 1 (got %d)", mode);
+    b->m_warm = mode;
+    b->have_mprev[0] = b->have_mprev[1] = b->have_mprev[2] = 0;
+    return PV_OK;
+}
+
+int pv_set_momentum_cg_steps(pv_bag *b, int exact)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (exact < 0 || exact > 1) return pv_set_error(PV_EINVAL, "momentum CG steps must be 0 or 1 (got %d)", exact);
+    b->m_exact = exact;
+    return PV_OK;
+}
+
 int pv_set_ch_solver(pv_bag *b, int spectral)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");

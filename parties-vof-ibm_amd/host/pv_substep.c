@@ -333,6 +333,7 @@ static int mom_solve(pv_bag *b, int mask)
 {
     int st = b->stage, n = 0, maxn = b->ph.CG_MAXIT + 1, c, chunk;
     pvk_mcg w = b->mcg;
+    w.exact = b->m_exact;
     for (c = 0; c < 2; c++)
         if (!(mask & (1 << c))) w.nb[c] = 0;
     chunk = (mask & 1) ? b->mom_chunk[0] : b->mom_chunk[1];
@@ -1095,7 +1096,23 @@ int pv_Temporal_int_all_the_equations(pv_bag *b)
     pvk_psi_fsigma(G(b), &b->ph, FD(b, PV_F_C_L), FD(b, PV_F_C_G), NULL, FD(b, PV_F_PSI_LG), FD(b, PV_F_FSX_NEW),
                    FD(b, PV_F_FSY_NEW), S(b));
     LAUNCHCHK();
+    if (b->m_warm && b->have_mprev[st]) {
+        pvk_mom_warm(G(b), 0, b->dt / b->mprev_dt[st], FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_FI_MUS),
+                     FD(b, PV_FI_MVS), FD(b, PV_FI_MU0 + st), FD(b, PV_FI_MV0 + st), S(b));
+        /* the CG's first residual reads ghosts (periodic image, wall mirrors)
+         * of the start: make them those of the shifted start */
+        pvk_vel_bc(G(b), FD(b, PV_F_U), 'u', PV_VEL_TYPE_NORMAL, S(b));
+        pvk_vel_bc(G(b), FD(b, PV_F_V), 'v', PV_VEL_TYPE_NORMAL, S(b));
+    } else if (b->m_warm)
+        pvk_mom_warm(G(b), 0, 0.0, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_FI_MUS), FD(b, PV_FI_MVS),
+                     FD(b, PV_FI_MU0 + st), FD(b, PV_FI_MV0 + st), S(b));
     RC(mom_solve(b, 3));
+    if (b->m_warm) {
+        pvk_mom_warm(G(b), 1, 0.0, FD(b, PV_F_U), FD(b, PV_F_V), FD(b, PV_FI_MUS), FD(b, PV_FI_MVS),
+                     FD(b, PV_FI_MU0 + st), FD(b, PV_FI_MV0 + st), S(b));
+        b->have_mprev[st] = 1;
+        b->mprev_dt[st] = b->dt;
+    }
     pvk_vel_bc(G(b), FD(b, PV_F_U), 'u', PV_VEL_TYPE_NORMAL, S(b));
     pvk_vel_bc(G(b), FD(b, PV_F_V), 'v', PV_VEL_TYPE_NORMAL, S(b));
     LAUNCHCHK();
@@ -1160,6 +1177,7 @@ int pv_init_from_F(pv_bag *b, const double *F_host)
     LAUNCHCHK();
     memset(&b->st, 0, sizeof b->st);
     b->time = 0.0; b->dt = b->dt_old = b->prm.default_dt; b->stage = 0; b->ntime = 1; b->time_nonzero = 0;
+    b->have_mprev[0] = b->have_mprev[1] = b->have_mprev[2] = 0;
     memset(b->h_sc, 0, sizeof(pvk_scal));
     RC(pv_sync_scalars_to_device(b));
     RC(pv_Velocity_cell_center(b));

@@ -37,6 +37,7 @@ def load_oracle(path=None):
     lib.pvo_get_params.argtypes = [vp, C.POINTER(pvof.Params)]
     lib.pvo_init_from_F.argtypes = [vp, dp]
     lib.pvo_set_time_state.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
+    lib.pvo_set_momentum_cg_steps.argtypes = [vp, C.c_int]
     lib.pvo_get_stats.argtypes = [vp, C.POINTER(pvof.Stats)]
     lib.pvo_Temporal_int_rk3.argtypes = [vp, C.c_int]
     lib.pvo_Dtime_cfl.restype = C.c_double

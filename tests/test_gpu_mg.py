@@ -200,3 +200,33 @@ def test_temporal_warm_start(lib):
     R = regions(pm)
     for f in ("P", "U", "V", "C_L"):
         assert relerr(out[1][0].get(f)[R["cell"]], out[0][0].get(f)[R["cell"]]) < 1e-9, f
+
+
+def test_momentum_warm_start(lib):
+    """Starting each momentum CG from u^{k-1} plus the same RK3 stage's last
+    increment (scaled by the ratio of the time steps) leaves the stop test as
+    it is: the fields agree with the cold start to the solve tolerance, the
+    momentum solves take fewer iterations, and resetting the option restores
+    the cold start exactly."""
+    n = 128
+    pm = params(n, n)      # CG_ETOL 1e-7: a few iterations per solve, above the EPS-guard stall of msolve_cg.c
+    F = pvof.ic(lib, "bubbles", pm, seed=227, nmin=8, nmax=24, rmin=0.02, rmax=0.1, gas_inside=1)
+    out = {}
+    for warm in (0, 1):
+        s = pvof.Solver(pm, lib)
+        s.init_from_F(F)
+        s.check(lib.pv_set_momentum_warm_start(s.h, warm))
+        its = []
+        for _ in range(6):
+            s.call("Temporal_int_rk3", 1)
+            st = s.stats()
+            its.append(sum(st["n_u"]) + sum(st["n_v"]))
+        out[warm] = (s, its)
+    print(f"\nmomentum CG iterations per step: cold {out[0][1]}, warm {out[1][1]}")
+    assert out[1][1][0] == out[0][1][0]             # no increment kept yet in the first step
+    assert sum(out[1][1][2:]) < sum(out[0][1][2:])
+    R = regions(pm)
+    for f in ("U", "V", "P", "C_L"):
+        assert relerr(out[1][0].get(f)[R["cell"]], out[0][0].get(f)[R["cell"]]) < 1e-6, f
+    with pytest.raises(RuntimeError, match="momentum warm start"):
+        out[1][0].check(lib.pv_set_momentum_warm_start(out[1][0].h, 2))

@@ -238,6 +238,32 @@ def test_trajectory_10_steps(gpu, lib, solver):
         assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-11, fld
 
 
+def test_trajectory_momentum_exact_steps(gpu, lib):
+    """The momentum CG without the reference's EPS in its step lengths
+    (pv_set_momentum_cg_steps(1)) against the oracle's same variant: 5 RK3
+    steps of the rising bubble at a CG_ETOL (1e-9) where the reference's
+    steps stall, with the same iteration counts per solve."""
+    p = params_small(pressure_solver=pvof.PV_PSOLVE_FFT_PCG, p_rel_tol=1e-13, CG_ETOL=1e-9)
+    F = pvof.ic(lib, "rising_bubble", p)
+    o = oracle_lib.Oracle(p)
+    o.init_from_F(F)
+    o.lib.pvo_set_momentum_cg_steps(o.h, 1)
+    s = pvof.Solver(p, lib)
+    s.init_from_F(F)
+    s.check(lib.pv_set_momentum_cg_steps(s.h, 1))
+    for _ in range(5):
+        assert o.call("Temporal_int_rk3", 1) == 0
+        s.call("Temporal_int_rk3", 1)
+        so, ss = o.stats(), s.stats()
+        for k in ("n_u", "n_v"):
+            assert ss[k] == so[k] and max(so[k]) < p.CG_MAXIT, (k, ss[k], so[k])
+    R = regions(p)
+    for fld, r in (("C_L", "cell"), ("U", "u"), ("V", "v"), ("P", "cell")):
+        assert relerr(s.get(fld)[R[r]], o.get(fld)[R[r]]) < 1e-11, fld
+    with pytest.raises(RuntimeError, match="momentum CG steps"):
+        s.check(lib.pv_set_momentum_cg_steps(s.h, 2))
+
+
 @pytest.mark.parametrize("n,m", [(48, 96), (96, 80)])
 def test_trajectory_fft_pcg_mixed_radix(gpu, lib, n, m):
     """FFT-PCG on grids the power-of-two transforms cannot take: rows of

@@ -131,6 +131,12 @@ MUTANTS = [
      Q.test_ghost_exchange_is_the_periodic_extension, {"component": "d", "pnodes": 3}),
     ("comm_layers", "const int precv = ulike ? pnodes + 1 : pnodes", "const int precv = ulike ? pnodes : pnodes",
      Q.test_ghost_exchange_is_the_periodic_extension, {"component": "u", "pnodes": 2}),
+    # the momentum CG's exact-step variant: steepest descent instead of CG,
+    # and the reference's EPS left in the step length
+    ("velocity_cg_exact_beta", "if (s->mom_exact) beta = (rz != 0.0) ? lv0 / rz : 0.0;",
+     "if (s->mom_exact) beta = 0.0;", Q.test_velocity_cg_exact_steps, {"comp": "u"}),
+    ("velocity_cg_exact_alpha", "if (s->mom_exact) alpha = (dAd != 0.0) ? rz / dAd : 0.0;",
+     "if (s->mom_exact) alpha = rz / (dAd + EPS);", Q.test_velocity_cg_exact_steps, {"comp": "v"}),
 ]
 
 

@@ -561,6 +561,51 @@ def test_velocity_cg_solves_its_system(olib, comp):
     assert np.sqrt((res**2).mean()) < 2e-8
 
 
+@pytest.mark.parametrize("comp", ["u", "v"])
+def test_velocity_cg_exact_steps(olib, comp):
+    """The momentum CG with its step lengths taken without EPS (the library's
+    pv_set_momentum_cg_steps(1) variant) meets the same stop test on the
+    operator of matVec, checked by an independent residual, in a few
+    iterations; with the reference's EPS = 1e-14 (msolve_cg.c:674, :705) the
+    same system needs more than ten times as many, and on a small right-hand
+    side it stalls at CG_MAXIT above the tolerance.  With dt = 10 the
+    operator is nearly the viscous one (condition number ~ grid size squared):
+    there conjugate directions matter, and the count stays at CG's (steepest
+    descent needs about seven times as many)."""
+    out = {}
+    for exact in (0, 1):
+        for scale, tol, dt in ((100.0, 1e-8, 2e-3), (1e-2, 1e-10, 2e-3), (100.0, 1e-8, 10.0)):
+            if dt > 1.0 and not exact:
+                continue
+            p = pvof.config1_params(NXM=32, NYM=24, xmax=1.0, ymax=0.75, CG_ETOL=tol, CG_MAXIT=2000)
+            o = oracle_lib.Oracle(p, olib)
+            o.lib.pvo_set_momentum_cg_steps(o.h, exact)
+            rng = np.random.default_rng(61)
+            o.set("MU", rng.uniform(0.01, 1.0, o.shape))
+            o.set("RHO", rng.uniform(1e-3, 1.0, o.shape))
+            o.call("set_time_state", 0.0, dt, dt, 1)
+            R = interior(o.params, comp)
+            fld, rhs, dfl = ("U", "U_RHS", "U_D") if comp == "u" else ("V", "V_RHS", "V_D")
+            b = np.zeros(o.shape)
+            b[R] = scale * rng.standard_normal(b[R].shape)
+            o.set(rhs, b)
+            o.set(fld, np.zeros(o.shape))
+            it = o.call("Velocity_solve_cg", ord(comp))
+            o.set(dfl, o.get(fld))
+            o.call("Velocity_update_boundaries", pvof.F[dfl], pvof.PV_VEL_TYPE_CG)
+            o.call("matVec", pvof.F["SCRATCH0"], pvof.F[dfl], ord(comp))
+            rms = float(np.sqrt(((o.get("SCRATCH0")[R] - b[R]) ** 2).mean()))
+            out[exact, scale, dt] = (it, rms, tol)
+    for scale in (100.0, 1e-2):
+        it, rms, tol = out[1, scale, 2e-3]
+        assert it <= 10 and rms < tol, (scale, it, rms)
+        assert out[0, scale, 2e-3][0] > 10 * it, (scale, out[0, scale, 2e-3])
+    it, rms, tol = out[0, 1e-2, 2e-3]
+    assert it == 2000 and rms > tol
+    it, rms, tol = out[1, 100.0, 10.0]
+    assert it <= 120 and rms < tol, (it, rms)
+
+
 def test_pressure_rhs_is_scaled_divergence(olib):
     """Pressure_set_RHS (Pressure.c:100-196) is the cell divergence of the
     predicted velocity over 2 beta dt (alpha dt of the substep), and
