@@ -373,6 +373,23 @@ int pv_learn_train(pv_bag *bag, int steps, int batch, float lr, unsigned long lo
  * forms); >= 1: the candidate was discarded */
 int pv_learn_start_residual(pv_bag *bag, double *res);
 
+/* ---- no-allocation variant -----------------------------------------------
+ * No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): max |div u| of Pressure_compute_velocity_divergence
+ * (Pressure.c:475-597, as pv_Pressure_compute_velocity_divergence) on the
+ * caller's device arrays u and v of a uniform grid (idx = 1/dx, idy = 1/dy),
+ * laid out as this header's canonical fields with row pitch `pitch` (>= NXM+1)
+ * and pointers at cell (0, 0).  The result goes to the device double
+ * *div_max, in order on the HIP stream `stream` (NULL: the null stream); the
+ * call does not synchronise.  It allocates nothing: its block partials live
+ * in a fixed 8 KiB device array per GPU, which is why calls that overlap on
+ * one GPU race.  NXM and NYM in 1 .. PV_NOALLOC_MAX_N; the value equals the
+ * handle entry point's when the handle's metric arrays are idx and idy. */
+#define PV_NOALLOC_MAX_N 16384
+int pv_Pressure_compute_velocity_divergence_noalloc(const double *u, const double *v, int NXM, int NYM, int pitch,
+                                                    double idx, double idy, double *div_max, void *stream);
+
 /* ---- initial volume fraction F on the G range (host only, no device) ----
  * 6 x 6 sub-sampled tanh profile of width 2 sqrt(2) Cn, as in
  * IO/VoF_Init.c:877-953 / :1032-1100 (F = 1 liquid, 0 gas).  The seeded

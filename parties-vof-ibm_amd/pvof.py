@@ -22,6 +22,7 @@ PV_PSOLVE_MG_PCG = 4
 PV_PSOLVE_LEARNED_MG = 5
 PV_PREC_FP64 = 0
 PV_PREC_MIXED = 1
+PV_NOALLOC_MAX_N = 16384
 PV_VEL_TYPE_NORMAL = 0
 PV_VEL_TYPE_CG = 1
 
@@ -141,6 +142,8 @@ def load_lib(path=LIB_PATH):
     lib.pv_init_from_F.argtypes = [vp, dp]
     lib.pv_set_time_state.argtypes = [vp, C.c_double, C.c_double, C.c_double, C.c_int]
     lib.pv_set_step_count.argtypes = [vp, C.c_int]
+    lib.pv_Pressure_compute_velocity_divergence_noalloc.argtypes = [vp, vp, C.c_int, C.c_int, C.c_int, C.c_double,
+                                                                     C.c_double, vp, vp]
     lib.pv_get_stats.argtypes = [vp, C.POINTER(Stats)]
     lib.pv_Temporal_int_rk3.argtypes = [vp, C.c_int]
     lib.pv_Dtime_cfl.argtypes = [vp, dp]
