@@ -281,7 +281,7 @@ typedef struct pvk_mg {
     const double *r0;                 /* FP32: level-0 right-hand side (r), pitch p0 */
     double *z0;                       /* FP32: level-0 result (z), pitch p0 */
     int p0;
-    double *G;                        /* bottom inverse, nb x nb */
+    double *G;                        /* bottom matrix P G P (G the grounded inverse), nb x nb */
 } pvk_mg;
 void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);
 int  pvk_mg_nrz(const pvk_mg *m);

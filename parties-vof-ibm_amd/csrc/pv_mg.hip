@@ -336,10 +336,12 @@ __global__ void __launch_bounds__(NTH) k_mg_faces0(Lev<double> L, Lev<float> C)
     C.by[(ptrdiff_t)J * C.pitch + I] = (float)ld(L.by, L.pitch, I, J);
 }
 
-// inverse of the bottom operator -A_B + s e0 e0^T (s = its (0,0) entry),
-// in FP64: symmetric positive definite, in-place Gauss-Jordan without
-// pivoting; one workgroup of 1024 threads, thread (r, q) owns row r, columns
-// q + 16 t
+// H = P G P, G the inverse of the bottom operator -A_B + s e0 e0^T (s = its
+// (0,0) entry) and P the projection to zero mean, in FP64: the bottom solve
+// is then x = -H f.  G by in-place Gauss-Jordan without pivoting (the matrix
+// is symmetric positive definite); one workgroup of 1024 threads, thread
+// (r, q) owns row r, columns q + 16 t.  (P G P)_rc = G_rc - (row mean)_r -
+// (column mean)_c + (mean of the column means).
 #define BT 1024
 template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B, double *G)
 {
@@ -383,8 +385,22 @@ template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B
         }
         __syncthreads();
     }
+    __shared__ double rm[PVK_MG_BOTTOM], cm[PVK_MG_BOTTOM];
+    if (threadIdx.x < (unsigned)nb) {
+        double a = 0.0, b = 0.0;
+        for (int c = 0; c < nb; c++) {
+            a += M[threadIdx.x][c];
+            b += M[c][threadIdx.x];
+        }
+        rm[threadIdx.x] = a / nb;
+        cm[threadIdx.x] = b / nb;
+    }
+    __syncthreads();
+    double tm = 0.0;
+    for (int c = 0; c < nb; c++) tm += cm[c];
+    tm /= nb;
     if (r < nb)
-        for (int c = q; c < nb; c += 16) G[r * nb + c] = M[r][c];
+        for (int c = q; c < nb; c += 16) G[r * nb + c] = ((M[r][c] - rm[r]) - cm[c]) + tm;
 }
 
 // ---------------------------------------------------------------------------
@@ -608,24 +624,51 @@ template <typename T> __device__ __forceinline__ T *lds_scratch(const pvk_mg &m,
     return q;
 }
 
+// the bottom matrix in LDS, transposed with row stride HS, after the
+// restriction scratch, 8-byte aligned
+template <typename T> __device__ __forceinline__ double *lds_bottom(const pvk_mg &m, T *lds)
+{
+    T *q = lds_scratch(m, lds);
+    if (m.nlev > m.c0 + 1) q += (m.L[m.c0].nx / 2) * m.L[m.c0].ny;
+    return (double *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
+}
+
+// bottom level, x = -H f (H = P G P, FP64), H's transpose in LDS (entry
+// (r, c) at Hs[c * HS + r]; the padded stride spreads the sixteen column
+// reads of a row over the banks).  Row r is applied by the 16 lanes
+// 16 r .. 16 r + 15, four columns each.
+#define HS (PVK_MG_BOTTOM + 1)
+template <typename T> __device__ __forceinline__ void bottom_solve(const LdsLev<T> &B, const double *Hs)
+{
+    const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
+    double v = 0.0;
+#pragma unroll
+    for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+        const int c = q + 16 * t;
+        if (r < n && c < n) v -= Hs[c * HS + r] * (double)B.f[c];
+    }
+    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
+    if (q == 0 && r < n) B.x[r] = (T)v;
+    __syncthreads();
+}
+
 template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m, pvk_scal *sc)
 {
     extern __shared__ double lds_raw[];
-    __shared__ double vbuf[PVK_MG_BOTTOM];
     __shared__ LdsLev<T> lv[PVK_MG_LOW + 1];      // levels c0 .. nb
     T *lds = (T *)lds_raw;
     const int nb = m.nlev - 1;
     // The loads below are valid whatever the stop flag says, so they are
     // issued before it is read: one memory round trip instead of two.
-    // This thread's entries of the bottom inverse (row threadIdx / 16,
+    // This thread's entries of the bottom matrix H (row threadIdx / 16,
     // columns threadIdx % 16 + 16 t), loaded first: their latency hides
-    // behind the smoothing.
+    // behind the rest.
     const int nbot = m.L[nb].nx * m.L[nb].ny, br = threadIdx.x >> 4, bq = threadIdx.x & 15;
-    double gv[PVK_MG_BOTTOM / 16];
+    double hv[PVK_MG_BOTTOM / 16];
 #pragma unroll
     for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
         const int c = bq + 16 * t;
-        gv[t] = (br < nbot && c < nbot) ? m.G[(size_t)br * nbot + c] : 0.0;
+        hv[t] = (br < nbot && c < nbot) ? m.G[(size_t)br * nbot + c] : 0.0;
     }
     // f of level c0 (and its faces with topf), the faces of the levels below
     // it: every global load of this thread is issued before the first LDS
@@ -660,7 +703,13 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     // the level descriptors, formed once: the schedule below reads them from
     // LDS instead of walking the kernel argument with a run-time index
     if (threadIdx.x <= (unsigned)(nb - m.c0)) lv[threadIdx.x] = lds_level(m, lds, m.c0 + (int)threadIdx.x);
+    double *Hs = lds_bottom(m, lds);               // H's transpose
     {
+#pragma unroll
+        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
+            const int c = bq + 16 * t;
+            if (br < nbot && c < nbot) Hs[c * HS + br] = hv[t];
+        }
         const LdsLev<T> L0 = lds_level(m, lds, m.c0);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -696,32 +745,7 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     unsigned second = 0u;
     for (;;) {
         if (l == nb) {
-            // bottom: x = -P G P f, P the projection to zero mean (in FP64).
-            // Row r of G is applied by the 16 lanes 16 r .. 16 r + 15; every
-            // group of 16 lanes forms the two means itself (same order, same
-            // values), so the bottom needs two workgroup barriers.
-            {
-                const LdsLev<T> B = lv[nb - m.c0];
-                const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
-                double s = 0.0;
-                for (int k = q; k < n; k += 16) s += (double)B.f[k];
-                for (int off = 8; off > 0; off >>= 1) s += __shfl_xor(s, off, 16);
-                const double mf = s / n;
-                double v = 0.0;
-#pragma unroll
-                for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-                    const int c = q + 16 * t;
-                    if (r < n && c < n) v -= gv[t] * ((double)B.f[c] - mf);
-                }
-                for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
-                if (q == 0 && r < n) vbuf[r] = v;
-                __syncthreads();
-                double sv = 0.0;
-                for (int k = q; k < n; k += 16) sv += vbuf[k];
-                for (int off = 8; off > 0; off >>= 1) sv += __shfl_xor(sv, off, 16);
-                if (q == 0 && r < n) B.x[r] = (T)(v - sv / n);
-                __syncthreads();
-            }
+            bottom_solve(lv[nb - m.c0], Hs);
             l--;
             down = false;
             if (l < m.c0) break;
@@ -846,7 +870,8 @@ extern "C" size_t pvk_mg_lds_bytes(const pvk_mg *m)
     size_t n = 0;
     for (int l = m->c0; l < m->nlev; l++) n += ((l == m->c0 && !m->topf) ? 2 : 4) * (size_t)m->L[l].nx * m->L[l].ny;
     if (m->nlev > m->c0 + 1) n += (size_t)(m->L[m->c0].nx / 2) * m->L[m->c0].ny;   // restriction scratch
-    return n * (m->fp32 ? sizeof(float) : sizeof(double));
+    // + the bottom matrix (FP64, 8-byte aligned, row stride HS)
+    return n * (m->fp32 ? sizeof(float) : sizeof(double)) + 8 + sizeof(double) * PVK_MG_BOTTOM * HS;
 }
 
 extern "C" int pvk_mg_nrz(const pvk_mg *m)
