@@ -77,8 +77,8 @@ BYTES_PER_ITER = {"fft": 144, "mg": 189, "mg32": 155, "jacobi": 112}
 # the 1024^2 MG-PCG mixed form (FETCH_SIZE scaled by 2.0, WRITE_SIZE by 1.0:
 # the factors a 256 MiB device copy gives in runs of its own); the summary is
 # profiles/mg_pcg_mixed_1024_pmc_traffic.md
-TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.2, "MG_POST0": 32.0, "P_APPLY": 51.5,
-                        "MOM_APPLY": 122.9, "MOM_UPDATE": 118.9, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
+TRAFFIC_MB_MG32_1024 = {"MG_PRE0": 77.1, "MG_POST0": 32.0, "P_APPLY": 51.5,
+                        "MOM_APPLY": 133.4, "MOM_UPDATE": 119.0, "CH_RHS": 69.2, "MOM_TERMS": 111.0}
 SEED = 20261202
 # the headline metric: the timed form is the learned-start mixed form, the plain
 # FP64 path is reported beside it (compare.plain_fp64), the dominant kernel's
@@ -99,13 +99,14 @@ def workload_params(n=1024, solver=pvof.PV_PSOLVE_FFT_PCG, precision=pvof.PV_PRE
         precision=precision)
 
 
-def step_bytes_per_cell(st, family):
+def step_bytes_per_cell(st, family, mixed=False):
     """Algorithmic bytes per cell for one RK3 step from the solver counts
-    (fixed 563 B per substep + 88 per biharmonic, 136 per momentum and
-    BYTES_PER_ITER per pressure iteration)."""
-    total = 0
+    (fixed 563 B per substep + 88 per biharmonic, 136 per momentum (140 in the
+    mixed form, whose operator kernel reads five FP32 coefficients instead of
+    rho and mu) and BYTES_PER_ITER per pressure iteration)."""
+    total, mom = 0, (140 if mixed else 136)
     for k in range(3):
-        total += 563 + 88 * st["n_ch"][k] + 136 * (st["n_u"][k] + st["n_v"][k]) + BYTES_PER_ITER[family] * st["n_p"][k]
+        total += 563 + 88 * st["n_ch"][k] + mom * (st["n_u"][k] + st["n_v"][k]) + BYTES_PER_ITER[family] * st["n_p"][k]
     return total
 
 
@@ -222,6 +223,10 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     bpc = {"jacobi": BYTES_PER_CELL, "fft": BYTES_PER_CELL_FFT, "mg": BYTES_PER_CELL_MG,
            "mg32": BYTES_PER_CELL_MG32}[fam]
     cells = p.NXM * p.NYM
+    if precision == "mixed":
+        # the FP32 operator reads r, diag, d_old and five FP32 coefficients per
+        # component (formed by the start kernel) instead of rho and mu
+        bpc = dict(bpc, MOM_APPLY=60)
     if not any(sum(st["n_ch"]) for st in stats):
         # spectral CH start: the CH CG launches only take the stop test (no cell work)
         bpc = {k: v for k, v in bpc.items() if k not in ("CH_APPLY", "CH_UPDATE")}
@@ -241,7 +246,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
             kernels[name] = {"timed_launches": int(cnt[k]), "avg_us": round(1e3 * avg_ms, 3),
                              "est_total_ms": round(ms[k] * max(1, args.timer_period), 3),
                              "GBs": (round(bpc[name] * cells / (avg_ms * 1e-3) / 1e9, 1) if name in bpc else None)}
-    bytes_step = sum(step_bytes_per_cell(st, fam) for st in stats) / len(stats)
+    bytes_step = sum(step_bytes_per_cell(st, fam, precision == "mixed") for st in stats) / len(stats)
     out = {"solver": solver, "precision": precision, "steps": steps, "warmup": warmup,
            "settings": o if family_of(solver) == "mg" else None,
            "value": round(cells * steps / elapsed / 1e6, 3), "ms_per_step": round(1e3 * elapsed / steps, 3),
@@ -398,8 +403,9 @@ def config2_run(lib, args):
     side needs.  Throughput, iteration counts, the phase-volume change, the
     clamp corrections as the reference reports them (summed over its G range,
     which includes the duplicated periodic column and the ghost row above the
-    top wall, VOF_DIFFUSE.c:1093-1127), the difference of the two, the
-    largest divergence and finiteness."""
+    top wall, VOF_DIFFUSE.c:1093-1127) and over the interior cells alone, the
+    differences of each from the volume change, the largest divergence and
+    finiteness."""
     p = workload_params(512, pvof.PV_PSOLVE_FFT_PCG, pvof.PV_PREC_FP64)
     for k, v in dict(NYM=1024, ymax=2.0, Re=3000.0, rho2=1.0 / 3.0, mu2=1.0, Cn=0.005, Pe_CH=200.0,
                      default_dt=1e-4).items():
@@ -411,7 +417,7 @@ def config2_run(lib, args):
     pp = s.params
     meas = (pp.xmax - pp.xmin) / p.NXM * (pp.ymax - pp.ymin) / p.NYM * pp.twod_slab_thickness
     vol0 = s.get("C_L")[R].sum() * meas
-    clamp, iters, div = 0.0, {"n_ch": 0, "n_u": 0, "n_v": 0, "n_p": 0}, 0.0
+    clamp, clamp_int, iters, div = 0.0, 0.0, {"n_ch": 0, "n_u": 0, "n_v": 0, "n_p": 0}, 0.0
     steps, done = args.config2_steps, 0
     t = 0.0
     while done < steps:
@@ -421,6 +427,7 @@ def config2_run(lib, args):
             s.call("Temporal_int_rk3", 1)
             st = s.stats()
             clamp += sum(st["clamp_delta"])
+            clamp_int += sum(st["clamp_delta_interior"])
             div = max(div, max(st["div_max"]))
             for key in iters:
                 iters[key] += sum(st[key])
@@ -436,6 +443,8 @@ def config2_run(lib, args):
             "iterations_per_step": {k: round(v / steps, 2) for k, v in iters.items()},
             "phase_volume_change_rel": float((vol - vol0) / vol0), "clamp_total_rel": float(clamp / vol0),
             "phase_volume_change_less_clamp_rel": float(abs((vol - vol0) - clamp) / vol0),
+            "clamp_interior_total_rel": float(clamp_int / vol0),
+            "phase_volume_change_less_interior_clamp_rel": float(abs((vol - vol0) - clamp_int) / vol0),
             "max_divergence": div, "finite": finite, "precision": "fp64", "pressure_solver": "fft"}
 
 

@@ -278,6 +278,12 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
         double ad = (double)mom_op<R>(g, ph, alpha_k, dt, w.rho, w.mu, x, comp, i, j);
         double rr = AT(rhs, i, j) - ad;
         double dgv = (double)mom_diag<R>(g, ph, alpha_k, dt, w.rho, w.mu, comp, i, j) + 1e-14;
+        if constexpr (sizeof(R) == sizeof(float)) {     // the FP32 iterations read these instead of rho, mu
+            float cc[5];
+            mom_coeffs(g, ph, alpha_k, dt, w.rho, w.mu, m.comp, i, j, cc);
+#pragma unroll
+            for (int k = 0; k < 5; k++) AT(w.cf[m.comp][k], i, j) = cc[k];
+        }
         double dd = rr / dgv;
         AT(Ad, i, j) = ad;
         AT(r, i, j) = rr;
@@ -364,9 +370,17 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
             const int a = q / MCH, bb = q - a * MCH, j = m.j0 + a, i = m.i0 + cb + bb;
             if (cb + bb < NXM) {
                 const double dc = sd[a + 1][bb + 1];
-                const double ad = (double)mom_op_vals<R>(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, R(dc),
-                                                         R(sd[a + 1][bb + 2]), R(sd[a + 1][bb]), R(sd[a + 2][bb + 1]),
-                                                         R(sd[a][bb + 1]));
+                double ad;
+                if constexpr (sizeof(R) == sizeof(float)) {
+                    float cc[5];
+#pragma unroll
+                    for (int k = 0; k < 5; k++) cc[k] = AT(w.cf[c][k], i, j);
+                    ad = (double)mom_op_coeffs(cc, float(dc), float(sd[a + 1][bb + 2]), float(sd[a + 1][bb]),
+                                               float(sd[a + 2][bb + 1]), float(sd[a][bb + 1]));
+                } else {
+                    ad = (double)mom_op_vals<R>(g, ph, alpha_k, dt, w.rho, w.mu, c, i, j, R(dc), R(sd[a + 1][bb + 2]),
+                                                R(sd[a + 1][bb]), R(sd[a + 2][bb + 1]), R(sd[a][bb + 1]));
+                }
                 if (n > 0) AT(dnew, i, j) = dc;
                 AT(Ad, i, j) = ad;
                 acc += dc * ad;

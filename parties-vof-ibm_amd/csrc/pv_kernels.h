@@ -117,7 +117,7 @@ typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
     int npap;                        /* <p, Ap> partials: one per 128 x 8 tile */
 } pvk_ccg;
 
-#define PVK_MOM_BAND 4                /* rows per block of the momentum CG kernels */
+#define PVK_MOM_BAND 2                /* rows per block of the momentum CG kernels */
 typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double *x[2], *r[2], *dg[2], *Ad[2];
     double *d[2][2];                 /* [component][n & 1]; d[c][0] is the public field */
@@ -126,6 +126,8 @@ typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double *rz[2][2], *dad[2], *rr[2];
     int nb[2], band;                 /* nb[c] = 0 leaves component c out */
     int maxit;
+    float *cf[2][5];                 /* FP32 form: operator coefficients (centre, E, W, N, S) of each
+                                        unknown, pitch P, formed by the start kernel */
     int exact;                       /* 0: EPS = 1e-14 added to the step denominators (the
                                         reference); 1: the denominators as they are, a zero
                                         one giving a zero step */

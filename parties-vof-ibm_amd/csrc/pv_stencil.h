@@ -245,6 +245,48 @@ __device__ __forceinline__ R mom_op_vals(const pvk_grid &g, const pvk_phys &ph, 
     return Ax_val - op;
 }
 
+// The operator's coefficients at (i, j) in FP32 (centre, E, W, N, S):
+// A x = c0 x - ((cE (xE - x) - cW (x - xW)) + (cN (xN - x) - cS (x - xS))),
+// the products of mom_op_vals with the x differences factored out.
+__device__ __forceinline__ void mom_coeffs(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
+                                           const double *rho, const double *mu, int comp, int i, int j, float c[5])
+{
+    const int P = g.P;
+    const float iRe = float(ph.iRe);
+    if (comp == 0) {
+        const float rho_face = 0.5f * (float(AT(rho, i, j)) + float(AT(rho, i - 1, j)));
+        c[0] = rho_face / (float(alpha_k) * float(dt));
+        const float mu_xp = float(AT(mu, i, j)), mu_xm = float(AT(mu, i - 1, j));
+        const float mu_yp = harm<float>(avg2<float>(float(AT(mu, i - 1, j + 1)), float(AT(mu, i, j + 1))),
+                                        avg2<float>(float(AT(mu, i - 1, j)), float(AT(mu, i, j))));
+        const float mu_ym = harm<float>(avg2<float>(float(AT(mu, i - 1, j)), float(AT(mu, i, j))),
+                                        avg2<float>(float(AT(mu, i - 1, j - 1)), float(AT(mu, i, j - 1))));
+        const float sx = iRe * float(g.idx_c[i - 1]), sy = iRe * float(g.idy_v[j]);
+        c[1] = sx * (mu_xp * float(g.idx_u[i]));
+        c[2] = sx * (mu_xm * float(g.idx_u[i - 1]));
+        c[3] = sy * (mu_yp * float(g.idy_c[j]));
+        c[4] = sy * (mu_ym * float(g.idy_c[j - 1]));
+        return;
+    }
+    const float rho_face = 0.5f * (float(AT(rho, i, j)) + float(AT(rho, i, j - 1)));
+    c[0] = rho_face / (float(alpha_k) * float(dt));
+    const float mu_yp = float(AT(mu, i, j)), mu_ym = float(AT(mu, i, j - 1));
+    const float mu_xp = harm<float>(avg2<float>(float(AT(mu, i + 1, j - 1)), float(AT(mu, i + 1, j))),
+                                    avg2<float>(float(AT(mu, i, j - 1)), float(AT(mu, i, j))));
+    const float mu_xm = harm<float>(avg2<float>(float(AT(mu, i, j - 1)), float(AT(mu, i, j))),
+                                    avg2<float>(float(AT(mu, i - 1, j - 1)), float(AT(mu, i - 1, j))));
+    const float sx = iRe * float(g.idx_u[i]), sy = iRe * float(g.idy_c[j - 1]);
+    c[1] = sx * (mu_xp * float(g.idx_c[i]));
+    c[2] = sx * (mu_xm * float(g.idx_c[i - 1]));
+    c[3] = sy * (mu_yp * float(g.idy_v[j]));
+    c[4] = sy * (mu_ym * float(g.idy_v[j - 1]));
+}
+
+__device__ __forceinline__ float mom_op_coeffs(const float c[5], float xc, float xE, float xW, float xN, float xS)
+{
+    return c[0] * xc - ((c[1] * (xE - xc) - c[2] * (xc - xW)) + (c[3] * (xN - xc) - c[4] * (xc - xS)));
+}
+
 template <class R = double>
 __device__ __forceinline__ R mom_op(const pvk_grid &g, const pvk_phys &ph, double alpha_k, double dt,
                                     const double *rho, const double *mu, const double *x, int comp, int i, int j)

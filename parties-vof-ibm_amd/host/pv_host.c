@@ -314,6 +314,10 @@ int pv_cg_create(pv_bag *b)
         b->mcg.dad[c] = q; q += nbk;
         b->mcg.rr[c] = q; q += nbk;
         b->mcg.nb[c] = nbk;
+        /* coefficient k of component c: half (k & 1) of field MCF(3c + k / 2),
+         * each half NYM rows of pitch P floats from the field's cell (0, 0) */
+        for (int k = 0; k < 5; k++)
+            b->mcg.cf[c][k] = (float *)b->f[PV_FI_MCF0 + 3 * c + k / 2] + (size_t)(k & 1) * ny * b->g.P;
     }
     return PV_OK;
 }

@@ -30,6 +30,12 @@ enum pv_internal_field {
     PV_FI_MV2,
     PV_FI_MUS,                 /* u and v before the momentum solve */
     PV_FI_MVS,
+    PV_FI_MCF0,                /* FP32 momentum operator coefficients, two arrays per field: */
+    PV_FI_MCF1,                /* u in MCF0..2, v in MCF3..5 (five used of six each) */
+    PV_FI_MCF2,
+    PV_FI_MCF3,
+    PV_FI_MCF4,
+    PV_FI_MCF5,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
