@@ -28,8 +28,73 @@
 // (VOF_DIFFUSE.c:571-683; all ghost layers equal the first interior value).
 __device__ __forceinline__ int chy(int j, int n) { return j < 0 ? 0 : (j >= n ? n - 1 : j); }
 
-// Explicit CH right-hand side and CG start (the body of k_ch_rhs_init) in
-// band form: rc -> rn, r = p = b - A C; partials ||b||^2, ||r||^2.
+// column index of the x-periodic CH map for i in [-NXM, 3 NXM)
+__device__ __forceinline__ int chx(int i, int n)
+{
+    while (i < 0) i += n;
+    while (i >= n) i -= n;
+    return i;
+}
+
+// The CG start kernels work on tiles of CIT_X x CIT_Y cells: C on the tile
+// with a three-cell halo through the CH ghost maps, then on the tile with a
+// one-cell halo S(C) and the inner Laplacian of C (CH ghost rules: column
+// NXM is column 0, rows -1 / NYM are rows 0 / NYM-1), and the WENO face
+// fluxes of the tile's faces, each formed once in LDS instead of once per
+// stencil use.  Every value is the arithmetic of the per-cell helpers
+// (weno_rhs_at, diffusion_at, bilap_at) on the same operands, so the result
+// is bitwise that of the untiled form.  One <b, b> and <r, r> partial per
+// tile (w.nt0 of them).
+#define CIT_X 64
+#define CIT_Y 16
+#define CIT_NX (CIT_X + 6)
+
+template <class R> struct CitLds {
+    double c[CIT_Y + 6][CIT_NX];           // C, halo 3
+    double l[CIT_Y + 2][CIT_X + 2];        // inner Laplacian of C, halo 1
+    R s[CIT_Y + 2][CIT_X + 2];             // S(C), halo 1
+    double fx[CIT_Y][CIT_X + 1];           // u C_face at x face c0 + b of row j0 + a
+    double fy[CIT_Y + 1][CIT_X];           // v C_face at y face j0 + a of column c0 + b
+};
+
+// tile of this block (XCD-aware order), its origin, C and the inner Laplacian
+__device__ __forceinline__ int cit_tile(int &c0, int &j0)
+{
+    const int t = xcd_band(blockIdx.y * gridDim.x + blockIdx.x, gridDim.x * gridDim.y);
+    const int ty = t / gridDim.x, tx = t - ty * gridDim.x;
+    c0 = tx * CIT_X;
+    j0 = ty * CIT_Y;
+    return t;
+}
+template <class R>
+__device__ __forceinline__ void cit_load(const pvk_grid &g, const double *C, int c0, int j0, CitLds<R> &T, bool with_s)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    for (int q = threadIdx.x; q < (CIT_Y + 6) * CIT_NX; q += NTH) {
+        const int a = q / CIT_NX, b = q - a * CIT_NX;
+        T.c[a][b] = AT(C, chx(c0 + b - 3, NXM), chy(j0 + a - 3, NYM));
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < (CIT_Y + 2) * (CIT_X + 2); q += NTH) {
+        const int a = q / (CIT_X + 2), b = q - a * (CIT_X + 2), jb = j0 + a - 1;
+        if (with_s) T.s[a][b] = bulkS<R>(R(T.c[a + 2][b + 2]));
+        if (jb >= 0 && jb < NYM)
+            T.l[a][b] = lap_vals(g, chx(c0 + b - 1, NXM), jb, T.c[a + 2][b + 2], T.c[a + 2][b + 3], T.c[a + 2][b + 1],
+                                 T.c[a + 3][b + 2], T.c[a + 1][b + 2]);
+    }
+}
+// L(L C) at tile cell (tr, tc) = global (i, j)
+template <class R>
+__device__ __forceinline__ double cit_bilap(const pvk_grid &g, const CitLds<R> &T, int tr, int tc, int i, int j)
+{
+    const double L0 = T.l[tr + 1][tc + 1];
+    const double LN = (j + 1 == g.NYM) ? L0 : T.l[tr + 2][tc + 1];
+    const double LS = (j == 0) ? L0 : T.l[tr][tc + 1];
+    return lap_vals(g, i, j, L0, T.l[tr + 1][tc + 2], T.l[tr + 1][tc], LN, LS);
+}
+
+// Explicit CH right-hand side and CG start: rc -> rn, r = p = b - A C;
+// partials ||b||^2, ||r||^2.
 template <class R>
 __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc,
                                                      const double *__restrict__ C, const double *__restrict__ u,
@@ -38,32 +103,55 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_rhs(pvk_grid g, pvk_phys ph, i
                                                      double *__restrict__ bout)
 {
     __shared__ double sh[NTH];
-    const int P = g.P, NXM = g.NXM;
-    const int band = xcd_band(blockIdx.x, gridDim.x);
-    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    __shared__ CitLds<R> T;
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM, wo = ph.weno_order;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    int c0, j0;
+    const int t = cit_tile(c0, j0);
+    cit_load<R>(g, C, c0, j0, T, true);
+    // face fluxes (the C tile is complete after cit_load's first barrier)
+    for (int q = threadIdx.x; q < CIT_Y * (CIT_X + 1); q += NTH) {
+        const int a = q / (CIT_X + 1), b = q - a * (CIT_X + 1), i = c0 + b, j = j0 + a;
+        if (i <= NXM && j < NYM) {
+            const double vel = AT(u, i, j);
+            const double *cr = &T.c[a + 3][b];                 // cr[k] = C at column i - 3 + k
+            T.fx[a][b] = vel * (double)face_vals<R>(cr[0], cr[1], cr[2], cr[3], cr[4], cr[5], vel, wo);
+        }
+    }
+    for (int q = threadIdx.x; q < (CIT_Y + 1) * CIT_X; q += NTH) {
+        const int a = q / CIT_X, b = q - a * CIT_X, i = c0 + b, j = j0 + a;
+        if (i < NXM && j <= NYM) {
+            const double vel = AT(v, i, j);
+            T.fy[a][b] = vel * (double)face_vals<R>(T.c[a][b + 3], T.c[a + 1][b + 3], T.c[a + 2][b + 3],
+                                                    T.c[a + 3][b + 3], T.c[a + 4][b + 3], T.c[a + 5][b + 3], vel, wo);
+        }
+    }
+    __syncthreads();
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
     double sbb = 0.0, srr = 0.0;
-    for (int j = j0; j < j1; j++)
-    for (int i = threadIdx.x; i < NXM; i += NTH) {
-        double rc = weno_rhs_at<R>(g, ph, C, u, v, i, j);
-        rc += diffusion_at<R>(g, ph, C, i, j);
-        double cc = AT(C, i, j);
-        double rv = (stage == 0) ? a_dt * cc + c_GAMB[0] * rc
-                                 : a_dt * cc + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
-        double L2 = bilap_at(g, C, i, j);
-        double b = rv - ph.bih * L2;
-        double Ap = a_dt * cc + ph.bih * L2;
-        double rr = b - Ap;
-        if (bout) AT(bout, i, j) = b;
-        AT(rn, i, j) = rc;
-        AT(r, i, j) = rr;
-        AT(p, i, j) = rr;
-        sbb += ch_w(g, i, j, b, b);
-        srr += ch_w(g, i, j, rr, rr);
+    for (int q = threadIdx.x; q < CIT_X * CIT_Y; q += NTH) {
+        const int tr = q / CIT_X, tc = q - tr * CIT_X, i = c0 + tc, j = j0 + tr;
+        if (i < NXM && j < NYM) {
+            double rc = weno_div(g, i, j, T.fx[tr][tc], T.fx[tr][tc + 1], T.fy[tr][tc], T.fy[tr + 1][tc]);
+            rc += diffusion_vals<R>(g, ph, i, j, T.s[tr + 1][tc + 1], T.s[tr + 1][tc + 2], T.s[tr + 1][tc],
+                                    T.s[tr + 2][tc + 1], T.s[tr][tc + 1]);
+            double cc = T.c[tr + 3][tc + 3];
+            double rv = (stage == 0) ? a_dt * cc + c_GAMB[0] * rc
+                                     : a_dt * cc + c_GAMB[stage] * rc + c_ZETB[stage] * AT(rn, i, j);
+            double L2 = cit_bilap<R>(g, T, tr, tc, i, j);
+            double b = rv - ph.bih * L2;
+            double Ap = a_dt * cc + ph.bih * L2;
+            double rr = b - Ap;
+            if (bout) AT(bout, i, j) = b;
+            AT(rn, i, j) = rc;
+            AT(r, i, j) = rr;
+            AT(p, i, j) = rr;
+            sbb += ch_w(g, i, j, b, b);
+            srr += ch_w(g, i, j, rr, rr);
+        }
     }
-    block_store_partial(sbb, w.bb, band, sh);
-    block_store_partial(srr, w.rr[0], band, sh);
+    block_store_partial(sbb, w.bb, t, sh);
+    block_store_partial(srr, w.rr[0], t, sh);
 }
 
 // CG start from a given b (b already holds rhs - (Cn^2/Pe) L^2 C).
@@ -72,25 +160,30 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int
                                                    double *__restrict__ r, double *__restrict__ p, pvk_ccg w)
 {
     __shared__ double sh[NTH];
-    const int P = g.P, NXM = g.NXM;
-    const int band = xcd_band(blockIdx.x, gridDim.x);
-    const int j0 = band * w.band, j1 = min(g.NYM, j0 + w.band), ncell = (j1 - j0) * NXM;
-    if (blockIdx.x == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    __shared__ CitLds<double> T;
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    if (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0) { sc->ch_done = 0; sc->ch_iter = 0; }
+    int c0, j0;
+    const int t = cit_tile(c0, j0);
+    cit_load<double>(g, C, c0, j0, T, false);
+    __syncthreads();
     const double a_dt = 1.0 / (sc->dt * c_BET[stage]);
     double sbb = 0.0, srr = 0.0;
-    for (int j = j0; j < j1; j++)
-    for (int i = threadIdx.x; i < NXM; i += NTH) {
-        double cc = AT(C, i, j);
-        double Ap = a_dt * cc + ph.bih * bilap_at(g, C, i, j);
-        double bb = AT(bv, i, j);
-        double rr = bb - Ap;
-        AT(r, i, j) = rr;
-        AT(p, i, j) = rr;
-        sbb += ch_w(g, i, j, bb, bb);
-        srr += ch_w(g, i, j, rr, rr);
+    for (int q = threadIdx.x; q < CIT_X * CIT_Y; q += NTH) {
+        const int tr = q / CIT_X, tc = q - tr * CIT_X, i = c0 + tc, j = j0 + tr;
+        if (i < NXM && j < NYM) {
+            double cc = T.c[tr + 3][tc + 3];
+            double Ap = a_dt * cc + ph.bih * cit_bilap<double>(g, T, tr, tc, i, j);
+            double bb = AT(bv, i, j);
+            double rr = bb - Ap;
+            AT(r, i, j) = rr;
+            AT(p, i, j) = rr;
+            sbb += ch_w(g, i, j, bb, bb);
+            srr += ch_w(g, i, j, rr, rr);
+        }
     }
-    block_store_partial(sbb, w.bb, band, sh);
-    block_store_partial(srr, w.rr[0], band, sh);
+    block_store_partial(sbb, w.bb, t, sh);
+    block_store_partial(srr, w.rr[0], t, sh);
 }
 
 // K1: stop test on ||r_n|| (VOF_DIFFUSE.c:403-414), beta, p_n = r + beta p_{n-1}
@@ -102,14 +195,6 @@ __global__ void __launch_bounds__(NTH) k_ccg_init_b(pvk_grid g, pvk_phys ph, int
 // (inner Laplacian with the CH ghost rules, then the outer one).
 #define CHT_X 128
 #define CHT_Y 8
-// column index of the x-periodic CH map for i in [-NXM, 3 NXM)
-__device__ __forceinline__ int chx(int i, int n)
-{
-    while (i < 0) i += n;
-    while (i >= n) i -= n;
-    return i;
-}
-
 __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_ccg w,
                                                   const double *__restrict__ r, const double *__restrict__ pold,
                                                   double *__restrict__ pnew, double *__restrict__ Ap, int n,
@@ -121,10 +206,10 @@ __global__ void __launch_bounds__(NTH) k_ccg_apply(pvk_grid g, pvk_phys ph, int 
     if (sc->ch_done) return;
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
     const bool lead = (blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0);
-    const double rr = 2.0 * block_sum(w.rr[n & 1], w.nb, sh);
+    const double rr = 2.0 * block_sum(w.rr[n & 1], (n == 0) ? w.nt0 : w.nb, sh);   // start: one per tile
     double bnorm;
     if (n == 0) {
-        bnorm = sqrt(2.0 * block_sum(w.bb, w.nb, sh));
+        bnorm = sqrt(2.0 * block_sum(w.bb, w.nt0, sh));
         if (bnorm < 1e-30) bnorm = 1.0;
         if (lead) sc->ch_bnorm = bnorm;
     } else {
@@ -213,14 +298,15 @@ extern "C" void pvk_ccg_init(const pvk_grid *g, const pvk_phys *ph, int stage, p
                              const double *C, const double *u, const double *v, double *rn, const double *b,
                              double *r, double *p, double *bout, hipStream_t s)
 {
+    const dim3 gr((g->NXM + CIT_X - 1) / CIT_X, (g->NYM + CIT_Y - 1) / CIT_Y);
     if (b)
-        hipLaunchKernelGGL(k_ccg_init_b, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, b, r, p, *w);
+        hipLaunchKernelGGL(k_ccg_init_b, gr, dim3(NTH), 0, s, *g, *ph, stage, sc, C, b, r, p, *w);
     else if (ph->precision)
-        hipLaunchKernelGGL(k_ccg_init_rhs<float>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p,
-                           *w, bout);
+        hipLaunchKernelGGL(k_ccg_init_rhs<float>, gr, dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p, *w,
+                           bout);
     else
-        hipLaunchKernelGGL(k_ccg_init_rhs<double>, dim3(w->nb), dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r,
-                           p, *w, bout);
+        hipLaunchKernelGGL(k_ccg_init_rhs<double>, gr, dim3(NTH), 0, s, *g, *ph, stage, sc, C, u, v, rn, r, p, *w,
+                           bout);
 }
 
 extern "C" void pvk_ccg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_ccg *w,

@@ -115,6 +115,7 @@ typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
     double *bb, *rr[2], *pap;        /* partials: ||b||^2, ||r||^2 (by parity), <p, Ap> */
     int nb, band;                    /* bands (= blocks) and rows per band */
     int npap;                        /* <p, Ap> partials: one per 128 x 8 tile */
+    int nt0;                         /* start partials (bb, rr[0] at n = 0): one per 64 x 16 tile */
 } pvk_ccg;
 
 #define PVK_MOM_BAND 2                /* rows per block of the momentum CG kernels */

@@ -92,32 +92,46 @@ __device__ __forceinline__ R weno5(R v0, R v1, R v2, R v3, R v4)
     return (a0 * q0 + a1 * q1 + a2 * q2) / asum;
 }
 
-// diffuse_face_state_x / _y (VOF_DIFFUSE.c:455-483); face i sits between i-1 and i
+// diffuse_face_state_x / _y (VOF_DIFFUSE.c:455-483) on the six cells around
+// a face, cm3 .. cp2 = C at face - 3 .. face + 2 along the direction (the
+// face sits between cells face - 1 and face)
+template <class R = double>
+__device__ __forceinline__ R face_vals(double cm3, double cm2, double cm1, double c0, double cp1, double cp2,
+                                       double vel, int wo)
+{
+    if (wo == 5) {
+        if (vel >= 0.0) return weno5<R>(R(cm3), R(cm2), R(cm1), R(c0), R(cp1));
+        return weno5<R>(R(cp2), R(cp1), R(c0), R(cm1), R(cm2));
+    }
+    return R((vel >= 0.0) ? cm1 : c0);
+}
 template <class R = double>
 __device__ __forceinline__ R face_x(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
 {
     const int P = g.P;
-    if (wo == 5) {
-        if (vel >= 0.0)
-            return weno5<R>(R(AT(C, i - 3, j)), R(AT(C, i - 2, j)), R(AT(C, i - 1, j)), R(AT(C, i, j)),
-                            R(AT(C, i + 1, j)));
-        return weno5<R>(R(AT(C, i + 2, j)), R(AT(C, i + 1, j)), R(AT(C, i, j)), R(AT(C, i - 1, j)),
-                        R(AT(C, i - 2, j)));
-    }
-    return R((vel >= 0.0) ? AT(C, i - 1, j) : AT(C, i, j));
+    if (wo == 5)
+        return face_vals<R>(AT(C, i - 3, j), AT(C, i - 2, j), AT(C, i - 1, j), AT(C, i, j), AT(C, i + 1, j),
+                            AT(C, i + 2, j), vel, wo);
+    return face_vals<R>(0.0, 0.0, AT(C, i - 1, j), AT(C, i, j), 0.0, 0.0, vel, wo);
 }
 template <class R = double>
 __device__ __forceinline__ R face_y(const pvk_grid &g, const double *C, int i, int j, double vel, int wo)
 {
     const int P = g.P;
-    if (wo == 5) {
-        if (vel >= 0.0)
-            return weno5<R>(R(AT(C, i, j - 3)), R(AT(C, i, j - 2)), R(AT(C, i, j - 1)), R(AT(C, i, j)),
-                            R(AT(C, i, j + 1)));
-        return weno5<R>(R(AT(C, i, j + 2)), R(AT(C, i, j + 1)), R(AT(C, i, j)), R(AT(C, i, j - 1)),
-                        R(AT(C, i, j - 2)));
-    }
-    return R((vel >= 0.0) ? AT(C, i, j - 1) : AT(C, i, j));
+    if (wo == 5)
+        return face_vals<R>(AT(C, i, j - 3), AT(C, i, j - 2), AT(C, i, j - 1), AT(C, i, j), AT(C, i, j + 1),
+                            AT(C, i, j + 2), vel, wo);
+    return face_vals<R>(0.0, 0.0, AT(C, i, j - 1), AT(C, i, j), 0.0, 0.0, vel, wo);
+}
+
+// -(div F) at cell (i,j) from its four face fluxes (TwodOps.c:201-229)
+__device__ __forceinline__ double weno_div(const pvk_grid &g, int i, int j, double fxW, double fxE, double fyS,
+                                           double fyN)
+{
+    double div_x = (fxE - fxW) / g.dx_c[i];
+    double div_y = (fyN - fyS) / g.dy_c[j];
+    double div = div_x + div_y;
+    return -div;
 }
 
 // -(div(u C)) at interior cell (i,j): VOF_DIFFUSE.c:931-996 + TwodOps.c:201-229.
@@ -134,10 +148,7 @@ __device__ __forceinline__ double weno_rhs_at(const pvk_grid &g, const pvk_phys 
     double fxE = ue * (double)face_x<R>(g, C, i + 1, j, ue, ph.weno_order);
     double fyS = vs * (double)face_y<R>(g, C, i, j, vs, ph.weno_order);
     double fyN = vn * (double)face_y<R>(g, C, i, j + 1, vn, ph.weno_order);
-    double div_x = (fxE - fxW) / g.dx_c[i];
-    double div_y = (fyN - fyS) / g.dy_c[j];
-    double div = div_x + div_y;
-    return -div;
+    return weno_div(g, i, j, fxW, fxE, fyS, fyN);
 }
 
 // L(S(C)) / Pe at interior (i,j) (VOF_DIFFUSE.c:998-1016); C ghosts valid.
@@ -145,18 +156,22 @@ __device__ __forceinline__ double weno_rhs_at(const pvk_grid &g, const pvk_phys 
 // so a face flux enters its two cells as the same double and the sum of the
 // term over the domain telescopes to FP64 round-off (phase volume kept).
 template <class R = double>
-__device__ __forceinline__ double diffusion_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
+__device__ __forceinline__ double diffusion_vals(const pvk_grid &g, const pvk_phys &ph, int i, int j, R S0, R SE,
+                                                 R SW, R SN, R SS)
 {
-    const int P = g.P;
-    const R S0 = bulkS<R>(R(AT(C, i, j)));
-    const R SE = bulkS<R>(R(AT(C, i + 1, j))), SW = bulkS<R>(R(AT(C, i - 1, j)));
-    const R SN = bulkS<R>(R(AT(C, i, j + 1))), SS = bulkS<R>(R(AT(C, i, j - 1)));
     const double dfdxE = (double)((SE - S0) * R(g.idx_c[i]));
     const double dfdxW = (double)((S0 - SW) * R((i != 0) ? g.idx_c[i - 1] : g.idx_c[i]));
     const double dfdyN = (double)((SN - S0) * R(g.idy_c[j]));
     const double dfdyS = (double)((S0 - SS) * R((j != 0) ? g.idy_c[j - 1] : g.idy_c[j]));
     const double lapS = (dfdxE - dfdxW) * g.idx_u[i] + (dfdyN - dfdyS) * g.idy_v[j];
     return lapS / ph.Pe_CH;
+}
+template <class R = double>
+__device__ __forceinline__ double diffusion_at(const pvk_grid &g, const pvk_phys &ph, const double *C, int i, int j)
+{
+    const int P = g.P;
+    return diffusion_vals<R>(g, ph, i, j, bulkS<R>(R(AT(C, i, j))), bulkS<R>(R(AT(C, i + 1, j))),
+                             bulkS<R>(R(AT(C, i - 1, j))), bulkS<R>(R(AT(C, i, j + 1))), bulkS<R>(R(AT(C, i, j - 1))));
 }
 
 // measure-weighted CH product contribution of interior cell (i,j) including

@@ -280,7 +280,8 @@ int pv_cg_create(pv_bag *b)
     const int band = 1, mband = PVK_MOM_BAND;
     int nbc = (ny + band - 1) / band, nbu = (ny + mband - 1) / mband, nbv = (ny - 1 + mband - 1) / mband, c;
     int npap = ((ny + 7) / 8) * ((nx + 127) / 128);
-    size_t n = 3 * (size_t)nbc + (size_t)npap + 4 * (size_t)nbu + 4 * (size_t)nbv;
+    int nt0 = ((ny + 15) / 16) * ((nx + 63) / 64), nbs = (nt0 > nbc) ? nt0 : nbc;   /* start partials */
+    size_t n = 2 * (size_t)nbs + (size_t)nbc + (size_t)npap + 4 * (size_t)nbu + 4 * (size_t)nbv;
     double *q;
     HIPCHK(hipMalloc((void **)&b->cgmem, sizeof(double) * n));
     HIPCHK(hipMemset(b->cgmem, 0, sizeof(double) * n));
@@ -288,8 +289,9 @@ int pv_cg_create(pv_bag *b)
     memset(&b->ccg, 0, sizeof b->ccg);
     b->ccg.nb = nbc; b->ccg.band = band;
     b->ccg.p[0] = b->f[PV_F_CH_DIR]; b->ccg.p[1] = b->f[PV_FI_CHP2];
-    b->ccg.bb = q; q += nbc;
-    b->ccg.rr[0] = q; q += nbc;
+    b->ccg.bb = q; q += nbs;
+    b->ccg.rr[0] = q; q += nbs;
+    b->ccg.nt0 = nt0;
     b->ccg.rr[1] = q; q += nbc;
     b->ccg.pap = q; q += npap;
     b->ccg.npap = npap;
