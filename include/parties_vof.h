@@ -312,6 +312,18 @@ int pv_batch_set_threads(pv_batch *bt, int nthreads);
 int pv_batch_autotune(pv_batch *bt, int *chunks);
 int pv_autotune_mg(pv_bag *bag, int *wlevels, int *sweeps);
 int pv_batch_autotune_mg(pv_batch *bt, int *wlevels, int *sweeps);
+/* Tile shapes.  pv_set_momentum_rows: rows of each 256-thread block of the
+ * momentum CG kernels, 1, 2 (default) or 4.  pv_set_mg_tile0: the tile of
+ * the multigrid's level-0 smoothing kernels, 0 = 64 x 16, 1 = 32 x 16,
+ * 2 = 32 x 8, -1 = by width (default: 64 x 16 above 512 columns).  Both
+ * change only the order of the partial sums.  pv_autotune_tiles times the
+ * candidates on the handle's own momentum system (16 iterations, u and v
+ * restored) and cold pressure solves (PHI restored) and keeps the fastest
+ * of each; the MG tile is left alone unless the handle runs MG-PCG with
+ * tiled levels. */
+int pv_set_momentum_rows(pv_bag *bag, int rows);
+int pv_set_mg_tile0(pv_bag *bag, int shape);
+int pv_autotune_tiles(pv_bag *bag, int *mom_rows, int *mg_tile0);
 
 /* ---- pressure and Cahn-Hilliard solver options ---------------------------
  * No counterpart in the reference. */

@@ -385,13 +385,13 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
 // d_n = r / dg + beta d_{n-1} (:705-713) evaluated at
 // the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
 #define MCH 256                                                  // columns per chunk of k_mcg_apply
-#define MSQ (((PVK_MOM_BAND + 2) * (MCH + 2) + NTH - 1) / NTH)     // staged values per thread
-template <class R>
+#define MSQ (((BAND + 2) * (MCH + 2) + NTH - 1) / NTH)     // staged values per thread
+template <class R, int BAND>
 __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w,
                                                   int n)
 {
     __shared__ double sh[NTH];
-    __shared__ double sd[PVK_MOM_BAND + 2][MCH + 2];
+    __shared__ double sd[BAND + 2][MCH + 2];
     const mom_band m = mom_locate(g, w);
     const int c = m.comp;
     if (sc->m_done[c]) return;
@@ -434,7 +434,7 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         if (b <= 0 || b >= NYM) return AT(dwall, ia, b);
         return (n == 0) ? AT(dwall, ia, b) : AT(r, ia, b) / AT(dg, ia, b) + beta * AT(dold, ia, b);
     };
-    // The band (at most PVK_MOM_BAND rows) in chunks of MCH columns: d of the
+    // The band (w.band = BAND rows, the last one may have fewer) in chunks of MCH columns: d of the
     // chunk and a one-cell ring is formed once into LDS (every value as dv
     // forms it), then A d of the chunk reads its neighbours there.
     const int nrow = m.j1 - m.j0;
@@ -449,7 +449,7 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
 #pragma unroll
         for (int k = 0; k < MSQ; k++) {
             const int q = threadIdx.x + k * NTH, a = q / (MCH + 2), bb = q - a * (MCH + 2);
-            if (a < PVK_MOM_BAND + 2) sd[a][bb] = sv[k];
+            if (a < BAND + 2) sd[a][bb] = sv[k];
         }
         __syncthreads();
         for (int q = threadIdx.x; q < nrow * MCH; q += NTH) {
@@ -519,10 +519,17 @@ extern "C" void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, p
 extern "C" void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                              int half, int n, hipStream_t s)
 {
-    if (half == 0 && ph->precision)
-        hipLaunchKernelGGL(k_mcg_apply<float>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
-    else if (half == 0)
-        hipLaunchKernelGGL(k_mcg_apply<double>, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n);
-    else
+#define MCG_APPLY(RT, B) \
+    hipLaunchKernelGGL((k_mcg_apply<RT, B>), dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n)
+    if (half == 0 && ph->precision) {
+        if (w->band == 1) MCG_APPLY(float, 1);
+        else if (w->band == 2) MCG_APPLY(float, 2);
+        else MCG_APPLY(float, 4);
+    } else if (half == 0) {
+        if (w->band == 1) MCG_APPLY(double, 1);
+        else if (w->band == 2) MCG_APPLY(double, 2);
+        else MCG_APPLY(double, 4);
+    } else
+#undef MCG_APPLY
         hipLaunchKernelGGL(k_mcg_update, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
 }

@@ -118,7 +118,8 @@ typedef struct pvk_ccg {             /* Cahn-Hilliard biharmonic CG */
     int nt0;                         /* start partials (bb, rr[0] at n = 0): one per 64 x 16 tile */
 } pvk_ccg;
 
-#define PVK_MOM_BAND 2                /* rows per block of the momentum CG kernels */
+#define PVK_MOM_BAND 4                /* most rows per block of the momentum CG kernels (1, 2 or 4;
+                                         pvk_mcg.band, default 2) */
 typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     double *x[2], *r[2], *dg[2], *Ad[2];
     double *d[2][2];                 /* [component][n & 1]; d[c][0] is the public field */
@@ -280,6 +281,8 @@ typedef struct pvk_mg {
     int nlev, c0, nu, fp32;           /* levels, first in-LDS level, sweeps per side, FP32 V-cycle */
     int topf;                         /* faces of level c0 copied to LDS too (when they fit) */
     int wmask;                        /* in-LDS levels (bit l) with two coarse corrections (a W there) */
+    int sh0;                          /* level-0 tile shape: 0 = 64 x 16, 1 = 32 x 16, 2 = 32 x 8,
+                                         -1 = by width as the other levels */
     pvk_mglev L[PVK_MG_MAXLEV];
     const double *r0;                 /* FP32: level-0 right-hand side (r), pitch p0 */
     double *z0;                       /* FP32: level-0 result (z), pitch p0 */

@@ -856,10 +856,17 @@ __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *s
 // launchers
 // ---------------------------------------------------------------------------
 // tile shape by level width: 0 = 64 x 16, 1 = 32 x 16, 2 = 32 x 8
-static int tile_shape(const pvk_mglev &L) { return (L.nx > 512) ? 0 : (L.nx > 256 ? 1 : 2); }
-static dim3 tile_grid(const pvk_mglev &L)
+// (level 0: m->sh0 when it is set)
+static int tile_shape(const pvk_mg *m, int l)
 {
-    const int sh = tile_shape(L), tx = sh ? 32 : 64, ty = (sh == 2) ? 8 : 16;
+    const pvk_mglev &L = m->L[l];
+    if (l == 0 && m->sh0 >= 0) return m->sh0;
+    return (L.nx > 512) ? 0 : (L.nx > 256 ? 1 : 2);
+}
+static dim3 tile_grid(const pvk_mg *m, int l)
+{
+    const pvk_mglev &L = m->L[l];
+    const int sh = tile_shape(m, l), tx = sh ? 32 : 64, ty = (sh == 2) ? 8 : 16;
     return dim3((L.nx + tx - 1) / tx, (L.ny + ty - 1) / ty);
 }
 static dim3 cell_grid(const pvk_mglev &L) { return dim3((L.nx + TXP - 1) / TXP, (L.ny + TYR - 1) / TYR); }
@@ -877,7 +884,7 @@ extern "C" size_t pvk_mg_lds_bytes(const pvk_mg *m)
 extern "C" int pvk_mg_nrz(const pvk_mg *m)
 {
     if (m->c0 == 0) return 1;
-    const dim3 gr = tile_grid(m->L[0]);
+    const dim3 gr = tile_grid(m, 0);
     return (int)(gr.x * gr.y);
 }
 
@@ -935,12 +942,12 @@ template <typename T>
 static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,
                   hipStream_t s)
 {
-    const dim3 gr = tile_grid(m->L[l]);
+    const dim3 gr = tile_grid(m, l);
     const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
     const int step = (l == 0 && n >= 0);
     const double *r0 = (l == 0) ? (step ? r_of(m, w, n) : r_in(m, w, n)) : nullptr;
     double *rnew = step ? r_of(m, w, n + 1) : nullptr;
-    const int sh = tile_shape(m->L[l]), chk = (l == 1), nc = m->L[0].nx * m->L[0].ny;
+    const int sh = tile_shape(m, l), chk = (l == 1), nc = m->L[0].nx * m->L[0].ny;
     if (m->nu == 1) MG_SHAPES(k_mg_pre, 1, L, C, r0, rnew, m->p0, sc, *w, n, tol, maxit, chk, step, nc);
     else MG_SHAPES(k_mg_pre, 2, L, C, r0, rnew, m->p0, sc, *w, n, tol, maxit, chk, step, nc);
     // levels 0 .. 1 only: the stop test of the residual pre level 0 formed
@@ -950,12 +957,12 @@ static void pre_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n,
 
 template <typename T> static void post_t(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, hipStream_t s)
 {
-    const dim3 gr = tile_grid(m->L[l]);
+    const dim3 gr = tile_grid(m, l);
     const Lev<T> L = lev<T>(m->L[l]), C = lev<T>(m->L[l + 1]);
     const double *r0 = (l == 0) ? r_in(m, w, n) : nullptr;
     double *z0 = (l == 0) ? m->z0 : nullptr;
     const pvk_scal *cs = sc;
-    const int sh = tile_shape(m->L[l]), slot = rz_slot(n), dot = (l == 0);
+    const int sh = tile_shape(m, l), slot = rz_slot(n), dot = (l == 0);
     if (m->nu == 1) MG_SHAPES(k_mg_post, 1, L, C, r0, z0, m->p0, cs, *w, dot, slot);
     else MG_SHAPES(k_mg_post, 2, L, C, r0, z0, m->p0, cs, *w, dot, slot);
 }

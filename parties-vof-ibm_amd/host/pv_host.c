@@ -131,6 +131,8 @@ static double *fld(pv_bag *b, int f) { return b->f[f]; }
 
 /* doubles of one field allocation, and its cell (0,0) given the allocation base */
 size_t pv_field_elems(const pv_bag *b) { return b->alloc_elems; }
+/* offset of cell (0, 0) from the start of its field's allocation */
+size_t pv_field_front(const pv_bag *b) { return (size_t)OFF0 + 3 * (size_t)b->P + 3; }
 double *pv_field_origin(const pv_bag *b, double *base) { return base + OFF0 + 3 * (size_t)b->P + 3; }
 
 static void fill_phys(pv_bag *b)
@@ -272,13 +274,14 @@ static int create_on_device(pv_bag *b, const pv_params *prm)
 }
 
 /* Work sets of the fused biharmonic and momentum CG loops: one band of rows
- * per 256-thread block (1 row for the biharmonic CG, PVK_MOM_BAND rows for
- * the momentum CG), one partial-sum slot per band and reduction. */
+ * per 256-thread block (1 row for the biharmonic CG, 1, 2 or 4 rows for the
+ * momentum CG: 2 unless pv_set_momentum_rows says otherwise), one
+ * partial-sum slot per band and reduction (sized for one-row bands). */
 int pv_cg_create(pv_bag *b)
 {
     const int nx = b->NXM, ny = b->NYM;
-    const int band = 1, mband = PVK_MOM_BAND;
-    int nbc = (ny + band - 1) / band, nbu = (ny + mband - 1) / mband, nbv = (ny - 1 + mband - 1) / mband, c;
+    const int band = 1, mband = 2;
+    int nbc = (ny + band - 1) / band, nbu = ny, nbv = ny - 1, c;
     int npap = ((ny + 7) / 8) * ((nx + 127) / 128);
     int nt0 = ((ny + 15) / 16) * ((nx + 63) / 64), nbs = (nt0 > nbc) ? nt0 : nbc;   /* start partials */
     size_t n = 2 * (size_t)nbs + (size_t)nbc + (size_t)npap + 4 * (size_t)nbu + 4 * (size_t)nbv;
@@ -303,7 +306,7 @@ int pv_cg_create(pv_bag *b)
     b->mcg.in_tot[0] = 1.0 / (double)(nx * ny);
     b->mcg.in_tot[1] = 1.0 / (double)(nx * (ny - 1));
     for (c = 0; c < 2; c++) {
-        int nbk = c ? nbv : nbu;
+        int nbk = c ? nbv : nbu;              /* slots for one-row bands */
         b->mcg.x[c] = b->f[c ? PV_F_V : PV_F_U];
         b->mcg.r[c] = b->f[c ? PV_F_V_R : PV_F_U_R];
         b->mcg.dg[c] = b->f[c ? PV_FI_DG_V : PV_FI_DG_U];
@@ -315,7 +318,7 @@ int pv_cg_create(pv_bag *b)
         b->mcg.rz[c][1] = q; q += nbk;
         b->mcg.dad[c] = q; q += nbk;
         b->mcg.rr[c] = q; q += nbk;
-        b->mcg.nb[c] = nbk;
+        b->mcg.nb[c] = (c ? ny - 1 + mband - 1 : ny + mband - 1) / mband;
         /* coefficient k of component c: half (k & 1) of field MCF(3c + k / 2),
          * each half NYM rows of pitch P floats from the field's cell (0, 0) */
         for (int k = 0; k < 5; k++)

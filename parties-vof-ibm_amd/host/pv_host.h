@@ -130,6 +130,7 @@ struct pv_bag {
     int nmggraph;
     const pvk_mg *mggraph_key;     /* hierarchy and sweeps the MG graphs were captured with */
     int mggraph_nu, mggraph_w;     /* sweeps and W levels the MG graphs were captured with */
+    int mggraph_sh;                /* and the level-0 tile shape */
     double learn_start_res;        /* relative residual after the last warm start */
 };
 
@@ -145,6 +146,8 @@ int  pv_cg_create(pv_bag *b);
 int  pv_mg_create(pv_bag *b);
 int  pv_set_mg_precision(pv_bag *b, int fp32);
 int  pv_set_ch_solver(pv_bag *b, int spectral);
+int  pv_set_mg_tile0(pv_bag *b, int shape);
+int  pv_set_momentum_rows(pv_bag *b, int rows);
 void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
 int  pv_learn_snapshot(pv_bag *b, int si);
@@ -153,6 +156,7 @@ int  pv_learn_source(pv_bag *b, double *out);
 void pv_learn_free(pv_bag *b);
 void pv_graphs_free(pv_bag *b);
 size_t pv_field_elems(const pv_bag *b);
+size_t pv_field_front(const pv_bag *b);
 double *pv_field_origin(const pv_bag *b, double *base);
 void pv_tbegin(pv_bag *b, int t);
 void pv_tend(pv_bag *b, int t);

@@ -66,7 +66,10 @@ int pv_mg_create(pv_bag *b)
         m->L[l].pitch = mf->L[l].pitch = lx[l];
     }
     for (l = 1; l < nl; l++) cells += (size_t)lx[l] * ly[l];
+    /* per-tile partials sized for the smallest level-0 tiles (the most) */
+    m->sh0 = mf->sh0 = 2;
     nrz = pvk_mg_nrz(m);
+    m->sh0 = mf->sh0 = -1;
     total = 5 * cells + 2 * (size_t)b->g.P * ny + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 5 * (size_t)ny +
             3 * (size_t)nrz;
     totalf = 5 * cells + 3 * nc0;
@@ -136,6 +139,7 @@ int pv_mg_create(pv_bag *b)
         w->rz[1] = q; q += nrz;
         w->r2 = q; q += (size_t)b->g.P * ny;     /* r_1, r_3, ...: pitch P like r */
         w->rrt = q; q += nrz;
+        nrz = pvk_mg_nrz(m);                     /* the tiles in use */
         w->nrrt = (c0 > 0) ? nrz : 0;
         w->ndad = ny;
         w->nrr = ny;
@@ -180,6 +184,30 @@ int pv_set_pressure_warm_start(pv_bag *b, int mode)
     b->have_prev[0] = b->have_prev[1] = b->have_prev[2] = 0;
     memset(b->warm_wins, 0, sizeof b->warm_wins);
     memset(b->warm_solves, 0, sizeof b->warm_solves);
+    return PV_OK;
+}
+
+int pv_set_mg_tile0(pv_bag *b, int shape)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (shape < -1 || shape > 2) return pv_set_error(PV_EINVAL, "MG level-0 tile shape must be -1, 0, 1 or 2 (got %d)", shape);
+    if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
+    b->mg.sh0 = b->mgf.sh0 = shape;
+    {
+        const int nrz = pvk_mg_nrz(&b->mg);
+        b->mgw.nrz = nrz;
+        b->mgw.nrrt = (b->mg.c0 > 0) ? nrz : 0;
+    }
+    return PV_OK;
+}
+
+int pv_set_momentum_rows(pv_bag *b, int rows)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (rows != 1 && rows != 2 && rows != 4) return pv_set_error(PV_EINVAL, "momentum rows per block must be 1, 2 or 4 (got %d)", rows);
+    b->mcg.band = rows;
+    b->mcg.nb[0] = (b->NYM + rows - 1) / rows;
+    b->mcg.nb[1] = (b->NYM - 1 + rows - 1) / rows;
     return PV_OK;
 }
 

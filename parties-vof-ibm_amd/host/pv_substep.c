@@ -630,11 +630,13 @@ static int mg_loop_graphs(pv_bag *b, double tol, int maxit)
 {
     const int C = b->graph_chunk;
     int kc = 0, n = 0;
-    if (b->mggraph_key != b->mgp || b->mggraph_nu != b->mgp->nu || b->mggraph_w != b->mgp->wmask) {
+    if (b->mggraph_key != b->mgp || b->mggraph_nu != b->mgp->nu || b->mggraph_w != b->mgp->wmask ||
+        b->mggraph_sh != b->mgp->sh0) {
         mg_graphs_free(b);
         b->mggraph_key = b->mgp;
         b->mggraph_nu = b->mgp->nu;
         b->mggraph_w = b->mgp->wmask;
+        b->mggraph_sh = b->mgp->sh0;
     }
     for (;;) {
         if (kc >= b->nmggraph) {
@@ -1254,6 +1256,118 @@ int pv_autotune(pv_bag *b, int *choice)
  * no temporal or network start; PHI and the iteration-chunk guess are put
  * back afterwards and the start bookkeeping is not touched, so the
  * trajectory is the one the chosen settings give without tuning. */
+/* event-timed ms of `iters` momentum CG iterations (u and v) from the
+ * current fields, stop test off (tolerance 0, so the solve runs exactly
+ * iters iterations) */
+static int mom_time(pv_bag *b, int iters, float *ms)
+{
+    hipEvent_t e0 = NULL, e1 = NULL;
+    pvk_mcg w = b->mcg;
+    int n, rc = PV_OK;
+    w.exact = b->m_exact;
+    w.tol = 0.0;
+    w.maxit = iters;
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        rc = pv_set_error(PV_EHIP, "tile autotune: hipEventCreate failed");
+    } else {
+        hipEventRecord(e0, S(b));
+        pvk_mcg_init(G(b), &b->ph, b->stage, b->d_sc, &w, S(b));
+        for (n = 0; n <= iters; n++) {
+            pvk_mcg_iter(G(b), &b->ph, b->stage, b->d_sc, &w, 0, n, S(b));
+            pvk_mcg_iter(G(b), &b->ph, b->stage, b->d_sc, &w, 1, n, S(b));
+        }
+        hipEventRecord(e1, S(b));
+        if (hipGetLastError() != hipSuccess || hipEventSynchronize(e1) != hipSuccess ||
+            hipEventElapsedTime(ms, e0, e1) != hipSuccess)
+            rc = pv_set_error(PV_EHIP, "tile autotune: momentum timing failed");
+    }
+    if (e0) hipEventDestroy(e0);
+    if (e1) hipEventDestroy(e1);
+    return rc;
+}
+
+/* Tile shapes: rows per block of the momentum CG (1, 2, 4), timed on 16
+ * iterations of the current momentum system (u and v saved and restored),
+ * and the level-0 tile shape of the multigrid (64 x 16, 32 x 16, 32 x 8),
+ * timed on cold pressure solves in place as pv_autotune_mg does; the
+ * fastest of each is kept.  A failure leaves both as they were. */
+int pv_autotune_tiles(pv_bag *b, int *mom_rows, int *mg_tile0)
+{
+    const int rows_c[3] = {1, 2, 4};
+    double *save = NULL;
+    size_t bytes;
+    int k, rep, rc = PV_OK, rows0 = b ? b->mcg.band : 2, sh00 = b ? b->mg.sh0 : -1, pick_r = rows0, pick_s = sh00;
+    int p_warm, chunk;
+    double start_res, best = 1e300;
+    RC(begin(b));
+    bytes = sizeof(double) * pv_field_elems(b);
+    if (hipMalloc((void **)&save, 2 * bytes) != hipSuccess)
+        return pv_set_error(PV_ENOMEM, "tile autotune: hipMalloc of %zu bytes failed", 2 * bytes);
+    /* momentum rows */
+    if (hipMemcpyAsync(save, FD(b, PV_F_U) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+        hipMemcpyAsync(save + bytes / sizeof(double), FD(b, PV_F_V) - pv_field_front(b), bytes,
+                       hipMemcpyDeviceToDevice, S(b)) != hipSuccess)
+        rc = pv_set_error(PV_EHIP, "tile autotune: saving u, v failed");
+    for (k = 0; k < 3 && rc == PV_OK; k++) {
+        float ms = 0.0f, ms2 = 0.0f;
+        if ((rc = pv_set_momentum_rows(b, rows_c[k]))) break;
+        if ((rc = mom_time(b, 16, &ms)) || (rc = mom_time(b, 16, &ms2))) break;   /* the first one warms up */
+        if (ms2 < best) { best = ms2; pick_r = rows_c[k]; }
+    }
+    if (hipMemcpyAsync(FD(b, PV_F_U) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+        hipMemcpyAsync(FD(b, PV_F_V) - pv_field_front(b), save + bytes / sizeof(double), bytes,
+                       hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+        hipStreamSynchronize(S(b)) != hipSuccess)
+        if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "tile autotune: restoring u, v failed");
+    if (rc == PV_OK) rc = pv_set_momentum_rows(b, pick_r);
+    /* multigrid level-0 tiles */
+    if (rc == PV_OK && b->mg_ok && b->mg.c0 > 0 &&
+        (b->prm.pressure_solver == PV_PSOLVE_MG_PCG || b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG)) {
+        if (hipMemcpyAsync(save, FD(b, PV_F_PHI) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) !=
+            hipSuccess)
+            rc = pv_set_error(PV_EHIP, "tile autotune: saving PHI failed");
+        p_warm = b->p_warm;
+        chunk = b->mg_chunk;
+        start_res = b->learn_start_res;
+        b->p_warm = 0;
+        best = 1e300;
+        for (k = 0; k < 3 && rc == PV_OK; k++) {
+            hipEvent_t e0 = NULL, e1 = NULL;
+            float ms = 0.0f;
+            if ((rc = pv_set_mg_tile0(b, k))) break;
+            if ((rc = mg_solve(b, 0))) break;                   /* warm-up */
+            if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+                rc = pv_set_error(PV_EHIP, "tile autotune: hipEventCreate failed");
+            } else {
+                hipEventRecord(e0, S(b));
+                for (rep = 0; rep < 2 && rc == PV_OK; rep++) rc = mg_solve(b, 0);
+                hipEventRecord(e1, S(b));
+                if (hipEventSynchronize(e1) != hipSuccess || hipEventElapsedTime(&ms, e0, e1) != hipSuccess)
+                    rc = pv_set_error(PV_EHIP, "tile autotune: MG timing failed");
+            }
+            if (e0) hipEventDestroy(e0);
+            if (e1) hipEventDestroy(e1);
+            if (rc == PV_OK && ms < best) { best = ms; pick_s = k; }
+        }
+        b->p_warm = p_warm;
+        b->mg_chunk = chunk;
+        b->learn_start_res = start_res;
+        if (hipMemcpyAsync(FD(b, PV_F_PHI) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice, S(b)) !=
+                hipSuccess || hipStreamSynchronize(S(b)) != hipSuccess)
+            if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "tile autotune: restoring PHI failed");
+        if (rc == PV_OK) rc = pv_set_mg_tile0(b, pick_s);
+    }
+    hipFree(save);
+    if (rc) {                                  /* a failed tuning leaves the shapes as they were */
+        pv_set_momentum_rows(b, rows0);
+        if (b->mg_ok) pv_set_mg_tile0(b, sh00);
+        return rc;
+    }
+    if (mom_rows) *mom_rows = b->mcg.band;
+    if (mg_tile0) *mg_tile0 = b->mg_ok ? b->mg.sh0 : -1;
+    return PV_OK;
+}
+
 int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
 {
     double best = 1e300, *phi_save = NULL;
@@ -1267,7 +1381,8 @@ int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
     bytes = sizeof(double) * pv_field_elems(b);
     if (hipMalloc((void **)&phi_save, bytes) != hipSuccess)
         return pv_set_error(PV_ENOMEM, "MG autotune: hipMalloc of %zu bytes failed", bytes);
-    if (hipMemcpyAsync(phi_save, FD(b, PV_F_PHI), bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess) {
+    if (hipMemcpyAsync(phi_save, FD(b, PV_F_PHI) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) !=
+        hipSuccess) {
         hipFree(phi_save);
         return pv_set_error(PV_EHIP, "MG autotune: saving PHI failed");
     }
@@ -1299,8 +1414,8 @@ int pv_autotune_mg(pv_bag *b, int *wlevels, int *sweeps)
     b->p_warm = p_warm;
     b->mg_chunk = chunk;
     b->learn_start_res = start_res;
-    if (hipMemcpyAsync(FD(b, PV_F_PHI), phi_save, bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
-        hipStreamSynchronize(S(b)) != hipSuccess)
+    if (hipMemcpyAsync(FD(b, PV_F_PHI) - pv_field_front(b), phi_save, bytes, hipMemcpyDeviceToDevice, S(b)) !=
+            hipSuccess || hipStreamSynchronize(S(b)) != hipSuccess)
         if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "MG autotune: restoring PHI failed");
     hipFree(phi_save);
     if (rc) {                          /* a failed tuning leaves the cycle as it was */

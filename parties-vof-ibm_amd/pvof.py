@@ -172,6 +172,9 @@ def load_lib(path=LIB_PATH):
     lib.pv_set_ch_solver.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_warm_start.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_cg_steps.argtypes = [vp, C.c_int]
+    lib.pv_set_momentum_rows.argtypes = [vp, C.c_int]
+    lib.pv_set_mg_tile0.argtypes = [vp, C.c_int]
+    lib.pv_autotune_tiles.argtypes = [vp, ip, ip]
     lib.pv_learn_collect_begin.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
     lib.pv_learn_collect_end.argtypes = [vp, ip]
     lib.pv_learn_train.argtypes = [vp, C.c_int, C.c_int, C.c_float, C.c_uint64, dp, dp]
