@@ -320,7 +320,8 @@ def mixed_error(lib, args):
 def batch_bench(lib, args):
     """Batched cases: --batch-cases cases of sizes drawn (seeded) from the
     batched configuration's {96^2, 192^2, 256x512, 384x768}, FP64, MG-PCG with
-    the temporal start, seeded bubbles
+    the temporal start, the momentum CG options of --mom-exact / --mom-warm,
+    seeded bubbles
     (the IBM cylinder and wetting of that configuration are not part of this
     path).  Aggregate Mcell-steps/s of pv_batch_rk3 (one stream and host
     thread per case) against the same cases stepped one after another.  Each
@@ -343,6 +344,8 @@ def batch_bench(lib, args):
         s.init_from_F(F)
         s.check(lib.pv_set_pressure_warm_start(s.h, 1))
         s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
+        s.check(lib.pv_set_momentum_cg_steps(s.h, args.mom_exact))
+        s.check(lib.pv_set_momentum_warm_start(s.h, args.mom_warm))
         if args.batch_graphs > 0:
             s.check(lib.pv_set_graphs(s.h, 1, args.batch_graphs))
 
@@ -385,6 +388,7 @@ def batch_bench(lib, args):
             "batch_Mcell_steps_per_s": round(cells * args.steps / tb / 1e6, 3),
             "one_after_another_Mcell_steps_per_s": round(cells * args.steps / ts / 1e6, 3),
             "batch_gain": round(ts / tb, 3), "precision": "fp64", "pressure_solver": "mg",
+            "momentum_exact_steps": args.mom_exact, "momentum_warm_start": args.mom_warm,
             "mg_cycle_w_levels": args.mg_cycle if choices is None else "autotuned",
             "graph_chunk": args.batch_graphs or None,
             "host_threads": min(args.batch_threads, len(params)) if args.batch_threads else len(params),

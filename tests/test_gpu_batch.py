@@ -68,6 +68,34 @@ def test_batch_cases_match_solo_runs(lib, threads):
     bt.close()
 
 
+@pytest.mark.parametrize("solver", [pvof.PV_PSOLVE_FFT_PCG, pvof.PV_PSOLVE_MG_PCG])
+def test_batch_of_the_batched_configuration_sizes(lib, solver):
+    """The batched configuration's sizes, 96^2, 192^2, 256 x 512 and
+    384 x 768 (rows of 2^a 3^b for the FFT-x preconditioner), two cases each,
+    three RK3 steps with either pressure solver: every case equals its solo
+    run bitwise and its pressure solves meet the tolerance."""
+    sizes = [(96, 96), (192, 192), (256, 512), (384, 768)] * 2
+    params = []
+    for nx, ny in sizes:
+        p = case_params(nx, ny)
+        p.pressure_solver = solver
+        params.append(p)
+    fields = [bubble_field(lib, p, 300 + k) for k, p in enumerate(params)]
+    bt = pvof.Batch(params, lib)
+    for c, F in zip(bt.cases, fields):
+        c.init_from_F(F)
+    bt.rk3(3)
+    for p, F, c in zip(params, fields, bt.cases):
+        assert max(c.stats()["p_res"]) <= 1e-10
+        s = pvof.Solver(p, lib)
+        s.init_from_F(F)
+        s.call("Temporal_int_rk3", 3)
+        for f in FIELDS:
+            assert np.array_equal(c.get(f), s.get(f)), (p.NXM, p.NYM, f)
+        s.close()
+    bt.close()
+
+
 def test_autotune_keeps_the_trajectory(lib):
     p = case_params(128, 128)
     F = bubble_field(lib, p, 9)
