@@ -1304,28 +1304,30 @@ int pv_autotune_tiles(pv_bag *b, int *mom_rows, int *mg_tile0)
     if (hipMalloc((void **)&save, 2 * bytes) != hipSuccess)
         return pv_set_error(PV_ENOMEM, "tile autotune: hipMalloc of %zu bytes failed", 2 * bytes);
     /* momentum rows */
-    if (hipMemcpyAsync(save, FD(b, PV_F_U) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+    const int saved_uv =
+        hipMemcpyAsync(save, FD(b, PV_F_U) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) == hipSuccess &&
         hipMemcpyAsync(save + bytes / sizeof(double), FD(b, PV_F_V) - pv_field_front(b), bytes,
-                       hipMemcpyDeviceToDevice, S(b)) != hipSuccess)
-        rc = pv_set_error(PV_EHIP, "tile autotune: saving u, v failed");
+                       hipMemcpyDeviceToDevice, S(b)) == hipSuccess;
+    if (!saved_uv) rc = pv_set_error(PV_EHIP, "tile autotune: saving u, v failed");
     for (k = 0; k < 3 && rc == PV_OK; k++) {
         float ms = 0.0f, ms2 = 0.0f;
         if ((rc = pv_set_momentum_rows(b, rows_c[k]))) break;
         if ((rc = mom_time(b, 16, &ms)) || (rc = mom_time(b, 16, &ms2))) break;   /* the first one warms up */
         if (ms2 < best) { best = ms2; pick_r = rows_c[k]; }
     }
-    if (hipMemcpyAsync(FD(b, PV_F_U) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
-        hipMemcpyAsync(FD(b, PV_F_V) - pv_field_front(b), save + bytes / sizeof(double), bytes,
-                       hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
-        hipStreamSynchronize(S(b)) != hipSuccess)
+    if (saved_uv && (hipMemcpyAsync(FD(b, PV_F_U) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice,
+                                    S(b)) != hipSuccess ||
+                     hipMemcpyAsync(FD(b, PV_F_V) - pv_field_front(b), save + bytes / sizeof(double), bytes,
+                                    hipMemcpyDeviceToDevice, S(b)) != hipSuccess ||
+                     hipStreamSynchronize(S(b)) != hipSuccess))
         if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "tile autotune: restoring u, v failed");
     if (rc == PV_OK) rc = pv_set_momentum_rows(b, pick_r);
     /* multigrid level-0 tiles */
     if (rc == PV_OK && b->mg_ok && b->mg.c0 > 0 &&
         (b->prm.pressure_solver == PV_PSOLVE_MG_PCG || b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG)) {
-        if (hipMemcpyAsync(save, FD(b, PV_F_PHI) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice, S(b)) !=
-            hipSuccess)
-            rc = pv_set_error(PV_EHIP, "tile autotune: saving PHI failed");
+        const int saved = (hipMemcpyAsync(save, FD(b, PV_F_PHI) - pv_field_front(b), bytes, hipMemcpyDeviceToDevice,
+                                          S(b)) == hipSuccess);
+        if (!saved) rc = pv_set_error(PV_EHIP, "tile autotune: saving PHI failed");
         p_warm = b->p_warm;
         chunk = b->mg_chunk;
         start_res = b->learn_start_res;
@@ -1352,8 +1354,8 @@ int pv_autotune_tiles(pv_bag *b, int *mom_rows, int *mg_tile0)
         b->p_warm = p_warm;
         b->mg_chunk = chunk;
         b->learn_start_res = start_res;
-        if (hipMemcpyAsync(FD(b, PV_F_PHI) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice, S(b)) !=
-                hipSuccess || hipStreamSynchronize(S(b)) != hipSuccess)
+        if (saved && (hipMemcpyAsync(FD(b, PV_F_PHI) - pv_field_front(b), save, bytes, hipMemcpyDeviceToDevice,
+                                     S(b)) != hipSuccess || hipStreamSynchronize(S(b)) != hipSuccess))
             if (rc == PV_OK) rc = pv_set_error(PV_EHIP, "tile autotune: restoring PHI failed");
         if (rc == PV_OK) rc = pv_set_mg_tile0(b, pick_s);
     }
