@@ -4,9 +4,10 @@ gas bubble of radius 0.25 at (0.5, 0.5), density ratio 1e-3, FP64, FFT-PCG
 pressure solve to 1e-13 relative, CH and momentum solves tight), for the GPU test that checks the plain
 path over the whole run (tests/test_gpu_parity.py::test_config1_200_steps).
 The oracle needs about 11 minutes for it on one core, too long for a test
-run; the fixture keeps every 4th cell of C_L, U, V and P in each direction,
-the maximum magnitude and the L2 norm of each full field, the time and the
-phase volume sum(C_L) before and after.  Run from the repository root:
+run; the fixture keeps the full interior fields of C_L, U, V and P (the
+unknowns of each kind), the maximum magnitude and the L2 norm of each, every
+4th cell of each in each direction, the time and the phase volume sum(C_L)
+before and after.  Run from the repository root:
     python tests/golden/make_config1_200.py
 """
 import os
@@ -50,6 +51,7 @@ def main():
            "vol": np.float64(o.get("C_L")[R["cell"]].sum())}
     for f, r in FIELDS:
         a = o.get(f)[R[r]]
+        out[f] = a.copy()
         out[f + "_sub"] = a[::4, ::4].copy()
         out[f + "_max"] = np.float64(np.abs(a).max())
         out[f + "_l2"] = np.float64(np.sqrt((a * a).sum()))
