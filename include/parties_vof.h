@@ -323,6 +323,12 @@ int pv_batch_autotune_mg(pv_batch *bt, int *wlevels, int *sweeps);
  * tiled levels. */
 int pv_set_momentum_rows(pv_bag *bag, int rows);
 int pv_set_mg_tile0(pv_bag *bag, int shape);
+/* Shape of the FFT preconditioner's y tridiagonal solves (partitioned
+ * Thomas): rows per thread's segment, 1..16 (0 = 8), and columns per block,
+ * a power of two <= 64 (0 = 2, or more where a block would hold less than a
+ * wavefront; halved while a block would exceed 512 threads).  Changes the order of the arithmetic
+ * (the separators' reduced system), so results differ at rounding level. */
+int pv_set_fft_tri_shape(pv_bag *bag, int rows, int cols);
 int pv_autotune_tiles(pv_bag *bag, int *mom_rows, int *mg_tile0);
 
 /* ---- pressure and Cahn-Hilliard solver options ---------------------------

@@ -446,8 +446,6 @@ __global__ void __launch_bounds__(NTH) k_dct_rows(pvk_grid g, pvk_fft F, pvk_sca
 // column 0 is shifted to zero mean afterwards.  In the PCG this kernel also
 // runs the stop test of iteration n (every block, from the <r, r> partials).
 // ---------------------------------------------------------------------------
-#define TRI_LMAX 16
-
 __device__ __forceinline__ double tri_sum(const double *part, int n, double *sh, int nt)
 {
     const int t = threadIdx.x;
@@ -464,6 +462,7 @@ __device__ __forceinline__ double tri_sum(const double *part, int n, double *sh,
     return r;
 }
 
+template <int TRI_LMAX>
 __global__ void __launch_bounds__(512) k_tri_cols(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, int n,
                                                    double tol, int maxit)
 {
@@ -485,65 +484,74 @@ __global__ void __launch_bounds__(512) k_tri_cols(pvk_grid g, pvk_fft F, pvk_sca
     }
     const int NX = g.NXM, NY = g.NYM, KW = F.tri_kw, S = F.tri_s, L = F.tri_l;
     const int kk = threadIdx.x & (KW - 1), s = threadIdx.x >> F.tri_lkw;
-    const int col = blockIdx.x * KW + kk;
+    const int grp = xcd_band(blockIdx.x, gridDim.x);     // neighbouring column groups share an XCD's L2
+    const int col = grp * KW + kk;
     const bool act = col < NX;
     const int j0 = s * L, has_sep = s < S - 1;
     const int nrow = has_sep ? L : NY - (S - 1) * L, nint = has_sep ? nrow - 1 : nrow;
     const double mw = act ? F.tri_mw[col] : 0.0;
     double *spec = F.spec;
-    auto A = [&](int j) { return (col == 0 && j == 0) ? 0.0 : F.tri_as[j]; };
-    auto Cc = [&](int j) { return (col == 0 && j == 0) ? 0.0 : F.tri_an[j]; };
-    auto B = [&](int j) { return (col == 0 && j == 0) ? 1.0 : (-F.tri_as[j] - F.tri_an[j]) - 0.0 - mw; };
-    double y[TRI_LMAX], gg[TRI_LMAX];
-    double dsep = 0.0;
+    // the segment's row coefficients are loaded with its right-hand side,
+    // before the sweeps: loads inside the sweeps would put their latency on
+    // the chain of dependent divisions
+    double y[TRI_LMAX], gg[TRI_LMAX], ra[TRI_LMAX], rc[TRI_LMAX];
 #pragma unroll
     for (int l = 0; l < TRI_LMAX; l++) {
-        double v = 0.0;
-        if (l < nrow && act) v = spec[(size_t)(j0 + l) * NX + col];
+        double v = 0.0, a = 0.0, c = 0.0;
+        if (l < nrow) {
+            a = F.tri_as[j0 + l];
+            c = F.tri_an[j0 + l];
+            if (act) v = spec[(size_t)(j0 + l) * NX + col];
+        }
         if (col == 0 && j0 + l == 0) v = 0.0;
-        y[l] = v;
-        if (has_sep && l == nrow - 1) dsep = v;
+        y[l] = v; ra[l] = a; rc[l] = c;
+    }
+    // A, C, B of the segment's row l (l a compile-time index after unrolling)
+    auto A = [&](int l) { return (col == 0 && j0 + l == 0) ? 0.0 : ra[l]; };
+    auto Cc = [&](int l) { return (col == 0 && j0 + l == 0) ? 0.0 : rc[l]; };
+    auto B = [&](int l) { return (col == 0 && j0 + l == 0) ? 1.0 : (-ra[l] - rc[l]) - 0.0 - mw; };
+    // the separator (the segment's last row) and the last interior row's C
+    double dsep = 0.0, as_ = 0.0, bs_ = 0.0, cs_ = 0.0, cbot = 0.0;
+#pragma unroll
+    for (int l = 0; l < TRI_LMAX; l++) {
+        if (has_sep && l == nrow - 1) { dsep = y[l]; as_ = A(l); bs_ = B(l); cs_ = Cc(l); }
+        if (has_sep && l == nint - 1) cbot = Cc(l);
     }
     // forward sweep over the interior rows; e1 = forward sweep of the first
-    // unit vector, kept only at the last interior row (Vl) and recovered
-    // backwards from the pivots: e1_{l-1} = -e1_l p_l / a_l (a_l != 0 on
-    // every row but j = 0, which is never l >= 1 of a segment)
-    double inv = 1.0 / B(j0), e1 = inv, Yl = 0.0, Vl = 0.0;
+    // unit vector, kept per row (the one division per row is the pivot's)
+    double e1s[TRI_LMAX];
+    double inv = 1.0 / B(0), e1 = inv, Yl = 0.0, Vl = 0.0;
     y[0] *= inv;
     gg[0] = 0.0;
+    e1s[0] = e1;
     if (nint == 1) { Yl = y[0]; Vl = e1; }
 #pragma unroll
     for (int l = 1; l < TRI_LMAX; l++) {
         gg[l] = 0.0;
+        e1s[l] = 0.0;
         if (l < nint) {
-            const int j = j0 + l;
-            const double a = A(j);
-            gg[l] = Cc(j - 1) * inv;
-            inv = 1.0 / (B(j) - a * gg[l]);
+            const double a = A(l);
+            gg[l] = Cc(l - 1) * inv;
+            inv = 1.0 / (B(l) - a * gg[l]);
             y[l] = (y[l] - a * y[l - 1]) * inv;
             e1 = -a * e1 * inv;
+            e1s[l] = e1;
             if (l == nint - 1) { Yl = y[l]; Vl = e1; }
         }
     }
     const double Wl = inv;                  // 1 / last pivot: forward sweep of the last unit vector
     // first-row values of the local solutions for d, the first and the last unit vector
-    double Yf = 0.0, Vf = 0.0, Wf = 0.0, ev = Vl;
+    double Yf = 0.0, Vf = 0.0, Wf = 0.0;
 #pragma unroll
     for (int l = TRI_LMAX - 1; l >= 0; l--) {
-        if (l == nint - 1) { Yf = y[l]; Vf = ev; Wf = Wl; }
+        if (l == nint - 1) { Yf = y[l]; Vf = e1s[l]; Wf = Wl; }
         else if (l < nint - 1) {
             Yf = y[l] - gg[l + 1] * Yf;
-            Vf = ev - gg[l + 1] * Vf;
+            Vf = e1s[l] - gg[l + 1] * Vf;
             Wf = -gg[l + 1] * Wf;
         }
-        if (l >= 1 && l < nint) {
-            const int j = j0 + l;
-            const double a = A(j);
-            ev = -ev * (B(j) - a * gg[l]) / a;
-        }
     }
-    const double atop = (s > 0) ? A(j0) : 0.0;
-    const double cbot = has_sep ? Cc(j0 + nint - 1) : 0.0;
+    const double atop = (s > 0) ? A(0) : 0.0;
     // separators' Schur complement: row s couples z_{s-1}, z_s, z_{s+1}
     const int nsep = S - 1, SK = S * KW, me = s * KW + kk;
     double *RA0 = lds, *RB0 = lds + SK, *RC0 = lds + 2 * SK, *RD0 = lds + 3 * SK;
@@ -551,8 +559,6 @@ __global__ void __launch_bounds__(512) k_tri_cols(pvk_grid g, pvk_fft F, pvk_sca
     RB1[me] = Yf; RC1[me] = atop * Vf; RD1[me] = cbot * Wf;      // for the separator above this segment
     __syncthreads();
     if (s < nsep) {
-        const int js = j0 + nint;
-        const double as_ = A(js), bs_ = B(js), cs_ = Cc(js);
         const int nb = me + KW;
         RA0[me] = -as_ * atop * Vl;
         RB0[me] = bs_ - as_ * cbot * Wl - cs_ * RC1[nb];
@@ -591,25 +597,19 @@ __global__ void __launch_bounds__(512) k_tri_cols(pvk_grid g, pvk_fft F, pvk_sca
     const double zprev = (s > 0) ? z[me - KW] : 0.0, zsep = has_sep ? z[me] : 0.0;
     // interior rows from the separators (back substitution in place)
     double x = 0.0, colsum = 0.0;
-    ev = Vl;
 #pragma unroll
     for (int l = TRI_LMAX - 1; l >= 0; l--) {
         if (l < nint) {
-            double t = y[l] - atop * zprev * ev;
+            double t = y[l] - atop * zprev * e1s[l];
             if (l == nint - 1) { t -= cbot * zsep * Wl; x = t; }
             else x = t - gg[l + 1] * x;
             y[l] = x;
             colsum += x;
-            if (l >= 1) {
-                const int j = j0 + l;
-                const double a = A(j);
-                ev = -ev * (B(j) - a * gg[l]) / a;
-            }
         }
     }
     if (has_sep) colsum += zsep;
     double shift = 0.0;
-    if (!w.keep_pin && blockIdx.x == 0) {     // block-uniform: column 0 lives in block 0
+    if (!w.keep_pin && grp == 0) {            // block-uniform: column 0 lives in group 0
         __syncthreads();
         if (kk == 0) na[s] = colsum;
         __syncthreads();
@@ -812,8 +812,12 @@ extern "C" void pvk_pcg_stage(const pvk_grid *g, const pvk_fft *F, pvk_scal *sc,
         break;
     case 6: {
         const int nt = F->tri_kw * F->tri_s;
-        hipLaunchKernelGGL(k_tri_cols, dim3((g->NXM + F->tri_kw - 1) / F->tri_kw), dim3(nt),
-                           (size_t)8 * nt * sizeof(double), s, *g, *F, sc, *w, n, tol, maxit);
+        if (F->tri_l <= 8)                   // registers for 8 or 16 rows a thread
+            hipLaunchKernelGGL(k_tri_cols<8>, dim3((g->NXM + F->tri_kw - 1) / F->tri_kw), dim3(nt),
+                               (size_t)8 * nt * sizeof(double), s, *g, *F, sc, *w, n, tol, maxit);
+        else
+            hipLaunchKernelGGL(k_tri_cols<16>, dim3((g->NXM + F->tri_kw - 1) / F->tri_kw), dim3(nt),
+                               (size_t)8 * nt * sizeof(double), s, *g, *F, sc, *w, n, tol, maxit);
     } break;
     default:
         if (F->pow2x)

@@ -472,6 +472,50 @@ static int xlen_ok(int n)
     return n == 1;
 }
 
+/* Launch shape of the FFT preconditioner's y solves (k_tri_cols): segments
+ * of rows rows per thread (at most 16, the registers a thread keeps per
+ * row; 0 = 8) and cols columns per block (a power of two; 0 = 2, or more
+ * where a block would hold less than a wavefront; halved while the block
+ * would exceed 512 threads).  The kernel is latency-bound (a chain of FP64 divisions per
+ * segment): short segments and many blocks (512 at 1024 columns) run
+ * fastest -- 20.6 us at 1024^2 against 25.1 us for 16 rows and 8 columns.
+ * The blocks of one XCD take neighbouring columns, so each row's cache lines
+ * come from HBM once per XCD.  Nonzero if no shape fits. */
+static int tri_shape(pv_bag *b, int rows, int cols)
+{
+    const int ny = b->NYM;
+    int L = rows ? rows : 8, S, kw = cols ? cols : 2;
+    if (L > ny) L = ny;
+    S = (ny + L - 1) / L;
+    L = (ny + S - 1) / S;
+    S = (ny + L - 1) / L;
+    if (!cols)
+        while (kw < 64 && kw * S < 64) kw <<= 1;   /* at least one wavefront a block */
+    while (kw > 1 && kw * S > 512) kw >>= 1;
+    if (kw * S > 512 || L > 16) return 1;
+    b->fft.tri_l = L; b->fft.tri_s = S; b->fft.tri_kw = kw;
+    for (b->fft.tri_lkw = 0; (1 << b->fft.tri_lkw) < kw; b->fft.tri_lkw++) {}
+    return 0;
+}
+
+int pv_set_fft_tri_shape(pv_bag *b, int rows, int cols)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (!b->fft.ok) return pv_set_error(PV_EINVAL, "the FFT solve is not available for this grid size");
+    if (rows < 0 || rows > 16) return pv_set_error(PV_EINVAL, "rows per thread must be in [0, 16] (got %d)", rows);
+    if (cols < 0 || cols > 64 || (cols & (cols - 1)))
+        return pv_set_error(PV_EINVAL, "columns per block must be 0 or a power of two <= 64 (got %d)", cols);
+    {
+        pvk_fft keep = b->fft;
+        if (tri_shape(b, rows, cols)) {
+            b->fft = keep;
+            return pv_set_error(PV_EINVAL, "no y-solve shape of %d rows and %d columns fits 512 threads", rows, cols);
+        }
+    }
+    pv_graphs_free(b);                      /* captured iterations hold the old shape */
+    return PV_OK;
+}
+
 int pv_fft_create(pv_bag *b)
 {
     const double PI_ = 3.14159265358979323846;
@@ -530,18 +574,7 @@ int pv_fft_create(pv_bag *b)
         int wave = (k < nx / 2) ? k : (nx - k);
         q[k] = 2.0 * (1 - cos(2. * PI_ * wave / (nx))) * (b->hm.idx_u[3] * b->hm.idx_u[3]);
     }
-    /* launch shape of the y solves: segments of <= 16 rows (the registers a
-     * thread keeps per row), as many columns per block as fit 512 threads */
-    {
-        int L = (ny < 16) ? ny : 16, S, kw = 64;
-        S = (ny + L - 1) / L;
-        L = (ny + S - 1) / S;
-        S = (ny + L - 1) / L;
-        while (kw > 1 && kw * S > 512) kw >>= 1;
-        if (kw * S > 512) { free(h); return PV_OK; }
-        b->fft.tri_l = L; b->fft.tri_s = S; b->fft.tri_kw = kw;
-        for (b->fft.tri_lkw = 0; (1 << b->fft.tri_lkw) < kw; b->fft.tri_lkw++) {}
-    }
+    if (tri_shape(b, 0, 0)) { free(h); return PV_OK; }
     if (hipMalloc((void **)&d, sizeof(double) * (total + (size_t)nx * ny)) != hipSuccess) {
         free(h);
         return pv_set_error(PV_ENOMEM, "FFT table device allocation failed");

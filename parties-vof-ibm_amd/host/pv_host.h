@@ -148,6 +148,7 @@ int  pv_set_mg_precision(pv_bag *b, int fp32);
 int  pv_set_ch_solver(pv_bag *b, int spectral);
 int  pv_set_mg_tile0(pv_bag *b, int shape);
 int  pv_set_momentum_rows(pv_bag *b, int rows);
+int  pv_set_fft_tri_shape(pv_bag *b, int rows, int cols);
 void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);
 int  pv_learn_snapshot(pv_bag *b, int si);

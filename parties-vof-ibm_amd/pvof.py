@@ -174,6 +174,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_set_momentum_cg_steps.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_rows.argtypes = [vp, C.c_int]
     lib.pv_set_mg_tile0.argtypes = [vp, C.c_int]
+    lib.pv_set_fft_tri_shape.argtypes = [vp, C.c_int, C.c_int]
     lib.pv_autotune_tiles.argtypes = [vp, ip, ip]
     lib.pv_learn_collect_begin.argtypes = [vp, ip, C.c_int, C.c_int, C.c_int, C.c_int, C.c_uint64]
     lib.pv_learn_collect_end.argtypes = [vp, ip]

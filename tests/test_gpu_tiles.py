@@ -124,3 +124,37 @@ def test_autotune_tiles_keeps_the_state(lib):
         with pytest.raises(RuntimeError):
             s.check(bad[0](s.h, bad[1]))
     s.close()
+
+
+def test_fft_tri_shapes_give_the_same_solve(lib):
+    """The FFT preconditioner's y solves (partitioned Thomas) in several
+    segment lengths and block widths: one RK3 step of the plain FP64 path
+    each, pressure iteration counts within a few, fields equal to the
+    solve's tolerance; shapes
+    that do not fit are rejected and leave the handle as it was."""
+    n = 256
+    p = params(n, n, pressure_solver=pvof.PV_PSOLVE_FFT_PCG, CG_ETOL=1e-9)
+    F = pvof.ic(lib, "bubbles", p, seed=47, nmin=8, nmax=24, rmin=0.02, rmax=0.1, gas_inside=1)
+    R = regions(p)
+    runs = {}
+    for shape in ((0, 0), (16, 8), (8, 2), (4, 2), (3, 64), (16, 1)):
+        s = pvof.Solver(p, lib)
+        s.init_from_F(F)
+        s.check(lib.pv_set_fft_tri_shape(s.h, *shape))
+        s.call("Temporal_int_rk3", 1)
+        st = s.stats()
+        assert max(st["p_res"]) <= 1e-10
+        runs[shape] = (st["n_p"], {f: s.get(f)[R["cell"]].copy() for f in ("U", "V", "P", "C_L")})
+        if shape == (0, 0):
+            for bad in ((17, 2), (-1, 2), (8, 3), (8, 128)):
+                with pytest.raises(RuntimeError):
+                    s.check(lib.pv_set_fft_tri_shape(s.h, *bad))
+            s.call("Temporal_int_rk3", 1)          # the default shape is still in place
+            assert max(s.stats()["p_res"]) <= 1e-10
+        s.close()
+    ref_its, ref = runs[0, 0]
+    for shape, (its, fields) in runs.items():
+        # ~300 iterations a solve: rounding moves the count by a few
+        assert max(abs(a - b) for a, b in zip(its, ref_its)) <= 4, (shape, its, ref_its)
+        for f, a in fields.items():
+            assert relerr(a, ref[f]) < 1e-9, (shape, f, relerr(a, ref[f]))
