@@ -29,6 +29,31 @@ __device__ __forceinline__ double block_sum(const double *part, int n, double *s
     return r;
 }
 
+// Two sums of partials in one pass (one tree, half the barriers of two
+// block_sum calls), each added in exactly block_sum's order; sh holds
+// 2 * NTH doubles.
+__device__ __forceinline__ void block_sum2(const double *pa, int na, const double *pb, int nb, double *sh,
+                                           double &ra, double &rb)
+{
+    const int t = threadIdx.x;
+    double sa = 0.0, sb = 0.0;
+    for (int b = t; b < na; b += NTH) sa += pa[b];
+    for (int b = t; b < nb; b += NTH) sb += pb[b];
+    sh[t] = sa;
+    sh[NTH + t] = sb;
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if (t < w) {
+            sh[t] += sh[t + w];
+            sh[NTH + t] += sh[NTH + t + w];
+        }
+        __syncthreads();
+    }
+    ra = sh[0];
+    rb = sh[NTH];
+    __syncthreads();
+}
+
 // Block total of v (wave shuffles, then the 4 wave sums in order) -> part[slot].
 __device__ __forceinline__ void block_store_partial(double v, double *part, int slot, double *sh)
 {

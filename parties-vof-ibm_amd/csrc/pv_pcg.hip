@@ -131,14 +131,17 @@ __global__ void __launch_bounds__(NTH) k_pcg_faces(pvk_grid g, pvk_pcg w)
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(NTH) k_pcg_apply(pvk_grid g, pvk_scal *sc, pvk_pcg w, int n)
 {
-    __shared__ double sh[NTH];
+    __shared__ double sh[2 * NTH];
     if (sc->p_done) return;
     const int P = g.P, NXM = g.NXM, NYM = g.NYM;
-    double beta = 0.0;
-    if (n > 0 && w.flex)      // flexible CG: A-orthogonal to the previous direction only
-        beta = -block_sum(w.zad, w.nrz, sh) / block_sum(w.dad[(n + 1) & 1], w.ndad, sh);
-    else if (n > 0)           // Fletcher-Reeves: gamma_n / gamma_{n-1}
-        beta = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.rz[(n + 1) & 1], w.nrz, sh);
+    double beta = 0.0, sa, sb;
+    if (n > 0 && w.flex) {    // flexible CG: A-orthogonal to the previous direction only
+        block_sum2(w.zad, w.nrz, w.dad[(n + 1) & 1], w.ndad, sh, sa, sb);
+        beta = -sa / sb;
+    } else if (n > 0) {       // Fletcher-Reeves: gamma_n / gamma_{n-1}
+        block_sum2(w.rz[n & 1], w.nrz, w.rz[(n + 1) & 1], w.nrz, sh, sa, sb);
+        beta = sa / sb;
+    }
     const double *__restrict__ z = w.z;
     const double *__restrict__ dold = w.d[(n + 1) & 1];
     const double *__restrict__ bx = w.bx, *__restrict__ by = w.by;
@@ -187,44 +190,67 @@ template <bool P2>
 __global__ void __launch_bounds__(NTH) k_pcg_update_fwd(pvk_grid g, pvk_fft F, pvk_scal *sc, pvk_pcg w, int n)
 {
     extern __shared__ double lds[];
-    __shared__ double sh[NTH];
+    __shared__ double sh[2 * NTH];
     if (w.chb == 0.0 && sc->p_done) return;
     const int P = g.P, N = g.NXM, logM = F.logmx, M = F.mx, R = w.rrow, RM = R * M;
     constexpr bool p2 = P2;     // power-of-two rows: shift-indexed passes
     double *sr = lds, *si = lds + RM;
     double *tw = stage_twiddles(lds, RM, M, F.wx);
-    double alpha = 0.0, mean = 0.0;
-    if (n >= 0) alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
-    else mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
     const double *__restrict__ d = (n >= 0) ? w.d[n & 1] : w.rhs;
     const double *__restrict__ Ad = w.Ad, *__restrict__ rhs = w.rhs;
     double *__restrict__ phi = w.phi, *__restrict__ r = w.r;
     const int j0 = blockIdx.x * R;
-    double xa[IT], xb[IT], ea[IT], eb[IT], ra[IT], rb[IT];
+    // the rows' loads go out before the scalar sums, whose partials and
+    // barriers would otherwise hold them back; the arithmetic is unchanged
+    double xa[IT], xb[IT], ea[IT], eb[IT], ra[IT], rb[IT], qa[IT], qb[IT];
 #pragma unroll
     for (int u = 0; u < IT; u++) {
         const int t = threadIdx.x + u * NTH;
         int q, m;
         row_split(t, M, logM, p2, q, m);
         const int j = j0 + q, i = 2 * m;
-        xa[u] = xb[u] = ea[u] = eb[u] = ra[u] = rb[u] = 0.0;
+        xa[u] = xb[u] = ea[u] = eb[u] = ra[u] = rb[u] = qa[u] = qb[u] = 0.0;
         if (t < RM && j < g.NYM) {
             if (n >= 0) {
                 xa[u] = AT(phi, i, j); xb[u] = AT(phi, i + 1, j);
                 ea[u] = AT(d, i, j);   eb[u] = AT(d, i + 1, j);
-                ra[u] = AT(r, i, j) - alpha * AT(Ad, i, j);
-                rb[u] = AT(r, i + 1, j) - alpha * AT(Ad, i + 1, j);
+                ra[u] = AT(r, i, j);   rb[u] = AT(r, i + 1, j);
+                qa[u] = AT(Ad, i, j);  qb[u] = AT(Ad, i + 1, j);
             } else if (n == -3) {               // M_fft of the current r, no update
                 ra[u] = AT(r, i, j);
                 rb[u] = AT(r, i + 1, j);
             } else if (n == -2) {               // warm restart: Ad holds A phi0
                 xa[u] = AT(phi, i, j); xb[u] = AT(phi, i + 1, j);
-                ra[u] = AT(rhs, i, j) - mean - AT(Ad, i, j);
-                rb[u] = AT(rhs, i + 1, j) - mean - AT(Ad, i + 1, j);
+                ra[u] = AT(rhs, i, j);     rb[u] = AT(rhs, i + 1, j);
+                qa[u] = AT(Ad, i, j);      qb[u] = AT(Ad, i + 1, j);
             } else {
-                ra[u] = AT(rhs, i, j) - mean;
-                rb[u] = AT(rhs, i + 1, j) - mean;
+                ra[u] = AT(rhs, i, j);
+                rb[u] = AT(rhs, i + 1, j);
             }
+        }
+    }
+    double alpha = 0.0, mean = 0.0;
+    if (n >= 0) {
+        double srz, sdad;
+        block_sum2(w.rz[n & 1], w.nrz, w.dad[n & 1], w.ndad, sh, srz, sdad);
+        alpha = srz / sdad;
+    } else if (n != -3)
+        mean = block_sum(w.mean, w.nmean, sh) / ((double)g.NXM * (double)g.NYM);
+#pragma unroll
+    for (int u = 0; u < IT; u++) {
+        const int t = threadIdx.x + u * NTH;
+        int q, m;
+        row_split(t, M, logM, p2, q, m);
+        if (!(t < RM && j0 + q < g.NYM)) continue;
+        if (n >= 0) {
+            ra[u] = ra[u] - alpha * qa[u];
+            rb[u] = rb[u] - alpha * qb[u];
+        } else if (n == -2) {
+            ra[u] = ra[u] - mean - qa[u];
+            rb[u] = rb[u] - mean - qb[u];
+        } else if (n != -3) {
+            ra[u] = ra[u] - mean;
+            rb[u] = rb[u] - mean;
         }
     }
     double acc = 0.0;
