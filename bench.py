@@ -99,12 +99,13 @@ def workload_params(n=1024, solver=pvof.PV_PSOLVE_FFT_PCG, precision=pvof.PV_PRE
         precision=precision)
 
 
-def step_bytes_per_cell(st, family, mixed=False):
+def step_bytes_per_cell(st, family, mixed=False, mom_pc=0):
     """Algorithmic bytes per cell for one RK3 step from the solver counts
     (fixed 563 B per substep + 88 per biharmonic, 136 per momentum (140 in the
     mixed form, whose operator kernel reads five FP32 coefficients instead of
-    rho and mu) and BYTES_PER_ITER per pressure iteration)."""
-    total, mom = 0, (140 if mixed else 136)
+    rho and mu; 16 more with the polynomial preconditioner) and BYTES_PER_ITER
+    per pressure iteration)."""
+    total, mom = 0, (140 if mixed else 136) + (16 if mom_pc else 0)
     for k in range(3):
         total += 563 + 88 * st["n_ch"][k] + mom * (st["n_u"][k] + st["n_v"][k]) + BYTES_PER_ITER[family] * st["n_p"][k]
     return total
@@ -187,7 +188,8 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
     if net_params is not None:
         pvof.Network(s).set_params(net_params)
     o = dict(warm_start=args.warm_start, ch_spectral=args.ch_spectral, mg_cycle=args.mg_cycle,
-             mg_sweeps=args.mg_sweeps, mom_warm=args.mom_warm, mom_exact=args.mom_exact)
+             mg_sweeps=args.mg_sweeps, mom_warm=args.mom_warm, mom_exact=args.mom_exact,
+             mom_pc=args.mom_pc if precision == "mixed" else 0)
     o.update(opts or {})
     if family_of(solver) == "mg":
         s.check(lib.pv_set_pressure_warm_start(s.h, o["warm_start"]))
@@ -196,6 +198,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
         s.check(lib.pv_set_mg_sweeps(s.h, o["mg_sweeps"]))
         s.check(lib.pv_set_momentum_warm_start(s.h, o["mom_warm"]))
         s.check(lib.pv_set_momentum_cg_steps(s.h, o["mom_exact"]))
+        s.check(lib.pv_set_momentum_preconditioner(s.h, o["mom_pc"]))
     if warmup:
         s.call("Temporal_int_rk3", warmup)
     s.call("synchronize")
@@ -227,6 +230,11 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
         # the FP32 operator reads r, diag, d_old and five FP32 coefficients per
         # component (formed by the start kernel) instead of rho and mu
         bpc = dict(bpc, MOM_APPLY=60)
+        if o["mom_pc"]:
+            # polynomial preconditioner: the operator reads z instead of r and
+            # diag; the update also reads diag and four coefficients and
+            # writes r_{n+1} and z
+            bpc = dict(bpc, MOM_APPLY=52, MOM_UPDATE=80)
     if not any(sum(st["n_ch"]) for st in stats):
         # spectral CH start: the CH CG launches only take the stop test (no cell work)
         bpc = {k: v for k, v in bpc.items() if k not in ("CH_APPLY", "CH_UPDATE")}
@@ -246,7 +254,7 @@ def run(lib, args, solver, precision, steps, warmup, F, net_params=None, timers=
             kernels[name] = {"timed_launches": int(cnt[k]), "avg_us": round(1e3 * avg_ms, 3),
                              "est_total_ms": round(ms[k] * max(1, args.timer_period), 3),
                              "GBs": (round(bpc[name] * cells / (avg_ms * 1e-3) / 1e9, 1) if name in bpc else None)}
-    bytes_step = sum(step_bytes_per_cell(st, fam, precision == "mixed") for st in stats) / len(stats)
+    bytes_step = sum(step_bytes_per_cell(st, fam, precision == "mixed", o["mom_pc"]) for st in stats) / len(stats)
     out = {"solver": solver, "precision": precision, "steps": steps, "warmup": warmup,
            "settings": o if family_of(solver) == "mg" else None,
            "value": round(cells * steps / elapsed / 1e6, 3), "ms_per_step": round(1e3 * elapsed / steps, 3),
@@ -288,6 +296,7 @@ def mixed_error(lib, args):
             s.check(lib.pv_set_mg_cycle(s.h, args.mg_cycle))
             s.check(lib.pv_set_momentum_warm_start(s.h, args.mom_warm))
             s.check(lib.pv_set_momentum_cg_steps(s.h, args.mom_exact))
+            s.check(lib.pv_set_momentum_preconditioner(s.h, args.mom_pc))
         runs[name], secs[name] = s, 0.0
     R = (slice(3, 3 + n), slice(3, 3 + n))
     marks = sorted({m for m in (10, 50, 100, 250, steps) if m <= steps})
@@ -479,6 +488,9 @@ def main():
                     help="MG solvers: Gauss-Seidel sweeps before and after each coarse correction")
     ap.add_argument("--mom-warm", type=int, choices=[0, 1], default=1,
                     help="MG-PCG forms: 1 = start each momentum CG from the same RK3 stage's last increment")
+    ap.add_argument("--mom-pc", type=int, choices=[0, 1], default=1,
+                    help="mixed MG-PCG forms: 1 = momentum CG preconditioned by the first-order Neumann polynomial "
+                         "of the Jacobi splitting (0 = Jacobi, as the reference)")
     ap.add_argument("--mom-exact", type=int, choices=[0, 1], default=1,
                     help="MG-PCG forms: 1 = momentum CG step lengths without the reference's EPS = 1e-14 "
                          "(same stop test; the EPS stalls the CG near CG_ETOL)")
@@ -578,7 +590,7 @@ def main():
             note(f"FP64 MG-PCG runs: {args.plain_steps} steps each")
             # the best FP64 path: multigrid V-cycle, the reference CH CG, cold starts, no network
             ref = run(lib, args, "mg", "fp64", args.plain_steps, 1, F, timers=False,
-                      opts=dict(warm_start=0, ch_spectral=0, mg_cycle=0, mom_warm=0, mom_exact=0))
+                      opts=dict(warm_start=0, ch_spectral=0, mg_cycle=0, mom_warm=0, mom_exact=0, mom_pc=0))
             compare["fp64_mg"] = public(ref)
             compare["speedup_vs_fp64_mg"] = round(main_run["value"] / ref["value"], 3)
             # the timed form's algorithm in FP64 throughout (isolates the FP32 stencils and cycle)
@@ -610,6 +622,8 @@ def main():
                    "ch_spectral": args.ch_spectral if family_of(args.solver) == "mg" else None,
                    "momentum_warm_start": args.mom_warm if family_of(args.solver) == "mg" else None,
                    "momentum_exact_steps": args.mom_exact if family_of(args.solver) == "mg" else None,
+                   "momentum_preconditioner": ("neumann1" if args.mom_pc else "jacobi")
+                   if family_of(args.solver) == "mg" and args.precision == "mixed" else "jacobi",
                    "mg_cycle_w_levels": args.mg_cycle if family_of(args.solver) == "mg" else None,
                    "p_rel_tol": p.p_rel_tol, "ch_tol": p.ch_tol, "CG_ETOL": p.CG_ETOL},
         "path_hbm_fraction": main_run["path_hbm_fraction"],

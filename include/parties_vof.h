@@ -366,6 +366,14 @@ int pv_set_momentum_warm_start(pv_bag *bag, int mode);
  * creeps on until CG_MAXIT; without it the same stop test (RMS residual <
  * CG_ETOL) is met in a few iterations.  The diagonal keeps its + EPS. */
 int pv_set_momentum_cg_steps(pv_bag *bag, int exact);
+/* Preconditioner of the momentum CG: 0 = Jacobi, as the reference
+ * (msolve_cg.c:627-645, default); 1 = the first-order Neumann polynomial of
+ * the Jacobi splitting A = D - N, z = D^-1 (r + N D^-1 r), formed from the
+ * FP32 operator coefficients (mixed-precision handles only).  Symmetric
+ * positive definite for the diagonally dominant momentum operator on a
+ * uniform grid; about half the iterations for one more stencil of r each.
+ * The CG's stop test (the RMS residual against CG_ETOL) is unchanged. */
+int pv_set_momentum_preconditioner(pv_bag *bag, int kind);
 /* Cahn-Hilliard implicit solve (the reference runs the CG of
  * VOF_DIFFUSE.c:318-429 from C): 0 = that CG (default); 1 = the CG started
  * from the exact solution of a_dt x + (Cn^2/Pe) L^2 x = b, which the

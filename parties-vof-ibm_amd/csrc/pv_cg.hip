@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         }
     }
     if (lead) sc->m_rzb[c][n & 1] = rz;
-    const double *r = w.r[c], *dg = w.dg[c], *dold = w.d[c][(n + 1) & 1], *dwall = w.d[c][0];
+    const double *r = w.r[c], *dg = w.dg[c], *dold = w.d[c][(n + 1) & 1], *dwall = w.d[c][0], *zc = w.z[c];
     double *dnew = w.d[c][n & 1], *Ad = w.Ad[c];
     const double alpha_k = c_BET[stage], dt = sc->dt;
     // d at (a, b) after the CG-type ghost fill of the new direction
@@ -427,12 +427,14 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
         if (c == 0) {
             const int ia = wrapu(a, NXM);
             const int jb = b < 0 ? 0 : (b >= NYM ? NYM - 1 : b);
-            double val = (n == 0) ? AT(dwall, ia, jb) : AT(r, ia, jb) / AT(dg, ia, jb) + beta * AT(dold, ia, jb);
+            double val = (n == 0) ? AT(dwall, ia, jb)
+                                  : (w.pc ? AT(zc, ia, jb) : AT(r, ia, jb) / AT(dg, ia, jb)) + beta * AT(dold, ia, jb);
             return (b < 0 || b >= NYM) ? -val : val;
         }
         const int ia = wrapc(a, NXM);
         if (b <= 0 || b >= NYM) return AT(dwall, ia, b);
-        return (n == 0) ? AT(dwall, ia, b) : AT(r, ia, b) / AT(dg, ia, b) + beta * AT(dold, ia, b);
+        return (n == 0) ? AT(dwall, ia, b)
+                        : (w.pc ? AT(zc, ia, b) : AT(r, ia, b) / AT(dg, ia, b)) + beta * AT(dold, ia, b);
     };
     // The band (w.band = BAND rows, the last one may have fewer) in chunks of MCH columns: d of the
     // chunk and a one-cell ring is formed once into LDS (every value as dv
@@ -507,6 +509,110 @@ __global__ void __launch_bounds__(NTH) k_mcg_update(pvk_grid g, pvk_scal *sc, pv
     block_store_partial(a1, w.rr[c], m.band, sh);
 }
 
+// Momentum preconditioner pc = 1, the first-order Neumann polynomial of
+// the Jacobi splitting A = D - N: z = D^-1 (r + N D^-1 r), i.e.
+// z_p = (r_p + sum_q c_pq r_q / D_q) / D_p over the unknown neighbours q of
+// p (periodic in x, no coupling across the walls), c_pq the FP32 operator
+// coefficients (E, W, N, S) the start kernel formed and D = dg.  On a
+// uniform grid c_pq = c_qp, so M = D^-1 + D^-1 N D^-1 is symmetric, and it
+// is positive definite where A is diagonally dominant (the spectral radius
+// of D^-1/2 N D^-1/2 below one).  It takes about half the Jacobi-PCG's
+// iterations for one more stencil of r per iteration, which the update
+// kernel forms from r_n and A d_n at the neighbours (r is double-buffered:
+// r_{n+1} goes to the other buffer while neighbours read r_n).
+template <class RF>
+__device__ __forceinline__ double mom_pc_z(const pvk_grid &g, const pvk_mcg &w, int c, int i, int j, RF rv)
+{
+    const int P = g.P, NXM = g.NXM, NYM = g.NYM;
+    const double *dg = w.dg[c];
+    const int ie = c ? wrapc(i + 1, NXM) : wrapu(i + 1, NXM);
+    const int iw = c ? wrapc(i - 1, NXM) : wrapu(i - 1, NXM);
+    const int jlo = c ? 1 : 0;
+    double s = rv(i, j);
+    s += (double)AT(w.cf[c][1], i, j) * (rv(ie, j) / AT(dg, ie, j));
+    s += (double)AT(w.cf[c][2], i, j) * (rv(iw, j) / AT(dg, iw, j));
+    if (j + 1 < NYM) s += (double)AT(w.cf[c][3], i, j) * (rv(i, j + 1) / AT(dg, i, j + 1));
+    if (j - 1 >= jlo) s += (double)AT(w.cf[c][4], i, j) * (rv(i, j - 1) / AT(dg, i, j - 1));
+    return s / AT(dg, i, j);
+}
+
+// pc = 1 start: d_0 = z_0 = M r_0, partial <r_0, z_0> (replacing the Jacobi
+// ones of the start kernel)
+__global__ void __launch_bounds__(NTH) k_mcg_pc_start(pvk_grid g, pvk_mcg w)
+{
+    __shared__ double sh[NTH];
+    const int P = g.P, NXM = g.NXM;
+    const mom_band m = mom_locate(g, w);
+    const int c = m.comp;
+    const double *r = w.r[c];
+    double *d = w.d[c][0];
+    auto rv = [&](int a, int b) { return AT(r, a, b); };
+    double acc = 0.0;
+    for (int j = m.j0; j < m.j1; j++)
+    for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) {
+        const double zz = mom_pc_z(g, w, c, i, j, rv);
+        AT(d, i, j) = zz;
+        acc += AT(r, i, j) * zz;
+    }
+    block_store_partial(acc, w.rz[c][0], m.band, sh);
+}
+
+// pc = 1 update: alpha as k_mcg_update; x += alpha d; r_{n+1} = r_n - alpha
+// A d_n into the other r buffer; z_{n+1} = M r_{n+1}; partials <r, z>, <r, r>
+__global__ void __launch_bounds__(NTH) k_mcg_update_pc(pvk_grid g, pvk_scal *sc, pvk_mcg w, int n)
+{
+    __shared__ double sh[NTH];
+    const mom_band m = mom_locate(g, w);
+    const int c = m.comp;
+    if (sc->m_done[c]) return;
+    const int P = g.P, NXM = g.NXM;
+    const double EPS = 1e-14;
+    const double dAd = block_sum(w.dad[c], w.nb[c], sh);
+    const double rzn = sc->m_rzb[c][n & 1];
+    const double alpha = w.exact ? (dAd != 0.0 ? rzn / dAd : 0.0) : rzn / (dAd + EPS);
+    if (m.band == 0 && threadIdx.x == 0) sc->m_iter[c] = n + 1;
+    double *x = w.x[c], *z = w.z[c];
+    const double *rold = (n & 1) ? w.r2[c] : w.r[c];
+    double *rnew = (n & 1) ? w.r[c] : w.r2[c];
+    const double *d = w.d[c][n & 1], *Ad = w.Ad[c];
+    auto rv = [&](int a, int b) { return AT(rold, a, b) - alpha * AT(Ad, a, b); };
+    double a0 = 0.0, a1 = 0.0;
+    for (int j = m.j0; j < m.j1; j++)
+    for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) {
+        AT(x, i, j) += alpha * AT(d, i, j);
+        const double ri = rv(i, j);
+        const double zz = mom_pc_z(g, w, c, i, j, rv);
+        AT(rnew, i, j) = ri;
+        AT(z, i, j) = zz;
+        a0 += ri * zz;
+        a1 += ri * ri;
+    }
+    block_store_partial(a0, w.rz[c][(n + 1) & 1], m.band, sh);
+    block_store_partial(a1, w.rr[c], m.band, sh);
+}
+
+// pc = 1, after the loop: a component that stopped after an odd number of
+// updates holds its residual in r2: copy it to r (the public field)
+__global__ void __launch_bounds__(NTH) k_mcg_pc_finish(pvk_grid g, const pvk_scal *sc, pvk_mcg w)
+{
+    const int P = g.P, NXM = g.NXM;
+    const mom_band m = mom_locate(g, w);
+    const int c = m.comp;
+    if (!(sc->m_iter[c] & 1)) return;
+    for (int j = m.j0; j < m.j1; j++)
+    for (int i = m.i0 + (int)threadIdx.x; i < m.i0 + NXM; i += NTH) AT(w.r[c], i, j) = AT(w.r2[c], i, j);
+}
+
+extern "C" void pvk_mcg_pc_start(const pvk_grid *g, const pvk_mcg *w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mcg_pc_start, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *w);
+}
+
+extern "C" void pvk_mcg_pc_finish(const pvk_grid *g, const pvk_scal *sc, const pvk_mcg *w, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_mcg_pc_finish, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w);
+}
+
 extern "C" void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                              hipStream_t s)
 {
@@ -529,7 +635,9 @@ extern "C" void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, p
         if (w->band == 1) MCG_APPLY(double, 1);
         else if (w->band == 2) MCG_APPLY(double, 2);
         else MCG_APPLY(double, 4);
-    } else
+    } else if (w->pc)
 #undef MCG_APPLY
+        hipLaunchKernelGGL(k_mcg_update_pc, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
+    else
         hipLaunchKernelGGL(k_mcg_update, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
 }

@@ -133,6 +133,9 @@ typedef struct pvk_mcg {             /* momentum Jacobi-PCG, [0] = u, [1] = v */
     int exact;                       /* 0: EPS = 1e-14 added to the step denominators (the
                                         reference); 1: the denominators as they are, a zero
                                         one giving a zero step */
+    int pc;                          /* 0: Jacobi (the reference); 1: first-order Neumann
+                                        polynomial of the Jacobi splitting (FP32 form) */
+    double *r2[2], *z[2];            /* pc = 1: r of odd iterations, preconditioned residual */
     double tol, in_tot[2];           /* CG_ETOL; 1 / number of unknowns */
 } pvk_mcg;
 
@@ -213,6 +216,10 @@ void pvk_mom_matvec(const pvk_grid *g, const pvk_phys *ph, int stage, const pvk_
 /* fused momentum Jacobi-PCG for the components with nb[c] > 0 */
 void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                   hipStream_t s);
+/* pc = 1: z_0 = M r_0 into d[c][0] and <r_0, z_0> after pvk_mcg_init; after the
+ * loop, the final residual back into r[c] when it ended in r2[c] */
+void pvk_mcg_pc_start(const pvk_grid *g, const pvk_mcg *w, hipStream_t s);
+void pvk_mcg_pc_finish(const pvk_grid *g, const pvk_scal *sc, const pvk_mcg *w, hipStream_t s);
 void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                   int half, int n, hipStream_t s);
 

@@ -313,6 +313,8 @@ int pv_cg_create(pv_bag *b)
         b->mcg.Ad[c] = b->f[c ? PV_F_V_AD : PV_F_U_AD];
         b->mcg.d[c][0] = b->f[c ? PV_F_V_D : PV_F_U_D];
         b->mcg.d[c][1] = b->f[c ? PV_FI_DV2 : PV_FI_DU2];
+        b->mcg.z[c] = b->f[c ? PV_FI_MZV : PV_FI_MZU];
+        b->mcg.r2[c] = b->f[c ? PV_FI_MRV2 : PV_FI_MRU2];
         b->mcg.rhs[c] = b->f[c ? PV_F_V_RHS : PV_F_U_RHS];
         b->mcg.rz[c][0] = q; q += nbk;
         b->mcg.rz[c][1] = q; q += nbk;

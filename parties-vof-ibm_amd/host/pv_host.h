@@ -36,6 +36,10 @@ enum pv_internal_field {
     PV_FI_MCF3,
     PV_FI_MCF4,
     PV_FI_MCF5,
+    PV_FI_MZU,                 /* momentum polynomial preconditioner: z of u and v ... */
+    PV_FI_MZV,
+    PV_FI_MRU2,                /* ... and the residuals of odd iterations */
+    PV_FI_MRV2,
     PV_FI_END
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
@@ -109,6 +113,7 @@ struct pv_bag {
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */
     int m_warm;                    /* 1: start each momentum solve from u + the stage's last increment */
     int m_exact;                   /* 1: momentum CG steps without the EPS = 1e-14 denominators */
+    int m_pc;                      /* momentum preconditioner: 0 Jacobi, 1 Neumann polynomial */
     int have_mprev[3];             /* PV_FI_MU0 / MV0 + stage hold an increment ... */
     double mprev_dt[3];            /* ... of a step of this dt */
     int warm_wins[3], warm_solves[3];  /* per stage: the temporal start's consecutive wins, solves since
@@ -148,6 +153,7 @@ int  pv_set_mg_precision(pv_bag *b, int fp32);
 int  pv_set_ch_solver(pv_bag *b, int spectral);
 int  pv_set_mg_tile0(pv_bag *b, int shape);
 int  pv_set_momentum_rows(pv_bag *b, int rows);
+int  pv_set_momentum_preconditioner(pv_bag *b, int kind);
 int  pv_set_fft_tri_shape(pv_bag *b, int rows, int cols);
 void pv_mg_destroy(pv_bag *b);
 void pv_cnn_free(pv_bag *b);

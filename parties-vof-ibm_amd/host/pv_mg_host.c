@@ -220,6 +220,17 @@ int pv_set_momentum_warm_start(pv_bag *b, int mode)
     return PV_OK;
 }
 
+int pv_set_momentum_preconditioner(pv_bag *b, int kind)
+{
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    if (kind < 0 || kind > 1) return pv_set_error(PV_EINVAL, "momentum preconditioner must be 0 or 1 (got %d)", kind);
+    if (kind == 1 && !b->ph.precision)
+        return pv_set_error(PV_EINVAL, "the polynomial momentum preconditioner needs the mixed-precision form "
+                                       "(it reads the FP32 operator coefficients)");
+    b->m_pc = kind;
+    return PV_OK;
+}
+
 int pv_set_momentum_cg_steps(pv_bag *b, int exact)
 {
     if (!b) return pv_set_error(PV_EINVAL, "null handle");

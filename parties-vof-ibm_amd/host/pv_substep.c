@@ -334,11 +334,13 @@ static int mom_solve(pv_bag *b, int mask)
     int st = b->stage, n = 0, maxn = b->ph.CG_MAXIT + 1, c, chunk;
     pvk_mcg w = b->mcg;
     w.exact = b->m_exact;
+    w.pc = b->m_pc;
     for (c = 0; c < 2; c++)
         if (!(mask & (1 << c))) w.nb[c] = 0;
     chunk = (mask & 1) ? b->mom_chunk[0] : b->mom_chunk[1];
     if ((mask & 2) && b->mom_chunk[1] > chunk) chunk = b->mom_chunk[1];
     pvk_mcg_init(G(b), &b->ph, st, b->d_sc, &w, S(b));
+    if (w.pc) pvk_mcg_pc_start(G(b), &w, S(b));
     for (;;) {
         int k, done = 1;
         for (k = 0; k < chunk && n <= maxn; k++, n++) {
@@ -356,6 +358,10 @@ static int mom_solve(pv_bag *b, int mask)
         if (done) break;
         if (n > maxn) return pv_set_error(PV_EHIP, "momentum CG did not stop (device flag)");
         chunk = 2;
+    }
+    if (w.pc) {
+        pvk_mcg_pc_finish(G(b), b->d_sc, &w, S(b));
+        LAUNCHCHK();
     }
     for (c = 0; c < 2; c++) {
         if (!(mask & (1 << c))) continue;

@@ -172,6 +172,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_set_ch_solver.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_warm_start.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_cg_steps.argtypes = [vp, C.c_int]
+    lib.pv_set_momentum_preconditioner.argtypes = [vp, C.c_int]
     lib.pv_set_momentum_rows.argtypes = [vp, C.c_int]
     lib.pv_set_mg_tile0.argtypes = [vp, C.c_int]
     lib.pv_set_fft_tri_shape.argtypes = [vp, C.c_int, C.c_int]

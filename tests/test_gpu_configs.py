@@ -79,10 +79,13 @@ def test_config4_mixed_against_fp64_2048(lib):
         F = pvof.ic(lib, "bubbles", p, seed=bench.SEED, nmin=32, nmax=32, rmin=0.02, rmax=0.06, gas_inside=0)
         s = pvof.Solver(p, lib)
         s.init_from_F(F)
-        if name == "mixed":          # the fast form: spectral CH start, temporal start, W cycle
-            s.check(lib.pv_set_ch_solver(s.h, 1))
-            s.check(lib.pv_set_pressure_warm_start(s.h, 1))
+        if name == "mixed":          # the fast form: spectral CH start, temporal starts, W cycle,
+            s.check(lib.pv_set_ch_solver(s.h, 1))              # momentum CG exact steps, polynomial
+            s.check(lib.pv_set_pressure_warm_start(s.h, 1))    # preconditioner
             s.check(lib.pv_set_mg_cycle(s.h, 3))
+            s.check(lib.pv_set_momentum_warm_start(s.h, 1))
+            s.check(lib.pv_set_momentum_cg_steps(s.h, 1))
+            s.check(lib.pv_set_momentum_preconditioner(s.h, 1))
         runs[name] = s
     for step in range(steps):
         for name, s in runs.items():
