@@ -386,7 +386,7 @@ __global__ void __launch_bounds__(NTH) k_mcg_init(pvk_grid g, pvk_phys ph, int s
 // the five operator points through the ghost maps; Ad = A d_n; partial <d, Ad>.
 #define MCH 256                                                  // columns per chunk of k_mcg_apply
 #define MSQ (((BAND + 2) * (MCH + 2) + NTH - 1) / NTH)     // staged values per thread
-template <class R, int BAND>
+template <class R, int BAND, bool PC>
 __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int stage, pvk_scal *sc, pvk_mcg w,
                                                   int n)
 {
@@ -428,13 +428,13 @@ __global__ void __launch_bounds__(NTH) k_mcg_apply(pvk_grid g, pvk_phys ph, int 
             const int ia = wrapu(a, NXM);
             const int jb = b < 0 ? 0 : (b >= NYM ? NYM - 1 : b);
             double val = (n == 0) ? AT(dwall, ia, jb)
-                                  : (w.pc ? AT(zc, ia, jb) : AT(r, ia, jb) / AT(dg, ia, jb)) + beta * AT(dold, ia, jb);
+                                  : (PC ? AT(zc, ia, jb) : AT(r, ia, jb) / AT(dg, ia, jb)) + beta * AT(dold, ia, jb);
             return (b < 0 || b >= NYM) ? -val : val;
         }
         const int ia = wrapc(a, NXM);
         if (b <= 0 || b >= NYM) return AT(dwall, ia, b);
         return (n == 0) ? AT(dwall, ia, b)
-                        : (w.pc ? AT(zc, ia, b) : AT(r, ia, b) / AT(dg, ia, b)) + beta * AT(dold, ia, b);
+                        : (PC ? AT(zc, ia, b) : AT(r, ia, b) / AT(dg, ia, b)) + beta * AT(dold, ia, b);
     };
     // The band (w.band = BAND rows, the last one may have fewer) in chunks of MCH columns: d of the
     // chunk and a one-cell ring is formed once into LDS (every value as dv
@@ -625,16 +625,20 @@ extern "C" void pvk_mcg_init(const pvk_grid *g, const pvk_phys *ph, int stage, p
 extern "C" void pvk_mcg_iter(const pvk_grid *g, const pvk_phys *ph, int stage, pvk_scal *sc, const pvk_mcg *w,
                              int half, int n, hipStream_t s)
 {
-#define MCG_APPLY(RT, B) \
-    hipLaunchKernelGGL((k_mcg_apply<RT, B>), dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n)
-    if (half == 0 && ph->precision) {
-        if (w->band == 1) MCG_APPLY(float, 1);
-        else if (w->band == 2) MCG_APPLY(float, 2);
-        else MCG_APPLY(float, 4);
+#define MCG_APPLY(RT, B, PC) \
+    hipLaunchKernelGGL((k_mcg_apply<RT, B, PC>), dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, *ph, stage, sc, *w, n)
+    if (half == 0 && ph->precision && w->pc) {      // the polynomial preconditioner: FP32 form only
+        if (w->band == 1) MCG_APPLY(float, 1, true);
+        else if (w->band == 2) MCG_APPLY(float, 2, true);
+        else MCG_APPLY(float, 4, true);
+    } else if (half == 0 && ph->precision) {
+        if (w->band == 1) MCG_APPLY(float, 1, false);
+        else if (w->band == 2) MCG_APPLY(float, 2, false);
+        else MCG_APPLY(float, 4, false);
     } else if (half == 0) {
-        if (w->band == 1) MCG_APPLY(double, 1);
-        else if (w->band == 2) MCG_APPLY(double, 2);
-        else MCG_APPLY(double, 4);
+        if (w->band == 1) MCG_APPLY(double, 1, false);
+        else if (w->band == 2) MCG_APPLY(double, 2, false);
+        else MCG_APPLY(double, 4, false);
     } else if (w->pc)
 #undef MCG_APPLY
         hipLaunchKernelGGL(k_mcg_update_pc, dim3(w->nb[0] + w->nb[1]), dim3(NTH), 0, s, *g, sc, *w, n);
