@@ -270,12 +270,14 @@ void pvk_pcg_apply(const pvk_grid *g, pvk_scal *sc, const pvk_pcg *w, int n, hip
  * level 0 x = z, f = r, faces = the PCG faces (pitch P).  FP32: level 0 x and
  * faces are FP32 copies (pitch nx), r is read from r0 and z written to z0
  * (pitch p0).  Levels c0.. (<= PVK_MG_LDS_CELLS cells) run inside one
- * workgroup; the last level (halving stops at <= PVK_MG_BOTTOM_MIN cells or
- * odd sizes; <= PVK_MG_BOTTOM cells) is solved with the grounded inverse G. */
+ * workgroup; the last level (halving stops at <= PVK_MG_BOTTOM_MIN cells and
+ * <= PVK_MG_BOTTOM_NX columns, or at odd sizes; <= PVK_MG_BOTTOM cells) is
+ * solved with the grounded inverse G. */
 #define PVK_MG_MAXLEV 16
 #define PVK_MG_LDS_CELLS 4096
-#define PVK_MG_BOTTOM 64
-#define PVK_MG_BOTTOM_MIN 64
+#define PVK_MG_BOTTOM 256
+#define PVK_MG_BOTTOM_MIN 256
+#define PVK_MG_BOTTOM_NX 64            /* the bottom's width: the band of its factorisation */
 #define PVK_MG_LOW 3                  /* most levels below the first in-LDS level */
 typedef struct pvk_mglev {
     int nx, ny, pitch;
@@ -295,6 +297,7 @@ typedef struct pvk_mg {
     double *z0;                       /* FP32: level-0 result (z), pitch p0 */
     int p0;
     double *G;                        /* bottom matrix P G P (G the grounded inverse), nb x nb */
+    double *cm;                       /* nb column means of G (scratch of the set-up) */
 } pvk_mg;
 void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);
 int  pvk_mg_nrz(const pvk_mg *m);

@@ -14,8 +14,9 @@
 //   restriction   sum of the four fine residuals;
 //   prolongation  piecewise constant;
 //   coarse faces  half the sum of the two fine faces across the coarse face;
-//   the last level (<= PVK_MG_BOTTOM cells) is solved with the inverse of its
-//   operator grounded at cell 0, both sides projected to zero mean.
+//   the last level (<= PVK_MG_BOTTOM cells) is solved exactly with the
+//   inverse of its operator grounded at cell 0, both sides projected to zero
+//   mean (built per solve by a band Cholesky, one column per workgroup).
 // Each level above PVK_MG_LDS_CELLS cells is two launches (pre-smoothing +
 // restriction; correction + post-smoothing) on LDS tiles with a halo, so f,
 // the faces and x cross HBM once per launch; the levels below run in one
@@ -336,71 +337,97 @@ __global__ void __launch_bounds__(NTH) k_mg_faces0(Lev<double> L, Lev<float> C)
     C.by[(ptrdiff_t)J * C.pitch + I] = (float)ld(L.by, L.pitch, I, J);
 }
 
-// H = P G P, G the inverse of the bottom operator -A_B + s e0 e0^T (s = its
-// (0,0) entry) and P the projection to zero mean, in FP64: the bottom solve
-// is then x = -H f.  G by in-place Gauss-Jordan without pivoting (the matrix
-// is symmetric positive definite); one workgroup of 1024 threads, thread
-// (r, q) owns row r, columns q + 16 t.  (P G P)_rc = G_rc - (row mean)_r -
-// (column mean)_c + (mean of the column means).
-#define BT 1024
-template <typename T> __global__ void __launch_bounds__(BT) k_mg_bottom(Lev<T> B, double *G)
+// H = P G P, G the inverse of the bottom operator M = -A_B + s e0 e0^T (s its
+// (0,0) entry: grounded at cell 0, symmetric positive definite) and P the
+// projection to zero mean, in FP64: the bottom solve is then x = -H f.
+// Numbered x fastest, M is banded: the x neighbours (the periodic one
+// included) lie within nx - 1 of the diagonal, the y neighbours at nx.  Block
+// c (one wavefront) factors M = L L^T in LDS in band storage and solves
+// L L^T g = e_c: column c of G, written as row c (G is symmetric), and its
+// mean.  Every block repeats the factorisation (nb blocks, one per CU, run
+// side by side).  A second kernel forms (P G P)_rc = G_rc - m_r - m_c + mean
+// of the m, m the column means (equal to the row means of a symmetric G).
+#define BW 64
+template <typename T> __global__ void __launch_bounds__(BW) k_mg_bottom_chol(Lev<T> B, double *G, double *cm)
 {
-    __shared__ double M[PVK_MG_BOTTOM][PVK_MG_BOTTOM + 1];
-    const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch;
-    const int r = threadIdx.x >> 4, q = threadIdx.x & 15;
-    for (int c = q; c < PVK_MG_BOTTOM; c += 16) M[r][c] = 0.0;
+    extern __shared__ double lb[];         // lb[r W + d] = M[r][r - d] (L after the factorisation), d <= nx
+    const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch, W = nx + 1;
+    const int c = blockIdx.x, t = threadIdx.x;
+    double *y = lb + (size_t)nb * W;       // the solve's vector
+    for (int q = t; q < nb * W; q += BW) lb[q] = 0.0;
     __syncthreads();
-    if (q == 0 && r < nb) {
+    // row r's diagonal and its couplings to the columns before it: the west
+    // neighbour, the periodic east neighbour (of i = nx - 1) and the south one
+    for (int r = t; r < nb; r += BW) {
         const int j = r / nx, i = r - j * nx;
         const int ie = (i + 1 == nx) ? 0 : i + 1, iw = (i == 0) ? nx - 1 : i - 1;
         const double ce = ld(B.bx, p, i, j), cw = ld(B.bx, p, iw, j);
         const double cn = (j + 1 < ny) ? (double)ld(B.by, p, i, j) : 0.0;
         const double cs = (j > 0) ? (double)ld(B.by, p, i, j - 1) : 0.0;
-        M[r][r] += ce + cw + cn + cs;
-        M[r][j * nx + ie] -= ce;
-        M[r][j * nx + iw] -= cw;
-        if (j + 1 < ny) M[r][(j + 1) * nx + i] -= cn;
-        if (j > 0) M[r][(j - 1) * nx + i] -= cs;
+        double dg = ce + cw + cn + cs;
+        if (ie == i) dg -= ce;             // nx = 1: the periodic image is the cell itself
+        if (iw == i) dg -= cw;
+        if (r == 0) dg += dg;              // grounding: the (0, 0) entry doubled
+        lb[r * W] = dg;
+        if (iw < i) lb[r * W + (i - iw)] -= cw;
+        if (ie < i) lb[r * W + (i - ie)] -= ce;
+        if (j > 0) lb[r * W + nx] -= cs;
     }
     __syncthreads();
-    if (threadIdx.x == 0) M[0][0] += M[0][0];
-    __syncthreads();
+    // right-looking band Cholesky
     for (int k = 0; k < nb; k++) {
-        // read phase: the pivot row and this row's factor, as they stand
-        const double piv = 1.0 / M[k][k], fr = M[r][k];
-        double mk[PVK_MG_BOTTOM / 16], mr[PVK_MG_BOTTOM / 16];
-#pragma unroll
-        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-            const int c = q + 16 * t;
-            mk[t] = (c == k) ? piv : M[k][c] * piv;           // new pivot row
-            mr[t] = M[r][c];
-        }
+        const double dkk = sqrt(lb[k * W]), inv = 1.0 / dkk;
+        const int m = min(nx, nb - 1 - k);
         __syncthreads();
-        if (r < nb) {
-#pragma unroll
-            for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-                const int c = q + 16 * t;
-                if (c < nb) M[r][c] = (r == k) ? mk[t] : ((c == k) ? 0.0 : mr[t]) - fr * mk[t];
-            }
+        if (t == 0) lb[k * W] = dkk;
+        for (int i = 1 + t; i <= m; i += BW) lb[(k + i) * W + i] *= inv;
+        __syncthreads();
+        for (int q = t; q < m * m; q += BW) {              // M[k+i][k+j] -= L[k+i][k] L[k+j][k], j <= i
+            const int i = 1 + q / m, j = 1 + q - (i - 1) * m;
+            if (j <= i) lb[(k + i) * W + (i - j)] -= lb[(k + i) * W + i] * lb[(k + j) * W + j];
         }
         __syncthreads();
     }
-    __shared__ double rm[PVK_MG_BOTTOM], cm[PVK_MG_BOTTOM];
-    if (threadIdx.x < (unsigned)nb) {
-        double a = 0.0, b = 0.0;
-        for (int c = 0; c < nb; c++) {
-            a += M[threadIdx.x][c];
-            b += M[c][threadIdx.x];
-        }
-        rm[threadIdx.x] = a / nb;
-        cm[threadIdx.x] = b / nb;
-    }
+    // L y = e_c, then L^T g = y (g in y), sums over the band by the wavefront
+    for (int r = t; r < nb; r += BW) y[r] = 0.0;
     __syncthreads();
-    double tm = 0.0;
-    for (int c = 0; c < nb; c++) tm += cm[c];
-    tm /= nb;
-    if (r < nb)
-        for (int c = q; c < nb; c += 16) G[r * nb + c] = ((M[r][c] - rm[r]) - cm[c]) + tm;
+    for (int r = c; r < nb; r++) {
+        double v = 0.0;
+        for (int d = 1 + t; d <= min(nx, r - c); d += BW) v += lb[r * W + d] * y[r - d];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, BW);
+        if (t == 0) y[r] = ((r == c ? 1.0 : 0.0) - v) / lb[r * W];
+        __syncthreads();
+    }
+    for (int r = nb - 1; r >= 0; r--) {
+        double v = 0.0;
+        for (int d = 1 + t; d <= min(nx, nb - 1 - r); d += BW) v += lb[(r + d) * W + d] * y[r + d];
+        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, BW);
+        if (t == 0) y[r] = (y[r] - v) / lb[r * W];
+        __syncthreads();
+    }
+    double s = 0.0;
+    for (int r = t; r < nb; r += BW) {
+        G[(size_t)c * nb + r] = y[r];
+        s += y[r];
+    }
+    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, BW);
+    if (t == 0) cm[c] = s / nb;
+}
+
+__global__ void __launch_bounds__(NTH) k_mg_bottom_proj(int nb, double *G, const double *cm)
+{
+    __shared__ double sh[NTH];
+    const int r = blockIdx.x, t = threadIdx.x;
+    double s = 0.0;
+    for (int q = t; q < nb; q += NTH) s += cm[q];
+    sh[t] = s;
+    __syncthreads();
+    for (int w = NTH / 2; w > 0; w >>= 1) {
+        if (t < w) sh[t] += sh[t + w];
+        __syncthreads();
+    }
+    const double tm = sh[0] / nb, mr = cm[r];
+    for (int c = t; c < nb; c += NTH) G[(size_t)r * nb + c] = ((G[(size_t)r * nb + c] - mr) - cm[c]) + tm;
 }
 
 // ---------------------------------------------------------------------------
@@ -624,7 +651,7 @@ template <typename T> __device__ __forceinline__ T *lds_scratch(const pvk_mg &m,
     return q;
 }
 
-// the bottom matrix in LDS, transposed with row stride HS, after the
+// the bottom solve's partial sums (4 PVK_MG_BOTTOM doubles) after the
 // restriction scratch, 8-byte aligned
 template <typename T> __device__ __forceinline__ double *lds_bottom(const pvk_mg &m, T *lds)
 {
@@ -633,22 +660,25 @@ template <typename T> __device__ __forceinline__ double *lds_bottom(const pvk_mg
     return (double *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
 }
 
-// bottom level, x = -H f (H = P G P, FP64), H's transpose in LDS (entry
-// (r, c) at Hs[c * HS + r]; the padded stride spreads the sixteen column
-// reads of a row over the banks).  Row r is applied by the 16 lanes
-// 16 r .. 16 r + 15, four columns each.
-#define HS (PVK_MG_BOTTOM + 1)
-template <typename T> __device__ __forceinline__ void bottom_solve(const LdsLev<T> &B, const double *Hs)
+// bottom level, x = -H f (H = P G P, FP64, symmetric), H read from memory
+// (up to 512 KiB: it stays in L2 across the solves of a cycle).  Thread
+// (q, r) = (threadIdx / 256, threadIdx % 256) sums a quarter of the columns
+// for row r, reading H[c][r] = H[r][c]: consecutive threads read consecutive
+// doubles of row c.  The four quarters are added in order through LDS.
+template <typename T> __device__ __forceinline__ void bottom_solve(const LdsLev<T> &B, const double *H, double *part)
 {
-    const int n = B.nx * B.ny, r = threadIdx.x >> 4, q = threadIdx.x & 15;
+    static_assert(CT == 4 * PVK_MG_BOTTOM, "four quarters of the bottom's columns, one row a thread");
+    const int n = B.nx * B.ny, r = threadIdx.x % PVK_MG_BOTTOM, q = threadIdx.x / PVK_MG_BOTTOM;
+    const int qn = (n + 3) / 4, c0 = q * qn, c1 = min(n, c0 + qn);
     double v = 0.0;
-#pragma unroll
-    for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-        const int c = q + 16 * t;
-        if (r < n && c < n) v -= Hs[c * HS + r] * (double)B.f[c];
+    if (r < n) {
+#pragma unroll 8
+        for (int c = c0; c < c1; c++) v -= H[(size_t)c * n + r] * (double)B.f[c];
     }
-    for (int off = 8; off > 0; off >>= 1) v += __shfl_xor(v, off, 16);
-    if (q == 0 && r < n) B.x[r] = (T)v;
+    part[threadIdx.x] = v;
+    __syncthreads();
+    if (q == 0 && r < n)
+        B.x[r] = (T)(((part[r] + part[PVK_MG_BOTTOM + r]) + part[2 * PVK_MG_BOTTOM + r]) + part[3 * PVK_MG_BOTTOM + r]);
     __syncthreads();
 }
 
@@ -660,16 +690,6 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     const int nb = m.nlev - 1;
     // The loads below are valid whatever the stop flag says, so they are
     // issued before it is read: one memory round trip instead of two.
-    // This thread's entries of the bottom matrix H (row threadIdx / 16,
-    // columns threadIdx % 16 + 16 t), loaded first: their latency hides
-    // behind the rest.
-    const int nbot = m.L[nb].nx * m.L[nb].ny, br = threadIdx.x >> 4, bq = threadIdx.x & 15;
-    double hv[PVK_MG_BOTTOM / 16];
-#pragma unroll
-    for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-        const int c = bq + 16 * t;
-        hv[t] = (br < nbot && c < nbot) ? m.G[(size_t)br * nbot + c] : 0.0;
-    }
     // f of level c0 (and its faces with topf), the faces of the levels below
     // it: every global load of this thread is issued before the first LDS
     // store, so the loads wait for memory once, not once per level and pass.
@@ -703,13 +723,8 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     // the level descriptors, formed once: the schedule below reads them from
     // LDS instead of walking the kernel argument with a run-time index
     if (threadIdx.x <= (unsigned)(nb - m.c0)) lv[threadIdx.x] = lds_level(m, lds, m.c0 + (int)threadIdx.x);
-    double *Hs = lds_bottom(m, lds);               // H's transpose
+    double *part = lds_bottom(m, lds);             // the bottom solve's partial sums
     {
-#pragma unroll
-        for (int t = 0; t < PVK_MG_BOTTOM / 16; t++) {
-            const int c = bq + 16 * t;
-            if (br < nbot && c < nbot) Hs[c * HS + br] = hv[t];
-        }
         const LdsLev<T> L0 = lds_level(m, lds, m.c0);
 #pragma unroll
         for (int u = 0; u < 4; u++) {
@@ -745,7 +760,7 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     unsigned second = 0u;
     for (;;) {
         if (l == nb) {
-            bottom_solve(lv[nb - m.c0], Hs);
+            bottom_solve(lv[nb - m.c0], (const double *)m.G, part);
             l--;
             down = false;
             if (l < m.c0) break;
@@ -877,8 +892,8 @@ extern "C" size_t pvk_mg_lds_bytes(const pvk_mg *m)
     size_t n = 0;
     for (int l = m->c0; l < m->nlev; l++) n += ((l == m->c0 && !m->topf) ? 2 : 4) * (size_t)m->L[l].nx * m->L[l].ny;
     if (m->nlev > m->c0 + 1) n += (size_t)(m->L[m->c0].nx / 2) * m->L[m->c0].ny;   // restriction scratch
-    // + the bottom matrix (FP64, 8-byte aligned, row stride HS)
-    return n * (m->fp32 ? sizeof(float) : sizeof(double)) + 8 + sizeof(double) * PVK_MG_BOTTOM * HS;
+    // + the bottom solve's partial sums (FP64, 8-byte aligned)
+    return n * (m->fp32 ? sizeof(float) : sizeof(double)) + 8 + sizeof(double) * 4 * PVK_MG_BOTTOM;
 }
 
 extern "C" int pvk_mg_nrz(const pvk_mg *m)
@@ -886,6 +901,32 @@ extern "C" int pvk_mg_nrz(const pvk_mg *m)
     if (m->c0 == 0) return 1;
     const dim3 gr = tile_grid(m, 0);
     return (int)(gr.x * gr.y);
+}
+
+// the bottom matrix of a solve: G column by column, then P G P.  Above 64 KiB
+// of LDS (bottoms wider than 16 columns) the factorisation kernel must be
+// allowed it, raised once per device under a lock as for k_mg_coarse.
+template <typename T> static void bottom_setup(const pvk_mg *m, hipStream_t s)
+{
+    const pvk_mglev &Lb = m->L[m->nlev - 1];
+    const int nb = Lb.nx * Lb.ny;
+    const size_t need = sizeof(double) * ((size_t)nb * (Lb.nx + 1) + nb);
+    if (need > 65536) {
+        static std::mutex mu;
+        static unsigned char raised[64];
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        std::lock_guard<std::mutex> lk(mu);
+        if (dev >= 0 && dev < 64 && !(raised[dev] & (1u << sizeof(T)))) {
+            if (hipFuncSetAttribute((const void *)k_mg_bottom_chol<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024) == hipSuccess)
+                raised[dev] |= (unsigned char)(1u << sizeof(T));
+            else
+                (void)hipGetLastError();
+        }
+    }
+    hipLaunchKernelGGL(k_mg_bottom_chol<T>, dim3(nb), dim3(BW), need, s, lev<T>(Lb), m->G, m->cm);
+    hipLaunchKernelGGL(k_mg_bottom_proj, dim3(nb), dim3(NTH), 0, s, nb, m->G, (const double *)m->cm);
 }
 
 // per solve: coarse faces of every level (FP32: and the level-0 copies) and
@@ -907,12 +948,12 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *b
         for (int l = 1; l + 1 < m->nlev; l++)
             hipLaunchKernelGGL((k_mg_faces<float, float>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
                                lev<float>(m->L[l]), lev<float>(m->L[l + 1]));
-        hipLaunchKernelGGL(k_mg_bottom<float>, dim3(1), dim3(BT), 0, s, lev<float>(m->L[m->nlev - 1]), m->G);
+        bottom_setup<float>(m, s);
     } else {
         for (int l = 0; l + 1 < m->nlev; l++)
             hipLaunchKernelGGL((k_mg_faces<double, double>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
                                lev<double>(m->L[l]), lev<double>(m->L[l + 1]));
-        hipLaunchKernelGGL(k_mg_bottom<double>, dim3(1), dim3(BT), 0, s, lev<double>(m->L[m->nlev - 1]), m->G);
+        bottom_setup<double>(m, s);
     }
 }
 

@@ -268,7 +268,8 @@ static int create_on_device(pv_bag *b, const pv_params *prm)
         if ((b->prm.pressure_solver == PV_PSOLVE_MG_PCG || b->prm.pressure_solver == PV_PSOLVE_LEARNED_MG) &&
             !b->mg_ok)
             return pv_set_error(PV_EINVAL, "MG-PCG: %d x %d does not halve to <= %d cells and a last level "
-                                "of <= %d cells", prm->NXM, prm->NYM, PVK_MG_LDS_CELLS, PVK_MG_BOTTOM);
+                                "of <= %d cells and <= %d columns", prm->NXM, prm->NYM, PVK_MG_LDS_CELLS,
+                                PVK_MG_BOTTOM, PVK_MG_BOTTOM_NX);
         return pv_sync_scalars_to_device(b);
     }
 }

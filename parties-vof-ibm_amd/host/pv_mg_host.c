@@ -7,9 +7,10 @@
  * CG, device pitch P).  Levels halve both sizes while the level has more than
  * PVK_MG_LDS_CELLS cells (the first level at or below that size, c0, and all
  * levels after it run in one workgroup from LDS), then on while both sizes
- * are even and the level has more than PVK_MG_BOTTOM_MIN cells; the last
- * level must have at most PVK_MG_BOTTOM cells.  A grid that does not halve
- * that way has no MG-PCG (mg_ok = 0).
+ * are even and the level has more than PVK_MG_BOTTOM_MIN cells or more than
+ * PVK_MG_BOTTOM_NX columns; the last level must have at most PVK_MG_BOTTOM
+ * cells and PVK_MG_BOTTOM_NX columns (the band of its factorisation).  A
+ * grid that does not halve that way has no MG-PCG (mg_ok = 0).
  *
  * Two hierarchies of the same sizes are kept: mg (FP64 V-cycle) and mgf
  * (FP32 V-cycle inside the FP64 CG; its level 0 keeps FP32 copies of the
@@ -46,13 +47,14 @@ int pv_mg_create(pv_bag *b)
         nl++;
     }
     c0 = nl - 1;
-    while (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM_MIN && !(lx[nl - 1] & 1) && !(ly[nl - 1] & 1) &&
-           nl < PVK_MG_MAXLEV) {
+    while ((lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM_MIN || lx[nl - 1] > PVK_MG_BOTTOM_NX) && !(lx[nl - 1] & 1) &&
+           !(ly[nl - 1] & 1) && nl < PVK_MG_MAXLEV) {
         lx[nl] = lx[nl - 1] / 2;
         ly[nl] = ly[nl - 1] / 2;
         nl++;
     }
-    if (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM || nl - 1 - c0 > PVK_MG_LOW) return PV_OK;
+    if (lx[nl - 1] * ly[nl - 1] > PVK_MG_BOTTOM || lx[nl - 1] > PVK_MG_BOTTOM_NX || nl - 1 - c0 > PVK_MG_LOW)
+        return PV_OK;
     /* the smoothing levels (all but the last) pair their cells along x */
     for (l = 0; l + 1 < nl; l++)
         if (lx[l] & 1) return PV_OK;
@@ -70,8 +72,8 @@ int pv_mg_create(pv_bag *b)
     m->sh0 = mf->sh0 = 2;
     nrz = pvk_mg_nrz(m);
     m->sh0 = mf->sh0 = -1;
-    total = 5 * cells + 2 * (size_t)b->g.P * ny + 2 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + 5 * (size_t)ny +
-            3 * (size_t)nrz;
+    total = 5 * cells + 2 * (size_t)b->g.P * ny + 2 * ((size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + PVK_MG_BOTTOM) +
+            5 * (size_t)ny + 3 * (size_t)nrz;
     totalf = 5 * cells + 3 * nc0;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
@@ -98,6 +100,7 @@ int pv_mg_create(pv_bag *b)
         m->L[l].xp = q; q += n;
     }
     m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
+    m->cm = q; q += PVK_MG_BOTTOM;
     m->p0 = b->g.P;
     /* FP32 V-cycle: level 0 has its own x and face copies; r in, z out */
     qf = b->mgfmem;
@@ -115,6 +118,7 @@ int pv_mg_create(pv_bag *b)
         mf->L[l].xp = qf; qf += n;
     }
     mf->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
+    mf->cm = q; q += PVK_MG_BOTTOM;
     mf->r0 = b->f[PV_F_P_RES];
     mf->z0 = b->f[PV_FI_Z];
     mf->p0 = b->g.P;

@@ -7,9 +7,10 @@ psolve_cg.c:63-101), x periodic, zero-gradient walls in y:
     (A x)_ij = bx_ij (x_E - x) - bx_(i-1)j (x - x_W) + by_ij (x_N - x) - by_i(j-1) (x - x_S).
 One V-cycle V(f), arrays indexed [j][i]:
   * levels halve both sizes while a level has more than 4096 cells, then
-    while both sizes are even and the level has more than 64 cells; the last
-    level must have at most 64 cells; coarse faces are half the sum of the
-    two fine faces across the coarse face;
+    while both sizes are even and the level has more than 256 cells or more
+    than 64 columns; the last level must have at most 256 cells and 64
+    columns; coarse faces are half the sum of the two fine faces across the
+    coarse face;
   * nu red-black Gauss-Seidel sweeps (red = (i + j) even first) from x = 0,
     restriction by the sum of the four fine residuals, the coarse V-cycle,
     piecewise-constant prolongation, nu sweeps black first; on the levels in
@@ -22,8 +23,9 @@ catch it): "post_red_first", "restrict_mean", "coarse_sum".
 import numpy as np
 
 LDS_CELLS = 4096
-BOTTOM_MIN = 64
-BOTTOM = 64
+BOTTOM_MIN = 256
+BOTTOM = 256
+BOTTOM_NX = 64
 
 
 def faces(rho, idx2, idy2):
@@ -44,10 +46,11 @@ def apply_a(bx, by, x):
 
 def level_sizes(nx, ny):
     sizes = [(nx, ny)]
-    while sizes[-1][0] * sizes[-1][1] > BOTTOM_MIN and sizes[-1][0] % 2 == 0 and sizes[-1][1] % 2 == 0:
+    while ((sizes[-1][0] * sizes[-1][1] > BOTTOM_MIN or sizes[-1][0] > BOTTOM_NX)
+           and sizes[-1][0] % 2 == 0 and sizes[-1][1] % 2 == 0):
         sizes.append((sizes[-1][0] // 2, sizes[-1][1] // 2))
-    if sizes[-1][0] * sizes[-1][1] > BOTTOM:
-        raise ValueError(f"{nx} x {ny} does not halve to a last level of <= {BOTTOM} cells")
+    if sizes[-1][0] * sizes[-1][1] > BOTTOM or sizes[-1][0] > BOTTOM_NX:
+        raise ValueError(f"{nx} x {ny} does not halve to a last level of <= {BOTTOM} cells and <= {BOTTOM_NX} columns")
     return sizes
 
 

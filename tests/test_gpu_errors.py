@@ -3,7 +3,7 @@
 Every rejected call returns PV_EINVAL (-1) with a message from
 pv_last_error, launches nothing, and leaves the handle usable: an RK3 step
 on it afterwards meets the pressure tolerance.  Size limits: MG-PCG needs a
-grid that halves to <= 4096 cells and on to a last level of <= 64 cells
+grid that halves to <= 4096 cells and on to a last level of <= 256 cells
 (rejected at pv_create), FFT-PCG needs NXM = 2^a 3^b, a multiple of 16
 (rejected by the pressure solve, the only routine that needs the transform).
 """
@@ -36,7 +36,7 @@ def message(lib):
 
 
 def test_mg_pcg_rejects_grid_that_does_not_halve(lib):
-    """98 x 98 halves once, to 49 x 49 (2401 cells): no last level of <= 64 cells."""
+    """98 x 98 halves once, to 49 x 49 (2401 cells): no last level of <= 256 cells."""
     h = C.c_void_p()
     p = params(98, pvof.PV_PSOLVE_MG_PCG)
     assert lib.pv_create(C.byref(p), 0, C.byref(h)) == PV_EINVAL

@@ -97,17 +97,20 @@ def test_last_level_alone_is_the_exact_inverse():
 
 
 def test_level_sizes():
-    assert M.level_sizes(1024, 1024)[-1] == (8, 8)
-    assert M.level_sizes(96, 96)[-1] == (6, 6)
-    assert M.level_sizes(384, 768)[-1] == (3, 6)
-    assert M.level_sizes(32, 64)[-1] == (4, 8)
+    assert M.level_sizes(1024, 1024)[-1] == (16, 16)
+    assert M.level_sizes(96, 96)[-1] == (12, 12)
+    assert M.level_sizes(384, 768)[-1] == (6, 12)
+    assert M.level_sizes(32, 64)[-1] == (8, 16)
+    assert M.level_sizes(2048, 32)[-1] == (64, 1)         # 128 x 2 has 256 cells but is too wide
     with pytest.raises(ValueError):
         M.level_sizes(100, 100)
+    with pytest.raises(ValueError):
+        M.level_sizes(256, 2)                              # ends at 128 x 1: too wide
 
 
 @pytest.mark.parametrize("mutant", [None, "restrict_mean", "coarse_sum"])
 def test_mg_pcg_iterations_and_answer(mutant):
-    n = 64
+    n = 128                 # two levels above the 16 x 16 bottom: enough coarsening for the mutants to show
     _, bx, by = problem(n)
     rng = np.random.default_rng(4)
     xs = rng.standard_normal((n, n))
