@@ -482,7 +482,7 @@ def main():
                     help="mixed: FP32 WENO / chemical-potential / momentum stencils, FP64 fluxes, solves and dots")
     ap.add_argument("--warm-start", type=int, choices=[0, 1], default=1,
                     help="MG-PCG solvers: 1 = also try the same RK3 stage's last correction as the start")
-    ap.add_argument("--mg-cycle", type=int, choices=[0, 1, 2, 3, 4], default=1,
+    ap.add_argument("--mg-cycle", type=int, choices=[0, 1, 2, 3, 4], default=2,
                     help="MG solvers: the first k in-workgroup levels take two coarse corrections (0 = V-cycle)")
     ap.add_argument("--mg-sweeps", type=int, choices=[1, 2], default=2,
                     help="MG solvers: Gauss-Seidel sweeps before and after each coarse correction")

@@ -296,8 +296,9 @@ typedef struct pvk_mg {
     const double *r0;                 /* FP32: level-0 right-hand side (r), pitch p0 */
     double *z0;                       /* FP32: level-0 result (z), pitch p0 */
     int p0;
-    double *G;                        /* bottom matrix P G P (G the grounded inverse), nb x nb */
-    double *cm;                       /* nb column means of G (scratch of the set-up) */
+    double *G;                        /* the grounded inverse G, nb x nb (set-up) */
+    double *cm;                       /* nb column means of G (set-up) */
+    void *H;                          /* the bottom solve's P G P, nb x nb, in the cycle's type */
 } pvk_mg;
 void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);
 int  pvk_mg_nrz(const pvk_mg *m);

@@ -414,7 +414,7 @@ template <typename T> __global__ void __launch_bounds__(BW) k_mg_bottom_chol(Lev
     if (t == 0) cm[c] = s / nb;
 }
 
-__global__ void __launch_bounds__(NTH) k_mg_bottom_proj(int nb, double *G, const double *cm)
+template <typename T> __global__ void __launch_bounds__(NTH) k_mg_bottom_proj(int nb, const double *G, const double *cm, T *H)
 {
     __shared__ double sh[NTH];
     const int r = blockIdx.x, t = threadIdx.x;
@@ -427,7 +427,7 @@ __global__ void __launch_bounds__(NTH) k_mg_bottom_proj(int nb, double *G, const
         __syncthreads();
     }
     const double tm = sh[0] / nb, mr = cm[r];
-    for (int c = t; c < nb; c += NTH) G[(size_t)r * nb + c] = ((G[(size_t)r * nb + c] - mr) - cm[c]) + tm;
+    for (int c = t; c < nb; c += NTH) H[(size_t)r * nb + c] = (T)(((G[(size_t)r * nb + c] - mr) - cm[c]) + tm);
 }
 
 // ---------------------------------------------------------------------------
@@ -660,20 +660,22 @@ template <typename T> __device__ __forceinline__ double *lds_bottom(const pvk_mg
     return (double *)(((uintptr_t)q + 7) & ~(uintptr_t)7);
 }
 
-// bottom level, x = -H f (H = P G P, FP64, symmetric), H read from memory
-// (up to 512 KiB: it stays in L2 across the solves of a cycle).  Thread
-// (q, r) = (threadIdx / 256, threadIdx % 256) sums a quarter of the columns
-// for row r, reading H[c][r] = H[r][c]: consecutive threads read consecutive
-// doubles of row c.  The four quarters are added in order through LDS.
-template <typename T> __device__ __forceinline__ void bottom_solve(const LdsLev<T> &B, const double *H, double *part)
+// bottom level, x = -H f (H = P G P, symmetric, in the cycle's type: the
+// FP32 cycle reads half the bytes), H read from memory (up to 512 KiB; the
+// one workgroup streams it through one CU, so many loads are kept in
+// flight).  Thread (q, r) = (threadIdx / 256, threadIdx % 256) sums a
+// quarter of the columns for row r, in FP64, reading H[c][r] = H[r][c]:
+// consecutive threads read consecutive entries of row c.  The four quarters
+// are added in order through LDS.
+template <typename T> __device__ __forceinline__ void bottom_solve(const LdsLev<T> &B, const T *H, double *part)
 {
     static_assert(CT == 4 * PVK_MG_BOTTOM, "four quarters of the bottom's columns, one row a thread");
     const int n = B.nx * B.ny, r = threadIdx.x % PVK_MG_BOTTOM, q = threadIdx.x / PVK_MG_BOTTOM;
     const int qn = (n + 3) / 4, c0 = q * qn, c1 = min(n, c0 + qn);
     double v = 0.0;
     if (r < n) {
-#pragma unroll 8
-        for (int c = c0; c < c1; c++) v -= H[(size_t)c * n + r] * (double)B.f[c];
+#pragma unroll 16
+        for (int c = c0; c < c1; c++) v -= (double)H[(size_t)c * n + r] * (double)B.f[c];
     }
     part[threadIdx.x] = v;
     __syncthreads();
@@ -760,7 +762,7 @@ template <typename T> __global__ void __launch_bounds__(CT) k_mg_coarse(pvk_mg m
     unsigned second = 0u;
     for (;;) {
         if (l == nb) {
-            bottom_solve(lv[nb - m.c0], (const double *)m.G, part);
+            bottom_solve(lv[nb - m.c0], (const T *)m.H, part);
             l--;
             down = false;
             if (l < m.c0) break;
@@ -926,7 +928,8 @@ template <typename T> static void bottom_setup(const pvk_mg *m, hipStream_t s)
         }
     }
     hipLaunchKernelGGL(k_mg_bottom_chol<T>, dim3(nb), dim3(BW), need, s, lev<T>(Lb), m->G, m->cm);
-    hipLaunchKernelGGL(k_mg_bottom_proj, dim3(nb), dim3(NTH), 0, s, nb, m->G, (const double *)m->cm);
+    hipLaunchKernelGGL(k_mg_bottom_proj<T>, dim3(nb), dim3(NTH), 0, s, nb, (const double *)m->G,
+                       (const double *)m->cm, (T *)m->H);
 }
 
 // per solve: coarse faces of every level (FP32: and the level-0 copies) and

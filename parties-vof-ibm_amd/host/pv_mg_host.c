@@ -72,9 +72,9 @@ int pv_mg_create(pv_bag *b)
     m->sh0 = mf->sh0 = 2;
     nrz = pvk_mg_nrz(m);
     m->sh0 = mf->sh0 = -1;
-    total = 5 * cells + 2 * (size_t)b->g.P * ny + 2 * ((size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM + PVK_MG_BOTTOM) +
-            5 * (size_t)ny + 3 * (size_t)nrz;
-    totalf = 5 * cells + 3 * nc0;
+    total = 5 * cells + 2 * (size_t)b->g.P * ny + 3 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM +
+            2 * (size_t)PVK_MG_BOTTOM + 5 * (size_t)ny + 3 * (size_t)nrz;
+    totalf = 5 * cells + 3 * nc0 + (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
         return pv_set_error(PV_ENOMEM, "MG-PCG work allocation failed");
@@ -101,6 +101,7 @@ int pv_mg_create(pv_bag *b)
     }
     m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     m->cm = q; q += PVK_MG_BOTTOM;
+    m->H = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     m->p0 = b->g.P;
     /* FP32 V-cycle: level 0 has its own x and face copies; r in, z out */
     qf = b->mgfmem;
@@ -119,6 +120,7 @@ int pv_mg_create(pv_bag *b)
     }
     mf->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     mf->cm = q; q += PVK_MG_BOTTOM;
+    mf->H = qf; qf += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     mf->r0 = b->f[PV_F_P_RES];
     mf->z0 = b->f[PV_FI_Z];
     mf->p0 = b->g.P;
