@@ -82,7 +82,7 @@ def test_config4_mixed_against_fp64_2048(lib):
         if name == "mixed":          # the fast form: spectral CH start, temporal starts, W cycle,
             s.check(lib.pv_set_ch_solver(s.h, 1))              # momentum CG exact steps, polynomial
             s.check(lib.pv_set_pressure_warm_start(s.h, 1))    # preconditioner
-            s.check(lib.pv_set_mg_cycle(s.h, 3))
+            s.check(lib.pv_set_mg_cycle(s.h, 2))
             s.check(lib.pv_set_momentum_warm_start(s.h, 1))
             s.check(lib.pv_set_momentum_cg_steps(s.h, 1))
             s.check(lib.pv_set_momentum_preconditioner(s.h, 1))
