@@ -298,6 +298,8 @@ typedef struct pvk_mg {
     int p0;
     double *G;                        /* the grounded inverse G, nb x nb (set-up) */
     double *cm;                       /* nb column means of G (set-up) */
+    double *Lb;                       /* the band Cholesky factor of the grounded bottom operator and
+                                         1 / its diagonal (set-up) */
     void *H;                          /* the bottom solve's P G P, nb x nb, in the cycle's type */
 } pvk_mg;
 void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);

@@ -341,20 +341,46 @@ __global__ void __launch_bounds__(NTH) k_mg_faces0(Lev<double> L, Lev<float> C)
 // (0,0) entry: grounded at cell 0, symmetric positive definite) and P the
 // projection to zero mean, in FP64: the bottom solve is then x = -H f.
 // Numbered x fastest, M is banded: the x neighbours (the periodic one
-// included) lie within nx - 1 of the diagonal, the y neighbours at nx.  Block
-// c (one wavefront) factors M = L L^T in LDS in band storage and solves
-// L L^T g = e_c: column c of G, written as row c (G is symmetric), and its
-// mean.  Every block repeats the factorisation (nb blocks, one per CU, run
-// side by side).  A second kernel forms (P G P)_rc = G_rc - m_r - m_c + mean
-// of the m, m the column means (equal to the row means of a symmetric G).
+// included) lie within nx - 1 of the diagonal, the y neighbours at nx.  Once
+// per solve: k_mg_bottom_factor (one wavefront) factors M = L L^T in LDS in
+// band storage and writes the band out; k_mg_bottom_cols solves L L^T g = e_c
+// for one column c per lane (the lanes of a block share the band in LDS and
+// keep their vectors there; no lane waits on another), writes g as row c of
+// G (G is symmetric) and its mean; k_mg_bottom_proj forms
+// (P G P)_rc = G_rc - m_r - m_c + mean of the m, m the column means (the row
+// means of a symmetric G), in the cycle's type.
 #define BW 64
-template <typename T> __global__ void __launch_bounds__(BW) k_mg_bottom_chol(Lev<T> B, double *G, double *cm)
+// NX: the largest band this instantiation takes.  NX = 16 (bottoms up to 16
+// wide, all the common sizes) uses a padded layout: row stride 17 and 16 zero
+// rows after the last, so every loop below runs its full, compile-time
+// length with no bounds tests (entries past the band stay zero through the
+// factorisation); wider bottoms use stride nx + 1 and tested bounds.
+template <int NX> struct BandLayout {
+    static constexpr bool PAD = (NX == 16);
+    __device__ static int stride(int nx) { return PAD ? NX + 1 : nx + 1; }
+    __device__ static int rows(int nb) { return PAD ? nb + NX : nb; }
+};
+
+template <typename T, int NX> __global__ void __launch_bounds__(BW) k_mg_bottom_factor(Lev<T> B, double *Lg, double *dinv)
 {
-    extern __shared__ double lb[];         // lb[r W + d] = M[r][r - d] (L after the factorisation), d <= nx
-    const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch, W = nx + 1;
-    const int c = blockIdx.x, t = threadIdx.x;
-    double *y = lb + (size_t)nb * W;       // the solve's vector
-    for (int q = t; q < nb * W; q += BW) lb[q] = 0.0;
+    using BL = BandLayout<NX>;
+    constexpr int TRI_PER_LANE = (NX * (NX + 1) / 2 + BW - 1) / BW;
+    extern __shared__ double lb[];         // lb[r W + d] = M[r][r - d] (L after the factorisation)
+    const int nx = B.nx, ny = B.ny, nb = nx * ny, p = B.pitch, t = threadIdx.x;
+    const int W = BL::stride(nx), RP = BL::rows(nb), nbw = BL::PAD ? NX : nx;
+    const int ntri = nbw * (nbw + 1) / 2;
+    // this lane's pairs (i, j), 1 <= j <= i <= nbw, of the trailing update:
+    // q = t + BW u in the order (1,1), (2,1), (2,2), (3,1), ...
+    int ti[TRI_PER_LANE], tj[TRI_PER_LANE];
+#pragma unroll
+    for (int u = 0; u < TRI_PER_LANE; u++) {
+        const int q = t + BW * u;
+        int i = 1, base = 0;
+        while (base + i <= q) { base += i; i++; }
+        ti[u] = (q < ntri) ? i : nbw + 1;                // nbw + 1: never active
+        tj[u] = q - base + 1;
+    }
+    for (int q = t; q < RP * W; q += BW) lb[q] = 0.0;
     __syncthreads();
     // row r's diagonal and its couplings to the columns before it: the west
     // neighbour, the periodic east neighbour (of i = nx - 1) and the south one
@@ -374,44 +400,216 @@ template <typename T> __global__ void __launch_bounds__(BW) k_mg_bottom_chol(Lev
         if (j > 0) lb[r * W + nx] -= cs;
     }
     __syncthreads();
-    // right-looking band Cholesky
+    // right-looking band Cholesky (padded: the steps touch the zero rows past
+    // nb and the zero entries past the band, which stay zero).  The block is
+    // one wavefront, whose LDS accesses complete in program order, so a step
+    // needs no workgroup barrier: it reads the pivot, column k (unscaled) and
+    // its pairs' entries, then writes L[k][k], the scaled column and the
+    // updated entries -- no two lanes write, or write and read, one entry in
+    // a step (the updated entries have j >= 1, the column entries d = i), and
+    // the next step's reads come after these writes.  The wave barriers keep
+    // the compiler from moving LDS accesses across steps.
     for (int k = 0; k < nb; k++) {
-        const double dkk = sqrt(lb[k * W]), inv = 1.0 / dkk;
-        const int m = min(nx, nb - 1 - k);
-        __syncthreads();
-        if (t == 0) lb[k * W] = dkk;
-        for (int i = 1 + t; i <= m; i += BW) lb[(k + i) * W + i] *= inv;
-        __syncthreads();
-        for (int q = t; q < m * m; q += BW) {              // M[k+i][k+j] -= L[k+i][k] L[k+j][k], j <= i
-            const int i = 1 + q / m, j = 1 + q - (i - 1) * m;
-            if (j <= i) lb[(k + i) * W + (i - j)] -= lb[(k + i) * W + i] * lb[(k + j) * W + j];
+        const int m = BL::PAD ? NX : min(nx, nb - 1 - k);
+        const double piv = lb[k * W];
+        const double ck = (t >= 1 && t <= m) ? lb[(k + t) * W + t] : 0.0;
+        double mi[TRI_PER_LANE], mj[TRI_PER_LANE], mij[TRI_PER_LANE];
+#pragma unroll
+        for (int u = 0; u < TRI_PER_LANE; u++) {
+            const int i = ti[u], j = tj[u];
+            mi[u] = mj[u] = mij[u] = 0.0;
+            if (i <= m) {
+                mi[u] = lb[(k + i) * W + i];
+                mj[u] = lb[(k + j) * W + j];
+                mij[u] = lb[(k + i) * W + (i - j)];
+            }
         }
-        __syncthreads();
+        // 1 / sqrt(piv): the hardware estimate (about 23 bits) and two Newton
+        // steps (quadratic: to FP64 rounding), far shorter a chain than an
+        // IEEE sqrt and divide; L[k][k] = piv / sqrt(piv)
+        double inv = __builtin_amdgcn_rsq(piv);
+        inv = inv * (1.5 - 0.5 * piv * inv * inv);
+        inv = inv * (1.5 - 0.5 * piv * inv * inv);
+        const double dkk = piv * inv;
+        __builtin_amdgcn_wave_barrier();
+        if (t == 0) {
+            lb[k * W] = dkk;
+            dinv[k] = inv;
+        }
+        if (t >= 1 && t <= m) lb[(k + t) * W + t] = ck * inv;
+#pragma unroll
+        for (int u = 0; u < TRI_PER_LANE; u++) {           // M[k+i][k+j] -= L[k+i][k] L[k+j][k]
+            const int i = ti[u], j = tj[u];
+            if (i <= m) lb[(k + i) * W + (i - j)] = mij[u] - (mi[u] * inv) * (mj[u] * inv);
+        }
+        __builtin_amdgcn_wave_barrier();
     }
-    // L y = e_c, then L^T g = y (g in y), sums over the band by the wavefront
-    for (int r = t; r < nb; r += BW) y[r] = 0.0;
     __syncthreads();
-    for (int r = c; r < nb; r++) {
-        double v = 0.0;
-        for (int d = 1 + t; d <= min(nx, r - c); d += BW) v += lb[r * W + d] * y[r - d];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, BW);
-        if (t == 0) y[r] = ((r == c ? 1.0 : 0.0) - v) / lb[r * W];
-        __syncthreads();
+    for (int q = t; q < RP * W; q += BW) Lg[q] = lb[q];
+}
+
+// sum over d = 2 .. NX of a[d] w[d - 1] as a pairwise tree: these terms do
+// not depend on the newest value w[0], so only one product and two adds of a
+// step wait on the step before
+template <int NX> __device__ __forceinline__ double band_sum(const double *a, const double *w)
+{
+    double p[NX];
+#pragma unroll
+    for (int d = 0; d < NX; d++) p[d] = (d >= 1) ? a[d + 1] * w[d] : 0.0;   // p[d] = a[d+1] w[d], d >= 1
+#pragma unroll
+    for (int h = 1; h < NX; h <<= 1)
+#pragma unroll
+        for (int d = 0; d + h < NX; d += 2 * h) p[d] += p[d + h];
+    return p[0];
+}
+
+// columns c = blockIdx * cpb + lane of G from the band L and 1 / diag(L),
+// both staged in LDS.  Each lane keeps the last NX values of its recurrence
+// in registers and reads the next band row while the current one is used,
+// so a step waits on one product, not on LDS.
+template <int NX>
+__global__ void __launch_bounds__(BW) k_mg_bottom_cols(int nx, int nb, int cpb, const double *Lg, const double *dinv,
+                                                       double *G, double *cm)
+{
+    using BL = BandLayout<NX>;
+    extern __shared__ double sm[];
+    const int W = BL::stride(nx), RP = BL::rows(nb), t = threadIdx.x;
+    double *lb = sm, *di = sm + (size_t)RP * W, *y = di + nb;    // y[r cpb + lane]
+    for (int q = t; q < RP * W; q += BW) lb[q] = Lg[q];
+    for (int q = t; q < nb; q += BW) di[q] = dinv[q];
+    __syncthreads();
+    const int c = blockIdx.x * cpb + t;
+    if (t >= cpb || c >= nb) return;
+    // a band entry, 0 past the band or past the last row (no test when padded)
+    auto band = [&](int row, int d) -> double {
+        if (BL::PAD) return lb[row * W + d];
+        return (d <= nx && row < nb) ? lb[row * W + d] : 0.0;
+    };
+    double w[NX], a[NX + 1], an[NX + 1];
+    // L y = e_c: every lane runs all rows (y_r = 0 before row c comes out of
+    // the same recurrence), so the loops stay uniform; w[d - 1] = y_{r - d},
+    // a[d] = L[r][r - d], a[0] = 1 / L[r][r]
+#pragma unroll
+    for (int d = 0; d < NX; d++) w[d] = 0.0;
+#pragma unroll
+    for (int d = 0; d <= NX; d++) a[d] = (d == 0) ? di[0] : band(0, d);
+    for (int r = 0; r < nb; r++) {
+        const int rn = (r + 1 < nb) ? r + 1 : r;
+#pragma unroll
+        for (int d = 0; d <= NX; d++) an[d] = (d == 0) ? di[rn] : band(rn, d);
+        const double v = ((r == c ? 1.0 : 0.0) - (band_sum<NX>(a, w) + a[1] * w[0])) * a[0];
+#pragma unroll
+        for (int d = NX - 1; d >= 1; d--) w[d] = w[d - 1];
+        w[0] = v;
+        y[r * cpb + t] = v;
+#pragma unroll
+        for (int d = 0; d <= NX; d++) a[d] = an[d];
     }
+    // L^T g = y, in place; w[d - 1] = g_{r + d}, a[d] = L[r + d][r],
+    // a[0] = 1 / L[r][r]
+#pragma unroll
+    for (int d = 0; d < NX; d++) w[d] = 0.0;
+#pragma unroll
+    for (int d = 0; d <= NX; d++) a[d] = (d == 0) ? di[nb - 1] : band(nb - 1 + d, d);
     for (int r = nb - 1; r >= 0; r--) {
-        double v = 0.0;
-        for (int d = 1 + t; d <= min(nx, nb - 1 - r); d += BW) v += lb[(r + d) * W + d] * y[r + d];
-        for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off, BW);
-        if (t == 0) y[r] = (y[r] - v) / lb[r * W];
-        __syncthreads();
+        const int rp = (r > 0) ? r - 1 : 0;
+#pragma unroll
+        for (int d = 0; d <= NX; d++) an[d] = (d == 0) ? di[rp] : band(rp + d, d);
+        const double yr = y[r * cpb + t];
+        const double v = (yr - (band_sum<NX>(a, w) + a[1] * w[0])) * a[0];
+#pragma unroll
+        for (int d = NX - 1; d >= 1; d--) w[d] = w[d - 1];
+        w[0] = v;
+        y[r * cpb + t] = v;
+#pragma unroll
+        for (int d = 0; d <= NX; d++) a[d] = an[d];
     }
     double s = 0.0;
-    for (int r = t; r < nb; r += BW) {
-        G[(size_t)c * nb + r] = y[r];
-        s += y[r];
+    for (int r = 0; r < nb; r++) {
+        const double g = y[r * cpb + t];
+        G[(size_t)c * nb + r] = g;
+        s += g;
     }
-    for (int off = 32; off > 0; off >>= 1) s += __shfl_xor(s, off, BW);
-    if (t == 0) cm[c] = s / nb;
+    cm[c] = s / nb;
+}
+
+// The padded (16-wide) form of k_mg_bottom_cols: the row loop is unrolled by
+// 16, so the window of the last 16 values is a ring indexed by row % 16 at
+// compile time (no register moves per step) and the band rows alternate
+// between two register sets; rows are taken up to a multiple of 16, past nb
+// the band and 1 / diag are zero, so those rows yield zero.
+__global__ void __launch_bounds__(BW) k_mg_bottom_cols16(int nb, int cpb, const double *Lg, const double *dinv,
+                                                         double *G, double *cm)
+{
+    constexpr int K = 16, W = K + 1;
+    extern __shared__ double sm[];
+    const int t = threadIdx.x, nbp = (nb + K - 1) / K * K, RP = nbp + K;
+    double *lb = sm, *di = sm + (size_t)RP * W, *y = di + RP;    // y[r cpb + lane]
+    for (int q = t; q < RP * W; q += BW) lb[q] = (q < (nb + K) * W) ? Lg[q] : 0.0;
+    for (int q = t; q < RP; q += BW) di[q] = (q < nb) ? dinv[q] : 0.0;
+    __syncthreads();
+    const int c = blockIdx.x * cpb + t;
+    if (t >= cpb || c >= nb) return;
+    double w[K], a0[W], a1[W];
+    // L y = e_c, from the first row of this block's columns (y is zero
+    // before its column): w[r % K] = y_r, a[d] = L[r][r - d], a[0] = 1 / L[r][r]
+    const int r0 = (blockIdx.x * cpb) / K * K;
+#pragma unroll
+    for (int q = 0; q < K; q++) w[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < W; d++) a0[d] = (d == 0) ? di[r0] : lb[r0 * W + d];
+    for (int rb = r0; rb < nbp; rb += K) {
+#pragma unroll
+        for (int s2 = 0; s2 < K; s2++) {
+            const int r = rb + s2;
+            double *a = (s2 & 1) ? a1 : a0, *an = (s2 & 1) ? a0 : a1;
+#pragma unroll
+            for (int d = 0; d < W; d++) an[d] = (d == 0) ? di[r + 1] : lb[(r + 1) * W + d];
+            double p[K];
+#pragma unroll
+            for (int d = 0; d < K; d++) p[d] = (d >= 1) ? a[d + 1] * w[(s2 - d - 1 + 2 * K) % K] : 0.0;
+#pragma unroll
+            for (int h = 1; h < K; h <<= 1)
+#pragma unroll
+                for (int d = 0; d + h < K; d += 2 * h) p[d] += p[d + h];
+            const double v = ((r == c ? 1.0 : 0.0) - (p[0] + a[1] * w[(s2 - 1 + K) % K])) * a[0];
+            w[s2] = v;
+            y[r * cpb + t] = v;
+        }
+    }
+    // L^T g = y in place, from the last row: w[r % K] = g_r, a[d] = L[r + d][r]
+#pragma unroll
+    for (int q = 0; q < K; q++) w[q] = 0.0;
+#pragma unroll
+    for (int d = 0; d < W; d++) a0[d] = (d == 0) ? di[nbp - 1] : lb[(nbp - 1 + d) * W + d];
+    for (int rb = nbp - K; rb >= 0; rb -= K) {
+#pragma unroll
+        for (int s2 = K - 1; s2 >= 0; s2--) {
+            const int r = rb + s2;
+            double *a = ((K - 1 - s2) & 1) ? a1 : a0, *an = ((K - 1 - s2) & 1) ? a0 : a1;
+            const int rp = (r > 0) ? r - 1 : 0;
+#pragma unroll
+            for (int d = 0; d < W; d++) an[d] = (d == 0) ? di[rp] : lb[(rp + d) * W + d];
+            const double yr = (r >= r0) ? y[r * cpb + t] : 0.0;
+            double p[K];
+#pragma unroll
+            for (int d = 0; d < K; d++) p[d] = (d >= 1) ? a[d + 1] * w[(s2 + d + 1) % K] : 0.0;
+#pragma unroll
+            for (int h = 1; h < K; h <<= 1)
+#pragma unroll
+                for (int d = 0; d + h < K; d += 2 * h) p[d] += p[d + h];
+            const double v = (yr - (p[0] + a[1] * w[(s2 + 1) % K])) * a[0];
+            w[s2] = v;
+            y[r * cpb + t] = v;
+        }
+    }
+    double sum = 0.0;
+    for (int r = 0; r < nb; r++) {
+        const double g = y[r * cpb + t];
+        G[(size_t)c * nb + r] = g;
+        sum += g;
+    }
+    cm[c] = sum / nb;
 }
 
 template <typename T> __global__ void __launch_bounds__(NTH) k_mg_bottom_proj(int nb, const double *G, const double *cm, T *H)
@@ -908,26 +1106,51 @@ extern "C" int pvk_mg_nrz(const pvk_mg *m)
 // the bottom matrix of a solve: G column by column, then P G P.  Above 64 KiB
 // of LDS (bottoms wider than 16 columns) the factorisation kernel must be
 // allowed it, raised once per device under a lock as for k_mg_coarse.
+// above 64 KiB of dynamic LDS a kernel must be allowed it (gfx950: up to 160
+// KiB per workgroup); raised once per device and kernel, under a lock
+static void allow_lds(const void *k, int slot)
+{
+    static std::mutex mu;
+    static unsigned char raised[64][4];
+    int dev = 0;
+    (void)hipGetDevice(&dev);
+    std::lock_guard<std::mutex> lk(mu);
+    if (dev >= 0 && dev < 64 && !raised[dev][slot]) {
+        if (hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024) == hipSuccess)
+            raised[dev][slot] = 1;
+        else
+            (void)hipGetLastError();
+    }
+}
+
+// the bottom matrix of a solve: factor, columns of G, then P G P
 template <typename T> static void bottom_setup(const pvk_mg *m, hipStream_t s)
 {
     const pvk_mglev &Lb = m->L[m->nlev - 1];
-    const int nb = Lb.nx * Lb.ny;
-    const size_t need = sizeof(double) * ((size_t)nb * (Lb.nx + 1) + nb);
-    if (need > 65536) {
-        static std::mutex mu;
-        static unsigned char raised[64];
-        int dev = 0;
-        (void)hipGetDevice(&dev);
-        std::lock_guard<std::mutex> lk(mu);
-        if (dev >= 0 && dev < 64 && !(raised[dev] & (1u << sizeof(T)))) {
-            if (hipFuncSetAttribute((const void *)k_mg_bottom_chol<T>, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                    160 * 1024) == hipSuccess)
-                raised[dev] |= (unsigned char)(1u << sizeof(T));
-            else
-                (void)hipGetLastError();
-        }
+    const int nx = Lb.nx, nb = nx * Lb.ny, pad = (nx <= 16);
+    const size_t W = pad ? 17 : (size_t)nx + 1, RP = pad ? (size_t)nb + 16 : (size_t)nb;
+    const size_t band = sizeof(double) * RP * W, budget = 160 * 1024;
+    int cpb = BW;                                   // columns per block: as many as LDS holds
+    while (cpb > 1 && band + sizeof(double) * (size_t)nb * (cpb + 1) > budget) cpb >>= 1;
+    const size_t need_cols = band + sizeof(double) * (size_t)nb * (cpb + 1);
+    double *dinv = m->Lb + (size_t)PVK_MG_BOTTOM * (PVK_MG_BOTTOM_NX + 1);
+    if (pad) {
+        hipLaunchKernelGGL((k_mg_bottom_factor<T, 16>), dim3(1), dim3(BW), band, s, lev<T>(Lb), m->Lb, dinv);
+        const size_t nbp = ((size_t)nb + 15) / 16 * 16;
+        int cp = BW;
+        while (cp > 1 && sizeof(double) * ((nbp + 16) * 18 + nbp * cp) > budget) cp >>= 1;
+        const size_t need16 = sizeof(double) * ((nbp + 16) * 18 + nbp * cp);
+        if (need16 > 65536) allow_lds((const void *)k_mg_bottom_cols16, 2);
+        hipLaunchKernelGGL(k_mg_bottom_cols16, dim3((nb + cp - 1) / cp), dim3(BW), need16, s, nb, cp,
+                           (const double *)m->Lb, (const double *)dinv, m->G, m->cm);
+    } else {
+        if (band > 65536) allow_lds((const void *)k_mg_bottom_factor<T, PVK_MG_BOTTOM_NX>, sizeof(T) == 4 ? 0 : 1);
+        hipLaunchKernelGGL((k_mg_bottom_factor<T, PVK_MG_BOTTOM_NX>), dim3(1), dim3(BW), band, s, lev<T>(Lb), m->Lb,
+                           dinv);
+        if (need_cols > 65536) allow_lds((const void *)k_mg_bottom_cols<PVK_MG_BOTTOM_NX>, 3);
+        hipLaunchKernelGGL(k_mg_bottom_cols<PVK_MG_BOTTOM_NX>, dim3((nb + cpb - 1) / cpb), dim3(BW), need_cols, s,
+                           nx, nb, cpb, (const double *)m->Lb, (const double *)dinv, m->G, m->cm);
     }
-    hipLaunchKernelGGL(k_mg_bottom_chol<T>, dim3(nb), dim3(BW), need, s, lev<T>(Lb), m->G, m->cm);
     hipLaunchKernelGGL(k_mg_bottom_proj<T>, dim3(nb), dim3(NTH), 0, s, nb, (const double *)m->G,
                        (const double *)m->cm, (T *)m->H);
 }

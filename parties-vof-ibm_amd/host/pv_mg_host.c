@@ -73,7 +73,7 @@ int pv_mg_create(pv_bag *b)
     nrz = pvk_mg_nrz(m);
     m->sh0 = mf->sh0 = -1;
     total = 5 * cells + 2 * (size_t)b->g.P * ny + 3 * (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM +
-            2 * (size_t)PVK_MG_BOTTOM + 5 * (size_t)ny + 3 * (size_t)nrz;
+            2 * (size_t)PVK_MG_BOTTOM * (PVK_MG_BOTTOM_NX + 3) + 5 * (size_t)ny + 3 * (size_t)nrz;
     totalf = 5 * cells + 3 * nc0 + (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     if (hipMalloc((void **)&b->mgmem, total * sizeof(double)) != hipSuccess) {
         b->mgmem = NULL;
@@ -101,6 +101,7 @@ int pv_mg_create(pv_bag *b)
     }
     m->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     m->cm = q; q += PVK_MG_BOTTOM;
+    m->Lb = q; q += (size_t)PVK_MG_BOTTOM * (PVK_MG_BOTTOM_NX + 2);    /* the band, then 1 / diag */
     m->H = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     m->p0 = b->g.P;
     /* FP32 V-cycle: level 0 has its own x and face copies; r in, z out */
@@ -120,6 +121,7 @@ int pv_mg_create(pv_bag *b)
     }
     mf->G = q; q += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     mf->cm = q; q += PVK_MG_BOTTOM;
+    mf->Lb = q; q += (size_t)PVK_MG_BOTTOM * (PVK_MG_BOTTOM_NX + 2);
     mf->H = qf; qf += (size_t)PVK_MG_BOTTOM * PVK_MG_BOTTOM;
     mf->r0 = b->f[PV_F_P_RES];
     mf->z0 = b->f[PV_FI_Z];
