@@ -302,7 +302,8 @@ typedef struct pvk_mg {
                                          1 / its diagonal (set-up) */
     void *H;                          /* the bottom solve's P G P, nb x nb, in the cycle's type */
 } pvk_mg;
-void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s);
+/* faces of every level; with bottom != 0 also the bottom matrix P G P */
+void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, int bottom, hipStream_t s);
 int  pvk_mg_nrz(const pvk_mg *m);
 size_t pvk_mg_lds_bytes(const pvk_mg *m);
 void pvk_mg_pre(const pvk_mg *m, int l, pvk_scal *sc, const pvk_pcg *w, int n, double tol, int maxit,

@@ -1155,9 +1155,10 @@ template <typename T> static void bottom_setup(const pvk_mg *m, hipStream_t s)
                        (const double *)m->cm, (T *)m->H);
 }
 
-// per solve: coarse faces of every level (FP32: and the level-0 copies) and
-// the bottom inverse, from the FP64 level-0 faces bx0, by0 (pitch pf)
-extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, hipStream_t s)
+// per solve: coarse faces of every level (FP32: and the level-0 copies) and,
+// when asked, the bottom inverse, from the FP64 level-0 faces bx0, by0 (pitch pf)
+extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *by0, int pf, int bottom,
+                             hipStream_t s)
 {
     Lev<double> F0;
     F0.nx = m->L[0].nx;
@@ -1174,12 +1175,12 @@ extern "C" void pvk_mg_setup(const pvk_mg *m, const double *bx0, const double *b
         for (int l = 1; l + 1 < m->nlev; l++)
             hipLaunchKernelGGL((k_mg_faces<float, float>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
                                lev<float>(m->L[l]), lev<float>(m->L[l + 1]));
-        bottom_setup<float>(m, s);
+        if (bottom) bottom_setup<float>(m, s);
     } else {
         for (int l = 0; l + 1 < m->nlev; l++)
             hipLaunchKernelGGL((k_mg_faces<double, double>), cell_grid(m->L[l + 1]), dim3(NTH), 0, s,
                                lev<double>(m->L[l]), lev<double>(m->L[l + 1]));
-        bottom_setup<double>(m, s);
+        if (bottom) bottom_setup<double>(m, s);
     }
 }
 

@@ -428,6 +428,7 @@ int pv_set_time_state(pv_bag *b, double time, double dt, double dt_old, int whic
     b->time = time; b->dt = dt; b->dt_old = dt_old; b->stage = which_stage;
     b->time_nonzero = (time != 0.0);
     b->have_mprev[0] = b->have_mprev[1] = b->have_mprev[2] = 0;   /* increments of another run */
+    b->mg_bottom_age[0] = b->mg_bottom_age[1] = -1;                /* rebuilt at the next solve */
     return pv_sync_scalars_to_device(b);
 }
 

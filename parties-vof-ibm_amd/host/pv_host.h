@@ -108,6 +108,7 @@ struct pv_bag {
     double *mgmem;
     float *mgfmem;
     int mg_ok, mg_chunk;
+    int mg_bottom_age[2];          /* solves since the bottom matrix of mg / mgf was built; -1: not built */
     int p_warm;                    /* 1: start each MG solve from the stage's last correction when better */
     int ch_spectral;               /* 1: biharmonic CG started from the exact spectral solve */
     int have_prev[3];              /* PV_FI_PHI0 + stage holds a correction */

@@ -37,6 +37,7 @@ int pv_mg_create(pv_bag *b)
     b->mgfmem = NULL;
     b->mg_ok = 0;
     b->mg_chunk = 4;
+    b->mg_bottom_age[0] = b->mg_bottom_age[1] = -1;
     b->mgp = m;
     lx[0] = nx;
     ly[0] = ny;

@@ -619,11 +619,32 @@ static void mg_iteration(pv_bag *b, int n, double tol, int maxit)
     mg_vcycle(b, n, tol, maxit);
 }
 
+/* Whether this solve builds the multigrid's bottom matrix: in the solve of
+ * RK3 stage 0 and whenever the matrix is missing or two solves old.  The
+ * bottom matrix is the exact inverse of the 16 x 16 (or smaller) last
+ * level's operator, whose coefficients are 2 x 2 sums down from the
+ * density: over the two later substeps of a step they change by a small
+ * fraction, and a slightly stale exact coarse solve is still a symmetric
+ * positive definite part of the preconditioner, so CG stays CG and its stop
+ * test is unchanged.  The build (a band Cholesky and 256 column solves,
+ * latency-bound) costs ~0.2 ms; once per step instead of per solve saves
+ * two of three. */
+static int mg_bottom_due(pv_bag *b)
+{
+    int *age = &b->mg_bottom_age[b->mgp == &b->mgf];
+    if (*age < 0 || b->stage == 0 || *age >= 2) {
+        *age = 0;
+        return 1;
+    }
+    (*age)++;
+    return 0;
+}
+
 /* faces, coarse levels, r = rhs - mean, phi = 0, z = V(r), stop test -1 */
 static void mg_cold_start(pv_bag *b, double tol, int maxit)
 {
     pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
-    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
+    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, mg_bottom_due(b), S(b));
     pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
     mg_vcycle(b, -1, tol, maxit);
 }
@@ -743,7 +764,7 @@ static int mg_solve(pv_bag *b, int learned)
     if (b->p_warm && b->have_prev[st] && !learned && b->warm_wins[st] >= 3 && (++b->warm_solves[st] & 7) != 0) {
         /* trusted temporal start: ||b|| from the cold residual, then the start */
         pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
-        pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
+        pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, mg_bottom_due(b), S(b));
         pvk_mg_start(G(b), b->d_sc, &b->mgw, 0, S(b));
         pvk_mg_check(b->d_sc, &b->mgw, -1, tol, maxit, S(b));
         pvk_mg_take(G(b), b->d_sc, FD(b, PV_FI_PHI0 + st), FD(b, PV_F_PHI), S(b));
@@ -1019,7 +1040,8 @@ int pv_mg_vcycle_apply(pv_bag *b)
     RC(begin(b));
     if (!b->mg_ok) return pv_set_error(PV_EINVAL, "MG-PCG is not available for this grid size");
     pvk_pcg_prep(G(b), b->d_sc, &b->mgw, S(b));
-    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, S(b));
+    pvk_mg_setup(b->mgp, b->mgw.bx, b->mgw.by, b->g.P, 1, S(b));
+    b->mg_bottom_age[b->mgp == &b->mgf] = 0;
     mg_vcycle(b, -3, 0.0, 1);
     pvk_learn_copy(G(b), b->mgw.z, FD(b, PV_F_SCRATCH0), S(b));
     LAUNCHCHK();
@@ -1186,6 +1208,7 @@ int pv_init_from_F(pv_bag *b, const double *F_host)
     memset(&b->st, 0, sizeof b->st);
     b->time = 0.0; b->dt = b->dt_old = b->prm.default_dt; b->stage = 0; b->ntime = 1; b->time_nonzero = 0;
     b->have_mprev[0] = b->have_mprev[1] = b->have_mprev[2] = 0;
+    b->mg_bottom_age[0] = b->mg_bottom_age[1] = -1;                /* rebuilt at the next solve */
     memset(b->h_sc, 0, sizeof(pvk_scal));
     RC(pv_sync_scalars_to_device(b));
     RC(pv_Velocity_cell_center(b));
