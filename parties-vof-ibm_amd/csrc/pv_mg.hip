@@ -71,10 +71,6 @@ struct MgStep {
     double alpha, mad;
     bool on;
 };
-__device__ __forceinline__ double mg_rnew(const MgStep &u, const double *r0, size_t idx)
-{
-    return r0[idx] - u.alpha * (u.Ad[idx] - u.mad);
-}
 
 // stop test of CG iteration n on the <r, r> partials (as k_transpose):
 // n = -1 sets the reference norm, n = -2 keeps it (warm restart).  The
@@ -114,6 +110,7 @@ template <int NU, int SH> struct MgTile {
     static constexpr int HT = TY + 2 * H;          // extended height
     static constexpr int NC = W * HT;
     static constexpr int K = (NC + NTH - 1) / NTH;    // cells per thread
+    static constexpr int KC = (TX * TY + NTH - 1) / NTH; // core cells per thread
 };
 
 // is extended-tile cell (lx, ly) in the core, and in the level (x0 + lx the
@@ -124,43 +121,179 @@ template <int NU, int SH> __device__ __forceinline__ bool tile_core(int lx, int 
     return lx >= TT::H && lx < TT::H + TT::TX && ly >= TT::H && ly < TT::H + TT::TY && x0 + lx < nx && gj < ny;
 }
 
-// f, faces and 1 / (their sum) of this thread's cells of the extended tile.
-// With the CG step on (level 0 of iteration n), f = r_{n+1} is formed from
-// r_n here, and the core cells also store r_{n+1}, update phi and add
-// r_{n+1}^2 to acc: every global access of the step is in this one batch.
-template <int NU, int SH, typename T>
-__device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, const MgStep &st, int x0,
-                                            int y0, T *f, T *ce, T *cw, T *cn, T *cs, T *id, double &acc)
+// Column and row in the level of extended-tile cell c (clamped into the
+// tile and, for rows, into the level: a cell outside it is masked later).
+template <int NU, int SH>
+__device__ __forceinline__ void tile_cell(int c, int x0, int y0, int nx, int ny, int &gi, int &gj)
 {
     using TT = MgTile<NU, SH>;
-    const int p = L.pitch, nx = L.nx, ny = L.ny;
+    c = min(c, TT::NC - 1);
+    const int ly = c / TT::W, lx = c - ly * TT::W;
+    gj = min(max(y0 + ly, 0), ny - 1);
+    gi = (x0 + lx) % nx;
+    if (gi < 0) gi += nx;
+}
+
+// is extended-tile cell c a cell of the level?
+template <int NU, int SH> __device__ __forceinline__ bool tile_in(int c, int y0, int ny)
+{
+    using TT = MgTile<NU, SH>;
+    const int gj = y0 + c / TT::W;
+    return c < TT::NC && gj >= 0 && gj < ny;
+}
+
+// The raw faces of this thread's cells of the extended tile.  Every load is
+// unconditional, from indices clamped into the level (the face arrays hold
+// every row, wall rows included), so the loads issue back to back with no
+// wait between them; tile_mask zeroes what lies outside the level or on a
+// wall once they have arrived.
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_faces(const Lev<T> &L, int x0, int y0, T *ce, T *cw, T *cn, T *cs)
+{
+    using TT = MgTile<NU, SH>;
+    const int p = L.pitch;
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {
+        int gi, gj;
+        tile_cell<NU, SH>(threadIdx.x + u * NTH, x0, y0, L.nx, L.ny, gi, gj);
+        const int giw = (gi == 0) ? L.nx - 1 : gi - 1;
+        ce[u] = ld(L.bx, p, gi, gj);
+        cw[u] = ld(L.bx, p, giw, gj);
+        cn[u] = ld(L.by, p, gi, gj);
+        cs[u] = ld(L.by, p, gi, max(gj - 1, 0));
+    }
+}
+
+// The level's right-hand side of this thread's cells of the extended tile
+// (raw, unconditional loads as tile_faces).
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_rhs(const Lev<T> &L, const double *r0, int p0, int x0, int y0, T *f)
+{
+    using TT = MgTile<NU, SH>;
+    if (r0) {
+#pragma unroll
+        for (int u = 0; u < TT::K; u++) {
+            int gi, gj;
+            tile_cell<NU, SH>(threadIdx.x + u * NTH, x0, y0, L.nx, L.ny, gi, gj);
+            f[u] = (T)r0[(ptrdiff_t)gj * p0 + gi];
+        }
+    } else {
+#pragma unroll
+        for (int u = 0; u < TT::K; u++) {
+            int gi, gj;
+            tile_cell<NU, SH>(threadIdx.x + u * NTH, x0, y0, L.nx, L.ny, gi, gj);
+            f[u] = ld(L.f, L.pitch, gi, gj);
+        }
+    }
+}
+
+// Zero f and the faces outside the level and the wall faces, and form
+// 1 / (sum of the faces).
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_mask(int y0, int ny, T *f, T *ce, T *cw, T *cn, T *cs, T *id)
+{
+    using TT = MgTile<NU, SH>;
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {
+        const int c = threadIdx.x + u * NTH, gj = y0 + c / TT::W;
+        const bool in = tile_in<NU, SH>(c, y0, ny);
+        f[u] = in ? f[u] : T(0);
+        ce[u] = in ? ce[u] : T(0);
+        cw[u] = in ? cw[u] : T(0);
+        cn[u] = (in && gj + 1 < ny) ? cn[u] : T(0);
+        cs[u] = (in && gj > 0) ? cs[u] : T(0);
+        id[u] = in ? T(1) / (ce[u] + cw[u] + cn[u] + cs[u]) : T(0);
+    }
+}
+
+// The CG step's loads of the level-0 pre-smoothing: r_n and A d_n of this
+// thread's cells of the extended tile (ra, aa; before the step's scalars
+// are summed, in flight meanwhile), then d_n and phi of its cells of the core
+// (core cell q = threadIdx.x + k NTH, row-major in the core: dd, pp; in
+// flight while f is formed).  Every load is unconditional, from an index
+// clamped into the level (cells outside it are masked when the values are
+// used), so the loads issue back to back with no wait between them.
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_step_loads(const Lev<T> &L, const double *r0, int p0, const MgStep &st,
+                                                int x0, int y0, double *ra, double *aa)
+{
+    using TT = MgTile<NU, SH>;
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {
+        int gi, gj;
+        tile_cell<NU, SH>(threadIdx.x + u * NTH, x0, y0, L.nx, L.ny, gi, gj);
+        const size_t idx = (size_t)gj * p0 + gi;
+        ra[u] = r0[idx];
+        aa[u] = st.Ad[idx];
+    }
+}
+
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_step_core_loads(const Lev<T> &L, int p0, const MgStep &st, int bx, int by,
+                                                     double *dd, double *pp)
+{
+    using TT = MgTile<NU, SH>;
+#pragma unroll
+    for (int k = 0; k < TT::KC; k++) {
+        const int q = min(threadIdx.x + k * NTH, TT::TX * TT::TY - 1), ly = q / TT::TX, lx = q - ly * TT::TX;
+        const int gi = min(bx * TT::TX + lx, L.nx - 1), gj = min(by * TT::TY + ly, L.ny - 1);
+        const size_t idx = (size_t)gj * p0 + gi;
+        dd[k] = st.d[idx];
+        pp[k] = st.phi[idx];
+    }
+}
+
+// f = r_{n+1} = r_n - alpha (A d_n - mean) of this thread's cells of the
+// extended tile from the step's loads (tile_mask zeroes it outside the
+// level); the core cells' r_{n+1} also go to rl (FP64, row-major in the
+// core) for the core pass.
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_rhs_step(const Lev<T> &L, const MgStep &st, int x0, int y0, const double *ra,
+                                              const double *aa, T *f, double *rl)
+{
+    using TT = MgTile<NU, SH>;
 #pragma unroll
     for (int u = 0; u < TT::K; u++) {
         const int c = threadIdx.x + u * NTH, ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
-        f[u] = ce[u] = cw[u] = cn[u] = cs[u] = id[u] = T(0);
-        if (c < TT::NC && gj >= 0 && gj < ny) {
-            int gi = (x0 + lx) % nx;
-            if (gi < 0) gi += nx;
-            const int giw = (gi == 0) ? nx - 1 : gi - 1;
-            if (st.on) {
-                const size_t idx = (size_t)gj * p0 + gi;
-                const double rn = mg_rnew(st, r0, idx);
-                f[u] = (T)rn;
-                if (tile_core<NU, SH>(lx, ly, x0, gj, nx, ny)) {
-                    st.rnew[idx] = rn;
-                    st.phi[idx] += st.alpha * st.d[idx];
-                    acc += rn * rn;
-                }
-            } else {
-                f[u] = fget(L, r0, p0, gi, gj);
-            }
-            ce[u] = ld(L.bx, p, gi, gj);
-            cw[u] = ld(L.bx, p, giw, gj);
-            cn[u] = (gj + 1 < ny) ? ld(L.by, p, gi, gj) : T(0);
-            cs[u] = (gj > 0) ? ld(L.by, p, gi, gj - 1) : T(0);
-            id[u] = T(1) / (ce[u] + cw[u] + cn[u] + cs[u]);
+        const double rn = ra[u] - st.alpha * (aa[u] - st.mad);
+        f[u] = (T)rn;
+        if (tile_in<NU, SH>(c, y0, L.ny) && tile_core<NU, SH>(lx, ly, x0, gj, L.nx, L.ny))
+            rl[(ly - TT::H) * TT::TX + lx - TT::H] = rn;
+    }
+}
+
+// the core pass of the CG step: r_{n+1} (from rl) to rnew, phi += alpha d_n,
+// and this thread's share of <r_{n+1}, r_{n+1}>
+template <int NU, int SH, typename T>
+__device__ __forceinline__ double tile_step_core(const Lev<T> &L, int p0, const MgStep &st, int bx, int by,
+                                                 const double *dd, const double *pp, const double *rl)
+{
+    using TT = MgTile<NU, SH>;
+    double acc = 0.0;
+#pragma unroll
+    for (int k = 0; k < TT::KC; k++) {
+        const int q = threadIdx.x + k * NTH, ly = q / TT::TX, lx = q - ly * TT::TX;
+        const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
+        if (q < TT::TX * TT::TY && gi < L.nx && gj < L.ny) {
+            const size_t idx = (size_t)gj * p0 + gi;
+            const double rn = rl[q];
+            st.rnew[idx] = rn;
+            st.phi[idx] = pp[k] + st.alpha * dd[k];
+            acc += rn * rn;
         }
     }
+    return acc;
+}
+
+// f, faces and 1 / (their sum) of this thread's cells of the extended tile:
+// every global load of the smoothing is in this one batch.
+template <int NU, int SH, typename T>
+__device__ __forceinline__ void tile_coeffs(const Lev<T> &L, const double *r0, int p0, int x0, int y0, T *f,
+                                            T *ce, T *cw, T *cn, T *cs, T *id)
+{
+    tile_faces<NU, SH>(L, x0, y0, ce, cw, cn, cs);
+    tile_rhs<NU, SH>(L, r0, p0, x0, y0, f);
+    tile_mask<NU, SH>(y0, L.ny, f, ce, cw, cn, cs, id);
 }
 
 // tile of this block, XCD-aware (consecutive tiles of a run share an L2)
@@ -205,16 +338,21 @@ __device__ __forceinline__ void tile_sweep(T *xs, int x0, int y0, int ny, int k,
 // (tile and halo), and the core writes r_{n+1} to rnew, phi += alpha d and
 // the partial <r_{n+1}, r_{n+1}> of the tile.
 template <int NU, int SH, typename T>
-__global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, double *rnew, int p0,
+__global__ void __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 3) k_mg_pre(Lev<T> L, Lev<T> C, const double *r0, double *rnew, int p0,
                                                pvk_scal *sc, pvk_pcg w, int n, double tol, int maxit, int check,
                                                int step, int ncell)
 {
     using TT = MgTile<NU, SH>;
     __shared__ T xs[TT::NC];
-    __shared__ T rs[TT::TX * TT::TY];
-    __shared__ double sh[NTH];
+    __shared__ double rl[TT::TX * TT::TY];      // r_{n+1} of the core; then rs
+    __shared__ double sh[3 * NTH];
+    T *rs = (T *)rl;                              // core residual, after the sweeps
     if (sc->p_done) return;
     if (check && n != -3 && blockIdx.x == 0 && blockIdx.y == 0) mg_stop_test(sc, w, n, tol, maxit, sh);
+    int bx, by;
+    tile_of(bx, by);
+    const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
+    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
     MgStep st;
     st.on = step && n >= 0;
     st.Ad = w.Ad;
@@ -222,18 +360,30 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
     st.phi = w.phi;
     st.rnew = rnew;
     st.alpha = st.mad = 0.0;
-    if (st.on) {
-        st.alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
-        st.mad = block_sum(w.sad, w.ndad, sh) / (double)ncell;
-    }
-    int bx, by;
-    tile_of(bx, by);
-    const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
-    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
     double acc = 0.0;
-    tile_coeffs<NU, SH>(L, r0, p0, st, x0, y0, f, ce, cw, cn, cs, id, acc);
-    for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
-    __syncthreads();
+    if (st.on) {
+        // the tile's loads go out first and are in flight while the step's
+        // scalars are summed (the three sums' loads together); nothing
+        // waits on memory before the sums
+        double ra[TT::K], aa[TT::K], dd[TT::KC], pp[TT::KC];
+        tile_faces<NU, SH>(L, x0, y0, ce, cw, cn, cs);
+        tile_step_loads<NU, SH>(L, r0, p0, st, x0, y0, ra, aa);
+        double rz, dad, sad;
+        block_sum3(w.rz[n & 1], w.nrz, w.dad[n & 1], w.ndad, w.sad, w.ndad, sh, rz, dad, sad);
+        st.alpha = rz / dad;
+        st.mad = sad / (double)ncell;
+        tile_step_core_loads<NU, SH>(L, p0, st, bx, by, dd, pp);
+        tile_rhs_step<NU, SH>(L, st, x0, y0, ra, aa, f, rl);
+        tile_mask<NU, SH>(y0, L.ny, f, ce, cw, cn, cs, id);
+        for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
+        __syncthreads();
+        // (rs, on rl's memory, is written after the sweeps' barriers)
+        acc = tile_step_core<NU, SH>(L, p0, st, bx, by, dd, pp, rl);
+    } else {
+        tile_coeffs<NU, SH>(L, r0, p0, x0, y0, f, ce, cw, cn, cs, id);
+        for (int c = threadIdx.x; c < TT::NC; c += NTH) xs[c] = T(0);
+        __syncthreads();
+    }
     for (int k = 0; k < 2 * NU; k++) tile_sweep<NU, SH>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
     // residual f - A x of each core cell from the registers (a wall face is
     // 0 there) to rs, and the core of x for the post-smoothing (to xp: the
@@ -274,7 +424,7 @@ __global__ void __launch_bounds__(NTH) k_mg_pre(Lev<T> L, Lev<T> C, const double
 // have finished.
 // dot: partial <f, x> of the core (FP64, with r0 on that level) into w.rz[slot]
 template <int NU, int SH, typename T>
-__global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const double *r0, double *z0, int p0,
+__global__ void __launch_bounds__(NTH, sizeof(T) == 4 ? 4 : 3) k_mg_post(Lev<T> L, Lev<T> C, const double *r0, double *z0, int p0,
                                                 const pvk_scal *sc, pvk_pcg w, int dot, int slot)
 {
     using TT = MgTile<NU, SH>;
@@ -284,32 +434,45 @@ __global__ void __launch_bounds__(NTH) k_mg_post(Lev<T> L, Lev<T> C, const doubl
     int bx, by;
     tile_of(bx, by);
     const int x0 = bx * TT::TX - TT::H, y0 = by * TT::TY - TT::H;
-    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K];
-    MgStep st;
-    st.on = false;
-    double unused = 0.0;
-    tile_coeffs<NU, SH>(L, r0, p0, st, x0, y0, f, ce, cw, cn, cs, id, unused);
-    for (int c = threadIdx.x; c < TT::NC; c += NTH) {
-        const int ly = c / TT::W, lx = c - ly * TT::W, gj = y0 + ly;
-        T v = T(0);
-        if (gj >= 0 && gj < L.ny) {
-            int gi = (x0 + lx) % L.nx;
-            if (gi < 0) gi += L.nx;
-            v = ld(L.xp, L.pitch, gi, gj) + ld(C.x, C.pitch, gi >> 1, gj >> 1);
+    T f[TT::K], ce[TT::K], cw[TT::K], cn[TT::K], cs[TT::K], id[TT::K], xa[TT::K], xb[TT::K];
+    double fd[TT::KC];
+    // every global load of the kernel first, unconditional (clamped indices)
+    tile_faces<NU, SH>(L, x0, y0, ce, cw, cn, cs);
+    tile_rhs<NU, SH>(L, r0, p0, x0, y0, f);
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {           // x = xp + the parent's correction
+        int gi, gj;
+        tile_cell<NU, SH>(threadIdx.x + u * NTH, x0, y0, L.nx, L.ny, gi, gj);
+        xa[u] = ld(L.xp, L.pitch, gi, gj);
+        xb[u] = ld(C.x, C.pitch, gi >> 1, gj >> 1);
+    }
+#pragma unroll
+    for (int k = 0; k < TT::KC; k++) {          // the core's f for the dot
+        fd[k] = 0.0;
+        if (dot) {
+            const int q = min(threadIdx.x + k * NTH, TT::TX * TT::TY - 1), ly = q / TT::TX, lx = q - ly * TT::TX;
+            const int gi = min(bx * TT::TX + lx, L.nx - 1), gj = min(by * TT::TY + ly, L.ny - 1);
+            fd[k] = r0 ? r0[(ptrdiff_t)gj * p0 + gi] : (double)ld(L.f, L.pitch, gi, gj);
         }
-        xs[c] = v;
+    }
+    tile_mask<NU, SH>(y0, L.ny, f, ce, cw, cn, cs, id);
+#pragma unroll
+    for (int u = 0; u < TT::K; u++) {
+        const int c = threadIdx.x + u * NTH;
+        if (c < TT::NC) xs[c] = tile_in<NU, SH>(c, y0, L.ny) ? xa[u] + xb[u] : T(0);
     }
     __syncthreads();
     for (int k = 1; k <= 2 * NU; k++) tile_sweep<NU, SH>(xs, x0, y0, L.ny, k, k & 1, f, ce, cw, cn, cs, id);
     double acc = 0.0;
-    for (int c = threadIdx.x; c < TT::TX * TT::TY; c += NTH) {
-        const int ly = c / TT::TX, lx = c - ly * TT::TX;
+#pragma unroll
+    for (int k = 0; k < TT::KC; k++) {
+        const int c = threadIdx.x + k * NTH, ly = c / TT::TX, lx = c - ly * TT::TX;
         const int gi = bx * TT::TX + lx, gj = by * TT::TY + ly;
-        if (gi < L.nx && gj < L.ny) {
+        if (c < TT::TX * TT::TY && gi < L.nx && gj < L.ny) {
             const T v = xs[(ly + TT::H) * TT::W + lx + TT::H];
             if (z0) z0[(ptrdiff_t)gj * p0 + gi] = (double)v;
             else L.x[(ptrdiff_t)gj * L.pitch + gi] = v;
-            if (dot) acc += (r0 ? r0[(ptrdiff_t)gj * p0 + gi] : (double)ld(L.f, L.pitch, gi, gj)) * (double)v;
+            if (dot) acc += fd[k] * (double)v;
         }
     }
     if (dot) block_store_partial(acc, w.rz[slot], by * gridDim.x + bx, sh);
@@ -1051,11 +1214,14 @@ __global__ void __launch_bounds__(NTH) k_mg_start(pvk_grid g, const pvk_scal *sc
 // partial <r, r>
 __global__ void __launch_bounds__(NTH) k_mg_update(pvk_grid g, const pvk_scal *sc, pvk_pcg w, int n)
 {
-    __shared__ double sh[NTH];
+    __shared__ double sh[3 * NTH];
     if (sc->p_done) return;
     const int P = g.P, j = xcd_band(blockIdx.x, gridDim.x);
-    const double alpha = block_sum(w.rz[n & 1], w.nrz, sh) / block_sum(w.dad[n & 1], w.ndad, sh);
-    const double mad = w.sad ? block_sum(w.sad, w.ndad, sh) / ((double)g.NXM * (double)g.NYM) : 0.0;
+    double rz, dad, sad = 0.0;
+    if (w.sad) block_sum3(w.rz[n & 1], w.nrz, w.dad[n & 1], w.ndad, w.sad, w.ndad, sh, rz, dad, sad);
+    else block_sum2(w.rz[n & 1], w.nrz, w.dad[n & 1], w.ndad, sh, rz, dad);
+    const double alpha = rz / dad;
+    const double mad = sad / ((double)g.NXM * (double)g.NYM);
     const double *d = w.d[n & 1];
     double acc = 0.0;
     for (int i = threadIdx.x; i < g.NXM; i += NTH) {
