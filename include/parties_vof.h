@@ -265,6 +265,10 @@ int pv_timers_enable(pv_bag *bag, int on);
 int pv_timers_reset(pv_bag *bag);
 /* total milliseconds and launch counts per timer (arrays of PV_NTIMERS) */
 int pv_timers_read(pv_bag *bag, double *ms, long long *count);
+/* the same over the working launches: of the sampled launches (the first 8192
+ * per timer since the last reset), those that ran at least a quarter of the
+ * timer's longest one (CG launches queued past convergence exit at once) */
+int pv_timers_read_working(pv_bag *bag, double *ms, long long *count);
 /* Block until all work queued on the handle's stream has finished. */
 int pv_synchronize(pv_bag *bag);
 

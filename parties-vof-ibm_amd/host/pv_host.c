@@ -679,6 +679,7 @@ int pv_timers_flush(pv_bag *b)
             HIPCHK(hipEventElapsedTime(&ms, b->tev[t][0][k], b->tev[t][1][k]));
             b->tms[t] += ms;
             b->tcount[t]++;
+            if (b->tsamp && b->tnsamp[t] < PV_TSAMP) b->tsamp[(size_t)t * PV_TSAMP + b->tnsamp[t]++] = ms;
         }
         b->tpending[t] = 0;
     }
@@ -692,6 +693,9 @@ int pv_timers_enable(pv_bag *b, int on)
     HIPCHK(hipSetDevice(b->device));
     HIPCHK(hipStreamSynchronize(b->stream));
     if (on && !b->timers_on) {
+        b->tsamp = (float *)calloc((size_t)PV_NTIMERS * PV_TSAMP, sizeof(float));
+        if (!b->tsamp) return pv_set_error(PV_ENOMEM, "timer sample allocation failed");
+        memset(b->tnsamp, 0, sizeof b->tnsamp);
         for (t = 0; t < PV_NTIMERS; t++)
             for (k = 0; k < PV_TPOOL; k++) {
                 /* timing-only events: no system-scope fence (cache writeback) per record */
@@ -705,6 +709,8 @@ int pv_timers_enable(pv_bag *b, int on)
                 hipEventDestroy(b->tev[t][0][k]);
                 hipEventDestroy(b->tev[t][1][k]);
             }
+        free(b->tsamp);
+        b->tsamp = NULL;
     }
     b->timers_on = (on > 0) ? on : 0;
     memset(b->tseen, 0, sizeof b->tseen);
@@ -718,6 +724,36 @@ int pv_timers_reset(pv_bag *b)
     if (b->timers_on) pv_timers_flush(b);
     memset(b->tms, 0, sizeof b->tms);
     memset(b->tcount, 0, sizeof b->tcount);
+    memset(b->tnsamp, 0, sizeof b->tnsamp);
+    return PV_OK;
+}
+
+/* Like pv_timers_read, over the working launches only: the sampled launches
+ * (the first PV_TSAMP per timer since the last reset) that ran at least a
+ * quarter of the timer's longest sampled launch.  A CG launch queued past
+ * convergence exits on the device stop flag in a few microseconds and does
+ * none of the work its class's bytes per launch count. */
+int pv_timers_read_working(pv_bag *b, double *ms, long long *count)
+{
+    int t, k;
+    if (!b) return pv_set_error(PV_EINVAL, "null handle");
+    HIPCHK(hipSetDevice(b->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    if (b->timers_on) pv_timers_flush(b);
+    for (t = 0; t < PV_NTIMERS; t++) {
+        const float *v = b->tsamp ? b->tsamp + (size_t)t * PV_TSAMP : NULL;
+        float mx = 0.0f;
+        double sum = 0.0;
+        long long n = 0;
+        for (k = 0; v && k < b->tnsamp[t]; k++) mx = (v[k] > mx) ? v[k] : mx;
+        for (k = 0; v && k < b->tnsamp[t]; k++)
+            if (v[k] >= 0.25f * mx) {
+                sum += v[k];
+                n++;
+            }
+        if (ms) ms[t] = sum;
+        if (count) count[t] = n;
+    }
     return PV_OK;
 }
 

@@ -44,6 +44,7 @@ enum pv_internal_field {
 };
 #define PV_NINTERNAL (PV_FI_END - PV_NFIELDS)
 #define PV_TPOOL 256                /* event pairs per timer between syncs */
+#define PV_TSAMP 8192               /* launch durations kept per timer */
 
 typedef struct pv_metrics {
     int nx, ny;
@@ -97,6 +98,8 @@ struct pv_bag {
     int tpending[PV_NTIMERS];
     double tms[PV_NTIMERS];
     long long tcount[PV_NTIMERS];
+    float *tsamp;                   /* PV_NTIMERS x PV_TSAMP sampled durations (ms) */
+    int tnsamp[PV_NTIMERS];
     /* FFT-x / DCT-y preconditioner */
     pvk_fft fft;
     double *fftmem;

@@ -156,6 +156,7 @@ def load_lib(path=LIB_PATH):
     lib.pv_timers_enable.argtypes = [vp, C.c_int]
     lib.pv_timers_reset.argtypes = [vp]
     lib.pv_timers_read.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
+    lib.pv_timers_read_working.argtypes = [vp, dp, C.POINTER(C.c_longlong)]
     fp = C.POINTER(C.c_float)
     lib.pv_cnn_init.argtypes = [vp, C.c_uint64]
     lib.pv_cnn_set_params.argtypes = [vp, fp]
